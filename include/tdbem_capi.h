@@ -1,0 +1,95 @@
+/*
+ * tdbem_capi.h -- C-ABI of the MI355X multi-domain TDBEM GMRES path.
+ *
+ * Plain pointers and sizes only.  One context per host thread; the context
+ * owns its device memory.  Every function returns 0 on success and a
+ * negative code on failure, with a message in tdbem_last_error().
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the reference tree Ting-Ji/parallel3.0_OMP_MultiDomin_1node).
+ */
+#ifndef TDBEM_CAPI_H
+#define TDBEM_CAPI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tdbem_ctx tdbem_ctx;
+
+/* Message of the last failing call on this thread. */
+const char* tdbem_last_error(void);
+
+/* Build the case: mesh, per-element BCID (123 / 456), the multi-domain
+ * datacard tail and the global dt.  Replaces ReadOptionalMultiDomainInput's
+ * consumer BuildMultiDomainModel + ApplyMultiDomainModelToElements +
+ * element GeoInit/PthPhyInit (DBEM1/multidomain.cpp:1364-1531,
+ * DBEM1/DBEM.cpp:1091-1160).
+ *   points[3*npoints]; ele_pid[8*ne] 0-based point ids (corners then midsides)
+ *   domain_count <= 0 disables the multi-domain tail (one domain);
+ *   mat_rows[4*domain_count] = (id, E, v, rho) with 0- or 1-based ids;
+ *   interface_count < 0 means "no InterfaceCount line" (implicit interfaces),
+ *   itf_rows[5*interface_count] = (domainA, eleA, domainB, eleB, normalSign)
+ *   with 0- or 1-based ids, normalised as the reference does.
+ *   default_E/v/rho: global material line (used for domains without a row). */
+int tdbem_create(tdbem_ctx** out, long npoints, const double* points, long ne, const int* ele_pid,
+                 const int* bcid, int domain_count, const double* mat_rows, int interface_count,
+                 const long* itf_rows, double default_E, double default_v, double default_rho, double dt,
+                 double gap);
+void tdbem_destroy(tdbem_ctx* ctx);
+
+/* GlobalDofMap export (DBEM1/multidomain.h:103-137).  counts[9] = unknown,
+ * known, equation, outerU, outerT, interfaceU, interfaceT, historyU,
+ * historyT.  Arrays of length ne (node == element); signs as doubles. */
+int tdbem_dofmap_export(tdbem_ctx* ctx, long* counts, long* u_block, double* u_sign, int* u_known,
+                        long* t_block, double* t_sign, int* t_known, long* hu_block, double* hu_sign,
+                        long* ht_block, double* ht_sign, long* preferred);
+
+/* Step-0 assembly of A and KnownM plus the leaf preconditioner
+ * (AssembleMultiDomainStep0, DBEM1/multidomain.cpp:2256-2327,2658-2693;
+ * BuildMappedLeafPreConditioner 3493-3565). */
+int tdbem_assemble_step0(tdbem_ctx* ctx, long maxleaf);
+
+/* History assembly MTS[k], MGS[k] for k = 1..maxn on the device: causality
+ * cull, compaction and quadrature (AssembleMultiDomainHistoryStep,
+ * DBEM1/multidomain.cpp:2329-2378,2695-2722). Reports the stored band entry
+ * count through *entries (may be null). */
+int tdbem_assemble_history(tdbem_ctx* ctx, int maxn, long* entries);
+
+/* Export one assembled matrix as block CCSR with row-major 3x3 blocks
+ * (CCSRMat, DBEM1/mat_vec.h:220-240): which 0 = A, 1 = KnownM, 2 = MTS[step],
+ * 3 = MGS[step] (symmetric blocks expanded to 9 values).  Call with null
+ * arrays to get *nrows and *nnz; row_ptr needs nrows+1 ints. */
+int tdbem_ccsr_export(tdbem_ctx* ctx, int which, int step, long* nrows, long* nnz, int* row_ptr, int* col,
+                      double* val);
+
+/* Singular self tables of every element (U1(0), T1(0), T2(1); column-major
+ * 9 values each), for checks (DBEM1/dsquareelementdyna.cpp:239-490). */
+int tdbem_self_tables(tdbem_ctx* ctx, double* u1, double* t1, double* t2);
+
+/* Time loop with GMRES per step (DynaGMRESSolverMultiDomainCCSR time loop,
+ * DBEM1/multidomain.cpp:3815-3893; gmres DBEM1/pre_gmres.cpp:1369-1605).
+ *   known_lu[(nstep+1)*ne*3]: the .bd triplets (element frame);
+ *   lu, lt [(nstep+1)*ne*3]: displacement / traction, global frame;
+ *   iters[nstep], flags[nstep]: GMRES inner iterations and flag per step;
+ *   times[3] (may be null): seconds spent in convolution, GMRES, total.
+ * Returns through *solved the number of solved steps; a GMRES flag != 0
+ * stops the loop as the reference does (the call still returns 0). */
+int tdbem_solve(tdbem_ctx* ctx, int nstep, const double* known_lu, long restart, double tol, double* lu,
+                double* lt, int* iters, int* flags, int* solved, double* times);
+
+/* Convenience: full DynaGMRESSolverMultiDomainCCSR (DBEM1/multidomain.cpp:
+ * 3664-3926): step 0, history for maxn = nstep, time loop.  Returns 1 when
+ * the final GMRES flag is 0 and -1 otherwise through *result. */
+int tdbem_dyna_gmres_solver(tdbem_ctx* ctx, long iterations, double error, long maxleafpointnum, int nstep,
+                            const double* known_lu, double* lu, double* lt, int* result);
+
+/* Device timing of the last history assembly and solve, seconds:
+ * t[0] singular tables, t[1] cull, t[2] history quadrature, t[3] step 0,
+ * t[4] convolution (sum over steps), t[5] GMRES (sum over steps). */
+int tdbem_timings(tdbem_ctx* ctx, double* t6);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -135,6 +135,10 @@ static double quad_n0(const qtables* q, double s1, double s2)
     return r0;
 }
 
+/* Separate libm calls for sin and cos (no sincos fusion by the compiler). */
+__attribute__((noinline)) static double ocos(double x) { return cos(x); }
+__attribute__((noinline)) static double osin(double x) { return sin(x); }
+
 /* DBEM1/quadinfo.cpp:5-48 (constant-element polar helpers) */
 static double const_theta_start(int a)
 {
@@ -154,10 +158,10 @@ static double const_theta_from_local(double lt, int a)
 static double const_alpha(double th, int a)
 {
     switch (a) {
-    case 0: return -1.0 / sin(th);
-    case 1: return 1.0 / cos(th);
-    case 2: return 1.0 / sin(th);
-    case 3: return -1.0 / cos(th);
+    case 0: return -1.0 / osin(th);
+    case 1: return 1.0 / ocos(th);
+    case 2: return 1.0 / osin(th);
+    case 3: return -1.0 / ocos(th);
     default: return 0.0;
     }
 }
@@ -265,8 +269,8 @@ static void build_tables(qtables* q, double gap)
             double th = const_theta_from_local(q->sp[n][0], a);
             double tempFA = 0.25 * const_alpha(th, a) * const_theta_diff();
             double rou = const_polar_from_local(q->sp[n][1], th, a);
-            q->secconstpos[a][0][n] = rou * cos(th);
-            q->secconstpos[a][1][n] = rou * sin(th);
+            q->secconstpos[a][0][n] = rou * ocos(th);
+            q->secconstpos[a][1][n] = rou * osin(th);
             q->secconstval[a][n] = rou * tempFA * q->sa[n];
             /* AssignFValue (DBEM1/quadinfo.cpp:335-345) */
             q->constfm1secval[a][n] = (rou * tempFA * q->sa[n]) / rou / rou;
@@ -285,8 +289,8 @@ static void build_tables(qtables* q, double gap)
                 double th = const_theta_from_local(q->sinlpospw[n][m][0], a);
                 double tempFA = 0.25 * const_alpha(th, a) * const_theta_diff();
                 double rou = const_polar_from_local(q->sinlpospw[n][m][1], th, a);
-                q->secconstpospw[a][0][n][m] = rou * cos(th);
-                q->secconstpospw[a][1][n][m] = rou * sin(th);
+                q->secconstpospw[a][0][n][m] = rou * ocos(th);
+                q->secconstpospw[a][1][n][m] = rou * osin(th);
                 q->secconstvalpw[a][n][m] = rou * tempFA * q->sinnpwgv[m];
             }
 }
@@ -1015,9 +1019,9 @@ static void self_integrals(const qtables* q, const dmat* m, elem* e)
         for (j = 0; j < NSG; ++j) {
             double th = q->constlinepos[a][j];
             double FVL1;
-            A[1] = D1[0] * cos(th) + D2[0] * sin(th);
-            A[2] = D1[1] * cos(th) + D2[1] * sin(th);
-            A[3] = D1[2] * cos(th) + D2[2] * sin(th);
+            A[1] = D1[0] * ocos(th) + D2[0] * osin(th);
+            A[2] = D1[1] * ocos(th) + D2[1] * osin(th);
+            A[3] = D1[2] * ocos(th) + D2[2] * osin(th);
             A[0] = sqrt(A[1] * A[1] + A[2] * A[2] + A[3] * A[3]);
             A[4] = A[0] * A[0] * A[0];
             A[5] = A[4] * A[0] * A[0];

@@ -1,0 +1,216 @@
+// extern "C" boundary of the MI355X TDBEM path (declared in include/tdbem_capi.h).
+#include "../../include/tdbem_capi.h"
+
+#include "solver.hpp"
+
+#include <cstring>
+#include <exception>
+#include <memory>
+#include <string>
+
+struct tdbem_ctx {
+    tdbem::Case c;
+};
+
+namespace {
+thread_local std::string g_err;
+
+int Fail(const char* where, const std::exception& e)
+{
+    g_err = std::string(where) + ": " + e.what();
+    return -1;
+}
+
+int Fail(const char* msg)
+{
+    g_err = msg;
+    return -2;
+}
+}  // namespace
+
+extern "C" {
+
+const char* tdbem_last_error(void) { return g_err.c_str(); }
+
+int tdbem_create(tdbem_ctx** out, long npoints, const double* points, long ne, const int* ele_pid, const int* bcid,
+                 int domain_count, const double* mat_rows, int interface_count, const long* itf_rows,
+                 double default_E, double default_v, double default_rho, double dt, double gap)
+{
+    if (!out || !points || !ele_pid || !bcid || ne <= 0 || npoints <= 0)
+        return Fail("tdbem_create: invalid arguments");
+    for (long i = 0; i < 8 * ne; ++i)
+        if (ele_pid[i] < 0 || ele_pid[i] >= npoints)
+            return Fail("tdbem_create: element point id out of range");
+    tdbem::MultiDomainInputConfig cfg;
+    if (domain_count > 0) {
+        cfg.enabled = true;
+        cfg.domainCount = domain_count;
+        for (int d = 0; d < domain_count; ++d)
+            cfg.materials.push_back(tdbem::DomainMaterialInput{(int)mat_rows[4 * d], mat_rows[4 * d + 1],
+                                                               mat_rows[4 * d + 2], mat_rows[4 * d + 3]});
+        if (interface_count >= 0) {
+            cfg.hasExplicitInterfaces = true;
+            for (int i = 0; i < interface_count; ++i) {
+                tdbem::InterfacePair f;
+                f.domainA = (int)itf_rows[5 * i];
+                f.eleA = itf_rows[5 * i + 1];
+                f.domainB = (int)itf_rows[5 * i + 2];
+                f.eleB = itf_rows[5 * i + 3];
+                f.normalSign = (int)itf_rows[5 * i + 4];
+                cfg.interfaces.push_back(f);
+            }
+        }
+    }
+    std::unique_ptr<tdbem_ctx> ctx(new tdbem_ctx());
+    try {
+        ctx->c.Create(npoints, points, ne, ele_pid, bcid, cfg, default_E, default_v, default_rho, dt, gap);
+    } catch (const std::exception& e) {
+        return Fail("tdbem_create", e);
+    }
+    *out = ctx.release();
+    return 0;
+}
+
+void tdbem_destroy(tdbem_ctx* ctx) { delete ctx; }
+
+int tdbem_dofmap_export(tdbem_ctx* ctx, long* counts, long* u_block, double* u_sign, int* u_known, long* t_block,
+                        double* t_sign, int* t_known, long* hu_block, double* hu_sign, long* ht_block,
+                        double* ht_sign, long* preferred)
+{
+    if (!ctx || !counts)
+        return Fail("tdbem_dofmap_export: invalid arguments");
+    const tdbem::GlobalDofMap& d = ctx->c.dof;
+    counts[0] = d.unknownBlockCount;
+    counts[1] = d.knownBlockCount;
+    counts[2] = d.equationBlockCount;
+    counts[3] = d.outerDisplacementUnknownBlocks;
+    counts[4] = d.outerTractionUnknownBlocks;
+    counts[5] = d.interfaceDisplacementUnknownBlocks;
+    counts[6] = d.interfaceTractionUnknownBlocks;
+    counts[7] = d.historyUBlockCount;
+    counts[8] = d.historyTBlockCount;
+    for (long i = 0; i < ctx->c.ne; ++i) {
+        int dm = ctx->c.domOf[(size_t)i];
+        tdbem::VariableRef u = d.GetU(dm, i), t = d.GetT(dm, i), hu = d.GetHistoryU(dm, i), ht = d.GetHistoryT(dm, i);
+        u_block[i] = u.globalBlock;
+        u_sign[i] = u.sign;
+        u_known[i] = u.known;
+        t_block[i] = t.globalBlock;
+        t_sign[i] = t.sign;
+        t_known[i] = t.known;
+        hu_block[i] = hu.globalBlock;
+        hu_sign[i] = hu.sign;
+        ht_block[i] = ht.globalBlock;
+        ht_sign[i] = ht.sign;
+        preferred[i] = d.GetPreferredUnknownBlockForRow(i);
+    }
+    return 0;
+}
+
+int tdbem_assemble_step0(tdbem_ctx* ctx, long maxleaf)
+{
+    if (!ctx || maxleaf <= 0)
+        return Fail("tdbem_assemble_step0: invalid arguments");
+    try {
+        ctx->c.AssembleStep0(maxleaf);
+    } catch (const std::exception& e) {
+        return Fail("tdbem_assemble_step0", e);
+    }
+    return 0;
+}
+
+int tdbem_assemble_history(tdbem_ctx* ctx, int maxn, long* entries)
+{
+    if (!ctx || maxn < 1)
+        return Fail("tdbem_assemble_history: invalid arguments");
+    try {
+        ctx->c.AssembleHistory(maxn);
+    } catch (const std::exception& e) {
+        return Fail("tdbem_assemble_history", e);
+    }
+    if (entries)
+        *entries = ctx->c.entries;
+    return 0;
+}
+
+int tdbem_ccsr_export(tdbem_ctx* ctx, int which, int step, long* nrows, long* nnz, int* row_ptr, int* col,
+                      double* val)
+{
+    if (!ctx || !nrows || !nnz || which < 0 || which > 3)
+        return Fail("tdbem_ccsr_export: invalid arguments");
+    std::vector<int> rp, c;
+    std::vector<double> v;
+    try {
+        ctx->c.ExportMatrix(which, step, rp, c, v);
+    } catch (const std::exception& e) {
+        return Fail("tdbem_ccsr_export", e);
+    }
+    *nrows = (long)rp.size() - 1;
+    *nnz = (long)c.size();
+    if (row_ptr)
+        std::memcpy(row_ptr, rp.data(), sizeof(int) * rp.size());
+    if (col && !c.empty())
+        std::memcpy(col, c.data(), sizeof(int) * c.size());
+    if (val && !v.empty())
+        std::memcpy(val, v.data(), sizeof(double) * v.size());
+    return 0;
+}
+
+int tdbem_self_tables(tdbem_ctx* ctx, double* u1, double* t1, double* t2)
+{
+    if (!ctx || !u1 || !t1 || !t2)
+        return Fail("tdbem_self_tables: invalid arguments");
+    size_t n = 9 * (size_t)ctx->c.ne;
+    if (hipMemcpy(u1, ctx->c.d_u1.p, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(t1, ctx->c.d_t1.p, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(t2, ctx->c.d_t2.p, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+        return Fail("tdbem_self_tables: device copy failed");
+    return 0;
+}
+
+int tdbem_solve(tdbem_ctx* ctx, int nstep, const double* known_lu, long restart, double tol, double* lu, double* lt,
+                int* iters, int* flags, int* solved, double* times)
+{
+    if (!ctx || nstep < 1 || !known_lu || !lu || !lt || !iters || !flags || !solved || restart < 1 || !(tol > 0.0))
+        return Fail("tdbem_solve: invalid arguments");
+    try {
+        ctx->c.Solve(nstep, known_lu, restart, tol, lu, lt, iters, flags, solved);
+    } catch (const std::exception& e) {
+        return Fail("tdbem_solve", e);
+    }
+    if (times) {
+        times[0] = ctx->c.timing[4];
+        times[1] = ctx->c.timing[5];
+        times[2] = ctx->c.timing[4] + ctx->c.timing[5];
+    }
+    return 0;
+}
+
+int tdbem_dyna_gmres_solver(tdbem_ctx* ctx, long iterations, double error, long maxleafpointnum, int nstep,
+                            const double* known_lu, double* lu, double* lt, int* result)
+{
+    if (!ctx || !result || nstep < 1)
+        return Fail("tdbem_dyna_gmres_solver: invalid arguments");
+    std::vector<int> it((size_t)nstep), fl((size_t)nstep);
+    int solved = 0;
+    try {
+        ctx->c.AssembleStep0(maxleafpointnum);
+        ctx->c.AssembleHistory(nstep);
+        ctx->c.Solve(nstep, known_lu, iterations, error, lu, lt, it.data(), fl.data(), &solved);
+    } catch (const std::exception& e) {
+        return Fail("tdbem_dyna_gmres_solver", e);
+    }
+    *result = (solved > 0 && fl[(size_t)solved - 1] == 0) ? 1 : -1;
+    return 0;
+}
+
+int tdbem_timings(tdbem_ctx* ctx, double* t6)
+{
+    if (!ctx || !t6)
+        return Fail("tdbem_timings: invalid arguments");
+    for (int i = 0; i < 6; ++i)
+        t6[i] = ctx->c.timing[i];
+    return 0;
+}
+
+}  // extern "C"

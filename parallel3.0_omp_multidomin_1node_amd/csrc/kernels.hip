@@ -1,0 +1,1414 @@
+// HIP kernels of the multi-domain TDBEM path for gfx950 (CDNA4).
+//
+// FP64 arithmetic keeps the reference's operation order (see common.hpp);
+// the quadrature of one (field, source, step) triple is accumulated by one
+// lane in the reference's point order, so block values and the 1e-14 drop
+// decisions match the CPU restatement bit for bit.
+#include "kernels.hpp"
+
+#pragma clang fp contract(off)
+
+namespace tdbem {
+
+__constant__ DynaMat c_mats[kMaxDomains];
+
+void UploadDynaMats(const DynaMat* mats, int nd, hipStream_t s)
+{
+    (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(c_mats), mats, sizeof(DynaMat) * (size_t)nd, 0, hipMemcpyHostToDevice,
+                                 s);
+}
+
+// ---------------------------------------------------------------------------
+// Time-integrated elastodynamic kernels (DBEM1/dsquareelement.h:631-903,
+// DBEM1/dsquareelement.cpp:1070-1590)
+// ---------------------------------------------------------------------------
+namespace dev {
+
+__device__ __forceinline__ int Fai(double n, double dt, double r, double c)
+{
+    double ndt = n * dt;
+    double rc = r / c;
+    return ((rc >= ndt) && (rc < (ndt + dt))) ? 1 : 0;
+}
+
+// ISeries / JSeries: the five-way window of the between-fronts term.
+__device__ __forceinline__ int Window(double n, double dt, double r, const DynaMat& m, double& ndt, double& np1dt)
+{
+    ndt = n * dt;
+    np1dt = ndt + dt;
+    double rc1 = r / m.C1, rc2 = r / m.C2;
+    if (np1dt <= rc1 || ndt >= rc2)
+        return 0;
+    if (ndt >= rc1 && np1dt <= rc2)
+        return 1;
+    if (ndt <= rc1 && (np1dt >= rc1 && np1dt <= rc2))
+        return 2;
+    if (ndt <= rc1 && np1dt >= rc2)
+        return 3;
+    if (np1dt >= rc2 && (ndt >= rc1 && ndt <= rc2))
+        return 4;
+    return 0;
+}
+
+__device__ __forceinline__ double ISer(int w, double n, double ndt, double np1dt, const double* R, const DynaMat& m)
+{
+    double t1;
+    switch (w) {
+    case 1: return (n + 0.5) * m.Dt_2 / R[2];
+    case 2: t1 = np1dt / R[1]; return 0.5 * (t1 * t1 - m.OneOverC1_2);
+    case 3: return m.Half_1OC22_1OC12;
+    case 4: t1 = ndt / R[1]; return 0.5 * (m.OneOverC2_2 - t1 * t1);
+    default: return 0.0;
+    }
+}
+
+__device__ __forceinline__ double JSer(int w, double n, double ndt, double np1dt, const double* R, const DynaMat& m)
+{
+    double t1;
+    switch (w) {
+    case 1: t1 = m.Dt_3 / R[3]; return (n * n + n + 0.33333333333333333) * t1;
+    case 2: t1 = np1dt / R[1]; return 0.33333333333333 * (t1 * t1 * t1 - m.OneOverC1_3);
+    case 3: return m.OneOverThree_1OC23_1OC13;
+    case 4: t1 = ndt / R[1]; return 0.333333333333333 * (m.OneOverC2_3 - t1 * t1 * t1);
+    default: return 0.0;
+    }
+}
+
+struct Dist {
+    double R[6];
+    double RI[4];
+    double X[6];  // ri*rj products
+};
+
+__device__ __forceinline__ void MakeDist(const double* src, const double* pt, Dist& d)
+{
+    d.RI[1] = pt[0] - src[0];
+    d.RI[2] = pt[1] - src[1];
+    d.RI[3] = pt[2] - src[2];
+    d.R[1] = sqrt(d.RI[1] * d.RI[1] + d.RI[2] * d.RI[2] + d.RI[3] * d.RI[3]);
+    d.R[2] = d.R[1] * d.R[1];
+    d.R[3] = d.R[2] * d.R[1];
+    d.R[4] = d.R[3] * d.R[1];
+    d.R[5] = d.R[4] * d.R[1];
+    d.X[0] = d.RI[1] * d.RI[1];
+    d.X[1] = d.RI[1] * d.RI[2];
+    d.X[2] = d.RI[1] * d.RI[3];
+    d.X[3] = d.RI[2] * d.RI[2];
+    d.X[4] = d.RI[2] * d.RI[3];
+    d.X[5] = d.RI[3] * d.RI[3];
+}
+
+// GetDynaUij: P-front (Aij), S-front (Bij) and between-fronts (Cij) terms.
+__device__ int DynaU(const DynaMat& m, double n, const Dist& d, double* U)
+{
+    const double* R = d.R;
+    const double* X = d.X;
+    int flag = 0;
+    for (int k = 0; k < 9; ++k)
+        U[k] = 0.0;
+    if (Fai(n, m.Dt, R[1], m.C1)) {
+        flag++;
+        double C1 = m.OneOverC1_2 / R[3];
+        double t1 = m.OneOverFourPaiRou;
+        double a0 = C1 * X[0], a1 = C1 * X[1], a2 = C1 * X[2], a4 = C1 * X[3], a5 = C1 * X[4], a8 = C1 * X[5];
+        U[0] += a0 * t1; U[1] += a1 * t1; U[2] += a2 * t1;
+        U[3] += a1 * t1; U[4] += a4 * t1; U[5] += a5 * t1;
+        U[6] += a2 * t1; U[7] += a5 * t1; U[8] += a8 * t1;
+    }
+    if (Fai(n, m.Dt, R[1], m.C2)) {
+        flag++;
+        double c = m.OneOverC2_2;
+        double b0 = c * (1.0 / R[1] - X[0] / R[3]);
+        double b1 = c * (-X[1] / R[3]);
+        double b2 = c * (-X[2] / R[3]);
+        double b4 = c * (1.0 / R[1] - X[3] / R[3]);
+        double b5 = c * (-X[4] / R[3]);
+        double b8 = c * (1.0 / R[1] - X[5] / R[3]);
+        double t1 = m.OneOverFourPaiRou;
+        U[0] += b0 * t1; U[1] += b1 * t1; U[2] += b2 * t1;
+        U[3] += b1 * t1; U[4] += b4 * t1; U[5] += b5 * t1;
+        U[6] += b2 * t1; U[7] += b5 * t1; U[8] += b8 * t1;
+    }
+    double ndt, np1dt;
+    int w = Window(n, m.Dt, R[1], m, ndt, np1dt);
+    if (w) {
+        flag++;
+        double res = ISer(w, n, ndt, np1dt, R, m);
+        double C1 = 3.0 / R[3], C2 = 1.0 / R[1];
+        double c0 = C1 * X[0] - C2, c1 = C1 * X[1], c2 = C1 * X[2], c4 = C1 * X[3] - C2, c5 = C1 * X[4],
+               c8 = C1 * X[5] - C2;
+        double t1 = m.OneOverFourPaiRou * res;
+        U[0] += c0 * t1; U[1] += c1 * t1; U[2] += c2 * t1;
+        U[3] += c1 * t1; U[4] += c4 * t1; U[5] += c5 * t1;
+        U[6] += c2 * t1; U[7] += c5 * t1; U[8] += c8 * t1;
+    }
+    return flag;
+}
+
+__device__ __forceinline__ void Dij(const DynaMat& m, const Dist& d, const double* N, double* o)
+{
+    const double *R = d.R, *RI = d.RI, *X = d.X;
+    double RmNm = RI[1] * N[0] + RI[2] * N[1] + RI[3] * N[2];
+    double C1 = (4.0 * m.C2_2OverC1_2 - 1.0) / R[3];
+    double C2 = m.C2_2OverC1_2 * 12.0 * RmNm / R[5];
+    double C3 = m.C2_2OverC1_2 * 2.0 / R[3];
+    o[0] = C1 * RI[1] * N[0] - (C2 * X[0] - C3 * (RI[1] * N[0] + RmNm));
+    o[1] = C1 * RI[2] * N[0] - (C2 * X[1] - C3 * (RI[1] * N[1]));
+    o[2] = C1 * RI[3] * N[0] - (C2 * X[2] - C3 * (RI[1] * N[2]));
+    o[3] = C1 * RI[1] * N[1] - (C2 * X[1] - C3 * (RI[2] * N[0]));
+    o[4] = C1 * RI[2] * N[1] - (C2 * X[3] - C3 * (RI[2] * N[1] + RmNm));
+    o[5] = C1 * RI[3] * N[1] - (C2 * X[4] - C3 * (RI[2] * N[2]));
+    o[6] = C1 * RI[1] * N[2] - (C2 * X[2] - C3 * (RI[3] * N[0]));
+    o[7] = C1 * RI[2] * N[2] - (C2 * X[4] - C3 * (RI[3] * N[1]));
+    o[8] = C1 * RI[3] * N[2] - (C2 * X[5] - C3 * (RI[3] * N[2] + RmNm));
+}
+
+__device__ __forceinline__ void Eij(const Dist& d, const double* N, double* o)
+{
+    const double *R = d.R, *RI = d.RI, *X = d.X;
+    double RmNm = RI[1] * N[0] + RI[2] * N[1] + RI[3] * N[2];
+    double C1 = 12.0 * RmNm / R[5];
+    double C2 = 2.0 / R[3];
+    double C3 = 3.0 / R[3];
+    o[0] = C1 * X[0] - C2 * RI[1] * N[0] - C3 * (RI[1] * N[0] + RmNm);
+    o[1] = C1 * X[1] - C2 * RI[2] * N[0] - C3 * (RI[1] * N[1]);
+    o[2] = C1 * X[2] - C2 * RI[3] * N[0] - C3 * (RI[1] * N[2]);
+    o[3] = C1 * X[1] - C2 * RI[1] * N[1] - C3 * (RI[2] * N[0]);
+    o[4] = C1 * X[3] - C2 * RI[2] * N[1] - C3 * (RI[2] * N[1] + RmNm);
+    o[5] = C1 * X[4] - C2 * RI[3] * N[1] - C3 * (RI[2] * N[2]);
+    o[6] = C1 * X[2] - C2 * RI[1] * N[2] - C3 * (RI[3] * N[0]);
+    o[7] = C1 * X[4] - C2 * RI[2] * N[2] - C3 * (RI[3] * N[1]);
+    o[8] = C1 * X[5] - C2 * RI[3] * N[2] - C3 * (RI[3] * N[2] + RmNm);
+}
+
+__device__ __forceinline__ void Fij(const DynaMat& m, const Dist& d, const double* N, double* o)
+{
+    const double *R = d.R, *RI = d.RI, *X = d.X;
+    double RmNm = RI[1] * N[0] + RI[2] * N[1] + RI[3] * N[2];
+    double C1 = -6.0 * m.C2_2;
+    double C2 = 5.0 * RmNm / R[5];
+    o[0] = C1 * (C2 * X[0] - (RI[1] * N[0] + RI[1] * N[0] + RmNm) / R[3]);
+    o[1] = C1 * (C2 * X[1] - (RI[2] * N[0] + RI[1] * N[1]) / R[3]);
+    o[2] = C1 * (C2 * X[2] - (RI[3] * N[0] + RI[1] * N[2]) / R[3]);
+    o[3] = o[1];
+    o[4] = C1 * (C2 * X[3] - (RI[2] * N[1] + RI[2] * N[1] + RmNm) / R[3]);
+    o[5] = C1 * (C2 * X[4] - (RI[3] * N[1] + RI[2] * N[2]) / R[3]);
+    o[6] = o[2];
+    o[7] = o[5];
+    o[8] = C1 * (C2 * X[5] - (RI[3] * N[2] + RI[3] * N[2] + RmNm) / R[3]);
+}
+
+__device__ __forceinline__ void Gij(const DynaMat& m, const Dist& d, const double* N, double* o)
+{
+    const double *R = d.R, *RI = d.RI, *X = d.X;
+    double RmNm = RI[1] * N[0] + RI[2] * N[1] + RI[3] * N[2];
+    double C1 = -2.0 * m.C2_2OverC1_3 * RmNm / R[4];
+    double C2 = 1.0 / m.C1 * (2.0 * m.C2_2OverC1_2 - 1.0) / R[2];
+    o[0] = C1 * X[0] + C2 * RI[1] * N[0];
+    o[1] = C1 * X[1] + C2 * RI[2] * N[0];
+    o[2] = C1 * X[2] + C2 * RI[3] * N[0];
+    o[3] = C1 * X[1] + C2 * RI[1] * N[1];
+    o[4] = C1 * X[3] + C2 * RI[2] * N[1];
+    o[5] = C1 * X[4] + C2 * RI[3] * N[1];
+    o[6] = C1 * X[2] + C2 * RI[1] * N[2];
+    o[7] = C1 * X[4] + C2 * RI[2] * N[2];
+    o[8] = C1 * X[5] + C2 * RI[3] * N[2];
+}
+
+__device__ __forceinline__ void Hij(const DynaMat& m, const Dist& d, const double* N, double* o)
+{
+    const double *R = d.R, *RI = d.RI, *X = d.X;
+    double RmNm = RI[1] * N[0] + RI[2] * N[1] + RI[3] * N[2];
+    double C1 = 2.0 / m.C2 * RmNm / R[4];
+    double C2 = -1.0 / m.C2 / R[2];
+    o[0] = C1 * X[0] + C2 * (RI[1] * N[0] + RmNm);
+    o[1] = C1 * X[1] + C2 * RI[1] * N[1];
+    o[2] = C1 * X[2] + C2 * RI[1] * N[2];
+    o[3] = C1 * X[1] + C2 * RI[2] * N[0];
+    o[4] = C1 * X[3] + C2 * (RI[2] * N[1] + RmNm);
+    o[5] = C1 * X[4] + C2 * RI[2] * N[2];
+    o[6] = C1 * X[2] + C2 * RI[3] * N[0];
+    o[7] = C1 * X[4] + C2 * RI[3] * N[1];
+    o[8] = C1 * X[5] + C2 * (RI[3] * N[2] + RmNm);
+}
+
+// GetDynaT1ij (type 1, time level n) and GetDynaT2ij (type 2, level n-1).
+__device__ int DynaT(const DynaMat& m, int type, double n, const Dist& d, const double* N, double* T)
+{
+    const double* R = d.R;
+    double K1[9], K2[9];
+    int flag = 0;
+    for (int k = 0; k < 9; ++k)
+        T[k] = 0.0;
+    double nn = type == 1 ? n : n - 1;
+    if (Fai(nn, m.Dt, R[1], m.C1)) {
+        flag++;
+        Dij(m, d, N, K1);
+        Gij(m, d, N, K2);
+        double t1, t2;
+        if (type == 1) {
+            t1 = kOneOver4Pi * (n + 1.0 - R[1] / m.C1Dt);
+            t2 = kOneOver4Pi / m.Dt;
+        } else {
+            t1 = kOneOver4Pi * (1.0 - n + R[1] / m.C1Dt);
+            t2 = -kOneOver4Pi / m.Dt;
+        }
+        for (int k = 0; k < 9; ++k)
+            T[k] += K1[k] * t1 + K2[k] * t2;
+    }
+    if (Fai(nn, m.Dt, R[1], m.C2)) {
+        flag++;
+        Eij(d, N, K1);
+        Hij(m, d, N, K2);
+        double t1, t2;
+        if (type == 1) {
+            t1 = kOneOver4Pi * (n + 1.0 - R[1] / m.C2Dt);
+            t2 = kOneOver4Pi / m.Dt;
+        } else {
+            t1 = kOneOver4Pi * (1.0 - n + R[1] / m.C2Dt);
+            t2 = -kOneOver4Pi / m.Dt;
+        }
+        for (int k = 0; k < 9; ++k)
+            T[k] += K1[k] * t1 + K2[k] * t2;
+    }
+    double ndt, np1dt;
+    int w = Window(nn, m.Dt, R[1], m, ndt, np1dt);
+    if (w) {
+        flag++;
+        double ri = ISer(w, nn, ndt, np1dt, R, m);
+        double rj = JSer(w, nn, ndt, np1dt, R, m);
+        Fij(m, d, N, K1);
+        double t1;
+        if (type == 1)
+            t1 = kOneOver4Pi * ((n + 1.0) * ri - R[1] * rj / m.Dt);
+        else
+            t1 = kOneOver4Pi * ((1.0 - n) * ri + R[1] * rj / m.Dt);
+        for (int k = 0; k < 9; ++k)
+            T[k] += K1[k] * t1;
+    }
+    return flag;
+}
+
+// Static traction kernel GetTij (DBEM1/dsquareelement.cpp:496-543).
+__device__ void StaticT(const DynaMat& m, const double* R, const double* n, double* T)
+{
+    double t1 = R[0] * R[0];
+    double t2 = m.STCoef1 / t1;
+    double t3 = m.STCoef2 / R[0];
+    double drdn = (R[1] * n[0] + R[2] * n[1] + R[3] * n[2]) / R[0];
+    double t5 = t3 * (R[2] * n[0] - R[1] * n[1]);
+    double t6 = t3 * (R[3] * n[0] - R[1] * n[2]);
+    double t7 = t3 * (R[3] * n[1] - R[2] * n[2]);
+    double t8 = t2 * drdn;
+    double t9 = t8 * m.STCoef2;
+    double t10 = t8 * 3.0 / t1;
+    double t4 = t10 * R[1];
+    T[0] = t9 + t4 * R[1];
+    T[1] = t4 * R[2] - t2 * t5;
+    T[2] = t4 * R[3] - t2 * t6;
+    T[3] = t4 * R[2] + t2 * t5;
+    T[4] = t9 + t10 * R[2] * R[2];
+    t4 = t10 * R[3];
+    T[5] = t4 * R[2] - t2 * t7;
+    T[6] = t4 * R[1] + t2 * t6;
+    T[7] = t4 * R[2] + t2 * t7;
+    T[8] = t9 + t10 * R[3] * R[3];
+}
+
+// ---------------------------------------------------------------------------
+// Causality judges (DBEM1/dsquareelement.cpp:1156-1236, .h:926-943)
+// ---------------------------------------------------------------------------
+__device__ int NeedCal(const DynaMat& m, const ElemGeo& e, int type, double n, const double* o)
+{
+    double nn = (type == 2) ? n - 1 : n;
+    double ndt = nn * m.Dt;
+    double np1dt = (nn + 1) * m.Dt;
+    double r = Dist3(e.center, o);
+    double Rmax = r + e.radius;
+    double Rmin = r - e.radius;
+    if (Rmin < 0.0)
+        Rmin = 0.0;
+    double a1 = Rmax / m.C1, a2 = Rmax / m.C2, b1 = Rmin / m.C1, b2 = Rmin / m.C2;
+    if (ndt >= a2 || np1dt <= b1)
+        return 0;
+    if (ndt <= b1 && (np1dt >= a1 && np1dt <= b2))
+        return 2;
+    if (ndt <= b1 && np1dt >= a2)
+        return 2;
+    if (ndt >= a1 && np1dt <= b2)
+        return 2;
+    if ((ndt >= a1 && ndt <= b2) && np1dt >= a2)
+        return 2;
+    return 1;
+}
+
+__device__ __forceinline__ int PWNeedCal(const DynaMat& m, const ElemGeo& e, int type, double n, const double* c,
+                                         const double* o)
+{
+    double nn = (type == 2 ? n - 1 : n);
+    double c2ndt = nn * m.C2Dt;
+    double c1np1dt = (nn + 1) * m.C1Dt;
+    double r = Dist3(c, o);
+    double rpr = r + e.radiusPW;
+    double rmr = r - e.radiusPW;
+    return (rmr >= c1np1dt || rpr < c2ndt) ? 0 : 1;
+}
+
+// ---------------------------------------------------------------------------
+// Quadrature: patch-wise (DBEM1/dsquareelementdyna.cpp:794-848,940-994) and
+// regular with the N0(gap) weight and T1+T2 sum (DBEM1/dsquareelement.cpp:
+// 1955-2009,2233-2290).  Column-major 3x3 output.
+// ---------------------------------------------------------------------------
+__device__ void IntUPW(const DynaMat& m, const ElemGeo& e, const ElemPoints& P, const double* src, double n,
+                       double* out)
+{
+    double o0 = 0.0, o1 = 0.0, o2 = 0.0, o4 = 0.0, o5 = 0.0, o8 = 0.0;
+    for (int i = 0; i < kNPW2; ++i) {
+        if (!PWNeedCal(m, e, 1, n, e.rcenpw[i], src))
+            continue;
+        for (int j = 0; j < kNPWG2; ++j) {
+            int q = i * kNPWG2 + j;
+            Dist d;
+            MakeDist(src, P.pw_pos[q], d);
+            double U[9];
+            DynaU(m, n, d, U);
+            double w = P.pw_w[q];
+            o0 += U[0] * w;
+            o1 += U[1] * w;
+            o2 += U[2] * w;
+            o4 += U[4] * w;
+            o5 += U[5] * w;
+            o8 += U[8] * w;
+        }
+    }
+    out[0] = o0; out[1] = o1; out[2] = o2; out[3] = o1; out[4] = o4;
+    out[5] = o5; out[6] = o2; out[7] = o5; out[8] = o8;
+}
+
+__device__ void IntTPW(const DynaMat& m, const ElemGeo& e, const ElemPoints& P, int type, const double* src,
+                       double n, double* out)
+{
+    double acc[9];
+    for (int k = 0; k < 9; ++k)
+        acc[k] = 0.0;
+    for (int i = 0; i < kNPW2; ++i) {
+        if (!PWNeedCal(m, e, type, n, e.rcenpw[i], src))
+            continue;
+        for (int j = 0; j < kNPWG2; ++j) {
+            int q = i * kNPWG2 + j;
+            Dist d;
+            MakeDist(src, P.pw_pos[q], d);
+            double T[9];
+            DynaT(m, type, n, d, P.pw_nor[q], T);
+            double w = P.pw_w[q];
+            for (int k = 0; k < 9; ++k)
+                acc[k] += T[k] * w;
+        }
+    }
+    for (int k = 0; k < 9; ++k)
+        out[k] = acc[k];
+}
+
+__device__ void IntUReg(const DynaMat& m, const ElemPoints& P, const double* src, double n, double* out)
+{
+    double o0 = 0.0, o1 = 0.0, o2 = 0.0, o4 = 0.0, o5 = 0.0, o8 = 0.0;
+    for (int i = 0; i < kNG2; ++i) {
+        Dist d;
+        MakeDist(src, P.rg_pos[i], d);
+        double U[9];
+        if (DynaU(m, n, d, U)) {
+            double w = P.rg_w[i];
+            o0 += U[0] * w;
+            o1 += U[1] * w;
+            o2 += U[2] * w;
+            o4 += U[4] * w;
+            o5 += U[5] * w;
+            o8 += U[8] * w;
+        }
+    }
+    out[0] = o0; out[1] = o1; out[2] = o2; out[3] = o1; out[4] = o4;
+    out[5] = o5; out[6] = o2; out[7] = o5; out[8] = o8;
+}
+
+__device__ void IntTReg(const DynaMat& m, const ElemPoints& P, const double* src, double n, double* out)
+{
+    double acc[9];
+    for (int k = 0; k < 9; ++k)
+        acc[k] = 0.0;
+    for (int i = 0; i < kNG2; ++i) {
+        Dist d;
+        MakeDist(src, P.rg_pos[i], d);
+        double T1[9], T2[9];
+        int flag = DynaT(m, 1, n, d, P.rg_nor[i], T1);
+        flag += DynaT(m, 2, n, d, P.rg_nor[i], T2);
+        if (flag) {
+            double w = P.rg_w[i];
+            for (int k = 0; k < 9; ++k)
+                acc[k] += (T1[k] + T2[k]) * w;
+        }
+    }
+    for (int k = 0; k < 9; ++k)
+        out[k] = acc[k];
+}
+
+__device__ int JudgeU(const DynaMat& m, const ElemGeo& e, const ElemPoints& P, const double* src, double n,
+                      double* out)
+{
+    int f = NeedCal(m, e, 1, n, src);
+    if (f == 1)
+        IntUPW(m, e, P, src, n, out);
+    else if (f == 2)
+        IntUReg(m, P, src, n, out);
+    return f;
+}
+
+__device__ int JudgeT(const DynaMat& m, const ElemGeo& e, const ElemPoints& P, int type, const double* src,
+                      double n, double* out)
+{
+    int f = NeedCal(m, e, type, n, src);
+    if (f == 1)
+        IntTPW(m, e, P, type, src, n, out);
+    else if (f == 2)
+        IntTReg(m, P, src, n, out);
+    return f;
+}
+
+// MultiDomainCCSRBuilder::AddBlock semantics for one key (transpose * sign,
+// 1e-14 drop of the input and of the accumulated value).
+struct BlockAcc {
+    double v[9];
+    bool alive = false;
+    __device__ void Add(double sign, const double* b)
+    {
+        bool zero = true, finite = true;
+        double mx = 0.0;
+        for (int i = 0; i < 9; ++i) {
+            if (b[i] != 0.0)
+                zero = false;
+            double a = fabs(b[i]);
+            if (a > mx)
+                mx = a;
+            if (!(b[i] == b[i] && a <= 1.7976931348623157e308))
+                finite = false;
+        }
+        if (sign == 0.0 || zero || mx <= kAbsTol || !finite)
+            return;
+        if (!alive) {
+            for (int i = 0; i < 9; ++i)
+                v[i] = 0.0;
+            alive = true;
+        }
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c)
+                v[r * 3 + c] += sign * b[c * 3 + r];
+        double m2 = 0.0;
+        for (int i = 0; i < 9; ++i) {
+            double a = fabs(v[i]);
+            if (a > m2)
+                m2 = a;
+        }
+        if (m2 <= kAbsTol)
+            alive = false;
+    }
+};
+
+__device__ __forceinline__ void DecodePair(const PairGrid& g, long p, int& s, int& f, int& d)
+{
+    d = 0;
+    while (d + 1 < g.nd && p >= g.pairBase[d + 1])
+        ++d;
+    long local = p - g.pairBase[d];
+    long nd = g.eleCount[d];
+    s = g.eleBegin[d] + (int)(local / nd);
+    f = g.eleBegin[d] + (int)(local % nd);
+}
+
+}  // namespace dev
+
+// ---------------------------------------------------------------------------
+// Setup kernels
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_points(const ElemGeo* __restrict__ geo, const QuadTables* __restrict__ q,
+                                                ElemPoints* __restrict__ pts, int ne)
+{
+    constexpr int kPer = kNPW2 * kNPWG2 + kNG2;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)ne * kPer)
+        return;
+    int e = (int)(t / kPer), i = (int)(t % kPer);
+    const ElemGeo& g = geo[e];
+    ElemPoints& P = pts[e];
+    if (i < kNPW2 * kNPWG2) {
+        int p = i / kNPWG2, m = i % kNPWG2;
+        double s1 = q->lpospw[p][m][0], s2 = q->lpospw[p][m][1];
+        FromLocal(g.cc, s1, s2, P.pw_pos[i]);
+        NormalAt(g.cc, s1, s2, P.pw_nor[i]);
+        P.pw_w[i] = q->pwgv[p][m] * Jacobi(g.cc, s1, s2);
+    } else {
+        int r = i - kNPW2 * kNPWG2;
+        double s1 = q->rp[r][0], s2 = q->rp[r][1];
+        FromLocal(g.cc, s1, s2, P.rg_pos[r]);
+        NormalAt(g.cc, s1, s2, P.rg_nor[r]);
+        P.rg_w[r] = q->nrgv0[r] * Jacobi(g.cc, s1, s2);
+    }
+}
+
+void LaunchPoints(const ElemGeo* geo, const QuadTables* q, ElemPoints* pts, int ne, hipStream_t s)
+{
+    long n = (long)ne * (kNPW2 * kNPWG2 + kNG2);
+    k_points<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(geo, q, pts, ne);
+}
+
+// Singular self integrals: InElementStaticTij + PthInElementIntDynamic
+// (DBEM1/dsquareelement.cpp:2292-2427, DBEM1/dsquareelementdyna.cpp:239-490).
+// One lane per element keeps the reference's summation order.
+__global__ void __launch_bounds__(64) k_singular(const ElemGeo* __restrict__ geo, const int* __restrict__ dom,
+                                                 const QuadTables* __restrict__ q, double* u1o, double* t1o,
+                                                 double* t2o, int ne)
+{
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne)
+        return;
+    const ElemGeo& g = geo[e];
+    const DynaMat& m = c_mats[dom[e]];
+    double node[3];
+    FromLocal(g.cc, 0.0, 0.0, node);
+    double st[9];
+    for (int k = 0; k < 9; ++k)
+        st[k] = 0.0;
+    double coef1 = -1.0 / (8.0 * kPi * (1.0 - m.v));
+    double coef2 = 1.0 - 2.0 * m.v;
+    // strong-singular static part: area terms, then F(-1) subtraction
+    for (int a = 0; a < 4; ++a)
+        for (int j = 0; j < kNSG2; ++j) {
+            double s1 = q->secconstpos[a][0][j], s2 = q->secconstpos[a][1][j];
+            double J = Jacobi(g.cc, s1, s2);
+            double nor[3], p[3], R[4], T[9];
+            NormalAt(g.cc, s1, s2, nor);
+            FromLocal(g.cc, s1, s2, p);
+            R[1] = p[0] - node[0];
+            R[2] = p[1] - node[1];
+            R[3] = p[2] - node[2];
+            R[0] = sqrt(R[1] * R[1] + R[2] * R[2] + R[3] * R[3]);
+            dev::StaticT(m, R, nor, T);
+            double temp = q->secconstval[a][j] * J;
+            for (int l = 0; l < 9; ++l)
+                st[l] += T[l] * temp;
+        }
+    double Jh[3], D1[3], D2[3];
+    {
+        double tj = Jacobi(g.cc, q->lc[0][0], q->lc[0][1]);
+        double n0[3];
+        NormalAt(g.cc, q->lc[0][0], q->lc[0][1], n0);
+        Jh[0] = tj * n0[0];
+        Jh[1] = tj * n0[1];
+        Jh[2] = tj * n0[2];
+    }
+    RDiffs1(g.cc, 0.0, 0.0, D1);
+    RDiffs2(g.cc, 0.0, 0.0, D2);
+    for (int a = 0; a < 4; ++a)
+        for (int j = 0; j < kNSG; ++j) {
+            double cs = q->linecos[a][j], sn = q->linesin[a][j];
+            double A[6], F[9];
+            A[1] = D1[0] * cs + D2[0] * sn;
+            A[2] = D1[1] * cs + D2[1] * sn;
+            A[3] = D1[2] * cs + D2[2] * sn;
+            A[0] = sqrt(A[1] * A[1] + A[2] * A[2] + A[3] * A[3]);
+            A[4] = A[0] * A[0] * A[0];
+            A[5] = A[4] * A[0] * A[0];
+            // GetFijMinusOne (DBEM1/dsquareelement.cpp:609-639)
+            double AkJk0 = A[1] * Jh[0] + A[2] * Jh[1] + A[3] * Jh[2];
+            double t1 = 3.0 * A[1] * AkJk0 / A[5];
+            double t2 = coef2 * (A[2] * Jh[0] - A[1] * Jh[1]) / A[4];
+            double t3 = coef2 * (A[3] * Jh[0] - A[1] * Jh[2]) / A[4];
+            double t4 = coef2 * (A[3] * Jh[1] - A[2] * Jh[2]) / A[4];
+            double temp = coef2 * AkJk0 / A[4];
+            F[0] = coef1 * (temp + t1 * A[1]);
+            F[1] = coef1 * (t1 * A[2] - t2);
+            F[2] = coef1 * (t1 * A[3] - t3);
+            F[3] = coef1 * (t1 * A[2] + t2);
+            F[4] = coef1 * (temp + 3.0 * A[2] * A[2] * AkJk0 / A[5]);
+            t1 = 3.0 * A[3] * AkJk0 / A[5];
+            F[5] = coef1 * (t1 * A[2] - t4);
+            F[6] = coef1 * (t1 * A[1] + t3);
+            F[7] = coef1 * (t1 * A[2] + t4);
+            F[8] = coef1 * (temp + t1 * A[3]);
+            for (int l = 0; l < kNSG; ++l) {
+                double fv = q->constfm1secval[a][l * kNSG + j];
+                for (int k = 0; k < 9; ++k)
+                    st[k] -= F[k] * fv;
+            }
+            double fl = q->constfm1lineval[a][j];
+            for (int k = 0; k < 9; ++k)
+                st[k] += F[k] * fl;
+        }
+    // dynamic parts over the 4 x 400 x 9 polar points
+    double u0 = 0.0, u1 = 0.0, u2 = 0.0, u4 = 0.0, u5 = 0.0, u8 = 0.0;
+    for (int a = 0; a < 4; ++a)
+        for (int k = 0; k < kNSPW2; ++k)
+            for (int l = 0; l < kNSPWG2; ++l) {
+                double s1 = q->secconstpospw[a][0][k][l], s2 = q->secconstpospw[a][1][k][l];
+                double J = Jacobi(g.cc, s1, s2);
+                double p[3];
+                FromLocal(g.cc, s1, s2, p);
+                dev::Dist d;
+                dev::MakeDist(node, p, d);
+                double U[9];
+                dev::DynaU(m, 0.0, d, U);
+                double temp = q->secconstvalpw[a][k][l] * J;
+                u0 += U[0] * temp;
+                u1 += U[1] * temp;
+                u2 += U[2] * temp;
+                u4 += U[4] * temp;
+                u5 += U[5] * temp;
+                u8 += U[8] * temp;
+            }
+    double* uo = u1o + 9 * (long)e;
+    uo[0] = u0; uo[1] = u1; uo[2] = u2; uo[3] = u1; uo[4] = u4;
+    uo[5] = u5; uo[6] = u2; uo[7] = u5; uo[8] = u8;
+    double a1[9], a2[9];
+    for (int k = 0; k < 9; ++k) {
+        a1[k] = 0.0;
+        a2[k] = 0.0;
+    }
+    for (int a = 0; a < 4; ++a)
+        for (int k = 0; k < kNSPW2; ++k)
+            for (int l = 0; l < kNSPWG2; ++l) {
+                double s1 = q->secconstpospw[a][0][k][l], s2 = q->secconstpospw[a][1][k][l];
+                double J = Jacobi(g.cc, s1, s2);
+                double nor[3], p[3];
+                NormalAt(g.cc, s1, s2, nor);
+                FromLocal(g.cc, s1, s2, p);
+                dev::Dist d;
+                dev::MakeDist(node, p, d);
+                double SR[4] = {d.R[1], d.RI[1], d.RI[2], d.RI[3]};
+                double T1[9], T2[9], ST[9];
+                dev::DynaT(m, 1, 0.0, d, nor, T1);
+                dev::DynaT(m, 2, 1.0, d, nor, T2);
+                dev::StaticT(m, SR, nor, ST);
+                double temp = q->secconstvalpw[a][k][l] * J;
+                for (int c = 0; c < 9; ++c) {
+                    a1[c] += (T1[c] - ST[c]) * temp;
+                    a2[c] += (T2[c] - ST[c]) * temp;
+                }
+            }
+    double* to1 = t1o + 9 * (long)e;
+    double* to2 = t2o + 9 * (long)e;
+    for (int c = 0; c < 9; ++c) {
+        to1[c] = a1[c] + st[c];
+        to2[c] = a2[c] + st[c];
+    }
+    to1[0] += 0.5;
+    to1[4] += 0.5;
+    to1[8] += 0.5;
+}
+
+void LaunchSingular(const ElemGeo* geo, const int* dom, const QuadTables* q, double* u1, double* t1, double* t2,
+                    int ne, hipStream_t s)
+{
+    k_singular<<<(ne + 63) / 64, 64, 0, s>>>(geo, dom, q, u1, t1, t2, ne);
+}
+
+// ---------------------------------------------------------------------------
+// Causality cull: one lane per (source, field) pair.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_cull(const ElemGeo* __restrict__ geo, const int* __restrict__ dom,
+                                              PairGrid grid, long npairs, int maxn, int* kb, int* len,
+                                              unsigned char* nearFlag)
+{
+    long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npairs)
+        return;
+    int s, f, d;
+    dev::DecodePair(grid, p, s, f, d);
+    const DynaMat& m = c_mats[d];
+    CullBounds b = PairBounds(m, geo[f], geo[s].center);
+    long cap = (long)maxn + 2;
+    long k1 = LastActive(m, b.a2, cap);
+    long k0 = FirstActive(m, b.b1, cap);
+    long kbeg = k0 > 1 ? k0 : 1;
+    long kend = k1 + 1 < maxn ? k1 + 1 : maxn;
+    if (s == f)
+        kbeg = 1;
+    long L = (k1 >= k0 || s == f) ? kend - kbeg + 1 : 0;
+    if (L < 0)
+        L = 0;
+    kb[p] = (int)kbeg;
+    len[p] = (int)L;
+    nearFlag[p] = (unsigned char)(((k0 <= 1 && k1 >= k0) || s == f) ? 1 : 0);
+}
+
+void LaunchCull(const ElemGeo* geo, const int* dom, PairGrid grid, long npairs, int maxn, int* kb, int* len,
+                unsigned char* nearFlag, hipStream_t s)
+{
+    k_cull<<<(unsigned)((npairs + 255) / 256), 256, 0, s>>>(geo, dom, grid, npairs, maxn, kb, len, nearFlag);
+}
+
+// ---------------------------------------------------------------------------
+// History quadrature: one lane per (pair, lag) band entry.
+// MTS entry = AddBlock(T2) then AddBlock(T1); MGS entry = AddBlock(U)
+// (DBEM1/multidomain.cpp:2017-2078).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_history(const ElemGeo* __restrict__ geo, const ElemPoints* __restrict__ pts,
+                                                 const int* __restrict__ dom, const NodeRefs* __restrict__ refs,
+                                                 const double* __restrict__ selfT2, PairGrid grid,
+                                                 const long* __restrict__ off, const int* __restrict__ kb, long npairs,
+                                                 long e0, long e1, double* __restrict__ mts, double* __restrict__ mgs,
+                                                 unsigned char* __restrict__ alive)
+{
+    long e = e0 + (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= e1)
+        return;
+    // last pair with off[p] <= e (that pair has a non-empty band)
+    long lo = 0, hi = npairs;  // off has npairs+1 entries
+    while (hi - lo > 1) {
+        long mid = (lo + hi) >> 1;
+        if (off[mid] <= e)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    long p = lo;
+    int s, f, d;
+    dev::DecodePair(grid, p, s, f, d);
+    int k = kb[p] + (int)(e - off[p]);
+    const DynaMat& m = c_mats[d];
+    const ElemGeo& fe = geo[f];
+    const ElemPoints& P = pts[f];
+    const double* src = geo[s].center;
+    double n = (double)k;
+    double blk[9];
+    dev::BlockAcc acc;
+    double su = refs[f].hu_sign;
+    if (k <= 1 && s == f) {
+        for (int i = 0; i < 9; ++i)
+            blk[i] = selfT2[9 * (long)f + i];
+        acc.Add(su, blk);
+    } else if (dev::JudgeT(m, fe, P, 2, src, n, blk)) {
+        acc.Add(su, blk);
+    }
+    if (dev::JudgeT(m, fe, P, 1, src, n, blk))
+        acc.Add(su, blk);
+    double* out = mts + 9 * e;
+    for (int i = 0; i < 9; ++i)
+        out[i] = acc.alive ? acc.v[i] : 0.0;
+    unsigned char bits = acc.alive ? 1 : 0;
+    dev::BlockAcc accg;
+    if (dev::JudgeU(m, fe, P, src, n, blk))
+        accg.Add(refs[f].ht_sign, blk);
+    double* og = mgs + 6 * e;
+    const int pick[6] = {0, 1, 2, 4, 5, 8};
+    for (int i = 0; i < 6; ++i)
+        og[i] = accg.alive ? accg.v[pick[i]] : 0.0;
+    bits |= accg.alive ? 2 : 0;
+    alive[e] = bits;
+}
+
+void LaunchHistory(const ElemGeo* geo, const ElemPoints* pts, const int* dom, const NodeRefs* refs,
+                   const double* selfT2, PairGrid grid, const long* off, const int* kb, long npairs, long e0, long e1,
+                   double* mts, double* mgs, unsigned char* alive, hipStream_t s)
+{
+    long n = e1 - e0;
+    if (n <= 0)
+        return;
+    k_history<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(geo, pts, dom, refs, selfT2, grid, off, kb, npairs, e0, e1,
+                                                          mts, mgs, alive);
+}
+
+// ---------------------------------------------------------------------------
+// Step-0 coefficient blocks (DBEM1/dsquareelement.cpp:2756-2868,3281-3394,
+// 4215-4550,5157-5491,5922-6078,6476-6632) for FlagDynaMethod 2:
+// U: 4 U(0) + U(1);  T: 4 T1(0) + T2(1) + T1(1);  then +-K * R^T.
+// ---------------------------------------------------------------------------
+namespace dev {
+
+__device__ int Step0Coef(const DynaMat& m, const ElemGeo& fe, const ElemPoints& P, bool self, const double* u1,
+                         const double* t1, const double* t2, const double* src, bool unknownPass, int bcid,
+                         double* coef)
+{
+    bool useU = unknownPass ? (bcid == 123) : (bcid == 456);
+    bool plus = unknownPass ? (bcid != 123) : (bcid == 456);
+    double K[9], tmp[9];
+    for (int k = 0; k < 9; ++k) {
+        K[k] = 0.0;
+        coef[k] = 0.0;
+    }
+    int fsum;
+    if (self) {
+        if (useU) {
+            for (int k = 0; k < 9; ++k)
+                K[k] += 4.0 * u1[k];
+            IntUPW(m, fe, P, src, 1.0, tmp);
+        } else {
+            for (int k = 0; k < 9; ++k) {
+                K[k] += 4.0 * t1[k];
+                K[k] += t2[k];
+            }
+            IntTPW(m, fe, P, 1, src, 1.0, tmp);
+        }
+        for (int k = 0; k < 9; ++k)
+            K[k] += tmp[k];
+        fsum = 1;
+    } else {
+        int f0 = NeedCal(m, fe, 1, 0.0, src);
+        fsum = f0;
+        if (f0) {
+            if (useU) {
+                if (f0 == 1)
+                    IntUPW(m, fe, P, src, 0.0, tmp);
+                else
+                    IntUReg(m, P, src, 0.0, tmp);
+                for (int k = 0; k < 9; ++k)
+                    K[k] += 4.0 * tmp[k];
+            } else {
+                if (f0 == 1)
+                    IntTPW(m, fe, P, 1, src, 0.0, tmp);
+                else
+                    IntTReg(m, P, src, 0.0, tmp);
+                for (int k = 0; k < 9; ++k)
+                    K[k] += 4.0 * tmp[k];
+                if (f0 == 1)
+                    IntTPW(m, fe, P, 2, src, 1.0, tmp);
+                else
+                    IntTReg(m, P, src, 1.0, tmp);
+                for (int k = 0; k < 9; ++k)
+                    K[k] += tmp[k];
+            }
+        }
+        int f1 = NeedCal(m, fe, 1, 1.0, src);
+        fsum += f1;
+        if (f1) {
+            if (useU) {
+                if (f1 == 1)
+                    IntUPW(m, fe, P, src, 1.0, tmp);
+                else
+                    IntUReg(m, P, src, 1.0, tmp);
+            } else {
+                if (f1 == 1)
+                    IntTPW(m, fe, P, 1, src, 1.0, tmp);
+                else
+                    IntTReg(m, P, src, 1.0, tmp);
+            }
+            for (int k = 0; k < 9; ++k)
+                K[k] += tmp[k];
+        }
+        if (!fsum)
+            return 0;
+    }
+    const double* R = fe.trans;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            for (int k = 0; k < 3; ++k) {
+                if (plus)
+                    coef[3 * j + i] += K[3 * k + i] * R[3 * j + k];
+                else
+                    coef[3 * j + i] -= K[3 * k + i] * R[3 * j + k];
+            }
+    return fsum;
+}
+
+// TransformColumnBlock9 (DBEM1/multidomain.cpp:1668-1680)
+__device__ void TransformColumnBlock(const double* b, const double* T, double* out)
+{
+    for (int r = 0; r < 3; ++r)
+        for (int q = 0; q < 3; ++q) {
+            double v = 0.0;
+            for (int c = 0; c < 3; ++c)
+                v += b[c * 3 + r] * T[c * 3 + q];
+            out[q * 3 + r] = v;
+        }
+}
+
+}  // namespace dev
+
+__global__ void __launch_bounds__(64) k_step0(const ElemGeo* __restrict__ geo, const ElemPoints* __restrict__ pts,
+                                              const int* __restrict__ dom, const NodeRefs* __restrict__ refs,
+                                              const double* __restrict__ su1, const double* __restrict__ st1,
+                                              const double* __restrict__ st2, PairGrid grid,
+                                              const long* __restrict__ pairIdx, long count, double* out, int* flags)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count)
+        return;
+    int s, f, d;
+    dev::DecodePair(grid, pairIdx[i], s, f, d);
+    const DynaMat& m = c_mats[d];
+    const ElemGeo& fe = geo[f];
+    const ElemPoints& P = pts[f];
+    const double* src = geo[s].center;
+    bool self = s == f;
+    const double* u1 = su1 + 9 * (long)f;
+    const double* t1 = st1 + 9 * (long)f;
+    const double* t2 = st2 + 9 * (long)f;
+    double b0[9], b1[9];
+    int fl0, fl1;
+    const NodeRefs& r = refs[f];
+    if (r.surface == 1) {
+        double c[9];
+        fl0 = dev::Step0Coef(m, fe, P, self, u1, t1, t2, src, true, 456, c);
+        dev::TransformColumnBlock(c, r.itf_T, b0);
+        fl1 = dev::Step0Coef(m, fe, P, self, u1, t1, t2, src, true, 123, c);
+        dev::TransformColumnBlock(c, r.itf_T, b1);
+    } else {
+        fl0 = dev::Step0Coef(m, fe, P, self, u1, t1, t2, src, true, r.bcid, b0);
+        fl1 = dev::Step0Coef(m, fe, P, self, u1, t1, t2, src, false, r.bcid, b1);
+    }
+    for (int k = 0; k < 9; ++k) {
+        out[18 * i + k] = b0[k];
+        out[18 * i + 9 + k] = b1[k];
+    }
+    flags[2 * i] = fl0;
+    flags[2 * i + 1] = fl1;
+}
+
+void LaunchStep0(const ElemGeo* geo, const ElemPoints* pts, const int* dom, const NodeRefs* refs,
+                 const double* selfU1, const double* selfT1, const double* selfT2, PairGrid grid,
+                 const long* pairIdx, long count, double* out, int* flags, hipStream_t s)
+{
+    if (count <= 0)
+        return;
+    k_step0<<<(unsigned)((count + 63) / 64), 64, 0, s>>>(geo, pts, dom, refs, selfU1, selfT1, selfT2, grid, pairIdx,
+                                                         count, out, flags);
+}
+
+// ---------------------------------------------------------------------------
+// History convolution (DBEM1/multidomain.cpp:3201-3225,3849-3866):
+//   rhs_K += sum_j [ MGS_j (T_{K-j+1} + 2T_{K-j} + T_{K-j-1})
+//                  - MTS_j (U_{K-j+1} + 2U_{K-j} + U_{K-j-1}) ]
+// with state indices restricted to (0, K).  One workgroup per source row;
+// a wavefront walks one pair's band with one lag per lane, so band blocks
+// and the per-column state time series are both read contiguously.
+// State history layout: g[(col * nstore + m) * 3 + c].
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_conv(PairGrid grid, const NodeRefs* __restrict__ refs,
+                                              const long* __restrict__ off, const int* __restrict__ kb,
+                                              const int* __restrict__ len, const double* __restrict__ mts,
+                                              const double* __restrict__ mgs, const double* __restrict__ gU,
+                                              const double* __restrict__ gT, int nstore, int K, int maxn,
+                                              double* __restrict__ rhs)
+{
+    int s = blockIdx.x;
+    int d = refs[s].domain;
+    int nd = grid.eleCount[d];
+    long rowStart = grid.pairBase[d] + (long)(s - grid.eleBegin[d]) * nd;
+    int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int jmax = K < maxn ? K : maxn;
+    double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+    for (int q = wave; q < nd; q += 4) {
+        long p = rowStart + q;
+        int L = len[p];
+        if (L == 0)
+            continue;
+        int k0 = kb[p];
+        int f = grid.eleBegin[d] + q;
+        const double* su = gU + (long)refs[f].hu_block * nstore * 3;
+        const double* st = gT + (long)refs[f].ht_block * nstore * 3;
+        long base = off[p];
+        for (int t = lane; t < L; t += 64) {
+            int j = k0 + t;
+            if (j > jmax)
+                break;
+            long e = base + t;
+            int mm[3] = {K - j + 1, K - j, K - j - 1};
+            const double co[3] = {1.0, 2.0, 1.0};
+            double cu0 = 0.0, cu1 = 0.0, cu2 = 0.0, ct0 = 0.0, ct1 = 0.0, ct2 = 0.0;
+            for (int z = 0; z < 3; ++z) {
+                int mz = mm[z];
+                if (mz > 0 && mz < K) {
+                    const double* a = su + 3 * (long)mz;
+                    const double* b = st + 3 * (long)mz;
+                    cu0 += co[z] * a[0];
+                    cu1 += co[z] * a[1];
+                    cu2 += co[z] * a[2];
+                    ct0 += co[z] * b[0];
+                    ct1 += co[z] * b[1];
+                    ct2 += co[z] * b[2];
+                }
+            }
+            const double* M = mts + 9 * e;
+            const double* G = mgs + 6 * e;
+            y0 += G[0] * ct0 + G[1] * ct1 + G[2] * ct2;
+            y1 += G[1] * ct0 + G[3] * ct1 + G[4] * ct2;
+            y2 += G[2] * ct0 + G[4] * ct1 + G[5] * ct2;
+            y0 -= M[0] * cu0 + M[1] * cu1 + M[2] * cu2;
+            y1 -= M[3] * cu0 + M[4] * cu1 + M[5] * cu2;
+            y2 -= M[6] * cu0 + M[7] * cu1 + M[8] * cu2;
+        }
+    }
+    __shared__ double red[3][256];
+    red[0][threadIdx.x] = y0;
+    red[1][threadIdx.x] = y1;
+    red[2][threadIdx.x] = y2;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + w];
+            red[1][threadIdx.x] += red[1][threadIdx.x + w];
+            red[2][threadIdx.x] += red[2][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        rhs[3 * (long)s] += red[0][0];
+        rhs[3 * (long)s + 1] += red[1][0];
+        rhs[3 * (long)s + 2] += red[2][0];
+    }
+}
+
+void LaunchConvolution(PairGrid grid, const NodeRefs* refs, const long* off, const int* kb, const int* len,
+                       const double* mts, const double* mgs, const double* gU, const double* gT, int nstore, int K,
+                       int maxn, int ne, double* rhs, hipStream_t s)
+{
+    k_conv<<<ne, 256, 0, s>>>(grid, refs, off, kb, len, mts, mgs, gU, gT, nstore, K, maxn, rhs);
+}
+
+// ---------------------------------------------------------------------------
+// Linear algebra for GMRES
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_spmv(long nrows, const int* __restrict__ rp, const int* __restrict__ col,
+                                              const double* __restrict__ val, const double* __restrict__ x,
+                                              double* __restrict__ y)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nrows)
+        return;
+    double b0 = 0.0, b1 = 0.0, b2 = 0.0;
+    for (int j = rp[i]; j < rp[i + 1]; ++j) {
+        const double* v = val + 9 * (long)j;
+        const double* xc = x + 3 * (long)col[j];
+        double x0 = xc[0], x1 = xc[1], x2 = xc[2];
+        b0 += v[0] * x0 + v[1] * x1 + v[2] * x2;
+        b1 += v[3] * x0 + v[4] * x1 + v[5] * x2;
+        b2 += v[6] * x0 + v[7] * x1 + v[8] * x2;
+    }
+    y[3 * i] = b0;
+    y[3 * i + 1] = b1;
+    y[3 * i + 2] = b2;
+}
+
+void LaunchSpmv(long nrows, const int* rp, const int* col, const double* val, const double* x, double* y,
+                hipStream_t s)
+{
+    k_spmv<<<(unsigned)((nrows + 255) / 256), 256, 0, s>>>(nrows, rp, col, val, x, y);
+}
+
+// GMRES_M_X (DBEM1/pre_gmres.cpp:837-895): each output scalar is owned by
+// one lane and summed in the reference order.
+__global__ void __launch_bounds__(256) k_precond(long n, const long* __restrict__ posLeaf,
+                                                 const long* __restrict__ leafBegin, const long* __restrict__ leafSize,
+                                                 const long* __restrict__ matOff, const double* __restrict__ inv,
+                                                 const long* __restrict__ repid, const long* __restrict__ outBlock,
+                                                 const double* __restrict__ x, double* __restrict__ y)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 3 * n)
+        return;
+    long i = t / 3;
+    int mrow = (int)(t % 3);
+    long leaf = posLeaf[i];
+    long b = leafBegin[leaf], c = leafSize[leaf], jl = i - b, n3 = 3 * c;
+    const double* P = inv + matOff[leaf] + (3 * jl + mrow) * n3;
+    double acc = 0.0;
+    for (long k = 0; k < c; ++k) {
+        const double* xc = x + 3 * repid[b + k];
+        acc += P[3 * k] * xc[0];
+        acc += P[3 * k + 1] * xc[1];
+        acc += P[3 * k + 2] * xc[2];
+    }
+    y[3 * outBlock[i] + mrow] = acc;
+}
+
+void LaunchPrecond(long n, const long* posLeaf, const long* leafBegin, const long* leafSize, const long* matOff,
+                   const double* inv, const long* repid, const long* outBlock, const double* x, double* y,
+                   hipStream_t s)
+{
+    k_precond<<<(unsigned)((3 * n + 255) / 256), 256, 0, s>>>(n, posLeaf, leafBegin, leafSize, matOff, inv, repid,
+                                                              outBlock, x, y);
+}
+
+// Deterministic two-stage reductions (fixed partition and order).
+__global__ void __launch_bounds__(256) k_multidot(const double* __restrict__ V, long ldv, int nv,
+                                                  const double* __restrict__ u, long n, double* __restrict__ partial)
+{
+    __shared__ double red[256];
+    for (int k = 0; k < nv; ++k) {
+        double a = 0.0;
+        for (long r = (long)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (long)gridDim.x * blockDim.x)
+            a += V[k * ldv + r] * u[r];
+        red[threadIdx.x] = a;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w)
+                red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0)
+            partial[(long)blockIdx.x * nv + k] = red[0];
+        __syncthreads();
+    }
+}
+
+__global__ void k_finish(const double* __restrict__ partial, int nblocks, int nv, double* __restrict__ h, int doSqrt)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nv)
+        return;
+    double a = 0.0;
+    for (int b = 0; b < nblocks; ++b)
+        a += partial[(long)b * nv + k];
+    h[k] = doSqrt ? sqrt(a) : a;
+}
+
+void LaunchMultiDot(const double* V, long ldv, int nv, const double* u, long n, double* partial, int nblocks,
+                    double* h, hipStream_t s)
+{
+    k_multidot<<<nblocks, 256, 0, s>>>(V, ldv, nv, u, n, partial);
+    k_finish<<<(nv + 63) / 64, 64, 0, s>>>(partial, nblocks, nv, h, 0);
+}
+
+void LaunchNorm(const double* x, long n, double* partial, int nblocks, double* out, hipStream_t s)
+{
+    k_multidot<<<nblocks, 256, 0, s>>>(x, n, 1, x, n, partial);
+    k_finish<<<1, 64, 0, s>>>(partial, nblocks, 1, out, 1);
+}
+
+__global__ void __launch_bounds__(256) k_multiaxpy(const double* __restrict__ V, long ldv, int nv,
+                                                   const double* __restrict__ h, double* __restrict__ u, long n,
+                                                   double* htot)
+{
+    long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (htot && r < nv)
+        htot[r] += h[r];
+    if (r >= n)
+        return;
+    double a = u[r];
+    for (int k = 0; k < nv; ++k)
+        a -= h[k] * V[k * ldv + r];
+    u[r] = a;
+}
+
+void LaunchMultiAxpy(const double* V, long ldv, int nv, const double* h, double* u, long n, double* htot,
+                     hipStream_t s)
+{
+    long g = n > nv ? n : nv;
+    k_multiaxpy<<<(unsigned)((g + 255) / 256), 256, 0, s>>>(V, ldv, nv, h, u, n, htot);
+}
+
+// y = x / a[0] (invert) or y = x * a[0]; a zero or tiny denominator (<= 1e-12)
+// gives y = 0, as the reference does for a lucky breakdown.
+__global__ void __launch_bounds__(256) k_scale(const double* __restrict__ x, const double* __restrict__ a, int invert,
+                                               double* __restrict__ y, long n)
+{
+    long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n)
+        return;
+    double s = a[0];
+    if (invert)
+        y[r] = s <= 1.0e-12 ? 0.0 : x[r] / s;
+    else
+        y[r] = x[r] * s;
+}
+
+void LaunchScaleCopy(const double* x, const double* a, int invert, double* y, long n, hipStream_t s)
+{
+    k_scale<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(x, a, invert, y, n);
+}
+
+__global__ void __launch_bounds__(256) k_residual(const double* __restrict__ b, double* __restrict__ y, long n)
+{
+    long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n)
+        y[r] = b[r] - y[r];
+}
+
+void LaunchResidual(const double* b, double* y, long n, hipStream_t s)
+{
+    k_residual<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(b, y, n);
+}
+
+// Givens rotation in QT form for column j (DBEM1/pre_gmres.cpp:1492-1533),
+// FP64 on the device.  h is 1-based (h[1..j+1]); QT is (restart+1)^2 and R is
+// restart^2, both row-major 1-based via the macros below.
+__global__ void __launch_bounds__(256) k_givens(double* h, double* QT, double* R, double* f, double* phibar,
+                                                int* stag, int j, int restart)
+{
+    const int lq = restart + 1;
+#define QQ(a, b) QT[((a)-1) * lq + ((b)-1)]
+#define RR(a, b) R[((a)-1) * restart + ((b)-1)]
+    for (int k = 1 + (int)threadIdx.x; k <= j; k += blockDim.x) {
+        double v = 0.0;
+        for (int l = 1; l <= j; ++l)
+            v += QQ(k, l) * h[l];
+        RR(k, j) = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double rt = RR(j, j), c, s, temp;
+        double hj1 = h[j + 1];
+        if (hj1 == 0.0) {
+            c = 1.0;
+            s = 0.0;
+        } else if (hj1 * hj1 > rt * rt) {
+            temp = rt / hj1;
+            s = 1.0 / sqrt(1.0 + fabs(temp) * fabs(temp));
+            c = -temp * s;
+        } else {
+            temp = hj1 / rt;
+            c = 1.0 / sqrt(1.0 + fabs(temp) * fabs(temp));
+            s = -temp * c;
+        }
+        RR(j, j) = c * rt - s * hj1;
+        for (int k = 1; k <= j; ++k) {
+            double qk = QQ(j, k);
+            QQ(j, k) = c * qk;
+            QQ(j + 1, k) = s * qk;
+        }
+        QQ(j, j + 1) = -s;
+        QQ(j + 1, j + 1) = c;
+        f[j] = c * phibar[0];
+        phibar[0] = s * phibar[0];
+        if (j < restart && f[j] == 0.0)
+            stag[0] = 1;
+    }
+#undef QQ
+#undef RR
+}
+
+void LaunchGivens(double* h, double* QT, double* R, double* f, double* phibar, int* stag, int j, int restart,
+                  hipStream_t s)
+{
+    k_givens<<<1, 256, 0, s>>>(h, QT, R, f, phibar, stag, j, restart);
+}
+
+// W(:,j) = (V(:,j) - sum_{l<j} W(:,l) R(l,j)) / R(j,j);  x += f_j W(:,j)
+// (DBEM1/pre_gmres.cpp:1535-1555).  V and W column-major with ld = n.
+__global__ void __launch_bounds__(256) k_wupdate(const double* __restrict__ V, double* __restrict__ W, long n,
+                                                 const double* __restrict__ R, int j, int restart,
+                                                 const double* __restrict__ f, double* __restrict__ x)
+{
+    long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n)
+        return;
+    double rjj = R[(j - 1) * restart + (j - 1)];
+    double w = V[(long)(j - 1) * n + k] / rjj;
+    for (int l = 1; l <= j - 1; ++l)
+        w -= W[(long)(l - 1) * n + k] * R[(l - 1) * restart + (j - 1)] / rjj;
+    W[(long)(j - 1) * n + k] = w;
+    x[k] += f[j] * w;
+}
+
+void LaunchWUpdate(const double* V, double* W, long n, const double* R, int j, int restart, const double* f,
+                   double* x, hipStream_t s)
+{
+    k_wupdate<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(V, W, n, R, j, restart, f, x);
+}
+
+// ---------------------------------------------------------------------------
+// State plumbing (DBEM1/multidomain.cpp:3047-3200; dsquareelement.h:88-122)
+// ---------------------------------------------------------------------------
+__global__ void k_known(const NodeRefs* __restrict__ refs, const double* __restrict__ bd, int ne,
+                        double* __restrict__ known)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ne)
+        return;
+    const NodeRefs& r = refs[i];
+    if (r.surface == 1)
+        return;
+    int blk = r.bcid == 123 ? r.u_block : r.t_block;
+    int kn = r.bcid == 123 ? r.u_known : r.t_known;
+    double sg = r.bcid == 123 ? r.u_sign : r.t_sign;
+    if (!kn)
+        return;
+    for (int c = 0; c < 3; ++c)
+        known[3 * (long)blk + c] = sg * bd[3 * (long)i + c];
+}
+
+void LaunchKnown(const NodeRefs* refs, const double* bd, int ne, double* known, long nknown, hipStream_t s)
+{
+    (void)nknown;
+    k_known<<<(ne + 255) / 256, 256, 0, s>>>(refs, bd, ne, known);
+}
+
+__global__ void k_scatter(const NodeRefs* __restrict__ refs, const ElemGeo* __restrict__ geo,
+                          const double* __restrict__ bd, const double* __restrict__ x, int ne, double* __restrict__ lu,
+                          double* __restrict__ lt)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ne)
+        return;
+    const NodeRefs& r = refs[i];
+    const double* T = r.itf_T;
+    double u[3], t[3];
+    const double* in = bd + 3 * (long)i;
+    if (!r.u_known) {
+        const double* v = x + 3 * (long)r.u_block;
+        for (int q = 0; q < 3; ++q)
+            u[q] = r.u_sign * (T[q * 3 + 0] * v[0] + T[q * 3 + 1] * v[1] + T[q * 3 + 2] * v[2]);
+    } else {
+        for (int q = 0; q < 3; ++q)
+            u[q] = r.u_sign * in[q];
+    }
+    if (!r.t_known) {
+        const double* v = x + 3 * (long)r.t_block;
+        for (int q = 0; q < 3; ++q)
+            t[q] = r.t_sign * (T[q * 3 + 0] * v[0] + T[q * 3 + 1] * v[1] + T[q * 3 + 2] * v[2]);
+    } else {
+        for (int q = 0; q < 3; ++q)
+            t[q] = r.t_sign * in[q];
+    }
+    const double* F = geo[i].trans;
+    double* ou = lu + 3 * (long)i;
+    double* ot = lt + 3 * (long)i;
+    ou[0] = F[0] * u[0] + F[3] * u[1] + F[6] * u[2];
+    ou[1] = F[1] * u[0] + F[4] * u[1] + F[7] * u[2];
+    ou[2] = F[2] * u[0] + F[5] * u[1] + F[8] * u[2];
+    ot[0] = F[0] * t[0] + F[3] * t[1] + F[6] * t[2];
+    ot[1] = F[1] * t[0] + F[4] * t[1] + F[7] * t[2];
+    ot[2] = F[2] * t[0] + F[5] * t[1] + F[8] * t[2];
+}
+
+void LaunchScatter(const NodeRefs* refs, const ElemGeo* geo, const double* bd, const double* x, int ne, double* lu,
+                   double* lt, hipStream_t s)
+{
+    k_scatter<<<(ne + 255) / 256, 256, 0, s>>>(refs, geo, bd, x, ne, lu, lt);
+}
+
+__global__ void k_state(const NodeRefs* __restrict__ refs, const double* __restrict__ lu,
+                        const double* __restrict__ lt, int ne, int K, int nstore, double* __restrict__ gU,
+                        double* __restrict__ gT)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ne)
+        return;
+    const NodeRefs& r = refs[i];
+    if (r.surface == 1 && !r.ref_side)
+        return;
+    double* du = gU + ((long)r.hu_block * nstore + K) * 3;
+    double* dt = gT + ((long)r.ht_block * nstore + K) * 3;
+    for (int c = 0; c < 3; ++c) {
+        du[c] = r.hu_sign * lu[3 * (long)i + c];
+        dt[c] = r.ht_sign * lt[3 * (long)i + c];
+    }
+}
+
+void LaunchState(const NodeRefs* refs, const double* lu, const double* lt, int ne, int K, int nstore, double* gU,
+                 double* gT, hipStream_t s)
+{
+    k_state<<<(ne + 255) / 256, 256, 0, s>>>(refs, lu, lt, ne, K, nstore, gU, gT);
+}
+
+__global__ void k_zero(double* x, long n)
+{
+    long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n)
+        x[r] = 0.0;
+}
+
+void LaunchZero(double* x, long n, hipStream_t s)
+{
+    if (n > 0)
+        k_zero<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(x, n);
+}
+
+}  // namespace tdbem

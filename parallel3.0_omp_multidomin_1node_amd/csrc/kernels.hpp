@@ -1,0 +1,101 @@
+// Device data layout and launch wrappers of the HIP kernels.
+#pragma once
+
+#include "common.hpp"
+
+namespace tdbem {
+
+// Quadrature points of one field element, precomputed once: position,
+// normal and weight*Jacobian of the 36x9 patch points and of the 4x4
+// regular points (recomputing them per triple would give the same values).
+struct ElemPoints {
+    double pw_pos[kNPW2 * kNPWG2][3];
+    double pw_nor[kNPW2 * kNPWG2][3];
+    double pw_w[kNPW2 * kNPWG2];
+    double rg_pos[kNG2][3];
+    double rg_nor[kNG2][3];
+    double rg_w[kNG2];
+};
+
+// Per-node variable references of the DOF map, flattened for the device.
+struct NodeRefs {
+    int u_block, t_block, hu_block, ht_block;
+    double u_sign, t_sign, hu_sign, ht_sign;
+    int u_known, t_known;
+    int bcid, domain;
+    int surface;   // 0 outer, 1 interface
+    int ref_side;  // 1 unless this is the B side of an interface
+    double itf_T[9];  // interface transform to the A frame (identity elsewhere)
+};
+
+// Pair grid: the pairs (source s, field f) of one domain d are numbered
+// pairBase[d] + (s - eleBegin[d]) * n_d + (f - eleBegin[d]), so the pairs of a
+// source row are contiguous.
+struct PairGrid {
+    int nd;
+    long pairBase[kMaxDomains + 1];
+    int eleBegin[kMaxDomains];
+    int eleCount[kMaxDomains];
+};
+
+void UploadDynaMats(const DynaMat* mats, int nd, hipStream_t s);
+
+void LaunchPoints(const ElemGeo* geo, const QuadTables* q, ElemPoints* pts, int ne, hipStream_t s);
+void LaunchSingular(const ElemGeo* geo, const int* dom, const QuadTables* q, double* u1, double* t1, double* t2,
+                    int ne, hipStream_t s);
+
+// Causality cull: band [kb, kb+len) of each pair and near-field flag.
+void LaunchCull(const ElemGeo* geo, const int* dom, PairGrid grid, long npairs, int maxn, int* kb, int* len,
+                unsigned char* nearFlag, hipStream_t s);
+
+// History quadrature for all band entries [e0, e1).
+void LaunchHistory(const ElemGeo* geo, const ElemPoints* pts, const int* dom, const NodeRefs* refs,
+                   const double* selfT2, PairGrid grid, const long* off, const int* kb, long npairs, long e0, long e1,
+                   double* mts, double* mgs, unsigned char* alive, hipStream_t s);
+
+// Step-0 blocks of the near pairs listed in pairIdx (column-major, ready for
+// AddBlock): out[2*i] and out[2*i+1] with flags.
+void LaunchStep0(const ElemGeo* geo, const ElemPoints* pts, const int* dom, const NodeRefs* refs,
+                 const double* selfU1, const double* selfT1, const double* selfT2, PairGrid grid,
+                 const long* pairIdx, long count, double* out, int* flags, hipStream_t s);
+
+// History convolution for step K into rhs (rhs += hist).
+void LaunchConvolution(PairGrid grid, const NodeRefs* refs, const long* off, const int* kb, const int* len,
+                       const double* mts, const double* mgs, const double* gU, const double* gT, int nstore, int K,
+                       int maxn, int ne, double* rhs, hipStream_t s);
+
+// Block CCSR SpMV y = A x (row-major 3x3 blocks).
+void LaunchSpmv(long nrows, const int* rp, const int* col, const double* val, const double* x, double* y,
+                hipStream_t s);
+// y = M^{-1} x with the leaf preconditioner.
+void LaunchPrecond(long n, const long* posLeaf, const long* leafBegin, const long* leafSize, const long* matOff,
+                   const double* inv, const long* repid, const long* outBlock, const double* x, double* y,
+                   hipStream_t s);
+
+// Reductions: partial sums of dot products of u with columns 0..nv-1 of V
+// (column stride ldv), finished into h[0..nv).
+void LaunchMultiDot(const double* V, long ldv, int nv, const double* u, long n, double* partial, int nblocks,
+                    double* h, hipStream_t s);
+// u -= V[:, 0:nv] h ; accumulate htot += h when htot != null
+void LaunchMultiAxpy(const double* V, long ldv, int nv, const double* h, double* u, long n, double* htot,
+                     hipStream_t s);
+void LaunchNorm(const double* x, long n, double* partial, int nblocks, double* out, hipStream_t s);
+// y = a*x   and   y = b - y
+void LaunchScaleCopy(const double* x, const double* a, int invert, double* y, long n, hipStream_t s);
+void LaunchResidual(const double* b, double* y, long n, hipStream_t s);
+// GMRES Givens step j (1-based) on device scalars.
+void LaunchGivens(double* h, double* QT, double* R, double* f, double* phibar, int* stag, int j, int restart,
+                  hipStream_t s);
+// W_j and x += f_j W_j.
+void LaunchWUpdate(const double* V, double* W, long n, const double* R, int j, int restart, const double* f,
+                   double* x, hipStream_t s);
+
+// Per-step state plumbing.
+void LaunchKnown(const NodeRefs* refs, const double* bd, int ne, double* known, long nknown, hipStream_t s);
+void LaunchScatter(const NodeRefs* refs, const ElemGeo* geo, const double* bd, const double* x, int ne, double* lu,
+                   double* lt, hipStream_t s);
+void LaunchState(const NodeRefs* refs, const double* lu, const double* lt, int ne, int K, int nstore, double* gU,
+                 double* gT, hipStream_t s);
+void LaunchZero(double* x, long n, hipStream_t s);
+
+}  // namespace tdbem

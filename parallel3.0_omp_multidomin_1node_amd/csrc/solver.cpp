@@ -1,0 +1,622 @@
+// Context of one multi-domain case: host model, device buffers, assembly
+// and the time loop.  This file is the host side of the drop-in
+// DynaGMRESSolverMultiDomainCCSR (DBEM1/multidomain.cpp:3664-3926).
+#include "solver.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+
+#pragma clang fp contract(off)
+
+namespace tdbem {
+
+namespace {
+
+void Check(hipError_t e, const char* what)
+{
+    if (e != hipSuccess)
+        throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+double Now()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+template <typename T>
+void DevBuf<T>::alloc(size_t count)
+{
+    free();
+    if (count == 0)
+        count = 1;
+    Check(hipMalloc(&p, count * sizeof(T)), "hipMalloc");
+    n = count;
+}
+
+template <typename T>
+void DevBuf<T>::free()
+{
+    if (p)
+        (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+}
+
+template <typename T>
+void DevBuf<T>::upload(const T* h, size_t count, hipStream_t s)
+{
+    alloc(count);
+    if (count)
+        Check(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s), "upload");
+}
+
+template struct DevBuf<double>;
+template struct DevBuf<int>;
+template struct DevBuf<long>;
+template struct DevBuf<unsigned char>;
+template struct DevBuf<ElemGeo>;
+template struct DevBuf<ElemPoints>;
+template struct DevBuf<NodeRefs>;
+template struct DevBuf<QuadTables>;
+
+Case::~Case()
+{
+    if (h_pin)
+        (void)hipHostFree(h_pin);
+    if (stream)
+        (void)hipStreamDestroy(stream);
+}
+
+void Case::Create(long npoints, const double* points, long nele, const int* ele_pid, const int* bcid_in,
+                  const MultiDomainInputConfig& cfg, double E, double v, double rho, double dt_in, double gap)
+{
+    (void)npoints;
+    if (nele <= 0)
+        throw std::runtime_error("no elements");
+    ne = nele;
+    dt = dt_in;
+    Check(hipStreamCreate(&stream), "hipStreamCreate");
+    if (!BuildMultiDomainModel(cfg, ne, ne, E, v, rho, dt, model))
+        throw std::runtime_error("BuildMultiDomainModel failed");
+    if (model.DomainCount() > kMaxDomains)
+        throw std::runtime_error("too many domains");
+    bcid.assign(bcid_in, bcid_in + ne);
+    if (!dof.Build(model, bcid.data(), ne))
+        throw std::runtime_error("GlobalDofMap::Build failed");
+
+    BuildQuadTables(tables, gap);
+    // element geometry (GeoInit / GetNodeInfo / PthPhyInit frame)
+    geo.resize((size_t)ne);
+    domOf.assign((size_t)ne, 0);
+    surface.assign((size_t)ne, 0);
+    for (const Domain& d : model.domains)
+        for (long e : d.elementIds)
+            domOf[(size_t)e] = d.id;
+    for (const InterfacePair& f : model.interfaces) {
+        surface[(size_t)f.eleA] = 1;
+        surface[(size_t)f.eleB] = 1;
+    }
+    for (long i = 0; i < ne; ++i) {
+        ElemGeo& g = geo[(size_t)i];
+        double coord[3][8];
+        for (int k = 0; k < 3; ++k) {
+            for (int j = 0; j < 8; ++j)
+                coord[k][j] = points[3 * (long)ele_pid[8 * i + j] + k];
+            GetCoef(g.cc[k], coord[k]);
+        }
+        for (int k = 0; k < kNPW2; ++k)
+            FromLocal(g.cc, tables.cenpw[k][0], tables.cenpw[k][1], g.rcenpw[k]);
+        FromLocal(g.cc, 0.0, 0.0, g.center);
+        g.radius = -1.0;
+        for (int j = 0; j < 8; ++j) {
+            double dd = Dist3(g.center, &points[3 * (long)ele_pid[8 * i + j]]);
+            if (g.radius < dd)
+                g.radius = dd;
+        }
+        g.radiusPW = g.radius / kNPW;
+        TranslateMatrix(g.cc, tables.lc[0][0], tables.lc[0][1], g.trans);
+    }
+    // per-node references and interface transforms
+    refs.resize((size_t)ne);
+    for (long i = 0; i < ne; ++i) {
+        NodeRefs& r = refs[(size_t)i];
+        int d = domOf[(size_t)i];
+        VariableRef u = dof.GetU(d, i), t = dof.GetT(d, i), hu = dof.GetHistoryU(d, i), ht = dof.GetHistoryT(d, i);
+        r.u_block = (int)u.globalBlock;
+        r.t_block = (int)t.globalBlock;
+        r.hu_block = (int)hu.globalBlock;
+        r.ht_block = (int)ht.globalBlock;
+        r.u_sign = u.sign;
+        r.t_sign = t.sign;
+        r.hu_sign = hu.sign;
+        r.ht_sign = ht.sign;
+        r.u_known = u.known ? 1 : 0;
+        r.t_known = t.known ? 1 : 0;
+        r.bcid = bcid[(size_t)i];
+        r.domain = d;
+        r.surface = surface[(size_t)i];
+        r.ref_side = 1;
+        for (int k = 0; k < 9; ++k)
+            r.itf_T[k] = (k % 4 == 0) ? 1.0 : 0.0;
+        // GetElementLocalTransformFromInterfaceReference / IsInterfaceReferenceSide
+        // (DBEM1/multidomain.cpp:1619-1658): first matching interface decides.
+        for (const InterfacePair& f : model.interfaces) {
+            if (f.domainA == d && f.eleA == i)
+                break;
+            if (f.domainB == d && f.eleB == i) {
+                r.ref_side = 0;
+                const double* side = geo[(size_t)f.eleB].trans;
+                const double* ref = geo[(size_t)f.eleA].trans;
+                for (int sc = 0; sc < 3; ++sc)
+                    for (int rc = 0; rc < 3; ++rc) {
+                        double dot = 0.0;
+                        for (int ax = 0; ax < 3; ++ax)
+                            dot += side[sc * 3 + ax] * ref[rc * 3 + ax];
+                        r.itf_T[sc * 3 + rc] = dot;
+                    }
+                break;
+            }
+        }
+    }
+    // pair grid
+    grid.nd = model.DomainCount();
+    long pb = 0;
+    for (int d = 0; d < grid.nd; ++d) {
+        grid.pairBase[d] = pb;
+        grid.eleBegin[d] = (int)model.domains[(size_t)d].eleBegin;
+        grid.eleCount[d] = (int)model.domains[(size_t)d].eleCount;
+        pb += (long)grid.eleCount[d] * grid.eleCount[d];
+    }
+    grid.pairBase[grid.nd] = pb;
+    npairs = pb;
+
+    // device upload and per-element setup kernels
+    std::vector<DynaMat> mats;
+    for (const Domain& d : model.domains)
+        mats.push_back(d.mat);
+    UploadDynaMats(mats.data(), (int)mats.size(), stream);
+    d_tables.upload(&tables, 1, stream);
+    d_geo.upload(geo.data(), geo.size(), stream);
+    d_dom.upload(domOf.data(), domOf.size(), stream);
+    d_refs.upload(refs.data(), refs.size(), stream);
+    d_pts.alloc((size_t)ne);
+    d_u1.alloc(9 * (size_t)ne);
+    d_t1.alloc(9 * (size_t)ne);
+    d_t2.alloc(9 * (size_t)ne);
+    double t0 = Now();
+    LaunchPoints(d_geo.p, d_tables.p, d_pts.p, (int)ne, stream);
+    LaunchSingular(d_geo.p, d_dom.p, d_tables.p, d_u1.p, d_t1.p, d_t2.p, (int)ne, stream);
+    Check(hipStreamSynchronize(stream), "setup kernels");
+    timing[0] = Now() - t0;
+    Check(hipGetLastError(), "setup launch");
+}
+
+void Case::Cull(int maxn)
+{
+    d_kb.alloc((size_t)npairs);
+    d_len.alloc((size_t)npairs);
+    d_near.alloc((size_t)npairs);
+    double t0 = Now();
+    LaunchCull(d_geo.p, d_dom.p, grid, npairs, maxn, d_kb.p, d_len.p, d_near.p, stream);
+    Check(hipStreamSynchronize(stream), "cull");
+    Check(hipGetLastError(), "cull launch");
+    h_len.resize((size_t)npairs);
+    h_kb.resize((size_t)npairs);
+    Check(hipMemcpy(h_len.data(), d_len.p, sizeof(int) * (size_t)npairs, hipMemcpyDeviceToHost), "len");
+    Check(hipMemcpy(h_kb.data(), d_kb.p, sizeof(int) * (size_t)npairs, hipMemcpyDeviceToHost), "kb");
+    h_off.resize((size_t)npairs + 1);
+    long acc = 0;
+    for (long p = 0; p < npairs; ++p) {
+        h_off[(size_t)p] = acc;
+        acc += h_len[(size_t)p];
+    }
+    h_off[(size_t)npairs] = acc;
+    entries = acc;
+    d_off.upload(h_off.data(), h_off.size(), stream);
+    culledFor = maxn;
+    timing[1] = Now() - t0;
+}
+
+void Case::AssembleStep0(long maxleaf)
+{
+    if (culledFor < 1)
+        Cull(1);
+    double t0 = Now();
+    std::vector<unsigned char> nearFlag((size_t)npairs);
+    Check(hipMemcpy(nearFlag.data(), d_near.p, (size_t)npairs, hipMemcpyDeviceToHost), "near");
+    std::vector<long> list;
+    for (long p = 0; p < npairs; ++p)
+        if (nearFlag[(size_t)p])
+            list.push_back(p);
+    DevBuf<long> d_list;
+    d_list.upload(list.data(), list.size(), stream);
+    DevBuf<double> d_out;
+    DevBuf<int> d_flags;
+    d_out.alloc(18 * list.size());
+    d_flags.alloc(2 * list.size());
+    LaunchStep0(d_geo.p, d_pts.p, d_dom.p, d_refs.p, d_u1.p, d_t1.p, d_t2.p, grid, d_list.p, (long)list.size(),
+                d_out.p, d_flags.p, stream);
+    std::vector<double> out(18 * list.size());
+    std::vector<int> flags(2 * list.size());
+    Check(hipStreamSynchronize(stream), "step0");
+    Check(hipGetLastError(), "step0 launch");
+    if (!list.empty()) {
+        Check(hipMemcpy(out.data(), d_out.p, sizeof(double) * out.size(), hipMemcpyDeviceToHost), "step0 out");
+        Check(hipMemcpy(flags.data(), d_flags.p, sizeof(int) * flags.size(), hipMemcpyDeviceToHost), "step0 flags");
+    }
+    // AddBlock into the unknown / known builders (DBEM1/multidomain.cpp:1907-1937,2286-2321)
+    MultiDomainCCSRBuilder bA(dof.equationBlockCount, dof.unknownBlockCount);
+    MultiDomainCCSRBuilder bK(dof.equationBlockCount, dof.knownBlockCount);
+    for (size_t i = 0; i < list.size(); ++i) {
+        long p = list[i];
+        int d = 0;
+        while (d + 1 < grid.nd && p >= grid.pairBase[d + 1])
+            ++d;
+        long local = p - grid.pairBase[d];
+        long s = grid.eleBegin[d] + local / grid.eleCount[d];
+        long f = grid.eleBegin[d] + local % grid.eleCount[d];
+        const double* b0 = &out[18 * i];
+        const double* b1 = &out[18 * i + 9];
+        if (surface[(size_t)f]) {
+            VariableRef u = dof.GetU(d, f), t = dof.GetT(d, f);
+            if (flags[2 * i])
+                bA.AddBlock(s, u.globalBlock, u.sign, b0);
+            if (flags[2 * i + 1])
+                bA.AddBlock(s, t.globalBlock, t.sign, b1);
+        } else {
+            if (flags[2 * i]) {
+                VariableRef r = bcid[(size_t)f] == 123 ? dof.GetT(d, f) : dof.GetU(d, f);
+                if (!r.known)
+                    bA.AddBlock(s, r.globalBlock, r.sign, b0);
+            }
+            if (flags[2 * i + 1]) {
+                VariableRef r = bcid[(size_t)f] == 123 ? dof.GetU(d, f) : dof.GetT(d, f);
+                if (r.known)
+                    bK.AddBlock(s, r.globalBlock, r.sign, b1);
+            }
+        }
+    }
+    bA.Build(A);
+    bK.Build(KnownM);
+    if (!StructureValid())
+        throw std::runtime_error("MultiDomain matrix structure check failed (zero rows/cols or non-finite blocks)");
+    d_Arp.upload(A.row_ptr.data(), A.row_ptr.size(), stream);
+    d_Acol.upload(A.col_index.data(), A.col_index.size(), stream);
+    d_Aval.upload(A.value.data(), A.value.size(), stream);
+    d_Krp.upload(KnownM.row_ptr.data(), KnownM.row_ptr.size(), stream);
+    d_Kcol.upload(KnownM.col_index.data(), KnownM.col_index.size(), stream);
+    d_Kval.upload(KnownM.value.data(), KnownM.value.size(), stream);
+    // preconditioner over element centres
+    std::vector<double> centres(3 * (size_t)ne);
+    for (long i = 0; i < ne; ++i)
+        for (int k = 0; k < 3; ++k)
+            centres[3 * (size_t)i + (size_t)k] = geo[(size_t)i].center[k];
+    if (!BuildMappedLeafPreConditioner(centres.data(), ne, maxleaf, A, dof, pre))
+        throw std::runtime_error("preconditioner build failed");
+    std::vector<long> posLeaf((size_t)ne), outBlock((size_t)ne);
+    for (long l = 0; l < pre.leafCount; ++l)
+        for (long j = 0; j < pre.leafSize[(size_t)l]; ++j)
+            posLeaf[(size_t)(pre.leafBegin[(size_t)l] + j)] = l;
+    for (long i = 0; i < ne; ++i)
+        outBlock[(size_t)i] = dof.GetPreferredUnknownBlockForRow(pre.repid[(size_t)i]);
+    d_posLeaf.upload(posLeaf.data(), posLeaf.size(), stream);
+    d_leafBegin.upload(pre.leafBegin.data(), pre.leafBegin.size(), stream);
+    d_leafSize.upload(pre.leafSize.data(), pre.leafSize.size(), stream);
+    d_matOff.upload(pre.matOffset.data(), pre.matOffset.size(), stream);
+    d_inv.upload(pre.inv.data(), pre.inv.size(), stream);
+    d_repid.upload(pre.repid.data(), pre.repid.size(), stream);
+    d_outBlock.upload(outBlock.data(), outBlock.size(), stream);
+    Check(hipStreamSynchronize(stream), "step0 upload");
+    step0Done = true;
+    timing[3] = Now() - t0;
+}
+
+bool Case::StructureValid() const
+{
+    std::vector<long> cc((size_t)A.n, 0);
+    for (long r = 0; r < A.n; ++r) {
+        if (A.row_ptr[(size_t)r] == A.row_ptr[(size_t)r + 1])
+            return false;
+        for (int p = A.row_ptr[(size_t)r]; p < A.row_ptr[(size_t)r + 1]; ++p) {
+            int c = A.col_index[(size_t)p];
+            if (c < 0 || c >= A.n || !IsFiniteBlock9(&A.value[(size_t)p * 9]))
+                return false;
+            cc[(size_t)c]++;
+        }
+    }
+    for (long c = 0; c < A.n; ++c)
+        if (!cc[(size_t)c])
+            return false;
+    return true;
+}
+
+void Case::AssembleHistory(int maxn_in)
+{
+    if (maxn_in < 1)
+        throw std::runtime_error("maxn must be >= 1");
+    maxn = maxn_in;
+    h_mts.clear();
+    h_mgs.clear();
+    h_alive.clear();
+    Cull(maxn);
+    size_t freeB = 0, totalB = 0;
+    Check(hipMemGetInfo(&freeB, &totalB), "hipMemGetInfo");
+    double need = (double)entries * (9.0 * 8 + 6.0 * 8 + 1.0);
+    if (need > 0.92 * (double)freeB)
+        throw std::runtime_error("history bands need " + std::to_string(need / 1e9) + " GB but only " +
+                                 std::to_string((double)freeB / 1e9) + " GB of device memory are free");
+    d_mts.alloc(9 * (size_t)entries);
+    d_mgs.alloc(6 * (size_t)entries);
+    d_alive.alloc((size_t)entries);
+    hipEvent_t a, b;
+    Check(hipEventCreate(&a), "event");
+    Check(hipEventCreate(&b), "event");
+    Check(hipEventRecord(a, stream), "event");
+    const long chunk = 1L << 24;
+    for (long e0 = 0; e0 < entries; e0 += chunk) {
+        long e1 = std::min(entries, e0 + chunk);
+        LaunchHistory(d_geo.p, d_pts.p, d_dom.p, d_refs.p, d_t2.p, grid, d_off.p, d_kb.p, npairs, e0, e1, d_mts.p,
+                      d_mgs.p, d_alive.p, stream);
+    }
+    Check(hipEventRecord(b, stream), "event");
+    Check(hipEventSynchronize(b), "history");
+    Check(hipGetLastError(), "history launch");
+    float ms = 0.f;
+    Check(hipEventElapsedTime(&ms, a, b), "elapsed");
+    timing[2] = ms * 1e-3;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    historyDone = true;
+}
+
+void Case::ExportMatrix(int which, int step, std::vector<int>& rp, std::vector<int>& col, std::vector<double>& val)
+{
+    if (which == 0 || which == 1) {
+        const CCSRMat& M = which == 0 ? A : KnownM;
+        if (!step0Done)
+            throw std::runtime_error("step 0 not assembled");
+        rp = M.row_ptr;
+        col = M.col_index;
+        val = M.value;
+        return;
+    }
+    if (!historyDone || step < 1 || step > maxn)
+        throw std::runtime_error("history step not assembled");
+    if (h_mts.empty()) {
+        h_mts.resize(9 * (size_t)entries);
+        h_mgs.resize(6 * (size_t)entries);
+        h_alive.resize((size_t)entries);
+        Check(hipMemcpy(h_mts.data(), d_mts.p, sizeof(double) * h_mts.size(), hipMemcpyDeviceToHost), "mts");
+        Check(hipMemcpy(h_mgs.data(), d_mgs.p, sizeof(double) * h_mgs.size(), hipMemcpyDeviceToHost), "mgs");
+        Check(hipMemcpy(h_alive.data(), d_alive.p, h_alive.size(), hipMemcpyDeviceToHost), "alive");
+    }
+    struct Item {
+        long row, col;
+        long e;
+    };
+    std::vector<Item> items;
+    for (int d = 0; d < grid.nd; ++d) {
+        long n = grid.eleCount[d];
+        for (long q = 0; q < n * n; ++q) {
+            long p = grid.pairBase[d] + q;
+            int L = h_len[(size_t)p], k0 = h_kb[(size_t)p];
+            if (step < k0 || step >= k0 + L)
+                continue;
+            long e = h_off[(size_t)p] + (step - k0);
+            unsigned char bit = which == 2 ? 1 : 2;
+            if (!(h_alive[(size_t)e] & bit))
+                continue;
+            long s = grid.eleBegin[d] + q / n, f = grid.eleBegin[d] + q % n;
+            long c = which == 2 ? refs[(size_t)f].hu_block : refs[(size_t)f].ht_block;
+            items.push_back(Item{s, c, e});
+        }
+    }
+    std::sort(items.begin(), items.end(),
+              [](const Item& x, const Item& y) { return x.row != y.row ? x.row < y.row : x.col < y.col; });
+    rp.assign((size_t)ne + 1, 0);
+    col.resize(items.size());
+    val.resize(items.size() * 9);
+    for (size_t i = 0; i < items.size(); ++i) {
+        rp[(size_t)items[i].row + 1]++;
+        col[i] = (int)items[i].col;
+        if (which == 2) {
+            for (int k = 0; k < 9; ++k)
+                val[i * 9 + (size_t)k] = h_mts[9 * (size_t)items[i].e + (size_t)k];
+        } else {
+            const double* g = &h_mgs[6 * (size_t)items[i].e];
+            const double full[9] = {g[0], g[1], g[2], g[1], g[3], g[4], g[2], g[4], g[5]};
+            for (int k = 0; k < 9; ++k)
+                val[i * 9 + (size_t)k] = full[k];
+        }
+    }
+    for (long r = 0; r < ne; ++r)
+        rp[(size_t)r + 1] += rp[(size_t)r];
+}
+
+// ---------------------------------------------------------------------------
+// GMRES (DBEM1/pre_gmres.cpp:1369-1605) with maxit = 1 and x0 = 0.  The
+// Arnoldi step uses classical Gram-Schmidt applied twice (two fused
+// multi-dot / multi-axpy passes instead of j sequential MGS passes); the
+// Hessenberg/Givens update, W = V R^-1 and the true residual test
+// ||b - A x|| <= tol ||b|| follow the reference.
+// ---------------------------------------------------------------------------
+void Case::PrepareGmres(long restart)
+{
+    long n = 3 * ne;
+    if (gm_restart == restart && gm_n == n)
+        return;
+    gm_restart = restart;
+    gm_n = n;
+    d_V.alloc((size_t)n * (size_t)(restart + 1));
+    d_W.alloc((size_t)n * (size_t)restart);
+    d_u.alloc((size_t)n);
+    d_u2.alloc((size_t)n);
+    d_vh.alloc((size_t)n);
+    d_h.alloc((size_t)restart + 2);
+    d_h2.alloc((size_t)restart + 2);
+    d_QT.alloc((size_t)(restart + 1) * (size_t)(restart + 1));
+    d_R.alloc((size_t)restart * (size_t)restart);
+    d_f.alloc((size_t)restart + 2);
+    d_scal.alloc(4);
+    d_stag.alloc(1);
+    gm_blocks = (int)std::min<long>(128, std::max<long>(1, (n + 255) / 256));
+    d_part.alloc((size_t)gm_blocks * (size_t)(restart + 1));
+    if (!h_pin)
+        Check(hipHostMalloc((void**)&h_pin, 4 * sizeof(double), hipHostMallocDefault), "pinned");
+}
+
+int Case::Gmres(const double* d_b, double* d_x, long restart, double tol, int* iters)
+{
+    const long n = 3 * ne;
+    double* V = d_V.p;
+    double* normr = d_scal.p;      // [0]
+    double* bnorm = d_scal.p + 1;  // [1]
+    double* phibar = d_scal.p + 2; // [2]
+    *iters = 0;
+    LaunchNorm(d_b, n, d_part.p, gm_blocks, bnorm, stream);
+    Check(hipMemcpyAsync(h_pin, bnorm, sizeof(double), hipMemcpyDeviceToHost, stream), "d2h");
+    Check(hipStreamSynchronize(stream), "gmres");
+    double n2b = h_pin[0];
+    LaunchZero(d_x, n, stream);
+    if (n2b == 0.0)
+        return 0;
+    double tolb = tol * n2b;
+    // r = b - A x0 (x0 = 0)
+    LaunchSpmv(ne, d_Arp.p, d_Acol.p, d_Aval.p, d_x, d_vh.p, stream);
+    LaunchResidual(d_b, d_vh.p, n, stream);
+    LaunchNorm(d_vh.p, n, d_part.p, gm_blocks, normr, stream);
+    Check(hipMemcpyAsync(h_pin, normr, sizeof(double), hipMemcpyDeviceToHost, stream), "d2h");
+    Check(hipStreamSynchronize(stream), "gmres");
+    if (h_pin[0] <= tolb)
+        return 0;
+    Check(hipMemsetAsync(d_QT.p, 0, sizeof(double) * d_QT.n, stream), "memset");
+    Check(hipMemsetAsync(d_R.p, 0, sizeof(double) * d_R.n, stream), "memset");
+    Check(hipMemsetAsync(d_stag.p, 0, sizeof(int), stream), "memset");
+    LaunchPrecond(ne, d_posLeaf.p, d_leafBegin.p, d_leafSize.p, d_matOff.p, d_inv.p, d_repid.p, d_outBlock.p, d_vh.p,
+                  d_u.p, stream);
+    LaunchNorm(d_u.p, n, d_part.p, gm_blocks, d_h.p + 1, stream);
+    LaunchScaleCopy(d_u.p, d_h.p + 1, 1, V, n, stream);
+    const double one = 1.0;
+    Check(hipMemcpyAsync(d_QT.p, &one, sizeof(double), hipMemcpyHostToDevice, stream), "qt");
+    Check(hipMemcpyAsync(phibar, d_h.p + 1, sizeof(double), hipMemcpyDeviceToDevice, stream), "phibar");
+    for (int j = 1; j <= restart; ++j) {
+        double* Vj = V + (size_t)(j - 1) * (size_t)n;
+        LaunchSpmv(ne, d_Arp.p, d_Acol.p, d_Aval.p, Vj, d_u2.p, stream);
+        LaunchPrecond(ne, d_posLeaf.p, d_leafBegin.p, d_leafSize.p, d_matOff.p, d_inv.p, d_repid.p, d_outBlock.p,
+                      d_u2.p, d_u.p, stream);
+        LaunchMultiDot(V, n, j, d_u.p, n, d_part.p, gm_blocks, d_h.p + 1, stream);
+        LaunchMultiAxpy(V, n, j, d_h.p + 1, d_u.p, n, nullptr, stream);
+        LaunchMultiDot(V, n, j, d_u.p, n, d_part.p, gm_blocks, d_h2.p, stream);
+        LaunchMultiAxpy(V, n, j, d_h2.p, d_u.p, n, d_h.p + 1, stream);
+        LaunchNorm(d_u.p, n, d_part.p, gm_blocks, d_h.p + j + 1, stream);
+        LaunchScaleCopy(d_u.p, d_h.p + j + 1, 1, V + (size_t)j * (size_t)n, n, stream);
+        LaunchGivens(d_h.p, d_QT.p, d_R.p, d_f.p, phibar, d_stag.p, j, (int)restart, stream);
+        LaunchWUpdate(V, d_W.p, n, d_R.p, j, (int)restart, d_f.p, d_x, stream);
+        LaunchSpmv(ne, d_Arp.p, d_Acol.p, d_Aval.p, d_x, d_vh.p, stream);
+        LaunchResidual(d_b, d_vh.p, n, stream);
+        LaunchNorm(d_vh.p, n, d_part.p, gm_blocks, normr, stream);
+        Check(hipMemcpyAsync(h_pin, normr, sizeof(double), hipMemcpyDeviceToHost, stream), "d2h");
+        Check(hipMemcpyAsync(h_pin + 3, d_stag.p, sizeof(int), hipMemcpyDeviceToHost, stream), "d2h");
+        Check(hipStreamSynchronize(stream), "gmres iteration");
+        *iters = j;
+        if (h_pin[0] <= tolb)
+            return 0;
+        int stag = 0;
+        std::memcpy(&stag, h_pin + 3, sizeof(int));
+        if (stag)
+            return 3;
+    }
+    return 1;
+}
+
+void Case::Solve(int nstep, const double* known_lu, long restart, double tol, double* lu, double* lt, int* iters,
+                 int* flags, int* solved)
+{
+    if (!step0Done)
+        AssembleStep0(10);
+    if (!historyDone || maxn < 1)
+        throw std::runtime_error("history not assembled");
+    PrepareGmres(restart);
+    const long n3 = 3 * ne;
+    const int nstore = nstep + 1;
+    DevBuf<double> d_bd, d_lu, d_lt, d_gU, d_gT, d_known, d_rhs, d_x;
+    d_bd.upload(known_lu, (size_t)nstore * (size_t)n3, stream);
+    d_lu.alloc((size_t)nstore * (size_t)n3);
+    d_lt.alloc((size_t)nstore * (size_t)n3);
+    d_gU.alloc((size_t)dof.historyUBlockCount * 3 * (size_t)nstore);
+    d_gT.alloc((size_t)dof.historyTBlockCount * 3 * (size_t)nstore);
+    d_known.alloc(3 * (size_t)std::max<long>(1, dof.knownBlockCount));
+    d_rhs.alloc((size_t)n3);
+    d_x.alloc((size_t)n3);
+    LaunchZero(d_gU.p, (long)d_gU.n, stream);
+    LaunchZero(d_gT.p, (long)d_gT.n, stream);
+    LaunchZero(d_known.p, (long)d_known.n, stream);
+    LaunchZero(d_lu.p, (long)d_lu.n, stream);
+    LaunchZero(d_lt.p, (long)d_lt.n, stream);
+    hipEvent_t c0, c1;
+    Check(hipEventCreate(&c0), "event");
+    Check(hipEventCreate(&c1), "event");
+    double convTime = 0.0, gmresTime = 0.0;
+    *solved = 0;
+    for (int K = 1; K <= nstep; ++K) {
+        LaunchKnown(d_refs.p, d_bd.p + (size_t)K * (size_t)n3, (int)ne, d_known.p, dof.knownBlockCount, stream);
+        LaunchSpmv(ne, d_Krp.p, d_Kcol.p, d_Kval.p, d_known.p, d_rhs.p, stream);
+        if (K > 1) {
+            Check(hipEventRecord(c0, stream), "event");
+            LaunchConvolution(grid, d_refs.p, d_off.p, d_kb.p, d_len.p, d_mts.p, d_mgs.p, d_gU.p, d_gT.p, nstore, K,
+                              maxn, (int)ne, d_rhs.p, stream);
+            Check(hipEventRecord(c1, stream), "event");
+            Check(hipEventSynchronize(c1), "conv");
+            float ms = 0.f;
+            Check(hipEventElapsedTime(&ms, c0, c1), "elapsed");
+            convTime += ms * 1e-3;
+        }
+        double g0 = Now();
+        int it = 0;
+        int flag = Gmres(d_rhs.p, d_x.p, restart, tol, &it);
+        gmresTime += Now() - g0;
+        iters[K - 1] = it;
+        flags[K - 1] = flag;
+        *solved = K;
+        if (flag != 0)
+            break;
+        double* slu = d_lu.p + (size_t)K * (size_t)n3;
+        double* slt = d_lt.p + (size_t)K * (size_t)n3;
+        LaunchScatter(d_refs.p, d_geo.p, d_bd.p + (size_t)K * (size_t)n3, d_x.p, (int)ne, slu, slt, stream);
+        LaunchState(d_refs.p, slu, slt, (int)ne, K, nstore, d_gU.p, d_gT.p, stream);
+    }
+    Check(hipStreamSynchronize(stream), "solve");
+    Check(hipGetLastError(), "solve launch");
+    Check(hipMemcpy(lu, d_lu.p, sizeof(double) * d_lu.n, hipMemcpyDeviceToHost), "lu");
+    Check(hipMemcpy(lt, d_lt.p, sizeof(double) * d_lt.n, hipMemcpyDeviceToHost), "lt");
+    // step 0: lt = 0, Convert (lu <-> lt for BCID 456), then LocToAbs
+    // (DBEM1/multidomain.cpp:3819-3822)
+    for (long i = 0; i < ne; ++i) {
+        double u[3], t[3];
+        for (int k = 0; k < 3; ++k) {
+            u[k] = known_lu[3 * i + k];
+            t[k] = 0.0;
+        }
+        if (bcid[(size_t)i] == 456)
+            for (int k = 0; k < 3; ++k)
+                std::swap(u[k], t[k]);
+        const double* F = geo[(size_t)i].trans;
+        lu[3 * i + 0] = F[0] * u[0] + F[3] * u[1] + F[6] * u[2];
+        lu[3 * i + 1] = F[1] * u[0] + F[4] * u[1] + F[7] * u[2];
+        lu[3 * i + 2] = F[2] * u[0] + F[5] * u[1] + F[8] * u[2];
+        lt[3 * i + 0] = F[0] * t[0] + F[3] * t[1] + F[6] * t[2];
+        lt[3 * i + 1] = F[1] * t[0] + F[4] * t[1] + F[7] * t[2];
+        lt[3 * i + 2] = F[2] * t[0] + F[5] * t[1] + F[8] * t[2];
+    }
+    (void)hipEventDestroy(c0);
+    (void)hipEventDestroy(c1);
+    timing[4] = convTime;
+    timing[5] = gmresTime;
+}
+
+}  // namespace tdbem

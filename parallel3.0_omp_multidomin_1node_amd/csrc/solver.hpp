@@ -1,0 +1,90 @@
+// One multi-domain case on one GPU: host model and device buffers.
+#pragma once
+
+#include "kernels.hpp"
+#include "model.hpp"
+#include "precond.hpp"
+
+#include <string>
+#include <vector>
+
+namespace tdbem {
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { free(); }
+    void alloc(size_t count);
+    void free();
+    void upload(const T* h, size_t count, hipStream_t s);
+};
+
+class Case {
+public:
+    ~Case();
+    void Create(long npoints, const double* points, long ne, const int* ele_pid, const int* bcid,
+                const MultiDomainInputConfig& cfg, double E, double v, double rho, double dt, double gap);
+    void AssembleStep0(long maxleaf);
+    void AssembleHistory(int maxn);
+    void Solve(int nstep, const double* known_lu, long restart, double tol, double* lu, double* lt, int* iters,
+               int* flags, int* solved);
+    void ExportMatrix(int which, int step, std::vector<int>& rp, std::vector<int>& col, std::vector<double>& val);
+
+    long ne = 0;
+    double dt = 0.0;
+    MultiDomainModel model;
+    GlobalDofMap dof;
+    std::vector<int> bcid, domOf, surface;
+    QuadTables tables;
+    std::vector<ElemGeo> geo;
+    std::vector<NodeRefs> refs;
+    PairGrid grid{};
+    long npairs = 0;
+    long entries = 0;
+    int maxn = 0;
+    CCSRMat A, KnownM;
+    LeafPreconditioner pre;
+    double timing[6] = {0, 0, 0, 0, 0, 0};
+    hipStream_t stream = nullptr;
+
+    DevBuf<double> d_u1, d_t1, d_t2;
+
+private:
+    void Cull(int maxn);
+    bool StructureValid() const;
+    void PrepareGmres(long restart);
+    int Gmres(const double* d_b, double* d_x, long restart, double tol, int* iters);
+
+    bool step0Done = false, historyDone = false;
+    int culledFor = 0;
+    DevBuf<QuadTables> d_tables;
+    DevBuf<ElemGeo> d_geo;
+    DevBuf<ElemPoints> d_pts;
+    DevBuf<int> d_dom;
+    DevBuf<NodeRefs> d_refs;
+    DevBuf<int> d_kb, d_len;
+    DevBuf<unsigned char> d_near;
+    DevBuf<long> d_off;
+    std::vector<int> h_kb, h_len;
+    std::vector<long> h_off;
+    DevBuf<double> d_mts, d_mgs;
+    DevBuf<unsigned char> d_alive;
+    std::vector<double> h_mts, h_mgs;
+    std::vector<unsigned char> h_alive;
+    DevBuf<int> d_Arp, d_Acol, d_Krp, d_Kcol;
+    DevBuf<double> d_Aval, d_Kval;
+    DevBuf<long> d_posLeaf, d_leafBegin, d_leafSize, d_matOff, d_repid, d_outBlock;
+    DevBuf<double> d_inv;
+    // GMRES workspace
+    long gm_restart = -1, gm_n = -1;
+    int gm_blocks = 1;
+    DevBuf<double> d_V, d_W, d_u, d_u2, d_vh, d_h, d_h2, d_QT, d_R, d_f, d_scal, d_part;
+    DevBuf<int> d_stag;
+    double* h_pin = nullptr;
+};
+
+}  // namespace tdbem

@@ -1,0 +1,160 @@
+"""Python binding of the MI355X multi-domain TDBEM GMRES path.
+
+The compute path is the HIP library ``lib/libtdbem.so`` reached through the
+C-ABI in ``include/tdbem_capi.h``; this module only marshals numpy arrays.
+There is no CPU fallback: if the library is missing or no GPU is present,
+calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+PKG_DIR = Path(__file__).resolve().parent.parent
+LIB_PATH = PKG_DIR / "lib" / "libtdbem.so"
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_lp = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+
+EXPORTED = [
+    "tdbem_last_error", "tdbem_create", "tdbem_destroy", "tdbem_dofmap_export", "tdbem_assemble_step0",
+    "tdbem_assemble_history", "tdbem_ccsr_export", "tdbem_self_tables", "tdbem_solve", "tdbem_dyna_gmres_solver",
+    "tdbem_timings",
+]
+
+
+def build(jobs: int = 8) -> Path:
+    """Compile the HIP library for gfx950 in-tree."""
+    subprocess.run(["make", "-s", "-j", str(jobs), "-C", str(PKG_DIR)], check=True)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is not built; run build() first")
+        L = C.CDLL(str(LIB_PATH))
+        L.tdbem_last_error.restype = C.c_char_p
+        L.tdbem_create.argtypes = [C.POINTER(C.c_void_p), C.c_long, _dp, C.c_long, _ip, _ip, C.c_int, _dp, C.c_int,
+                                   _lp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double]
+        L.tdbem_destroy.argtypes = [C.c_void_p]
+        L.tdbem_dofmap_export.argtypes = [C.c_void_p, _lp, _lp, _dp, _ip, _lp, _dp, _ip, _lp, _dp, _lp, _dp, _lp]
+        L.tdbem_assemble_step0.argtypes = [C.c_void_p, C.c_long]
+        L.tdbem_assemble_history.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_long)]
+        L.tdbem_ccsr_export.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_long), C.POINTER(C.c_long),
+                                        C.c_void_p, C.c_void_p, C.c_void_p]
+        L.tdbem_self_tables.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.tdbem_solve.argtypes = [C.c_void_p, C.c_int, _dp, C.c_long, C.c_double, _dp, _dp, _ip, _ip,
+                                  C.POINTER(C.c_int), _dp]
+        L.tdbem_dyna_gmres_solver.argtypes = [C.c_void_p, C.c_long, C.c_double, C.c_long, C.c_int, _dp, _dp, _dp,
+                                              C.POINTER(C.c_int)]
+        L.tdbem_timings.argtypes = [C.c_void_p, _dp]
+        _lib = L
+    return _lib
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = lib().tdbem_last_error()
+        raise RuntimeError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+
+class Case:
+    """One multi-domain case on the GPU (mirrors DynaGMRESSolverMultiDomainCCSR's inputs)."""
+
+    def __init__(self, points, ele, bcid, dt, gap, domain_count=0, mat_rows=None, itf_rows=None,
+                 default_mat=(1.0, 0.0, 1.0)):
+        self.points = np.ascontiguousarray(points, np.float64).reshape(-1, 3)
+        self.ele = np.ascontiguousarray(ele, np.int32).reshape(-1, 8)
+        self.bcid = np.ascontiguousarray(bcid, np.int32)
+        self.ne = len(self.ele)
+        mats = np.ascontiguousarray(mat_rows if mat_rows is not None else np.zeros((0, 4)), np.float64).reshape(-1)
+        if itf_rows is None:
+            nitf, itf = -1, np.zeros(1, np.int64)
+        else:
+            itf = np.ascontiguousarray(itf_rows, np.int64).reshape(-1)
+            nitf = len(itf) // 5
+            if itf.size == 0:
+                itf = np.zeros(1, np.int64)
+        if mats.size == 0:
+            mats = np.zeros(1)
+        h = C.c_void_p()
+        _check(lib().tdbem_create(C.byref(h), len(self.points), self.points.ravel(), self.ne, self.ele.ravel(),
+                                  self.bcid, int(domain_count), mats, nitf, itf, float(default_mat[0]),
+                                  float(default_mat[1]), float(default_mat[2]), float(dt), float(gap)),
+               "tdbem_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().tdbem_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def dofmap(self):
+        n = self.ne
+        counts = np.zeros(9, np.int64)
+        a = dict(u_block=np.zeros(n, np.int64), u_sign=np.zeros(n), u_known=np.zeros(n, np.int32),
+                 t_block=np.zeros(n, np.int64), t_sign=np.zeros(n), t_known=np.zeros(n, np.int32),
+                 hu_block=np.zeros(n, np.int64), hu_sign=np.zeros(n), ht_block=np.zeros(n, np.int64),
+                 ht_sign=np.zeros(n), preferred=np.zeros(n, np.int64))
+        _check(lib().tdbem_dofmap_export(self.h, counts, a["u_block"], a["u_sign"], a["u_known"], a["t_block"],
+                                         a["t_sign"], a["t_known"], a["hu_block"], a["hu_sign"], a["ht_block"],
+                                         a["ht_sign"], a["preferred"]), "tdbem_dofmap_export")
+        a["counts"] = counts
+        return a
+
+    def assemble_step0(self, maxleaf=10):
+        _check(lib().tdbem_assemble_step0(self.h, int(maxleaf)), "tdbem_assemble_step0")
+
+    def assemble_history(self, maxn):
+        e = C.c_long()
+        _check(lib().tdbem_assemble_history(self.h, int(maxn), C.byref(e)), "tdbem_assemble_history")
+        return e.value
+
+    def ccsr(self, which, step=0):
+        nr, nnz = C.c_long(), C.c_long()
+        _check(lib().tdbem_ccsr_export(self.h, which, step, C.byref(nr), C.byref(nnz), None, None, None),
+               "tdbem_ccsr_export")
+        rp = np.zeros(nr.value + 1, np.int32)
+        col = np.zeros(max(nnz.value, 1), np.int32)
+        val = np.zeros(max(nnz.value, 1) * 9)
+        _check(lib().tdbem_ccsr_export(self.h, which, step, C.byref(nr), C.byref(nnz), rp.ctypes.data,
+                                       col.ctypes.data, val.ctypes.data), "tdbem_ccsr_export")
+        return rp, col[:nnz.value], val[:9 * nnz.value].reshape(-1, 3, 3)
+
+    def self_tables(self):
+        n = 9 * self.ne
+        u, t1, t2 = np.zeros(n), np.zeros(n), np.zeros(n)
+        _check(lib().tdbem_self_tables(self.h, u, t1, t2), "tdbem_self_tables")
+        return u.reshape(-1, 9), t1.reshape(-1, 9), t2.reshape(-1, 9)
+
+    def solve(self, known_lu, nstep, restart=200, tol=1e-5):
+        known = np.ascontiguousarray(known_lu, np.float64).reshape(-1)
+        if known.size != (nstep + 1) * self.ne * 3:
+            raise ValueError("known_lu must hold (nstep+1) x ne x 3 values")
+        lu = np.zeros_like(known)
+        lt = np.zeros_like(known)
+        iters = np.zeros(nstep, np.int32)
+        flags = np.zeros(nstep, np.int32)
+        solved = C.c_int()
+        times = np.zeros(3)
+        _check(lib().tdbem_solve(self.h, int(nstep), known, int(restart), float(tol), lu, lt, iters, flags,
+                                 C.byref(solved), times), "tdbem_solve")
+        return (lu.reshape(nstep + 1, self.ne, 3), lt.reshape(nstep + 1, self.ne, 3), iters, flags, solved.value,
+                times)
+
+    def timings(self):
+        t = np.zeros(6)
+        _check(lib().tdbem_timings(self.h, t), "tdbem_timings")
+        return dict(singular=t[0], cull=t[1], history=t[2], step0=t[3], convolution=t[4], gmres=t[5])

@@ -1,0 +1,184 @@
+"""GPU parity of the HIP path (through the C-ABI) against the CPU oracle.
+
+Integer / pattern data must be bit-exact; FP64 blocks are compared bitwise
+where the arithmetic order is the reference's (history quadrature, DOF map)
+and to 1e-10 relative otherwise; solutions to the stated tolerances."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _tdbem():
+    import tdbem
+    if not tdbem.LIB_PATH.exists():
+        pytest.fail("HIP library not built")
+    return tdbem
+
+
+def _rod_cases(oracle_mod, f):
+    tdbem = _tdbem()
+    card = f["card"]
+    dt = oracle_mod.reference_dt(f["pts"], f["ele"], card.beta, f["E_global"], card.v, f["rho_global"])
+    oc = oracle_mod.Case(f["pts"], f["ele"], f["bcid"], f["mats"], f["itf"], dt, card.gap)
+    rows = np.array([[m[0], m[1], m[2], m[3]] for m in card.materials], float)
+    gc = tdbem.Case(f["pts"], f["ele"], f["bcid"], dt, card.gap, domain_count=card.domain_count, mat_rows=rows,
+                    itf_rows=np.array(card.interfaces), default_mat=(f["E_global"], card.v, f["rho_global"]))
+    return oc, gc, dt
+
+
+@pytest.fixture(scope="module")
+def rod_pair(oracle_mod, rod_fixture):
+    print("case: rod120_100 fixture", flush=True)
+    oc, gc, dt = _rod_cases(oracle_mod, rod_fixture)
+    nstep = rod_fixture["card"].nstep
+    gc.assemble_step0(10)
+    gc.assemble_history(nstep)
+    oc.assemble(nstep)
+    return oc, gc, rod_fixture
+
+
+def test_dofmap_bitexact(rod_pair):
+    oc, gc, f = rod_pair
+    od, gd = oc.dofmap(), gc.dofmap()
+    for k in gd:
+        assert np.array_equal(od[k], gd[k]), k
+
+
+def test_self_tables_bitexact(rod_pair):
+    oc, gc, f = rod_pair
+    u, t1, t2 = gc.self_tables()
+    ref = np.array([oc.self_tables(e) for e in range(len(f["ele"]))])
+    assert np.array_equal(u, ref[:, 0]) and np.array_equal(t1, ref[:, 1]) and np.array_equal(t2, ref[:, 2])
+
+
+def _same_ccsr(a, b, rtol):
+    assert np.array_equal(a[0], b[0]), "row_ptr"
+    assert np.array_equal(a[1], b[1]), "col"
+    scale = max(np.abs(b[2]).max(), 1e-300)
+    assert np.abs(a[2] - b[2]).max() <= rtol * scale
+
+
+def test_step0_matrices(rod_pair):
+    oc, gc, f = rod_pair
+    for which in (0, 1):
+        _same_ccsr(gc.ccsr(which), oc.ccsr(which), 1e-15)
+
+
+def test_history_bitexact_all_steps(rod_pair):
+    oc, gc, f = rod_pair
+    for k in range(1, f["card"].nstep + 1):
+        for which in (2, 3):
+            a, b = gc.ccsr(which, k), oc.ccsr(which, k)
+            assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), (which, k)
+            assert np.array_equal(a[2], b[2]), (which, k)
+
+
+def test_rod_known_answer_on_gpu(rod_pair):
+    """Reference-held known answer (DBEM1/analytic_validation.txt) on the GPU path
+    with the reference's GMRES settings (restart 200, tol 1e-5)."""
+    oc, gc, f = rod_pair
+    card = f["card"]
+    U, T, it, fl, solved, times = gc.solve(f["lu"], card.nstep, restart=card.iterations, tol=card.error)
+    assert solved == card.nstep and not fl.any()
+    num = U[100, [100, 101, 102, 103], 0].mean()
+    assert abs(num - float(f["expected"]["numeric"])) < 1e-12
+    OU, OT, oit, ofl, osolved = oc.solve(f["lu"], card.nstep, restart=card.iterations, tol=card.error)
+    assert np.array_equal(it, oit)
+    assert np.abs(U - OU).max() <= 1e-10 * np.abs(OU).max()
+    assert np.abs(T - OT).max() <= 1e-10 * np.abs(OT).max()
+    for da, ea, db, eb, s in f["itf"]:
+        assert np.array_equal(U[:, ea], U[:, eb])
+
+
+def _workload_cases(oracle_mod, w):
+    from tdbem import io
+    tdbem = _tdbem()
+    m = w.mesh
+    card = io.DataCard(title="x", thread_num=1, flag_dyna=2, beta=0.5, nstep=w.nstep, gap=w.gap, E=1.0, v=0.0,
+                       rho=1.0, amplitude=1, solver=2, domain_count=m.domain_count,
+                       materials=[(int(r[0]), r[1], r[2], r[3]) for r in w.mats.tolist()],
+                       interfaces=[tuple(int(x) for x in r) for r in m.interfaces.tolist()])
+    mats, itf = io.normalise_model(card, len(m.ele))
+    oc = oracle_mod.Case(m.points, m.ele, m.bcid, mats, itf, w.dt, w.gap)
+    gc = tdbem.Case(m.points, m.ele, m.bcid, w.dt, w.gap, domain_count=m.domain_count, mat_rows=w.mats,
+                    itf_rows=m.interfaces)
+    return oc, gc
+
+
+@pytest.mark.parametrize("which", ["rod_config1", "layered_small"])
+def test_generated_cases_match_oracle(oracle_mod, which):
+    from tdbem import mesh
+    if which == "rod_config1":
+        w = mesh.rod_config1(nstep=40)
+    else:
+        w = mesh.layered_config2(h=0.5, nstep=40)
+    print("case:", w.name, len(w.mesh.ele), "elements", flush=True)
+    oc, gc = _workload_cases(oracle_mod, w)
+    gc.assemble_step0(10)
+    gc.assemble_history(w.nstep)
+    oc.assemble(w.nstep)
+    for which_m in (0, 1):
+        _same_ccsr(gc.ccsr(which_m), oc.ccsr(which_m), 1e-14)
+    for k in range(1, w.nstep + 1):
+        for which_m in (2, 3):
+            a, b = gc.ccsr(which_m, k), oc.ccsr(which_m, k)
+            assert np.array_equal(a[1], b[1]) and np.array_equal(a[0], b[0]), (which_m, k)
+            assert np.array_equal(a[2], b[2]), (which_m, k)
+    U, T, it, fl, solved, _ = gc.solve(w.known, w.nstep, tol=1e-12)
+    OU, OT, oit, ofl, osolved = oc.solve(w.known, w.nstep, tol=1e-12)
+    assert solved == osolved == w.nstep
+    assert np.abs(U - OU).max() <= 1e-9 * np.abs(OU).max()
+    assert np.abs(T - OT).max() <= 1e-9 * np.abs(OT).max()
+
+
+def test_regular_path_exercised_and_matched(oracle_mod):
+    """A case where whole elements sit between the P and S fronts for a whole
+    step (NeedCal = 2, regular path with the N0(gap) weight and T1+T2 sum):
+    history blocks still bit-exact."""
+    from tdbem import mesh
+    m = mesh.box_mesh(2, 4.0, 1.0, 1.0, 8, 2, 2)
+    mats = np.array([[1, 10.0, 0.45, 1.0], [2, 10.0, 0.45, 1.0]])
+    known = mesh.normal_load_history(len(m.ele), 30, 0.02, m.top, np.r_[0.0, np.ones(30)])
+    w = mesh.Workload("regular_path_probe", m, mats, 0.02, 30, known)
+    oc, gc = _workload_cases(oracle_mod, w)
+    regular = 0
+    ne = len(m.ele)
+    half = ne // 2
+    for f in range(0, half, 3):
+        for s in range(0, half, 5):
+            for k in range(1, 30):
+                regular += oc.needcal(f, s, 1, k) == 2
+    assert regular > 0
+    gc.assemble_step0(10)
+    gc.assemble_history(w.nstep)
+    oc.assemble(w.nstep)
+    for k in range(1, w.nstep + 1):
+        for which_m in (2, 3):
+            a, b = gc.ccsr(which_m, k), oc.ccsr(which_m, k)
+            assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2]), (which_m, k)
+
+
+def test_full_size_properties():
+    """Size-independent properties on a larger layered box (no oracle run):
+    zero load gives zero response, the response is linear in the load, MGS
+    blocks are symmetric and interface displacements agree across sides."""
+    tdbem = _tdbem()
+    from tdbem import mesh
+    w = mesh.layered_config2(h=0.25, nstep=60)
+    m = w.mesh
+    print("case:", w.name, len(m.ele), "elements", flush=True)
+    gc = tdbem.Case(m.points, m.ele, m.bcid, w.dt, w.gap, domain_count=4, mat_rows=w.mats, itf_rows=m.interfaces)
+    gc.assemble_step0(10)
+    gc.assemble_history(w.nstep)
+    U, T, it, fl, solved, _ = gc.solve(w.known, w.nstep, tol=1e-12)
+    assert solved == w.nstep and not fl.any()
+    U2, T2, *_ = gc.solve(2.0 * w.known, w.nstep, tol=1e-12)
+    assert np.abs(U2 - 2 * U).max() <= 1e-9 * np.abs(U).max()
+    Z, ZT, *_ = gc.solve(np.zeros_like(w.known), w.nstep, tol=1e-12)
+    assert not Z.any() and not ZT.any()
+    for k in (1, 10, 30):
+        rp, col, val = gc.ccsr(3, k)
+        assert np.array_equal(val, np.transpose(val, (0, 2, 1)))
+    for da, ea, db, eb, s in m.interfaces:
+        assert np.abs(U[:, ea - 1] - U[:, eb - 1]).max() <= 1e-12 * np.abs(U).max()

@@ -1,0 +1,187 @@
+#!/usr/bin/env python3
+"""Benchmark of the multi-domain TDBEM GMRES path on one MI355X.
+
+Workload (default): the layered 2x2x4 block of 4 stacked sub-domains with
+seeded materials (E in [1,10], v in [0.2,0.35], rho in [1,3]), constant square
+elements with h = 0.1 (6400 elements), Ricker normal traction on the top face,
+bottom fixed, dt = 0.5 h / max c1, 400 time steps, plain FP64.
+
+One bench step is one whole transient solve: case setup (geometry, singular
+self integrals), step-0 assembly and preconditioner, history assembly
+(causality cull + quadrature of every MTS/MGS block) and the 400-step time
+loop (history convolution + GMRES per step).  The reported value is
+time steps per second of that whole job: K * NStep / wall time.
+
+Inputs are generated on the host and uploaded by the case setup inside the
+timed region; the GPU holds everything else.  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+REPO = Path(__file__).resolve().parent
+sys.path.insert(0, str(REPO / "parallel3.0_omp_multidomin_1node_amd"))
+sys.path.insert(0, str(REPO))
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_case(tdbem, w):
+    m = w.mesh
+    return tdbem.Case(m.points, m.ele, m.bcid, w.dt, w.gap, domain_count=m.domain_count, mat_rows=w.mats,
+                      itf_rows=m.interfaces)
+
+
+def run_once(tdbem, w, restart=200, tol=1e-5, keep=False):
+    t0 = time.perf_counter()
+    c = make_case(tdbem, w)
+    c.assemble_step0(10)
+    entries = c.assemble_history(w.nstep)
+    U, T, iters, flags, solved, times = c.solve(w.known, w.nstep, restart=restart, tol=tol)
+    tdbem.synchronize()
+    wall = time.perf_counter() - t0
+    info = dict(wall=wall, entries=entries, iters=iters, flags=flags, solved=solved, timings=c.timings(),
+                U=U if keep else None)
+    if keep:
+        info["case"] = c
+    else:
+        c.close()
+    return info
+
+
+def conv_bytes_per_launch(counts, npairs, ne, nhist, nstep):
+    """Algorithmic HBM bytes of the convolution kernel per time step, averaged
+    over steps 2..nstep: each band entry with lag j <= K read once (9 MTS +
+    6 MGS doubles), per-pair band metadata (kb, len, offset: 16 B), the state
+    time series read once (two histories, 3 doubles per column and step) and
+    the right-hand side read + written."""
+    cum = np.cumsum(counts)
+    total = 0.0
+    for K in range(2, nstep + 1):
+        j = min(K, len(counts))
+        total += cum[j - 1] * 120.0 + npairs * 16.0 + 2.0 * nhist * (K - 1) * 24.0 + ne * 48.0
+    return total / (nstep - 1)
+
+
+def cpu_baseline(threads):
+    """Oracle (CPU FP64 restatement of the reference, OpenMP over the
+    reference's independent loops) on a bounded sample of the same workload
+    family: the layered block at h = 0.25 (1024 elements), 100 steps."""
+    from oracle import oracle as O
+    from tdbem import io, mesh
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    w = mesh.layered_config2(h=0.25, nstep=100)
+    m = w.mesh
+    card = io.DataCard(title="x", thread_num=threads, flag_dyna=2, beta=0.5, nstep=w.nstep, gap=w.gap, E=1.0, v=0.0,
+                       rho=1.0, amplitude=1, solver=2, domain_count=4,
+                       materials=[(int(r[0]), r[1], r[2], r[3]) for r in w.mats.tolist()],
+                       interfaces=[tuple(int(x) for x in r) for r in m.interfaces.tolist()])
+    mats, itf = io.normalise_model(card, len(m.ele))
+    t0 = time.perf_counter()
+    c = O.Case(m.points, m.ele, m.bcid, mats, itf, w.dt, w.gap)
+    U, T, it, fl, solved = c.solve(w.known, w.nstep, tol=1e-5)
+    dt = time.perf_counter() - t0
+    return dict(value=w.nstep / dt, wall=dt, w=w, U=U)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1, help="timed whole-job solves")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed whole-job solves")
+    ap.add_argument("--h", type=float, default=0.1)
+    ap.add_argument("--nstep", type=int, default=400)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
+              file=sys.stderr)
+        sys.exit(2)
+    if int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU only; WORLD_SIZE > 1 is not supported", file=sys.stderr)
+        sys.exit(2)
+
+    import tdbem
+    from tdbem import mesh
+
+    w = mesh.layered_config2(h=args.h, nstep=args.nstep)
+    log(f"workload {w.name}: {len(w.mesh.ele)} elements, {w.nstep} steps, dt={w.dt:.6g}")
+    for i in range(args.warmup):
+        r = run_once(tdbem, w)
+        log(f"warmup {i}: {r['wall']:.2f} s, entries {r['entries']}")
+    tdbem.synchronize()
+    t0 = time.perf_counter()
+    runs = []
+    for i in range(args.steps):
+        runs.append(run_once(tdbem, w, keep=(i == args.steps - 1)))
+        log(f"step {i}: {runs[-1]['wall']:.2f} s")
+    tdbem.synchronize()
+    elapsed = time.perf_counter() - t0
+    last = runs[-1]
+    value = args.steps * w.nstep / elapsed
+    tm = last["timings"]
+    c = last["case"]
+    counts, npairs = c.history_stats(w.nstep)
+    dof = c.dofmap()["counts"]
+    nhist = int(dof[7] + dof[8]) // 2
+    conv_launches = w.nstep - 1
+    conv_avg = tm["convolution"] / conv_launches
+    bytes_launch = conv_bytes_per_launch(counts, npairs, c.ne, nhist, w.nstep)
+    conv_gbs = bytes_launch / conv_avg / 1e9
+    loop_time = tm["convolution"] + tm["gmres"]
+    out = {
+        "metric": "time steps/s (whole job: setup + step-0 + history assembly + time loop), FP64",
+        "value": round(value, 4),
+        "unit": "time steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1000.0 * elapsed / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded layered block, Ricker load)",
+        "config": {"workload": w.name, "elements": int(len(w.mesh.ele)), "domains": 4, "nstep": w.nstep,
+                   "dt": w.dt, "gmres_restart": 200, "gmres_tol": 1e-5, "history_band_entries": int(last["entries"])},
+        "breakdown_s": {k: round(float(v), 4) for k, v in tm.items()},
+        "time_loop_steps_per_s": round(w.nstep / max(loop_time, 1e-9), 3),
+        "gmres_inner_iterations_mean": float(np.mean(last["iters"])),
+        "roofline": {"kernel": "k_conv (history convolution)", "bound": "hbm", "achieved": round(conv_gbs, 2),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(conv_gbs / HBM_PEAK_GBS, 4),
+                     "traffic": None, "avg_launch_ms": round(1000 * conv_avg, 4),
+                     "bytes_per_launch": round(bytes_launch)},
+    }
+    c.close()
+    if not args.no_cpu_baseline:
+        log(f"cpu baseline on {args.cpu_threads} threads ...")
+        cb = cpu_baseline(args.cpu_threads)
+        # the GPU path on the same bounded sample, for a like-for-like ratio
+        g = run_once(tdbem, cb["w"], keep=True)
+        err = float(np.abs(g["U"] - cb["U"]).max() / max(np.abs(cb["U"]).max(), 1e-300))
+        g["case"].close()
+        out["cpu_baseline"] = {"value": round(cb["value"], 4), "unit": "time steps/s", "cores": args.cpu_threads,
+                               "kind": "port",
+                               "sample": "oracle (C FP64 restatement, OpenMP) whole job on the layered block at "
+                                         "h=0.25 (1024 elements), 100 steps, same seeds and load family",
+                               "wall_s": round(cb["wall"], 3),
+                               "gpu_same_sample_steps_per_s": round(cb["w"].nstep / g["wall"], 3),
+                               "gpu_vs_cpu_max_rel_diff_u": err}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -89,6 +89,14 @@ int tdbem_dyna_gmres_solver(tdbem_ctx* ctx, long iterations, double error, long 
  * t[4] convolution (sum over steps), t[5] GMRES (sum over steps). */
 int tdbem_timings(tdbem_ctx* ctx, double* t6);
 
+/* Number of stored history band entries per lag j = 1..maxn
+ * (counts[j-1]) and the pair count, for byte accounting of the convolution.
+ * Build-side helper with no counterpart in the reference. */
+int tdbem_history_stats(tdbem_ctx* ctx, long* counts, long* npairs);
+
+/* Wait for all device work of this process (hipDeviceSynchronize). */
+int tdbem_device_synchronize(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1519,6 +1519,7 @@ orc_case* orc_create(int np, const double* points, int ne, const int* ele_pid, c
         c->el[c->itf[i].ea].surface = 1;
         c->el[c->itf[i].eb].surface = 1;
     }
+#pragma omp parallel for schedule(dynamic, 16)
     for (i = 0; i < ne; ++i) {
         elem* e = &c->el[i];
         double coord[3][8], dd;
@@ -2433,6 +2434,9 @@ int orc_assemble(orc_case* c, int maxn)
     c->mts = (ccsr*)calloc((size_t)maxn + 1, sizeof(ccsr));
     c->mgs = (ccsr*)calloc((size_t)maxn + 1, sizeof(ccsr));
     c->mgs_sym = (int*)calloc((size_t)maxn + 1, sizeof(int));
+    /* history steps are independent builders: parallel over steps, each
+     * step's sums stay sequential, so results do not depend on threads */
+#pragma omp parallel for schedule(dynamic, 1)
     for (k = 1; k <= maxn; ++k) {
         builder bt, bg;
         bld_init(&bt, c->counts[2], c->counts[7]);

@@ -213,4 +213,25 @@ int tdbem_timings(tdbem_ctx* ctx, double* t6)
     return 0;
 }
 
+int tdbem_history_stats(tdbem_ctx* ctx, long* counts, long* npairs)
+{
+    if (!ctx || !counts || !npairs)
+        return Fail("tdbem_history_stats: invalid arguments");
+    try {
+        ctx->c.HistoryStats(counts);
+    } catch (const std::exception& e) {
+        return Fail("tdbem_history_stats", e);
+    }
+    *npairs = ctx->c.npairs;
+    return 0;
+}
+
+int tdbem_device_synchronize(void)
+{
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess)
+        return Fail(hipGetErrorString(e));
+    return 0;
+}
+
 }  // extern "C"

@@ -374,6 +374,19 @@ void Case::AssembleHistory(int maxn_in)
     historyDone = true;
 }
 
+void Case::HistoryStats(long* counts) const
+{
+    if (!historyDone)
+        throw std::runtime_error("history not assembled");
+    for (int j = 0; j < maxn; ++j)
+        counts[j] = 0;
+    for (long p = 0; p < npairs; ++p) {
+        int k0 = h_kb[(size_t)p], L = h_len[(size_t)p];
+        for (int t = 0; t < L; ++t)
+            counts[k0 + t - 1]++;
+    }
+}
+
 void Case::ExportMatrix(int which, int step, std::vector<int>& rp, std::vector<int>& col, std::vector<double>& val)
 {
     if (which == 0 || which == 1) {

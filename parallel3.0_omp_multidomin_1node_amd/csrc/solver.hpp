@@ -32,6 +32,7 @@ public:
     void AssembleHistory(int maxn);
     void Solve(int nstep, const double* known_lu, long restart, double tol, double* lu, double* lt, int* iters,
                int* flags, int* solved);
+    void HistoryStats(long* counts) const;
     void ExportMatrix(int which, int step, std::vector<int>& rp, std::vector<int>& col, std::vector<double>& val);
 
     long ne = 0;

@@ -23,7 +23,7 @@ _lp = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
 EXPORTED = [
     "tdbem_last_error", "tdbem_create", "tdbem_destroy", "tdbem_dofmap_export", "tdbem_assemble_step0",
     "tdbem_assemble_history", "tdbem_ccsr_export", "tdbem_self_tables", "tdbem_solve", "tdbem_dyna_gmres_solver",
-    "tdbem_timings",
+    "tdbem_timings", "tdbem_history_stats", "tdbem_device_synchronize",
 ]
 
 
@@ -57,6 +57,8 @@ def lib():
         L.tdbem_dyna_gmres_solver.argtypes = [C.c_void_p, C.c_long, C.c_double, C.c_long, C.c_int, _dp, _dp, _dp,
                                               C.POINTER(C.c_int)]
         L.tdbem_timings.argtypes = [C.c_void_p, _dp]
+        L.tdbem_history_stats.argtypes = [C.c_void_p, _lp, C.POINTER(C.c_long)]
+        L.tdbem_device_synchronize.argtypes = []
         _lib = L
     return _lib
 
@@ -65,6 +67,10 @@ def _check(rc: int, what: str) -> None:
     if rc != 0:
         msg = lib().tdbem_last_error()
         raise RuntimeError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+
+def synchronize() -> None:
+    _check(lib().tdbem_device_synchronize(), "tdbem_device_synchronize")
 
 
 class Case:
@@ -153,6 +159,12 @@ class Case:
                                  C.byref(solved), times), "tdbem_solve")
         return (lu.reshape(nstep + 1, self.ne, 3), lt.reshape(nstep + 1, self.ne, 3), iters, flags, solved.value,
                 times)
+
+    def history_stats(self, maxn):
+        counts = np.zeros(maxn, np.int64)
+        npairs = C.c_long()
+        _check(lib().tdbem_history_stats(self.h, counts, C.byref(npairs)), "tdbem_history_stats")
+        return counts, npairs.value
 
     def timings(self):
         t = np.zeros(6)

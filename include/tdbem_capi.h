@@ -84,15 +84,23 @@ int tdbem_solve(tdbem_ctx* ctx, int nstep, const double* known_lu, long restart,
 int tdbem_dyna_gmres_solver(tdbem_ctx* ctx, long iterations, double error, long maxleafpointnum, int nstep,
                             const double* known_lu, double* lu, double* lt, int* result);
 
-/* Device timing of the last history assembly and solve, seconds:
- * t[0] singular tables, t[1] cull, t[2] history quadrature, t[3] step 0,
- * t[4] convolution (sum over steps), t[5] GMRES (sum over steps). */
-int tdbem_timings(tdbem_ctx* ctx, double* t6);
+/* Timing of the last assembly and solve, seconds (device events where the
+ * work is a kernel): t[0] singular tables, t[1] cull, t[2] history
+ * quadrature, t[3] step 0, t[4] convolution (sum over steps), t[5] GMRES
+ * (sum over steps), t[6] far-part convolution kernels, t[7] their count. */
+int tdbem_timings(tdbem_ctx* ctx, double* t8);
 
 /* Number of stored history band entries per lag j = 1..maxn
  * (counts[j-1]) and the pair count, for byte accounting of the convolution.
  * Build-side helper with no counterpart in the reference. */
 int tdbem_history_stats(tdbem_ctx* ctx, long* counts, long* npairs);
+
+/* Tuning options (no counterpart in the reference):
+ *   "conv_tiled" 1 (default) time-blocked history convolution, 0 one full pass
+ *   over the history bands per step (the reference's per-step summation);
+ *   "quad_fused" 1 (default) fused wave-per-pair history quadrature, 0 one
+ *   lane per band entry with separate T2 / T1 / U passes (same results). */
+int tdbem_set_option(tdbem_ctx* ctx, const char* name, double value);
 
 /* Wait for all device work of this process (hipDeviceSynchronize). */
 int tdbem_device_synchronize(void);

@@ -204,12 +204,12 @@ int tdbem_dyna_gmres_solver(tdbem_ctx* ctx, long iterations, double error, long 
     return 0;
 }
 
-int tdbem_timings(tdbem_ctx* ctx, double* t6)
+int tdbem_timings(tdbem_ctx* ctx, double* t8)
 {
-    if (!ctx || !t6)
+    if (!ctx || !t8)
         return Fail("tdbem_timings: invalid arguments");
-    for (int i = 0; i < 6; ++i)
-        t6[i] = ctx->c.timing[i];
+    for (int i = 0; i < 8; ++i)
+        t8[i] = ctx->c.timing[i];
     return 0;
 }
 
@@ -224,6 +224,22 @@ int tdbem_history_stats(tdbem_ctx* ctx, long* counts, long* npairs)
     }
     *npairs = ctx->c.npairs;
     return 0;
+}
+
+int tdbem_set_option(tdbem_ctx* ctx, const char* name, double value)
+{
+    if (!ctx || !name)
+        return Fail("tdbem_set_option: invalid arguments");
+    std::string n(name);
+    if (n == "conv_tiled") {
+        ctx->c.convTiled = value != 0.0;
+        return 0;
+    }
+    if (n == "quad_fused") {
+        ctx->c.quadFused = value != 0.0;
+        return 0;
+    }
+    return Fail("tdbem_set_option: unknown option");
 }
 
 int tdbem_device_synchronize(void)

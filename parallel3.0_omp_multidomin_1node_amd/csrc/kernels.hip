@@ -816,6 +816,267 @@ void LaunchHistory(const ElemGeo* geo, const ElemPoints* pts, const int* dom, co
 }
 
 // ---------------------------------------------------------------------------
+// Fused history quadrature: one wavefront per (pair, 64-lag chunk), one lag
+// per lane.  The lanes of a wave share the field element and the source, so
+// the point geometry is wave-uniform; the T2, T1 and U judged quadratures of
+// every lane run in one pass over the patch points (and one over the 16
+// regular points), sharing the distance and the geometric tensors
+// D, G, E, H, F of each point.  Each lane keeps the reference's arithmetic:
+// per point the kernel value is formed term by term (P front, S front,
+// between-fronts window) exactly as GetDynaT1ij / GetDynaT2ij / GetDynaUij
+// do, and each accumulator adds points in patch-then-point order, so the
+// results are identical to the unfused kernel bit for bit.  The regular
+// path's T1(k) + T2(k) (quirk Q2) is formed once and added for each judge
+// that selected it.
+// ---------------------------------------------------------------------------
+namespace dev {
+
+struct TCoef {
+    bool P, S;
+    int w;
+    double p1, p2, s1, s2, w1;
+};
+
+// Window flags and time coefficients of GetDynaT1ij (type 1) or
+// GetDynaT2ij (type 2) at level n for distance R[1].
+__device__ __forceinline__ void TimeCoef(const DynaMat& m, int type, double n, const double* R, TCoef& c)
+{
+    double nn = type == 1 ? n : n - 1;
+    c.P = Fai(nn, m.Dt, R[1], m.C1) != 0;
+    c.S = Fai(nn, m.Dt, R[1], m.C2) != 0;
+    double ndt, np1dt;
+    c.w = Window(nn, m.Dt, R[1], m, ndt, np1dt);
+    if (type == 1) {
+        c.p1 = kOneOver4Pi * (n + 1.0 - R[1] / m.C1Dt);
+        c.p2 = kOneOver4Pi / m.Dt;
+        c.s1 = kOneOver4Pi * (n + 1.0 - R[1] / m.C2Dt);
+        c.s2 = kOneOver4Pi / m.Dt;
+    } else {
+        c.p1 = kOneOver4Pi * (1.0 - n + R[1] / m.C1Dt);
+        c.p2 = -kOneOver4Pi / m.Dt;
+        c.s1 = kOneOver4Pi * (1.0 - n + R[1] / m.C2Dt);
+        c.s2 = -kOneOver4Pi / m.Dt;
+    }
+    c.w1 = 0.0;
+    if (c.w) {
+        double ri = ISer(c.w, nn, ndt, np1dt, R, m);
+        double rj = JSer(c.w, nn, ndt, np1dt, R, m);
+        if (type == 1)
+            c.w1 = kOneOver4Pi * ((n + 1.0) * ri - R[1] * rj / m.Dt);
+        else
+            c.w1 = kOneOver4Pi * ((1.0 - n) * ri + R[1] * rj / m.Dt);
+    }
+}
+
+// Kernel values of both types at one point: Ta (coefficients ca, if useA)
+// and Tb (cb, if useB), each formed as 0 + P-term + S-term + window-term.
+__device__ __forceinline__ void TwoT(const DynaMat& m, const Dist& d, const double* N, bool useA, const TCoef& ca,
+                                     bool useB, const TCoef& cb, double* Ta, double* Tb)
+{
+    double K1[9], K2[9];
+    for (int k = 0; k < 9; ++k) {
+        Ta[k] = 0.0;
+        Tb[k] = 0.0;
+    }
+    bool pa = useA && ca.P, pb = useB && cb.P;
+    if (pa || pb) {
+        Dij(m, d, N, K1);
+        Gij(m, d, N, K2);
+        if (pa)
+            for (int k = 0; k < 9; ++k)
+                Ta[k] += K1[k] * ca.p1 + K2[k] * ca.p2;
+        if (pb)
+            for (int k = 0; k < 9; ++k)
+                Tb[k] += K1[k] * cb.p1 + K2[k] * cb.p2;
+    }
+    bool sa = useA && ca.S, sb = useB && cb.S;
+    if (sa || sb) {
+        Eij(d, N, K1);
+        Hij(m, d, N, K2);
+        if (sa)
+            for (int k = 0; k < 9; ++k)
+                Ta[k] += K1[k] * ca.s1 + K2[k] * ca.s2;
+        if (sb)
+            for (int k = 0; k < 9; ++k)
+                Tb[k] += K1[k] * cb.s1 + K2[k] * cb.s2;
+    }
+    bool wa = useA && ca.w, wb = useB && cb.w;
+    if (wa || wb) {
+        Fij(m, d, N, K1);
+        if (wa)
+            for (int k = 0; k < 9; ++k)
+                Ta[k] += K1[k] * ca.w1;
+        if (wb)
+            for (int k = 0; k < 9; ++k)
+                Tb[k] += K1[k] * cb.w1;
+    }
+}
+
+__device__ __forceinline__ bool WaveAny(bool v) { return __ballot(v) != 0; }
+
+}  // namespace dev
+
+__global__ void __launch_bounds__(256) k_history_fused(
+    const ElemGeo* __restrict__ geo, const ElemPoints* __restrict__ pts, const NodeRefs* __restrict__ refs,
+    const double* __restrict__ selfT2, PairGrid grid, const long* __restrict__ coff, const long* __restrict__ off,
+    const int* __restrict__ kb, const int* __restrict__ len, long npairs, long c0, long c1, double* __restrict__ mts,
+    double* __restrict__ mgs, unsigned char* __restrict__ alive)
+{
+    // blockDim.x == 256: four wavefronts per block; the wave index is made
+    // explicitly uniform so the pair decode and all point loads are scalar.
+    long wv = c0 + (long)__builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4u + (threadIdx.x >> 6)));
+    int lane = threadIdx.x & 63;
+    if (wv >= c1)
+        return;  // whole wavefront
+    long lo = 0, hi = npairs;
+    while (hi - lo > 1) {
+        long mid = (lo + hi) >> 1;
+        if (coff[mid] <= wv)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    long p = lo;
+    int s, f, d;
+    dev::DecodePair(grid, p, s, f, d);
+    int t = (int)(wv - coff[p]) * 64 + lane;
+    bool active = t < len[p];
+    int k = kb[p] + t;
+    const DynaMat& m = c_mats[d];
+    const ElemGeo& fe = geo[f];
+    const ElemPoints& P = pts[f];
+    const double* src = geo[s].center;
+    double n = (double)k;
+    bool self1 = active && k <= 1 && s == f;
+    int f2 = (active && !self1) ? dev::NeedCal(m, fe, 2, n, src) : 0;
+    int f1 = active ? dev::NeedCal(m, fe, 1, n, src) : 0;
+    double accA[9], accB[9], accU[6];
+    for (int i = 0; i < 9; ++i) {
+        accA[i] = 0.0;
+        accB[i] = 0.0;
+    }
+    for (int i = 0; i < 6; ++i)
+        accU[i] = 0.0;
+    // patch-wise path
+    if (dev::WaveAny(f2 == 1 || f1 == 1)) {
+        for (int i = 0; i < kNPW2; ++i) {
+            bool m2 = f2 == 1 && dev::PWNeedCal(m, fe, 2, n, fe.rcenpw[i], src);
+            bool m1 = f1 == 1 && dev::PWNeedCal(m, fe, 1, n, fe.rcenpw[i], src);
+            if (!dev::WaveAny(m2 || m1))
+                continue;
+            for (int j = 0; j < kNPWG2; ++j) {
+                int q = i * kNPWG2 + j;
+                dev::Dist dd;
+                dev::MakeDist(src, P.pw_pos[q], dd);
+                double w = P.pw_w[q];
+                dev::TCoef c2{}, c1{};
+                if (m2)
+                    dev::TimeCoef(m, 2, n, dd.R, c2);
+                if (m1)
+                    dev::TimeCoef(m, 1, n, dd.R, c1);
+                double T2[9], T1[9];
+                dev::TwoT(m, dd, P.pw_nor[q], m2, c2, m1, c1, T2, T1);
+                if (m2)
+                    for (int kk = 0; kk < 9; ++kk)
+                        accA[kk] += T2[kk] * w;
+                if (m1) {
+                    for (int kk = 0; kk < 9; ++kk)
+                        accB[kk] += T1[kk] * w;
+                    double U[9];
+                    dev::DynaU(m, n, dd, U);
+                    accU[0] += U[0] * w;
+                    accU[1] += U[1] * w;
+                    accU[2] += U[2] * w;
+                    accU[3] += U[4] * w;
+                    accU[4] += U[5] * w;
+                    accU[5] += U[8] * w;
+                }
+            }
+        }
+    }
+    // regular path: T1(n) + T2(n) at the 16 points with the N0(gap) weight
+    bool r2 = f2 == 2, r1 = f1 == 2;
+    if (dev::WaveAny(r2 || r1)) {
+        for (int i = 0; i < kNG2; ++i) {
+            dev::Dist dd;
+            dev::MakeDist(src, P.rg_pos[i], dd);
+            double w = P.rg_w[i];
+            bool rt = r2 || r1;
+            dev::TCoef ca{}, cb{};
+            if (rt) {
+                dev::TimeCoef(m, 1, n, dd.R, ca);
+                dev::TimeCoef(m, 2, n, dd.R, cb);
+            }
+            double Ta[9], Tb[9];
+            dev::TwoT(m, dd, P.rg_nor[i], rt, ca, rt, cb, Ta, Tb);
+            int flag = (ca.P ? 1 : 0) + (ca.S ? 1 : 0) + (ca.w ? 1 : 0) + (cb.P ? 1 : 0) + (cb.S ? 1 : 0) +
+                       (cb.w ? 1 : 0);
+            if (rt && flag) {
+                for (int kk = 0; kk < 9; ++kk) {
+                    double v = (Ta[kk] + Tb[kk]) * w;
+                    if (r2)
+                        accA[kk] += v;
+                    if (r1)
+                        accB[kk] += v;
+                }
+            }
+            if (r1) {
+                double U[9];
+                if (dev::DynaU(m, n, dd, U)) {
+                    accU[0] += U[0] * w;
+                    accU[1] += U[1] * w;
+                    accU[2] += U[2] * w;
+                    accU[3] += U[4] * w;
+                    accU[4] += U[5] * w;
+                    accU[5] += U[8] * w;
+                }
+            }
+        }
+    }
+    if (!active)
+        return;
+    long e = off[p] + t;
+    dev::BlockAcc acc;
+    double su = refs[f].hu_sign;
+    double blk[9];
+    if (self1) {
+        for (int i = 0; i < 9; ++i)
+            blk[i] = selfT2[9 * (long)f + i];
+        acc.Add(su, blk);
+    } else if (f2) {
+        acc.Add(su, accA);
+    }
+    if (f1)
+        acc.Add(su, accB);
+    double* out = mts + 9 * e;
+    for (int i = 0; i < 9; ++i)
+        out[i] = acc.alive ? acc.v[i] : 0.0;
+    unsigned char bits = acc.alive ? 1 : 0;
+    dev::BlockAcc accg;
+    if (f1) {
+        double u9[9] = {accU[0], accU[1], accU[2], accU[1], accU[3], accU[4], accU[2], accU[4], accU[5]};
+        accg.Add(refs[f].ht_sign, u9);
+    }
+    double* og = mgs + 6 * e;
+    const int pick[6] = {0, 1, 2, 4, 5, 8};
+    for (int i = 0; i < 6; ++i)
+        og[i] = accg.alive ? accg.v[pick[i]] : 0.0;
+    bits |= accg.alive ? 2 : 0;
+    alive[e] = bits;
+}
+
+void LaunchHistoryFused(const ElemGeo* geo, const ElemPoints* pts, const NodeRefs* refs, const double* selfT2,
+                        PairGrid grid, const long* coff, const long* off, const int* kb, const int* len, long npairs,
+                        long c0, long c1, double* mts, double* mgs, unsigned char* alive, hipStream_t s)
+{
+    long waves = c1 - c0;
+    if (waves <= 0)
+        return;
+    k_history_fused<<<(unsigned)((waves + 3) / 4), 256, 0, s>>>(geo, pts, refs, selfT2, grid, coff, off, kb, len,
+                                                                npairs, c0, c1, mts, mgs, alive);
+}
+
+// ---------------------------------------------------------------------------
 // Step-0 coefficient blocks (DBEM1/dsquareelement.cpp:2756-2868,3281-3394,
 // 4215-4550,5157-5491,5922-6078,6476-6632) for FlagDynaMethod 2:
 // U: 4 U(0) + U(1);  T: 4 T1(0) + T2(1) + T1(1);  then +-K * R^T.
@@ -1060,6 +1321,201 @@ void LaunchConvolution(PairGrid grid, const NodeRefs* refs, const long* off, con
                        int maxn, int ne, double* rhs, hipStream_t s)
 {
     k_conv<<<ne, 256, 0, s>>>(grid, refs, off, kb, len, mts, mgs, gU, gT, nstore, K, maxn, rhs);
+}
+
+// ---------------------------------------------------------------------------
+// Time-blocked history convolution.
+//
+// Writing the history of step K as a causal convolution over solved states,
+//   h_K = sum_{m=1}^{K-1} sum_j [j, m] (MGS_j T_m - MTS_j U_m),
+// with weight 1, 2, 1 for m = K-j+1, K-j, K-j-1, the steps K0 .. K0+T-1 of a
+// tile split exactly into
+//   far_K  : states m <= K0-1 (all solved when the tile starts), and
+//   near_K : states K0 <= m <= K-1, which only reach lags j <= T+1.
+// k_conv_far reads every band block once per tile and applies it to the T
+// steps of the tile (states not yet solved are still zero in the history
+// buffers), so the block traffic per time step drops by T.  The arithmetic
+// is reassociated relative to the per-step sum, so results agree with it to
+// rounding, not bitwise.
+// ---------------------------------------------------------------------------
+namespace dev {
+
+__device__ __forceinline__ void BlockApply(const double* M, const double* G, const double* u, const double* t,
+                                           double* v)
+{
+    v[0] = (G[0] * t[0] + G[1] * t[1] + G[2] * t[2]) - (M[0] * u[0] + M[1] * u[1] + M[2] * u[2]);
+    v[1] = (G[1] * t[0] + G[3] * t[1] + G[4] * t[2]) - (M[3] * u[0] + M[4] * u[1] + M[5] * u[2]);
+    v[2] = (G[2] * t[0] + G[4] * t[1] + G[5] * t[2]) - (M[6] * u[0] + M[7] * u[1] + M[8] * u[2]);
+}
+
+__device__ __forceinline__ double WaveSum(double v)
+{
+    for (int o = 32; o > 0; o >>= 1)
+        v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+}  // namespace dev
+
+template <int T>
+__global__ void __launch_bounds__(256) k_conv_far(PairGrid grid, const NodeRefs* __restrict__ refs,
+                                                  const long* __restrict__ off, const int* __restrict__ kb,
+                                                  const int* __restrict__ len, const double* __restrict__ mts,
+                                                  const double* __restrict__ mgs, const double* __restrict__ gU,
+                                                  const double* __restrict__ gT, int nstore, int K0, int Tn, int maxn,
+                                                  int ne, double* __restrict__ hfar)
+{
+    int s = blockIdx.x;
+    int d = refs[s].domain;
+    int nd = grid.eleCount[d];
+    long rowStart = grid.pairBase[d] + (long)(s - grid.eleBegin[d]) * nd;
+    int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int Kmax = K0 + Tn - 1;
+    int jmax = Kmax < maxn ? Kmax : maxn;
+    double acc[T][3];
+#pragma unroll
+    for (int i = 0; i < T; ++i)
+        acc[i][0] = acc[i][1] = acc[i][2] = 0.0;
+    for (int q = wave; q < nd; q += 4) {
+        long p = rowStart + q;
+        int L = len[p];
+        if (L == 0)
+            continue;
+        int k0 = kb[p];
+        int f = grid.eleBegin[d] + q;
+        const double* su = gU + (long)refs[f].hu_block * nstore * 3;
+        const double* st = gT + (long)refs[f].ht_block * nstore * 3;
+        long base = off[p];
+        for (int t = lane; t < L; t += 64) {
+            int j = k0 + t;
+            if (j > jmax)
+                break;
+            long e = base + t;
+            double M[9], G[6];
+#pragma unroll
+            for (int i = 0; i < 9; ++i)
+                M[i] = mts[9 * e + i];
+#pragma unroll
+            for (int i = 0; i < 6; ++i)
+                G[i] = mgs[6 * e + i];
+            // rolling window of v_m = MGS T_m - MTS U_m for m = K-j-1, K-j, K-j+1
+            double vp[3], vc[3], vn[3];
+            auto load = [&](int m, double* v) {
+                if (m >= 1 && m <= K0 - 1) {
+                    const double* a = su + 3 * (long)m;
+                    const double* b = st + 3 * (long)m;
+                    double uu[3] = {a[0], a[1], a[2]}, tt[3] = {b[0], b[1], b[2]};
+                    dev::BlockApply(M, G, uu, tt, v);
+                } else {
+                    v[0] = v[1] = v[2] = 0.0;
+                }
+            };
+            load(K0 - j - 1, vp);
+            load(K0 - j, vc);
+#pragma unroll
+            for (int i = 0; i < T; ++i) {
+                load(K0 + i - j + 1, vn);
+                if (i < Tn) {
+                    acc[i][0] += vp[0] + 2.0 * vc[0] + vn[0];
+                    acc[i][1] += vp[1] + 2.0 * vc[1] + vn[1];
+                    acc[i][2] += vp[2] + 2.0 * vc[2] + vn[2];
+                }
+                for (int c = 0; c < 3; ++c) {
+                    vp[c] = vc[c];
+                    vc[c] = vn[c];
+                }
+            }
+        }
+    }
+    __shared__ double red[4][T * 3];
+#pragma unroll
+    for (int i = 0; i < T; ++i)
+        for (int c = 0; c < 3; ++c) {
+            double v = dev::WaveSum(acc[i][c]);
+            if (lane == 0)
+                red[wave][i * 3 + c] = v;
+        }
+    __syncthreads();
+    for (int x = threadIdx.x; x < Tn * 3; x += blockDim.x) {
+        double v = ((red[0][x] + red[1][x]) + red[2][x]) + red[3][x];
+        int i = x / 3, c = x % 3;
+        hfar[(long)i * 3 * ne + 3 * (long)s + c] = v;
+    }
+}
+
+// Near part of step K in the tile starting at K0 plus the far part:
+// rhs_K += hfar[K-K0] + sum over near pairs (band start <= K-K0+1) of the
+// taps with K0 <= m <= K-1.  One wavefront per source row.
+__global__ void __launch_bounds__(64) k_conv_near(const NodeRefs* __restrict__ refs, const long* __restrict__ nearPtr,
+                                                  const long* __restrict__ nearPair, const int* __restrict__ nearField,
+                                                  const long* __restrict__ off, const int* __restrict__ kb,
+                                                  const int* __restrict__ len, const double* __restrict__ mts,
+                                                  const double* __restrict__ mgs, const double* __restrict__ gU,
+                                                  const double* __restrict__ gT, int nstore, int K, int K0, int maxn,
+                                                  int ne, const double* __restrict__ hfar, double* __restrict__ rhs)
+{
+    int s = blockIdx.x;
+    int lane = threadIdx.x;
+    int jlim = K - K0 + 1;
+    if (jlim > maxn)
+        jlim = maxn;
+    double y[3] = {0.0, 0.0, 0.0};
+    for (long q = nearPtr[s]; q < nearPtr[s + 1]; ++q) {
+        long p = nearPair[q];
+        int k0 = kb[p], L = len[p];
+        int f = nearField[q];
+        const double* su = gU + (long)refs[f].hu_block * nstore * 3;
+        const double* st = gT + (long)refs[f].ht_block * nstore * 3;
+        for (int t = lane; t < L; t += 64) {
+            int j = k0 + t;
+            if (j > jlim)
+                break;
+            long e = off[p] + t;
+            const double* M = mts + 9 * e;
+            const double* G = mgs + 6 * e;
+            const int mm[3] = {K - j + 1, K - j, K - j - 1};
+            const double co[3] = {1.0, 2.0, 1.0};
+            double cu[3] = {0.0, 0.0, 0.0}, ct[3] = {0.0, 0.0, 0.0};
+            for (int z = 0; z < 3; ++z) {
+                int m = mm[z];
+                if (m >= K0 && m <= K - 1 && m >= 1) {
+                    for (int c = 0; c < 3; ++c) {
+                        cu[c] += co[z] * su[3 * (long)m + c];
+                        ct[c] += co[z] * st[3 * (long)m + c];
+                    }
+                }
+            }
+            double v[3];
+            dev::BlockApply(M, G, cu, ct, v);
+            y[0] += v[0];
+            y[1] += v[1];
+            y[2] += v[2];
+        }
+    }
+    for (int c = 0; c < 3; ++c)
+        y[c] = dev::WaveSum(y[c]);
+    if (lane == 0) {
+        const double* hf = hfar + (long)(K - K0) * 3 * ne + 3 * (long)s;
+        for (int c = 0; c < 3; ++c)
+            rhs[3 * (long)s + c] += hf[c] + y[c];
+    }
+}
+
+void LaunchConvFar(PairGrid grid, const NodeRefs* refs, const long* off, const int* kb, const int* len,
+                   const double* mts, const double* mgs, const double* gU, const double* gT, int nstore, int K0,
+                   int Tn, int maxn, int ne, double* hfar, hipStream_t s)
+{
+    k_conv_far<kConvTile><<<ne, 256, 0, s>>>(grid, refs, off, kb, len, mts, mgs, gU, gT, nstore, K0, Tn, maxn, ne,
+                                             hfar);
+}
+
+void LaunchConvNear(const NodeRefs* refs, const long* nearPtr, const long* nearPair, const int* nearField,
+                    const long* off, const int* kb, const int* len, const double* mts, const double* mgs,
+                    const double* gU, const double* gT, int nstore, int K, int K0, int maxn, int ne,
+                    const double* hfar, double* rhs, hipStream_t s)
+{
+    k_conv_near<<<ne, 64, 0, s>>>(refs, nearPtr, nearPair, nearField, off, kb, len, mts, mgs, gU, gT, nstore, K, K0,
+                                  maxn, ne, hfar, rhs);
 }
 
 // ---------------------------------------------------------------------------

@@ -53,6 +53,12 @@ void LaunchHistory(const ElemGeo* geo, const ElemPoints* pts, const int* dom, co
                    const double* selfT2, PairGrid grid, const long* off, const int* kb, long npairs, long e0, long e1,
                    double* mts, double* mgs, unsigned char* alive, hipStream_t s);
 
+// Fused history quadrature: one wavefront per (pair, 64-lag chunk); chunk
+// offsets coff[p] (npairs+1 entries), chunks [c0, c1).
+void LaunchHistoryFused(const ElemGeo* geo, const ElemPoints* pts, const NodeRefs* refs, const double* selfT2,
+                        PairGrid grid, const long* coff, const long* off, const int* kb, const int* len, long npairs,
+                        long c0, long c1, double* mts, double* mgs, unsigned char* alive, hipStream_t s);
+
 // Step-0 blocks of the near pairs listed in pairIdx (column-major, ready for
 // AddBlock): out[2*i] and out[2*i+1] with flags.
 void LaunchStep0(const ElemGeo* geo, const ElemPoints* pts, const int* dom, const NodeRefs* refs,
@@ -63,6 +69,17 @@ void LaunchStep0(const ElemGeo* geo, const ElemPoints* pts, const int* dom, cons
 void LaunchConvolution(PairGrid grid, const NodeRefs* refs, const long* off, const int* kb, const int* len,
                        const double* mts, const double* mgs, const double* gU, const double* gT, int nstore, int K,
                        int maxn, int ne, double* rhs, hipStream_t s);
+
+// Time-blocked convolution: far part of the tile [K0, K0+Tn) into
+// hfar[i][3*ne] (i < Tn), then per step the near part plus hfar into rhs.
+constexpr int kConvTile = 8;
+void LaunchConvFar(PairGrid grid, const NodeRefs* refs, const long* off, const int* kb, const int* len,
+                   const double* mts, const double* mgs, const double* gU, const double* gT, int nstore, int K0,
+                   int Tn, int maxn, int ne, double* hfar, hipStream_t s);
+void LaunchConvNear(const NodeRefs* refs, const long* nearPtr, const long* nearPair, const int* nearField,
+                    const long* off, const int* kb, const int* len, const double* mts, const double* mgs,
+                    const double* gU, const double* gT, int nstore, int K, int K0, int maxn, int ne,
+                    const double* hfar, double* rhs, hipStream_t s);
 
 // Block CCSR SpMV y = A x (row-major 3x3 blocks).
 void LaunchSpmv(long nrows, const int* rp, const int* col, const double* val, const double* x, double* y,

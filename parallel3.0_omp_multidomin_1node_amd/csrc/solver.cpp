@@ -218,6 +218,38 @@ void Case::Cull(int maxn)
     h_off[(size_t)npairs] = acc;
     entries = acc;
     d_off.upload(h_off.data(), h_off.size(), stream);
+    {
+        std::vector<long> coff((size_t)npairs + 1);
+        long c = 0;
+        for (long p = 0; p < npairs; ++p) {
+            coff[(size_t)p] = c;
+            c += (h_len[(size_t)p] + 63) / 64;
+        }
+        coff[(size_t)npairs] = c;
+        chunks = c;
+        d_coff.upload(coff.data(), coff.size(), stream);
+    }
+    // per source row, the pairs whose band starts at a lag <= tile + 1: the
+    // only ones the near part of the time-blocked convolution touches
+    std::vector<long> nearPtr((size_t)ne + 1, 0), nearPair;
+    std::vector<int> nearField;
+    for (int d = 0; d < grid.nd; ++d) {
+        long n = grid.eleCount[d];
+        for (long sl = 0; sl < n; ++sl) {
+            long srow = grid.eleBegin[d] + sl;
+            for (long fl = 0; fl < n; ++fl) {
+                long p = grid.pairBase[d] + sl * n + fl;
+                if (h_len[(size_t)p] > 0 && h_kb[(size_t)p] <= kConvTile + 1) {
+                    nearPair.push_back(p);
+                    nearField.push_back((int)(grid.eleBegin[d] + fl));
+                }
+            }
+            nearPtr[(size_t)srow + 1] = (long)nearPair.size();
+        }
+    }
+    d_nearPtr.upload(nearPtr.data(), nearPtr.size(), stream);
+    d_nearPair.upload(nearPair.data(), nearPair.size(), stream);
+    d_nearField.upload(nearField.data(), nearField.size(), stream);
     culledFor = maxn;
     timing[1] = Now() - t0;
 }
@@ -343,6 +375,10 @@ void Case::AssembleHistory(int maxn_in)
     h_mts.clear();
     h_mgs.clear();
     h_alive.clear();
+    d_mts.free();
+    d_mgs.free();
+    d_alive.free();
+    historyDone = false;
     Cull(maxn);
     size_t freeB = 0, totalB = 0;
     Check(hipMemGetInfo(&freeB, &totalB), "hipMemGetInfo");
@@ -357,11 +393,18 @@ void Case::AssembleHistory(int maxn_in)
     Check(hipEventCreate(&a), "event");
     Check(hipEventCreate(&b), "event");
     Check(hipEventRecord(a, stream), "event");
-    const long chunk = 1L << 24;
-    for (long e0 = 0; e0 < entries; e0 += chunk) {
-        long e1 = std::min(entries, e0 + chunk);
-        LaunchHistory(d_geo.p, d_pts.p, d_dom.p, d_refs.p, d_t2.p, grid, d_off.p, d_kb.p, npairs, e0, e1, d_mts.p,
-                      d_mgs.p, d_alive.p, stream);
+    if (quadFused) {
+        const long step = 1L << 18;  // wavefronts per launch
+        for (long c0 = 0; c0 < chunks; c0 += step)
+            LaunchHistoryFused(d_geo.p, d_pts.p, d_refs.p, d_t2.p, grid, d_coff.p, d_off.p, d_kb.p, d_len.p, npairs,
+                               c0, std::min(chunks, c0 + step), d_mts.p, d_mgs.p, d_alive.p, stream);
+    } else {
+        const long chunk = 1L << 24;
+        for (long e0 = 0; e0 < entries; e0 += chunk) {
+            long e1 = std::min(entries, e0 + chunk);
+            LaunchHistory(d_geo.p, d_pts.p, d_dom.p, d_refs.p, d_t2.p, grid, d_off.p, d_kb.p, npairs, e0, e1,
+                          d_mts.p, d_mgs.p, d_alive.p, stream);
+        }
     }
     Check(hipEventRecord(b, stream), "event");
     Check(hipEventSynchronize(b), "history");
@@ -557,7 +600,7 @@ void Case::Solve(int nstep, const double* known_lu, long restart, double tol, do
     PrepareGmres(restart);
     const long n3 = 3 * ne;
     const int nstore = nstep + 1;
-    DevBuf<double> d_bd, d_lu, d_lt, d_gU, d_gT, d_known, d_rhs, d_x;
+    DevBuf<double> d_bd, d_lu, d_lt, d_gU, d_gT, d_known, d_rhs, d_x, d_hfar;
     d_bd.upload(known_lu, (size_t)nstore * (size_t)n3, stream);
     d_lu.alloc((size_t)nstore * (size_t)n3);
     d_lt.alloc((size_t)nstore * (size_t)n3);
@@ -566,6 +609,7 @@ void Case::Solve(int nstep, const double* known_lu, long restart, double tol, do
     d_known.alloc(3 * (size_t)std::max<long>(1, dof.knownBlockCount));
     d_rhs.alloc((size_t)n3);
     d_x.alloc((size_t)n3);
+    d_hfar.alloc((size_t)kConvTile * (size_t)n3);
     LaunchZero(d_gU.p, (long)d_gU.n, stream);
     LaunchZero(d_gT.p, (long)d_gT.n, stream);
     LaunchZero(d_known.p, (long)d_known.n, stream);
@@ -574,15 +618,39 @@ void Case::Solve(int nstep, const double* known_lu, long restart, double tol, do
     hipEvent_t c0, c1;
     Check(hipEventCreate(&c0), "event");
     Check(hipEventCreate(&c1), "event");
-    double convTime = 0.0, gmresTime = 0.0;
+    double convTime = 0.0, gmresTime = 0.0, farTime = 0.0;
+    int farLaunches = 0;
+    hipEvent_t f0, f1;
+    Check(hipEventCreate(&f0), "event");
+    Check(hipEventCreate(&f1), "event");
     *solved = 0;
+    int K0 = 2;
     for (int K = 1; K <= nstep; ++K) {
         LaunchKnown(d_refs.p, d_bd.p + (size_t)K * (size_t)n3, (int)ne, d_known.p, dof.knownBlockCount, stream);
         LaunchSpmv(ne, d_Krp.p, d_Kcol.p, d_Kval.p, d_known.p, d_rhs.p, stream);
         if (K > 1) {
             Check(hipEventRecord(c0, stream), "event");
-            LaunchConvolution(grid, d_refs.p, d_off.p, d_kb.p, d_len.p, d_mts.p, d_mgs.p, d_gU.p, d_gT.p, nstore, K,
-                              maxn, (int)ne, d_rhs.p, stream);
+            if (convTiled) {
+                if (K == 2 || K - K0 >= kConvTile) {
+                    K0 = K;
+                    int Tn = std::min(kConvTile, nstep - K0 + 1);
+                    Check(hipEventRecord(f0, stream), "event");
+                    LaunchConvFar(grid, d_refs.p, d_off.p, d_kb.p, d_len.p, d_mts.p, d_mgs.p, d_gU.p, d_gT.p, nstore,
+                                  K0, Tn, maxn, (int)ne, d_hfar.p, stream);
+                    Check(hipEventRecord(f1, stream), "event");
+                    Check(hipEventSynchronize(f1), "far");
+                    float fms = 0.f;
+                    Check(hipEventElapsedTime(&fms, f0, f1), "elapsed");
+                    farTime += fms * 1e-3;
+                    ++farLaunches;
+                }
+                LaunchConvNear(d_refs.p, d_nearPtr.p, d_nearPair.p, d_nearField.p, d_off.p, d_kb.p, d_len.p,
+                               d_mts.p, d_mgs.p, d_gU.p, d_gT.p, nstore, K, K0, maxn, (int)ne, d_hfar.p, d_rhs.p,
+                               stream);
+            } else {
+                LaunchConvolution(grid, d_refs.p, d_off.p, d_kb.p, d_len.p, d_mts.p, d_mgs.p, d_gU.p, d_gT.p, nstore,
+                                  K, maxn, (int)ne, d_rhs.p, stream);
+            }
             Check(hipEventRecord(c1, stream), "event");
             Check(hipEventSynchronize(c1), "conv");
             float ms = 0.f;
@@ -628,8 +696,12 @@ void Case::Solve(int nstep, const double* known_lu, long restart, double tol, do
     }
     (void)hipEventDestroy(c0);
     (void)hipEventDestroy(c1);
+    (void)hipEventDestroy(f0);
+    (void)hipEventDestroy(f1);
     timing[4] = convTime;
     timing[5] = gmresTime;
+    timing[6] = farTime;
+    timing[7] = farLaunches;
 }
 
 }  // namespace tdbem

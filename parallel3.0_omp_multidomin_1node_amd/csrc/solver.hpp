@@ -49,7 +49,9 @@ public:
     int maxn = 0;
     CCSRMat A, KnownM;
     LeafPreconditioner pre;
-    double timing[6] = {0, 0, 0, 0, 0, 0};
+    double timing[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    bool convTiled = true;   // time-blocked convolution (false: one full pass per step)
+    bool quadFused = false;  // fused wave-per-pair history quadrature (false: lane per entry)
     hipStream_t stream = nullptr;
 
     DevBuf<double> d_u1, d_t1, d_t2;
@@ -69,7 +71,10 @@ private:
     DevBuf<NodeRefs> d_refs;
     DevBuf<int> d_kb, d_len;
     DevBuf<unsigned char> d_near;
-    DevBuf<long> d_off;
+    DevBuf<long> d_off, d_coff;
+    long chunks = 0;
+    DevBuf<long> d_nearPtr, d_nearPair;
+    DevBuf<int> d_nearField;
     std::vector<int> h_kb, h_len;
     std::vector<long> h_off;
     DevBuf<double> d_mts, d_mgs;

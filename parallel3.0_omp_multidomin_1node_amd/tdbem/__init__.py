@@ -24,6 +24,7 @@ EXPORTED = [
     "tdbem_last_error", "tdbem_create", "tdbem_destroy", "tdbem_dofmap_export", "tdbem_assemble_step0",
     "tdbem_assemble_history", "tdbem_ccsr_export", "tdbem_self_tables", "tdbem_solve", "tdbem_dyna_gmres_solver",
     "tdbem_timings", "tdbem_history_stats", "tdbem_device_synchronize",
+    "tdbem_set_option",
 ]
 
 
@@ -59,6 +60,7 @@ def lib():
         L.tdbem_timings.argtypes = [C.c_void_p, _dp]
         L.tdbem_history_stats.argtypes = [C.c_void_p, _lp, C.POINTER(C.c_long)]
         L.tdbem_device_synchronize.argtypes = []
+        L.tdbem_set_option.argtypes = [C.c_void_p, C.c_char_p, C.c_double]
         _lib = L
     return _lib
 
@@ -160,6 +162,9 @@ class Case:
         return (lu.reshape(nstep + 1, self.ne, 3), lt.reshape(nstep + 1, self.ne, 3), iters, flags, solved.value,
                 times)
 
+    def set_option(self, name, value):
+        _check(lib().tdbem_set_option(self.h, name.encode(), float(value)), "tdbem_set_option")
+
     def history_stats(self, maxn):
         counts = np.zeros(maxn, np.int64)
         npairs = C.c_long()
@@ -167,6 +172,7 @@ class Case:
         return counts, npairs.value
 
     def timings(self):
-        t = np.zeros(6)
+        t = np.zeros(8)
         _check(lib().tdbem_timings(self.h, t), "tdbem_timings")
-        return dict(singular=t[0], cull=t[1], history=t[2], step0=t[3], convolution=t[4], gmres=t[5])
+        return dict(singular=t[0], cull=t[1], history=t[2], step0=t[3], convolution=t[4], gmres=t[5],
+                    conv_far=t[6], conv_far_launches=int(t[7]))

@@ -182,3 +182,52 @@ def test_full_size_properties():
         assert np.array_equal(val, np.transpose(val, (0, 2, 1)))
     for da, ea, db, eb, s in m.interfaces:
         assert np.abs(U[:, ea - 1] - U[:, eb - 1]).max() <= 1e-12 * np.abs(U).max()
+
+
+def test_tiled_convolution_matches_per_step_sum(oracle_mod):
+    """The time-blocked convolution (far part per 8-step tile + near part per
+    step) reassociates the reference's per-step history sum; both give the same
+    solution to rounding."""
+    tdbem = _tdbem()
+    from tdbem import mesh
+    w = mesh.layered_config2(h=0.5, nstep=45)
+    m = w.mesh
+    print("case:", w.name, "tiled vs per-step", flush=True)
+    gc = tdbem.Case(m.points, m.ele, m.bcid, w.dt, w.gap, domain_count=4, mat_rows=w.mats, itf_rows=m.interfaces)
+    gc.assemble_step0(10)
+    gc.assemble_history(w.nstep)
+    U1, T1, *_ = gc.solve(w.known, w.nstep, tol=1e-13)
+    gc.set_option("conv_tiled", 0)
+    U0, T0, *_ = gc.solve(w.known, w.nstep, tol=1e-13)
+    assert np.abs(U1 - U0).max() <= 1e-11 * np.abs(U0).max()
+    assert np.abs(T1 - T0).max() <= 1e-11 * np.abs(T0).max()
+
+
+@pytest.mark.parametrize("which", ["regular_path_probe", "layered_small"])
+def test_fused_quadrature_matches_per_entry(oracle_mod, which):
+    """The fused wave-per-pair history quadrature and the lane-per-entry
+    quadrature give the same MTS/MGS bands bit for bit, and both equal the
+    oracle's blocks."""
+    from tdbem import mesh
+    if which == "regular_path_probe":
+        m = mesh.box_mesh(2, 4.0, 1.0, 1.0, 8, 2, 2)
+        mats = np.array([[1, 10.0, 0.45, 1.0], [2, 10.0, 0.45, 1.0]])
+        known = mesh.normal_load_history(len(m.ele), 30, 0.02, m.top, np.r_[0.0, np.ones(30)])
+        w = mesh.Workload("regular_path_probe", m, mats, 0.02, 30, known)
+    else:
+        w = mesh.layered_config2(h=0.5, nstep=40)
+    print("case:", w.name, "fused vs per-entry quadrature", flush=True)
+    oc, gc = _workload_cases(oracle_mod, w)
+    gc.assemble_step0(10)
+    gc.set_option("quad_fused", 0)
+    gc.assemble_history(w.nstep)
+    ref = [(gc.ccsr(2, k), gc.ccsr(3, k)) for k in range(1, w.nstep + 1)]
+    gc.set_option("quad_fused", 1)
+    gc.assemble_history(w.nstep)
+    oc.assemble(w.nstep)
+    for k in range(1, w.nstep + 1):
+        for i, which_m in enumerate((2, 3)):
+            a, b, o = gc.ccsr(which_m, k), ref[k - 1][i], oc.ccsr(which_m, k)
+            for x, y in ((a, b), (a, o)):
+                assert np.array_equal(x[0], y[0]) and np.array_equal(x[1], y[1]), (which_m, k)
+                assert np.array_equal(x[2], y[2]), (which_m, k)

@@ -98,8 +98,9 @@ int tdbem_history_stats(tdbem_ctx* ctx, long* counts, long* npairs);
 /* Tuning options (no counterpart in the reference):
  *   "conv_tiled" 1 (default) time-blocked history convolution, 0 one full pass
  *   over the history bands per step (the reference's per-step summation);
- *   "quad_fused" 1 (default) fused wave-per-pair history quadrature, 0 one
- *   lane per band entry with separate T2 / T1 / U passes (same results). */
+ *   "quad_staged" 1 (default) wave-per-pair history quadrature with the
+ *   point geometry staged in LDS, 0 one lane per band entry with separate
+ *   T2 / T1 / U passes (same results). */
 int tdbem_set_option(tdbem_ctx* ctx, const char* name, double value);
 
 /* Wait for all device work of this process (hipDeviceSynchronize). */

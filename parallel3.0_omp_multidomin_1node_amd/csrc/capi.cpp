@@ -235,8 +235,8 @@ int tdbem_set_option(tdbem_ctx* ctx, const char* name, double value)
         ctx->c.convTiled = value != 0.0;
         return 0;
     }
-    if (n == "quad_fused") {
-        ctx->c.quadFused = value != 0.0;
+    if (n == "quad_staged") {
+        ctx->c.quadStaged = value != 0.0;
         return 0;
     }
     return Fail("tdbem_set_option: unknown option");

@@ -816,118 +816,271 @@ void LaunchHistory(const ElemGeo* geo, const ElemPoints* pts, const int* dom, co
 }
 
 // ---------------------------------------------------------------------------
-// Fused history quadrature: one wavefront per (pair, 64-lag chunk), one lag
-// per lane.  The lanes of a wave share the field element and the source, so
-// the point geometry is wave-uniform; the T2, T1 and U judged quadratures of
-// every lane run in one pass over the patch points (and one over the 16
-// regular points), sharing the distance and the geometric tensors
-// D, G, E, H, F of each point.  Each lane keeps the reference's arithmetic:
-// per point the kernel value is formed term by term (P front, S front,
-// between-fronts window) exactly as GetDynaT1ij / GetDynaT2ij / GetDynaUij
-// do, and each accumulator adds points in patch-then-point order, so the
-// results are identical to the unfused kernel bit for bit.  The regular
-// path's T1(k) + T2(k) (quirk Q2) is formed once and added for each judge
-// that selected it.
+// Staged history quadrature: one wavefront per (pair, chunk of up to 64 lags),
+// one lag per lane.  The lanes of a wave share the field element and the
+// source, so everything about a quadrature point that does not depend on the
+// time level -- the distance powers and ratios R/c, R/(c dt), the geometric
+// tensors D, G, E, H, F of GetDynaT1ij / GetDynaT2ij and the P-front,
+// S-front and between-fronts tensors of GetDynaUij -- is computed once per
+// point by one lane and staged in LDS.  Each lane then forms its own time
+// coefficients (front flags, ISeries / JSeries window, the n-dependent
+// factors) and adds term by term in the reference's order: per point
+// T = 0 + P term + S term + window term, and each accumulator adds the points
+// patch by patch in ascending order.  Every staged value is the same
+// expression of the same inputs as in the per-point kernels, so the blocks
+// equal the lane-per-entry kernel's bit for bit.  The regular path's
+// T1(n) + T2(n) (quirk Q2) is formed once and added for each judge that
+// selected it.
 // ---------------------------------------------------------------------------
 namespace dev {
 
-struct TCoef {
-    bool P, S;
-    int w;
-    double p1, p2, s1, s2, w1;
+constexpr int kStageStride = 70;  // doubles per staged point (16-byte aligned records)
+enum StageField : int {
+    sRc1 = 0,     // R / C1
+    sRc2 = 1,     // R / C2
+    sRC1Dt = 2,   // R / (C1 dt)
+    sRC2Dt = 3,   // R / (C2 dt)
+    sR1 = 4,
+    sR2 = 5,
+    sDt3R3 = 6,   // dt^3 / R^3
+    sW = 7,       // weight * Jacobian
+    sD = 8,       // Dij
+    sGt = 17,     // Gij * (1 / (4 pi dt))
+    sE = 26,      // Eij
+    sHt = 35,     // Hij * (1 / (4 pi dt))
+    sF = 44,      // Fij, entries 0 1 2 4 5 8
+    sUP = 50,     // P-front U terms, entries 0 1 2 4 5 8
+    sUS = 56,     // S-front U terms
+    sUC = 62,     // between-fronts U tensor (times the ISeries factor per lane)
 };
 
-// Window flags and time coefficients of GetDynaT1ij (type 1) or
-// GetDynaT2ij (type 2) at level n for distance R[1].
-__device__ __forceinline__ void TimeCoef(const DynaMat& m, int type, double n, const double* R, TCoef& c)
+__device__ void StagePoint(const DynaMat& m, const double* src, const double* pos, const double* nor, double w,
+                           double* g)
 {
-    double nn = type == 1 ? n : n - 1;
-    c.P = Fai(nn, m.Dt, R[1], m.C1) != 0;
-    c.S = Fai(nn, m.Dt, R[1], m.C2) != 0;
-    double ndt, np1dt;
-    c.w = Window(nn, m.Dt, R[1], m, ndt, np1dt);
-    if (type == 1) {
-        c.p1 = kOneOver4Pi * (n + 1.0 - R[1] / m.C1Dt);
-        c.p2 = kOneOver4Pi / m.Dt;
-        c.s1 = kOneOver4Pi * (n + 1.0 - R[1] / m.C2Dt);
-        c.s2 = kOneOver4Pi / m.Dt;
-    } else {
-        c.p1 = kOneOver4Pi * (1.0 - n + R[1] / m.C1Dt);
-        c.p2 = -kOneOver4Pi / m.Dt;
-        c.s1 = kOneOver4Pi * (1.0 - n + R[1] / m.C2Dt);
-        c.s2 = -kOneOver4Pi / m.Dt;
+    Dist d;
+    MakeDist(src, pos, d);
+    const double* R = d.R;
+    const double* X = d.X;
+    g[sRc1] = R[1] / m.C1;
+    g[sRc2] = R[1] / m.C2;
+    g[sRC1Dt] = R[1] / m.C1Dt;
+    g[sRC2Dt] = R[1] / m.C2Dt;
+    g[sR1] = R[1];
+    g[sR2] = R[2];
+    g[sDt3R3] = m.Dt_3 / R[3];
+    g[sW] = w;
+    double K[9];
+    double t2 = kOneOver4Pi / m.Dt;
+    Dij(m, d, nor, K);
+    for (int k = 0; k < 9; ++k)
+        g[sD + k] = K[k];
+    Gij(m, d, nor, K);
+    for (int k = 0; k < 9; ++k)
+        g[sGt + k] = K[k] * t2;
+    Eij(d, nor, K);
+    for (int k = 0; k < 9; ++k)
+        g[sE + k] = K[k];
+    Hij(m, d, nor, K);
+    for (int k = 0; k < 9; ++k)
+        g[sHt + k] = K[k] * t2;
+    Fij(m, d, nor, K);
+    g[sF + 0] = K[0];
+    g[sF + 1] = K[1];
+    g[sF + 2] = K[2];
+    g[sF + 3] = K[4];
+    g[sF + 4] = K[5];
+    g[sF + 5] = K[8];
+    double t1 = m.OneOverFourPaiRou;
+    {
+        double C1 = m.OneOverC1_2 / R[3];
+        g[sUP + 0] = (C1 * X[0]) * t1;
+        g[sUP + 1] = (C1 * X[1]) * t1;
+        g[sUP + 2] = (C1 * X[2]) * t1;
+        g[sUP + 3] = (C1 * X[3]) * t1;
+        g[sUP + 4] = (C1 * X[4]) * t1;
+        g[sUP + 5] = (C1 * X[5]) * t1;
     }
-    c.w1 = 0.0;
-    if (c.w) {
-        double ri = ISer(c.w, nn, ndt, np1dt, R, m);
-        double rj = JSer(c.w, nn, ndt, np1dt, R, m);
-        if (type == 1)
-            c.w1 = kOneOver4Pi * ((n + 1.0) * ri - R[1] * rj / m.Dt);
-        else
-            c.w1 = kOneOver4Pi * ((1.0 - n) * ri + R[1] * rj / m.Dt);
+    {
+        double c = m.OneOverC2_2;
+        g[sUS + 0] = (c * (1.0 / R[1] - X[0] / R[3])) * t1;
+        g[sUS + 1] = (c * (-X[1] / R[3])) * t1;
+        g[sUS + 2] = (c * (-X[2] / R[3])) * t1;
+        g[sUS + 3] = (c * (1.0 / R[1] - X[3] / R[3])) * t1;
+        g[sUS + 4] = (c * (-X[4] / R[3])) * t1;
+        g[sUS + 5] = (c * (1.0 / R[1] - X[5] / R[3])) * t1;
+    }
+    {
+        double C1 = 3.0 / R[3], C2 = 1.0 / R[1];
+        g[sUC + 0] = C1 * X[0] - C2;
+        g[sUC + 1] = C1 * X[1];
+        g[sUC + 2] = C1 * X[2];
+        g[sUC + 3] = C1 * X[3] - C2;
+        g[sUC + 4] = C1 * X[4];
+        g[sUC + 5] = C1 * X[5] - C2;
     }
 }
 
-// Kernel values of both types at one point: Ta (coefficients ca, if useA)
-// and Tb (cb, if useB), each formed as 0 + P-term + S-term + window-term.
-__device__ __forceinline__ void TwoT(const DynaMat& m, const Dist& d, const double* N, bool useA, const TCoef& ca,
-                                     bool useB, const TCoef& cb, double* Ta, double* Tb)
+// Time coefficients of one lane at one staged point: GetDynaT1ij (type 1,
+// level n) or GetDynaT2ij (type 2, level n - 1).
+struct LaneCoef {
+    bool P = false, S = false;
+    int w = 0;
+    double p1 = 0.0, s1 = 0.0, w1 = 0.0, ri = 0.0, sg = 1.0;
+};
+
+__device__ __forceinline__ double ISerS(int w, double n, double ndt, double np1dt, const double* g, const DynaMat& m)
 {
-    double K1[9], K2[9];
-    for (int k = 0; k < 9; ++k) {
-        Ta[k] = 0.0;
-        Tb[k] = 0.0;
+    double t1;
+    switch (w) {
+    case 1: return (n + 0.5) * m.Dt_2 / g[sR2];
+    case 2: t1 = np1dt / g[sR1]; return 0.5 * (t1 * t1 - m.OneOverC1_2);
+    case 3: return m.Half_1OC22_1OC12;
+    case 4: t1 = ndt / g[sR1]; return 0.5 * (m.OneOverC2_2 - t1 * t1);
+    default: return 0.0;
     }
-    bool pa = useA && ca.P, pb = useB && cb.P;
-    if (pa || pb) {
-        Dij(m, d, N, K1);
-        Gij(m, d, N, K2);
-        if (pa)
-            for (int k = 0; k < 9; ++k)
-                Ta[k] += K1[k] * ca.p1 + K2[k] * ca.p2;
-        if (pb)
-            for (int k = 0; k < 9; ++k)
-                Tb[k] += K1[k] * cb.p1 + K2[k] * cb.p2;
+}
+
+__device__ __forceinline__ double JSerS(int w, double n, double ndt, double np1dt, const double* g, const DynaMat& m)
+{
+    double t1;
+    switch (w) {
+    case 1: t1 = g[sDt3R3]; return (n * n + n + 0.33333333333333333) * t1;
+    case 2: t1 = np1dt / g[sR1]; return 0.33333333333333 * (t1 * t1 * t1 - m.OneOverC1_3);
+    case 3: return m.OneOverThree_1OC23_1OC13;
+    case 4: t1 = ndt / g[sR1]; return 0.333333333333333 * (m.OneOverC2_3 - t1 * t1 * t1);
+    default: return 0.0;
     }
-    bool sa = useA && ca.S, sb = useB && cb.S;
-    if (sa || sb) {
-        Eij(d, N, K1);
-        Hij(m, d, N, K2);
-        if (sa)
-            for (int k = 0; k < 9; ++k)
-                Ta[k] += K1[k] * ca.s1 + K2[k] * ca.s2;
-        if (sb)
-            for (int k = 0; k < 9; ++k)
-                Tb[k] += K1[k] * cb.s1 + K2[k] * cb.s2;
+}
+
+__device__ __forceinline__ void MakeLaneCoef(const DynaMat& m, int type, double n, const double* g, LaneCoef& c)
+{
+    double nn = type == 1 ? n : n - 1;
+    double ndt = nn * m.Dt;
+    double np1dt = ndt + m.Dt;
+    double rc1 = g[sRc1], rc2 = g[sRc2];
+    c.P = (rc1 >= ndt) && (rc1 < np1dt);
+    c.S = (rc2 >= ndt) && (rc2 < np1dt);
+    int w = 0;
+    if (np1dt <= rc1 || ndt >= rc2)
+        w = 0;
+    else if (ndt >= rc1 && np1dt <= rc2)
+        w = 1;
+    else if (ndt <= rc1 && (np1dt >= rc1 && np1dt <= rc2))
+        w = 2;
+    else if (ndt <= rc1 && np1dt >= rc2)
+        w = 3;
+    else if (np1dt >= rc2 && (ndt >= rc1 && ndt <= rc2))
+        w = 4;
+    c.w = w;
+    if (type == 1) {
+        c.p1 = kOneOver4Pi * (n + 1.0 - g[sRC1Dt]);
+        c.s1 = kOneOver4Pi * (n + 1.0 - g[sRC2Dt]);
+        c.sg = 1.0;
+    } else {
+        c.p1 = kOneOver4Pi * (1.0 - n + g[sRC1Dt]);
+        c.s1 = kOneOver4Pi * (1.0 - n + g[sRC2Dt]);
+        c.sg = -1.0;
     }
-    bool wa = useA && ca.w, wb = useB && cb.w;
-    if (wa || wb) {
-        Fij(m, d, N, K1);
-        if (wa)
-            for (int k = 0; k < 9; ++k)
-                Ta[k] += K1[k] * ca.w1;
-        if (wb)
-            for (int k = 0; k < 9; ++k)
-                Tb[k] += K1[k] * cb.w1;
+    c.w1 = 0.0;
+    c.ri = 0.0;
+    if (w) {
+        double ri = ISerS(w, nn, ndt, np1dt, g, m);
+        double rj = JSerS(w, nn, ndt, np1dt, g, m);
+        c.ri = ri;
+        if (type == 1)
+            c.w1 = kOneOver4Pi * ((n + 1.0) * ri - g[sR1] * rj / m.Dt);
+        else
+            c.w1 = kOneOver4Pi * ((1.0 - n) * ri + g[sR1] * rj / m.Dt);
     }
 }
 
 __device__ __forceinline__ bool WaveAny(bool v) { return __ballot(v) != 0; }
 
+// Kernel values Ta (coefficients ca, if ua) and Tb (cb, if ub) at one staged
+// point, each 0 + P term + S term + window term.  The G and H products with
+// +-1/(4 pi dt) are staged for type 1; type 2's are their exact negations.
+__device__ __forceinline__ void StagedT(const double* g, bool ua, const LaneCoef& ca, bool ub, const LaneCoef& cb,
+                                        double* Ta, double* Tb)
+{
+    for (int k = 0; k < 9; ++k) {
+        Ta[k] = 0.0;
+        Tb[k] = 0.0;
+    }
+    bool pa = ua && ca.P, pb = ub && cb.P;
+    if (WaveAny(pa || pb)) {
+        for (int k = 0; k < 9; ++k) {
+            double dk = g[sD + k], gk = g[sGt + k];
+            if (pa)
+                Ta[k] += dk * ca.p1 + ca.sg * gk;
+            if (pb)
+                Tb[k] += dk * cb.p1 + cb.sg * gk;
+        }
+    }
+    bool sa = ua && ca.S, sb = ub && cb.S;
+    if (WaveAny(sa || sb)) {
+        for (int k = 0; k < 9; ++k) {
+            double ek = g[sE + k], hk = g[sHt + k];
+            if (sa)
+                Ta[k] += ek * ca.s1 + ca.sg * hk;
+            if (sb)
+                Tb[k] += ek * cb.s1 + cb.sg * hk;
+        }
+    }
+    bool wa = ua && ca.w, wb = ub && cb.w;
+    if (WaveAny(wa || wb)) {
+        const int map[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
+        for (int k = 0; k < 9; ++k) {
+            double fk = g[sF + map[k]];
+            if (wa)
+                Ta[k] += fk * ca.w1;
+            if (wb)
+                Tb[k] += fk * cb.w1;
+        }
+    }
+}
+
+// GetDynaUij at one staged point with the type-1 coefficients c (same level
+// and window as the U kernel): unique entries 0 1 2 4 5 8.
+__device__ __forceinline__ void StagedU(const DynaMat& m, const double* g, bool u, const LaneCoef& c, double* U)
+{
+    for (int k = 0; k < 6; ++k)
+        U[k] = 0.0;
+    if (WaveAny(u && c.P)) {
+        if (u && c.P)
+            for (int k = 0; k < 6; ++k)
+                U[k] += g[sUP + k];
+    }
+    if (WaveAny(u && c.S)) {
+        if (u && c.S)
+            for (int k = 0; k < 6; ++k)
+                U[k] += g[sUS + k];
+    }
+    if (WaveAny(u && c.w)) {
+        if (u && c.w) {
+            double t1 = m.OneOverFourPaiRou * c.ri;
+            for (int k = 0; k < 6; ++k)
+                U[k] += g[sUC + k] * t1;
+        }
+    }
+}
+
 }  // namespace dev
 
-__global__ void __launch_bounds__(256) k_history_fused(
+constexpr int kStagePatches = 4;  // patches staged per batch (36 points)
+
+__global__ void __launch_bounds__(64) k_history_staged(
     const ElemGeo* __restrict__ geo, const ElemPoints* __restrict__ pts, const NodeRefs* __restrict__ refs,
     const double* __restrict__ selfT2, PairGrid grid, const long* __restrict__ coff, const long* __restrict__ off,
     const int* __restrict__ kb, const int* __restrict__ len, long npairs, long c0, long c1, double* __restrict__ mts,
     double* __restrict__ mgs, unsigned char* __restrict__ alive)
 {
-    // blockDim.x == 256: four wavefronts per block; the wave index is made
-    // explicitly uniform so the pair decode and all point loads are scalar.
-    long wv = c0 + (long)__builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4u + (threadIdx.x >> 6)));
-    int lane = threadIdx.x & 63;
+    __shared__ double sgeo[kStagePatches * kNPWG2 * dev::kStageStride];
+    __shared__ double spr[2 * kNPW2];
+    using namespace dev;
+    long wv = c0 + (long)blockIdx.x;  // one wavefront per block
     if (wv >= c1)
-        return;  // whole wavefront
+        return;
+    int lane = threadIdx.x;
     long lo = 0, hi = npairs;
     while (hi - lo > 1) {
         long mid = (lo + hi) >> 1;
@@ -938,7 +1091,7 @@ __global__ void __launch_bounds__(256) k_history_fused(
     }
     long p = lo;
     int s, f, d;
-    dev::DecodePair(grid, p, s, f, d);
+    DecodePair(grid, p, s, f, d);
     int t = (int)(wv - coff[p]) * 64 + lane;
     bool active = t < len[p];
     int k = kb[p] + t;
@@ -948,8 +1101,8 @@ __global__ void __launch_bounds__(256) k_history_fused(
     const double* src = geo[s].center;
     double n = (double)k;
     bool self1 = active && k <= 1 && s == f;
-    int f2 = (active && !self1) ? dev::NeedCal(m, fe, 2, n, src) : 0;
-    int f1 = active ? dev::NeedCal(m, fe, 1, n, src) : 0;
+    int f2 = (active && !self1) ? NeedCal(m, fe, 2, n, src) : 0;
+    int f1 = active ? NeedCal(m, fe, 1, n, src) : 0;
     double accA[9], accB[9], accU[6];
     for (int i = 0; i < 9; ++i) {
         accA[i] = 0.0;
@@ -957,60 +1110,96 @@ __global__ void __launch_bounds__(256) k_history_fused(
     }
     for (int i = 0; i < 6; ++i)
         accU[i] = 0.0;
-    // patch-wise path
-    if (dev::WaveAny(f2 == 1 || f1 == 1)) {
-        for (int i = 0; i < kNPW2; ++i) {
-            bool m2 = f2 == 1 && dev::PWNeedCal(m, fe, 2, n, fe.rcenpw[i], src);
-            bool m1 = f1 == 1 && dev::PWNeedCal(m, fe, 1, n, fe.rcenpw[i], src);
-            if (!dev::WaveAny(m2 || m1))
-                continue;
-            for (int j = 0; j < kNPWG2; ++j) {
-                int q = i * kNPWG2 + j;
-                dev::Dist dd;
-                dev::MakeDist(src, P.pw_pos[q], dd);
-                double w = P.pw_w[q];
-                dev::TCoef c2{}, c1{};
-                if (m2)
-                    dev::TimeCoef(m, 2, n, dd.R, c2);
-                if (m1)
-                    dev::TimeCoef(m, 1, n, dd.R, c1);
-                double T2[9], T1[9];
-                dev::TwoT(m, dd, P.pw_nor[q], m2, c2, m1, c1, T2, T1);
-                if (m2)
-                    for (int kk = 0; kk < 9; ++kk)
-                        accA[kk] += T2[kk] * w;
-                if (m1) {
-                    for (int kk = 0; kk < 9; ++kk)
-                        accB[kk] += T1[kk] * w;
-                    double U[9];
-                    dev::DynaU(m, n, dd, U);
-                    accU[0] += U[0] * w;
-                    accU[1] += U[1] * w;
-                    accU[2] += U[2] * w;
-                    accU[3] += U[4] * w;
-                    accU[4] += U[5] * w;
-                    accU[5] += U[8] * w;
+
+    // patch-wise path (PWNeedCal per lane and patch; patches staged in batches)
+    if (WaveAny(f2 == 1 || f1 == 1)) {
+        if (lane < kNPW2) {
+            double r = Dist3(fe.rcenpw[lane], src);
+            spr[2 * lane] = r + fe.radiusPW;
+            spr[2 * lane + 1] = r - fe.radiusPW;
+        }
+        __syncthreads();
+        unsigned long long m2 = 0, m1 = 0;
+        {
+            double nn2 = n - 1, nn1 = n;
+            double lo2 = nn2 * m.C2Dt, hi2 = (nn2 + 1) * m.C1Dt;
+            double lo1 = nn1 * m.C2Dt, hi1 = (nn1 + 1) * m.C1Dt;
+            for (int i = 0; i < kNPW2; ++i) {
+                double rpr = spr[2 * i], rmr = spr[2 * i + 1];
+                if (f2 == 1 && !(rmr >= hi2 || rpr < lo2))
+                    m2 |= 1ull << i;
+                if (f1 == 1 && !(rmr >= hi1 || rpr < lo1))
+                    m1 |= 1ull << i;
+            }
+        }
+        unsigned long long need = 0;
+        for (int i = 0; i < kNPW2; ++i)
+            if (WaveAny(((m2 | m1) >> i) & 1ull))
+                need |= 1ull << i;
+        while (need) {
+            int pl[kStagePatches];
+            int np = 0;
+            for (int b = 0; b < kStagePatches; ++b) {
+                pl[b] = need ? __builtin_ctzll(need) : 0;
+                if (need) {
+                    need &= need - 1;
+                    np = b + 1;
                 }
             }
+            if (lane < np * kNPWG2) {
+                int b = lane / kNPWG2;
+                int i = b == 0 ? pl[0] : b == 1 ? pl[1] : b == 2 ? pl[2] : pl[3];
+                int q = i * kNPWG2 + lane % kNPWG2;
+                StagePoint(m, src, P.pw_pos[q], P.pw_nor[q], P.pw_w[q], sgeo + lane * kStageStride);
+            }
+            __syncthreads();
+            for (int b = 0; b < np; ++b) {
+                int i = b == 0 ? pl[0] : b == 1 ? pl[1] : b == 2 ? pl[2] : pl[3];
+                bool u2 = (m2 >> i) & 1ull, u1 = (m1 >> i) & 1ull;
+                for (int j = 0; j < kNPWG2; ++j) {
+                    const double* g = sgeo + (b * kNPWG2 + j) * kStageStride;
+                    LaneCoef c2, cf1;
+                    if (u2)
+                        MakeLaneCoef(m, 2, n, g, c2);
+                    if (u1)
+                        MakeLaneCoef(m, 1, n, g, cf1);
+                    double T2[9], T1[9];
+                    StagedT(g, u2, c2, u1, cf1, T2, T1);
+                    double w = g[sW];
+                    if (u2)
+                        for (int kk = 0; kk < 9; ++kk)
+                            accA[kk] += T2[kk] * w;
+                    double U[6];
+                    StagedU(m, g, u1, cf1, U);
+                    if (u1) {
+                        for (int kk = 0; kk < 9; ++kk)
+                            accB[kk] += T1[kk] * w;
+                        for (int kk = 0; kk < 6; ++kk)
+                            accU[kk] += U[kk] * w;
+                    }
+                }
+            }
+            __syncthreads();
         }
     }
     // regular path: T1(n) + T2(n) at the 16 points with the N0(gap) weight
     bool r2 = f2 == 2, r1 = f1 == 2;
-    if (dev::WaveAny(r2 || r1)) {
+    if (WaveAny(r2 || r1)) {
+        if (lane < kNG2)
+            StagePoint(m, src, P.rg_pos[lane], P.rg_nor[lane], P.rg_w[lane], sgeo + lane * kStageStride);
+        __syncthreads();
+        bool rt = r2 || r1;
         for (int i = 0; i < kNG2; ++i) {
-            dev::Dist dd;
-            dev::MakeDist(src, P.rg_pos[i], dd);
-            double w = P.rg_w[i];
-            bool rt = r2 || r1;
-            dev::TCoef ca{}, cb{};
+            const double* g = sgeo + i * kStageStride;
+            LaneCoef ca, cb;
             if (rt) {
-                dev::TimeCoef(m, 1, n, dd.R, ca);
-                dev::TimeCoef(m, 2, n, dd.R, cb);
+                MakeLaneCoef(m, 1, n, g, ca);
+                MakeLaneCoef(m, 2, n, g, cb);
             }
             double Ta[9], Tb[9];
-            dev::TwoT(m, dd, P.rg_nor[i], rt, ca, rt, cb, Ta, Tb);
-            int flag = (ca.P ? 1 : 0) + (ca.S ? 1 : 0) + (ca.w ? 1 : 0) + (cb.P ? 1 : 0) + (cb.S ? 1 : 0) +
-                       (cb.w ? 1 : 0);
+            StagedT(g, rt, ca, rt, cb, Ta, Tb);
+            double w = g[sW];
+            bool flag = ca.P || ca.S || ca.w || cb.P || cb.S || cb.w;
             if (rt && flag) {
                 for (int kk = 0; kk < 9; ++kk) {
                     double v = (Ta[kk] + Tb[kk]) * w;
@@ -1020,26 +1209,21 @@ __global__ void __launch_bounds__(256) k_history_fused(
                         accB[kk] += v;
                 }
             }
-            if (r1) {
-                double U[9];
-                if (dev::DynaU(m, n, dd, U)) {
-                    accU[0] += U[0] * w;
-                    accU[1] += U[1] * w;
-                    accU[2] += U[2] * w;
-                    accU[3] += U[4] * w;
-                    accU[4] += U[5] * w;
-                    accU[5] += U[8] * w;
-                }
-            }
+            bool fu = r1 && (ca.P || ca.S || ca.w);
+            double U[6];
+            StagedU(m, g, fu, ca, U);
+            if (fu)
+                for (int kk = 0; kk < 6; ++kk)
+                    accU[kk] += U[kk] * w;
         }
     }
     if (!active)
         return;
     long e = off[p] + t;
-    dev::BlockAcc acc;
+    BlockAcc acc;
     double su = refs[f].hu_sign;
-    double blk[9];
     if (self1) {
+        double blk[9];
         for (int i = 0; i < 9; ++i)
             blk[i] = selfT2[9 * (long)f + i];
         acc.Add(su, blk);
@@ -1052,7 +1236,7 @@ __global__ void __launch_bounds__(256) k_history_fused(
     for (int i = 0; i < 9; ++i)
         out[i] = acc.alive ? acc.v[i] : 0.0;
     unsigned char bits = acc.alive ? 1 : 0;
-    dev::BlockAcc accg;
+    BlockAcc accg;
     if (f1) {
         double u9[9] = {accU[0], accU[1], accU[2], accU[1], accU[3], accU[4], accU[2], accU[4], accU[5]};
         accg.Add(refs[f].ht_sign, u9);
@@ -1065,15 +1249,15 @@ __global__ void __launch_bounds__(256) k_history_fused(
     alive[e] = bits;
 }
 
-void LaunchHistoryFused(const ElemGeo* geo, const ElemPoints* pts, const NodeRefs* refs, const double* selfT2,
-                        PairGrid grid, const long* coff, const long* off, const int* kb, const int* len, long npairs,
-                        long c0, long c1, double* mts, double* mgs, unsigned char* alive, hipStream_t s)
+void LaunchHistoryStaged(const ElemGeo* geo, const ElemPoints* pts, const NodeRefs* refs, const double* selfT2,
+                         PairGrid grid, const long* coff, const long* off, const int* kb, const int* len, long npairs,
+                         long c0, long c1, double* mts, double* mgs, unsigned char* alive, hipStream_t s)
 {
     long waves = c1 - c0;
     if (waves <= 0)
         return;
-    k_history_fused<<<(unsigned)((waves + 3) / 4), 256, 0, s>>>(geo, pts, refs, selfT2, grid, coff, off, kb, len,
-                                                                npairs, c0, c1, mts, mgs, alive);
+    k_history_staged<<<(unsigned)waves, 64, 0, s>>>(geo, pts, refs, selfT2, grid, coff, off, kb, len, npairs, c0, c1,
+                                                    mts, mgs, alive);
 }
 
 // ---------------------------------------------------------------------------

@@ -53,11 +53,11 @@ void LaunchHistory(const ElemGeo* geo, const ElemPoints* pts, const int* dom, co
                    const double* selfT2, PairGrid grid, const long* off, const int* kb, long npairs, long e0, long e1,
                    double* mts, double* mgs, unsigned char* alive, hipStream_t s);
 
-// Fused history quadrature: one wavefront per (pair, 64-lag chunk); chunk
+// Staged history quadrature: one wavefront per (pair, 64-lag chunk); chunk
 // offsets coff[p] (npairs+1 entries), chunks [c0, c1).
-void LaunchHistoryFused(const ElemGeo* geo, const ElemPoints* pts, const NodeRefs* refs, const double* selfT2,
-                        PairGrid grid, const long* coff, const long* off, const int* kb, const int* len, long npairs,
-                        long c0, long c1, double* mts, double* mgs, unsigned char* alive, hipStream_t s);
+void LaunchHistoryStaged(const ElemGeo* geo, const ElemPoints* pts, const NodeRefs* refs, const double* selfT2,
+                         PairGrid grid, const long* coff, const long* off, const int* kb, const int* len, long npairs,
+                         long c0, long c1, double* mts, double* mgs, unsigned char* alive, hipStream_t s);
 
 // Step-0 blocks of the near pairs listed in pairIdx (column-major, ready for
 // AddBlock): out[2*i] and out[2*i+1] with flags.

@@ -393,11 +393,11 @@ void Case::AssembleHistory(int maxn_in)
     Check(hipEventCreate(&a), "event");
     Check(hipEventCreate(&b), "event");
     Check(hipEventRecord(a, stream), "event");
-    if (quadFused) {
-        const long step = 1L << 18;  // wavefronts per launch
+    if (quadStaged) {
+        const long step = 1L << 20;  // wavefronts per launch
         for (long c0 = 0; c0 < chunks; c0 += step)
-            LaunchHistoryFused(d_geo.p, d_pts.p, d_refs.p, d_t2.p, grid, d_coff.p, d_off.p, d_kb.p, d_len.p, npairs,
-                               c0, std::min(chunks, c0 + step), d_mts.p, d_mgs.p, d_alive.p, stream);
+            LaunchHistoryStaged(d_geo.p, d_pts.p, d_refs.p, d_t2.p, grid, d_coff.p, d_off.p, d_kb.p, d_len.p, npairs,
+                                c0, std::min(chunks, c0 + step), d_mts.p, d_mgs.p, d_alive.p, stream);
     } else {
         const long chunk = 1L << 24;
         for (long e0 = 0; e0 < entries; e0 += chunk) {

@@ -51,7 +51,7 @@ public:
     LeafPreconditioner pre;
     double timing[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     bool convTiled = true;   // time-blocked convolution (false: one full pass per step)
-    bool quadFused = false;  // fused wave-per-pair history quadrature (false: lane per entry)
+    bool quadStaged = true;  // wave-per-pair quadrature with LDS-staged point geometry (false: lane per entry)
     hipStream_t stream = nullptr;
 
     DevBuf<double> d_u1, d_t1, d_t2;

@@ -204,8 +204,8 @@ def test_tiled_convolution_matches_per_step_sum(oracle_mod):
 
 
 @pytest.mark.parametrize("which", ["regular_path_probe", "layered_small"])
-def test_fused_quadrature_matches_per_entry(oracle_mod, which):
-    """The fused wave-per-pair history quadrature and the lane-per-entry
+def test_staged_quadrature_matches_per_entry(oracle_mod, which):
+    """The staged wave-per-pair history quadrature and the lane-per-entry
     quadrature give the same MTS/MGS bands bit for bit, and both equal the
     oracle's blocks."""
     from tdbem import mesh
@@ -216,13 +216,13 @@ def test_fused_quadrature_matches_per_entry(oracle_mod, which):
         w = mesh.Workload("regular_path_probe", m, mats, 0.02, 30, known)
     else:
         w = mesh.layered_config2(h=0.5, nstep=40)
-    print("case:", w.name, "fused vs per-entry quadrature", flush=True)
+    print("case:", w.name, "staged vs per-entry quadrature", flush=True)
     oc, gc = _workload_cases(oracle_mod, w)
     gc.assemble_step0(10)
-    gc.set_option("quad_fused", 0)
+    gc.set_option("quad_staged", 0)
     gc.assemble_history(w.nstep)
     ref = [(gc.ccsr(2, k), gc.ccsr(3, k)) for k in range(1, w.nstep + 1)]
-    gc.set_option("quad_fused", 1)
+    gc.set_option("quad_staged", 1)
     gc.assemble_history(w.nstep)
     oc.assemble(w.nstep)
     for k in range(1, w.nstep + 1):

@@ -925,7 +925,7 @@ __device__ void StagePoint(const DynaMat& m, const double* src, const double* po
 struct LaneCoef {
     bool P = false, S = false;
     int w = 0;
-    double p1 = 0.0, s1 = 0.0, w1 = 0.0, ri = 0.0, sg = 1.0;
+    double p1 = 0.0, s1 = 0.0, w1 = 0.0, ri = 0.0;
 };
 
 __device__ __forceinline__ double ISerS(int w, double n, double ndt, double np1dt, const double* g, const DynaMat& m)
@@ -975,11 +975,9 @@ __device__ __forceinline__ void MakeLaneCoef(const DynaMat& m, int type, double 
     if (type == 1) {
         c.p1 = kOneOver4Pi * (n + 1.0 - g[sRC1Dt]);
         c.s1 = kOneOver4Pi * (n + 1.0 - g[sRC2Dt]);
-        c.sg = 1.0;
     } else {
         c.p1 = kOneOver4Pi * (1.0 - n + g[sRC1Dt]);
         c.s1 = kOneOver4Pi * (1.0 - n + g[sRC2Dt]);
-        c.sg = -1.0;
     }
     c.w1 = 0.0;
     c.ri = 0.0;
@@ -996,9 +994,11 @@ __device__ __forceinline__ void MakeLaneCoef(const DynaMat& m, int type, double 
 
 __device__ __forceinline__ bool WaveAny(bool v) { return __ballot(v) != 0; }
 
-// Kernel values Ta (coefficients ca, if ua) and Tb (cb, if ub) at one staged
-// point, each 0 + P term + S term + window term.  The G and H products with
-// +-1/(4 pi dt) are staged for type 1; type 2's are their exact negations.
+// Kernel values Ta (type TA, coefficients ca, if ua) and Tb (type TB, cb, if
+// ub) at one staged point, each 0 + P term + S term + window term.  The G and
+// H products with 1/(4 pi dt) are staged; the type-2 factor -1/(4 pi dt)
+// gives their exact negations, so type 2 subtracts them.
+template <int TA, int TB>
 __device__ __forceinline__ void StagedT(const double* g, bool ua, const LaneCoef& ca, bool ub, const LaneCoef& cb,
                                         double* Ta, double* Tb)
 {
@@ -1011,9 +1011,9 @@ __device__ __forceinline__ void StagedT(const double* g, bool ua, const LaneCoef
         for (int k = 0; k < 9; ++k) {
             double dk = g[sD + k], gk = g[sGt + k];
             if (pa)
-                Ta[k] += dk * ca.p1 + ca.sg * gk;
+                Ta[k] += TA == 1 ? dk * ca.p1 + gk : dk * ca.p1 - gk;
             if (pb)
-                Tb[k] += dk * cb.p1 + cb.sg * gk;
+                Tb[k] += TB == 1 ? dk * cb.p1 + gk : dk * cb.p1 - gk;
         }
     }
     bool sa = ua && ca.S, sb = ub && cb.S;
@@ -1021,9 +1021,9 @@ __device__ __forceinline__ void StagedT(const double* g, bool ua, const LaneCoef
         for (int k = 0; k < 9; ++k) {
             double ek = g[sE + k], hk = g[sHt + k];
             if (sa)
-                Ta[k] += ek * ca.s1 + ca.sg * hk;
+                Ta[k] += TA == 1 ? ek * ca.s1 + hk : ek * ca.s1 - hk;
             if (sb)
-                Tb[k] += ek * cb.s1 + cb.sg * hk;
+                Tb[k] += TB == 1 ? ek * cb.s1 + hk : ek * cb.s1 - hk;
         }
     }
     bool wa = ua && ca.w, wb = ub && cb.w;
@@ -1066,18 +1066,183 @@ __device__ __forceinline__ void StagedU(const DynaMat& m, const double* g, bool 
 
 }  // namespace dev
 
-constexpr int kStagePatches = 4;  // patches staged per batch (36 points)
+namespace dev {
 
-__global__ void __launch_bounds__(64) k_history_staged(
+// NeedCal with the centre distance r already formed (same operations).
+__device__ __forceinline__ int NeedCalR(const DynaMat& m, double radius, double r, int type, double n)
+{
+    double nn = (type == 2) ? n - 1 : n;
+    double ndt = nn * m.Dt;
+    double np1dt = (nn + 1) * m.Dt;
+    double Rmax = r + radius;
+    double Rmin = r - radius;
+    if (Rmin < 0.0)
+        Rmin = 0.0;
+    double a1 = Rmax / m.C1, a2 = Rmax / m.C2, b1 = Rmin / m.C1, b2 = Rmin / m.C2;
+    if (ndt >= a2 || np1dt <= b1)
+        return 0;
+    if (ndt <= b1 && (np1dt >= a1 && np1dt <= b2))
+        return 2;
+    if (ndt <= b1 && np1dt >= a2)
+        return 2;
+    if (ndt >= a1 && np1dt <= b2)
+        return 2;
+    if ((ndt >= a1 && ndt <= b2) && np1dt >= a2)
+        return 2;
+    return 1;
+}
+
+// Judges of one band entry: f2 (T2 at level k - 1, unless the self table
+// replaces it), f1 (T1 and U at level k).
+struct EntryFlags {
+    bool self1;
+    int f2, f1;
+    __device__ bool patchwise() const { return f2 == 1 || f1 == 1; }
+};
+
+__device__ __forceinline__ EntryFlags Judge(const DynaMat& m, double radius, double r, int k, bool same)
+{
+    EntryFlags e;
+    double n = (double)k;
+    e.self1 = k <= 1 && same;
+    e.f2 = e.self1 ? 0 : NeedCalR(m, radius, r, 2, n);
+    e.f1 = NeedCalR(m, radius, r, 1, n);
+    return e;
+}
+
+// Regular-path contribution of one staged point to the lane's accumulators.
+__device__ __forceinline__ void ConsumeRegular(const DynaMat& m, const double* g, double n, bool r2, bool r1,
+                                               double* accA, double* accB, double* accU)
+{
+    bool rt = r2 || r1;
+    LaneCoef ca, cb;
+    if (rt) {
+        MakeLaneCoef(m, 1, n, g, ca);
+        MakeLaneCoef(m, 2, n, g, cb);
+    }
+    double Ta[9], Tb[9];
+    StagedT<1, 2>(g, rt, ca, rt, cb, Ta, Tb);
+    double w = g[sW];
+    bool flag = ca.P || ca.S || ca.w || cb.P || cb.S || cb.w;
+    if (rt && flag) {
+        for (int kk = 0; kk < 9; ++kk) {
+            double v = (Ta[kk] + Tb[kk]) * w;
+            if (r2)
+                accA[kk] += v;
+            if (r1)
+                accB[kk] += v;
+        }
+    }
+    bool fu = r1 && (ca.P || ca.S || ca.w);
+    double U[6];
+    StagedU(m, g, fu, ca, U);
+    if (fu)
+        for (int kk = 0; kk < 6; ++kk)
+            accU[kk] += U[kk] * w;
+}
+
+// Patch-wise contribution of one staged point (patch flags u2 for T2, u1 for
+// T1 and U).
+__device__ __forceinline__ void ConsumePatchwise(const DynaMat& m, const double* g, double n, bool u2, bool u1,
+                                                 double* accA, double* accB, double* accU)
+{
+    LaneCoef c2, c1;
+    if (u2)
+        MakeLaneCoef(m, 2, n, g, c2);
+    if (u1)
+        MakeLaneCoef(m, 1, n, g, c1);
+    double T2[9], T1[9];
+    StagedT<2, 1>(g, u2, c2, u1, c1, T2, T1);
+    double w = g[sW];
+    if (u2)
+        for (int kk = 0; kk < 9; ++kk)
+            accA[kk] += T2[kk] * w;
+    double U[6];
+    StagedU(m, g, u1, c1, U);
+    if (u1) {
+        for (int kk = 0; kk < 9; ++kk)
+            accB[kk] += T1[kk] * w;
+        for (int kk = 0; kk < 6; ++kk)
+            accU[kk] += U[kk] * w;
+    }
+}
+
+// MTS entry = AddBlock(T2 or the self table) then AddBlock(T1); MGS entry =
+// AddBlock(U) (DBEM1/multidomain.cpp:2017-2078).
+__device__ __forceinline__ void FinalizeEntry(const EntryFlags& fl, const double* selfT2f, double hu_sign,
+                                              double ht_sign, const double* accA, const double* accB,
+                                              const double* accU, long e, double* mts, double* mgs,
+                                              unsigned char* alive)
+{
+    BlockAcc acc;
+    if (fl.self1) {
+        double blk[9];
+        for (int i = 0; i < 9; ++i)
+            blk[i] = selfT2f[i];
+        acc.Add(hu_sign, blk);
+    } else if (fl.f2) {
+        acc.Add(hu_sign, accA);
+    }
+    if (fl.f1)
+        acc.Add(hu_sign, accB);
+    double* out = mts + 9 * e;
+    for (int i = 0; i < 9; ++i)
+        out[i] = acc.alive ? acc.v[i] : 0.0;
+    unsigned char bits = acc.alive ? 1 : 0;
+    BlockAcc accg;
+    if (fl.f1) {
+        double u9[9] = {accU[0], accU[1], accU[2], accU[1], accU[3], accU[4], accU[2], accU[4], accU[5]};
+        accg.Add(ht_sign, u9);
+    }
+    double* og = mgs + 6 * e;
+    const int pick[6] = {0, 1, 2, 4, 5, 8};
+    for (int i = 0; i < 6; ++i)
+        og[i] = accg.alive ? accg.v[pick[i]] : 0.0;
+    bits |= accg.alive ? 2 : 0;
+    alive[e] = bits;
+}
+
+}  // namespace dev
+
+// Number of patch-wise entries (f2 == 1 or f1 == 1) in each pair's band.
+__global__ void __launch_bounds__(256) k_pwcount(const ElemGeo* __restrict__ geo, PairGrid grid,
+                                                 const int* __restrict__ kb, const int* __restrict__ len, long npairs,
+                                                 int* __restrict__ cnt)
+{
+    long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npairs)
+        return;
+    int s, f, d;
+    dev::DecodePair(grid, p, s, f, d);
+    const DynaMat& m = c_mats[d];
+    double r = Dist3(geo[f].center, geo[s].center);
+    double radius = geo[f].radius;
+    int c = 0, k0 = kb[p], L = len[p];
+    for (int t = 0; t < L; ++t)
+        c += dev::Judge(m, radius, r, k0 + t, s == f).patchwise() ? 1 : 0;
+    cnt[p] = c;
+}
+
+void LaunchPwCount(const ElemGeo* geo, PairGrid grid, const int* kb, const int* len, long npairs, int* cnt,
+                   hipStream_t s)
+{
+    if (npairs <= 0)
+        return;
+    k_pwcount<<<(unsigned)((npairs + 255) / 256), 256, 0, s>>>(geo, grid, kb, len, npairs, cnt);
+}
+
+// Regular-path entries: one wavefront per (pair, chunk of up to 64 lags), one
+// lag per lane; the lanes of patch-wise entries are left to k_history_pw.
+// The 16 regular points are staged once per wave.
+__global__ void __launch_bounds__(64) k_history_reg(
     const ElemGeo* __restrict__ geo, const ElemPoints* __restrict__ pts, const NodeRefs* __restrict__ refs,
     const double* __restrict__ selfT2, PairGrid grid, const long* __restrict__ coff, const long* __restrict__ off,
     const int* __restrict__ kb, const int* __restrict__ len, long npairs, long c0, long c1, double* __restrict__ mts,
     double* __restrict__ mgs, unsigned char* __restrict__ alive)
 {
-    __shared__ double sgeo[kStagePatches * kNPWG2 * dev::kStageStride];
-    __shared__ double spr[2 * kNPW2];
+    __shared__ double sgeo[kNG2 * dev::kStageStride];
     using namespace dev;
-    long wv = c0 + (long)blockIdx.x;  // one wavefront per block
+    long wv = c0 + (long)blockIdx.x;
     if (wv >= c1)
         return;
     int lane = threadIdx.x;
@@ -1093,16 +1258,14 @@ __global__ void __launch_bounds__(64) k_history_staged(
     int s, f, d;
     DecodePair(grid, p, s, f, d);
     int t = (int)(wv - coff[p]) * 64 + lane;
-    bool active = t < len[p];
-    int k = kb[p] + t;
     const DynaMat& m = c_mats[d];
     const ElemGeo& fe = geo[f];
     const ElemPoints& P = pts[f];
     const double* src = geo[s].center;
+    int k = kb[p] + t;
     double n = (double)k;
-    bool self1 = active && k <= 1 && s == f;
-    int f2 = (active && !self1) ? NeedCal(m, fe, 2, n, src) : 0;
-    int f1 = active ? NeedCal(m, fe, 1, n, src) : 0;
+    EntryFlags fl = Judge(m, fe.radius, Dist3(fe.center, src), k, s == f);
+    bool mine = t < len[p] && !fl.patchwise();
     double accA[9], accB[9], accU[6];
     for (int i = 0; i < 9; ++i) {
         accA[i] = 0.0;
@@ -1110,154 +1273,184 @@ __global__ void __launch_bounds__(64) k_history_staged(
     }
     for (int i = 0; i < 6; ++i)
         accU[i] = 0.0;
-
-    // patch-wise path (PWNeedCal per lane and patch; patches staged in batches)
-    if (WaveAny(f2 == 1 || f1 == 1)) {
-        if (lane < kNPW2) {
-            double r = Dist3(fe.rcenpw[lane], src);
-            spr[2 * lane] = r + fe.radiusPW;
-            spr[2 * lane + 1] = r - fe.radiusPW;
-        }
-        __syncthreads();
-        unsigned long long m2 = 0, m1 = 0;
-        {
-            double nn2 = n - 1, nn1 = n;
-            double lo2 = nn2 * m.C2Dt, hi2 = (nn2 + 1) * m.C1Dt;
-            double lo1 = nn1 * m.C2Dt, hi1 = (nn1 + 1) * m.C1Dt;
-            for (int i = 0; i < kNPW2; ++i) {
-                double rpr = spr[2 * i], rmr = spr[2 * i + 1];
-                if (f2 == 1 && !(rmr >= hi2 || rpr < lo2))
-                    m2 |= 1ull << i;
-                if (f1 == 1 && !(rmr >= hi1 || rpr < lo1))
-                    m1 |= 1ull << i;
-            }
-        }
-        unsigned long long need = 0;
-        for (int i = 0; i < kNPW2; ++i)
-            if (WaveAny(((m2 | m1) >> i) & 1ull))
-                need |= 1ull << i;
-        while (need) {
-            int pl[kStagePatches];
-            int np = 0;
-            for (int b = 0; b < kStagePatches; ++b) {
-                pl[b] = need ? __builtin_ctzll(need) : 0;
-                if (need) {
-                    need &= need - 1;
-                    np = b + 1;
-                }
-            }
-            if (lane < np * kNPWG2) {
-                int b = lane / kNPWG2;
-                int i = b == 0 ? pl[0] : b == 1 ? pl[1] : b == 2 ? pl[2] : pl[3];
-                int q = i * kNPWG2 + lane % kNPWG2;
-                StagePoint(m, src, P.pw_pos[q], P.pw_nor[q], P.pw_w[q], sgeo + lane * kStageStride);
-            }
-            __syncthreads();
-            for (int b = 0; b < np; ++b) {
-                int i = b == 0 ? pl[0] : b == 1 ? pl[1] : b == 2 ? pl[2] : pl[3];
-                bool u2 = (m2 >> i) & 1ull, u1 = (m1 >> i) & 1ull;
-                for (int j = 0; j < kNPWG2; ++j) {
-                    const double* g = sgeo + (b * kNPWG2 + j) * kStageStride;
-                    LaneCoef c2, cf1;
-                    if (u2)
-                        MakeLaneCoef(m, 2, n, g, c2);
-                    if (u1)
-                        MakeLaneCoef(m, 1, n, g, cf1);
-                    double T2[9], T1[9];
-                    StagedT(g, u2, c2, u1, cf1, T2, T1);
-                    double w = g[sW];
-                    if (u2)
-                        for (int kk = 0; kk < 9; ++kk)
-                            accA[kk] += T2[kk] * w;
-                    double U[6];
-                    StagedU(m, g, u1, cf1, U);
-                    if (u1) {
-                        for (int kk = 0; kk < 9; ++kk)
-                            accB[kk] += T1[kk] * w;
-                        for (int kk = 0; kk < 6; ++kk)
-                            accU[kk] += U[kk] * w;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-    }
-    // regular path: T1(n) + T2(n) at the 16 points with the N0(gap) weight
-    bool r2 = f2 == 2, r1 = f1 == 2;
+    bool r2 = mine && fl.f2 == 2, r1 = mine && fl.f1 == 2;
     if (WaveAny(r2 || r1)) {
         if (lane < kNG2)
             StagePoint(m, src, P.rg_pos[lane], P.rg_nor[lane], P.rg_w[lane], sgeo + lane * kStageStride);
         __syncthreads();
-        bool rt = r2 || r1;
-        for (int i = 0; i < kNG2; ++i) {
-            const double* g = sgeo + i * kStageStride;
-            LaneCoef ca, cb;
-            if (rt) {
-                MakeLaneCoef(m, 1, n, g, ca);
-                MakeLaneCoef(m, 2, n, g, cb);
-            }
-            double Ta[9], Tb[9];
-            StagedT(g, rt, ca, rt, cb, Ta, Tb);
-            double w = g[sW];
-            bool flag = ca.P || ca.S || ca.w || cb.P || cb.S || cb.w;
-            if (rt && flag) {
-                for (int kk = 0; kk < 9; ++kk) {
-                    double v = (Ta[kk] + Tb[kk]) * w;
-                    if (r2)
-                        accA[kk] += v;
-                    if (r1)
-                        accB[kk] += v;
+        for (int i = 0; i < kNG2; ++i)
+            ConsumeRegular(m, sgeo + i * kStageStride, n, r2, r1, accA, accB, accU);
+    }
+    if (!mine)
+        return;
+    FinalizeEntry(fl, selfT2 + 9 * (long)f, refs[f].hu_sign, refs[f].ht_sign, accA, accB, accU, off[p] + t, mts, mgs,
+                  alive);
+}
+
+// Patch-wise entries: one wavefront per PwWave, whose lanes are the
+// patch-wise lags of up to kPwSlots pairs sharing one field element (so the
+// patch points are shared); the point terms of every (source, point) are
+// staged once per patch.  Regular parts of these entries (the other judge
+// picking the regular path) are added here too, so each entry is finished by
+// exactly one kernel.
+__global__ void __launch_bounds__(64) k_history_pw(
+    const ElemGeo* __restrict__ geo, const ElemPoints* __restrict__ pts, const NodeRefs* __restrict__ refs,
+    const double* __restrict__ selfT2, PairGrid grid, const PwWave* __restrict__ waves, long w0, long w1,
+    const long* __restrict__ off, const int* __restrict__ kb, const int* __restrict__ len, double* __restrict__ mts,
+    double* __restrict__ mgs, unsigned char* __restrict__ alive)
+{
+    __shared__ double sgeo[kPwSlots * kNPWG2 * dev::kStageStride];
+    __shared__ double spr[kPwSlots * 2 * kNPW2];
+    __shared__ int ssrc[kPwSlots];
+    using namespace dev;
+    long wi = w0 + (long)blockIdx.x;
+    if (wi >= w1)
+        return;
+    int lane = threadIdx.x;
+    const PwWave& W = waves[wi];
+    int nseg = W.nseg;
+    // field element and domain (the same for every slot)
+    int s0, f, d;
+    DecodePair(grid, W.pair[0], s0, f, d);
+    const DynaMat& m = c_mats[d];
+    const ElemGeo& fe = geo[f];
+    const ElemPoints& P = pts[f];
+    if (lane < nseg) {
+        int ss, ff, dd;
+        DecodePair(grid, W.pair[lane], ss, ff, dd);
+        ssrc[lane] = ss;
+    }
+    // lane -> (slot, rank among the pair's patch-wise lags)
+    int seg = -1, rank = 0, base = 0;
+    for (int i = 0; i < kPwSlots; ++i) {
+        int c = i < nseg ? W.cnt[i] : 0;
+        if (seg < 0 && lane >= base && lane < base + c) {
+            seg = i;
+            rank = W.rank0[i] + (lane - base);
+        }
+        base += c;
+    }
+    __syncthreads();
+    // patch radii of every slot's source
+    for (int idx = lane; idx < nseg * kNPW2; idx += 64) {
+        int slot = idx / kNPW2, i = idx - slot * kNPW2;
+        double r = Dist3(fe.rcenpw[i], geo[ssrc[slot]].center);
+        spr[slot * 2 * kNPW2 + 2 * i] = r + fe.radiusPW;
+        spr[slot * 2 * kNPW2 + 2 * i + 1] = r - fe.radiusPW;
+    }
+    bool active = seg >= 0;
+    int myseg = active ? seg : 0;
+    long p = W.pair[myseg];
+    int s = ssrc[myseg];
+    const double* src = geo[s].center;
+    double r0 = Dist3(fe.center, src);
+    int t = -1, k = 0;
+    EntryFlags fl{false, 0, 0};
+    if (active) {
+        int k0 = kb[p], L = len[p], c = 0;
+        for (int tt = 0; tt < L; ++tt) {
+            EntryFlags e = Judge(m, fe.radius, r0, k0 + tt, s == f);
+            if (e.patchwise()) {
+                if (c == rank) {
+                    t = tt;
+                    fl = e;
+                    break;
                 }
+                ++c;
             }
-            bool fu = r1 && (ca.P || ca.S || ca.w);
-            double U[6];
-            StagedU(m, g, fu, ca, U);
-            if (fu)
-                for (int kk = 0; kk < 6; ++kk)
-                    accU[kk] += U[kk] * w;
+        }
+        k = k0 + t;
+        if (t < 0)
+            active = false;
+    }
+    double n = (double)k;
+    __syncthreads();
+    unsigned long long m2 = 0, m1 = 0;
+    {
+        double nn2 = n - 1, nn1 = n;
+        double lo2 = nn2 * m.C2Dt, hi2 = (nn2 + 1) * m.C1Dt;
+        double lo1 = nn1 * m.C2Dt, hi1 = (nn1 + 1) * m.C1Dt;
+        bool pw2 = active && fl.f2 == 1, pw1 = active && fl.f1 == 1;
+        const double* sp = spr + myseg * 2 * kNPW2;
+        for (int i = 0; i < kNPW2; ++i) {
+            double rpr = sp[2 * i], rmr = sp[2 * i + 1];
+            if (pw2 && !(rmr >= hi2 || rpr < lo2))
+                m2 |= 1ull << i;
+            if (pw1 && !(rmr >= hi1 || rpr < lo1))
+                m1 |= 1ull << i;
+        }
+    }
+    unsigned long long need = 0;
+    for (int i = 0; i < kNPW2; ++i)
+        if (WaveAny(((m2 | m1) >> i) & 1ull))
+            need |= 1ull << i;
+    double accA[9], accB[9], accU[6];
+    for (int i = 0; i < 9; ++i) {
+        accA[i] = 0.0;
+        accB[i] = 0.0;
+    }
+    for (int i = 0; i < 6; ++i)
+        accU[i] = 0.0;
+    // one patch at a time, all slots' sources staged together
+    int stSlot = lane / kNPWG2, stJ = lane - stSlot * kNPWG2;
+    while (need) {
+        int i = __builtin_ctzll(need);
+        need &= need - 1;
+        if (stSlot < nseg) {
+            int q = i * kNPWG2 + stJ;
+            StagePoint(m, geo[ssrc[stSlot]].center, P.pw_pos[q], P.pw_nor[q], P.pw_w[q],
+                       sgeo + lane * kStageStride);
+        }
+        __syncthreads();
+        bool u2 = (m2 >> i) & 1ull, u1 = (m1 >> i) & 1ull;
+        const double* gs = sgeo + myseg * kNPWG2 * kStageStride;
+        for (int j = 0; j < kNPWG2; ++j)
+            ConsumePatchwise(m, gs + j * kStageStride, n, u2, u1, accA, accB, accU);
+        __syncthreads();
+    }
+    // regular parts (the other judge picking the regular path), 8 points per half
+    bool r2 = active && fl.f2 == 2, r1 = active && fl.f1 == 2;
+    if (WaveAny(r2 || r1)) {
+        int rgSlot = lane >> 3, rgJ = lane & 7;
+        for (int half = 0; half < 2; ++half) {
+            if (rgSlot < nseg) {
+                int q = half * 8 + rgJ;
+                StagePoint(m, geo[ssrc[rgSlot]].center, P.rg_pos[q], P.rg_nor[q], P.rg_w[q],
+                           sgeo + lane * kStageStride);
+            }
+            __syncthreads();
+            const double* gs = sgeo + myseg * 8 * kStageStride;
+            for (int j = 0; j < 8; ++j)
+                ConsumeRegular(m, gs + j * kStageStride, n, r2, r1, accA, accB, accU);
+            __syncthreads();
         }
     }
     if (!active)
         return;
-    long e = off[p] + t;
-    BlockAcc acc;
-    double su = refs[f].hu_sign;
-    if (self1) {
-        double blk[9];
-        for (int i = 0; i < 9; ++i)
-            blk[i] = selfT2[9 * (long)f + i];
-        acc.Add(su, blk);
-    } else if (f2) {
-        acc.Add(su, accA);
-    }
-    if (f1)
-        acc.Add(su, accB);
-    double* out = mts + 9 * e;
-    for (int i = 0; i < 9; ++i)
-        out[i] = acc.alive ? acc.v[i] : 0.0;
-    unsigned char bits = acc.alive ? 1 : 0;
-    BlockAcc accg;
-    if (f1) {
-        double u9[9] = {accU[0], accU[1], accU[2], accU[1], accU[3], accU[4], accU[2], accU[4], accU[5]};
-        accg.Add(refs[f].ht_sign, u9);
-    }
-    double* og = mgs + 6 * e;
-    const int pick[6] = {0, 1, 2, 4, 5, 8};
-    for (int i = 0; i < 6; ++i)
-        og[i] = accg.alive ? accg.v[pick[i]] : 0.0;
-    bits |= accg.alive ? 2 : 0;
-    alive[e] = bits;
+    FinalizeEntry(fl, selfT2 + 9 * (long)f, refs[f].hu_sign, refs[f].ht_sign, accA, accB, accU, off[p] + t, mts, mgs,
+                  alive);
 }
 
-void LaunchHistoryStaged(const ElemGeo* geo, const ElemPoints* pts, const NodeRefs* refs, const double* selfT2,
-                         PairGrid grid, const long* coff, const long* off, const int* kb, const int* len, long npairs,
-                         long c0, long c1, double* mts, double* mgs, unsigned char* alive, hipStream_t s)
+void LaunchHistoryReg(const ElemGeo* geo, const ElemPoints* pts, const NodeRefs* refs, const double* selfT2,
+                      PairGrid grid, const long* coff, const long* off, const int* kb, const int* len, long npairs,
+                      long c0, long c1, double* mts, double* mgs, unsigned char* alive, hipStream_t s)
 {
     long waves = c1 - c0;
     if (waves <= 0)
         return;
-    k_history_staged<<<(unsigned)waves, 64, 0, s>>>(geo, pts, refs, selfT2, grid, coff, off, kb, len, npairs, c0, c1,
-                                                    mts, mgs, alive);
+    k_history_reg<<<(unsigned)waves, 64, 0, s>>>(geo, pts, refs, selfT2, grid, coff, off, kb, len, npairs, c0, c1, mts,
+                                                 mgs, alive);
+}
+
+void LaunchHistoryPw(const ElemGeo* geo, const ElemPoints* pts, const NodeRefs* refs, const double* selfT2,
+                     PairGrid grid, const PwWave* waves, long w0, long w1, const long* off, const int* kb,
+                     const int* len, double* mts, double* mgs, unsigned char* alive, hipStream_t s)
+{
+    long n = w1 - w0;
+    if (n <= 0)
+        return;
+    k_history_pw<<<(unsigned)n, 64, 0, s>>>(geo, pts, refs, selfT2, grid, waves, w0, w1, off, kb, len, mts, mgs,
+                                            alive);
 }
 
 // ---------------------------------------------------------------------------

@@ -63,6 +63,7 @@ template struct DevBuf<ElemGeo>;
 template struct DevBuf<ElemPoints>;
 template struct DevBuf<NodeRefs>;
 template struct DevBuf<QuadTables>;
+template struct DevBuf<PwWave>;
 
 Case::~Case()
 {
@@ -394,10 +395,57 @@ void Case::AssembleHistory(int maxn_in)
     Check(hipEventCreate(&b), "event");
     Check(hipEventRecord(a, stream), "event");
     if (quadStaged) {
+        // patch-wise lags per pair, then waves that pack the patch-wise lags
+        // of consecutive sources with the same field element
+        DevBuf<int> d_pwcnt;
+        d_pwcnt.alloc((size_t)npairs);
+        LaunchPwCount(d_geo.p, grid, d_kb.p, d_len.p, npairs, d_pwcnt.p, stream);
+        std::vector<int> pwcnt((size_t)npairs);
+        Check(hipMemcpyAsync(pwcnt.data(), d_pwcnt.p, sizeof(int) * (size_t)npairs, hipMemcpyDeviceToHost, stream),
+              "pw counts");
+        Check(hipStreamSynchronize(stream), "pw counts");
+        std::vector<PwWave> waves;
+        PwWave cur{};
+        int lanes = 0;
+        auto flush = [&]() {
+            if (cur.nseg > 0)
+                waves.push_back(cur);
+            cur = PwWave{};
+            lanes = 0;
+        };
+        for (int d = 0; d < grid.nd; ++d) {
+            long nd = grid.eleCount[d];
+            for (long fl = 0; fl < nd; ++fl) {
+                for (long sl = 0; sl < nd; ++sl) {
+                    long p = grid.pairBase[d] + sl * nd + fl;
+                    int c = pwcnt[(size_t)p], r0 = 0;
+                    while (c > 0) {
+                        if (cur.nseg == kPwSlots || lanes == 64)
+                            flush();
+                        int take = std::min(c, 64 - lanes);
+                        cur.pair[cur.nseg] = p;
+                        cur.rank0[cur.nseg] = r0;
+                        cur.cnt[cur.nseg] = take;
+                        cur.nseg++;
+                        lanes += take;
+                        c -= take;
+                        r0 += take;
+                    }
+                }
+                flush();
+            }
+        }
+        pwWaves = (long)waves.size();
+        DevBuf<PwWave> d_waves;
+        d_waves.upload(waves.data(), waves.size(), stream);
         const long step = 1L << 20;  // wavefronts per launch
         for (long c0 = 0; c0 < chunks; c0 += step)
-            LaunchHistoryStaged(d_geo.p, d_pts.p, d_refs.p, d_t2.p, grid, d_coff.p, d_off.p, d_kb.p, d_len.p, npairs,
-                                c0, std::min(chunks, c0 + step), d_mts.p, d_mgs.p, d_alive.p, stream);
+            LaunchHistoryReg(d_geo.p, d_pts.p, d_refs.p, d_t2.p, grid, d_coff.p, d_off.p, d_kb.p, d_len.p, npairs, c0,
+                             std::min(chunks, c0 + step), d_mts.p, d_mgs.p, d_alive.p, stream);
+        for (long w0 = 0; w0 < pwWaves; w0 += step)
+            LaunchHistoryPw(d_geo.p, d_pts.p, d_refs.p, d_t2.p, grid, d_waves.p, w0, std::min(pwWaves, w0 + step),
+                            d_off.p, d_kb.p, d_len.p, d_mts.p, d_mgs.p, d_alive.p, stream);
+        Check(hipStreamSynchronize(stream), "history");
     } else {
         const long chunk = 1L << 24;
         for (long e0 = 0; e0 < entries; e0 += chunk) {

@@ -73,6 +73,7 @@ private:
     DevBuf<unsigned char> d_near;
     DevBuf<long> d_off, d_coff;
     long chunks = 0;
+    long pwWaves = 0;  // patch-wise waves of the last history assembly
     DevBuf<long> d_nearPtr, d_nearPair;
     DevBuf<int> d_nearField;
     std::vector<int> h_kb, h_len;

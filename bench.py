@@ -44,9 +44,10 @@ def make_case(tdbem, w):
                       itf_rows=m.interfaces)
 
 
-def run_once(tdbem, w, restart=200, tol=1e-5, keep=False):
+def run_once(tdbem, w, restart=200, tol=1e-5, keep=False, tile=16):
     t0 = time.perf_counter()
     c = make_case(tdbem, w)
+    c.set_option("conv_tile", tile)
     c.assemble_step0(10)
     entries = c.assemble_history(w.nstep)
     U, T, iters, flags, solved, times = c.solve(w.known, w.nstep, restart=restart, tol=tol)
@@ -61,20 +62,17 @@ def run_once(tdbem, w, restart=200, tol=1e-5, keep=False):
     return info
 
 
-TILE = 8  # time steps per far-part launch (kConvTile in csrc/kernels.hpp)
-
-
-def conv_far_bytes_per_launch(counts, npairs, ne, nhist, nstep):
+def conv_far_bytes_per_launch(counts, npairs, ne, nhist, nstep, tile):
     """Algorithmic HBM bytes of one far-part convolution launch (tile of up to
-    TILE steps starting at K0 = 2, 2+TILE, ...), averaged over launches: each
+    `tile` steps starting at K0 = 2, 2+tile, ...), averaged over launches: each
     band entry with lag j <= K0+Tn-1 read once (9 MTS + 6 MGS doubles), per-pair
     band metadata (kb, len, offset: 16 B), the solved state time series read
     once (two histories, 3 doubles per column and solved step) and Tn
     right-hand-side slices written."""
     cum = np.cumsum(counts)
     total, launches = 0.0, 0
-    for K0 in range(2, nstep + 1, TILE):
-        Tn = min(TILE, nstep - K0 + 1)
+    for K0 in range(2, nstep + 1, tile):
+        Tn = min(tile, nstep - K0 + 1)
         j = min(K0 + Tn - 1, len(counts))
         total += cum[j - 1] * 120.0 + npairs * 16.0 + 2.0 * nhist * (K0 - 1) * 24.0 + Tn * ne * 24.0
         launches += 1
@@ -109,6 +107,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1, help="untimed whole-job solves")
     ap.add_argument("--h", type=float, default=0.1)
     ap.add_argument("--nstep", type=int, default=400)
+    ap.add_argument("--conv-tile", type=int, default=16, choices=(8, 16), help="steps per far-part tile")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
     args = ap.parse_args()
@@ -126,13 +125,13 @@ def main():
     w = mesh.layered_config2(h=args.h, nstep=args.nstep)
     log(f"workload {w.name}: {len(w.mesh.ele)} elements, {w.nstep} steps, dt={w.dt:.6g}")
     for i in range(args.warmup):
-        r = run_once(tdbem, w)
+        r = run_once(tdbem, w, tile=args.conv_tile)
         log(f"warmup {i}: {r['wall']:.2f} s, entries {r['entries']}")
     tdbem.synchronize()
     t0 = time.perf_counter()
     runs = []
     for i in range(args.steps):
-        runs.append(run_once(tdbem, w, keep=(i == args.steps - 1)))
+        runs.append(run_once(tdbem, w, keep=(i == args.steps - 1), tile=args.conv_tile))
         log(f"step {i}: {runs[-1]['wall']:.2f} s")
     tdbem.synchronize()
     elapsed = time.perf_counter() - t0
@@ -144,7 +143,7 @@ def main():
     dof = c.dofmap()["counts"]
     nhist = int(dof[7] + dof[8]) // 2
     conv_avg = tm["conv_far"] / max(tm["conv_far_launches"], 1)
-    bytes_launch = conv_far_bytes_per_launch(counts, npairs, c.ne, nhist, w.nstep)
+    bytes_launch = conv_far_bytes_per_launch(counts, npairs, c.ne, nhist, w.nstep, args.conv_tile)
     conv_gbs = bytes_launch / conv_avg / 1e9
     loop_time = tm["convolution"] + tm["gmres"]
     out = {
@@ -161,11 +160,13 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded layered block, Ricker load)",
         "config": {"workload": w.name, "elements": int(len(w.mesh.ele)), "domains": 4, "nstep": w.nstep,
-                   "dt": w.dt, "gmres_restart": 200, "gmres_tol": 1e-5, "history_band_entries": int(last["entries"])},
+                   "dt": w.dt, "gmres_restart": 200, "gmres_tol": 1e-5, "history_band_entries": int(last["entries"]),
+                   "conv_tile": args.conv_tile},
         "breakdown_s": {k: round(float(v), 4) for k, v in tm.items() if k != "conv_far_launches"},
         "time_loop_steps_per_s": round(w.nstep / max(loop_time, 1e-9), 3),
         "gmres_inner_iterations_mean": float(np.mean(last["iters"])),
-        "roofline": {"kernel": "k_conv_far (time-blocked history convolution, 8 steps per launch)", "bound": "hbm", "achieved": round(conv_gbs, 2),
+        "roofline": {"kernel": f"k_conv_far (time-blocked history convolution, {args.conv_tile} steps per launch)",
+                     "bound": "hbm", "achieved": round(conv_gbs, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(conv_gbs / HBM_PEAK_GBS, 4),
                      "traffic": None, "avg_launch_ms": round(1000 * conv_avg, 4),
                      "bytes_per_launch": round(bytes_launch)},
@@ -175,7 +176,7 @@ def main():
         log(f"cpu baseline on {args.cpu_threads} threads ...")
         cb = cpu_baseline(args.cpu_threads)
         # the GPU path on the same bounded sample, for a like-for-like ratio
-        g = run_once(tdbem, cb["w"], keep=True)
+        g = run_once(tdbem, cb["w"], keep=True, tile=args.conv_tile)
         err = float(np.abs(g["U"] - cb["U"]).max() / max(np.abs(cb["U"]).max(), 1e-300))
         g["case"].close()
         out["cpu_baseline"] = {"value": round(cb["value"], 4), "unit": "time steps/s", "cores": args.cpu_threads,

@@ -98,6 +98,8 @@ int tdbem_history_stats(tdbem_ctx* ctx, long* counts, long* npairs);
 /* Tuning options (no counterpart in the reference):
  *   "conv_tiled" 1 (default) time-blocked history convolution, 0 one full pass
  *   over the history bands per step (the reference's per-step summation);
+ *   "conv_tile" 16 (default) or 8 steps per far-part tile of the
+ *   time-blocked convolution;
  *   "quad_staged" 1 (default) wave-per-pair history quadrature with the
  *   point geometry staged in LDS, 0 one lane per band entry with separate
  *   T2 / T1 / U passes (same results). */

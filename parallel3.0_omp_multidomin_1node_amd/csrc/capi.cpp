@@ -235,6 +235,12 @@ int tdbem_set_option(tdbem_ctx* ctx, const char* name, double value)
         ctx->c.convTiled = value != 0.0;
         return 0;
     }
+    if (n == "conv_tile") {
+        if (value != 8.0 && value != 16.0)
+            return Fail("conv_tile must be 8 or 16");
+        ctx->c.convTile = (int)value;  // near lists are rebuilt by the next solve
+        return 0;
+    }
     if (n == "quad_staged") {
         ctx->c.quadStaged = value != 0.0;
         return 0;

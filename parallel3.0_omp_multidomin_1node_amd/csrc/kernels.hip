@@ -1734,6 +1734,26 @@ __device__ __forceinline__ double WaveSum(double v)
 
 }  // namespace dev
 
+namespace dev {
+
+// Orders LDS writes and reads of one wavefront's lanes (LDS operations of a
+// wavefront complete in issue order).
+__device__ __forceinline__ void WaveSync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+}  // namespace dev
+
+// One block of four wavefronts per source row; each wavefront takes every
+// fourth pair of the row and 64 lags of it at a time.  The 1-2-1 combination
+// of the three states a lag touches is applied to the state series first:
+// the wave stages the solved states (m <= K0-1) its lags reach in LDS and
+// forms the filtered series u~[x] = u[x-1] + 2 u[x] + u[x+1] (t~ likewise),
+// so each (entry, step) costs one block product G t~ - M u~, formed as 18
+// fused multiply-adds into the step's accumulator.
 template <int T>
 __global__ void __launch_bounds__(256) k_conv_far(PairGrid grid, const NodeRefs* __restrict__ refs,
                                                   const long* __restrict__ off, const int* __restrict__ kb,
@@ -1742,66 +1762,98 @@ __global__ void __launch_bounds__(256) k_conv_far(PairGrid grid, const NodeRefs*
                                                   const double* __restrict__ gT, int nstore, int K0, int Tn, int maxn,
                                                   int ne, double* __restrict__ hfar)
 {
+    constexpr int kRaw = 64 + T + 1;  // states a 64-lag chunk reaches
+    __shared__ double raw[4][kRaw * 6];
+    __shared__ double filt[4][kRaw * 6];
     int s = blockIdx.x;
     int d = refs[s].domain;
     int nd = grid.eleCount[d];
-    long rowStart = grid.pairBase[d] + (long)(s - grid.eleBegin[d]) * nd;
-    int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int eb = grid.eleBegin[d];
+    long rowStart = grid.pairBase[d] + (long)(s - eb) * nd;
+    int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    int lane = threadIdx.x & 63;
     int Kmax = K0 + Tn - 1;
     int jmax = Kmax < maxn ? Kmax : maxn;
+    double* rw = raw[wave];
+    double* fw = filt[wave];
     double acc[T][3];
 #pragma unroll
     for (int i = 0; i < T; ++i)
         acc[i][0] = acc[i][1] = acc[i][2] = 0.0;
-    for (int q = wave; q < nd; q += 4) {
+    // per-pair metadata, loaded one pair ahead
+    int nL = 0, nk0 = 0, nhb = 0, ntb = 0;
+    long nbase = 0;
+    auto meta = [&](int q) {
         long p = rowStart + q;
-        int L = len[p];
-        if (L == 0)
+        nL = len[p];
+        nk0 = kb[p];
+        nbase = off[p];
+        nhb = refs[eb + q].hu_block;
+        ntb = refs[eb + q].ht_block;
+    };
+    if (wave < nd)
+        meta(wave);
+    for (int q = wave; q < nd; q += 4) {
+        int L = nL, k0 = nk0, hb = nhb, tb = ntb;
+        long base = nbase;
+        if (q + 4 < nd)
+            meta(q + 4);
+        int Lj = jmax - k0 + 1 < L ? jmax - k0 + 1 : L;  // lags j <= jmax
+        if (Lj <= 0)
             continue;
-        int k0 = kb[p];
-        int f = grid.eleBegin[d] + q;
-        const double* su = gU + (long)refs[f].hu_block * nstore * 3;
-        const double* st = gT + (long)refs[f].ht_block * nstore * 3;
-        long base = off[p];
-        for (int t = lane; t < L; t += 64) {
-            int j = k0 + t;
-            if (j > jmax)
-                break;
-            long e = base + t;
+        const double* su = gU + (long)hb * nstore * 3;
+        const double* st = gT + (long)tb * nstore * 3;
+        for (int t0 = 0; t0 < Lj; t0 += 64) {
+            int nl = Lj - t0 < 64 ? Lj - t0 : 64;
+            int t = t0 + lane;
+            bool own = lane < nl;
+            // the lane's blocks, issued together with the state staging loads
             double M[9], G[6];
+            long e = base + (own ? t : 0);
 #pragma unroll
             for (int i = 0; i < 9; ++i)
                 M[i] = mts[9 * e + i];
 #pragma unroll
             for (int i = 0; i < 6; ++i)
                 G[i] = mgs[6 * e + i];
-            // rolling window of v_m = MGS T_m - MTS U_m for m = K-j-1, K-j, K-j+1
-            double vp[3], vc[3], vn[3];
-            auto load = [&](int m, double* v) {
-                if (m >= 1 && m <= K0 - 1) {
-                    const double* a = su + 3 * (long)m;
-                    const double* b = st + 3 * (long)m;
-                    double uu[3] = {a[0], a[1], a[2]}, tt[3] = {b[0], b[1], b[2]};
-                    dev::BlockApply(M, G, uu, tt, v);
-                } else {
-                    v[0] = v[1] = v[2] = 0.0;
-                }
-            };
-            load(K0 - j - 1, vp);
-            load(K0 - j, vc);
+            // x = K - j for the chunk's lags and the tile's steps
+            int xlo = K0 - (k0 + t0 + nl - 1), xhi = Kmax - (k0 + t0);
+            int nx = xhi - xlo + 1;  // <= nl + Tn - 1
+            int mlo = xlo - 1;
+            for (int idx = lane; idx < (nx + 2) * 6; idx += 64) {
+                int r = idx / 6, c = idx - 6 * r;
+                int mm = mlo + r;
+                double v = 0.0;
+                if (mm >= 1 && mm <= K0 - 1)
+                    v = c < 3 ? su[3 * (long)mm + c] : st[3 * (long)mm + (c - 3)];
+                rw[idx] = v;
+            }
+            dev::WaveSync();
+            for (int idx = lane; idx < nx * 6; idx += 64)
+                fw[idx] = (rw[idx] + 2.0 * rw[idx + 6]) + rw[idx + 12];
+            dev::WaveSync();
+            if (own) {
+                int x0 = K0 - (k0 + t) - xlo;  // filtered index of step K0
 #pragma unroll
-            for (int i = 0; i < T; ++i) {
-                load(K0 + i - j + 1, vn);
-                if (i < Tn) {
-                    acc[i][0] += vp[0] + 2.0 * vc[0] + vn[0];
-                    acc[i][1] += vp[1] + 2.0 * vc[1] + vn[1];
-                    acc[i][2] += vp[2] + 2.0 * vc[2] + vn[2];
-                }
-                for (int c = 0; c < 3; ++c) {
-                    vp[c] = vc[c];
-                    vc[c] = vn[c];
+                for (int i = 0; i < T; ++i) {
+                    if (i < Tn) {
+                        // acc += MGS t~ - MTS u~, fused multiply-adds straight into the accumulator
+                        const double* w = fw + 6 * (x0 + i);
+                        double u0 = w[0], u1 = w[1], u2 = w[2], t0v = w[3], t1v = w[4], t2v = w[5];
+                        double a0 = acc[i][0], a1 = acc[i][1], a2 = acc[i][2];
+                        a0 = fma(G[0], t0v, a0); a0 = fma(G[1], t1v, a0); a0 = fma(G[2], t2v, a0);
+                        a1 = fma(G[1], t0v, a1); a1 = fma(G[3], t1v, a1); a1 = fma(G[4], t2v, a1);
+                        a2 = fma(G[2], t0v, a2); a2 = fma(G[4], t1v, a2); a2 = fma(G[5], t2v, a2);
+                        a0 = fma(-M[0], u0, a0); a0 = fma(-M[1], u1, a0); a0 = fma(-M[2], u2, a0);
+                        a1 = fma(-M[3], u0, a1); a1 = fma(-M[4], u1, a1); a1 = fma(-M[5], u2, a1);
+                        a2 = fma(-M[6], u0, a2); a2 = fma(-M[7], u1, a2); a2 = fma(-M[8], u2, a2);
+                        acc[i][0] = a0;
+                        acc[i][1] = a1;
+                        acc[i][2] = a2;
+                    }
                 }
             }
+            dev::WaveSync();
         }
     }
     __shared__ double red[4][T * 3];
@@ -1880,10 +1932,12 @@ __global__ void __launch_bounds__(64) k_conv_near(const NodeRefs* __restrict__ r
 
 void LaunchConvFar(PairGrid grid, const NodeRefs* refs, const long* off, const int* kb, const int* len,
                    const double* mts, const double* mgs, const double* gU, const double* gT, int nstore, int K0,
-                   int Tn, int maxn, int ne, double* hfar, hipStream_t s)
+                   int Tn, int tile, int maxn, int ne, double* hfar, hipStream_t s)
 {
-    k_conv_far<kConvTile><<<ne, 256, 0, s>>>(grid, refs, off, kb, len, mts, mgs, gU, gT, nstore, K0, Tn, maxn, ne,
-                                             hfar);
+    if (tile == 16)
+        k_conv_far<16><<<ne, 256, 0, s>>>(grid, refs, off, kb, len, mts, mgs, gU, gT, nstore, K0, Tn, maxn, ne, hfar);
+    else
+        k_conv_far<8><<<ne, 256, 0, s>>>(grid, refs, off, kb, len, mts, mgs, gU, gT, nstore, K0, Tn, maxn, ne, hfar);
 }
 
 void LaunchConvNear(const NodeRefs* refs, const long* nearPtr, const long* nearPair, const int* nearField,

@@ -86,12 +86,13 @@ void LaunchConvolution(PairGrid grid, const NodeRefs* refs, const long* off, con
                        const double* mts, const double* mgs, const double* gU, const double* gT, int nstore, int K,
                        int maxn, int ne, double* rhs, hipStream_t s);
 
-// Time-blocked convolution: far part of the tile [K0, K0+Tn) into
-// hfar[i][3*ne] (i < Tn), then per step the near part plus hfar into rhs.
-constexpr int kConvTile = 8;
+// Time-blocked convolution: far part of the tile [K0, K0+Tn) (tile 8 or 16
+// steps) into hfar[i][3*ne] (i < Tn), then per step the near part plus hfar
+// into rhs.
+constexpr int kConvTileMax = 16;
 void LaunchConvFar(PairGrid grid, const NodeRefs* refs, const long* off, const int* kb, const int* len,
                    const double* mts, const double* mgs, const double* gU, const double* gT, int nstore, int K0,
-                   int Tn, int maxn, int ne, double* hfar, hipStream_t s);
+                   int Tn, int tile, int maxn, int ne, double* hfar, hipStream_t s);
 void LaunchConvNear(const NodeRefs* refs, const long* nearPtr, const long* nearPair, const int* nearField,
                     const long* off, const int* kb, const int* len, const double* mts, const double* mgs,
                     const double* gU, const double* gT, int nstore, int K, int K0, int maxn, int ne,

@@ -230,8 +230,15 @@ void Case::Cull(int maxn)
         chunks = c;
         d_coff.upload(coff.data(), coff.size(), stream);
     }
-    // per source row, the pairs whose band starts at a lag <= tile + 1: the
-    // only ones the near part of the time-blocked convolution touches
+    BuildNearLists(convTile);
+    culledFor = maxn;
+    timing[1] = Now() - t0;
+}
+
+// Per source row, the pairs whose band starts at a lag <= tile + 1: the only
+// ones the near part of the time-blocked convolution touches.
+void Case::BuildNearLists(int tile)
+{
     std::vector<long> nearPtr((size_t)ne + 1, 0), nearPair;
     std::vector<int> nearField;
     for (int d = 0; d < grid.nd; ++d) {
@@ -240,7 +247,7 @@ void Case::Cull(int maxn)
             long srow = grid.eleBegin[d] + sl;
             for (long fl = 0; fl < n; ++fl) {
                 long p = grid.pairBase[d] + sl * n + fl;
-                if (h_len[(size_t)p] > 0 && h_kb[(size_t)p] <= kConvTile + 1) {
+                if (h_len[(size_t)p] > 0 && h_kb[(size_t)p] <= tile + 1) {
                     nearPair.push_back(p);
                     nearField.push_back((int)(grid.eleBegin[d] + fl));
                 }
@@ -251,8 +258,8 @@ void Case::Cull(int maxn)
     d_nearPtr.upload(nearPtr.data(), nearPtr.size(), stream);
     d_nearPair.upload(nearPair.data(), nearPair.size(), stream);
     d_nearField.upload(nearField.data(), nearField.size(), stream);
-    culledFor = maxn;
-    timing[1] = Now() - t0;
+    Check(hipStreamSynchronize(stream), "near lists");
+    nearTile = tile;
 }
 
 void Case::AssembleStep0(long maxleaf)
@@ -646,6 +653,10 @@ void Case::Solve(int nstep, const double* known_lu, long restart, double tol, do
     if (!historyDone || maxn < 1)
         throw std::runtime_error("history not assembled");
     PrepareGmres(restart);
+    if (convTile != 8 && convTile != 16)
+        throw std::runtime_error("convolution tile must be 8 or 16");
+    if (convTiled && nearTile != convTile)
+        BuildNearLists(convTile);
     const long n3 = 3 * ne;
     const int nstore = nstep + 1;
     DevBuf<double> d_bd, d_lu, d_lt, d_gU, d_gT, d_known, d_rhs, d_x, d_hfar;
@@ -657,7 +668,7 @@ void Case::Solve(int nstep, const double* known_lu, long restart, double tol, do
     d_known.alloc(3 * (size_t)std::max<long>(1, dof.knownBlockCount));
     d_rhs.alloc((size_t)n3);
     d_x.alloc((size_t)n3);
-    d_hfar.alloc((size_t)kConvTile * (size_t)n3);
+    d_hfar.alloc((size_t)kConvTileMax * (size_t)n3);
     LaunchZero(d_gU.p, (long)d_gU.n, stream);
     LaunchZero(d_gT.p, (long)d_gT.n, stream);
     LaunchZero(d_known.p, (long)d_known.n, stream);
@@ -679,12 +690,12 @@ void Case::Solve(int nstep, const double* known_lu, long restart, double tol, do
         if (K > 1) {
             Check(hipEventRecord(c0, stream), "event");
             if (convTiled) {
-                if (K == 2 || K - K0 >= kConvTile) {
+                if (K == 2 || K - K0 >= convTile) {
                     K0 = K;
-                    int Tn = std::min(kConvTile, nstep - K0 + 1);
+                    int Tn = std::min(convTile, nstep - K0 + 1);
                     Check(hipEventRecord(f0, stream), "event");
                     LaunchConvFar(grid, d_refs.p, d_off.p, d_kb.p, d_len.p, d_mts.p, d_mgs.p, d_gU.p, d_gT.p, nstore,
-                                  K0, Tn, maxn, (int)ne, d_hfar.p, stream);
+                                  K0, Tn, convTile, maxn, (int)ne, d_hfar.p, stream);
                     Check(hipEventRecord(f1, stream), "event");
                     Check(hipEventSynchronize(f1), "far");
                     float fms = 0.f;

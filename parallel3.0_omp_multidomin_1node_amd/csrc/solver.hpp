@@ -51,6 +51,7 @@ public:
     LeafPreconditioner pre;
     double timing[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     bool convTiled = true;   // time-blocked convolution (false: one full pass per step)
+    int convTile = 16;       // steps per far-part tile (8 or 16); fixes the near lists built by the cull
     bool quadStaged = true;  // wave-per-pair quadrature with LDS-staged point geometry (false: lane per entry)
     hipStream_t stream = nullptr;
 
@@ -58,6 +59,7 @@ public:
 
 private:
     void Cull(int maxn);
+    void BuildNearLists(int tile);
     bool StructureValid() const;
     void PrepareGmres(long restart);
     int Gmres(const double* d_b, double* d_x, long restart, double tol, int* iters);
@@ -76,6 +78,7 @@ private:
     long pwWaves = 0;  // patch-wise waves of the last history assembly
     DevBuf<long> d_nearPtr, d_nearPair;
     DevBuf<int> d_nearField;
+    int nearTile = 0;  // tile the near lists were built for
     std::vector<int> h_kb, h_len;
     std::vector<long> h_off;
     DevBuf<double> d_mts, d_mgs;

@@ -928,30 +928,11 @@ struct LaneCoef {
     double p1 = 0.0, s1 = 0.0, w1 = 0.0, ri = 0.0;
 };
 
-__device__ __forceinline__ double ISerS(int w, double n, double ndt, double np1dt, const double* g, const DynaMat& m)
-{
-    double t1;
-    switch (w) {
-    case 1: return (n + 0.5) * m.Dt_2 / g[sR2];
-    case 2: t1 = np1dt / g[sR1]; return 0.5 * (t1 * t1 - m.OneOverC1_2);
-    case 3: return m.Half_1OC22_1OC12;
-    case 4: t1 = ndt / g[sR1]; return 0.5 * (m.OneOverC2_2 - t1 * t1);
-    default: return 0.0;
-    }
-}
-
-__device__ __forceinline__ double JSerS(int w, double n, double ndt, double np1dt, const double* g, const DynaMat& m)
-{
-    double t1;
-    switch (w) {
-    case 1: t1 = g[sDt3R3]; return (n * n + n + 0.33333333333333333) * t1;
-    case 2: t1 = np1dt / g[sR1]; return 0.33333333333333 * (t1 * t1 * t1 - m.OneOverC1_3);
-    case 3: return m.OneOverThree_1OC23_1OC13;
-    case 4: t1 = ndt / g[sR1]; return 0.333333333333333 * (m.OneOverC2_3 - t1 * t1 * t1);
-    default: return 0.0;
-    }
-}
-
+// The ISeries / JSeries cases need one quotient each: ((n + 1/2) dt^2) / R^2
+// (case 1), ((n + 1) dt) / R (case 2) or (n dt) / R (case 4); the operands
+// are selected per lane so every lane of a wave runs one division, and the
+// case formulas are then applied to that quotient exactly as in ISeries /
+// JSeries.
 __device__ __forceinline__ void MakeLaneCoef(const DynaMat& m, int type, double n, const double* g, LaneCoef& c)
 {
     double nn = type == 1 ? n : n - 1;
@@ -979,17 +960,20 @@ __device__ __forceinline__ void MakeLaneCoef(const DynaMat& m, int type, double 
         c.p1 = kOneOver4Pi * (1.0 - n + g[sRC1Dt]);
         c.s1 = kOneOver4Pi * (1.0 - n + g[sRC2Dt]);
     }
-    c.w1 = 0.0;
-    c.ri = 0.0;
-    if (w) {
-        double ri = ISerS(w, nn, ndt, np1dt, g, m);
-        double rj = JSerS(w, nn, ndt, np1dt, g, m);
-        c.ri = ri;
-        if (type == 1)
-            c.w1 = kOneOver4Pi * ((n + 1.0) * ri - g[sR1] * rj / m.Dt);
-        else
-            c.w1 = kOneOver4Pi * ((1.0 - n) * ri + g[sR1] * rj / m.Dt);
-    }
+    double num = w == 1 ? (nn + 0.5) * m.Dt_2 : (w == 2 ? np1dt : ndt);
+    double den = w == 1 ? g[sR2] : g[sR1];
+    double qv = num / den;
+    double q2 = qv * qv;
+    double q3 = q2 * qv;
+    double ri1 = qv, rj1 = (nn * nn + nn + 0.33333333333333333) * g[sDt3R3];
+    double ri2 = 0.5 * (q2 - m.OneOverC1_2), rj2 = 0.33333333333333 * (q3 - m.OneOverC1_3);
+    double ri4 = 0.5 * (m.OneOverC2_2 - q2), rj4 = 0.333333333333333 * (m.OneOverC2_3 - q3);
+    double ri = w == 1 ? ri1 : w == 2 ? ri2 : w == 3 ? m.Half_1OC22_1OC12 : w == 4 ? ri4 : 0.0;
+    double rj = w == 1 ? rj1 : w == 2 ? rj2 : w == 3 ? m.OneOverThree_1OC23_1OC13 : w == 4 ? rj4 : 0.0;
+    double rq = g[sR1] * rj / m.Dt;
+    double w1 = type == 1 ? kOneOver4Pi * ((n + 1.0) * ri - rq) : kOneOver4Pi * ((1.0 - n) * ri + rq);
+    c.ri = w ? ri : 0.0;
+    c.w1 = w ? w1 : 0.0;
 }
 
 __device__ __forceinline__ bool WaveAny(bool v) { return __ballot(v) != 0; }

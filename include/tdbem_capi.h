@@ -100,6 +100,13 @@ int tdbem_history_stats(tdbem_ctx* ctx, long* counts, long* npairs);
  *   over the history bands per step (the reference's per-step summation);
  *   "conv_tile" 16 (default) or 8 steps per far-part tile of the
  *   time-blocked convolution;
+ *   "mixed" 0 (default) FP64 throughout; 1 or 2 mixed precision for the
+ *   next tdbem_assemble_history and the solves that use it: blocks computed
+ *   in FP64 and stored in FP32 (1) or as FP32 plus a bf16 residual (2, about
+ *   32 significant bits, 90 B per band entry instead of 120), FP32-stored
+ *   inner GMRES under an FP64 iterative-refinement residual (singular
+ *   tables, step-0 matrices, Hessenberg / Givens and all reductions stay
+ *   FP64);
  *   "quad_staged" 1 (default) wave-per-pair history quadrature with the
  *   point geometry staged in LDS, 0 one lane per band entry with separate
  *   T2 / T1 / U passes (same results). */

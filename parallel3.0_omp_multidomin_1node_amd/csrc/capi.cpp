@@ -241,6 +241,12 @@ int tdbem_set_option(tdbem_ctx* ctx, const char* name, double value)
         ctx->c.convTile = (int)value;  // near lists are rebuilt by the next solve
         return 0;
     }
+    if (n == "mixed") {
+        if (value != 0.0 && value != 1.0 && value != 2.0)
+            return Fail("mixed must be 0, 1 or 2");
+        ctx->c.mixed = (int)value;
+        return 0;
+    }
     if (n == "quad_staged") {
         ctx->c.quadStaged = value != 0.0;
         return 0;

@@ -1151,12 +1151,28 @@ __device__ __forceinline__ void ConsumePatchwise(const DynaMat& m, const double*
     }
 }
 
+// Split storage of the mixed mode: an FP64 value v is kept as hi = (float)v
+// and, optionally, lo = bf16(v - hi) (round to nearest even), so hi + lo
+// carries about 32 significant bits.
+__device__ __forceinline__ unsigned short Bf16Residual(double v, double hi)
+{
+    unsigned int bits = __float_as_uint((float)(v - hi));
+    bits += 0x7fffu + ((bits >> 16) & 1u);
+    return (unsigned short)(bits >> 16);
+}
+
+__device__ __forceinline__ double Bf16ToDouble(unsigned short lo)
+{
+    return (double)__uint_as_float(((unsigned int)lo) << 16);
+}
+
 // MTS entry = AddBlock(T2 or the self table) then AddBlock(T1); MGS entry =
 // AddBlock(U) (DBEM1/multidomain.cpp:2017-2078).
+template <typename TB>
 __device__ __forceinline__ void FinalizeEntry(const EntryFlags& fl, const double* selfT2f, double hu_sign,
                                               double ht_sign, const double* accA, const double* accB,
-                                              const double* accU, long e, double* mts, double* mgs,
-                                              unsigned char* alive)
+                                              const double* accU, long e, TB* mts, TB* mgs, unsigned short* mtsLo,
+                                              unsigned short* mgsLo, unsigned char* alive)
 {
     BlockAcc acc;
     if (fl.self1) {
@@ -1169,19 +1185,27 @@ __device__ __forceinline__ void FinalizeEntry(const EntryFlags& fl, const double
     }
     if (fl.f1)
         acc.Add(hu_sign, accB);
-    double* out = mts + 9 * e;
-    for (int i = 0; i < 9; ++i)
-        out[i] = acc.alive ? acc.v[i] : 0.0;
+    TB* out = mts + 9 * e;
+    for (int i = 0; i < 9; ++i) {
+        double v = acc.alive ? acc.v[i] : 0.0;
+        out[i] = (TB)v;
+        if (mtsLo)
+            mtsLo[9 * e + i] = Bf16Residual(v, (double)(TB)v);
+    }
     unsigned char bits = acc.alive ? 1 : 0;
     BlockAcc accg;
     if (fl.f1) {
         double u9[9] = {accU[0], accU[1], accU[2], accU[1], accU[3], accU[4], accU[2], accU[4], accU[5]};
         accg.Add(ht_sign, u9);
     }
-    double* og = mgs + 6 * e;
+    TB* og = mgs + 6 * e;
     const int pick[6] = {0, 1, 2, 4, 5, 8};
-    for (int i = 0; i < 6; ++i)
-        og[i] = accg.alive ? accg.v[pick[i]] : 0.0;
+    for (int i = 0; i < 6; ++i) {
+        double v = accg.alive ? accg.v[pick[i]] : 0.0;
+        og[i] = (TB)v;
+        if (mgsLo)
+            mgsLo[6 * e + i] = Bf16Residual(v, (double)(TB)v);
+    }
     bits |= accg.alive ? 2 : 0;
     alive[e] = bits;
 }
@@ -1218,11 +1242,13 @@ void LaunchPwCount(const ElemGeo* geo, PairGrid grid, const int* kb, const int* 
 // Regular-path entries: one wavefront per (pair, chunk of up to 64 lags), one
 // lag per lane; the lanes of patch-wise entries are left to k_history_pw.
 // The 16 regular points are staged once per wave.
+template <typename TB>
 __global__ void __launch_bounds__(64) k_history_reg(
     const ElemGeo* __restrict__ geo, const ElemPoints* __restrict__ pts, const NodeRefs* __restrict__ refs,
     const double* __restrict__ selfT2, PairGrid grid, const long* __restrict__ coff, const long* __restrict__ off,
-    const int* __restrict__ kb, const int* __restrict__ len, long npairs, long c0, long c1, double* __restrict__ mts,
-    double* __restrict__ mgs, unsigned char* __restrict__ alive)
+    const int* __restrict__ kb, const int* __restrict__ len, long npairs, long c0, long c1, TB* __restrict__ mts,
+    TB* __restrict__ mgs, unsigned short* __restrict__ mtsLo, unsigned short* __restrict__ mgsLo,
+    unsigned char* __restrict__ alive)
 {
     __shared__ double sgeo[kNG2 * dev::kStageStride];
     using namespace dev;
@@ -1268,7 +1294,7 @@ __global__ void __launch_bounds__(64) k_history_reg(
     if (!mine)
         return;
     FinalizeEntry(fl, selfT2 + 9 * (long)f, refs[f].hu_sign, refs[f].ht_sign, accA, accB, accU, off[p] + t, mts, mgs,
-                  alive);
+                  mtsLo, mgsLo, alive);
 }
 
 // Patch-wise entries: one wavefront per PwWave, whose lanes are the
@@ -1277,11 +1303,13 @@ __global__ void __launch_bounds__(64) k_history_reg(
 // staged once per patch.  Regular parts of these entries (the other judge
 // picking the regular path) are added here too, so each entry is finished by
 // exactly one kernel.
+template <typename TB>
 __global__ void __launch_bounds__(64) k_history_pw(
     const ElemGeo* __restrict__ geo, const ElemPoints* __restrict__ pts, const NodeRefs* __restrict__ refs,
     const double* __restrict__ selfT2, PairGrid grid, const PwWave* __restrict__ waves, long w0, long w1,
-    const long* __restrict__ off, const int* __restrict__ kb, const int* __restrict__ len, double* __restrict__ mts,
-    double* __restrict__ mgs, unsigned char* __restrict__ alive)
+    const long* __restrict__ off, const int* __restrict__ kb, const int* __restrict__ len, TB* __restrict__ mts,
+    TB* __restrict__ mgs, unsigned short* __restrict__ mtsLo, unsigned short* __restrict__ mgsLo,
+    unsigned char* __restrict__ alive)
 {
     __shared__ double sgeo[kPwSlots * kNPWG2 * dev::kStageStride];
     __shared__ double spr[kPwSlots * 2 * kNPW2];
@@ -1412,30 +1440,45 @@ __global__ void __launch_bounds__(64) k_history_pw(
     if (!active)
         return;
     FinalizeEntry(fl, selfT2 + 9 * (long)f, refs[f].hu_sign, refs[f].ht_sign, accA, accB, accU, off[p] + t, mts, mgs,
-                  alive);
+                  mtsLo, mgsLo, alive);
 }
 
+template <typename TB>
 void LaunchHistoryReg(const ElemGeo* geo, const ElemPoints* pts, const NodeRefs* refs, const double* selfT2,
                       PairGrid grid, const long* coff, const long* off, const int* kb, const int* len, long npairs,
-                      long c0, long c1, double* mts, double* mgs, unsigned char* alive, hipStream_t s)
+                      long c0, long c1, TB* mts, TB* mgs, unsigned short* mtsLo, unsigned short* mgsLo,
+                      unsigned char* alive, hipStream_t s)
 {
     long waves = c1 - c0;
     if (waves <= 0)
         return;
-    k_history_reg<<<(unsigned)waves, 64, 0, s>>>(geo, pts, refs, selfT2, grid, coff, off, kb, len, npairs, c0, c1, mts,
-                                                 mgs, alive);
+    k_history_reg<TB><<<(unsigned)waves, 64, 0, s>>>(geo, pts, refs, selfT2, grid, coff, off, kb, len, npairs, c0, c1,
+                                                     mts, mgs, mtsLo, mgsLo, alive);
 }
 
+template <typename TB>
 void LaunchHistoryPw(const ElemGeo* geo, const ElemPoints* pts, const NodeRefs* refs, const double* selfT2,
                      PairGrid grid, const PwWave* waves, long w0, long w1, const long* off, const int* kb,
-                     const int* len, double* mts, double* mgs, unsigned char* alive, hipStream_t s)
+                     const int* len, TB* mts, TB* mgs, unsigned short* mtsLo, unsigned short* mgsLo,
+                     unsigned char* alive, hipStream_t s)
 {
     long n = w1 - w0;
     if (n <= 0)
         return;
-    k_history_pw<<<(unsigned)n, 64, 0, s>>>(geo, pts, refs, selfT2, grid, waves, w0, w1, off, kb, len, mts, mgs,
-                                            alive);
+    k_history_pw<TB><<<(unsigned)n, 64, 0, s>>>(geo, pts, refs, selfT2, grid, waves, w0, w1, off, kb, len, mts, mgs,
+                                                mtsLo, mgsLo, alive);
 }
+
+#define TDBEM_HIST_INSTANTIATE(TB)                                                                                   \
+    template void LaunchHistoryReg<TB>(const ElemGeo*, const ElemPoints*, const NodeRefs*, const double*, PairGrid,   \
+                                       const long*, const long*, const int*, const int*, long, long, long, TB*, TB*,   \
+                                       unsigned short*, unsigned short*, unsigned char*, hipStream_t);                 \
+    template void LaunchHistoryPw<TB>(const ElemGeo*, const ElemPoints*, const NodeRefs*, const double*, PairGrid,    \
+                                      const PwWave*, long, long, const long*, const int*, const int*, TB*, TB*,        \
+                                      unsigned short*, unsigned short*, unsigned char*, hipStream_t);
+TDBEM_HIST_INSTANTIATE(double)
+TDBEM_HIST_INSTANTIATE(float)
+#undef TDBEM_HIST_INSTANTIATE
 
 // ---------------------------------------------------------------------------
 // Step-0 coefficient blocks (DBEM1/dsquareelement.cpp:2756-2868,3281-3394,
@@ -1738,13 +1781,15 @@ __device__ __forceinline__ void WaveSync()
 // forms the filtered series u~[x] = u[x-1] + 2 u[x] + u[x+1] (t~ likewise),
 // so each (entry, step) costs one block product G t~ - M u~, formed as 18
 // fused multiply-adds into the step's accumulator.
-template <int T>
+template <int T, typename TB>
 __global__ void __launch_bounds__(256) k_conv_far(PairGrid grid, const NodeRefs* __restrict__ refs,
                                                   const long* __restrict__ off, const int* __restrict__ kb,
-                                                  const int* __restrict__ len, const double* __restrict__ mts,
-                                                  const double* __restrict__ mgs, const double* __restrict__ gU,
-                                                  const double* __restrict__ gT, int nstore, int K0, int Tn, int maxn,
-                                                  int ne, double* __restrict__ hfar)
+                                                  const int* __restrict__ len, const TB* __restrict__ mts,
+                                                  const TB* __restrict__ mgs, const unsigned short* __restrict__ mtsLo,
+                                                  const unsigned short* __restrict__ mgsLo,
+                                                  const double* __restrict__ gU, const double* __restrict__ gT,
+                                                  int nstore, int K0, int Tn, int maxn, int ne,
+                                                  double* __restrict__ hfar)
 {
     constexpr int kRaw = 64 + T + 1;  // states a 64-lag chunk reaches
     __shared__ double raw[4][kRaw * 6];
@@ -1796,10 +1841,18 @@ __global__ void __launch_bounds__(256) k_conv_far(PairGrid grid, const NodeRefs*
             long e = base + (own ? t : 0);
 #pragma unroll
             for (int i = 0; i < 9; ++i)
-                M[i] = mts[9 * e + i];
+                M[i] = (double)mts[9 * e + i];
 #pragma unroll
             for (int i = 0; i < 6; ++i)
-                G[i] = mgs[6 * e + i];
+                G[i] = (double)mgs[6 * e + i];
+            if (mtsLo) {
+#pragma unroll
+                for (int i = 0; i < 9; ++i)
+                    M[i] += dev::Bf16ToDouble(mtsLo[9 * e + i]);
+#pragma unroll
+                for (int i = 0; i < 6; ++i)
+                    G[i] += dev::Bf16ToDouble(mgsLo[6 * e + i]);
+            }
             // x = K - j for the chunk's lags and the tile's steps
             int xlo = K0 - (k0 + t0 + nl - 1), xhi = Kmax - (k0 + t0);
             int nx = xhi - xlo + 1;  // <= nl + Tn - 1
@@ -1859,11 +1912,14 @@ __global__ void __launch_bounds__(256) k_conv_far(PairGrid grid, const NodeRefs*
 // Near part of step K in the tile starting at K0 plus the far part:
 // rhs_K += hfar[K-K0] + sum over near pairs (band start <= K-K0+1) of the
 // taps with K0 <= m <= K-1.  One wavefront per source row.
+template <typename TB>
 __global__ void __launch_bounds__(64) k_conv_near(const NodeRefs* __restrict__ refs, const long* __restrict__ nearPtr,
                                                   const long* __restrict__ nearPair, const int* __restrict__ nearField,
                                                   const long* __restrict__ off, const int* __restrict__ kb,
-                                                  const int* __restrict__ len, const double* __restrict__ mts,
-                                                  const double* __restrict__ mgs, const double* __restrict__ gU,
+                                                  const int* __restrict__ len, const TB* __restrict__ mts,
+                                                  const TB* __restrict__ mgs, const unsigned short* __restrict__ mtsLo,
+                                                  const unsigned short* __restrict__ mgsLo,
+                                                  const double* __restrict__ gU,
                                                   const double* __restrict__ gT, int nstore, int K, int K0, int maxn,
                                                   int ne, const double* __restrict__ hfar, double* __restrict__ rhs)
 {
@@ -1884,8 +1940,11 @@ __global__ void __launch_bounds__(64) k_conv_near(const NodeRefs* __restrict__ r
             if (j > jlim)
                 break;
             long e = off[p] + t;
-            const double* M = mts + 9 * e;
-            const double* G = mgs + 6 * e;
+            double M[9], G[6];
+            for (int i = 0; i < 9; ++i)
+                M[i] = (double)mts[9 * e + i] + (mtsLo ? dev::Bf16ToDouble(mtsLo[9 * e + i]) : 0.0);
+            for (int i = 0; i < 6; ++i)
+                G[i] = (double)mgs[6 * e + i] + (mgsLo ? dev::Bf16ToDouble(mgsLo[6 * e + i]) : 0.0);
             const int mm[3] = {K - j + 1, K - j, K - j - 1};
             const double co[3] = {1.0, 2.0, 1.0};
             double cu[3] = {0.0, 0.0, 0.0}, ct[3] = {0.0, 0.0, 0.0};
@@ -1914,62 +1973,84 @@ __global__ void __launch_bounds__(64) k_conv_near(const NodeRefs* __restrict__ r
     }
 }
 
-void LaunchConvFar(PairGrid grid, const NodeRefs* refs, const long* off, const int* kb, const int* len,
-                   const double* mts, const double* mgs, const double* gU, const double* gT, int nstore, int K0,
-                   int Tn, int tile, int maxn, int ne, double* hfar, hipStream_t s)
+template <typename TB>
+void LaunchConvFar(PairGrid grid, const NodeRefs* refs, const long* off, const int* kb, const int* len, const TB* mts,
+                   const TB* mgs, const unsigned short* mtsLo, const unsigned short* mgsLo, const double* gU,
+                   const double* gT, int nstore, int K0, int Tn, int tile, int maxn, int ne, double* hfar, hipStream_t s)
 {
     if (tile == 16)
-        k_conv_far<16><<<ne, 256, 0, s>>>(grid, refs, off, kb, len, mts, mgs, gU, gT, nstore, K0, Tn, maxn, ne, hfar);
+        k_conv_far<16, TB><<<ne, 256, 0, s>>>(grid, refs, off, kb, len, mts, mgs, mtsLo, mgsLo, gU, gT, nstore, K0, Tn,
+                                              maxn, ne, hfar);
     else
-        k_conv_far<8><<<ne, 256, 0, s>>>(grid, refs, off, kb, len, mts, mgs, gU, gT, nstore, K0, Tn, maxn, ne, hfar);
+        k_conv_far<8, TB><<<ne, 256, 0, s>>>(grid, refs, off, kb, len, mts, mgs, mtsLo, mgsLo, gU, gT, nstore, K0, Tn,
+                                             maxn, ne, hfar);
 }
 
+template <typename TB>
 void LaunchConvNear(const NodeRefs* refs, const long* nearPtr, const long* nearPair, const int* nearField,
-                    const long* off, const int* kb, const int* len, const double* mts, const double* mgs,
-                    const double* gU, const double* gT, int nstore, int K, int K0, int maxn, int ne,
-                    const double* hfar, double* rhs, hipStream_t s)
+                    const long* off, const int* kb, const int* len, const TB* mts, const TB* mgs,
+                    const unsigned short* mtsLo, const unsigned short* mgsLo, const double* gU, const double* gT,
+                    int nstore, int K, int K0, int maxn, int ne, const double* hfar, double* rhs, hipStream_t s)
 {
-    k_conv_near<<<ne, 64, 0, s>>>(refs, nearPtr, nearPair, nearField, off, kb, len, mts, mgs, gU, gT, nstore, K, K0,
-                                  maxn, ne, hfar, rhs);
+    k_conv_near<TB><<<ne, 64, 0, s>>>(refs, nearPtr, nearPair, nearField, off, kb, len, mts, mgs, mtsLo, mgsLo, gU, gT,
+                                      nstore, K, K0, maxn, ne, hfar, rhs);
 }
+
+#define TDBEM_CONV_INSTANTIATE(TB)                                                                                   \
+    template void LaunchConvFar<TB>(PairGrid, const NodeRefs*, const long*, const int*, const int*, const TB*,        \
+                                    const TB*, const unsigned short*, const unsigned short*, const double*,           \
+                                    const double*, int, int, int, int, int, int, double*, hipStream_t);               \
+    template void LaunchConvNear<TB>(const NodeRefs*, const long*, const long*, const int*, const long*, const int*,  \
+                                     const int*, const TB*, const TB*, const unsigned short*, const unsigned short*,  \
+                                     const double*, const double*, int, int, int, int, int, const double*, double*,   \
+                                     hipStream_t);
+TDBEM_CONV_INSTANTIATE(double)
+TDBEM_CONV_INSTANTIATE(float)
+#undef TDBEM_CONV_INSTANTIATE
 
 // ---------------------------------------------------------------------------
 // Linear algebra for GMRES
 // ---------------------------------------------------------------------------
+// Storage types: TA (matrix / inverse values), TX (input vector), TY (output
+// vector); every product and sum is formed in FP64.  With all types double
+// these are the FP64 kernels of the parity path; float storage serves the
+// inner solves of the mixed-precision mode.
+template <typename TA, typename TX, typename TY>
 __global__ void __launch_bounds__(256) k_spmv(long nrows, const int* __restrict__ rp, const int* __restrict__ col,
-                                              const double* __restrict__ val, const double* __restrict__ x,
-                                              double* __restrict__ y)
+                                              const TA* __restrict__ val, const TX* __restrict__ x,
+                                              TY* __restrict__ y)
 {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nrows)
         return;
     double b0 = 0.0, b1 = 0.0, b2 = 0.0;
     for (int j = rp[i]; j < rp[i + 1]; ++j) {
-        const double* v = val + 9 * (long)j;
-        const double* xc = x + 3 * (long)col[j];
-        double x0 = xc[0], x1 = xc[1], x2 = xc[2];
-        b0 += v[0] * x0 + v[1] * x1 + v[2] * x2;
-        b1 += v[3] * x0 + v[4] * x1 + v[5] * x2;
-        b2 += v[6] * x0 + v[7] * x1 + v[8] * x2;
+        const TA* v = val + 9 * (long)j;
+        const TX* xc = x + 3 * (long)col[j];
+        double x0 = (double)xc[0], x1 = (double)xc[1], x2 = (double)xc[2];
+        b0 += (double)v[0] * x0 + (double)v[1] * x1 + (double)v[2] * x2;
+        b1 += (double)v[3] * x0 + (double)v[4] * x1 + (double)v[5] * x2;
+        b2 += (double)v[6] * x0 + (double)v[7] * x1 + (double)v[8] * x2;
     }
-    y[3 * i] = b0;
-    y[3 * i + 1] = b1;
-    y[3 * i + 2] = b2;
+    y[3 * i] = (TY)b0;
+    y[3 * i + 1] = (TY)b1;
+    y[3 * i + 2] = (TY)b2;
 }
 
-void LaunchSpmv(long nrows, const int* rp, const int* col, const double* val, const double* x, double* y,
-                hipStream_t s)
+template <typename TA, typename TX, typename TY>
+void LaunchSpmv(long nrows, const int* rp, const int* col, const TA* val, const TX* x, TY* y, hipStream_t s)
 {
-    k_spmv<<<(unsigned)((nrows + 255) / 256), 256, 0, s>>>(nrows, rp, col, val, x, y);
+    k_spmv<TA, TX, TY><<<(unsigned)((nrows + 255) / 256), 256, 0, s>>>(nrows, rp, col, val, x, y);
 }
 
 // GMRES_M_X (DBEM1/pre_gmres.cpp:837-895): each output scalar is owned by
 // one lane and summed in the reference order.
+template <typename TA, typename TX, typename TY>
 __global__ void __launch_bounds__(256) k_precond(long n, const long* __restrict__ posLeaf,
                                                  const long* __restrict__ leafBegin, const long* __restrict__ leafSize,
-                                                 const long* __restrict__ matOff, const double* __restrict__ inv,
+                                                 const long* __restrict__ matOff, const TA* __restrict__ inv,
                                                  const long* __restrict__ repid, const long* __restrict__ outBlock,
-                                                 const double* __restrict__ x, double* __restrict__ y)
+                                                 const TX* __restrict__ x, TY* __restrict__ y)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= 3 * n)
@@ -1978,34 +2059,35 @@ __global__ void __launch_bounds__(256) k_precond(long n, const long* __restrict_
     int mrow = (int)(t % 3);
     long leaf = posLeaf[i];
     long b = leafBegin[leaf], c = leafSize[leaf], jl = i - b, n3 = 3 * c;
-    const double* P = inv + matOff[leaf] + (3 * jl + mrow) * n3;
+    const TA* P = inv + matOff[leaf] + (3 * jl + mrow) * n3;
     double acc = 0.0;
     for (long k = 0; k < c; ++k) {
-        const double* xc = x + 3 * repid[b + k];
-        acc += P[3 * k] * xc[0];
-        acc += P[3 * k + 1] * xc[1];
-        acc += P[3 * k + 2] * xc[2];
+        const TX* xc = x + 3 * repid[b + k];
+        acc += (double)P[3 * k] * (double)xc[0];
+        acc += (double)P[3 * k + 1] * (double)xc[1];
+        acc += (double)P[3 * k + 2] * (double)xc[2];
     }
-    y[3 * outBlock[i] + mrow] = acc;
+    y[3 * outBlock[i] + mrow] = (TY)acc;
 }
 
+template <typename TA, typename TX, typename TY>
 void LaunchPrecond(long n, const long* posLeaf, const long* leafBegin, const long* leafSize, const long* matOff,
-                   const double* inv, const long* repid, const long* outBlock, const double* x, double* y,
-                   hipStream_t s)
+                   const TA* inv, const long* repid, const long* outBlock, const TX* x, TY* y, hipStream_t s)
 {
-    k_precond<<<(unsigned)((3 * n + 255) / 256), 256, 0, s>>>(n, posLeaf, leafBegin, leafSize, matOff, inv, repid,
-                                                              outBlock, x, y);
+    k_precond<TA, TX, TY><<<(unsigned)((3 * n + 255) / 256), 256, 0, s>>>(n, posLeaf, leafBegin, leafSize, matOff,
+                                                                          inv, repid, outBlock, x, y);
 }
 
 // Deterministic two-stage reductions (fixed partition and order).
-__global__ void __launch_bounds__(256) k_multidot(const double* __restrict__ V, long ldv, int nv,
-                                                  const double* __restrict__ u, long n, double* __restrict__ partial)
+template <typename TV, typename TU>
+__global__ void __launch_bounds__(256) k_multidot(const TV* __restrict__ V, long ldv, int nv,
+                                                  const TU* __restrict__ u, long n, double* __restrict__ partial)
 {
     __shared__ double red[256];
     for (int k = 0; k < nv; ++k) {
         double a = 0.0;
         for (long r = (long)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (long)gridDim.x * blockDim.x)
-            a += V[k * ldv + r] * u[r];
+            a += (double)V[k * ldv + r] * (double)u[r];
         red[threadIdx.x] = a;
         __syncthreads();
         for (int w = 128; w > 0; w >>= 1) {
@@ -2030,21 +2112,24 @@ __global__ void k_finish(const double* __restrict__ partial, int nblocks, int nv
     h[k] = doSqrt ? sqrt(a) : a;
 }
 
-void LaunchMultiDot(const double* V, long ldv, int nv, const double* u, long n, double* partial, int nblocks,
-                    double* h, hipStream_t s)
+template <typename TV, typename TU>
+void LaunchMultiDot(const TV* V, long ldv, int nv, const TU* u, long n, double* partial, int nblocks, double* h,
+                    hipStream_t s)
 {
-    k_multidot<<<nblocks, 256, 0, s>>>(V, ldv, nv, u, n, partial);
+    k_multidot<TV, TU><<<nblocks, 256, 0, s>>>(V, ldv, nv, u, n, partial);
     k_finish<<<(nv + 63) / 64, 64, 0, s>>>(partial, nblocks, nv, h, 0);
 }
 
-void LaunchNorm(const double* x, long n, double* partial, int nblocks, double* out, hipStream_t s)
+template <typename T>
+void LaunchNorm(const T* x, long n, double* partial, int nblocks, double* out, hipStream_t s)
 {
-    k_multidot<<<nblocks, 256, 0, s>>>(x, n, 1, x, n, partial);
+    k_multidot<T, T><<<nblocks, 256, 0, s>>>(x, n, 1, x, n, partial);
     k_finish<<<1, 64, 0, s>>>(partial, nblocks, 1, out, 1);
 }
 
-__global__ void __launch_bounds__(256) k_multiaxpy(const double* __restrict__ V, long ldv, int nv,
-                                                   const double* __restrict__ h, double* __restrict__ u, long n,
+template <typename TV, typename TU>
+__global__ void __launch_bounds__(256) k_multiaxpy(const TV* __restrict__ V, long ldv, int nv,
+                                                   const double* __restrict__ h, TU* __restrict__ u, long n,
                                                    double* htot)
 {
     long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -2052,37 +2137,39 @@ __global__ void __launch_bounds__(256) k_multiaxpy(const double* __restrict__ V,
         htot[r] += h[r];
     if (r >= n)
         return;
-    double a = u[r];
+    double a = (double)u[r];
     for (int k = 0; k < nv; ++k)
-        a -= h[k] * V[k * ldv + r];
-    u[r] = a;
+        a -= h[k] * (double)V[k * ldv + r];
+    u[r] = (TU)a;
 }
 
-void LaunchMultiAxpy(const double* V, long ldv, int nv, const double* h, double* u, long n, double* htot,
-                     hipStream_t s)
+template <typename TV, typename TU>
+void LaunchMultiAxpy(const TV* V, long ldv, int nv, const double* h, TU* u, long n, double* htot, hipStream_t s)
 {
     long g = n > nv ? n : nv;
-    k_multiaxpy<<<(unsigned)((g + 255) / 256), 256, 0, s>>>(V, ldv, nv, h, u, n, htot);
+    k_multiaxpy<TV, TU><<<(unsigned)((g + 255) / 256), 256, 0, s>>>(V, ldv, nv, h, u, n, htot);
 }
 
 // y = x / a[0] (invert) or y = x * a[0]; a zero or tiny denominator (<= 1e-12)
 // gives y = 0, as the reference does for a lucky breakdown.
-__global__ void __launch_bounds__(256) k_scale(const double* __restrict__ x, const double* __restrict__ a, int invert,
-                                               double* __restrict__ y, long n)
+template <typename TX, typename TY>
+__global__ void __launch_bounds__(256) k_scale(const TX* __restrict__ x, const double* __restrict__ a, int invert,
+                                               TY* __restrict__ y, long n)
 {
     long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n)
         return;
     double s = a[0];
     if (invert)
-        y[r] = s <= 1.0e-12 ? 0.0 : x[r] / s;
+        y[r] = (TY)(s <= 1.0e-12 ? 0.0 : (double)x[r] / s);
     else
-        y[r] = x[r] * s;
+        y[r] = (TY)((double)x[r] * s);
 }
 
-void LaunchScaleCopy(const double* x, const double* a, int invert, double* y, long n, hipStream_t s)
+template <typename TX, typename TY>
+void LaunchScaleCopy(const TX* x, const double* a, int invert, TY* y, long n, hipStream_t s)
 {
-    k_scale<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(x, a, invert, y, n);
+    k_scale<TX, TY><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(x, a, invert, y, n);
 }
 
 __global__ void __launch_bounds__(256) k_residual(const double* __restrict__ b, double* __restrict__ y, long n)
@@ -2095,6 +2182,33 @@ __global__ void __launch_bounds__(256) k_residual(const double* __restrict__ b, 
 void LaunchResidual(const double* b, double* y, long n, hipStream_t s)
 {
     k_residual<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(b, y, n);
+}
+
+// x += y (FP64), the refinement update of the mixed-precision solve.
+__global__ void __launch_bounds__(256) k_axpy1(const double* __restrict__ y, double* __restrict__ x, long n)
+{
+    long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n)
+        x[r] += y[r];
+}
+
+void LaunchAdd(const double* y, double* x, long n, hipStream_t s)
+{
+    k_axpy1<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(y, x, n);
+}
+
+// float copy of an FP64 array (mixed-precision matrices).
+__global__ void __launch_bounds__(256) k_tofloat(const double* __restrict__ x, float* __restrict__ y, long n)
+{
+    long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n)
+        y[r] = (float)x[r];
+}
+
+void LaunchToFloat(const double* x, float* y, long n, hipStream_t s)
+{
+    if (n > 0)
+        k_tofloat<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(x, y, n);
 }
 
 // Givens rotation in QT form for column j (DBEM1/pre_gmres.cpp:1492-1533),
@@ -2153,7 +2267,8 @@ void LaunchGivens(double* h, double* QT, double* R, double* f, double* phibar, i
 
 // W(:,j) = (V(:,j) - sum_{l<j} W(:,l) R(l,j)) / R(j,j);  x += f_j W(:,j)
 // (DBEM1/pre_gmres.cpp:1535-1555).  V and W column-major with ld = n.
-__global__ void __launch_bounds__(256) k_wupdate(const double* __restrict__ V, double* __restrict__ W, long n,
+template <typename TV>
+__global__ void __launch_bounds__(256) k_wupdate(const TV* __restrict__ V, TV* __restrict__ W, long n,
                                                  const double* __restrict__ R, int j, int restart,
                                                  const double* __restrict__ f, double* __restrict__ x)
 {
@@ -2161,18 +2276,37 @@ __global__ void __launch_bounds__(256) k_wupdate(const double* __restrict__ V, d
     if (k >= n)
         return;
     double rjj = R[(j - 1) * restart + (j - 1)];
-    double w = V[(long)(j - 1) * n + k] / rjj;
+    double w = (double)V[(long)(j - 1) * n + k] / rjj;
     for (int l = 1; l <= j - 1; ++l)
-        w -= W[(long)(l - 1) * n + k] * R[(l - 1) * restart + (j - 1)] / rjj;
-    W[(long)(j - 1) * n + k] = w;
+        w -= (double)W[(long)(l - 1) * n + k] * R[(l - 1) * restart + (j - 1)] / rjj;
+    W[(long)(j - 1) * n + k] = (TV)w;
     x[k] += f[j] * w;
 }
 
-void LaunchWUpdate(const double* V, double* W, long n, const double* R, int j, int restart, const double* f,
-                   double* x, hipStream_t s)
+template <typename TV>
+void LaunchWUpdate(const TV* V, TV* W, long n, const double* R, int j, int restart, const double* f, double* x,
+                   hipStream_t s)
 {
-    k_wupdate<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(V, W, n, R, j, restart, f, x);
+    k_wupdate<TV><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(V, W, n, R, j, restart, f, x);
 }
+
+#define TDBEM_LA_INSTANTIATE(T)                                                                                      \
+    template void LaunchSpmv<T, T, T>(long, const int*, const int*, const T*, const T*, T*, hipStream_t);           \
+    template void LaunchPrecond<T, T, T>(long, const long*, const long*, const long*, const long*, const T*,        \
+                                         const long*, const long*, const T*, T*, hipStream_t);                      \
+    template void LaunchMultiDot<T, T>(const T*, long, int, const T*, long, double*, int, double*, hipStream_t);    \
+    template void LaunchNorm<T>(const T*, long, double*, int, double*, hipStream_t);                                \
+    template void LaunchMultiAxpy<T, T>(const T*, long, int, const double*, T*, long, double*, hipStream_t);        \
+    template void LaunchScaleCopy<T, T>(const T*, const double*, int, T*, long, hipStream_t);                       \
+    template void LaunchWUpdate<T>(const T*, T*, long, const double*, int, int, const double*, double*, hipStream_t);
+TDBEM_LA_INSTANTIATE(double)
+TDBEM_LA_INSTANTIATE(float)
+#undef TDBEM_LA_INSTANTIATE
+template void LaunchSpmv<float, double, double>(long, const int*, const int*, const float*, const double*, double*,
+                                                hipStream_t);
+template void LaunchPrecond<float, double, float>(long, const long*, const long*, const long*, const long*,
+                                                  const float*, const long*, const long*, const double*, float*,
+                                                  hipStream_t);
 
 // ---------------------------------------------------------------------------
 // State plumbing (DBEM1/multidomain.cpp:3047-3200; dsquareelement.h:88-122)

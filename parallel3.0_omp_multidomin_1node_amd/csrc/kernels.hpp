@@ -68,12 +68,16 @@ struct PwWave {
 };
 void LaunchPwCount(const ElemGeo* geo, PairGrid grid, const int* kb, const int* len, long npairs, int* cnt,
                    hipStream_t s);
+template <typename TB>  // TB: stored block type (double, or float in the mixed-precision mode)
 void LaunchHistoryReg(const ElemGeo* geo, const ElemPoints* pts, const NodeRefs* refs, const double* selfT2,
                       PairGrid grid, const long* coff, const long* off, const int* kb, const int* len, long npairs,
-                      long c0, long c1, double* mts, double* mgs, unsigned char* alive, hipStream_t s);
+                      long c0, long c1, TB* mts, TB* mgs, unsigned short* mtsLo, unsigned short* mgsLo,
+                      unsigned char* alive, hipStream_t s);
+template <typename TB>
 void LaunchHistoryPw(const ElemGeo* geo, const ElemPoints* pts, const NodeRefs* refs, const double* selfT2,
                      PairGrid grid, const PwWave* waves, long w0, long w1, const long* off, const int* kb,
-                     const int* len, double* mts, double* mgs, unsigned char* alive, hipStream_t s);
+                     const int* len, TB* mts, TB* mgs, unsigned short* mtsLo, unsigned short* mgsLo,
+                     unsigned char* alive, hipStream_t s);
 
 // Step-0 blocks of the near pairs listed in pairIdx (column-major, ready for
 // AddBlock): out[2*i] and out[2*i+1] with flags.
@@ -90,39 +94,51 @@ void LaunchConvolution(PairGrid grid, const NodeRefs* refs, const long* off, con
 // steps) into hfar[i][3*ne] (i < Tn), then per step the near part plus hfar
 // into rhs.
 constexpr int kConvTileMax = 16;
-void LaunchConvFar(PairGrid grid, const NodeRefs* refs, const long* off, const int* kb, const int* len,
-                   const double* mts, const double* mgs, const double* gU, const double* gT, int nstore, int K0,
-                   int Tn, int tile, int maxn, int ne, double* hfar, hipStream_t s);
-void LaunchConvNear(const NodeRefs* refs, const long* nearPtr, const long* nearPair, const int* nearField,
-                    const long* off, const int* kb, const int* len, const double* mts, const double* mgs,
-                    const double* gU, const double* gT, int nstore, int K, int K0, int maxn, int ne,
-                    const double* hfar, double* rhs, hipStream_t s);
-
-// Block CCSR SpMV y = A x (row-major 3x3 blocks).
-void LaunchSpmv(long nrows, const int* rp, const int* col, const double* val, const double* x, double* y,
-                hipStream_t s);
-// y = M^{-1} x with the leaf preconditioner.
-void LaunchPrecond(long n, const long* posLeaf, const long* leafBegin, const long* leafSize, const long* matOff,
-                   const double* inv, const long* repid, const long* outBlock, const double* x, double* y,
+// mtsLo / mgsLo (nullable): bf16 residuals of the split FP32 storage.
+template <typename TB>
+void LaunchConvFar(PairGrid grid, const NodeRefs* refs, const long* off, const int* kb, const int* len, const TB* mts,
+                   const TB* mgs, const unsigned short* mtsLo, const unsigned short* mgsLo, const double* gU,
+                   const double* gT, int nstore, int K0, int Tn, int tile, int maxn, int ne, double* hfar,
                    hipStream_t s);
+template <typename TB>
+void LaunchConvNear(const NodeRefs* refs, const long* nearPtr, const long* nearPair, const int* nearField,
+                    const long* off, const int* kb, const int* len, const TB* mts, const TB* mgs,
+                    const unsigned short* mtsLo, const unsigned short* mgsLo, const double* gU, const double* gT,
+                    int nstore, int K, int K0, int maxn, int ne, const double* hfar, double* rhs, hipStream_t s);
 
+// GMRES linear algebra.  TA: matrix / inverse storage, TX / TY: vector
+// storage (double, or float for the inner solves of the mixed mode); every
+// product and sum is formed in FP64.
+// Block CCSR SpMV y = A x (row-major 3x3 blocks).
+template <typename TA, typename TX, typename TY>
+void LaunchSpmv(long nrows, const int* rp, const int* col, const TA* val, const TX* x, TY* y, hipStream_t s);
+// y = M^{-1} x with the leaf preconditioner.
+template <typename TA, typename TX, typename TY>
+void LaunchPrecond(long n, const long* posLeaf, const long* leafBegin, const long* leafSize, const long* matOff,
+                   const TA* inv, const long* repid, const long* outBlock, const TX* x, TY* y, hipStream_t s);
 // Reductions: partial sums of dot products of u with columns 0..nv-1 of V
 // (column stride ldv), finished into h[0..nv).
-void LaunchMultiDot(const double* V, long ldv, int nv, const double* u, long n, double* partial, int nblocks,
-                    double* h, hipStream_t s);
+template <typename TV, typename TU>
+void LaunchMultiDot(const TV* V, long ldv, int nv, const TU* u, long n, double* partial, int nblocks, double* h,
+                    hipStream_t s);
 // u -= V[:, 0:nv] h ; accumulate htot += h when htot != null
-void LaunchMultiAxpy(const double* V, long ldv, int nv, const double* h, double* u, long n, double* htot,
-                     hipStream_t s);
-void LaunchNorm(const double* x, long n, double* partial, int nblocks, double* out, hipStream_t s);
-// y = a*x   and   y = b - y
-void LaunchScaleCopy(const double* x, const double* a, int invert, double* y, long n, hipStream_t s);
+template <typename TV, typename TU>
+void LaunchMultiAxpy(const TV* V, long ldv, int nv, const double* h, TU* u, long n, double* htot, hipStream_t s);
+template <typename T>
+void LaunchNorm(const T* x, long n, double* partial, int nblocks, double* out, hipStream_t s);
+// y = a*x   and   y = b - y   and   x += y
+template <typename TX, typename TY>
+void LaunchScaleCopy(const TX* x, const double* a, int invert, TY* y, long n, hipStream_t s);
 void LaunchResidual(const double* b, double* y, long n, hipStream_t s);
+void LaunchAdd(const double* y, double* x, long n, hipStream_t s);
+void LaunchToFloat(const double* x, float* y, long n, hipStream_t s);
 // GMRES Givens step j (1-based) on device scalars.
 void LaunchGivens(double* h, double* QT, double* R, double* f, double* phibar, int* stag, int j, int restart,
                   hipStream_t s);
 // W_j and x += f_j W_j.
-void LaunchWUpdate(const double* V, double* W, long n, const double* R, int j, int restart, const double* f,
-                   double* x, hipStream_t s);
+template <typename TV>
+void LaunchWUpdate(const TV* V, TV* W, long n, const double* R, int j, int restart, const double* f, double* x,
+                   hipStream_t s);
 
 // Per-step state plumbing.
 void LaunchKnown(const NodeRefs* refs, const double* bd, int ne, double* known, long nknown, hipStream_t s);

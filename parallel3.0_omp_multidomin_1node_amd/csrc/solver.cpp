@@ -385,17 +385,35 @@ void Case::AssembleHistory(int maxn_in)
     h_alive.clear();
     d_mts.free();
     d_mgs.free();
+    d_mts32.free();
+    d_mgs32.free();
+    d_mtsLo.free();
+    d_mgsLo.free();
     d_alive.free();
     historyDone = false;
+    historyMixed = mixed != 0;
+    historySplit = mixed == 2;
+    if (historyMixed && !quadStaged)
+        throw std::runtime_error("the mixed-precision mode needs the staged history quadrature");
     Cull(maxn);
     size_t freeB = 0, totalB = 0;
     Check(hipMemGetInfo(&freeB, &totalB), "hipMemGetInfo");
-    double need = (double)entries * (9.0 * 8 + 6.0 * 8 + 1.0);
+    const double vbytes = historyMixed ? (historySplit ? 6.0 : 4.0) : 8.0;
+    double need = (double)entries * (15.0 * vbytes + 1.0);
     if (need > 0.92 * (double)freeB)
         throw std::runtime_error("history bands need " + std::to_string(need / 1e9) + " GB but only " +
                                  std::to_string((double)freeB / 1e9) + " GB of device memory are free");
-    d_mts.alloc(9 * (size_t)entries);
-    d_mgs.alloc(6 * (size_t)entries);
+    if (historyMixed) {
+        d_mts32.alloc(9 * (size_t)entries);
+        d_mgs32.alloc(6 * (size_t)entries);
+        if (historySplit) {
+            d_mtsLo.alloc(9 * (size_t)entries);
+            d_mgsLo.alloc(6 * (size_t)entries);
+        }
+    } else {
+        d_mts.alloc(9 * (size_t)entries);
+        d_mgs.alloc(6 * (size_t)entries);
+    }
     d_alive.alloc((size_t)entries);
     hipEvent_t a, b;
     Check(hipEventCreate(&a), "event");
@@ -446,12 +464,18 @@ void Case::AssembleHistory(int maxn_in)
         DevBuf<PwWave> d_waves;
         d_waves.upload(waves.data(), waves.size(), stream);
         const long step = 1L << 20;  // wavefronts per launch
-        for (long c0 = 0; c0 < chunks; c0 += step)
-            LaunchHistoryReg(d_geo.p, d_pts.p, d_refs.p, d_t2.p, grid, d_coff.p, d_off.p, d_kb.p, d_len.p, npairs, c0,
-                             std::min(chunks, c0 + step), d_mts.p, d_mgs.p, d_alive.p, stream);
-        for (long w0 = 0; w0 < pwWaves; w0 += step)
-            LaunchHistoryPw(d_geo.p, d_pts.p, d_refs.p, d_t2.p, grid, d_waves.p, w0, std::min(pwWaves, w0 + step),
-                            d_off.p, d_kb.p, d_len.p, d_mts.p, d_mgs.p, d_alive.p, stream);
+        auto launch = [&](auto* mts, auto* mgs) {
+            for (long c0 = 0; c0 < chunks; c0 += step)
+                LaunchHistoryReg(d_geo.p, d_pts.p, d_refs.p, d_t2.p, grid, d_coff.p, d_off.p, d_kb.p, d_len.p, npairs,
+                                 c0, std::min(chunks, c0 + step), mts, mgs, d_mtsLo.p, d_mgsLo.p, d_alive.p, stream);
+            for (long w0 = 0; w0 < pwWaves; w0 += step)
+                LaunchHistoryPw(d_geo.p, d_pts.p, d_refs.p, d_t2.p, grid, d_waves.p, w0, std::min(pwWaves, w0 + step),
+                                d_off.p, d_kb.p, d_len.p, mts, mgs, d_mtsLo.p, d_mgsLo.p, d_alive.p, stream);
+        };
+        if (historyMixed)
+            launch(d_mts32.p, d_mgs32.p);
+        else
+            launch(d_mts.p, d_mgs.p);
         Check(hipStreamSynchronize(stream), "history");
     } else {
         const long chunk = 1L << 24;
@@ -502,8 +526,31 @@ void Case::ExportMatrix(int which, int step, std::vector<int>& rp, std::vector<i
         h_mts.resize(9 * (size_t)entries);
         h_mgs.resize(6 * (size_t)entries);
         h_alive.resize((size_t)entries);
-        Check(hipMemcpy(h_mts.data(), d_mts.p, sizeof(double) * h_mts.size(), hipMemcpyDeviceToHost), "mts");
-        Check(hipMemcpy(h_mgs.data(), d_mgs.p, sizeof(double) * h_mgs.size(), hipMemcpyDeviceToHost), "mgs");
+        if (historyMixed) {
+            std::vector<float> a(h_mts.size()), b(h_mgs.size());
+            Check(hipMemcpy(a.data(), d_mts32.p, sizeof(float) * a.size(), hipMemcpyDeviceToHost), "mts");
+            Check(hipMemcpy(b.data(), d_mgs32.p, sizeof(float) * b.size(), hipMemcpyDeviceToHost), "mgs");
+            std::copy(a.begin(), a.end(), h_mts.begin());
+            std::copy(b.begin(), b.end(), h_mgs.begin());
+            if (historySplit) {  // value = hi + bf16 residual
+                std::vector<unsigned short> la(h_mts.size()), lb(h_mgs.size());
+                Check(hipMemcpy(la.data(), d_mtsLo.p, 2 * la.size(), hipMemcpyDeviceToHost), "mts lo");
+                Check(hipMemcpy(lb.data(), d_mgsLo.p, 2 * lb.size(), hipMemcpyDeviceToHost), "mgs lo");
+                auto lo = [](unsigned short v) {
+                    unsigned int bits = (unsigned int)v << 16;
+                    float f;
+                    std::memcpy(&f, &bits, 4);
+                    return (double)f;
+                };
+                for (size_t i = 0; i < la.size(); ++i)
+                    h_mts[i] += lo(la[i]);
+                for (size_t i = 0; i < lb.size(); ++i)
+                    h_mgs[i] += lo(lb[i]);
+            }
+        } else {
+            Check(hipMemcpy(h_mts.data(), d_mts.p, sizeof(double) * h_mts.size(), hipMemcpyDeviceToHost), "mts");
+            Check(hipMemcpy(h_mgs.data(), d_mgs.p, sizeof(double) * h_mgs.size(), hipMemcpyDeviceToHost), "mgs");
+        }
         Check(hipMemcpy(h_alive.data(), d_alive.p, h_alive.size(), hipMemcpyDeviceToHost), "alive");
     }
     struct Item {
@@ -579,12 +626,38 @@ void Case::PrepareGmres(long restart)
     d_part.alloc((size_t)gm_blocks * (size_t)(restart + 1));
     if (!h_pin)
         Check(hipHostMalloc((void**)&h_pin, 4 * sizeof(double), hipHostMallocDefault), "pinned");
+    gm_mixed = false;
 }
 
-int Case::Gmres(const double* d_b, double* d_x, long restart, double tol, int* iters)
+// FP32 copies of A and the leaf inverses and the FP32 Krylov workspace of the
+// mixed-precision inner solves.
+void Case::PrepareMixed(long restart)
+{
+    if (gm_mixed)
+        return;
+    long n = 3 * ne;
+    d_V32.alloc((size_t)n * (size_t)(restart + 1));
+    d_W32.alloc((size_t)n * (size_t)restart);
+    d_u32.alloc((size_t)n);
+    d_u232.alloc((size_t)n);
+    d_r.alloc((size_t)n);
+    d_dx.alloc((size_t)n);
+    d_Aval32.alloc(d_Aval.n);
+    LaunchToFloat(d_Aval.p, d_Aval32.p, (long)d_Aval.n, stream);
+    d_inv32.alloc(d_inv.n);
+    LaunchToFloat(d_inv.p, d_inv32.p, (long)d_inv.n, stream);
+    Check(hipStreamSynchronize(stream), "mixed setup");
+    gm_mixed = true;
+}
+
+// Left-preconditioned GMRES (DBEM1/pre_gmres.cpp:1369-1605) with the matrix,
+// inverses and Krylov vectors stored as T; reductions, Hessenberg, Givens and
+// the solution x are FP64.  T = double is the parity path.
+template <typename T>
+int Case::GmresCore(const double* d_b, double* d_x, long restart, double tol, int* iters, const T* Aval, const T* inv,
+                    T* V, T* W, T* u, T* u2)
 {
     const long n = 3 * ne;
-    double* V = d_V.p;
     double* normr = d_scal.p;      // [0]
     double* bnorm = d_scal.p + 1;  // [1]
     double* phibar = d_scal.p + 2; // [2]
@@ -598,9 +671,9 @@ int Case::Gmres(const double* d_b, double* d_x, long restart, double tol, int* i
         return 0;
     double tolb = tol * n2b;
     // r = b - A x0 (x0 = 0)
-    LaunchSpmv(ne, d_Arp.p, d_Acol.p, d_Aval.p, d_x, d_vh.p, stream);
+    LaunchSpmv(ne, d_Arp.p, d_Acol.p, Aval, (const double*)d_x, d_vh.p, stream);
     LaunchResidual(d_b, d_vh.p, n, stream);
-    LaunchNorm(d_vh.p, n, d_part.p, gm_blocks, normr, stream);
+    LaunchNorm((const double*)d_vh.p, n, d_part.p, gm_blocks, normr, stream);
     Check(hipMemcpyAsync(h_pin, normr, sizeof(double), hipMemcpyDeviceToHost, stream), "d2h");
     Check(hipStreamSynchronize(stream), "gmres");
     if (h_pin[0] <= tolb)
@@ -608,29 +681,29 @@ int Case::Gmres(const double* d_b, double* d_x, long restart, double tol, int* i
     Check(hipMemsetAsync(d_QT.p, 0, sizeof(double) * d_QT.n, stream), "memset");
     Check(hipMemsetAsync(d_R.p, 0, sizeof(double) * d_R.n, stream), "memset");
     Check(hipMemsetAsync(d_stag.p, 0, sizeof(int), stream), "memset");
-    LaunchPrecond(ne, d_posLeaf.p, d_leafBegin.p, d_leafSize.p, d_matOff.p, d_inv.p, d_repid.p, d_outBlock.p, d_vh.p,
-                  d_u.p, stream);
-    LaunchNorm(d_u.p, n, d_part.p, gm_blocks, d_h.p + 1, stream);
-    LaunchScaleCopy(d_u.p, d_h.p + 1, 1, V, n, stream);
+    LaunchPrecond(ne, d_posLeaf.p, d_leafBegin.p, d_leafSize.p, d_matOff.p, inv, d_repid.p, d_outBlock.p,
+                  (const double*)d_vh.p, u, stream);
+    LaunchNorm((const T*)u, n, d_part.p, gm_blocks, d_h.p + 1, stream);
+    LaunchScaleCopy((const T*)u, d_h.p + 1, 1, V, n, stream);
     const double one = 1.0;
     Check(hipMemcpyAsync(d_QT.p, &one, sizeof(double), hipMemcpyHostToDevice, stream), "qt");
     Check(hipMemcpyAsync(phibar, d_h.p + 1, sizeof(double), hipMemcpyDeviceToDevice, stream), "phibar");
     for (int j = 1; j <= restart; ++j) {
-        double* Vj = V + (size_t)(j - 1) * (size_t)n;
-        LaunchSpmv(ne, d_Arp.p, d_Acol.p, d_Aval.p, Vj, d_u2.p, stream);
-        LaunchPrecond(ne, d_posLeaf.p, d_leafBegin.p, d_leafSize.p, d_matOff.p, d_inv.p, d_repid.p, d_outBlock.p,
-                      d_u2.p, d_u.p, stream);
-        LaunchMultiDot(V, n, j, d_u.p, n, d_part.p, gm_blocks, d_h.p + 1, stream);
-        LaunchMultiAxpy(V, n, j, d_h.p + 1, d_u.p, n, nullptr, stream);
-        LaunchMultiDot(V, n, j, d_u.p, n, d_part.p, gm_blocks, d_h2.p, stream);
-        LaunchMultiAxpy(V, n, j, d_h2.p, d_u.p, n, d_h.p + 1, stream);
-        LaunchNorm(d_u.p, n, d_part.p, gm_blocks, d_h.p + j + 1, stream);
-        LaunchScaleCopy(d_u.p, d_h.p + j + 1, 1, V + (size_t)j * (size_t)n, n, stream);
+        T* Vj = V + (size_t)(j - 1) * (size_t)n;
+        LaunchSpmv(ne, d_Arp.p, d_Acol.p, Aval, (const T*)Vj, u2, stream);
+        LaunchPrecond(ne, d_posLeaf.p, d_leafBegin.p, d_leafSize.p, d_matOff.p, inv, d_repid.p, d_outBlock.p,
+                      (const T*)u2, u, stream);
+        LaunchMultiDot((const T*)V, n, j, (const T*)u, n, d_part.p, gm_blocks, d_h.p + 1, stream);
+        LaunchMultiAxpy((const T*)V, n, j, d_h.p + 1, u, n, nullptr, stream);
+        LaunchMultiDot((const T*)V, n, j, (const T*)u, n, d_part.p, gm_blocks, d_h2.p, stream);
+        LaunchMultiAxpy((const T*)V, n, j, d_h2.p, u, n, d_h.p + 1, stream);
+        LaunchNorm((const T*)u, n, d_part.p, gm_blocks, d_h.p + j + 1, stream);
+        LaunchScaleCopy((const T*)u, d_h.p + j + 1, 1, V + (size_t)j * (size_t)n, n, stream);
         LaunchGivens(d_h.p, d_QT.p, d_R.p, d_f.p, phibar, d_stag.p, j, (int)restart, stream);
-        LaunchWUpdate(V, d_W.p, n, d_R.p, j, (int)restart, d_f.p, d_x, stream);
-        LaunchSpmv(ne, d_Arp.p, d_Acol.p, d_Aval.p, d_x, d_vh.p, stream);
+        LaunchWUpdate((const T*)V, W, n, d_R.p, j, (int)restart, d_f.p, d_x, stream);
+        LaunchSpmv(ne, d_Arp.p, d_Acol.p, Aval, (const double*)d_x, d_vh.p, stream);
         LaunchResidual(d_b, d_vh.p, n, stream);
-        LaunchNorm(d_vh.p, n, d_part.p, gm_blocks, normr, stream);
+        LaunchNorm((const double*)d_vh.p, n, d_part.p, gm_blocks, normr, stream);
         Check(hipMemcpyAsync(h_pin, normr, sizeof(double), hipMemcpyDeviceToHost, stream), "d2h");
         Check(hipMemcpyAsync(h_pin + 3, d_stag.p, sizeof(int), hipMemcpyDeviceToHost, stream), "d2h");
         Check(hipStreamSynchronize(stream), "gmres iteration");
@@ -645,6 +718,61 @@ int Case::Gmres(const double* d_b, double* d_x, long restart, double tol, int* i
     return 1;
 }
 
+template int Case::GmresCore<double>(const double*, double*, long, double, int*, const double*, const double*,
+                                     double*, double*, double*, double*);
+template int Case::GmresCore<float>(const double*, double*, long, double, int*, const float*, const float*, float*,
+                                    float*, float*, float*);
+
+int Case::Gmres(const double* d_b, double* d_x, long restart, double tol, int* iters)
+{
+    return GmresCore<double>(d_b, d_x, restart, tol, iters, d_Aval.p, d_inv.p, d_V.p, d_W.p, d_u.p, d_u2.p);
+}
+
+// Mixed precision: FP32-stored inner GMRES (GmresCore<float>) on A d = r
+// under an FP64 iterative-refinement residual r = b - A x with the FP64 A,
+// until ||r|| <= tol ||b|| (the reference's stopping test,
+// DBEM1/pre_gmres.cpp:1572).  Returns 0, 1 (no convergence within kMaxRefine
+// rounds) or 3 (stagnation); iters = total inner iterations.
+int Case::GmresMixed(const double* d_b, double* d_x, long restart, double tol, int* iters)
+{
+    constexpr int kMaxRefine = 12;
+    constexpr double kInnerTol = 1e-6;
+    const long n = 3 * ne;
+    *iters = 0;
+    double* scal = d_scal.p + 3;
+    LaunchNorm(d_b, n, d_part.p, gm_blocks, scal, stream);
+    Check(hipMemcpyAsync(h_pin, scal, sizeof(double), hipMemcpyDeviceToHost, stream), "d2h");
+    Check(hipStreamSynchronize(stream), "gmres");
+    double n2b = h_pin[0];
+    LaunchZero(d_x, n, stream);
+    if (n2b == 0.0)
+        return 0;
+    Check(hipMemcpyAsync(d_r.p, d_b, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, stream), "r");
+    double normr = n2b;
+    int total = 0;
+    for (int round = 0; round < kMaxRefine; ++round) {
+        double itol = std::max(kInnerTol, tol * n2b / normr);
+        int it = 0;
+        int f = GmresCore<float>(d_r.p, d_dx.p, restart, itol, &it, d_Aval32.p, d_inv32.p, d_V32.p, d_W32.p, d_u32.p,
+                                 d_u232.p);
+        total += it;
+        LaunchAdd(d_dx.p, d_x, n, stream);
+        LaunchSpmv(ne, d_Arp.p, d_Acol.p, (const double*)d_Aval.p, (const double*)d_x, d_r.p, stream);
+        LaunchResidual(d_b, d_r.p, n, stream);
+        LaunchNorm((const double*)d_r.p, n, d_part.p, gm_blocks, scal, stream);
+        Check(hipMemcpyAsync(h_pin, scal, sizeof(double), hipMemcpyDeviceToHost, stream), "d2h");
+        Check(hipStreamSynchronize(stream), "refinement");
+        double prev = normr;
+        normr = h_pin[0];
+        *iters = total;
+        if (normr <= tol * n2b)
+            return 0;
+        if (f == 3 || normr >= 0.9 * prev)
+            return 3;
+    }
+    return 1;
+}
+
 void Case::Solve(int nstep, const double* known_lu, long restart, double tol, double* lu, double* lt, int* iters,
                  int* flags, int* solved)
 {
@@ -653,6 +781,10 @@ void Case::Solve(int nstep, const double* known_lu, long restart, double tol, do
     if (!historyDone || maxn < 1)
         throw std::runtime_error("history not assembled");
     PrepareGmres(restart);
+    if (historyMixed)
+        PrepareMixed(restart);
+    if (historyMixed && !convTiled)
+        throw std::runtime_error("the mixed-precision mode needs the time-blocked convolution");
     if (convTile != 8 && convTile != 16)
         throw std::runtime_error("convolution tile must be 8 or 16");
     if (convTiled && nearTile != convTile)
@@ -694,8 +826,15 @@ void Case::Solve(int nstep, const double* known_lu, long restart, double tol, do
                     K0 = K;
                     int Tn = std::min(convTile, nstep - K0 + 1);
                     Check(hipEventRecord(f0, stream), "event");
-                    LaunchConvFar(grid, d_refs.p, d_off.p, d_kb.p, d_len.p, d_mts.p, d_mgs.p, d_gU.p, d_gT.p, nstore,
-                                  K0, Tn, convTile, maxn, (int)ne, d_hfar.p, stream);
+                    if (historyMixed)
+                        LaunchConvFar(grid, d_refs.p, d_off.p, d_kb.p, d_len.p, (const float*)d_mts32.p,
+                                      (const float*)d_mgs32.p, (const unsigned short*)d_mtsLo.p,
+                                      (const unsigned short*)d_mgsLo.p, d_gU.p, d_gT.p, nstore, K0, Tn, convTile,
+                                      maxn, (int)ne, d_hfar.p, stream);
+                    else
+                        LaunchConvFar(grid, d_refs.p, d_off.p, d_kb.p, d_len.p, (const double*)d_mts.p,
+                                      (const double*)d_mgs.p, nullptr, nullptr, d_gU.p, d_gT.p, nstore, K0, Tn,
+                                      convTile, maxn, (int)ne, d_hfar.p, stream);
                     Check(hipEventRecord(f1, stream), "event");
                     Check(hipEventSynchronize(f1), "far");
                     float fms = 0.f;
@@ -703,9 +842,15 @@ void Case::Solve(int nstep, const double* known_lu, long restart, double tol, do
                     farTime += fms * 1e-3;
                     ++farLaunches;
                 }
-                LaunchConvNear(d_refs.p, d_nearPtr.p, d_nearPair.p, d_nearField.p, d_off.p, d_kb.p, d_len.p,
-                               d_mts.p, d_mgs.p, d_gU.p, d_gT.p, nstore, K, K0, maxn, (int)ne, d_hfar.p, d_rhs.p,
-                               stream);
+                if (historyMixed)
+                    LaunchConvNear(d_refs.p, d_nearPtr.p, d_nearPair.p, d_nearField.p, d_off.p, d_kb.p, d_len.p,
+                                   (const float*)d_mts32.p, (const float*)d_mgs32.p, (const unsigned short*)d_mtsLo.p,
+                                   (const unsigned short*)d_mgsLo.p, d_gU.p, d_gT.p, nstore, K, K0, maxn, (int)ne,
+                                   d_hfar.p, d_rhs.p, stream);
+                else
+                    LaunchConvNear(d_refs.p, d_nearPtr.p, d_nearPair.p, d_nearField.p, d_off.p, d_kb.p, d_len.p,
+                                   (const double*)d_mts.p, (const double*)d_mgs.p, nullptr, nullptr, d_gU.p, d_gT.p,
+                                   nstore, K, K0, maxn, (int)ne, d_hfar.p, d_rhs.p, stream);
             } else {
                 LaunchConvolution(grid, d_refs.p, d_off.p, d_kb.p, d_len.p, d_mts.p, d_mgs.p, d_gU.p, d_gT.p, nstore,
                                   K, maxn, (int)ne, d_rhs.p, stream);
@@ -718,7 +863,8 @@ void Case::Solve(int nstep, const double* known_lu, long restart, double tol, do
         }
         double g0 = Now();
         int it = 0;
-        int flag = Gmres(d_rhs.p, d_x.p, restart, tol, &it);
+        int flag = historyMixed ? GmresMixed(d_rhs.p, d_x.p, restart, tol, &it)
+                                : Gmres(d_rhs.p, d_x.p, restart, tol, &it);
         gmresTime += Now() - g0;
         iters[K - 1] = it;
         flags[K - 1] = flag;

@@ -53,6 +53,8 @@ public:
     bool convTiled = true;   // time-blocked convolution (false: one full pass per step)
     int convTile = 16;       // steps per far-part tile (8 or 16); fixes the near lists built by the cull
     bool quadStaged = true;  // wave-per-pair quadrature with LDS-staged point geometry (false: lane per entry)
+    int mixed = 0;  // for the next history assembly: 0 FP64, 1 FP32 blocks, 2 FP32 + bf16 residual blocks
+    bool historyMixed = false, historySplit = false;
     hipStream_t stream = nullptr;
 
     DevBuf<double> d_u1, d_t1, d_t2;
@@ -62,7 +64,12 @@ private:
     void BuildNearLists(int tile);
     bool StructureValid() const;
     void PrepareGmres(long restart);
+    void PrepareMixed(long restart);
     int Gmres(const double* d_b, double* d_x, long restart, double tol, int* iters);
+    template <typename T>
+    int GmresCore(const double* d_b, double* d_x, long restart, double tol, int* iters, const T* Aval, const T* inv,
+                  T* V, T* W, T* u, T* u2);
+    int GmresMixed(const double* d_b, double* d_x, long restart, double tol, int* iters);
 
     bool step0Done = false, historyDone = false;
     int culledFor = 0;
@@ -82,6 +89,8 @@ private:
     std::vector<int> h_kb, h_len;
     std::vector<long> h_off;
     DevBuf<double> d_mts, d_mgs;
+    DevBuf<float> d_mts32, d_mgs32;  // mixed precision: FP32 band blocks
+    DevBuf<unsigned short> d_mtsLo, d_mgsLo;  // and their bf16 residuals (mixed == 2)
     DevBuf<unsigned char> d_alive;
     std::vector<double> h_mts, h_mgs;
     std::vector<unsigned char> h_alive;
@@ -94,6 +103,9 @@ private:
     int gm_blocks = 1;
     DevBuf<double> d_V, d_W, d_u, d_u2, d_vh, d_h, d_h2, d_QT, d_R, d_f, d_scal, d_part;
     DevBuf<int> d_stag;
+    bool gm_mixed = false;
+    DevBuf<float> d_V32, d_W32, d_u32, d_u232, d_Aval32, d_inv32;
+    DevBuf<double> d_r, d_dx;
     double* h_pin = nullptr;
 };
 

@@ -235,3 +235,41 @@ def test_staged_quadrature_matches_per_entry(oracle_mod, which):
             for x, y in ((a, b), (a, o)):
                 assert np.array_equal(x[0], y[0]) and np.array_equal(x[1], y[1]), (which_m, k)
                 assert np.array_equal(x[2], y[2]), (which_m, k)
+
+
+def test_mixed_precision_error_against_fp64():
+    """Mixed precision (blocks computed in FP64 and stored in FP32, mode 1, or
+    as FP32 + bf16 residual, mode 2; FP32-stored inner GMRES under an FP64
+    refinement residual): stored blocks are the FP64 blocks rounded to FP32
+    (mode 1) or within 2^-30 of them (mode 2), and displacements and tractions
+    agree with the FP64 run to <= 1e-6 relative (mode 1) and <= 1e-8 (mode 2),
+    all solved to tol 1e-12."""
+    tdbem = _tdbem()
+    from tdbem import mesh
+    w = mesh.layered_config2(h=0.25, nstep=60)
+    m = w.mesh
+    print("case:", w.name, "mixed vs fp64", flush=True)
+    out = {}
+    for mixed in (0, 1, 2):
+        gc = tdbem.Case(m.points, m.ele, m.bcid, w.dt, w.gap, domain_count=4, mat_rows=w.mats,
+                        itf_rows=m.interfaces)
+        gc.set_option("mixed", mixed)
+        gc.assemble_step0(10)
+        gc.assemble_history(w.nstep)
+        U, T, it, fl, solved, _ = gc.solve(w.known, w.nstep, tol=1e-12)
+        assert solved == w.nstep and not fl.any()
+        out[mixed] = (U, T, [gc.ccsr(which, k) for which in (2, 3) for k in (1, 7, 30)])
+        gc.close()
+    U0, T0, B0 = out[0]
+    for mixed, bound in ((1, 1e-6), (2, 1e-8)):
+        U1, T1, B1 = out[mixed]
+        for a, b in zip(B0, B1):
+            assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+            if mixed == 1:
+                assert np.array_equal(a[2].astype(np.float32).astype(np.float64), b[2])
+            else:
+                assert np.all(np.abs(a[2] - b[2]) <= 2.0 ** -30 * np.abs(a[2]))
+        eu = np.abs(U1 - U0).max() / np.abs(U0).max()
+        et = np.abs(T1 - T0).max() / np.abs(T0).max()
+        print("mixed", mixed, "vs fp64: max rel err U", eu, "T", et, flush=True)
+        assert eu <= bound and et <= bound, mixed

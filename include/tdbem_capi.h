@@ -78,6 +78,16 @@ int tdbem_self_tables(tdbem_ctx* ctx, double* u1, double* t1, double* t2);
 int tdbem_solve(tdbem_ctx* ctx, int nstep, const double* known_lu, long restart, double tol, double* lu,
                 double* lt, int* iters, int* flags, int* solved, double* times);
 
+/* Batched load cases over the same bands (no counterpart in the reference,
+ * which solves one load history per run; each case follows the tdbem_solve
+ * time loop above): nrhs histories known_lu[nrhs][nstep+1][ne*3] give
+ * lu, lt [nrhs][nstep+1][ne*3], iters / flags [nrhs][nstep], solved[nrhs].
+ * The GMRES of all cases run in lockstep; a case whose GMRES flag is != 0
+ * stops there (its later steps are zero) while the others continue.
+ * times[3] as for tdbem_solve. */
+int tdbem_solve_batched(tdbem_ctx* ctx, int nstep, int nrhs, const double* known_lu, long restart, double tol,
+                        double* lu, double* lt, int* iters, int* flags, int* solved, double* times);
+
 /* Convenience: full DynaGMRESSolverMultiDomainCCSR (DBEM1/multidomain.cpp:
  * 3664-3926): step 0, history for maxn = nstep, time loop.  Returns 1 when
  * the final GMRES flag is 0 and -1 otherwise through *result. */

@@ -1,0 +1,269 @@
+// Batched load cases on one context: B right-hand-side histories share the
+// history bands, the step-0 matrices and the preconditioner; every load case
+// follows the reference's time loop (DBEM1/multidomain.cpp:3793-3926) with
+// its own GMRES (DBEM1/pre_gmres.cpp:1369-1605) run in lockstep with the
+// others.
+#include "batched.hpp"
+#include "solver.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <stdexcept>
+
+namespace tdbem {
+
+namespace {
+
+void CheckB(hipError_t e, const char* what)
+{
+    if (e != hipSuccess)
+        throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+double NowB()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+// Lockstep GMRES for the B systems A x_b = rhs_b.  Each case stops at its own
+// convergence (x frozen, iteration count and flag recorded); the loop ends
+// when no case is active.
+void Case::GmresBatched(int B, const double* rhs, double* x, long restart, double tol, int* its, int* flg)
+{
+    const long n = 3 * ne;
+    const int nblk = (int)std::min<long>(64, std::max<long>(1, (n + 1023) / 1024));
+    BatchWork& w = bw;
+    std::vector<double> bn((size_t)B), nr((size_t)B);
+    std::vector<int> act((size_t)B), stag((size_t)B);
+    for (int b = 0; b < B; ++b) {
+        its[b] = 0;
+        flg[b] = 0;
+    }
+    LaunchNormB(rhs, n, B, w.part.p, nblk, w.scal.p, stream);
+    CheckB(hipMemcpyAsync(bn.data(), w.scal.p, sizeof(double) * (size_t)B, hipMemcpyDeviceToHost, stream), "d2h");
+    LaunchZero(x, n * B, stream);
+    // r = b - A x0 with x0 = 0
+    LaunchSpmmB(ne, d_Arp.p, d_Acol.p, d_Aval.p, x, w.vh.p, B, stream);
+    LaunchResidualB(rhs, w.vh.p, n, B, stream);
+    LaunchNormB(w.vh.p, n, B, w.part.p, nblk, w.scal.p + B, stream);
+    CheckB(hipMemcpyAsync(nr.data(), w.scal.p + B, sizeof(double) * (size_t)B, hipMemcpyDeviceToHost, stream), "d2h");
+    CheckB(hipStreamSynchronize(stream), "gmres");
+    int nact = 0;
+    for (int b = 0; b < B; ++b) {
+        act[(size_t)b] = (bn[(size_t)b] != 0.0 && !(nr[(size_t)b] <= tol * bn[(size_t)b])) ? 1 : 0;
+        nact += act[(size_t)b];
+    }
+    if (nact == 0)
+        return;
+    CheckB(hipMemcpyAsync(w.active.p, act.data(), sizeof(int) * (size_t)B, hipMemcpyHostToDevice, stream), "active");
+    CheckB(hipMemsetAsync(w.QT.p, 0, sizeof(double) * w.QT.n, stream), "memset");
+    CheckB(hipMemsetAsync(w.R.p, 0, sizeof(double) * w.R.n, stream), "memset");
+    CheckB(hipMemsetAsync(w.stag.p, 0, sizeof(int) * (size_t)B, stream), "memset");
+    LaunchPrecondB(ne, d_posLeaf.p, d_leafBegin.p, d_leafSize.p, d_matOff.p, d_inv.p, d_repid.p, d_outBlock.p,
+                   w.vh.p, w.u.p, B, stream);
+    double* h = w.h.p;  // h[k * B + b], k = 0 .. restart + 1
+    LaunchNormB(w.u.p, n, B, w.part.p, nblk, h + B, stream);
+    LaunchScaleB(w.u.p, h + B, w.V.p, n, B, stream);
+    {
+        std::vector<double> one((size_t)B, 1.0);
+        CheckB(hipMemcpyAsync(w.QT.p, one.data(), sizeof(double) * (size_t)B, hipMemcpyHostToDevice, stream), "qt");
+        CheckB(hipMemcpyAsync(w.phibar.p, h + B, sizeof(double) * (size_t)B, hipMemcpyDeviceToDevice, stream),
+               "phibar");
+        CheckB(hipStreamSynchronize(stream), "gmres");
+    }
+    for (int j = 1; j <= restart; ++j) {
+        const double* Vj = w.V.p + (size_t)(j - 1) * (size_t)n * (size_t)B;
+        LaunchSpmmB(ne, d_Arp.p, d_Acol.p, d_Aval.p, Vj, w.u2.p, B, stream);
+        LaunchPrecondB(ne, d_posLeaf.p, d_leafBegin.p, d_leafSize.p, d_matOff.p, d_inv.p, d_repid.p, d_outBlock.p,
+                       w.u2.p, w.u.p, B, stream);
+        LaunchMultiDotB(w.V.p, n, j, w.u.p, B, w.part.p, nblk, h + B, stream);
+        LaunchMultiAxpyB(w.V.p, n, j, h + B, w.u.p, B, nullptr, stream);
+        LaunchMultiDotB(w.V.p, n, j, w.u.p, B, w.part.p, nblk, w.h2.p, stream);
+        LaunchMultiAxpyB(w.V.p, n, j, w.h2.p, w.u.p, B, h + B, stream);
+        LaunchNormB(w.u.p, n, B, w.part.p, nblk, h + (size_t)(j + 1) * B, stream);
+        LaunchScaleB(w.u.p, h + (size_t)(j + 1) * B, w.V.p + (size_t)j * (size_t)n * (size_t)B, n, B, stream);
+        LaunchGivensB(h, w.QT.p, w.R.p, w.f.p, w.phibar.p, w.stag.p, w.active.p, j, (int)restart, B, stream);
+        LaunchWUpdateB(w.V.p, w.W.p, n, w.R.p, j, (int)restart, w.f.p, x, w.active.p, B, stream);
+        LaunchSpmmB(ne, d_Arp.p, d_Acol.p, d_Aval.p, x, w.vh.p, B, stream);
+        LaunchResidualB(rhs, w.vh.p, n, B, stream);
+        LaunchNormB(w.vh.p, n, B, w.part.p, nblk, w.scal.p + B, stream);
+        CheckB(hipMemcpyAsync(nr.data(), w.scal.p + B, sizeof(double) * (size_t)B, hipMemcpyDeviceToHost, stream),
+               "d2h");
+        CheckB(hipMemcpyAsync(stag.data(), w.stag.p, sizeof(int) * (size_t)B, hipMemcpyDeviceToHost, stream), "d2h");
+        CheckB(hipStreamSynchronize(stream), "gmres iteration");
+        bool changed = false;
+        nact = 0;
+        for (int b = 0; b < B; ++b) {
+            if (!act[(size_t)b])
+                continue;
+            its[b] = j;
+            if (nr[(size_t)b] <= tol * bn[(size_t)b]) {
+                act[(size_t)b] = 0;
+                changed = true;
+            } else if (stag[(size_t)b]) {
+                act[(size_t)b] = 0;
+                flg[b] = 3;
+                changed = true;
+            } else {
+                ++nact;
+            }
+        }
+        if (nact == 0)
+            return;
+        if (changed)
+            CheckB(hipMemcpyAsync(w.active.p, act.data(), sizeof(int) * (size_t)B, hipMemcpyHostToDevice, stream),
+                   "active");
+    }
+    for (int b = 0; b < B; ++b)
+        if (act[(size_t)b])
+            flg[b] = 1;
+}
+
+void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, double tol, double* lu, double* lt,
+                        int* iters, int* flags, int* solved)
+{
+    if (B < 1)
+        throw std::runtime_error("batch size must be >= 1");
+    if (!step0Done)
+        AssembleStep0(10);
+    if (!historyDone || maxn < 1)
+        throw std::runtime_error("history not assembled");
+    if (nearTile != kBatchTile)
+        BuildNearLists(kBatchTile);
+    const long n = 3 * ne;
+    const int nstore = nstep + 1;
+    const long histU = dof.historyUBlockCount, histT = dof.historyTBlockCount;
+    double t0 = NowB();
+    BatchWork& w = bw;
+    w.V.alloc((size_t)n * (size_t)(restart + 1) * (size_t)B);
+    w.W.alloc((size_t)n * (size_t)restart * (size_t)B);
+    w.u.alloc((size_t)n * B);
+    w.u2.alloc((size_t)n * B);
+    w.vh.alloc((size_t)n * B);
+    w.h.alloc((size_t)(restart + 2) * B);
+    w.h2.alloc((size_t)(restart + 2) * B);
+    w.QT.alloc((size_t)(restart + 1) * (size_t)(restart + 1) * B);
+    w.R.alloc((size_t)restart * (size_t)restart * B);
+    w.f.alloc((size_t)(restart + 2) * B);
+    w.phibar.alloc((size_t)B);
+    w.scal.alloc(2 * (size_t)B);
+    w.stag.alloc((size_t)B);
+    w.active.alloc((size_t)B);
+    w.part.alloc((size_t)64 * (size_t)(restart + 1) * B);
+    DevBuf<double> d_bd, d_lu, d_lt, d_gU, d_gT, d_FU, d_FT, d_known, d_rhs, d_x, d_hfar;
+    d_bd.upload(known_lu, (size_t)B * nstore * (size_t)n, stream);
+    d_lu.alloc((size_t)B * nstore * (size_t)n);
+    d_lt.alloc((size_t)B * nstore * (size_t)n);
+    d_gU.alloc((size_t)histU * 3 * nstore * B);
+    d_gT.alloc((size_t)histT * 3 * nstore * B);
+    d_FU.alloc(d_gU.n);
+    d_FT.alloc(d_gT.n);
+    d_known.alloc(3 * (size_t)std::max<long>(1, dof.knownBlockCount) * B);
+    d_rhs.alloc((size_t)n * B);
+    d_x.alloc((size_t)n * B);
+    d_hfar.alloc((size_t)kBatchTile * (size_t)n * B);
+    for (DevBuf<double>* z : {&d_lu, &d_lt, &d_gU, &d_gT, &d_FU, &d_FT, &d_known})
+        LaunchZero(z->p, (long)z->n, stream);
+    const unsigned short* mLo = historySplit ? d_mtsLo.p : nullptr;
+    const unsigned short* gLo = historySplit ? d_mgsLo.p : nullptr;
+    std::vector<int> it((size_t)B), fl((size_t)B);
+    std::vector<char> alive((size_t)B, 1);
+    for (int b = 0; b < B; ++b)
+        solved[b] = 0;
+    double convTime = 0.0, gmresTime = 0.0;
+    int K0 = 0;
+    for (int K = 1; K <= nstep; ++K) {
+        LaunchKnownB(d_refs.p, d_bd.p, (int)ne, nstore, K, B, d_known.p, stream);
+        LaunchSpmmB(ne, d_Krp.p, d_Kcol.p, d_Kval.p, d_known.p, d_rhs.p, B, stream);
+        if (K > 1) {
+            double c0 = NowB();
+            if (K == 2 || K - K0 >= kBatchTile) {
+                int prev = K0;
+                K0 = K;
+                int x0 = prev == 0 ? 0 : prev - 1;
+                LaunchFilterB(d_gU.p, histU, nstore, B, x0, K0, K0, d_FU.p, stream);
+                LaunchFilterB(d_gT.p, histT, nstore, B, x0, K0, K0, d_FT.p, stream);
+                int Tn = std::min(kBatchTile, nstep - K0 + 1);
+                if (historyMixed)
+                    LaunchConvFarB(grid, d_refs.p, d_off.p, d_kb.p, d_len.p, (const float*)d_mts32.p,
+                                   (const float*)d_mgs32.p, mLo, gLo, d_FU.p, d_FT.p, nstore, K0, Tn, maxn, (int)ne, B,
+                                   d_hfar.p, stream);
+                else
+                    LaunchConvFarB(grid, d_refs.p, d_off.p, d_kb.p, d_len.p, (const double*)d_mts.p,
+                                   (const double*)d_mgs.p, nullptr, nullptr, d_FU.p, d_FT.p, nstore, K0, Tn, maxn,
+                                   (int)ne, B, d_hfar.p, stream);
+            }
+            if (historyMixed)
+                LaunchConvNearB(d_refs.p, d_nearPtr.p, d_nearPair.p, d_nearField.p, d_off.p, d_kb.p, d_len.p,
+                                (const float*)d_mts32.p, (const float*)d_mgs32.p, mLo, gLo, d_gU.p, d_gT.p, nstore, K,
+                                K0, maxn, (int)ne, B, d_hfar.p, d_rhs.p, stream);
+            else
+                LaunchConvNearB(d_refs.p, d_nearPtr.p, d_nearPair.p, d_nearField.p, d_off.p, d_kb.p, d_len.p,
+                                (const double*)d_mts.p, (const double*)d_mgs.p, nullptr, nullptr, d_gU.p, d_gT.p,
+                                nstore, K, K0, maxn, (int)ne, B, d_hfar.p, d_rhs.p, stream);
+            CheckB(hipStreamSynchronize(stream), "batched convolution");
+            convTime += NowB() - c0;
+        }
+        double g0 = NowB();
+        GmresBatched(B, d_rhs.p, d_x.p, restart, tol, it.data(), fl.data());
+        gmresTime += NowB() - g0;
+        bool any = false;
+        for (int b = 0; b < B; ++b) {
+            if (!alive[(size_t)b])
+                continue;
+            iters[(size_t)b * nstep + (K - 1)] = it[(size_t)b];
+            flags[(size_t)b * nstep + (K - 1)] = fl[(size_t)b];
+            solved[b] = K;
+            if (fl[(size_t)b] != 0)
+                alive[(size_t)b] = 0;
+            else
+                any = true;
+        }
+        LaunchScatterB(d_refs.p, d_geo.p, d_bd.p, d_x.p, (int)ne, nstore, K, B, d_lu.p, d_lt.p, stream);
+        LaunchStateB(d_refs.p, d_lu.p, d_lt.p, (int)ne, nstore, K, B, d_gU.p, d_gT.p, stream);
+        if (!any)
+            break;
+    }
+    CheckB(hipStreamSynchronize(stream), "batched solve");
+    CheckB(hipGetLastError(), "batched solve launch");
+    CheckB(hipMemcpy(lu, d_lu.p, sizeof(double) * d_lu.n, hipMemcpyDeviceToHost), "lu");
+    CheckB(hipMemcpy(lt, d_lt.p, sizeof(double) * d_lt.n, hipMemcpyDeviceToHost), "lt");
+    // a case's steps after its GMRES failure are left zero, as a single solve leaves them
+    for (int b = 0; b < B; ++b)
+        for (int K = solved[b] + 1; K <= nstep; ++K) {
+            std::fill_n(lu + ((size_t)b * nstore + K) * n, n, 0.0);
+            std::fill_n(lt + ((size_t)b * nstore + K) * n, n, 0.0);
+        }
+    // step 0 of each case: lt = 0, Convert (lu <-> lt for BCID 456), LocToAbs
+    for (int b = 0; b < B; ++b) {
+        const double* in = known_lu + (size_t)b * nstore * n;
+        double* ou = lu + (size_t)b * nstore * n;
+        double* ot = lt + (size_t)b * nstore * n;
+        for (long i = 0; i < ne; ++i) {
+            double u[3], t[3];
+            for (int k = 0; k < 3; ++k) {
+                u[k] = in[3 * i + k];
+                t[k] = 0.0;
+            }
+            if (bcid[(size_t)i] == 456)
+                for (int k = 0; k < 3; ++k)
+                    std::swap(u[k], t[k]);
+            const double* F = geo[(size_t)i].trans;
+            for (int r = 0; r < 3; ++r) {
+                ou[3 * i + r] = F[r] * u[0] + F[3 + r] * u[1] + F[6 + r] * u[2];
+                ot[3 * i + r] = F[r] * t[0] + F[3 + r] * t[1] + F[6 + r] * t[2];
+            }
+        }
+    }
+    timing[4] = convTime;
+    timing[5] = gmresTime;
+    timing[6] = 0.0;
+    timing[7] = 0.0;
+    (void)t0;
+    bw.release();
+}
+
+}  // namespace tdbem

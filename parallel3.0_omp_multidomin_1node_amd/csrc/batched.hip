@@ -1,0 +1,658 @@
+// HIP kernels of the batched load-case solve for gfx950.  The load case is
+// the lane index: a wavefront carries up to 64 load cases of one scalar, so
+// every matrix block, band entry and preconditioner value is wave-uniform
+// (scalar loads, SGPR operands) and each lane streams only its own vectors.
+#include "batched.hpp"
+
+#pragma clang fp contract(off)
+
+namespace tdbem {
+
+namespace {
+
+__device__ __forceinline__ double Bf16Lo(unsigned short lo)
+{
+    return (double)__uint_as_float(((unsigned int)lo) << 16);
+}
+
+__device__ __forceinline__ int WaveIndex() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Per-step plumbing (DBEM1/multidomain.cpp:3047-3200 for each load case)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_known_b(const NodeRefs* __restrict__ refs, const double* __restrict__ bd,
+                                                 int ne, int nstore, int K, int B, double* __restrict__ known)
+{
+    long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)ne * B)
+        return;
+    int i = (int)(idx / B), b = (int)(idx % B);
+    const NodeRefs& r = refs[i];
+    if (r.surface == 1)
+        return;
+    int blk = r.bcid == 123 ? r.u_block : r.t_block;
+    int kn = r.bcid == 123 ? r.u_known : r.t_known;
+    double sg = r.bcid == 123 ? r.u_sign : r.t_sign;
+    if (!kn)
+        return;
+    const double* in = bd + ((long)b * nstore + K) * 3 * ne + 3 * (long)i;
+    for (int c = 0; c < 3; ++c)
+        known[(3 * (long)blk + c) * B + b] = sg * in[c];
+}
+
+void LaunchKnownB(const NodeRefs* refs, const double* bd, int ne, int nstore, int K, int B, double* known,
+                  hipStream_t s)
+{
+    long n = (long)ne * B;
+    k_known_b<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(refs, bd, ne, nstore, K, B, known);
+}
+
+__global__ void __launch_bounds__(256) k_scatter_b(const NodeRefs* __restrict__ refs, const ElemGeo* __restrict__ geo,
+                                                   const double* __restrict__ bd, const double* __restrict__ x,
+                                                   int ne, int nstore, int K, int B, double* __restrict__ lu,
+                                                   double* __restrict__ lt)
+{
+    long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)ne * B)
+        return;
+    int i = (int)(idx / B), b = (int)(idx % B);
+    const NodeRefs& r = refs[i];
+    const double* T = r.itf_T;
+    long slice = ((long)b * nstore + K) * 3 * ne;
+    const double* in = bd + slice + 3 * (long)i;
+    double u[3], t[3], v[3];
+    if (!r.u_known) {
+        for (int q = 0; q < 3; ++q)
+            v[q] = x[(3 * (long)r.u_block + q) * B + b];
+        for (int q = 0; q < 3; ++q)
+            u[q] = r.u_sign * (T[q * 3 + 0] * v[0] + T[q * 3 + 1] * v[1] + T[q * 3 + 2] * v[2]);
+    } else {
+        for (int q = 0; q < 3; ++q)
+            u[q] = r.u_sign * in[q];
+    }
+    if (!r.t_known) {
+        for (int q = 0; q < 3; ++q)
+            v[q] = x[(3 * (long)r.t_block + q) * B + b];
+        for (int q = 0; q < 3; ++q)
+            t[q] = r.t_sign * (T[q * 3 + 0] * v[0] + T[q * 3 + 1] * v[1] + T[q * 3 + 2] * v[2]);
+    } else {
+        for (int q = 0; q < 3; ++q)
+            t[q] = r.t_sign * in[q];
+    }
+    const double* F = geo[i].trans;
+    double* ou = lu + slice + 3 * (long)i;
+    double* ot = lt + slice + 3 * (long)i;
+    ou[0] = F[0] * u[0] + F[3] * u[1] + F[6] * u[2];
+    ou[1] = F[1] * u[0] + F[4] * u[1] + F[7] * u[2];
+    ou[2] = F[2] * u[0] + F[5] * u[1] + F[8] * u[2];
+    ot[0] = F[0] * t[0] + F[3] * t[1] + F[6] * t[2];
+    ot[1] = F[1] * t[0] + F[4] * t[1] + F[7] * t[2];
+    ot[2] = F[2] * t[0] + F[5] * t[1] + F[8] * t[2];
+}
+
+void LaunchScatterB(const NodeRefs* refs, const ElemGeo* geo, const double* bd, const double* x, int ne, int nstore,
+                    int K, int B, double* lu, double* lt, hipStream_t s)
+{
+    long n = (long)ne * B;
+    k_scatter_b<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(refs, geo, bd, x, ne, nstore, K, B, lu, lt);
+}
+
+__global__ void __launch_bounds__(256) k_state_b(const NodeRefs* __restrict__ refs, const double* __restrict__ lu,
+                                                 const double* __restrict__ lt, int ne, int nstore, int K, int B,
+                                                 double* __restrict__ gU, double* __restrict__ gT)
+{
+    long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)ne * B)
+        return;
+    int i = (int)(idx / B), b = (int)(idx % B);
+    const NodeRefs& r = refs[i];
+    if (r.surface == 1 && !r.ref_side)
+        return;
+    long slice = ((long)b * nstore + K) * 3 * ne + 3 * (long)i;
+    for (int c = 0; c < 3; ++c) {
+        gU[(((long)r.hu_block * nstore + K) * 3 + c) * B + b] = r.hu_sign * lu[slice + c];
+        gT[(((long)r.ht_block * nstore + K) * 3 + c) * B + b] = r.ht_sign * lt[slice + c];
+    }
+}
+
+void LaunchStateB(const NodeRefs* refs, const double* lu, const double* lt, int ne, int nstore, int K, int B,
+                  double* gU, double* gT, hipStream_t s)
+{
+    long n = (long)ne * B;
+    k_state_b<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(refs, lu, lt, ne, nstore, K, B, gU, gT);
+}
+
+__global__ void __launch_bounds__(256) k_filter_b(const double* __restrict__ g, long cols, int nstore, int B, int x0,
+                                                  int nx, int K0, double* __restrict__ F)
+{
+    long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long per = (long)nx * 3 * B;
+    if (idx >= cols * per)
+        return;
+    long col = idx / per;
+    long rem = idx - col * per;
+    int x = x0 + (int)(rem / (3 * B));
+    long cb = rem % (3 * B);  // c * B + b
+    auto raw = [&](int m) {
+        return (m >= 1 && m <= K0 - 1 && m < nstore) ? g[(col * nstore + m) * 3 * B + cb] : 0.0;
+    };
+    F[(col * nstore + x) * 3 * B + cb] = (raw(x - 1) + 2.0 * raw(x)) + raw(x + 1);
+}
+
+void LaunchFilterB(const double* g, long cols, int nstore, int B, int x0, int x1, int K0, double* F, hipStream_t s)
+{
+    if (x0 < 0)
+        x0 = 0;
+    if (x1 > nstore - 1)
+        x1 = nstore - 1;
+    int nx = x1 - x0 + 1;
+    if (nx <= 0 || cols <= 0)
+        return;
+    long n = cols * nx * 3 * B;
+    k_filter_b<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(g, cols, nstore, B, x0, nx, K0, F);
+}
+
+// ---------------------------------------------------------------------------
+// y = A x for B vectors: one wavefront per block row, one lane per load case,
+// the reference's per-block summation (DBEM1/mat_vec.cpp:2273-2300).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_spmm_b(long nrows, const int* __restrict__ rp, const int* __restrict__ col,
+                                                const double* __restrict__ val, const double* __restrict__ x,
+                                                double* __restrict__ y, int B)
+{
+    long row = (long)blockIdx.x * 4 + WaveIndex();
+    int b = blockIdx.y * 64 + (threadIdx.x & 63);
+    if (row >= nrows || b >= B)
+        return;
+    double b0 = 0.0, b1 = 0.0, b2 = 0.0;
+    for (int j = rp[row]; j < rp[row + 1]; ++j) {
+        const double* v = val + 9 * (long)j;
+        long c3 = 3 * (long)col[j];
+        double x0 = x[c3 * B + b], x1 = x[(c3 + 1) * B + b], x2 = x[(c3 + 2) * B + b];
+        b0 += v[0] * x0 + v[1] * x1 + v[2] * x2;
+        b1 += v[3] * x0 + v[4] * x1 + v[5] * x2;
+        b2 += v[6] * x0 + v[7] * x1 + v[8] * x2;
+    }
+    y[(3 * row) * B + b] = b0;
+    y[(3 * row + 1) * B + b] = b1;
+    y[(3 * row + 2) * B + b] = b2;
+}
+
+void LaunchSpmmB(long nrows, const int* rp, const int* col, const double* val, const double* x, double* y, int B,
+                 hipStream_t s)
+{
+    dim3 grid((unsigned)((nrows + 3) / 4), (unsigned)((B + 63) / 64));
+    k_spmm_b<<<grid, 256, 0, s>>>(nrows, rp, col, val, x, y, B);
+}
+
+// ---------------------------------------------------------------------------
+// Batched time-blocked convolution.  Far part: one workgroup of four
+// wavefronts per (source row, 64 load cases); a wavefront takes every fourth
+// pair of the row.  The blocks
+// of up to 64 lags of a pair are staged in LDS with coalesced loads and read
+// back as wave-uniform broadcasts.  Each lane keeps the filtered states u~, t~
+// of x = K0 - j .. K0 - j + T - 1 in a register window whose slots rotate with
+// the lag (the slot of x is fixed per lag within a group of T lags), so a new
+// lag needs one state (6 values, loaded one lag ahead) and applies its block
+// to all T steps (18 FMA per step).
+// ---------------------------------------------------------------------------
+template <int T, typename TB>
+__global__ void __launch_bounds__(256) k_conv_far_b(PairGrid grid, const NodeRefs* __restrict__ refs,
+                                                    const long* __restrict__ off, const int* __restrict__ kb,
+                                                    const int* __restrict__ len, const TB* __restrict__ mts,
+                                                    const TB* __restrict__ mgs,
+                                                    const unsigned short* __restrict__ mtsLo,
+                                                    const unsigned short* __restrict__ mgsLo,
+                                                    const double* __restrict__ FU, const double* __restrict__ FT,
+                                                    int nstore, int K0, int Tn, int maxn, int ne, int B,
+                                                    double* __restrict__ hfar)
+{
+    constexpr int CH = 64;  // lags per staged chunk (a multiple of T)
+    static_assert(CH % T == 0, "chunk must hold whole lag groups");
+    __shared__ double sblk[4][CH * 15];
+    __shared__ double red[4][T * 3][64];
+    int wave = WaveIndex();
+    int s = blockIdx.x;
+    int lane = threadIdx.x & 63;
+    int b = blockIdx.y * 64 + lane;
+    bool live = b < B;
+    int bb = live ? b : B - 1;
+    int d = refs[s].domain;
+    int nd = grid.eleCount[d];
+    int eb = grid.eleBegin[d];
+    long rowStart = grid.pairBase[d] + (long)(s - eb) * nd;
+    double* sb = sblk[wave];
+    int Kmax = K0 + Tn - 1;
+    int jmax = Kmax < maxn ? Kmax : maxn;
+    double acc[T][3];
+#pragma unroll
+    for (int i = 0; i < T; ++i)
+        acc[i][0] = acc[i][1] = acc[i][2] = 0.0;
+    for (int q = wave; q < nd; q += 4) {
+        long p = rowStart + q;
+        int L = len[p];
+        int k0 = kb[p];
+        int jend = k0 + L - 1 < jmax ? k0 + L - 1 : jmax;
+        if (L == 0 || k0 > jend)
+            continue;
+        int f = eb + q;
+        const double* fu = FU + (long)refs[f].hu_block * nstore * 3 * B + bb;
+        const double* ft = FT + (long)refs[f].ht_block * nstore * 3 * B + bb;
+        long base = off[p];
+        auto loadx = [&](int x, double* w) {
+            if (x >= 0 && x < nstore) {
+                const double* a = fu + (long)x * 3 * B;
+                const double* c = ft + (long)x * 3 * B;
+                w[0] = a[0];
+                w[1] = a[B];
+                w[2] = a[2 * B];
+                w[3] = c[0];
+                w[4] = c[B];
+                w[5] = c[2 * B];
+            } else {
+                for (int k = 0; k < 6; ++k)
+                    w[k] = 0.0;
+            }
+        };
+        double win[T][6], tmp[6];
+#pragma unroll
+        for (int qq = 0; qq < T; ++qq)
+            loadx(K0 - k0 + qq, win[qq]);
+        for (int c0 = k0; c0 <= jend; c0 += CH) {
+            int c1 = c0 + CH - 1 < jend ? c0 + CH - 1 : jend;
+            int nlag = c1 - c0 + 1;
+            long e0 = base + (c0 - k0);
+            for (int idx = lane; idx < nlag * 9; idx += 64) {
+                int r = idx / 9, k = idx - 9 * r;
+                double v = (double)mts[9 * e0 + idx];
+                if (mtsLo)
+                    v += Bf16Lo(mtsLo[9 * e0 + idx]);
+                sb[r * 15 + k] = v;
+            }
+            for (int idx = lane; idx < nlag * 6; idx += 64) {
+                int r = idx / 6, k = idx - 6 * r;
+                double v = (double)mgs[6 * e0 + idx];
+                if (mgsLo)
+                    v += Bf16Lo(mgsLo[6 * e0 + idx]);
+                sb[r * 15 + 9 + k] = v;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            for (int jg = c0; jg <= c1; jg += T) {
+#pragma unroll
+                for (int u = 0; u < T; ++u) {
+                    int j = jg + u;
+                    if (j <= c1) {
+                        if (u > 0 || jg > k0) {
+#pragma unroll
+                            for (int k = 0; k < 6; ++k)
+                                win[(T - u) % T][k] = tmp[k];
+                        }
+                        loadx(K0 - j - 1, tmp);  // the state lag j + 1 adds
+                        const double* bl = sb + (j - c0) * 15;
+                        double M[9], G[6];
+#pragma unroll
+                        for (int k = 0; k < 9; ++k)
+                            M[k] = bl[k];
+#pragma unroll
+                        for (int k = 0; k < 6; ++k)
+                            G[k] = bl[9 + k];
+                        // all T steps (one basic block; steps past Tn are never stored)
+#pragma unroll
+                        for (int i = 0; i < T; ++i) {
+                            const double* w = win[(i - u + T) % T];
+                            double a0 = acc[i][0], a1 = acc[i][1], a2 = acc[i][2];
+                            a0 = fma(G[0], w[3], a0); a0 = fma(G[1], w[4], a0); a0 = fma(G[2], w[5], a0);
+                            a1 = fma(G[1], w[3], a1); a1 = fma(G[3], w[4], a1); a1 = fma(G[4], w[5], a1);
+                            a2 = fma(G[2], w[3], a2); a2 = fma(G[4], w[4], a2); a2 = fma(G[5], w[5], a2);
+                            a0 = fma(-M[0], w[0], a0); a0 = fma(-M[1], w[1], a0); a0 = fma(-M[2], w[2], a0);
+                            a1 = fma(-M[3], w[0], a1); a1 = fma(-M[4], w[1], a1); a1 = fma(-M[5], w[2], a1);
+                            a2 = fma(-M[6], w[0], a2); a2 = fma(-M[7], w[1], a2); a2 = fma(-M[8], w[2], a2);
+                            acc[i][0] = a0;
+                            acc[i][1] = a1;
+                            acc[i][2] = a2;
+                        }
+                    }
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < T; ++i)
+        for (int c = 0; c < 3; ++c)
+            red[wave][i * 3 + c][lane] = acc[i][c];
+    __syncthreads();
+    for (int idx = wave; idx < Tn * 3; idx += 4) {
+        double v = ((red[0][idx][lane] + red[1][idx][lane]) + red[2][idx][lane]) + red[3][idx][lane];
+        int i = idx / 3, c = idx - 3 * i;
+        if (live)
+            hfar[((long)i * 3 * ne + 3 * (long)s + c) * B + b] = v;
+    }
+}
+
+template <typename TB>
+void LaunchConvFarB(PairGrid grid, const NodeRefs* refs, const long* off, const int* kb, const int* len,
+                    const TB* mts, const TB* mgs, const unsigned short* mtsLo, const unsigned short* mgsLo,
+                    const double* FU, const double* FT, int nstore, int K0, int Tn, int maxn, int ne, int B,
+                    double* hfar, hipStream_t s)
+{
+    dim3 g((unsigned)ne, (unsigned)((B + 63) / 64));
+    k_conv_far_b<kBatchTile, TB><<<g, 256, 0, s>>>(grid, refs, off, kb, len, mts, mgs, mtsLo, mgsLo, FU, FT, nstore,
+                                                   K0, Tn, maxn, ne, B, hfar);
+}
+
+// Near part of step K (states K0 .. K-1 inside the tile) plus the tile's far
+// part: one wavefront per (source row, 64 load cases), the near pairs'
+// lags summed per lane in order.
+template <typename TB>
+__global__ void __launch_bounds__(64) k_conv_near_b(const NodeRefs* __restrict__ refs,
+                                                    const long* __restrict__ nearPtr,
+                                                    const long* __restrict__ nearPair,
+                                                    const int* __restrict__ nearField, const long* __restrict__ off,
+                                                    const int* __restrict__ kb, const int* __restrict__ len,
+                                                    const TB* __restrict__ mts, const TB* __restrict__ mgs,
+                                                    const unsigned short* __restrict__ mtsLo,
+                                                    const unsigned short* __restrict__ mgsLo,
+                                                    const double* __restrict__ gU, const double* __restrict__ gT,
+                                                    int nstore, int K, int K0, int maxn, int ne, int B,
+                                                    const double* __restrict__ hfar, double* __restrict__ rhs)
+{
+    int s = blockIdx.x;
+    int b = blockIdx.y * 64 + threadIdx.x;
+    bool live = b < B;
+    int bb = live ? b : B - 1;
+    int jlim = K - K0 + 1;
+    if (jlim > maxn)
+        jlim = maxn;
+    double y[3] = {0.0, 0.0, 0.0};
+    for (long q = nearPtr[s]; q < nearPtr[s + 1]; ++q) {
+        long p = nearPair[q];
+        int k0 = kb[p], L = len[p];
+        int f = nearField[q];
+        const double* su = gU + (long)refs[f].hu_block * nstore * 3 * B + bb;
+        const double* st = gT + (long)refs[f].ht_block * nstore * 3 * B + bb;
+        for (int t = 0; t < L; ++t) {
+            int j = k0 + t;
+            if (j > jlim)
+                break;
+            long e = off[p] + t;
+            double M[9], G[6];
+            for (int i = 0; i < 9; ++i)
+                M[i] = (double)mts[9 * e + i] + (mtsLo ? Bf16Lo(mtsLo[9 * e + i]) : 0.0);
+            for (int i = 0; i < 6; ++i)
+                G[i] = (double)mgs[6 * e + i] + (mgsLo ? Bf16Lo(mgsLo[6 * e + i]) : 0.0);
+            const int mm[3] = {K - j + 1, K - j, K - j - 1};
+            const double co[3] = {1.0, 2.0, 1.0};
+            double cu[3] = {0.0, 0.0, 0.0}, ct[3] = {0.0, 0.0, 0.0};
+            for (int z = 0; z < 3; ++z) {
+                int m = mm[z];
+                if (m >= K0 && m <= K - 1 && m >= 1) {
+                    for (int c = 0; c < 3; ++c) {
+                        cu[c] += co[z] * su[((long)m * 3 + c) * B];
+                        ct[c] += co[z] * st[((long)m * 3 + c) * B];
+                    }
+                }
+            }
+            y[0] += (G[0] * ct[0] + G[1] * ct[1] + G[2] * ct[2]) - (M[0] * cu[0] + M[1] * cu[1] + M[2] * cu[2]);
+            y[1] += (G[1] * ct[0] + G[3] * ct[1] + G[4] * ct[2]) - (M[3] * cu[0] + M[4] * cu[1] + M[5] * cu[2]);
+            y[2] += (G[2] * ct[0] + G[4] * ct[1] + G[5] * ct[2]) - (M[6] * cu[0] + M[7] * cu[1] + M[8] * cu[2]);
+        }
+    }
+    if (!live)
+        return;
+    const double* hf = hfar + ((long)(K - K0) * 3 * ne + 3 * (long)s) * B + b;
+    for (int c = 0; c < 3; ++c)
+        rhs[(3 * (long)s + c) * B + b] += hf[(long)c * B] + y[c];
+}
+
+template <typename TB>
+void LaunchConvNearB(const NodeRefs* refs, const long* nearPtr, const long* nearPair, const int* nearField,
+                     const long* off, const int* kb, const int* len, const TB* mts, const TB* mgs,
+                     const unsigned short* mtsLo, const unsigned short* mgsLo, const double* gU, const double* gT,
+                     int nstore, int K, int K0, int maxn, int ne, int B, const double* hfar, double* rhs,
+                     hipStream_t s)
+{
+    dim3 g((unsigned)ne, (unsigned)((B + 63) / 64));
+    k_conv_near_b<TB><<<g, 64, 0, s>>>(refs, nearPtr, nearPair, nearField, off, kb, len, mts, mgs, mtsLo, mgsLo, gU,
+                                       gT, nstore, K, K0, maxn, ne, B, hfar, rhs);
+}
+
+#define TDBEM_BATCH_INSTANTIATE(TB)                                                                                  \
+    template void LaunchConvFarB<TB>(PairGrid, const NodeRefs*, const long*, const int*, const int*, const TB*,       \
+                                     const TB*, const unsigned short*, const unsigned short*, const double*,          \
+                                     const double*, int, int, int, int, int, int, double*, hipStream_t);              \
+    template void LaunchConvNearB<TB>(const NodeRefs*, const long*, const long*, const int*, const long*, const int*, \
+                                      const int*, const TB*, const TB*, const unsigned short*,                        \
+                                      const unsigned short*, const double*, const double*, int, int, int, int, int,   \
+                                      int, const double*, double*, hipStream_t);
+TDBEM_BATCH_INSTANTIATE(double)
+TDBEM_BATCH_INSTANTIATE(float)
+#undef TDBEM_BATCH_INSTANTIATE
+
+// ---------------------------------------------------------------------------
+// Batched GMRES linear algebra.  Vectors of a batch: (r, b) at r * B + b.
+// Per-load-case scalars (Hessenberg h, Givens QT / R, f, phibar, flags):
+// entry k of case b at k * B + b.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_precond_b(long n, const long* __restrict__ posLeaf,
+                                                   const long* __restrict__ leafBegin,
+                                                   const long* __restrict__ leafSize, const long* __restrict__ matOff,
+                                                   const double* __restrict__ inv, const long* __restrict__ repid,
+                                                   const long* __restrict__ outBlock, const double* __restrict__ x,
+                                                   double* __restrict__ y, int B)
+{
+    long t = (long)blockIdx.x * 4 + WaveIndex();
+    int b = blockIdx.y * 64 + (threadIdx.x & 63);
+    if (t >= 3 * n || b >= B)
+        return;
+    long i = t / 3;
+    int mrow = (int)(t % 3);
+    long leaf = posLeaf[i];
+    long lb = leafBegin[leaf], c = leafSize[leaf], jl = i - lb, n3 = 3 * c;
+    const double* P = inv + matOff[leaf] + (3 * jl + mrow) * n3;
+    double acc = 0.0;
+    for (long k = 0; k < c; ++k) {
+        long r3 = 3 * repid[lb + k];
+        acc += P[3 * k] * x[r3 * B + b];
+        acc += P[3 * k + 1] * x[(r3 + 1) * B + b];
+        acc += P[3 * k + 2] * x[(r3 + 2) * B + b];
+    }
+    y[(3 * outBlock[i] + mrow) * B + b] = acc;
+}
+
+void LaunchPrecondB(long n, const long* posLeaf, const long* leafBegin, const long* leafSize, const long* matOff,
+                    const double* inv, const long* repid, const long* outBlock, const double* x, double* y, int B,
+                    hipStream_t s)
+{
+    dim3 g((unsigned)((3 * n + 3) / 4), (unsigned)((B + 63) / 64));
+    k_precond_b<<<g, 256, 0, s>>>(n, posLeaf, leafBegin, leafSize, matOff, inv, repid, outBlock, x, y, B);
+}
+
+// Partial dot products: workgroup x (four wavefronts) sums rows
+// r = 4 x + w, 4 x + w + 4 gridDim.x, ... of each column k.
+__global__ void __launch_bounds__(256) k_multidot_b(const double* __restrict__ V, long n, int nv,
+                                                    const double* __restrict__ u, int B, double* __restrict__ partial)
+{
+    __shared__ double red[4][64];
+    int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int b = blockIdx.y * 64 + lane;
+    bool live = b < B;
+    int bb = live ? b : B - 1;
+    long stride = (long)gridDim.x * 4;
+    for (int k = 0; k < nv; ++k) {
+        const double* Vk = V + (long)k * n * B;
+        double a = 0.0;
+        for (long r = (long)blockIdx.x * 4 + w; r < n; r += stride)
+            a += Vk[r * B + bb] * u[r * B + bb];
+        red[w][lane] = a;
+        __syncthreads();
+        if (w == 0 && live)
+            partial[((long)blockIdx.x * nv + k) * B + b] = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+        __syncthreads();
+    }
+}
+
+__global__ void k_finish_b(const double* __restrict__ partial, int nblocks, int nv, int B, double* __restrict__ h,
+                           int doSqrt)
+{
+    long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)nv * B)
+        return;
+    double a = 0.0;
+    for (int x = 0; x < nblocks; ++x)
+        a += partial[(long)x * nv * B + idx];
+    h[idx] = doSqrt ? sqrt(a) : a;
+}
+
+void LaunchMultiDotB(const double* V, long n, int nv, const double* u, int B, double* partial, int nblocks,
+                     double* h, hipStream_t s)
+{
+    dim3 g((unsigned)nblocks, (unsigned)((B + 63) / 64));
+    k_multidot_b<<<g, 256, 0, s>>>(V, n, nv, u, B, partial);
+    long m = (long)nv * B;
+    k_finish_b<<<(unsigned)((m + 255) / 256), 256, 0, s>>>(partial, nblocks, nv, B, h, 0);
+}
+
+void LaunchNormB(const double* x, long n, int B, double* partial, int nblocks, double* out, hipStream_t s)
+{
+    dim3 g((unsigned)nblocks, (unsigned)((B + 63) / 64));
+    k_multidot_b<<<g, 256, 0, s>>>(x, n, 1, x, B, partial);
+    k_finish_b<<<(unsigned)((B + 255) / 256), 256, 0, s>>>(partial, nblocks, 1, B, out, 1);
+}
+
+__global__ void __launch_bounds__(256) k_multiaxpy_b(const double* __restrict__ V, long n, int nv,
+                                                     const double* __restrict__ h, double* __restrict__ u, int B,
+                                                     double* htot)
+{
+    long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (htot && idx < (long)nv * B)
+        htot[idx] += h[idx];
+    if (idx >= n * B)
+        return;
+    int b = (int)(idx % B);
+    double a = u[idx];
+    for (int k = 0; k < nv; ++k)
+        a -= h[(long)k * B + b] * V[(long)k * n * B + idx];
+    u[idx] = a;
+}
+
+void LaunchMultiAxpyB(const double* V, long n, int nv, const double* h, double* u, int B, double* htot, hipStream_t s)
+{
+    long m = n * B > (long)nv * B ? n * B : (long)nv * B;
+    k_multiaxpy_b<<<(unsigned)((m + 255) / 256), 256, 0, s>>>(V, n, nv, h, u, B, htot);
+}
+
+__global__ void __launch_bounds__(256) k_scale_b(const double* __restrict__ x, const double* __restrict__ a,
+                                                 double* __restrict__ y, long n, int B)
+{
+    long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n * B)
+        return;
+    double s = a[idx % B];
+    y[idx] = s <= 1.0e-12 ? 0.0 : x[idx] / s;
+}
+
+void LaunchScaleB(const double* x, const double* a, double* y, long n, int B, hipStream_t s)
+{
+    k_scale_b<<<(unsigned)((n * B + 255) / 256), 256, 0, s>>>(x, a, y, n, B);
+}
+
+__global__ void __launch_bounds__(256) k_residual_b(const double* __restrict__ rhs, double* __restrict__ y, long m)
+{
+    long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < m)
+        y[idx] = rhs[idx] - y[idx];
+}
+
+void LaunchResidualB(const double* rhs, double* y, long n, int B, hipStream_t s)
+{
+    k_residual_b<<<(unsigned)((n * B + 255) / 256), 256, 0, s>>>(rhs, y, n * B);
+}
+
+// Givens rotation in QT form for column j of every active load case
+// (DBEM1/pre_gmres.cpp:1492-1533); one lane per case.
+__global__ void __launch_bounds__(64) k_givens_b(double* h, double* QT, double* R, double* f, double* phibar,
+                                                 int* stag, const int* __restrict__ active, int j, int restart, int B)
+{
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B || !active[b])
+        return;
+    const int lq = restart + 1;
+#define QQ(a, c) QT[((long)((a)-1) * lq + ((c)-1)) * B + b]
+#define RR(a, c) R[((long)((a)-1) * restart + ((c)-1)) * B + b]
+#define HH(a) h[(long)(a) * B + b]
+    for (int k = 1; k <= j; ++k) {
+        double v = 0.0;
+        for (int l = 1; l <= j; ++l)
+            v += QQ(k, l) * HH(l);
+        RR(k, j) = v;
+    }
+    double rt = RR(j, j), c, s, temp;
+    double hj1 = HH(j + 1);
+    if (hj1 == 0.0) {
+        c = 1.0;
+        s = 0.0;
+    } else if (hj1 * hj1 > rt * rt) {
+        temp = rt / hj1;
+        s = 1.0 / sqrt(1.0 + fabs(temp) * fabs(temp));
+        c = -temp * s;
+    } else {
+        temp = hj1 / rt;
+        c = 1.0 / sqrt(1.0 + fabs(temp) * fabs(temp));
+        s = -temp * c;
+    }
+    RR(j, j) = c * rt - s * hj1;
+    for (int k = 1; k <= j; ++k) {
+        double qk = QQ(j, k);
+        QQ(j, k) = c * qk;
+        QQ(j + 1, k) = s * qk;
+    }
+    QQ(j, j + 1) = -s;
+    QQ(j + 1, j + 1) = c;
+    f[(long)j * B + b] = c * phibar[b];
+    phibar[b] = s * phibar[b];
+    if (j < restart && f[(long)j * B + b] == 0.0)
+        stag[b] = 1;
+#undef QQ
+#undef RR
+#undef HH
+}
+
+void LaunchGivensB(double* h, double* QT, double* R, double* f, double* phibar, int* stag, const int* active, int j,
+                   int restart, int B, hipStream_t s)
+{
+    k_givens_b<<<(unsigned)((B + 63) / 64), 64, 0, s>>>(h, QT, R, f, phibar, stag, active, j, restart, B);
+}
+
+__global__ void __launch_bounds__(256) k_wupdate_b(const double* __restrict__ V, double* __restrict__ W, long n,
+                                                   const double* __restrict__ R, int j, int restart,
+                                                   const double* __restrict__ f, double* __restrict__ x,
+                                                   const int* __restrict__ active, int B)
+{
+    long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n * B)
+        return;
+    int b = (int)(idx % B);
+    if (!active[b])
+        return;
+    double rjj = R[((long)(j - 1) * restart + (j - 1)) * B + b];
+    double w = V[(long)(j - 1) * n * B + idx] / rjj;
+    for (int l = 1; l <= j - 1; ++l)
+        w -= W[(long)(l - 1) * n * B + idx] * R[((long)(l - 1) * restart + (j - 1)) * B + b] / rjj;
+    W[(long)(j - 1) * n * B + idx] = w;
+    x[idx] += f[(long)j * B + b] * w;
+}
+
+void LaunchWUpdateB(const double* V, double* W, long n, const double* R, int j, int restart, const double* f,
+                    double* x, const int* active, int B, hipStream_t s)
+{
+    k_wupdate_b<<<(unsigned)((n * B + 255) / 256), 256, 0, s>>>(V, W, n, R, j, restart, f, x, active, B);
+}
+
+}  // namespace tdbem

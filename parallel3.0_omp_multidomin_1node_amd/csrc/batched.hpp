@@ -1,0 +1,64 @@
+// Batched load cases: B right-hand-side histories solved together over the
+// same history bands, step-0 matrices and preconditioner.  Device vectors of
+// a batch keep the load case fastest: element (r, b) of an n-vector batch is
+// at r * B + b, so the 64 lanes of a wavefront read 64 load cases of one
+// scalar with one coalesced access while matrix blocks are wave-uniform.
+#pragma once
+
+#include "kernels.hpp"
+
+namespace tdbem {
+
+constexpr int kBatchTile = 8;  // steps per far-part tile of the batched convolution
+
+// Per-step plumbing.  bd / lu / lt are [b][K][3 ne] (host layout).
+void LaunchKnownB(const NodeRefs* refs, const double* bd, int ne, int nstore, int K, int B, double* known,
+                  hipStream_t s);
+void LaunchScatterB(const NodeRefs* refs, const ElemGeo* geo, const double* bd, const double* x, int ne, int nstore,
+                    int K, int B, double* lu, double* lt, hipStream_t s);
+// raw state series g[((col * nstore + K) * 3 + c) * B + b]
+void LaunchStateB(const NodeRefs* refs, const double* lu, const double* lt, int ne, int nstore, int K, int B,
+                  double* gU, double* gT, hipStream_t s);
+// 1-2-1 filtered state series for x in [x0, x1] with the solved states m in
+// [1, K0 - 1]: F[x] = (g[x-1] + 2 g[x]) + g[x+1].
+void LaunchFilterB(const double* g, long cols, int nstore, int B, int x0, int x1, int K0, double* F, hipStream_t s);
+
+// y = A x for B vectors (block CCSR, row-major 3x3 blocks).
+void LaunchSpmmB(long nrows, const int* rp, const int* col, const double* val, const double* x, double* y, int B,
+                 hipStream_t s);
+
+// Far part of the tile [K0, K0+Tn) for B load cases into hfar[(i*3ne + r)*B + b].
+template <typename TB>
+void LaunchConvFarB(PairGrid grid, const NodeRefs* refs, const long* off, const int* kb, const int* len,
+                    const TB* mts, const TB* mgs, const unsigned short* mtsLo, const unsigned short* mgsLo,
+                    const double* FU, const double* FT, int nstore, int K0, int Tn, int maxn, int ne, int B,
+                    double* hfar, hipStream_t s);
+// Near part of step K plus the tile's far part into rhs.
+template <typename TB>
+void LaunchConvNearB(const NodeRefs* refs, const long* nearPtr, const long* nearPair, const int* nearField,
+                     const long* off, const int* kb, const int* len, const TB* mts, const TB* mgs,
+                     const unsigned short* mtsLo, const unsigned short* mgsLo, const double* gU, const double* gT,
+                     int nstore, int K, int K0, int maxn, int ne, int B, const double* hfar, double* rhs,
+                     hipStream_t s);
+
+// Batched GMRES linear algebra (every reduction per load case, FP64).
+void LaunchPrecondB(long n, const long* posLeaf, const long* leafBegin, const long* leafSize, const long* matOff,
+                    const double* inv, const long* repid, const long* outBlock, const double* x, double* y, int B,
+                    hipStream_t s);
+// h[k*B + b] = <V_k(:, b), u(:, b)> for k < nv (V_k at V + k*n*B); partial
+// holds nblocks * nv * B doubles.
+void LaunchMultiDotB(const double* V, long n, int nv, const double* u, int B, double* partial, int nblocks,
+                     double* h, hipStream_t s);
+void LaunchNormB(const double* x, long n, int B, double* partial, int nblocks, double* out, hipStream_t s);
+// u -= sum_k h[k] V_k ; htot[k] += h[k] when htot != null
+void LaunchMultiAxpyB(const double* V, long n, int nv, const double* h, double* u, int B, double* htot,
+                      hipStream_t s);
+// y(:, b) = x(:, b) / a[b] (a <= 1e-12 gives 0)
+void LaunchScaleB(const double* x, const double* a, double* y, long n, int B, hipStream_t s);
+void LaunchResidualB(const double* rhs, double* y, long n, int B, hipStream_t s);
+void LaunchGivensB(double* h, double* QT, double* R, double* f, double* phibar, int* stag, const int* active, int j,
+                   int restart, int B, hipStream_t s);
+void LaunchWUpdateB(const double* V, double* W, long n, const double* R, int j, int restart, const double* f,
+                    double* x, const int* active, int B, hipStream_t s);
+
+}  // namespace tdbem

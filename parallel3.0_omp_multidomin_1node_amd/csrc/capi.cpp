@@ -186,6 +186,25 @@ int tdbem_solve(tdbem_ctx* ctx, int nstep, const double* known_lu, long restart,
     return 0;
 }
 
+int tdbem_solve_batched(tdbem_ctx* ctx, int nstep, int nrhs, const double* known_lu, long restart, double tol,
+                        double* lu, double* lt, int* iters, int* flags, int* solved, double* times)
+{
+    if (!ctx || nstep < 1 || nrhs < 1 || !known_lu || !lu || !lt || !iters || !flags || !solved || restart < 1 ||
+        !(tol > 0.0))
+        return Fail("tdbem_solve_batched: invalid arguments");
+    try {
+        ctx->c.SolveBatched(nstep, nrhs, known_lu, restart, tol, lu, lt, iters, flags, solved);
+    } catch (const std::exception& e) {
+        return Fail("tdbem_solve_batched", e);
+    }
+    if (times) {
+        times[0] = ctx->c.timing[4];
+        times[1] = ctx->c.timing[5];
+        times[2] = ctx->c.timing[4] + ctx->c.timing[5];
+    }
+    return 0;
+}
+
 int tdbem_dyna_gmres_solver(tdbem_ctx* ctx, long iterations, double error, long maxleafpointnum, int nstep,
                             const double* known_lu, double* lu, double* lt, int* result)
 {

@@ -1873,8 +1873,10 @@ __global__ void __launch_bounds__(256) k_conv_far(PairGrid grid, const NodeRefs*
                 int x0 = K0 - (k0 + t) - xlo;  // filtered index of step K0
 #pragma unroll
                 for (int i = 0; i < T; ++i) {
-                    if (i < Tn) {
-                        // acc += MGS t~ - MTS u~, fused multiply-adds straight into the accumulator
+                    {
+                        // acc += MGS t~ - MTS u~, fused multiply-adds straight into the accumulator;
+                        // all T steps in one block (the staged series is zero past the tile, and
+                        // steps past Tn are never stored)
                         const double* w = fw + 6 * (x0 + i);
                         double u0 = w[0], u1 = w[1], u2 = w[2], t0v = w[3], t1v = w[4], t2v = w[5];
                         double a0 = acc[i][0], a1 = acc[i][1], a2 = acc[i][2];

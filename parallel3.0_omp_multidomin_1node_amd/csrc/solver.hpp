@@ -23,6 +23,19 @@ struct DevBuf {
     void upload(const T* h, size_t count, hipStream_t s);
 };
 
+// Workspace of the lockstep GMRES of a batch of load cases.
+struct BatchWork {
+    DevBuf<double> V, W, u, u2, vh, h, h2, QT, R, f, phibar, scal, part;
+    DevBuf<int> stag, active;
+    void release()
+    {
+        for (DevBuf<double>* b : {&V, &W, &u, &u2, &vh, &h, &h2, &QT, &R, &f, &phibar, &scal, &part})
+            b->free();
+        stag.free();
+        active.free();
+    }
+};
+
 class Case {
 public:
     ~Case();
@@ -32,6 +45,10 @@ public:
     void AssembleHistory(int maxn);
     void Solve(int nstep, const double* known_lu, long restart, double tol, double* lu, double* lt, int* iters,
                int* flags, int* solved);
+    // B load cases over the same bands: known_lu, lu, lt are [b][NStep+1][3N],
+    // iters / flags [b][NStep], solved [b].
+    void SolveBatched(int nstep, int B, const double* known_lu, long restart, double tol, double* lu, double* lt,
+                      int* iters, int* flags, int* solved);
     void HistoryStats(long* counts) const;
     void ExportMatrix(int which, int step, std::vector<int>& rp, std::vector<int>& col, std::vector<double>& val);
 
@@ -70,6 +87,8 @@ private:
     int GmresCore(const double* d_b, double* d_x, long restart, double tol, int* iters, const T* Aval, const T* inv,
                   T* V, T* W, T* u, T* u2);
     int GmresMixed(const double* d_b, double* d_x, long restart, double tol, int* iters);
+    void GmresBatched(int B, const double* rhs, double* x, long restart, double tol, int* its, int* flg);
+    BatchWork bw;
 
     bool step0Done = false, historyDone = false;
     int culledFor = 0;

@@ -22,7 +22,8 @@ _lp = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
 
 EXPORTED = [
     "tdbem_last_error", "tdbem_create", "tdbem_destroy", "tdbem_dofmap_export", "tdbem_assemble_step0",
-    "tdbem_assemble_history", "tdbem_ccsr_export", "tdbem_self_tables", "tdbem_solve", "tdbem_dyna_gmres_solver",
+    "tdbem_assemble_history", "tdbem_ccsr_export", "tdbem_self_tables", "tdbem_solve", "tdbem_solve_batched",
+    "tdbem_dyna_gmres_solver",
     "tdbem_timings", "tdbem_history_stats", "tdbem_device_synchronize",
     "tdbem_set_option",
 ]
@@ -55,6 +56,8 @@ def lib():
         L.tdbem_self_tables.argtypes = [C.c_void_p, _dp, _dp, _dp]
         L.tdbem_solve.argtypes = [C.c_void_p, C.c_int, _dp, C.c_long, C.c_double, _dp, _dp, _ip, _ip,
                                   C.POINTER(C.c_int), _dp]
+        L.tdbem_solve_batched.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, C.c_long, C.c_double, _dp, _dp, _ip,
+                                          _ip, _ip, _dp]
         L.tdbem_dyna_gmres_solver.argtypes = [C.c_void_p, C.c_long, C.c_double, C.c_long, C.c_int, _dp, _dp, _dp,
                                               C.POINTER(C.c_int)]
         L.tdbem_timings.argtypes = [C.c_void_p, _dp]
@@ -161,6 +164,25 @@ class Case:
                                  C.byref(solved), times), "tdbem_solve")
         return (lu.reshape(nstep + 1, self.ne, 3), lt.reshape(nstep + 1, self.ne, 3), iters, flags, solved.value,
                 times)
+
+    def solve_batched(self, known_lu, nstep, restart=200, tol=1e-5):
+        """known_lu: (nrhs, nstep+1, ne, 3) load histories -> lu, lt of the same
+        shape, iters / flags (nrhs, nstep), solved (nrhs,), times."""
+        known = np.ascontiguousarray(known_lu, np.float64)
+        nrhs = known.shape[0]
+        known = known.reshape(-1)
+        if known.size != nrhs * (nstep + 1) * self.ne * 3:
+            raise ValueError("known_lu must hold nrhs x (nstep+1) x ne x 3 values")
+        lu = np.zeros_like(known)
+        lt = np.zeros_like(known)
+        iters = np.zeros((nrhs, nstep), np.int32)
+        flags = np.zeros((nrhs, nstep), np.int32)
+        solved = np.zeros(nrhs, np.int32)
+        times = np.zeros(3)
+        _check(lib().tdbem_solve_batched(self.h, int(nstep), int(nrhs), known, int(restart), float(tol), lu, lt,
+                                         iters, flags, solved, times), "tdbem_solve_batched")
+        shape = (nrhs, nstep + 1, self.ne, 3)
+        return lu.reshape(shape), lt.reshape(shape), iters, flags, solved, times
 
     def set_option(self, name, value):
         _check(lib().tdbem_set_option(self.h, name.encode(), float(value)), "tdbem_set_option")

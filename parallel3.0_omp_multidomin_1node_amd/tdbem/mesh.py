@@ -199,3 +199,33 @@ def mixed_config3(h: float = 0.125, nstep: int = 600, seed: int = 2) -> Workload
     known = normal_load_history(len(m.ele), nstep, dt, m.top, _heaviside(nstep))
     mats = np.array([[d + 1, E[d], v[d], rho[d]] for d in range(8)])
     return Workload(f"stacked_2x2x8_h{h:g}", m, mats, dt, nstep, known)
+
+
+def load_cases(w: Workload, nrhs: int = 64, seed: int = 4) -> np.ndarray:
+    """(nrhs, nstep+1, ne, 3) seeded load cases on the workload's mesh: each a
+    Ricker (centre frequency in [0.5, 1.5] / (40 dt)) or Heaviside pulse on a
+    random patch of the top face (the elements within a random radius of a
+    random top element), with a random direction (element frame) and an
+    amplitude in [0.5, 2]."""
+    m = w.mesh
+    ne = len(m.ele)
+    rng = np.random.default_rng(seed)
+    top = np.asarray(m.top, np.int64)
+    centre = m.points[m.ele[top, :4]].mean(axis=1)
+    span = float(np.ptp(centre, axis=0).max()) or 1.0
+    t = np.arange(w.nstep + 1) * w.dt
+    out = np.zeros((nrhs, w.nstep + 1, ne, 3))
+    for b in range(nrhs):
+        c = centre[rng.integers(len(top))]
+        radius = rng.uniform(0.15, 0.5) * span
+        patch = top[np.linalg.norm(centre - c, axis=1) <= radius]
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        amp = rng.uniform(0.5, 2.0)
+        if rng.random() < 0.5:
+            sig = ricker(t, rng.uniform(0.5, 1.5) / (40 * w.dt))
+            sig[0] = 0.0
+        else:
+            sig = _heaviside(w.nstep)
+        out[b] = normal_load_history(ne, w.nstep, w.dt, patch, amp * sig, direction=d)
+    return out

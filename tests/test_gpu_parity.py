@@ -273,3 +273,30 @@ def test_mixed_precision_error_against_fp64():
         et = np.abs(T1 - T0).max() / np.abs(T0).max()
         print("mixed", mixed, "vs fp64: max rel err U", eu, "T", et, flush=True)
         assert eu <= bound and et <= bound, mixed
+
+
+@pytest.mark.parametrize("nrhs,mixed", [(3, 0), (67, 0), (5, 2)])
+def test_batched_load_cases_match_single_solves(nrhs, mixed):
+    """Load cases solved as one batch (shared bands, lane-per-case convolution,
+    lockstep GMRES) give each case's single-solve result: the same GMRES
+    iteration counts and displacements / tractions to 1e-9 relative
+    (tol 1e-12; the batch sums its dot products and the far convolution in
+    another order).  67 cases exercise a second, partial group of lanes."""
+    tdbem = _tdbem()
+    from tdbem import mesh
+    w = mesh.layered_config2(h=0.5, nstep=30)
+    m = w.mesh
+    cases = mesh.load_cases(w, nrhs)
+    print("case:", w.name, "batched", nrhs, "mixed", mixed, flush=True)
+    gc = tdbem.Case(m.points, m.ele, m.bcid, w.dt, w.gap, domain_count=4, mat_rows=w.mats, itf_rows=m.interfaces)
+    gc.set_option("mixed", mixed)
+    gc.assemble_step0(10)
+    gc.assemble_history(w.nstep)
+    BU, BT, bit, bfl, bsolved, _ = gc.solve_batched(cases, w.nstep, tol=1e-12)
+    assert (bsolved == w.nstep).all() and not bfl.any()
+    for b in sorted({0, nrhs // 2, nrhs - 1}):
+        U, T, it, fl, solved, _ = gc.solve(cases[b], w.nstep, tol=1e-12)
+        assert solved == w.nstep
+        assert np.abs(BU[b] - U).max() <= 1e-9 * np.abs(U).max(), b
+        assert np.abs(BT[b] - T).max() <= 1e-9 * np.abs(T).max(), b
+        assert np.abs(bit[b] - it).max() <= 1, b

@@ -13,7 +13,11 @@ loop (history convolution + GMRES per step).  The reported value is
 time steps per second of that whole job: K * NStep / wall time.
 
 Inputs are generated on the host and uploaded by the case setup inside the
-timed region; the GPU holds everything else.  Prints one JSON line.
+timed region; the GPU holds everything else.  Prints one JSON line.  Beside
+the headline FP64 value the line carries a "batched" block (64 seeded load
+cases on the same block solved as one batch) and a "mixed_precision" block
+(the 8-domain 8192-element 600-step box in FP64 and in both mixed modes, with
+the error of the mixed runs against the FP64 run).
 """
 from __future__ import annotations
 
@@ -31,6 +35,12 @@ sys.path.insert(0, str(REPO))
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per k_conv_far<16> launch on the default workload (layered block,
+# h = 0.1, 400 steps, FP64), from FETCH_SIZE and WRITE_SIZE counter passes
+# calibrated with a 1 GiB device copy (fetch x2.000, write x1.000); the
+# summary with its commands is kept in
+# profiles/k_conv_far16_traffic_layered2x2x4_h0.1_fp64.md.
+MEASURED_TRAFFIC = {("layered_2x2x4_h0.1", 400, 16): 70341037698.0}
 FP64_PEAK_TFLOPS = 78.6
 
 
@@ -100,6 +110,87 @@ def cpu_baseline(threads):
     return dict(value=w.nstep / dt, wall=dt, w=w, U=U)
 
 
+def copy_calibration(nbytes):
+    """One device-to-device copy of nbytes through the HIP runtime the library
+    already loaded, so a profiler's FETCH_SIZE / WRITE_SIZE of this dispatch
+    calibrates the counters against a known byte count."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so.7", mode=C.RTLD_GLOBAL)
+    a, b = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) or hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, C.c_size_t(nbytes))
+    hip.hipDeviceSynchronize()
+    rc = hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    if rc:
+        raise RuntimeError("hipMemcpy failed")
+    log(f"calibration copy of {nbytes} bytes done")
+
+
+def mixed_precision_report(tdbem, h=0.125, nstep=600):
+    """Mixed precision on the 8-domain stacked box (8192 elements, 600 steps):
+    whole-job steps/s of the FP64 run and of both mixed modes, each solved to
+    GMRES tol 1e-12, and the max relative displacement / traction error of the
+    mixed runs against the FP64 run."""
+    from tdbem import mesh
+    w = mesh.mixed_config3(h=h, nstep=nstep)
+    m = w.mesh
+    res = {}
+    for mixed in (0, 1, 2):
+        t0 = time.perf_counter()
+        c = tdbem.Case(m.points, m.ele, m.bcid, w.dt, w.gap, domain_count=m.domain_count, mat_rows=w.mats,
+                       itf_rows=m.interfaces)
+        c.set_option("mixed", mixed)
+        c.assemble_step0(10)
+        c.assemble_history(w.nstep)
+        U, T, it, fl, solved, _ = c.solve(w.known, w.nstep, tol=1e-12)
+        tdbem.synchronize()
+        wall = time.perf_counter() - t0
+        tm = c.timings()
+        res[mixed] = dict(U=U, T=T, wall=wall, solved=solved, loop=tm["convolution"] + tm["gmres"])
+        c.close()
+        log(f"config 3 mixed={mixed}: {wall:.2f} s")
+    out = {"workload": w.name, "elements": int(len(m.ele)), "domains": int(m.domain_count), "nstep": w.nstep,
+           "gmres_tol": 1e-12}
+    U0, T0 = res[0]["U"], res[0]["T"]
+    names = {0: "fp64", 1: "fp32_blocks", 2: "fp32_bf16_blocks"}
+    for k, r in res.items():
+        e = {"steps_per_s": round(w.nstep / r["wall"], 3), "time_loop_steps_per_s": round(w.nstep / r["loop"], 3)}
+        if k:
+            e["max_rel_err_u"] = float(np.abs(r["U"] - U0).max() / np.abs(U0).max())
+            e["max_rel_err_t"] = float(np.abs(r["T"] - T0).max() / np.abs(T0).max())
+        out[names[k]] = e
+    return out
+
+
+def batched_report(tdbem, w, single_steps_per_s, nrhs=64):
+    """64 seeded load cases on the layered block solved as one batch (shared
+    bands, lane-per-case convolution, lockstep GMRES), whole job timed."""
+    from tdbem import mesh
+    cases = mesh.load_cases(w, nrhs)
+    m = w.mesh
+    t0 = time.perf_counter()
+    c = make_case(tdbem, w)
+    c.assemble_step0(10)
+    c.assemble_history(w.nstep)
+    t1 = time.perf_counter()
+    BU, BT, it, fl, solved, times = c.solve_batched(cases, w.nstep, tol=1e-5)
+    tdbem.synchronize()
+    t2 = time.perf_counter()
+    c.close()
+    v = nrhs * w.nstep / (t2 - t0)
+    log(f"batched {nrhs}: {t2 - t0:.2f} s")
+    return {"load_cases": nrhs, "case_steps_per_s": round(v, 2),
+            "time_loop_case_steps_per_s": round(nrhs * w.nstep / (t2 - t1), 2),
+            "vs_single_case_whole_job": round(v / single_steps_per_s, 2),
+            "all_solved": bool((solved == w.nstep).all()), "gmres_inner_iterations_mean": float(it.mean()),
+            "breakdown_s": {"setup_step0_history": round(t1 - t0, 3), "convolution": round(float(times[0]), 3),
+                            "gmres": round(float(times[1]), 3)}}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,6 +200,10 @@ def main():
     ap.add_argument("--nstep", type=int, default=400)
     ap.add_argument("--conv-tile", type=int, default=16, choices=(8, 16), help="steps per far-part tile")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the config-3 mixed-precision report")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="also run one device copy of BYTES (profiler counter calibration)")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 64-load-case batched report")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
     args = ap.parse_args()
     if args.gpus != 1:
@@ -123,6 +218,9 @@ def main():
     from tdbem import mesh
 
     w = mesh.layered_config2(h=args.h, nstep=args.nstep)
+    if args.copy_calibration:
+        tdbem.synchronize()
+        copy_calibration(args.copy_calibration)
     log(f"workload {w.name}: {len(w.mesh.ele)} elements, {w.nstep} steps, dt={w.dt:.6g}")
     for i in range(args.warmup):
         r = run_once(tdbem, w, tile=args.conv_tile)
@@ -168,10 +266,15 @@ def main():
         "roofline": {"kernel": f"k_conv_far (time-blocked history convolution, {args.conv_tile} steps per launch)",
                      "bound": "hbm", "achieved": round(conv_gbs, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(conv_gbs / HBM_PEAK_GBS, 4),
-                     "traffic": None, "avg_launch_ms": round(1000 * conv_avg, 4),
+                     "traffic": MEASURED_TRAFFIC.get((w.name, w.nstep, args.conv_tile)),
+                     "avg_launch_ms": round(1000 * conv_avg, 4),
                      "bytes_per_launch": round(bytes_launch)},
     }
     c.close()
+    if not args.no_batched:
+        out["batched"] = batched_report(tdbem, w, value)
+    if not args.no_mixed:
+        out["mixed_precision"] = mixed_precision_report(tdbem)
     if not args.no_cpu_baseline:
         log(f"cpu baseline on {args.cpu_threads} threads ...")
         cb = cpu_baseline(args.cpu_threads)

@@ -78,6 +78,28 @@ int tdbem_self_tables(tdbem_ctx* ctx, double* u1, double* t1, double* t2);
 int tdbem_solve(tdbem_ctx* ctx, int nstep, const double* known_lu, long restart, double tol, double* lu,
                 double* lt, int* iters, int* flags, int* solved, double* times);
 
+/* Reverse-time adjoint (no counterpart in the reference): gradient of the
+ * misfit J = sum_{K=1..nstep} sum_r |lu_K[r] - obs[K][r]|^2 over the receiver
+ * elements receivers[nrec] (0-based; lu in the global frame as tdbem_solve
+ * returns it; obs[(nstep+1)][nrec][3]) with respect to each domain's
+ * (E, v, rho): grad[3*d + p].  The forward solve, the backward sweep
+ * (A^T solves with the transposed leaf preconditioner and the transposed
+ * history convolution) and the gradient contraction all run on the device;
+ * the operator derivatives are central differences over the materials with
+ * relative step fd_step (band structure held fixed).  Needs FP64 history
+ * blocks assembled for at least nstep steps; lu (may be null) receives the
+ * forward displacements [(nstep+1)][ne][3]. */
+int tdbem_adjoint(tdbem_ctx* ctx, int nstep, const double* known_lu, int nrec, const int* receivers,
+                  const double* obs, long restart, double tol, double fd_step, double* J, double* grad, double* lu);
+
+/* Per-domain materials (E, v, rho), 3 per domain (no counterpart in the
+ * reference, whose materials are fixed per run); get fills at most capacity
+ * domains and reports the domain count.  Setting them rebuilds the
+ * dynamic constants and the singular self tables; assemble step 0 and the
+ * history again before solving. */
+int tdbem_get_materials(tdbem_ctx* ctx, int capacity, int* ndomains, double* evr);
+int tdbem_set_materials(tdbem_ctx* ctx, const double* evr);
+
 /* Batched load cases over the same bands (no counterpart in the reference,
  * which solves one load history per run; each case follows the tdbem_solve
  * time loop above): nrhs histories known_lu[nrhs][nstep+1][ne*3] give
@@ -117,6 +139,10 @@ int tdbem_history_stats(tdbem_ctx* ctx, long* counts, long* npairs);
  *   inner GMRES under an FP64 iterative-refinement residual (singular
  *   tables, step-0 matrices, Hessenberg / Givens and all reductions stay
  *   FP64);
+ *   "freeze_structure" 1 freezes the band structure and the judges
+ *   (patch-wise / regular path of every history entry and step-0 pair) at
+ *   the current materials, so assemblies after tdbem_set_materials stay on
+ *   the same piece of the piecewise-smooth operator (0 releases them);
  *   "quad_staged" 1 (default) wave-per-pair history quadrature with the
  *   point geometry staged in LDS, 0 one lane per band entry with separate
  *   T2 / T1 / U passes (same results). */

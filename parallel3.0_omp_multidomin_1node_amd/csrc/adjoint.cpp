@@ -1,0 +1,338 @@
+// Reverse-time adjoint of the FP64 multi-domain time loop and the gradient of
+// a receiver misfit with respect to the per-domain materials (E, v, rho).
+//
+// Forward (DBEM1/multidomain.cpp:3815-3893): A x_K = KnownM k_K + h_K(x_<K).
+// Adjoint: A^T lambda_K = dJ/dx_K + sum_{K' > K} (dh_K'/dx_K)^T lambda_K',
+// solved backward in time by GMRES on A^T with the transposed leaf
+// preconditioner.  Gradient: dJ/dtheta = sum_K lambda_K^T (dKnownM k_K
+// - dA x_K + dh_K), where the operator derivatives are central differences
+// over the materials of the same quadrature kernels with the band structure
+// held fixed (every block of domain d depends only on domain d's material,
+// so all domains are perturbed at once, one parameter at a time).
+#include "adjoint.hpp"
+#include "solver.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+
+namespace tdbem {
+
+namespace {
+
+void CheckA(hipError_t e, const char* what)
+{
+    if (e != hipSuccess)
+        throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Block transpose of a square block CCSR matrix (rows sorted by column).
+CCSRMat TransposeBlocks(const CCSRMat& M, long n)
+{
+    CCSRMat T;
+    T.n = n;
+    T.row_ptr.assign((size_t)n + 1, 0);
+    for (int c : M.col_index)
+        T.row_ptr[(size_t)c + 1]++;
+    for (long r = 0; r < n; ++r)
+        T.row_ptr[(size_t)r + 1] += T.row_ptr[(size_t)r];
+    std::vector<int> fill(T.row_ptr.begin(), T.row_ptr.end() - 1);
+    T.col_index.resize(M.col_index.size());
+    T.value.resize(M.value.size());
+    for (long r = 0; r < (long)M.row_ptr.size() - 1; ++r)
+        for (int k = M.row_ptr[(size_t)r]; k < M.row_ptr[(size_t)r + 1]; ++k) {
+            int c = M.col_index[(size_t)k];
+            int dst = fill[(size_t)c]++;
+            T.col_index[(size_t)dst] = (int)r;
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b)
+                    T.value[9 * (size_t)dst + 3 * a + b] = M.value[9 * (size_t)k + 3 * b + a];
+        }
+    return T;
+}
+
+// Block (row, col) of M, or null.
+const double* FindBlock(const CCSRMat& M, long row, int col)
+{
+    auto b = M.col_index.begin() + M.row_ptr[(size_t)row];
+    auto e = M.col_index.begin() + M.row_ptr[(size_t)row + 1];
+    auto it = std::lower_bound(b, e, col);
+    if (it == e || *it != col)
+        return nullptr;
+    return &M.value[9 * (size_t)(it - M.col_index.begin())];
+}
+
+// Central difference (P - M) / (2 delta_row) on the pattern of C.
+CCSRMat DiffOnPattern(const CCSRMat& C, const CCSRMat& P, const CCSRMat& Mm, const std::vector<double>& inv2delta)
+{
+    CCSRMat D = C;
+    for (long r = 0; r < (long)C.row_ptr.size() - 1; ++r)
+        for (int k = C.row_ptr[(size_t)r]; k < C.row_ptr[(size_t)r + 1]; ++k) {
+            const double* a = FindBlock(P, r, C.col_index[(size_t)k]);
+            const double* b = FindBlock(Mm, r, C.col_index[(size_t)k]);
+            for (int q = 0; q < 9; ++q)
+                D.value[9 * (size_t)k + q] = ((a ? a[q] : 0.0) - (b ? b[q] : 0.0)) * inv2delta[(size_t)r];
+        }
+    return D;
+}
+
+struct DevCCSR {
+    DevBuf<int> rp, col;
+    DevBuf<double> val;
+    void Upload(const CCSRMat& M, hipStream_t s)
+    {
+        rp.upload(M.row_ptr.data(), M.row_ptr.size(), s);
+        col.upload(M.col_index.data(), std::max<size_t>(1, M.col_index.size()), s);
+        val.upload(M.value.data(), std::max<size_t>(9, M.value.size()), s);
+    }
+};
+
+}  // namespace
+
+void Case::GetMaterials(double* evr) const
+{
+    for (size_t d = 0; d < model.domains.size(); ++d) {
+        evr[3 * d] = model.domains[d].E;
+        evr[3 * d + 1] = model.domains[d].v;
+        evr[3 * d + 2] = model.domains[d].Rou;
+    }
+}
+
+void Case::SetMaterials(const double* evr)
+{
+    std::vector<DynaMat> mats;
+    for (size_t d = 0; d < model.domains.size(); ++d) {
+        Domain& D = model.domains[d];
+        D.E = evr[3 * d];
+        D.v = evr[3 * d + 1];
+        D.Rou = evr[3 * d + 2];
+        double G = D.E / 2.0 / (1.0 + D.v);
+        D.mat = BuildDynaMat(D.v, G, D.Rou, dt);
+        mats.push_back(D.mat);
+    }
+    UploadDynaMats(mats.data(), (int)mats.size(), stream);
+    LaunchSingular(d_geo.p, d_dom.p, d_tables.p, d_u1.p, d_t1.p, d_t2.p, (int)ne, stream);
+    CheckA(hipStreamSynchronize(stream), "materials");
+}
+
+double Case::Adjoint(int nstep, const double* known_lu, const int* recv, int nrec, const double* obs, long restart,
+                     double tol, double fdStep, double* grad, double* lu_out)
+{
+    if (!historyDone || maxn < nstep)
+        throw std::runtime_error("history not assembled for the requested steps");
+    if (historyMixed)
+        throw std::runtime_error("the adjoint needs FP64 history blocks (mixed 0)");
+    if (!quadStaged)
+        throw std::runtime_error("the adjoint needs the staged history quadrature");
+    for (int r = 0; r < nrec; ++r)
+        if (recv[r] < 0 || recv[r] >= ne)
+            throw std::runtime_error("receiver element out of range");
+    const long n3 = 3 * ne;
+    const int nstore = nstep + 1;
+    const int nd = grid.nd;
+
+    // 1. forward solve, keeping x_K, the state columns and the known inputs
+    AdjointKeep kp;
+    std::vector<double> lu((size_t)nstore * n3), lt((size_t)nstore * n3);
+    std::vector<int> its((size_t)nstep), fls((size_t)nstep);
+    int solved = 0;
+    keep = &kp;
+    try {
+        Solve(nstep, known_lu, restart, tol, lu.data(), lt.data(), its.data(), fls.data(), &solved);
+    } catch (...) {
+        keep = nullptr;
+        throw;
+    }
+    keep = nullptr;
+    if (solved != nstep || fls[(size_t)nstep - 1] != 0)
+        throw std::runtime_error("forward solve did not converge");
+
+    // 2. misfit and dJ/dx_K through lu = F^T (u_sign T x_u)
+    double J = 0.0;
+    std::vector<double> dJ((size_t)nstore * n3, 0.0);
+    for (int K = 1; K <= nstep; ++K)
+        for (int r = 0; r < nrec; ++r) {
+            long i = recv[r];
+            double l[3];
+            for (int c = 0; c < 3; ++c) {
+                double diff = lu[((size_t)K * ne + i) * 3 + c] - obs[((size_t)K * nrec + r) * 3 + c];
+                J += diff * diff;
+                l[c] = 2.0 * diff;
+            }
+            const NodeRefs& R = refs[(size_t)i];
+            if (R.u_known)
+                continue;
+            const double* F = geo[(size_t)i].trans;
+            double ub[3];
+            for (int rr = 0; rr < 3; ++rr)
+                ub[rr] = F[rr * 3] * l[0] + F[rr * 3 + 1] * l[1] + F[rr * 3 + 2] * l[2];
+            for (int k = 0; k < 3; ++k)
+                dJ[(size_t)K * n3 + 3 * (size_t)R.u_block + k] +=
+                    R.u_sign * (R.itf_T[k] * ub[0] + R.itf_T[3 + k] * ub[1] + R.itf_T[6 + k] * ub[2]);
+        }
+
+    // 3. A^T and the transposed leaf preconditioner
+    CCSRMat At = TransposeBlocks(A, ne);
+    std::vector<long> outBlock((size_t)ne);
+    CheckA(hipMemcpy(outBlock.data(), d_outBlock.p, sizeof(long) * (size_t)ne, hipMemcpyDeviceToHost), "outBlock");
+    struct Tri {
+        int r, c;
+        double v[9];
+    };
+    std::vector<Tri> tri;
+    for (long l = 0; l < pre.leafCount; ++l) {
+        long b = pre.leafBegin[(size_t)l], c = pre.leafSize[(size_t)l], n3l = 3 * c;
+        const double* inv = &pre.inv[(size_t)pre.matOffset[(size_t)l]];
+        for (long jl = 0; jl < c; ++jl)
+            for (long k = 0; k < c; ++k) {
+                Tri t;
+                t.r = (int)pre.repid[(size_t)(b + k)];
+                t.c = (int)outBlock[(size_t)(b + jl)];
+                for (int mrow = 0; mrow < 3; ++mrow)
+                    for (int cc = 0; cc < 3; ++cc)
+                        t.v[3 * cc + mrow] = inv[(3 * jl + mrow) * n3l + 3 * k + cc];
+                tri.push_back(t);
+            }
+    }
+    std::sort(tri.begin(), tri.end(), [](const Tri& a, const Tri& b) { return a.r != b.r ? a.r < b.r : a.c < b.c; });
+    CCSRMat Mt;
+    Mt.n = ne;
+    Mt.row_ptr.assign((size_t)ne + 1, 0);
+    for (const Tri& t : tri) {  // (row, col) pairs are distinct: each row lies in one leaf
+        Mt.row_ptr[(size_t)t.r + 1]++;
+        Mt.col_index.push_back(t.c);
+        Mt.value.insert(Mt.value.end(), t.v, t.v + 9);
+    }
+    for (long r = 0; r < ne; ++r)
+        Mt.row_ptr[(size_t)r + 1] += Mt.row_ptr[(size_t)r];
+    DevCCSR dAt, dMt;
+    dAt.Upload(At, stream);
+    dMt.Upload(Mt, stream);
+
+    // history columns -> the elements that read them
+    auto columns = [&](bool u, std::vector<long>& ptr, std::vector<int>& mem) {
+        long ncol = u ? dof.historyUBlockCount : dof.historyTBlockCount;
+        ptr.assign((size_t)ncol + 1, 0);
+        for (long f = 0; f < ne; ++f)
+            ptr[(size_t)(u ? refs[(size_t)f].hu_block : refs[(size_t)f].ht_block) + 1]++;
+        for (long c = 0; c < ncol; ++c)
+            ptr[(size_t)c + 1] += ptr[(size_t)c];
+        mem.assign((size_t)ne, 0);
+        std::vector<long> fill(ptr.begin(), ptr.end() - 1);
+        for (long f = 0; f < ne; ++f)
+            mem[(size_t)fill[(size_t)(u ? refs[(size_t)f].hu_block : refs[(size_t)f].ht_block)]++] = (int)f;
+    };
+    std::vector<long> cuPtr, ctPtr;
+    std::vector<int> cuMem, ctMem;
+    columns(true, cuPtr, cuMem);
+    columns(false, ctPtr, ctMem);
+    DevBuf<long> d_cuPtr, d_ctPtr;
+    DevBuf<int> d_cuMem, d_ctMem;
+    d_cuPtr.upload(cuPtr.data(), cuPtr.size(), stream);
+    d_ctPtr.upload(ctPtr.data(), ctPtr.size(), stream);
+    d_cuMem.upload(cuMem.data(), cuMem.size(), stream);
+    d_ctMem.upload(ctMem.data(), ctMem.size(), stream);
+
+    // 4. backward sweep
+    DevBuf<double> d_lam, d_y, d_z, d_dj;
+    d_lam.alloc((size_t)nstore * n3);
+    LaunchZero(d_lam.p, (long)d_lam.n, stream);
+    d_y.alloc((size_t)n3);
+    d_z.alloc(6 * (size_t)ne);
+    d_dj.upload(dJ.data(), dJ.size(), stream);
+    PrepareGmres(restart);
+    GmresOp<double> opT{dAt.rp.p, dAt.col.p, dAt.val.p, nullptr, dMt.rp.p, dMt.col.p, dMt.val.p};
+    for (int K = nstep; K >= 1; --K) {
+        CheckA(hipMemcpyAsync(d_y.p, d_dj.p + (size_t)K * n3, sizeof(double) * (size_t)n3, hipMemcpyDeviceToDevice,
+                              stream),
+               "adjoint rhs");
+        if (K < nstep) {
+            LaunchConvT(grid, d_refs.p, d_off.p, d_kb.p, d_len.p, d_mts.p, d_mgs.p, d_lam.p, n3, nstep, K, (int)ne,
+                        d_z.p, stream);
+            LaunchStateT(d_refs.p, d_geo.p, d_cuPtr.p, d_cuMem.p, d_ctPtr.p, d_ctMem.p, d_z.p, (int)ne, d_y.p, stream);
+        }
+        int it = 0;
+        int flag = GmresCore<double>(d_y.p, d_lam.p + (size_t)K * n3, restart, tol, &it, opT, d_V.p, d_W.p, d_u.p,
+                                     d_u2.p);
+        if (flag != 0)
+            throw std::runtime_error("adjoint GMRES did not converge at step " + std::to_string(K));
+    }
+
+    // 5. gradient: central differences of the operators, contracted with lambda,
+    // with the bands and judges frozen at the current materials
+    bool wasFrozen = frozen;
+    if (!wasFrozen)
+        FreezeStructure(true);
+    std::vector<double> theta((size_t)nd * 3);
+    GetMaterials(theta.data());
+    DevBuf<double> d_dM, d_dG, d_acc, d_v, d_t1, d_t2, d_kn;
+    DevBuf<unsigned char> d_aliveS;
+    d_dM.alloc(9 * (size_t)entries);
+    d_dG.alloc(6 * (size_t)entries);
+    d_aliveS.alloc((size_t)entries);
+    d_acc.alloc(3 * (size_t)ne);
+    LaunchZero(d_acc.p, (long)d_acc.n, stream);
+    d_v.alloc((size_t)n3);
+    d_t1.alloc((size_t)n3);
+    d_t2.alloc((size_t)n3);
+    d_kn.alloc(3 * (size_t)std::max<long>(1, dof.knownBlockCount));
+    LaunchZero(d_kn.p, (long)d_kn.n, stream);
+    std::vector<double> delta((size_t)nd), inv2row((size_t)ne);
+    DevCCSR dA, dK;
+    for (int p = 0; p < 3; ++p) {
+        std::vector<double> tp = theta, tm = theta;
+        for (int d = 0; d < nd; ++d) {
+            double base = theta[(size_t)d * 3 + p];
+            delta[(size_t)d] = fdStep * std::max(std::fabs(base), 0.1);
+            tp[(size_t)d * 3 + p] = base + delta[(size_t)d];
+            tm[(size_t)d * 3 + p] = base - delta[(size_t)d];
+        }
+        for (long s = 0; s < ne; ++s)
+            inv2row[(size_t)s] = 1.0 / (2.0 * delta[(size_t)domOf[(size_t)s]]);
+        CCSRMat Ap, Kp, Am, Km;
+        SetMaterials(tp.data());
+        ComputeStep0(Ap, Kp);
+        StagedQuadrature(d_dM.p, d_dG.p, nullptr, nullptr, d_aliveS.p);
+        SetMaterials(tm.data());
+        ComputeStep0(Am, Km);
+        StagedQuadrature(d_mts.p, d_mgs.p, nullptr, nullptr, d_aliveS.p);
+        for (int d = 0; d < nd; ++d) {
+            long e0 = h_off[(size_t)grid.pairBase[d]], e1 = h_off[(size_t)grid.pairBase[d + 1]];
+            double sc = 1.0 / (2.0 * delta[(size_t)d]);
+            LaunchFdBand(d_dM.p, d_mts.p, 9 * e0, 9 * e1, sc, stream);
+            LaunchFdBand(d_dG.p, d_mgs.p, 6 * e0, 6 * e1, sc, stream);
+        }
+        dA.Upload(DiffOnPattern(A, Ap, Am, inv2row), stream);
+        dK.Upload(DiffOnPattern(KnownM, Kp, Km, inv2row), stream);
+        for (int K = 1; K <= nstep; ++K) {
+            LaunchZero(d_v.p, n3, stream);
+            LaunchConvolution(grid, d_refs.p, d_off.p, d_kb.p, d_len.p, d_dM.p, d_dG.p, kp.gU.p, kp.gT.p, nstore, K,
+                              maxn, (int)ne, d_v.p, stream);
+            LaunchKnown(d_refs.p, kp.bd.p + (size_t)K * n3, (int)ne, d_kn.p, dof.knownBlockCount, stream);
+            LaunchSpmv(ne, dK.rp.p, dK.col.p, (const double*)dK.val.p, (const double*)d_kn.p, d_t1.p, stream);
+            LaunchSpmv(ne, dA.rp.p, dA.col.p, (const double*)dA.val.p, (const double*)(kp.X.p + (size_t)K * n3),
+                       d_t2.p, stream);
+            LaunchAddSub(d_v.p, d_t1.p, d_t2.p, n3, stream);
+            LaunchRowDotAcc(d_lam.p + (size_t)K * n3, d_v.p, ne, d_acc.p + (size_t)p * ne, stream);
+        }
+        CheckA(hipStreamSynchronize(stream), "gradient");
+    }
+    // restore the materials, self tables and FP64 bands (same kernels, same inputs)
+    SetMaterials(theta.data());
+    StagedQuadrature(d_mts.p, d_mgs.p, nullptr, nullptr, d_alive.p);
+    if (!wasFrozen)
+        FreezeStructure(false);
+    std::vector<double> acc(3 * (size_t)ne);
+    CheckA(hipMemcpy(acc.data(), d_acc.p, sizeof(double) * acc.size(), hipMemcpyDeviceToHost), "gradient");
+    for (int i = 0; i < 3 * nd; ++i)
+        grad[i] = 0.0;
+    for (int p = 0; p < 3; ++p)
+        for (long s = 0; s < ne; ++s)
+            grad[3 * domOf[(size_t)s] + p] += acc[(size_t)p * ne + s];
+    if (lu_out)
+        std::memcpy(lu_out, lu.data(), sizeof(double) * lu.size());
+    return J;
+}
+
+}  // namespace tdbem

@@ -205,6 +205,44 @@ int tdbem_solve_batched(tdbem_ctx* ctx, int nstep, int nrhs, const double* known
     return 0;
 }
 
+int tdbem_adjoint(tdbem_ctx* ctx, int nstep, const double* known_lu, int nrec, const int* receivers,
+                  const double* obs, long restart, double tol, double fd_step, double* J, double* grad, double* lu)
+{
+    if (!ctx || nstep < 1 || !known_lu || nrec < 1 || !receivers || !obs || restart < 1 || !(tol > 0.0) ||
+        !(fd_step > 0.0) || !J || !grad)
+        return Fail("tdbem_adjoint: invalid arguments");
+    try {
+        *J = ctx->c.Adjoint(nstep, known_lu, receivers, nrec, obs, restart, tol, fd_step, grad, lu);
+    } catch (const std::exception& e) {
+        return Fail("tdbem_adjoint", e);
+    }
+    return 0;
+}
+
+int tdbem_get_materials(tdbem_ctx* ctx, int capacity, int* ndomains, double* evr)
+{
+    if (!ctx || !evr || !ndomains)
+        return Fail("tdbem_get_materials: invalid arguments");
+    int nd = (int)ctx->c.model.domains.size();
+    *ndomains = nd;
+    if (capacity < nd)
+        return Fail("tdbem_get_materials: capacity below the domain count");
+    ctx->c.GetMaterials(evr);
+    return 0;
+}
+
+int tdbem_set_materials(tdbem_ctx* ctx, const double* evr)
+{
+    if (!ctx || !evr)
+        return Fail("tdbem_set_materials: invalid arguments");
+    try {
+        ctx->c.SetMaterials(evr);
+    } catch (const std::exception& e) {
+        return Fail("tdbem_set_materials", e);
+    }
+    return 0;
+}
+
 int tdbem_dyna_gmres_solver(tdbem_ctx* ctx, long iterations, double error, long maxleafpointnum, int nstep,
                             const double* known_lu, double* lu, double* lt, int* result)
 {
@@ -264,6 +302,14 @@ int tdbem_set_option(tdbem_ctx* ctx, const char* name, double value)
         if (value != 0.0 && value != 1.0 && value != 2.0)
             return Fail("mixed must be 0, 1 or 2");
         ctx->c.mixed = (int)value;
+        return 0;
+    }
+    if (n == "freeze_structure") {
+        try {
+            ctx->c.FreezeStructure(value != 0.0);
+        } catch (const std::exception& e) {
+            return Fail("tdbem_set_option freeze_structure", e);
+        }
         return 0;
     }
     if (n == "quad_staged") {

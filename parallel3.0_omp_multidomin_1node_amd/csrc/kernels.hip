@@ -1094,6 +1094,20 @@ __device__ __forceinline__ EntryFlags Judge(const DynaMat& m, double radius, dou
     return e;
 }
 
+// Judges of a band entry, or the frozen ones (f2 | f1 << 2) when cls is set.
+__device__ __forceinline__ EntryFlags JudgeAt(const DynaMat& m, double radius, double r, int k, bool same,
+                                              const unsigned char* cls, long e)
+{
+    if (!cls)
+        return Judge(m, radius, r, k, same);
+    EntryFlags f;
+    f.self1 = k <= 1 && same;
+    unsigned char c = cls[e];
+    f.f2 = c & 3;
+    f.f1 = (c >> 2) & 3;
+    return f;
+}
+
 // Regular-path contribution of one staged point to the lane's accumulators.
 __device__ __forceinline__ void ConsumeRegular(const DynaMat& m, const double* g, double n, bool r2, bool r1,
                                                double* accA, double* accB, double* accU)
@@ -1214,8 +1228,9 @@ __device__ __forceinline__ void FinalizeEntry(const EntryFlags& fl, const double
 
 // Number of patch-wise entries (f2 == 1 or f1 == 1) in each pair's band.
 __global__ void __launch_bounds__(256) k_pwcount(const ElemGeo* __restrict__ geo, PairGrid grid,
-                                                 const int* __restrict__ kb, const int* __restrict__ len, long npairs,
-                                                 int* __restrict__ cnt)
+                                                 const int* __restrict__ kb, const int* __restrict__ len,
+                                                 const long* __restrict__ off, const unsigned char* __restrict__ cls,
+                                                 long npairs, int* __restrict__ cnt)
 {
     long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= npairs)
@@ -1226,17 +1241,47 @@ __global__ void __launch_bounds__(256) k_pwcount(const ElemGeo* __restrict__ geo
     double r = Dist3(geo[f].center, geo[s].center);
     double radius = geo[f].radius;
     int c = 0, k0 = kb[p], L = len[p];
+    long e0 = off[p];
     for (int t = 0; t < L; ++t)
-        c += dev::Judge(m, radius, r, k0 + t, s == f).patchwise() ? 1 : 0;
+        c += dev::JudgeAt(m, radius, r, k0 + t, s == f, cls, e0 + t).patchwise() ? 1 : 0;
     cnt[p] = c;
 }
 
-void LaunchPwCount(const ElemGeo* geo, PairGrid grid, const int* kb, const int* len, long npairs, int* cnt,
-                   hipStream_t s)
+void LaunchPwCount(const ElemGeo* geo, PairGrid grid, const int* kb, const int* len, const long* off,
+                   const unsigned char* cls, long npairs, int* cnt, hipStream_t s)
 {
     if (npairs <= 0)
         return;
-    k_pwcount<<<(unsigned)((npairs + 255) / 256), 256, 0, s>>>(geo, grid, kb, len, npairs, cnt);
+    k_pwcount<<<(unsigned)((npairs + 255) / 256), 256, 0, s>>>(geo, grid, kb, len, off, cls, npairs, cnt);
+}
+
+// Judges of every band entry (f2 | f1 << 2) with the current materials.
+__global__ void __launch_bounds__(256) k_classify(const ElemGeo* __restrict__ geo, PairGrid grid,
+                                                  const int* __restrict__ kb, const int* __restrict__ len,
+                                                  const long* __restrict__ off, long npairs,
+                                                  unsigned char* __restrict__ cls)
+{
+    long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npairs)
+        return;
+    int s, f, d;
+    dev::DecodePair(grid, p, s, f, d);
+    const DynaMat& m = c_mats[d];
+    double r = Dist3(geo[f].center, geo[s].center);
+    double radius = geo[f].radius;
+    int k0 = kb[p], L = len[p];
+    long e0 = off[p];
+    for (int t = 0; t < L; ++t) {
+        dev::EntryFlags e = dev::Judge(m, radius, r, k0 + t, s == f);
+        cls[e0 + t] = (unsigned char)(e.f2 | (e.f1 << 2));
+    }
+}
+
+void LaunchClassify(const ElemGeo* geo, PairGrid grid, const int* kb, const int* len, const long* off, long npairs,
+                    unsigned char* cls, hipStream_t s)
+{
+    if (npairs > 0)
+        k_classify<<<(unsigned)((npairs + 255) / 256), 256, 0, s>>>(geo, grid, kb, len, off, npairs, cls);
 }
 
 // Regular-path entries: one wavefront per (pair, chunk of up to 64 lags), one
@@ -1246,9 +1291,9 @@ template <typename TB>
 __global__ void __launch_bounds__(64) k_history_reg(
     const ElemGeo* __restrict__ geo, const ElemPoints* __restrict__ pts, const NodeRefs* __restrict__ refs,
     const double* __restrict__ selfT2, PairGrid grid, const long* __restrict__ coff, const long* __restrict__ off,
-    const int* __restrict__ kb, const int* __restrict__ len, long npairs, long c0, long c1, TB* __restrict__ mts,
-    TB* __restrict__ mgs, unsigned short* __restrict__ mtsLo, unsigned short* __restrict__ mgsLo,
-    unsigned char* __restrict__ alive)
+    const int* __restrict__ kb, const int* __restrict__ len, const unsigned char* __restrict__ cls, long npairs,
+    long c0, long c1, TB* __restrict__ mts, TB* __restrict__ mgs, unsigned short* __restrict__ mtsLo,
+    unsigned short* __restrict__ mgsLo, unsigned char* __restrict__ alive)
 {
     __shared__ double sgeo[kNG2 * dev::kStageStride];
     using namespace dev;
@@ -1274,8 +1319,9 @@ __global__ void __launch_bounds__(64) k_history_reg(
     const double* src = geo[s].center;
     int k = kb[p] + t;
     double n = (double)k;
-    EntryFlags fl = Judge(m, fe.radius, Dist3(fe.center, src), k, s == f);
-    bool mine = t < len[p] && !fl.patchwise();
+    bool inBand = t < len[p];
+    EntryFlags fl = JudgeAt(m, fe.radius, Dist3(fe.center, src), k, s == f, inBand ? cls : nullptr, off[p] + t);
+    bool mine = inBand && !fl.patchwise();
     double accA[9], accB[9], accU[6];
     for (int i = 0; i < 9; ++i) {
         accA[i] = 0.0;
@@ -1307,9 +1353,9 @@ template <typename TB>
 __global__ void __launch_bounds__(64) k_history_pw(
     const ElemGeo* __restrict__ geo, const ElemPoints* __restrict__ pts, const NodeRefs* __restrict__ refs,
     const double* __restrict__ selfT2, PairGrid grid, const PwWave* __restrict__ waves, long w0, long w1,
-    const long* __restrict__ off, const int* __restrict__ kb, const int* __restrict__ len, TB* __restrict__ mts,
-    TB* __restrict__ mgs, unsigned short* __restrict__ mtsLo, unsigned short* __restrict__ mgsLo,
-    unsigned char* __restrict__ alive)
+    const long* __restrict__ off, const int* __restrict__ kb, const int* __restrict__ len,
+    const unsigned char* __restrict__ cls, TB* __restrict__ mts, TB* __restrict__ mgs,
+    unsigned short* __restrict__ mtsLo, unsigned short* __restrict__ mgsLo, unsigned char* __restrict__ alive)
 {
     __shared__ double sgeo[kPwSlots * kNPWG2 * dev::kStageStride];
     __shared__ double spr[kPwSlots * 2 * kNPW2];
@@ -1360,8 +1406,9 @@ __global__ void __launch_bounds__(64) k_history_pw(
     EntryFlags fl{false, 0, 0};
     if (active) {
         int k0 = kb[p], L = len[p], c = 0;
+        long eb0 = off[p];
         for (int tt = 0; tt < L; ++tt) {
-            EntryFlags e = Judge(m, fe.radius, r0, k0 + tt, s == f);
+            EntryFlags e = JudgeAt(m, fe.radius, r0, k0 + tt, s == f, cls, eb0 + tt);
             if (e.patchwise()) {
                 if (c == rank) {
                     t = tt;
@@ -1445,37 +1492,39 @@ __global__ void __launch_bounds__(64) k_history_pw(
 
 template <typename TB>
 void LaunchHistoryReg(const ElemGeo* geo, const ElemPoints* pts, const NodeRefs* refs, const double* selfT2,
-                      PairGrid grid, const long* coff, const long* off, const int* kb, const int* len, long npairs,
-                      long c0, long c1, TB* mts, TB* mgs, unsigned short* mtsLo, unsigned short* mgsLo,
-                      unsigned char* alive, hipStream_t s)
+                      PairGrid grid, const long* coff, const long* off, const int* kb, const int* len,
+                      const unsigned char* cls, long npairs, long c0, long c1, TB* mts, TB* mgs,
+                      unsigned short* mtsLo, unsigned short* mgsLo, unsigned char* alive, hipStream_t s)
 {
     long waves = c1 - c0;
     if (waves <= 0)
         return;
-    k_history_reg<TB><<<(unsigned)waves, 64, 0, s>>>(geo, pts, refs, selfT2, grid, coff, off, kb, len, npairs, c0, c1,
-                                                     mts, mgs, mtsLo, mgsLo, alive);
+    k_history_reg<TB><<<(unsigned)waves, 64, 0, s>>>(geo, pts, refs, selfT2, grid, coff, off, kb, len, cls, npairs,
+                                                     c0, c1, mts, mgs, mtsLo, mgsLo, alive);
 }
 
 template <typename TB>
 void LaunchHistoryPw(const ElemGeo* geo, const ElemPoints* pts, const NodeRefs* refs, const double* selfT2,
                      PairGrid grid, const PwWave* waves, long w0, long w1, const long* off, const int* kb,
-                     const int* len, TB* mts, TB* mgs, unsigned short* mtsLo, unsigned short* mgsLo,
-                     unsigned char* alive, hipStream_t s)
+                     const int* len, const unsigned char* cls, TB* mts, TB* mgs, unsigned short* mtsLo,
+                     unsigned short* mgsLo, unsigned char* alive, hipStream_t s)
 {
     long n = w1 - w0;
     if (n <= 0)
         return;
-    k_history_pw<TB><<<(unsigned)n, 64, 0, s>>>(geo, pts, refs, selfT2, grid, waves, w0, w1, off, kb, len, mts, mgs,
-                                                mtsLo, mgsLo, alive);
+    k_history_pw<TB><<<(unsigned)n, 64, 0, s>>>(geo, pts, refs, selfT2, grid, waves, w0, w1, off, kb, len, cls, mts,
+                                                mgs, mtsLo, mgsLo, alive);
 }
 
 #define TDBEM_HIST_INSTANTIATE(TB)                                                                                   \
     template void LaunchHistoryReg<TB>(const ElemGeo*, const ElemPoints*, const NodeRefs*, const double*, PairGrid,   \
-                                       const long*, const long*, const int*, const int*, long, long, long, TB*, TB*,   \
-                                       unsigned short*, unsigned short*, unsigned char*, hipStream_t);                 \
+                                       const long*, const long*, const int*, const int*, const unsigned char*, long,   \
+                                       long, long, TB*, TB*, unsigned short*, unsigned short*, unsigned char*,         \
+                                       hipStream_t);                                                                   \
     template void LaunchHistoryPw<TB>(const ElemGeo*, const ElemPoints*, const NodeRefs*, const double*, PairGrid,    \
-                                      const PwWave*, long, long, const long*, const int*, const int*, TB*, TB*,        \
-                                      unsigned short*, unsigned short*, unsigned char*, hipStream_t);
+                                      const PwWave*, long, long, const long*, const int*, const int*,                  \
+                                      const unsigned char*, TB*, TB*, unsigned short*, unsigned short*,                \
+                                      unsigned char*, hipStream_t);
 TDBEM_HIST_INSTANTIATE(double)
 TDBEM_HIST_INSTANTIATE(float)
 #undef TDBEM_HIST_INSTANTIATE
@@ -1489,7 +1538,7 @@ namespace dev {
 
 __device__ int Step0Coef(const DynaMat& m, const ElemGeo& fe, const ElemPoints& P, bool self, const double* u1,
                          const double* t1, const double* t2, const double* src, bool unknownPass, int bcid,
-                         double* coef)
+                         double* coef, int frozen)
 {
     bool useU = unknownPass ? (bcid == 123) : (bcid == 456);
     bool plus = unknownPass ? (bcid != 123) : (bcid == 456);
@@ -1515,7 +1564,7 @@ __device__ int Step0Coef(const DynaMat& m, const ElemGeo& fe, const ElemPoints& 
             K[k] += tmp[k];
         fsum = 1;
     } else {
-        int f0 = NeedCal(m, fe, 1, 0.0, src);
+        int f0 = frozen >= 0 ? (frozen & 3) : NeedCal(m, fe, 1, 0.0, src);
         fsum = f0;
         if (f0) {
             if (useU) {
@@ -1540,7 +1589,7 @@ __device__ int Step0Coef(const DynaMat& m, const ElemGeo& fe, const ElemPoints& 
                     K[k] += tmp[k];
             }
         }
-        int f1 = NeedCal(m, fe, 1, 1.0, src);
+        int f1 = frozen >= 0 ? ((frozen >> 2) & 3) : NeedCal(m, fe, 1, 1.0, src);
         fsum += f1;
         if (f1) {
             if (useU) {
@@ -1590,7 +1639,8 @@ __global__ void __launch_bounds__(64) k_step0(const ElemGeo* __restrict__ geo, c
                                               const int* __restrict__ dom, const NodeRefs* __restrict__ refs,
                                               const double* __restrict__ su1, const double* __restrict__ st1,
                                               const double* __restrict__ st2, PairGrid grid,
-                                              const long* __restrict__ pairIdx, long count, double* out, int* flags)
+                                              const long* __restrict__ pairIdx, long count,
+                                              const unsigned char* __restrict__ cls0, double* out, int* flags)
 {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count)
@@ -1608,15 +1658,16 @@ __global__ void __launch_bounds__(64) k_step0(const ElemGeo* __restrict__ geo, c
     double b0[9], b1[9];
     int fl0, fl1;
     const NodeRefs& r = refs[f];
+    int fz = cls0 ? (int)cls0[i] : -1;
     if (r.surface == 1) {
         double c[9];
-        fl0 = dev::Step0Coef(m, fe, P, self, u1, t1, t2, src, true, 456, c);
+        fl0 = dev::Step0Coef(m, fe, P, self, u1, t1, t2, src, true, 456, c, fz);
         dev::TransformColumnBlock(c, r.itf_T, b0);
-        fl1 = dev::Step0Coef(m, fe, P, self, u1, t1, t2, src, true, 123, c);
+        fl1 = dev::Step0Coef(m, fe, P, self, u1, t1, t2, src, true, 123, c, fz);
         dev::TransformColumnBlock(c, r.itf_T, b1);
     } else {
-        fl0 = dev::Step0Coef(m, fe, P, self, u1, t1, t2, src, true, r.bcid, b0);
-        fl1 = dev::Step0Coef(m, fe, P, self, u1, t1, t2, src, false, r.bcid, b1);
+        fl0 = dev::Step0Coef(m, fe, P, self, u1, t1, t2, src, true, r.bcid, b0, fz);
+        fl1 = dev::Step0Coef(m, fe, P, self, u1, t1, t2, src, false, r.bcid, b1, fz);
     }
     for (int k = 0; k < 9; ++k) {
         out[18 * i + k] = b0[k];
@@ -1626,14 +1677,39 @@ __global__ void __launch_bounds__(64) k_step0(const ElemGeo* __restrict__ geo, c
     flags[2 * i + 1] = fl1;
 }
 
+// Step-0 judges (lag 0 | lag 1 << 2) of the listed near pairs with the
+// current materials.
+__global__ void __launch_bounds__(256) k_classify0(const ElemGeo* __restrict__ geo, PairGrid grid,
+                                                   const long* __restrict__ pairIdx, long count,
+                                                   unsigned char* __restrict__ cls0)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count)
+        return;
+    int s, f, d;
+    dev::DecodePair(grid, pairIdx[i], s, f, d);
+    const DynaMat& m = c_mats[d];
+    const double* src = geo[s].center;
+    int f0 = dev::NeedCal(m, geo[f], 1, 0.0, src);
+    int f1 = dev::NeedCal(m, geo[f], 1, 1.0, src);
+    cls0[i] = (unsigned char)(f0 | (f1 << 2));
+}
+
+void LaunchClassify0(const ElemGeo* geo, PairGrid grid, const long* pairIdx, long count, unsigned char* cls0,
+                     hipStream_t s)
+{
+    if (count > 0)
+        k_classify0<<<(unsigned)((count + 255) / 256), 256, 0, s>>>(geo, grid, pairIdx, count, cls0);
+}
+
 void LaunchStep0(const ElemGeo* geo, const ElemPoints* pts, const int* dom, const NodeRefs* refs,
                  const double* selfU1, const double* selfT1, const double* selfT2, PairGrid grid,
-                 const long* pairIdx, long count, double* out, int* flags, hipStream_t s)
+                 const long* pairIdx, long count, const unsigned char* cls0, double* out, int* flags, hipStream_t s)
 {
     if (count <= 0)
         return;
     k_step0<<<(unsigned)((count + 63) / 64), 64, 0, s>>>(geo, pts, dom, refs, selfU1, selfT1, selfT2, grid, pairIdx,
-                                                         count, out, flags);
+                                                         count, cls0, out, flags);
 }
 
 // ---------------------------------------------------------------------------

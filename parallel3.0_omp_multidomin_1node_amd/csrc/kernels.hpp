@@ -66,24 +66,32 @@ struct PwWave {
     int nseg;
     int pad;
 };
-void LaunchPwCount(const ElemGeo* geo, PairGrid grid, const int* kb, const int* len, long npairs, int* cnt,
-                   hipStream_t s);
+// cls (nullable): frozen judges of every band entry (f2 | f1 << 2) used
+// instead of judging with the current materials.
+void LaunchPwCount(const ElemGeo* geo, PairGrid grid, const int* kb, const int* len, const long* off,
+                   const unsigned char* cls, long npairs, int* cnt, hipStream_t s);
+void LaunchClassify(const ElemGeo* geo, PairGrid grid, const int* kb, const int* len, const long* off, long npairs,
+                    unsigned char* cls, hipStream_t s);
 template <typename TB>  // TB: stored block type (double, or float in the mixed-precision mode)
 void LaunchHistoryReg(const ElemGeo* geo, const ElemPoints* pts, const NodeRefs* refs, const double* selfT2,
-                      PairGrid grid, const long* coff, const long* off, const int* kb, const int* len, long npairs,
-                      long c0, long c1, TB* mts, TB* mgs, unsigned short* mtsLo, unsigned short* mgsLo,
-                      unsigned char* alive, hipStream_t s);
+                      PairGrid grid, const long* coff, const long* off, const int* kb, const int* len,
+                      const unsigned char* cls, long npairs, long c0, long c1, TB* mts, TB* mgs,
+                      unsigned short* mtsLo, unsigned short* mgsLo, unsigned char* alive, hipStream_t s);
 template <typename TB>
 void LaunchHistoryPw(const ElemGeo* geo, const ElemPoints* pts, const NodeRefs* refs, const double* selfT2,
                      PairGrid grid, const PwWave* waves, long w0, long w1, const long* off, const int* kb,
-                     const int* len, TB* mts, TB* mgs, unsigned short* mtsLo, unsigned short* mgsLo,
-                     unsigned char* alive, hipStream_t s);
+                     const int* len, const unsigned char* cls, TB* mts, TB* mgs, unsigned short* mtsLo,
+                     unsigned short* mgsLo, unsigned char* alive, hipStream_t s);
 
 // Step-0 blocks of the near pairs listed in pairIdx (column-major, ready for
 // AddBlock): out[2*i] and out[2*i+1] with flags.
+// cls0 (nullable): frozen step-0 judges (lag 0 | lag 1 << 2) of the listed pairs.
 void LaunchStep0(const ElemGeo* geo, const ElemPoints* pts, const int* dom, const NodeRefs* refs,
                  const double* selfU1, const double* selfT1, const double* selfT2, PairGrid grid,
-                 const long* pairIdx, long count, double* out, int* flags, hipStream_t s);
+                 const long* pairIdx, long count, const unsigned char* cls0, double* out, int* flags,
+                 hipStream_t s);
+void LaunchClassify0(const ElemGeo* geo, PairGrid grid, const long* pairIdx, long count, unsigned char* cls0,
+                     hipStream_t s);
 
 // History convolution for step K into rhs (rhs += hist).
 void LaunchConvolution(PairGrid grid, const NodeRefs* refs, const long* off, const int* kb, const int* len,

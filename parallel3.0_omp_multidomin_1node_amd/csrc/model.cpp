@@ -112,6 +112,9 @@ int BuildMultiDomainModel(const MultiDomainInputConfig& config, long eleNum, lon
         D.nodeCount = cnt;
         double G = mats[(size_t)d].E / 2.0 / (1.0 + mats[(size_t)d].v);
         D.mat = BuildDynaMat(mats[(size_t)d].v, G, mats[(size_t)d].Rou, dt);
+        D.E = mats[(size_t)d].E;
+        D.v = mats[(size_t)d].v;
+        D.Rou = mats[(size_t)d].Rou;
         for (long e = 0; e < cnt; ++e)
             D.elementIds.push_back(b + e);
         b += cnt;

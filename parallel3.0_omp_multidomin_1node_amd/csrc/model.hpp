@@ -17,6 +17,7 @@ namespace tdbem {
 struct Domain {
     int id = 0;
     DynaMat mat{};
+    double E = 0.0, v = 0.0, Rou = 0.0;  // the material mat was built from
     long eleBegin = 0;
     long eleCount = 0;
     long nodeBegin = 0;

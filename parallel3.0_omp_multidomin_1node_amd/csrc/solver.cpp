@@ -4,6 +4,7 @@
 #include "solver.hpp"
 
 #include <algorithm>
+#include <type_traits>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -262,17 +263,48 @@ void Case::BuildNearLists(int tile)
     nearTile = tile;
 }
 
-void Case::AssembleStep0(long maxleaf)
+// The pairs that have step-0 blocks (near flag of the cull), in pair order.
+std::vector<long> Case::NearPairs()
 {
-    if (culledFor < 1)
-        Cull(1);
-    double t0 = Now();
     std::vector<unsigned char> nearFlag((size_t)npairs);
     Check(hipMemcpy(nearFlag.data(), d_near.p, (size_t)npairs, hipMemcpyDeviceToHost), "near");
     std::vector<long> list;
     for (long p = 0; p < npairs; ++p)
         if (nearFlag[(size_t)p])
             list.push_back(p);
+    return list;
+}
+
+// Freeze (or release) the band structure and the judges of every history
+// entry and step-0 pair at the current materials: later quadratures at other
+// materials keep the same bands and take the same patch-wise / regular paths.
+void Case::FreezeStructure(bool on)
+{
+    if (!on) {
+        frozen = false;
+        d_cls.free();
+        d_cls0.free();
+        return;
+    }
+    if (!historyDone || !step0Done)
+        throw std::runtime_error("freezing needs assembled step 0 and history");
+    d_cls.alloc((size_t)std::max<long>(1, entries));
+    LaunchClassify(d_geo.p, grid, d_kb.p, d_len.p, d_off.p, npairs, d_cls.p, stream);
+    std::vector<long> list = NearPairs();
+    DevBuf<long> d_list;
+    d_list.upload(list.data(), list.size(), stream);
+    d_cls0.alloc(std::max<size_t>(1, list.size()));
+    LaunchClassify0(d_geo.p, grid, d_list.p, (long)list.size(), d_cls0.p, stream);
+    Check(hipStreamSynchronize(stream), "freeze");
+    frozen = true;
+}
+
+// Step-0 matrices A (unknown) and KnownM (known) with the current materials
+// (DBEM1/multidomain.cpp:2256-2327): device blocks of the near pairs, then the
+// AddBlock builders on the host.
+void Case::ComputeStep0(CCSRMat& Aout, CCSRMat& Kout)
+{
+    std::vector<long> list = NearPairs();
     DevBuf<long> d_list;
     d_list.upload(list.data(), list.size(), stream);
     DevBuf<double> d_out;
@@ -280,7 +312,7 @@ void Case::AssembleStep0(long maxleaf)
     d_out.alloc(18 * list.size());
     d_flags.alloc(2 * list.size());
     LaunchStep0(d_geo.p, d_pts.p, d_dom.p, d_refs.p, d_u1.p, d_t1.p, d_t2.p, grid, d_list.p, (long)list.size(),
-                d_out.p, d_flags.p, stream);
+                frozen ? d_cls0.p : nullptr, d_out.p, d_flags.p, stream);
     std::vector<double> out(18 * list.size());
     std::vector<int> flags(2 * list.size());
     Check(hipStreamSynchronize(stream), "step0");
@@ -321,8 +353,16 @@ void Case::AssembleStep0(long maxleaf)
             }
         }
     }
-    bA.Build(A);
-    bK.Build(KnownM);
+    bA.Build(Aout);
+    bK.Build(Kout);
+}
+
+void Case::AssembleStep0(long maxleaf)
+{
+    if (culledFor < 1)
+        Cull(1);
+    double t0 = Now();
+    ComputeStep0(A, KnownM);
     if (!StructureValid())
         throw std::runtime_error("MultiDomain matrix structure check failed (zero rows/cols or non-finite blocks)");
     d_Arp.upload(A.row_ptr.data(), A.row_ptr.size(), stream);
@@ -379,6 +419,8 @@ void Case::AssembleHistory(int maxn_in)
 {
     if (maxn_in < 1)
         throw std::runtime_error("maxn must be >= 1");
+    if (frozen && maxn_in != maxn)
+        throw std::runtime_error("the frozen structure holds the bands of the last history assembly");
     maxn = maxn_in;
     h_mts.clear();
     h_mgs.clear();
@@ -395,7 +437,8 @@ void Case::AssembleHistory(int maxn_in)
     historySplit = mixed == 2;
     if (historyMixed && !quadStaged)
         throw std::runtime_error("the mixed-precision mode needs the staged history quadrature");
-    Cull(maxn);
+    if (!frozen)
+        Cull(maxn);
     size_t freeB = 0, totalB = 0;
     Check(hipMemGetInfo(&freeB, &totalB), "hipMemGetInfo");
     const double vbytes = historyMixed ? (historySplit ? 6.0 : 4.0) : 8.0;
@@ -420,63 +463,10 @@ void Case::AssembleHistory(int maxn_in)
     Check(hipEventCreate(&b), "event");
     Check(hipEventRecord(a, stream), "event");
     if (quadStaged) {
-        // patch-wise lags per pair, then waves that pack the patch-wise lags
-        // of consecutive sources with the same field element
-        DevBuf<int> d_pwcnt;
-        d_pwcnt.alloc((size_t)npairs);
-        LaunchPwCount(d_geo.p, grid, d_kb.p, d_len.p, npairs, d_pwcnt.p, stream);
-        std::vector<int> pwcnt((size_t)npairs);
-        Check(hipMemcpyAsync(pwcnt.data(), d_pwcnt.p, sizeof(int) * (size_t)npairs, hipMemcpyDeviceToHost, stream),
-              "pw counts");
-        Check(hipStreamSynchronize(stream), "pw counts");
-        std::vector<PwWave> waves;
-        PwWave cur{};
-        int lanes = 0;
-        auto flush = [&]() {
-            if (cur.nseg > 0)
-                waves.push_back(cur);
-            cur = PwWave{};
-            lanes = 0;
-        };
-        for (int d = 0; d < grid.nd; ++d) {
-            long nd = grid.eleCount[d];
-            for (long fl = 0; fl < nd; ++fl) {
-                for (long sl = 0; sl < nd; ++sl) {
-                    long p = grid.pairBase[d] + sl * nd + fl;
-                    int c = pwcnt[(size_t)p], r0 = 0;
-                    while (c > 0) {
-                        if (cur.nseg == kPwSlots || lanes == 64)
-                            flush();
-                        int take = std::min(c, 64 - lanes);
-                        cur.pair[cur.nseg] = p;
-                        cur.rank0[cur.nseg] = r0;
-                        cur.cnt[cur.nseg] = take;
-                        cur.nseg++;
-                        lanes += take;
-                        c -= take;
-                        r0 += take;
-                    }
-                }
-                flush();
-            }
-        }
-        pwWaves = (long)waves.size();
-        DevBuf<PwWave> d_waves;
-        d_waves.upload(waves.data(), waves.size(), stream);
-        const long step = 1L << 20;  // wavefronts per launch
-        auto launch = [&](auto* mts, auto* mgs) {
-            for (long c0 = 0; c0 < chunks; c0 += step)
-                LaunchHistoryReg(d_geo.p, d_pts.p, d_refs.p, d_t2.p, grid, d_coff.p, d_off.p, d_kb.p, d_len.p, npairs,
-                                 c0, std::min(chunks, c0 + step), mts, mgs, d_mtsLo.p, d_mgsLo.p, d_alive.p, stream);
-            for (long w0 = 0; w0 < pwWaves; w0 += step)
-                LaunchHistoryPw(d_geo.p, d_pts.p, d_refs.p, d_t2.p, grid, d_waves.p, w0, std::min(pwWaves, w0 + step),
-                                d_off.p, d_kb.p, d_len.p, mts, mgs, d_mtsLo.p, d_mgsLo.p, d_alive.p, stream);
-        };
         if (historyMixed)
-            launch(d_mts32.p, d_mgs32.p);
+            StagedQuadrature(nullptr, nullptr, d_mts32.p, d_mgs32.p, d_alive.p);
         else
-            launch(d_mts.p, d_mgs.p);
-        Check(hipStreamSynchronize(stream), "history");
+            StagedQuadrature(d_mts.p, d_mgs.p, nullptr, nullptr, d_alive.p);
     } else {
         const long chunk = 1L << 24;
         for (long e0 = 0; e0 < entries; e0 += chunk) {
@@ -494,6 +484,71 @@ void Case::AssembleHistory(int maxn_in)
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     historyDone = true;
+}
+
+// Staged history quadrature of the current bands with the current materials
+// into mts / mgs (FP64) or mts32 / mgs32 (FP32, plus the bf16 residuals of the
+// split mode); alive receives the presence bits.
+void Case::StagedQuadrature(double* mts, double* mgs, float* mts32, float* mgs32, unsigned char* alive)
+{
+    // patch-wise lags per pair, then waves that pack the patch-wise lags
+    // of consecutive sources with the same field element
+    DevBuf<int> d_pwcnt;
+    d_pwcnt.alloc((size_t)npairs);
+    const unsigned char* cls = frozen ? d_cls.p : nullptr;
+    LaunchPwCount(d_geo.p, grid, d_kb.p, d_len.p, d_off.p, cls, npairs, d_pwcnt.p, stream);
+    std::vector<int> pwcnt((size_t)npairs);
+    Check(hipMemcpyAsync(pwcnt.data(), d_pwcnt.p, sizeof(int) * (size_t)npairs, hipMemcpyDeviceToHost, stream),
+          "pw counts");
+    Check(hipStreamSynchronize(stream), "pw counts");
+    std::vector<PwWave> waves;
+    PwWave cur{};
+    int lanes = 0;
+    auto flush = [&]() {
+        if (cur.nseg > 0)
+            waves.push_back(cur);
+        cur = PwWave{};
+        lanes = 0;
+    };
+    for (int d = 0; d < grid.nd; ++d) {
+        long nd = grid.eleCount[d];
+        for (long fl = 0; fl < nd; ++fl) {
+            for (long sl = 0; sl < nd; ++sl) {
+                long p = grid.pairBase[d] + sl * nd + fl;
+                int c = pwcnt[(size_t)p], r0 = 0;
+                while (c > 0) {
+                    if (cur.nseg == kPwSlots || lanes == 64)
+                        flush();
+                    int take = std::min(c, 64 - lanes);
+                    cur.pair[cur.nseg] = p;
+                    cur.rank0[cur.nseg] = r0;
+                    cur.cnt[cur.nseg] = take;
+                    cur.nseg++;
+                    lanes += take;
+                    c -= take;
+                    r0 += take;
+                }
+            }
+            flush();
+        }
+    }
+    pwWaves = (long)waves.size();
+    DevBuf<PwWave> d_waves;
+    d_waves.upload(waves.data(), waves.size(), stream);
+    const long step = 1L << 20;  // wavefronts per launch
+    auto launch = [&](auto* mts, auto* mgs) {
+        for (long c0 = 0; c0 < chunks; c0 += step)
+            LaunchHistoryReg(d_geo.p, d_pts.p, d_refs.p, d_t2.p, grid, d_coff.p, d_off.p, d_kb.p, d_len.p, cls,
+                             npairs, c0, std::min(chunks, c0 + step), mts, mgs, d_mtsLo.p, d_mgsLo.p, alive, stream);
+        for (long w0 = 0; w0 < pwWaves; w0 += step)
+            LaunchHistoryPw(d_geo.p, d_pts.p, d_refs.p, d_t2.p, grid, d_waves.p, w0, std::min(pwWaves, w0 + step),
+                            d_off.p, d_kb.p, d_len.p, cls, mts, mgs, d_mtsLo.p, d_mgsLo.p, alive, stream);
+    };
+    if (mts32)
+        launch(mts32, mgs32);
+    else
+        launch(mts, mgs);
+    Check(hipStreamSynchronize(stream), "history");
 }
 
 void Case::HistoryStats(long* counts) const
@@ -654,10 +709,20 @@ void Case::PrepareMixed(long restart)
 // inverses and Krylov vectors stored as T; reductions, Hessenberg, Givens and
 // the solution x are FP64.  T = double is the parity path.
 template <typename T>
-int Case::GmresCore(const double* d_b, double* d_x, long restart, double tol, int* iters, const T* Aval, const T* inv,
-                    T* V, T* W, T* u, T* u2)
+int Case::GmresCore(const double* d_b, double* d_x, long restart, double tol, int* iters, const GmresOp<T>& op, T* V,
+                    T* W, T* u, T* u2)
 {
     const long n = 3 * ne;
+    auto applyPre = [&](auto in, T* out) {
+        if constexpr (std::is_same<T, double>::value) {
+            if (op.prp) {
+                LaunchSpmv(ne, op.prp, op.pcol, op.pval, in, out, stream);
+                return;
+            }
+        }
+        LaunchPrecond(ne, d_posLeaf.p, d_leafBegin.p, d_leafSize.p, d_matOff.p, op.inv, d_repid.p, d_outBlock.p, in,
+                      out, stream);
+    };
     double* normr = d_scal.p;      // [0]
     double* bnorm = d_scal.p + 1;  // [1]
     double* phibar = d_scal.p + 2; // [2]
@@ -671,7 +736,7 @@ int Case::GmresCore(const double* d_b, double* d_x, long restart, double tol, in
         return 0;
     double tolb = tol * n2b;
     // r = b - A x0 (x0 = 0)
-    LaunchSpmv(ne, d_Arp.p, d_Acol.p, Aval, (const double*)d_x, d_vh.p, stream);
+    LaunchSpmv(ne, op.rp, op.col, op.val, (const double*)d_x, d_vh.p, stream);
     LaunchResidual(d_b, d_vh.p, n, stream);
     LaunchNorm((const double*)d_vh.p, n, d_part.p, gm_blocks, normr, stream);
     Check(hipMemcpyAsync(h_pin, normr, sizeof(double), hipMemcpyDeviceToHost, stream), "d2h");
@@ -681,8 +746,7 @@ int Case::GmresCore(const double* d_b, double* d_x, long restart, double tol, in
     Check(hipMemsetAsync(d_QT.p, 0, sizeof(double) * d_QT.n, stream), "memset");
     Check(hipMemsetAsync(d_R.p, 0, sizeof(double) * d_R.n, stream), "memset");
     Check(hipMemsetAsync(d_stag.p, 0, sizeof(int), stream), "memset");
-    LaunchPrecond(ne, d_posLeaf.p, d_leafBegin.p, d_leafSize.p, d_matOff.p, inv, d_repid.p, d_outBlock.p,
-                  (const double*)d_vh.p, u, stream);
+    applyPre((const double*)d_vh.p, u);
     LaunchNorm((const T*)u, n, d_part.p, gm_blocks, d_h.p + 1, stream);
     LaunchScaleCopy((const T*)u, d_h.p + 1, 1, V, n, stream);
     const double one = 1.0;
@@ -690,9 +754,8 @@ int Case::GmresCore(const double* d_b, double* d_x, long restart, double tol, in
     Check(hipMemcpyAsync(phibar, d_h.p + 1, sizeof(double), hipMemcpyDeviceToDevice, stream), "phibar");
     for (int j = 1; j <= restart; ++j) {
         T* Vj = V + (size_t)(j - 1) * (size_t)n;
-        LaunchSpmv(ne, d_Arp.p, d_Acol.p, Aval, (const T*)Vj, u2, stream);
-        LaunchPrecond(ne, d_posLeaf.p, d_leafBegin.p, d_leafSize.p, d_matOff.p, inv, d_repid.p, d_outBlock.p,
-                      (const T*)u2, u, stream);
+        LaunchSpmv(ne, op.rp, op.col, op.val, (const T*)Vj, u2, stream);
+        applyPre((const T*)u2, u);
         LaunchMultiDot((const T*)V, n, j, (const T*)u, n, d_part.p, gm_blocks, d_h.p + 1, stream);
         LaunchMultiAxpy((const T*)V, n, j, d_h.p + 1, u, n, nullptr, stream);
         LaunchMultiDot((const T*)V, n, j, (const T*)u, n, d_part.p, gm_blocks, d_h2.p, stream);
@@ -701,7 +764,7 @@ int Case::GmresCore(const double* d_b, double* d_x, long restart, double tol, in
         LaunchScaleCopy((const T*)u, d_h.p + j + 1, 1, V + (size_t)j * (size_t)n, n, stream);
         LaunchGivens(d_h.p, d_QT.p, d_R.p, d_f.p, phibar, d_stag.p, j, (int)restart, stream);
         LaunchWUpdate((const T*)V, W, n, d_R.p, j, (int)restart, d_f.p, d_x, stream);
-        LaunchSpmv(ne, d_Arp.p, d_Acol.p, Aval, (const double*)d_x, d_vh.p, stream);
+        LaunchSpmv(ne, op.rp, op.col, op.val, (const double*)d_x, d_vh.p, stream);
         LaunchResidual(d_b, d_vh.p, n, stream);
         LaunchNorm((const double*)d_vh.p, n, d_part.p, gm_blocks, normr, stream);
         Check(hipMemcpyAsync(h_pin, normr, sizeof(double), hipMemcpyDeviceToHost, stream), "d2h");
@@ -718,14 +781,15 @@ int Case::GmresCore(const double* d_b, double* d_x, long restart, double tol, in
     return 1;
 }
 
-template int Case::GmresCore<double>(const double*, double*, long, double, int*, const double*, const double*,
-                                     double*, double*, double*, double*);
-template int Case::GmresCore<float>(const double*, double*, long, double, int*, const float*, const float*, float*,
-                                    float*, float*, float*);
+template int Case::GmresCore<double>(const double*, double*, long, double, int*, const GmresOp<double>&, double*,
+                                     double*, double*, double*);
+template int Case::GmresCore<float>(const double*, double*, long, double, int*, const GmresOp<float>&, float*, float*,
+                                    float*, float*);
 
 int Case::Gmres(const double* d_b, double* d_x, long restart, double tol, int* iters)
 {
-    return GmresCore<double>(d_b, d_x, restart, tol, iters, d_Aval.p, d_inv.p, d_V.p, d_W.p, d_u.p, d_u2.p);
+    GmresOp<double> op{d_Arp.p, d_Acol.p, d_Aval.p, d_inv.p, nullptr, nullptr, nullptr};
+    return GmresCore<double>(d_b, d_x, restart, tol, iters, op, d_V.p, d_W.p, d_u.p, d_u2.p);
 }
 
 // Mixed precision: FP32-stored inner GMRES (GmresCore<float>) on A d = r
@@ -753,8 +817,8 @@ int Case::GmresMixed(const double* d_b, double* d_x, long restart, double tol, i
     for (int round = 0; round < kMaxRefine; ++round) {
         double itol = std::max(kInnerTol, tol * n2b / normr);
         int it = 0;
-        int f = GmresCore<float>(d_r.p, d_dx.p, restart, itol, &it, d_Aval32.p, d_inv32.p, d_V32.p, d_W32.p, d_u32.p,
-                                 d_u232.p);
+        GmresOp<float> op{d_Arp.p, d_Acol.p, d_Aval32.p, d_inv32.p, nullptr, nullptr, nullptr};
+        int f = GmresCore<float>(d_r.p, d_dx.p, restart, itol, &it, op, d_V32.p, d_W32.p, d_u32.p, d_u232.p);
         total += it;
         LaunchAdd(d_dx.p, d_x, n, stream);
         LaunchSpmv(ne, d_Arp.p, d_Acol.p, (const double*)d_Aval.p, (const double*)d_x, d_r.p, stream);
@@ -791,6 +855,10 @@ void Case::Solve(int nstep, const double* known_lu, long restart, double tol, do
         BuildNearLists(convTile);
     const long n3 = 3 * ne;
     const int nstore = nstep + 1;
+    if (keep) {
+        keep->X.alloc((size_t)nstore * (size_t)n3);
+        LaunchZero(keep->X.p, (long)keep->X.n, stream);
+    }
     DevBuf<double> d_bd, d_lu, d_lt, d_gU, d_gT, d_known, d_rhs, d_x, d_hfar;
     d_bd.upload(known_lu, (size_t)nstore * (size_t)n3, stream);
     d_lu.alloc((size_t)nstore * (size_t)n3);
@@ -875,8 +943,20 @@ void Case::Solve(int nstep, const double* known_lu, long restart, double tol, do
         double* slt = d_lt.p + (size_t)K * (size_t)n3;
         LaunchScatter(d_refs.p, d_geo.p, d_bd.p + (size_t)K * (size_t)n3, d_x.p, (int)ne, slu, slt, stream);
         LaunchState(d_refs.p, slu, slt, (int)ne, K, nstore, d_gU.p, d_gT.p, stream);
+        if (keep)
+            Check(hipMemcpyAsync(keep->X.p + (size_t)K * (size_t)n3, d_x.p, sizeof(double) * (size_t)n3,
+                                 hipMemcpyDeviceToDevice, stream),
+                  "keep x");
     }
     Check(hipStreamSynchronize(stream), "solve");
+    if (keep) {  // hand the forward history to the adjoint
+        std::swap(keep->gU.p, d_gU.p);
+        std::swap(keep->gU.n, d_gU.n);
+        std::swap(keep->gT.p, d_gT.p);
+        std::swap(keep->gT.n, d_gT.n);
+        std::swap(keep->bd.p, d_bd.p);
+        std::swap(keep->bd.n, d_bd.n);
+    }
     Check(hipGetLastError(), "solve launch");
     Check(hipMemcpy(lu, d_lu.p, sizeof(double) * d_lu.n, hipMemcpyDeviceToHost), "lu");
     Check(hipMemcpy(lt, d_lt.p, sizeof(double) * d_lt.n, hipMemcpyDeviceToHost), "lt");

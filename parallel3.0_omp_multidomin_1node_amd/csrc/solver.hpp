@@ -23,6 +23,26 @@ struct DevBuf {
     void upload(const T* h, size_t count, hipStream_t s);
 };
 
+// Operator of one GMRES solve: block CCSR matrix (rp, col, val) and either the
+// leaf preconditioner inverses (inv) or an explicit block CCSR preconditioner
+// (prp, pcol, pval; FP64 only).
+template <typename T>
+struct GmresOp {
+    const int* rp;
+    const int* col;
+    const T* val;
+    const T* inv;
+    const int* prp;
+    const int* pcol;
+    const T* pval;
+};
+
+// Forward history a solve hands to the adjoint: unknown vectors x_K
+// (K = 0 .. NStep), state columns and the device copy of the known inputs.
+struct AdjointKeep {
+    DevBuf<double> X, gU, gT, bd;
+};
+
 // Workspace of the lockstep GMRES of a batch of load cases.
 struct BatchWork {
     DevBuf<double> V, W, u, u2, vh, h, h2, QT, R, f, phibar, scal, part;
@@ -50,6 +70,20 @@ public:
     void SolveBatched(int nstep, int B, const double* known_lu, long restart, double tol, double* lu, double* lt,
                       int* iters, int* flags, int* solved);
     void HistoryStats(long* counts) const;
+    // Reverse-time adjoint of the FP64 time loop for the misfit
+    // J = sum_{K=1..NStep} sum_r |lu_K[r] - obs[K][r]|^2 over the receiver
+    // elements r (global-frame displacements).  Returns J; grad[3 d + p] =
+    // dJ/d(E, v, rho)_p of domain d; lu (may be null) receives the forward
+    // displacements.  fdStep: relative step of the central differences of
+    // the operators over the materials.
+    double Adjoint(int nstep, const double* known_lu, const int* receivers, int nrec, const double* obs, long restart,
+                   double tol, double fdStep, double* grad, double* lu);
+    // Per-domain materials (E, v, rho) x domains; setting them rebuilds the
+    // DynaMats and the singular self tables (step 0 and history must be
+    // assembled again before a solve).
+    void GetMaterials(double* evr) const;
+    void FreezeStructure(bool on);
+    void SetMaterials(const double* evr);
     void ExportMatrix(int which, int step, std::vector<int>& rp, std::vector<int>& col, std::vector<double>& val);
 
     long ne = 0;
@@ -79,13 +113,19 @@ public:
 private:
     void Cull(int maxn);
     void BuildNearLists(int tile);
+    void StagedQuadrature(double* mts, double* mgs, float* mts32, float* mgs32, unsigned char* alive);
+    void ComputeStep0(CCSRMat& Aout, CCSRMat& Kout);
+    std::vector<long> NearPairs();
+    bool frozen = false;
+    DevBuf<unsigned char> d_cls, d_cls0;
+    AdjointKeep* keep = nullptr;
     bool StructureValid() const;
     void PrepareGmres(long restart);
     void PrepareMixed(long restart);
     int Gmres(const double* d_b, double* d_x, long restart, double tol, int* iters);
     template <typename T>
-    int GmresCore(const double* d_b, double* d_x, long restart, double tol, int* iters, const T* Aval, const T* inv,
-                  T* V, T* W, T* u, T* u2);
+    int GmresCore(const double* d_b, double* d_x, long restart, double tol, int* iters, const GmresOp<T>& op, T* V,
+                  T* W, T* u, T* u2);
     int GmresMixed(const double* d_b, double* d_x, long restart, double tol, int* iters);
     void GmresBatched(int B, const double* rhs, double* x, long restart, double tol, int* its, int* flg);
     BatchWork bw;

@@ -23,7 +23,7 @@ _lp = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
 EXPORTED = [
     "tdbem_last_error", "tdbem_create", "tdbem_destroy", "tdbem_dofmap_export", "tdbem_assemble_step0",
     "tdbem_assemble_history", "tdbem_ccsr_export", "tdbem_self_tables", "tdbem_solve", "tdbem_solve_batched",
-    "tdbem_dyna_gmres_solver",
+    "tdbem_adjoint", "tdbem_get_materials", "tdbem_set_materials", "tdbem_dyna_gmres_solver",
     "tdbem_timings", "tdbem_history_stats", "tdbem_device_synchronize",
     "tdbem_set_option",
 ]
@@ -58,6 +58,10 @@ def lib():
                                   C.POINTER(C.c_int), _dp]
         L.tdbem_solve_batched.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, C.c_long, C.c_double, _dp, _dp, _ip,
                                           _ip, _ip, _dp]
+        L.tdbem_adjoint.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, _ip, _dp, C.c_long, C.c_double, C.c_double,
+                                    C.POINTER(C.c_double), _dp, C.c_void_p]
+        L.tdbem_get_materials.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), _dp]
+        L.tdbem_set_materials.argtypes = [C.c_void_p, _dp]
         L.tdbem_dyna_gmres_solver.argtypes = [C.c_void_p, C.c_long, C.c_double, C.c_long, C.c_int, _dp, _dp, _dp,
                                               C.POINTER(C.c_int)]
         L.tdbem_timings.argtypes = [C.c_void_p, _dp]
@@ -183,6 +187,34 @@ class Case:
                                          iters, flags, solved, times), "tdbem_solve_batched")
         shape = (nrhs, nstep + 1, self.ne, 3)
         return lu.reshape(shape), lt.reshape(shape), iters, flags, solved, times
+
+    def adjoint(self, known_lu, nstep, receivers, obs, restart=200, tol=1e-12, fd_step=1e-6):
+        """Misfit J = sum_{K>=1} sum_r |lu_K[r] - obs[K, r]|^2 over the receiver
+        elements and its gradient (domains, 3) with respect to (E, v, rho) per
+        domain, by the reverse-time adjoint; also returns the forward lu."""
+        known = np.ascontiguousarray(known_lu, np.float64).reshape(-1)
+        rec = np.ascontiguousarray(receivers, np.int32).reshape(-1)
+        ob = np.ascontiguousarray(obs, np.float64).reshape(-1)
+        if ob.size != (nstep + 1) * rec.size * 3:
+            raise ValueError("obs must hold (nstep+1) x nrec x 3 values")
+        nd = len(self.materials())
+        J = C.c_double()
+        grad = np.zeros(3 * nd)
+        lu = np.zeros((nstep + 1) * self.ne * 3)
+        _check(lib().tdbem_adjoint(self.h, int(nstep), known, int(rec.size), rec, ob, int(restart), float(tol),
+                                   float(fd_step), C.byref(J), grad, lu.ctypes.data_as(C.c_void_p)), "tdbem_adjoint")
+        return J.value, grad.reshape(nd, 3), lu.reshape(nstep + 1, self.ne, 3)
+
+    def materials(self):
+        """(domains, 3) array of (E, v, rho)."""
+        out = np.zeros(3 * 64)
+        nd = C.c_int()
+        _check(lib().tdbem_get_materials(self.h, 64, C.byref(nd), out), "tdbem_get_materials")
+        return out[:3 * nd.value].reshape(nd.value, 3)
+
+    def set_materials(self, evr):
+        _check(lib().tdbem_set_materials(self.h, np.ascontiguousarray(evr, np.float64).reshape(-1)),
+               "tdbem_set_materials")
 
     def set_option(self, name, value):
         _check(lib().tdbem_set_option(self.h, name.encode(), float(value)), "tdbem_set_option")

@@ -17,7 +17,8 @@ timed region; the GPU holds everything else.  Prints one JSON line.  Beside
 the headline FP64 value the line carries a "batched" block (64 seeded load
 cases on the same block solved as one batch) and a "mixed_precision" block
 (the 8-domain 8192-element 600-step box in FP64 and in both mixed modes, with
-the error of the mixed runs against the FP64 run).
+the error of the mixed runs against the FP64 run) and an "adjoint" block
+(adjoint gradients/s of a receiver misfit on the 4-domain 3456-element block).
 """
 from __future__ import annotations
 
@@ -191,6 +192,41 @@ def batched_report(tdbem, w, single_steps_per_s, nrhs=64):
                             "gmres": round(float(times[1]), 3)}}
 
 
+def adjoint_report(tdbem):
+    """Reverse-time adjoint on the 1x1x4 block (3456 elements, 4 domains, 300
+    steps): misfit over 32 receiver elements against observations from
+    materials perturbed by +-10 %, gradient with respect to each domain's
+    (E, v, rho).  Timed: one gradient (forward solve, backward sweep, operator
+    differences and contraction) after the case's history assembly."""
+    from tdbem import mesh
+    a = mesh.adjoint_config5()
+    w = a.workload
+    m = w.mesh
+
+    def case(mats):
+        c = tdbem.Case(m.points, m.ele, m.bcid, w.dt, w.gap, domain_count=4, mat_rows=mats, itf_rows=m.interfaces)
+        c.assemble_step0(10)
+        c.assemble_history(w.nstep)
+        return c
+
+    c = case(a.obs_mats)
+    U, *_ = c.solve(w.known, w.nstep, tol=1e-12)
+    c.close()
+    obs = U[:, a.receivers, :]
+    t0 = time.perf_counter()
+    c = case(w.mats)
+    t1 = time.perf_counter()
+    J, g, _ = c.adjoint(w.known, w.nstep, a.receivers, obs, tol=1e-12, fd_step=1e-3)
+    tdbem.synchronize()
+    t2 = time.perf_counter()
+    c.close()
+    log(f"adjoint: assembly {t1 - t0:.2f} s, gradient {t2 - t1:.2f} s")
+    return {"workload": w.name, "elements": int(len(m.ele)), "domains": 4, "nstep": w.nstep,
+            "receivers": int(len(a.receivers)), "gradients_per_s": round(1.0 / (t2 - t1), 4),
+            "gradient_s": round(t2 - t1, 3), "assembly_s": round(t1 - t0, 3), "misfit": J,
+            "gradient_E_v_rho": np.asarray(g).round(12).tolist(), "operator_difference_step": 1e-3}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -204,6 +240,7 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="also run one device copy of BYTES (profiler counter calibration)")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-load-case batched report")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-gradient report")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
     args = ap.parse_args()
     if args.gpus != 1:
@@ -275,6 +312,8 @@ def main():
         out["batched"] = batched_report(tdbem, w, value)
     if not args.no_mixed:
         out["mixed_precision"] = mixed_precision_report(tdbem)
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_report(tdbem)
     if not args.no_cpu_baseline:
         log(f"cpu baseline on {args.cpu_threads} threads ...")
         cb = cpu_baseline(args.cpu_threads)

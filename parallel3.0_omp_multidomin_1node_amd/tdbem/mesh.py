@@ -229,3 +229,34 @@ def load_cases(w: Workload, nrhs: int = 64, seed: int = 4) -> np.ndarray:
             sig = _heaviside(w.nstep)
         out[b] = normal_load_history(ne, w.nstep, w.dt, patch, amp * sig, direction=d)
     return out
+
+
+@dataclass
+class AdjointWorkload:
+    workload: Workload
+    receivers: np.ndarray   # element ids
+    obs_mats: np.ndarray    # materials of the observed run (id, E, v, rho)
+
+
+def adjoint_config5(h: float = 1.0 / 12.0, nstep: int = 300, seed: int = 5, nrec: int = 32) -> AdjointWorkload:
+    """1x1x4 block of 4 unit-cube domains (h = 1/12 gives 3456 elements), seeded E in [1,10], v in
+    [0.2,0.35], rho in [1,3], Ricker normal traction on the top face, dt = 0.5 h / max c1; nrec
+    seeded receiver elements; observations come from the same run with every material value
+    perturbed by a seeded factor in [0.9, 1.1]."""
+    n = int(round(1.0 / h))
+    m = box_mesh(4, 4.0, 1.0, 1.0, n, n, n)
+    rng = np.random.default_rng(seed)
+    E = rng.uniform(1.0, 10.0, 4)
+    v = rng.uniform(0.2, 0.35, 4)
+    rho = rng.uniform(1.0, 3.0, 4)
+    c1max = max(wave_speeds(E[d], v[d], rho[d])[0] for d in range(4))
+    dt = 0.5 * h / c1max
+    f0 = rng.uniform(0.5, 1.5) / (40 * dt)
+    sig = ricker(np.arange(nstep + 1) * dt, f0)
+    sig[0] = 0.0
+    known = normal_load_history(len(m.ele), nstep, dt, m.top, sig)
+    mats = np.array([[d + 1, E[d], v[d], rho[d]] for d in range(4)])
+    recv = np.sort(rng.choice(len(m.ele), nrec, replace=False))
+    obs = mats.copy()
+    obs[:, 1:] *= rng.uniform(0.9, 1.1, size=(4, 3))
+    return AdjointWorkload(Workload(f"block_1x1x4_h{h:.4g}", m, mats, dt, nstep, known), recv, obs)

@@ -144,8 +144,9 @@ int tdbem_history_stats(tdbem_ctx* ctx, long* counts, long* npairs);
  *   the current materials, so assemblies after tdbem_set_materials stay on
  *   the same piece of the piecewise-smooth operator (0 releases them);
  *   "quad_staged" 1 (default) wave-per-pair history quadrature with the
- *   point geometry staged in LDS, 0 one lane per band entry with separate
- *   T2 / T1 / U passes (same results). */
+ *   point terms staged in LDS as basis tensors (same band pattern, blocks
+ *   within 1e-10 relative of the reference's arithmetic), 0 one lane per
+ *   band entry in the reference's operation order (bit-identical to it). */
 int tdbem_set_option(tdbem_ctx* ctx, const char* name, double value);
 
 /* Wait for all device work of this process (hipDeviceSynchronize). */

@@ -834,219 +834,174 @@ void LaunchHistory(const ElemGeo* geo, const ElemPoints* pts, const int* dom, co
 // ---------------------------------------------------------------------------
 namespace dev {
 
-constexpr int kStageStride = 70;  // doubles per staged point (16-byte aligned records)
-enum StageField : int {
-    sRc1 = 0,     // R / C1
-    sRc2 = 1,     // R / C2
-    sRC1Dt = 2,   // R / (C1 dt)
-    sRC2Dt = 3,   // R / (C2 dt)
-    sR1 = 4,
-    sR2 = 5,
-    sDt3R3 = 6,   // dt^3 / R^3
-    sW = 7,       // weight * Jacobian
-    sD = 8,       // Dij
-    sGt = 17,     // Gij * (1 / (4 pi dt))
-    sE = 26,      // Eij
-    sHt = 35,     // Hij * (1 / (4 pi dt))
-    sF = 44,      // Fij, entries 0 1 2 4 5 8
-    sUP = 50,     // P-front U terms, entries 0 1 2 4 5 8
-    sUS = 56,     // S-front U terms
-    sUC = 62,     // between-fronts U tensor (times the ISeries factor per lane)
+// Staged (source, point) record of the wave-per-pair quadrature.  Every
+// kernel tensor at one point is a combination of three basis tensors,
+//   X_ab = r_a r_b,  Q_ab = r_b n_a,  B_ab = r_a n_b + delta_ab (r . n),
+//   Dij = sXd X + sQd Q + sBd B,   Eij = sXe X + sQe Q + sBe B,
+//   Fij = sXf X + sQBf (Q + B),    Gij = sXg X + sQg Q,   Hij = sXh X + sBh B,
+// (DBEM1/dsquareelement.h:631-903), and every U term is a combination of X
+// and delta_ab.  The record keeps the basis and those scalars (times the
+// point weight, with 1/(4 pi dt) folded into G and H and 1/(4 pi rho) into
+// U), so a lane's contribution at a point is cX X + cQ Q + cB B with three
+// scalars built from its time coefficients.
+constexpr int kRec = 42;  // doubles per staged point (16-byte multiple)
+enum RecField : int {
+    qRc1 = 0,   // R / C1
+    qRc2 = 1,   // R / C2
+    qPc1 = 2,   // R / (C1 dt) / (4 pi)
+    qPc2 = 3,   // R / (C2 dt) / (4 pi)
+    qIR2h = 4,  // 1 / (2 R^2)
+    qIR3t = 5,  // 1 / (3 R^3)
+    qRk = 6,    // R / dt / (4 pi)
+    qSXd = 7,
+    qSQd = 8,
+    qSBd = 9,
+    qSXe = 10,
+    qSQe = 11,
+    qSBe = 12,
+    qSXf = 13,
+    qSQBf = 14,
+    qSXg = 15,
+    qSQg = 16,
+    qSXh = 17,
+    qSBh = 18,
+    qUP = 19,   // P-front U = uP X
+    qUS1 = 20,  // S-front U = uS1 X + uS0 delta
+    qUS0 = 21,
+    qUC1 = 22,  // between-fronts U = ri (uC1 X + uC0 delta)
+    qUC0 = 23,
+    qX = 24,    // X: 00 01 02 11 12 22
+    qQ = 30,    // Q: 9 entries, row-major
+    qBd = 39,   // diagonal of B: Q_aa + r . n
 };
 
-__device__ void StagePoint(const DynaMat& m, const double* src, const double* pos, const double* nor, double w,
-                           double* g)
+__device__ void StageRecord(const DynaMat& m, const double* src, const double* pos, const double* nor, double w,
+                            double* g)
 {
-    Dist d;
-    MakeDist(src, pos, d);
-    const double* R = d.R;
-    const double* X = d.X;
-    g[sRc1] = R[1] / m.C1;
-    g[sRc2] = R[1] / m.C2;
-    g[sRC1Dt] = R[1] / m.C1Dt;
-    g[sRC2Dt] = R[1] / m.C2Dt;
-    g[sR1] = R[1];
-    g[sR2] = R[2];
-    g[sDt3R3] = m.Dt_3 / R[3];
-    g[sW] = w;
-    double K[9];
-    double t2 = kOneOver4Pi / m.Dt;
-    Dij(m, d, nor, K);
-    for (int k = 0; k < 9; ++k)
-        g[sD + k] = K[k];
-    Gij(m, d, nor, K);
-    for (int k = 0; k < 9; ++k)
-        g[sGt + k] = K[k] * t2;
-    Eij(d, nor, K);
-    for (int k = 0; k < 9; ++k)
-        g[sE + k] = K[k];
-    Hij(m, d, nor, K);
-    for (int k = 0; k < 9; ++k)
-        g[sHt + k] = K[k] * t2;
-    Fij(m, d, nor, K);
-    g[sF + 0] = K[0];
-    g[sF + 1] = K[1];
-    g[sF + 2] = K[2];
-    g[sF + 3] = K[4];
-    g[sF + 4] = K[5];
-    g[sF + 5] = K[8];
-    double t1 = m.OneOverFourPaiRou;
-    {
-        double C1 = m.OneOverC1_2 / R[3];
-        g[sUP + 0] = (C1 * X[0]) * t1;
-        g[sUP + 1] = (C1 * X[1]) * t1;
-        g[sUP + 2] = (C1 * X[2]) * t1;
-        g[sUP + 3] = (C1 * X[3]) * t1;
-        g[sUP + 4] = (C1 * X[4]) * t1;
-        g[sUP + 5] = (C1 * X[5]) * t1;
-    }
-    {
-        double c = m.OneOverC2_2;
-        g[sUS + 0] = (c * (1.0 / R[1] - X[0] / R[3])) * t1;
-        g[sUS + 1] = (c * (-X[1] / R[3])) * t1;
-        g[sUS + 2] = (c * (-X[2] / R[3])) * t1;
-        g[sUS + 3] = (c * (1.0 / R[1] - X[3] / R[3])) * t1;
-        g[sUS + 4] = (c * (-X[4] / R[3])) * t1;
-        g[sUS + 5] = (c * (1.0 / R[1] - X[5] / R[3])) * t1;
-    }
-    {
-        double C1 = 3.0 / R[3], C2 = 1.0 / R[1];
-        g[sUC + 0] = C1 * X[0] - C2;
-        g[sUC + 1] = C1 * X[1];
-        g[sUC + 2] = C1 * X[2];
-        g[sUC + 3] = C1 * X[3] - C2;
-        g[sUC + 4] = C1 * X[4];
-        g[sUC + 5] = C1 * X[5] - C2;
-    }
+    double r[3] = {pos[0] - src[0], pos[1] - src[1], pos[2] - src[2]};
+    double R = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+    double iR = 1.0 / R;
+    double iR2 = iR * iR, iR3 = iR2 * iR, iR4 = iR2 * iR2, iR5 = iR4 * iR;
+    double rn = r[0] * nor[0] + r[1] * nor[1] + r[2] * nor[2];
+    g[qRc1] = R / m.C1;
+    g[qRc2] = R / m.C2;
+    g[qPc1] = kOneOver4Pi * (R / m.C1Dt);
+    g[qPc2] = kOneOver4Pi * (R / m.C2Dt);
+    g[qIR2h] = 0.5 * iR2;
+    g[qIR3t] = iR3 * (1.0 / 3.0);
+    g[qRk] = kOneOver4Pi * (R / m.Dt);
+    double c = m.C2_2OverC1_2;
+    double tg = kOneOver4Pi / m.Dt * w;
+    g[qSXd] = -(c * 12.0 * rn * iR5) * w;
+    g[qSQd] = (4.0 * c - 1.0) * iR3 * w;
+    g[qSBd] = c * 2.0 * iR3 * w;
+    g[qSXe] = 12.0 * rn * iR5 * w;
+    g[qSQe] = -2.0 * iR3 * w;
+    g[qSBe] = -3.0 * iR3 * w;
+    g[qSXf] = -30.0 * m.C2_2 * rn * iR5 * w;
+    g[qSQBf] = 6.0 * m.C2_2 * iR3 * w;
+    g[qSXg] = -2.0 * m.C2_2OverC1_3 * rn * iR4 * tg;
+    g[qSQg] = (2.0 * c - 1.0) / m.C1 * iR2 * tg;
+    g[qSXh] = 2.0 / m.C2 * rn * iR4 * tg;
+    g[qSBh] = -(iR2 / m.C2) * tg;
+    double tu = m.OneOverFourPaiRou * w;
+    g[qUP] = m.OneOverC1_2 * iR3 * tu;
+    g[qUS1] = -(m.OneOverC2_2 * iR3) * tu;
+    g[qUS0] = m.OneOverC2_2 * iR * tu;
+    g[qUC1] = 3.0 * iR3 * tu;
+    g[qUC0] = -iR * tu;
+    g[qX + 0] = r[0] * r[0];
+    g[qX + 1] = r[0] * r[1];
+    g[qX + 2] = r[0] * r[2];
+    g[qX + 3] = r[1] * r[1];
+    g[qX + 4] = r[1] * r[2];
+    g[qX + 5] = r[2] * r[2];
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+            g[qQ + 3 * a + b] = r[b] * nor[a];
+    for (int a = 0; a < 3; ++a)
+        g[qBd + a] = r[a] * nor[a] + rn;
 }
 
-// Time coefficients of one lane at one staged point: GetDynaT1ij (type 1,
-// level n) or GetDynaT2ij (type 2, level n - 1).
-struct LaneCoef {
-    bool P = false, S = false;
-    int w = 0;
-    double p1 = 0.0, s1 = 0.0, w1 = 0.0, ri = 0.0;
+// One lane's time span: GetDynaT1ij at level n (type 1) or GetDynaT2ij at
+// level n - 1 (type 2); nfac = (n + 1) / (4 pi) or (1 - n) / (4 pi).
+struct Span {
+    double ndt, np1, nfac;
 };
 
-// The ISeries / JSeries cases need one quotient each: ((n + 1/2) dt^2) / R^2
-// (case 1), ((n + 1) dt) / R (case 2) or (n dt) / R (case 4); the operands
-// are selected per lane so every lane of a wave runs one division, and the
-// case formulas are then applied to that quotient exactly as in ISeries /
-// JSeries.
-__device__ __forceinline__ void MakeLaneCoef(const DynaMat& m, int type, double n, const double* g, LaneCoef& c)
+__device__ __forceinline__ Span MakeSpan(const DynaMat& m, int type, double n)
 {
+    Span s;
     double nn = type == 1 ? n : n - 1;
-    double ndt = nn * m.Dt;
-    double np1dt = ndt + m.Dt;
-    double rc1 = g[sRc1], rc2 = g[sRc2];
-    c.P = (rc1 >= ndt) && (rc1 < np1dt);
-    c.S = (rc2 >= ndt) && (rc2 < np1dt);
-    int w = 0;
-    if (np1dt <= rc1 || ndt >= rc2)
-        w = 0;
-    else if (ndt >= rc1 && np1dt <= rc2)
-        w = 1;
-    else if (ndt <= rc1 && (np1dt >= rc1 && np1dt <= rc2))
-        w = 2;
-    else if (ndt <= rc1 && np1dt >= rc2)
-        w = 3;
-    else if (np1dt >= rc2 && (ndt >= rc1 && ndt <= rc2))
-        w = 4;
-    c.w = w;
-    if (type == 1) {
-        c.p1 = kOneOver4Pi * (n + 1.0 - g[sRC1Dt]);
-        c.s1 = kOneOver4Pi * (n + 1.0 - g[sRC2Dt]);
-    } else {
-        c.p1 = kOneOver4Pi * (1.0 - n + g[sRC1Dt]);
-        c.s1 = kOneOver4Pi * (1.0 - n + g[sRC2Dt]);
-    }
-    double num = w == 1 ? (nn + 0.5) * m.Dt_2 : (w == 2 ? np1dt : ndt);
-    double den = w == 1 ? g[sR2] : g[sR1];
-    double qv = num / den;
-    double q2 = qv * qv;
-    double q3 = q2 * qv;
-    double ri1 = qv, rj1 = (nn * nn + nn + 0.33333333333333333) * g[sDt3R3];
-    double ri2 = 0.5 * (q2 - m.OneOverC1_2), rj2 = 0.33333333333333 * (q3 - m.OneOverC1_3);
-    double ri4 = 0.5 * (m.OneOverC2_2 - q2), rj4 = 0.333333333333333 * (m.OneOverC2_3 - q3);
-    double ri = w == 1 ? ri1 : w == 2 ? ri2 : w == 3 ? m.Half_1OC22_1OC12 : w == 4 ? ri4 : 0.0;
-    double rj = w == 1 ? rj1 : w == 2 ? rj2 : w == 3 ? m.OneOverThree_1OC23_1OC13 : w == 4 ? rj4 : 0.0;
-    double rq = g[sR1] * rj / m.Dt;
-    double w1 = type == 1 ? kOneOver4Pi * ((n + 1.0) * ri - rq) : kOneOver4Pi * ((1.0 - n) * ri + rq);
-    c.ri = w ? ri : 0.0;
-    c.w1 = w ? w1 : 0.0;
+    s.ndt = nn * m.Dt;
+    s.np1 = s.ndt + m.Dt;
+    s.nfac = type == 1 ? kOneOver4Pi * (n + 1.0) : kOneOver4Pi * (1.0 - n);
+    return s;
+}
+
+// Basis scalars of one span at one staged point (all zero when !on).  The
+// front flags are the reference's (Fai); the ISeries / JSeries window
+// integrals are evaluated in their common closed form
+//   ri = (hi^2 - lo^2) / (2 R^2),  rj = (hi^3 - lo^3) / (3 R^3),
+//   lo = max(ndt, R / C1), hi = min(np1, R / C2), zero when hi <= lo,
+// which is each of the four window cases of DBEM1/dsquareelement.h:662-716
+// up to rounding.
+struct SpanMix {
+    double cX, cQ, cB, ri;
+    bool P, S;
+};
+
+template <int TYPE>
+__device__ __forceinline__ SpanMix MixAt(const double* g, const Span& s, bool on)
+{
+    SpanMix o;
+    double rc1 = g[qRc1], rc2 = g[qRc2];
+    o.P = on && rc1 >= s.ndt && rc1 < s.np1;
+    o.S = on && rc2 >= s.ndt && rc2 < s.np1;
+    double lo = fmax(s.ndt, rc1), hi = fmin(s.np1, rc2);
+    double d = on ? fmax(hi - lo, 0.0) : 0.0;
+    double sm = hi + lo;
+    double ri = d * sm * g[qIR2h];
+    double rj = d * fma(hi, sm, lo * lo) * g[qIR3t];
+    double rq = g[qRk] * rj;
+    double gam = TYPE == 1 ? fma(s.nfac, ri, -rq) : fma(s.nfac, ri, rq);
+    double aP = o.P ? (TYPE == 1 ? s.nfac - g[qPc1] : s.nfac + g[qPc1]) : 0.0;
+    double aS = o.S ? (TYPE == 1 ? s.nfac - g[qPc2] : s.nfac + g[qPc2]) : 0.0;
+    double bP = o.P ? (TYPE == 1 ? 1.0 : -1.0) : 0.0;
+    double bS = o.S ? (TYPE == 1 ? 1.0 : -1.0) : 0.0;
+    o.cX = fma(aP, g[qSXd], fma(aS, g[qSXe], fma(bP, g[qSXg], fma(bS, g[qSXh], gam * g[qSXf]))));
+    o.cQ = fma(aP, g[qSQd], fma(aS, g[qSQe], fma(bP, g[qSQg], gam * g[qSQBf])));
+    o.cB = fma(aP, g[qSBd], fma(aS, g[qSBe], fma(bS, g[qSBh], gam * g[qSQBf])));
+    o.ri = ri;
+    return o;
+}
+
+// acc += cX X + cQ Q + cB B (3x3, row-major)
+__device__ __forceinline__ void AddBasis(const double* g, double cX, double cQ, double cB, double* acc)
+{
+    const int xm[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+            int k = 3 * a + b;
+            double bab = a == b ? g[qBd + a] : g[qQ + 3 * b + a];
+            acc[k] = fma(cX, g[qX + xm[k]], fma(cQ, g[qQ + k], fma(cB, bab, acc[k])));
+        }
+}
+
+// accU (00 01 02 11 12 22) += U with the type-1 flags and window integral.
+__device__ __forceinline__ void AddU(const double* g, bool P, bool S, double ri, double* accU)
+{
+    double uX = fma(ri, g[qUC1], (P ? g[qUP] : 0.0) + (S ? g[qUS1] : 0.0));
+    double uD = fma(ri, g[qUC0], S ? g[qUS0] : 0.0);
+    accU[0] = fma(uX, g[qX + 0], accU[0] + uD);
+    accU[1] = fma(uX, g[qX + 1], accU[1]);
+    accU[2] = fma(uX, g[qX + 2], accU[2]);
+    accU[3] = fma(uX, g[qX + 3], accU[3] + uD);
+    accU[4] = fma(uX, g[qX + 4], accU[4]);
+    accU[5] = fma(uX, g[qX + 5], accU[5] + uD);
 }
 
 __device__ __forceinline__ bool WaveAny(bool v) { return __ballot(v) != 0; }
-
-// Kernel values Ta (type TA, coefficients ca, if ua) and Tb (type TB, cb, if
-// ub) at one staged point, each 0 + P term + S term + window term.  The G and
-// H products with 1/(4 pi dt) are staged; the type-2 factor -1/(4 pi dt)
-// gives their exact negations, so type 2 subtracts them.
-template <int TA, int TB>
-__device__ __forceinline__ void StagedT(const double* g, bool ua, const LaneCoef& ca, bool ub, const LaneCoef& cb,
-                                        double* Ta, double* Tb)
-{
-    for (int k = 0; k < 9; ++k) {
-        Ta[k] = 0.0;
-        Tb[k] = 0.0;
-    }
-    bool pa = ua && ca.P, pb = ub && cb.P;
-    if (WaveAny(pa || pb)) {
-        for (int k = 0; k < 9; ++k) {
-            double dk = g[sD + k], gk = g[sGt + k];
-            if (pa)
-                Ta[k] += TA == 1 ? dk * ca.p1 + gk : dk * ca.p1 - gk;
-            if (pb)
-                Tb[k] += TB == 1 ? dk * cb.p1 + gk : dk * cb.p1 - gk;
-        }
-    }
-    bool sa = ua && ca.S, sb = ub && cb.S;
-    if (WaveAny(sa || sb)) {
-        for (int k = 0; k < 9; ++k) {
-            double ek = g[sE + k], hk = g[sHt + k];
-            if (sa)
-                Ta[k] += TA == 1 ? ek * ca.s1 + hk : ek * ca.s1 - hk;
-            if (sb)
-                Tb[k] += TB == 1 ? ek * cb.s1 + hk : ek * cb.s1 - hk;
-        }
-    }
-    bool wa = ua && ca.w, wb = ub && cb.w;
-    if (WaveAny(wa || wb)) {
-        const int map[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
-        for (int k = 0; k < 9; ++k) {
-            double fk = g[sF + map[k]];
-            if (wa)
-                Ta[k] += fk * ca.w1;
-            if (wb)
-                Tb[k] += fk * cb.w1;
-        }
-    }
-}
-
-// GetDynaUij at one staged point with the type-1 coefficients c (same level
-// and window as the U kernel): unique entries 0 1 2 4 5 8.
-__device__ __forceinline__ void StagedU(const DynaMat& m, const double* g, bool u, const LaneCoef& c, double* U)
-{
-    for (int k = 0; k < 6; ++k)
-        U[k] = 0.0;
-    if (WaveAny(u && c.P)) {
-        if (u && c.P)
-            for (int k = 0; k < 6; ++k)
-                U[k] += g[sUP + k];
-    }
-    if (WaveAny(u && c.S)) {
-        if (u && c.S)
-            for (int k = 0; k < 6; ++k)
-                U[k] += g[sUS + k];
-    }
-    if (WaveAny(u && c.w)) {
-        if (u && c.w) {
-            double t1 = m.OneOverFourPaiRou * c.ri;
-            for (int k = 0; k < 6; ++k)
-                U[k] += g[sUC + k] * t1;
-        }
-    }
-}
 
 }  // namespace dev
 
@@ -1108,61 +1063,27 @@ __device__ __forceinline__ EntryFlags JudgeAt(const DynaMat& m, double radius, d
     return f;
 }
 
-// Regular-path contribution of one staged point to the lane's accumulators.
-__device__ __forceinline__ void ConsumeRegular(const DynaMat& m, const double* g, double n, bool r2, bool r1,
-                                               double* accA, double* accB, double* accU)
+// Regular-path contribution of one staged point: the T1 (level n) + T2
+// (level n - 1) sum and the U term, into accR / accRU.
+__device__ __forceinline__ void ConsumeRegular(const double* g, const Span& s1, const Span& s2, bool rt, double* accR,
+                                               double* accRU)
 {
-    bool rt = r2 || r1;
-    LaneCoef ca, cb;
-    if (rt) {
-        MakeLaneCoef(m, 1, n, g, ca);
-        MakeLaneCoef(m, 2, n, g, cb);
-    }
-    double Ta[9], Tb[9];
-    StagedT<1, 2>(g, rt, ca, rt, cb, Ta, Tb);
-    double w = g[sW];
-    bool flag = ca.P || ca.S || ca.w || cb.P || cb.S || cb.w;
-    if (rt && flag) {
-        for (int kk = 0; kk < 9; ++kk) {
-            double v = (Ta[kk] + Tb[kk]) * w;
-            if (r2)
-                accA[kk] += v;
-            if (r1)
-                accB[kk] += v;
-        }
-    }
-    bool fu = r1 && (ca.P || ca.S || ca.w);
-    double U[6];
-    StagedU(m, g, fu, ca, U);
-    if (fu)
-        for (int kk = 0; kk < 6; ++kk)
-            accU[kk] += U[kk] * w;
+    SpanMix a = MixAt<1>(g, s1, rt);
+    SpanMix b = MixAt<2>(g, s2, rt);
+    AddBasis(g, a.cX + b.cX, a.cQ + b.cQ, a.cB + b.cB, accR);
+    AddU(g, a.P, a.S, a.ri, accRU);
 }
 
 // Patch-wise contribution of one staged point (patch flags u2 for T2, u1 for
 // T1 and U).
-__device__ __forceinline__ void ConsumePatchwise(const DynaMat& m, const double* g, double n, bool u2, bool u1,
+__device__ __forceinline__ void ConsumePatchwise(const double* g, const Span& s2, const Span& s1, bool u2, bool u1,
                                                  double* accA, double* accB, double* accU)
 {
-    LaneCoef c2, c1;
-    if (u2)
-        MakeLaneCoef(m, 2, n, g, c2);
-    if (u1)
-        MakeLaneCoef(m, 1, n, g, c1);
-    double T2[9], T1[9];
-    StagedT<2, 1>(g, u2, c2, u1, c1, T2, T1);
-    double w = g[sW];
-    if (u2)
-        for (int kk = 0; kk < 9; ++kk)
-            accA[kk] += T2[kk] * w;
-    double U[6];
-    StagedU(m, g, u1, c1, U);
-    if (u1) {
-        for (int kk = 0; kk < 9; ++kk)
-            accB[kk] += T1[kk] * w;
-        for (int kk = 0; kk < 6; ++kk)
-            accU[kk] += U[kk] * w;
-    }
+    SpanMix a = MixAt<2>(g, s2, u2);
+    SpanMix b = MixAt<1>(g, s1, u1);
+    AddBasis(g, a.cX, a.cQ, a.cB, accA);
+    AddBasis(g, b.cX, b.cQ, b.cB, accB);
+    AddU(g, b.P, b.S, b.ri, accU);
 }
 
 // Split storage of the mixed mode: an FP64 value v is kept as hi = (float)v
@@ -1295,7 +1216,7 @@ __global__ void __launch_bounds__(64) k_history_reg(
     long c0, long c1, TB* __restrict__ mts, TB* __restrict__ mgs, unsigned short* __restrict__ mtsLo,
     unsigned short* __restrict__ mgsLo, unsigned char* __restrict__ alive)
 {
-    __shared__ double sgeo[kNG2 * dev::kStageStride];
+    __shared__ double sgeo[kNG2 * dev::kRec];
     using namespace dev;
     long wv = c0 + (long)blockIdx.x;
     if (wv >= c1)
@@ -1322,24 +1243,26 @@ __global__ void __launch_bounds__(64) k_history_reg(
     bool inBand = t < len[p];
     EntryFlags fl = JudgeAt(m, fe.radius, Dist3(fe.center, src), k, s == f, inBand ? cls : nullptr, off[p] + t);
     bool mine = inBand && !fl.patchwise();
-    double accA[9], accB[9], accU[6];
-    for (int i = 0; i < 9; ++i) {
-        accA[i] = 0.0;
-        accB[i] = 0.0;
-    }
+    // a regular entry's T2 and T1 parts are the same T1 + T2 sum, so one
+    // accumulator serves both
+    double accR[9], accRU[6];
+    for (int i = 0; i < 9; ++i)
+        accR[i] = 0.0;
     for (int i = 0; i < 6; ++i)
-        accU[i] = 0.0;
-    bool r2 = mine && fl.f2 == 2, r1 = mine && fl.f1 == 2;
-    if (WaveAny(r2 || r1)) {
+        accRU[i] = 0.0;
+    bool rt = mine && (fl.f2 == 2 || fl.f1 == 2);
+    if (WaveAny(rt)) {
         if (lane < kNG2)
-            StagePoint(m, src, P.rg_pos[lane], P.rg_nor[lane], P.rg_w[lane], sgeo + lane * kStageStride);
+            StageRecord(m, src, P.rg_pos[lane], P.rg_nor[lane], P.rg_w[lane], sgeo + lane * kRec);
         __syncthreads();
+        Span s1 = MakeSpan(m, 1, n), s2 = MakeSpan(m, 2, n);
+#pragma unroll 1
         for (int i = 0; i < kNG2; ++i)
-            ConsumeRegular(m, sgeo + i * kStageStride, n, r2, r1, accA, accB, accU);
+            ConsumeRegular(sgeo + i * kRec, s1, s2, rt, accR, accRU);
     }
     if (!mine)
         return;
-    FinalizeEntry(fl, selfT2 + 9 * (long)f, refs[f].hu_sign, refs[f].ht_sign, accA, accB, accU, off[p] + t, mts, mgs,
+    FinalizeEntry(fl, selfT2 + 9 * (long)f, refs[f].hu_sign, refs[f].ht_sign, accR, accR, accRU, off[p] + t, mts, mgs,
                   mtsLo, mgsLo, alive);
 }
 
@@ -1357,7 +1280,7 @@ __global__ void __launch_bounds__(64) k_history_pw(
     const unsigned char* __restrict__ cls, TB* __restrict__ mts, TB* __restrict__ mgs,
     unsigned short* __restrict__ mtsLo, unsigned short* __restrict__ mgsLo, unsigned char* __restrict__ alive)
 {
-    __shared__ double sgeo[kPwSlots * kNPWG2 * dev::kStageStride];
+    __shared__ double sgeo[kPwSlots * kNPWG2 * dev::kRec];
     __shared__ double spr[kPwSlots * 2 * kNPW2];
     __shared__ int ssrc[kPwSlots];
     using namespace dev;
@@ -1443,13 +1366,39 @@ __global__ void __launch_bounds__(64) k_history_pw(
     for (int i = 0; i < kNPW2; ++i)
         if (WaveAny(((m2 | m1) >> i) & 1ull))
             need |= 1ull << i;
+    // regular parts first (the other judge picking the regular path; at most
+    // one of r2 / r1 holds for a patch-wise entry), 8 points per half
+    bool r2 = active && fl.f2 == 2, r1 = active && fl.f1 == 2, rt = r2 || r1;
+    Span s1 = MakeSpan(m, 1, n), s2 = MakeSpan(m, 2, n);
     double accA[9], accB[9], accU[6];
-    for (int i = 0; i < 9; ++i) {
-        accA[i] = 0.0;
-        accB[i] = 0.0;
+    {
+        double accR[9], accRU[6];
+        for (int i = 0; i < 9; ++i)
+            accR[i] = 0.0;
+        for (int i = 0; i < 6; ++i)
+            accRU[i] = 0.0;
+        if (WaveAny(rt)) {
+            int rgSlot = lane >> 3, rgJ = lane & 7;
+            for (int half = 0; half < 2; ++half) {
+                if (rgSlot < nseg) {
+                    int q = half * 8 + rgJ;
+                    StageRecord(m, geo[ssrc[rgSlot]].center, P.rg_pos[q], P.rg_nor[q], P.rg_w[q], sgeo + lane * kRec);
+                }
+                __syncthreads();
+                const double* gs = sgeo + myseg * 8 * kRec;
+#pragma unroll 1
+                for (int j = 0; j < 8; ++j)
+                    ConsumeRegular(gs + j * kRec, s1, s2, rt, accR, accRU);
+                __syncthreads();
+            }
+        }
+        for (int i = 0; i < 9; ++i) {
+            accA[i] = r2 ? accR[i] : 0.0;
+            accB[i] = r1 ? accR[i] : 0.0;
+        }
+        for (int i = 0; i < 6; ++i)
+            accU[i] = r1 ? accRU[i] : 0.0;
     }
-    for (int i = 0; i < 6; ++i)
-        accU[i] = 0.0;
     // one patch at a time, all slots' sources staged together
     int stSlot = lane / kNPWG2, stJ = lane - stSlot * kNPWG2;
     while (need) {
@@ -1457,32 +1406,15 @@ __global__ void __launch_bounds__(64) k_history_pw(
         need &= need - 1;
         if (stSlot < nseg) {
             int q = i * kNPWG2 + stJ;
-            StagePoint(m, geo[ssrc[stSlot]].center, P.pw_pos[q], P.pw_nor[q], P.pw_w[q],
-                       sgeo + lane * kStageStride);
+            StageRecord(m, geo[ssrc[stSlot]].center, P.pw_pos[q], P.pw_nor[q], P.pw_w[q], sgeo + lane * kRec);
         }
         __syncthreads();
         bool u2 = (m2 >> i) & 1ull, u1 = (m1 >> i) & 1ull;
-        const double* gs = sgeo + myseg * kNPWG2 * kStageStride;
+        const double* gs = sgeo + myseg * kNPWG2 * kRec;
+#pragma unroll 1
         for (int j = 0; j < kNPWG2; ++j)
-            ConsumePatchwise(m, gs + j * kStageStride, n, u2, u1, accA, accB, accU);
+            ConsumePatchwise(gs + j * kRec, s2, s1, u2, u1, accA, accB, accU);
         __syncthreads();
-    }
-    // regular parts (the other judge picking the regular path), 8 points per half
-    bool r2 = active && fl.f2 == 2, r1 = active && fl.f1 == 2;
-    if (WaveAny(r2 || r1)) {
-        int rgSlot = lane >> 3, rgJ = lane & 7;
-        for (int half = 0; half < 2; ++half) {
-            if (rgSlot < nseg) {
-                int q = half * 8 + rgJ;
-                StagePoint(m, geo[ssrc[rgSlot]].center, P.rg_pos[q], P.rg_nor[q], P.rg_w[q],
-                           sgeo + lane * kStageStride);
-            }
-            __syncthreads();
-            const double* gs = sgeo + myseg * 8 * kStageStride;
-            for (int j = 0; j < 8; ++j)
-                ConsumeRegular(m, gs + j * kStageStride, n, r2, r1, accA, accB, accU);
-            __syncthreads();
-        }
     }
     if (!active)
         return;

@@ -1,8 +1,9 @@
 """GPU parity of the HIP path (through the C-ABI) against the CPU oracle.
 
 Integer / pattern data must be bit-exact; FP64 blocks are compared bitwise
-where the arithmetic order is the reference's (history quadrature, DOF map)
-and to 1e-10 relative otherwise; solutions to the stated tolerances."""
+where the arithmetic order is the reference's (the lane-per-entry history
+quadrature, DOF map) and to 1e-10 relative per 3x3 block otherwise (the
+default wave-per-pair quadrature); solutions to the stated tolerances."""
 from pathlib import Path
 
 import numpy as np
@@ -54,6 +55,19 @@ def test_self_tables_bitexact(rod_pair):
     assert np.array_equal(u, ref[:, 0]) and np.array_equal(t1, ref[:, 1]) and np.array_equal(t2, ref[:, 2])
 
 
+def _bands_close(x, y, rtol=1e-10):
+    """Same CCSR pattern bit for bit; every 3x3 block within rtol of its own
+    largest entry."""
+    assert np.array_equal(x[0], y[0]) and np.array_equal(x[1], y[1]), "pattern"
+    if len(y[2]) == 0:
+        return 0.0
+    vy = y[2].reshape(len(y[2]), -1)
+    err = np.abs(x[2].reshape(len(y[2]), -1) - vy).max(1)
+    rel = err / np.maximum(np.abs(vy).max(1), 1e-300)
+    assert rel.max() <= rtol, rel.max()
+    return float(rel.max())
+
+
 def _same_ccsr(a, b, rtol):
     assert np.array_equal(a[0], b[0]), "row_ptr"
     assert np.array_equal(a[1], b[1]), "col"
@@ -67,13 +81,28 @@ def test_step0_matrices(rod_pair):
         _same_ccsr(gc.ccsr(which), oc.ccsr(which), 1e-15)
 
 
-def test_history_bitexact_all_steps(rod_pair):
+def test_history_blocks_all_steps(rod_pair):
     oc, gc, f = rod_pair
     for k in range(1, f["card"].nstep + 1):
+        for which in (2, 3):
+            _bands_close(gc.ccsr(which, k), oc.ccsr(which, k))
+
+
+def test_history_exact_path_bitexact(oracle_mod, rod_fixture):
+    """The lane-per-entry quadrature (quad_staged 0) keeps the reference's
+    arithmetic order: MTS / MGS bands equal the oracle's bit for bit."""
+    oc, gc, dt = _rod_cases(oracle_mod, rod_fixture)
+    nstep = rod_fixture["card"].nstep
+    gc.assemble_step0(10)
+    gc.set_option("quad_staged", 0)
+    gc.assemble_history(nstep)
+    oc.assemble(nstep)
+    for k in range(1, nstep + 1):
         for which in (2, 3):
             a, b = gc.ccsr(which, k), oc.ccsr(which, k)
             assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), (which, k)
             assert np.array_equal(a[2], b[2]), (which, k)
+    gc.close()
 
 
 def test_rod_known_answer_on_gpu(rod_pair):
@@ -124,9 +153,7 @@ def test_generated_cases_match_oracle(oracle_mod, which):
         _same_ccsr(gc.ccsr(which_m), oc.ccsr(which_m), 1e-14)
     for k in range(1, w.nstep + 1):
         for which_m in (2, 3):
-            a, b = gc.ccsr(which_m, k), oc.ccsr(which_m, k)
-            assert np.array_equal(a[1], b[1]) and np.array_equal(a[0], b[0]), (which_m, k)
-            assert np.array_equal(a[2], b[2]), (which_m, k)
+            _bands_close(gc.ccsr(which_m, k), oc.ccsr(which_m, k))
     U, T, it, fl, solved, _ = gc.solve(w.known, w.nstep, tol=1e-12)
     OU, OT, oit, ofl, osolved = oc.solve(w.known, w.nstep, tol=1e-12)
     assert solved == osolved == w.nstep
@@ -153,12 +180,18 @@ def test_regular_path_exercised_and_matched(oracle_mod):
                 regular += oc.needcal(f, s, 1, k) == 2
     assert regular > 0
     gc.assemble_step0(10)
-    gc.assemble_history(w.nstep)
     oc.assemble(w.nstep)
-    for k in range(1, w.nstep + 1):
-        for which_m in (2, 3):
-            a, b = gc.ccsr(which_m, k), oc.ccsr(which_m, k)
-            assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2]), (which_m, k)
+    for staged in (0, 1):
+        gc.set_option("quad_staged", staged)
+        gc.assemble_history(w.nstep)
+        for k in range(1, w.nstep + 1):
+            for which_m in (2, 3):
+                a, b = gc.ccsr(which_m, k), oc.ccsr(which_m, k)
+                if staged:
+                    _bands_close(a, b)
+                else:
+                    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), (which_m, k)
+                    assert np.array_equal(a[2], b[2]), (which_m, k)
 
 
 def test_full_size_properties():
@@ -211,9 +244,10 @@ def test_tiled_convolution_matches_per_step_sum(oracle_mod):
 
 @pytest.mark.parametrize("which", ["regular_path_probe", "layered_small"])
 def test_staged_quadrature_matches_per_entry(oracle_mod, which):
-    """The staged wave-per-pair history quadrature and the lane-per-entry
-    quadrature give the same MTS/MGS bands bit for bit, and both equal the
-    oracle's blocks."""
+    """The lane-per-entry quadrature equals the oracle bit for bit; the default
+    wave-per-pair quadrature (basis-tensor form, closed-form window integrals,
+    fused multiply-adds) has the same band pattern and agrees to 1e-10 per
+    block, and both give the same solution to 1e-12."""
     from tdbem import mesh
     if which == "regular_path_probe":
         m = mesh.box_mesh(2, 4.0, 1.0, 1.0, 8, 2, 2)
@@ -228,15 +262,21 @@ def test_staged_quadrature_matches_per_entry(oracle_mod, which):
     gc.set_option("quad_staged", 0)
     gc.assemble_history(w.nstep)
     ref = [(gc.ccsr(2, k), gc.ccsr(3, k)) for k in range(1, w.nstep + 1)]
+    U0, T0, *_ = gc.solve(w.known, w.nstep, tol=1e-12)
     gc.set_option("quad_staged", 1)
     gc.assemble_history(w.nstep)
+    U1, T1, *_ = gc.solve(w.known, w.nstep, tol=1e-12)
     oc.assemble(w.nstep)
+    worst = 0.0
     for k in range(1, w.nstep + 1):
         for i, which_m in enumerate((2, 3)):
             a, b, o = gc.ccsr(which_m, k), ref[k - 1][i], oc.ccsr(which_m, k)
-            for x, y in ((a, b), (a, o)):
-                assert np.array_equal(x[0], y[0]) and np.array_equal(x[1], y[1]), (which_m, k)
-                assert np.array_equal(x[2], y[2]), (which_m, k)
+            assert np.array_equal(b[0], o[0]) and np.array_equal(b[1], o[1]), (which_m, k)
+            assert np.array_equal(b[2], o[2]), (which_m, k)
+            worst = max(worst, _bands_close(a, b))
+    print("worst block-relative difference", worst, flush=True)
+    assert np.abs(U1 - U0).max() <= 1e-12 * np.abs(U0).max()
+    assert np.abs(T1 - T0).max() <= 1e-12 * np.abs(T0).max()
 
 
 def test_mixed_precision_error_against_fp64():

@@ -278,8 +278,16 @@ double Case::Adjoint(int nstep, const double* known_lu, const int* recv, int nre
     d_t2.alloc((size_t)n3);
     d_kn.alloc(3 * (size_t)std::max<long>(1, dof.knownBlockCount));
     LaunchZero(d_kn.p, (long)d_kn.n, stream);
-    std::vector<double> delta((size_t)nd), inv2row((size_t)ne);
+    std::vector<double> delta((size_t)nd), inv2row((size_t)ne), scaleDom((size_t)nd);
     DevCCSR dA, dK;
+    // filtered state series for the entry-wise history contraction
+    DevBuf<double> d_FU, d_FT, d_part, d_scale;
+    d_FU.alloc(std::max<size_t>(1, (size_t)dof.historyUBlockCount * 3 * (size_t)nstore));
+    d_FT.alloc(std::max<size_t>(1, (size_t)dof.historyTBlockCount * 3 * (size_t)nstore));
+    LaunchFilterSeries(kp.gU.p, dof.historyUBlockCount, nstore, nstep, d_FU.p, stream);
+    LaunchFilterSeries(kp.gT.p, dof.historyTBlockCount, nstore, nstep, d_FT.p, stream);
+    d_part.alloc(std::max<size_t>(1, (size_t)chunks));
+    d_scale.alloc((size_t)nd);
     for (int p = 0; p < 3; ++p) {
         std::vector<double> tp = theta, tm = theta;
         for (int d = 0; d < nd; ++d) {
@@ -297,18 +305,19 @@ double Case::Adjoint(int nstep, const double* known_lu, const int* recv, int nre
         SetMaterials(tm.data());
         ComputeStep0(Am, Km);
         StagedQuadrature(d_mts.p, d_mgs.p, nullptr, nullptr, d_aliveS.p);
-        for (int d = 0; d < nd; ++d) {
-            long e0 = h_off[(size_t)grid.pairBase[d]], e1 = h_off[(size_t)grid.pairBase[d + 1]];
-            double sc = 1.0 / (2.0 * delta[(size_t)d]);
-            LaunchFdBand(d_dM.p, d_mts.p, 9 * e0, 9 * e1, sc, stream);
-            LaunchFdBand(d_dG.p, d_mgs.p, 6 * e0, 6 * e1, sc, stream);
-        }
+        // history blocks: sum_K lambda_K . h_K(dM), entry by entry
+        for (int d = 0; d < nd; ++d)
+            scaleDom[(size_t)d] = 1.0 / (2.0 * delta[(size_t)d]);
+        d_scale.upload(scaleDom.data(), scaleDom.size(), stream);
+        LaunchCorrFd(grid, d_refs.p, d_coff.p, d_off.p, d_kb.p, d_len.p, npairs, chunks, d_dM.p, d_dG.p, d_mts.p,
+                     d_mgs.p, d_scale.p, d_lam.p, n3, d_FU.p, d_FT.p, kp.gU.p, kp.gT.p, nstore, nstep, d_part.p,
+                     stream);
+        LaunchRowChunks(grid, d_refs.p, d_coff.p, d_part.p, (int)ne, d_acc.p + (size_t)p * ne, stream);
+        // step-0 blocks: lambda_K . (dKnownM k_K - dA x_K)
         dA.Upload(DiffOnPattern(A, Ap, Am, inv2row), stream);
         dK.Upload(DiffOnPattern(KnownM, Kp, Km, inv2row), stream);
         for (int K = 1; K <= nstep; ++K) {
             LaunchZero(d_v.p, n3, stream);
-            LaunchConvolution(grid, d_refs.p, d_off.p, d_kb.p, d_len.p, d_dM.p, d_dG.p, kp.gU.p, kp.gT.p, nstore, K,
-                              maxn, (int)ne, d_v.p, stream);
             LaunchKnown(d_refs.p, kp.bd.p + (size_t)K * n3, (int)ne, d_kn.p, dof.knownBlockCount, stream);
             LaunchSpmv(ne, dK.rp.p, dK.col.p, (const double*)dK.val.p, (const double*)d_kn.p, d_t1.p, stream);
             LaunchSpmv(ne, dA.rp.p, dA.col.p, (const double*)dA.val.p, (const double*)(kp.X.p + (size_t)K * n3),

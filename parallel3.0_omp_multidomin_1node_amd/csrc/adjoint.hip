@@ -9,6 +9,8 @@
 // transposed state map.
 #include "adjoint.hpp"
 
+#include <algorithm>
+
 #pragma clang fp contract(off)
 
 namespace tdbem {
@@ -176,6 +178,166 @@ __global__ void __launch_bounds__(256) k_addsub(double* __restrict__ v, const do
 void LaunchAddSub(double* v, const double* a, const double* b, long n, hipStream_t s)
 {
     k_addsub<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(v, a, b, n);
+}
+
+// 1-2-1 filtered state series of every history column: F[c][t] = g_{t+1} +
+// 2 g_t + g_{t-1} over the stored steps 1 .. N (zero outside), t = 0 .. N,
+// so the forward history term of lag j at step K reads F[c][K - j] (minus
+// g_K when j = 1: the forward sum only reads states before step K).
+__global__ void __launch_bounds__(256) k_filter_series(const double* __restrict__ g, long ncol, int nstore, int N,
+                                                       double* __restrict__ F)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long per = (long)(N + 1) * 3;
+    if (i >= ncol * per)
+        return;
+    long c = i / per;
+    int r = (int)(i - c * per), t = r / 3, k = r - 3 * t;
+    const double* sg = g + c * (long)nstore * 3;
+    double v = 0.0;
+    if (t + 1 <= N)
+        v += sg[3 * (t + 1) + k];
+    if (t >= 1)
+        v += 2.0 * sg[3 * t + k];
+    if (t >= 2)
+        v += sg[3 * (t - 1) + k];
+    F[i] = v;
+}
+
+void LaunchFilterSeries(const double* g, long ncol, int nstore, int N, double* F, hipStream_t s)
+{
+    long n = ncol * (long)(N + 1) * 3;
+    if (n > 0)
+        k_filter_series<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(g, ncol, nstore, N, F);
+}
+
+// History part of the material gradient for one parameter: for every band
+// entry e = (s, f, j) of the operator difference dM = (M+ - M-) scale[d],
+//   sum_{K >= j} lambda_K[s]^T (dMGS_e cT_{K-j} - dMTS_e cU_{K-j})
+// with c the filtered series above; this is sum_K lambda_K . h_K(dM) of the
+// forward history term, summed entry by entry instead of step by step.  One
+// wavefront per (pair, chunk of 64 lags), one lag per lane; lambda_K[s] is
+// the same for every lane.  The wave's sum goes to part[chunk].
+__global__ void __launch_bounds__(64) k_corr_fd(PairGrid grid, const NodeRefs* __restrict__ refs,
+                                                const long* __restrict__ coff, const long* __restrict__ off,
+                                                const int* __restrict__ kb, const int* __restrict__ len, long npairs,
+                                                long c0, long c1, const double* __restrict__ mp,
+                                                const double* __restrict__ gp, const double* __restrict__ mm,
+                                                const double* __restrict__ gm, const double* __restrict__ scale,
+                                                const double* __restrict__ lam, long n3, const double* __restrict__ FU,
+                                                const double* __restrict__ FT, const double* __restrict__ gU,
+                                                const double* __restrict__ gT, int nstore, int N,
+                                                double* __restrict__ part)
+{
+    long wv = c0 + (long)blockIdx.x;
+    if (wv >= c1)
+        return;
+    int lane = threadIdx.x;
+    long lo = 0, hi = npairs;
+    while (hi - lo > 1) {
+        long mid = (lo + hi) >> 1;
+        if (coff[mid] <= wv)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    long p = lo;
+    int s, f, d;
+    DecodePairOf(grid, p, s, f, d);
+    int t0 = (int)(wv - coff[p]) * 64;
+    int t = t0 + lane;
+    bool in = t < len[p];
+    int j = kb[p] + t, jlo = kb[p] + t0;
+    const double* fu = FU + (long)refs[f].hu_block * (N + 1) * 3;
+    const double* ft = FT + (long)refs[f].ht_block * (N + 1) * 3;
+    double cu[9], ct[9];
+    for (int i = 0; i < 9; ++i) {
+        cu[i] = 0.0;
+        ct[i] = 0.0;
+    }
+    for (int K = jlo; K <= N; ++K) {
+        const double* l = lam + (long)K * n3 + 3 * (long)s;
+        double l0 = l[0], l1 = l[1], l2 = l[2];
+        if (in && K >= j) {
+            int tt = K - j;
+            double u[3] = {fu[3 * tt], fu[3 * tt + 1], fu[3 * tt + 2]};
+            double v[3] = {ft[3 * tt], ft[3 * tt + 1], ft[3 * tt + 2]};
+            if (j == 1) {
+                const double* gu = gU + ((long)refs[f].hu_block * nstore + K) * 3;
+                const double* gt = gT + ((long)refs[f].ht_block * nstore + K) * 3;
+                for (int c = 0; c < 3; ++c) {
+                    u[c] -= gu[c];
+                    v[c] -= gt[c];
+                }
+            }
+            for (int c = 0; c < 3; ++c) {
+                cu[c] = fma(l0, u[c], cu[c]);
+                cu[3 + c] = fma(l1, u[c], cu[3 + c]);
+                cu[6 + c] = fma(l2, u[c], cu[6 + c]);
+                ct[c] = fma(l0, v[c], ct[c]);
+                ct[3 + c] = fma(l1, v[c], ct[3 + c]);
+                ct[6 + c] = fma(l2, v[c], ct[6 + c]);
+            }
+        }
+    }
+    double val = 0.0;
+    if (in) {
+        long e = off[p] + t;
+        const double* Mp = mp + 9 * e;
+        const double* Mm = mm + 9 * e;
+        const double* Gp = gp + 6 * e;
+        const double* Gm = gm + 6 * e;
+        double a = 0.0;
+        for (int i = 0; i < 9; ++i)
+            a = fma(Mp[i] - Mm[i], cu[i], a);
+        double dg[6];
+        for (int i = 0; i < 6; ++i)
+            dg[i] = Gp[i] - Gm[i];
+        double b = dg[0] * ct[0] + dg[3] * ct[4] + dg[5] * ct[8] + dg[1] * (ct[1] + ct[3]) +
+                   dg[2] * (ct[2] + ct[6]) + dg[4] * (ct[5] + ct[7]);
+        val = (b - a) * scale[d];
+    }
+    for (int w = 32; w > 0; w >>= 1)
+        val += __shfl_xor(val, w);
+    if (lane == 0)
+        part[wv] = val;
+}
+
+void LaunchCorrFd(PairGrid grid, const NodeRefs* refs, const long* coff, const long* off, const int* kb,
+                  const int* len, long npairs, long chunks, const double* mp, const double* gp, const double* mm,
+                  const double* gm, const double* scale, const double* lam, long n3, const double* FU,
+                  const double* FT, const double* gU, const double* gT, int nstore, int N, double* part,
+                  hipStream_t s)
+{
+    const long step = 1L << 20;
+    for (long c0 = 0; c0 < chunks; c0 += step) {
+        long c1 = std::min(chunks, c0 + step);
+        k_corr_fd<<<(unsigned)(c1 - c0), 64, 0, s>>>(grid, refs, coff, off, kb, len, npairs, c0, c1, mp, gp, mm, gm,
+                                                    scale, lam, n3, FU, FT, gU, gT, nstore, N, part);
+    }
+}
+
+// acc[s] += sum of the chunk partials of row s (its pairs are contiguous).
+__global__ void __launch_bounds__(256) k_row_chunks(PairGrid grid, const NodeRefs* __restrict__ refs,
+                                                    const long* __restrict__ coff, const double* __restrict__ part,
+                                                    int ne, double* __restrict__ acc)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ne)
+        return;
+    int d = refs[s].domain;
+    long nd = grid.eleCount[d];
+    long p0 = grid.pairBase[d] + (long)(s - grid.eleBegin[d]) * nd;
+    double a = 0.0;
+    for (long c = coff[p0]; c < coff[p0 + nd]; ++c)
+        a += part[c];
+    acc[s] += a;
+}
+
+void LaunchRowChunks(PairGrid grid, const NodeRefs* refs, const long* coff, const double* part, int ne, double* acc,
+                     hipStream_t s)
+{
+    k_row_chunks<<<(ne + 255) / 256, 256, 0, s>>>(grid, refs, coff, part, ne, acc);
 }
 
 }  // namespace tdbem

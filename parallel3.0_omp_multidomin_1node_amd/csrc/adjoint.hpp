@@ -17,5 +17,16 @@ void LaunchStateT(const NodeRefs* refs, const ElemGeo* geo, const long* colU, co
 void LaunchFdBand(double* dB, const double* B, long e0, long e1, double scale, hipStream_t s);
 void LaunchRowDotAcc(const double* lam, const double* v, long nrows, double* acc, hipStream_t s);
 void LaunchAddSub(double* v, const double* a, const double* b, long n, hipStream_t s);
+// F[c][t] = g_{t+1} + 2 g_t + g_{t-1} per history column (t = 0 .. N).
+void LaunchFilterSeries(const double* g, long ncol, int nstore, int N, double* F, hipStream_t s);
+// History part of one parameter's gradient, per (pair, chunk) into part;
+// then acc[s] += the partials of row s.
+void LaunchCorrFd(PairGrid grid, const NodeRefs* refs, const long* coff, const long* off, const int* kb,
+                  const int* len, long npairs, long chunks, const double* mp, const double* gp, const double* mm,
+                  const double* gm, const double* scale, const double* lam, long n3, const double* FU,
+                  const double* FT, const double* gU, const double* gT, int nstore, int N, double* part,
+                  hipStream_t s);
+void LaunchRowChunks(PairGrid grid, const NodeRefs* refs, const long* coff, const double* part, int ne, double* acc,
+                     hipStream_t s);
 
 }  // namespace tdbem

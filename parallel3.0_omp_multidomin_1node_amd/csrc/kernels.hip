@@ -514,13 +514,7 @@ struct BlockAcc {
 
 __device__ __forceinline__ void DecodePair(const PairGrid& g, long p, int& s, int& f, int& d)
 {
-    d = 0;
-    while (d + 1 < g.nd && p >= g.pairBase[d + 1])
-        ++d;
-    long local = p - g.pairBase[d];
-    long nd = g.eleCount[d];
-    s = g.eleBegin[d] + (int)(local / nd);
-    f = g.eleBegin[d] + (int)(local % nd);
+    DecodePairOf(g, p, s, f, d);
 }
 
 }  // namespace dev

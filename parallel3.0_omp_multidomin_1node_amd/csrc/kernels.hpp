@@ -38,6 +38,19 @@ struct PairGrid {
     int eleCount[kMaxDomains];
 };
 
+// Pair p of the grid -> source s, field f, domain d (pairs of a domain are
+// numbered source-major).
+__device__ __forceinline__ void DecodePairOf(const PairGrid& g, long p, int& s, int& f, int& d)
+{
+    d = 0;
+    while (d + 1 < g.nd && p >= g.pairBase[d + 1])
+        ++d;
+    long local = p - g.pairBase[d];
+    long nd = g.eleCount[d];
+    s = g.eleBegin[d] + (int)(local / nd);
+    f = g.eleBegin[d] + (int)(local % nd);
+}
+
 void UploadDynaMats(const DynaMat* mats, int nd, hipStream_t s);
 
 void LaunchPoints(const ElemGeo* geo, const QuadTables* q, ElemPoints* pts, int ne, hipStream_t s);

@@ -112,7 +112,8 @@ void Case::SetMaterials(const double* evr)
         mats.push_back(D.mat);
     }
     UploadDynaMats(mats.data(), (int)mats.size(), stream);
-    LaunchSingular(d_geo.p, d_dom.p, d_tables.p, d_u1.p, d_t1.p, d_t2.p, (int)ne, stream);
+    LaunchSingular(d_geo.p, d_dom.p, d_tables.p, d_u1.p, d_t1.p, d_t2.p, (int)ne, !quadStaged, stream);
+    selfMode = quadStaged ? 1 : 0;
     CheckA(hipStreamSynchronize(stream), "materials");
 }
 

@@ -160,6 +160,11 @@ int tdbem_self_tables(tdbem_ctx* ctx, double* u1, double* t1, double* t2)
 {
     if (!ctx || !u1 || !t1 || !t2)
         return Fail("tdbem_self_tables: invalid arguments");
+    try {
+        ctx->c.EnsureSelfTables();
+    } catch (const std::exception& e) {
+        return Fail("tdbem_self_tables", e);
+    }
     size_t n = 9 * (size_t)ctx->c.ne;
     if (hipMemcpy(u1, ctx->c.d_u1.p, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
         hipMemcpy(t1, ctx->c.d_t1.p, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||

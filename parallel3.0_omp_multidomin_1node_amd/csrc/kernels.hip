@@ -697,10 +697,162 @@ __global__ void __launch_bounds__(64) k_singular(const ElemGeo* __restrict__ geo
     to1[8] += 0.5;
 }
 
-void LaunchSingular(const ElemGeo* geo, const int* dom, const QuadTables* q, double* u1, double* t1, double* t2,
-                    int ne, hipStream_t s)
+// The same self integrals with one wavefront per element: the lanes share the
+// 64 static points, the 16 line points and the 1600 polar rows (strided),
+// and the partial sums are added in a fixed tree order (deterministic; equal
+// to the lane-per-element order up to rounding).
+__global__ void __launch_bounds__(64) k_singular_wave(const ElemGeo* __restrict__ geo, const int* __restrict__ dom,
+                                                      const QuadTables* __restrict__ q, double* u1o, double* t1o,
+                                                      double* t2o, int ne)
 {
-    k_singular<<<(ne + 63) / 64, 64, 0, s>>>(geo, dom, q, u1, t1, t2, ne);
+    constexpr int kVals = 33;  // st 9, u 6, a1 9, a2 9
+    __shared__ double red[kVals][64];
+    int e = blockIdx.x;
+    int lane = threadIdx.x;
+    const ElemGeo& g = geo[e];
+    const DynaMat& m = c_mats[dom[e]];
+    double node[3];
+    FromLocal(g.cc, 0.0, 0.0, node);
+    double v[kVals];
+    for (int k = 0; k < kVals; ++k)
+        v[k] = 0.0;
+    double* st = v;
+    double coef1 = -1.0 / (8.0 * kPi * (1.0 - m.v));
+    double coef2 = 1.0 - 2.0 * m.v;
+    for (int idx = lane; idx < 4 * kNSG2; idx += 64) {
+        int a = idx / kNSG2, j = idx - a * kNSG2;
+        double s1 = q->secconstpos[a][0][j], s2 = q->secconstpos[a][1][j];
+        double J = Jacobi(g.cc, s1, s2);
+        double nor[3], p[3], R[4], T[9];
+        NormalAt(g.cc, s1, s2, nor);
+        FromLocal(g.cc, s1, s2, p);
+        R[1] = p[0] - node[0];
+        R[2] = p[1] - node[1];
+        R[3] = p[2] - node[2];
+        R[0] = sqrt(R[1] * R[1] + R[2] * R[2] + R[3] * R[3]);
+        dev::StaticT(m, R, nor, T);
+        double temp = q->secconstval[a][j] * J;
+        for (int l = 0; l < 9; ++l)
+            st[l] += T[l] * temp;
+    }
+    if (lane < 4 * kNSG) {
+        double Jh[3], D1[3], D2[3];
+        double tj = Jacobi(g.cc, q->lc[0][0], q->lc[0][1]);
+        double n0[3];
+        NormalAt(g.cc, q->lc[0][0], q->lc[0][1], n0);
+        Jh[0] = tj * n0[0];
+        Jh[1] = tj * n0[1];
+        Jh[2] = tj * n0[2];
+        RDiffs1(g.cc, 0.0, 0.0, D1);
+        RDiffs2(g.cc, 0.0, 0.0, D2);
+        int a = lane / kNSG, j = lane - a * kNSG;
+        double cs = q->linecos[a][j], sn = q->linesin[a][j];
+        double A[6], F[9];
+        A[1] = D1[0] * cs + D2[0] * sn;
+        A[2] = D1[1] * cs + D2[1] * sn;
+        A[3] = D1[2] * cs + D2[2] * sn;
+        A[0] = sqrt(A[1] * A[1] + A[2] * A[2] + A[3] * A[3]);
+        A[4] = A[0] * A[0] * A[0];
+        A[5] = A[4] * A[0] * A[0];
+        double AkJk0 = A[1] * Jh[0] + A[2] * Jh[1] + A[3] * Jh[2];
+        double t1 = 3.0 * A[1] * AkJk0 / A[5];
+        double t2 = coef2 * (A[2] * Jh[0] - A[1] * Jh[1]) / A[4];
+        double t3 = coef2 * (A[3] * Jh[0] - A[1] * Jh[2]) / A[4];
+        double t4 = coef2 * (A[3] * Jh[1] - A[2] * Jh[2]) / A[4];
+        double temp = coef2 * AkJk0 / A[4];
+        F[0] = coef1 * (temp + t1 * A[1]);
+        F[1] = coef1 * (t1 * A[2] - t2);
+        F[2] = coef1 * (t1 * A[3] - t3);
+        F[3] = coef1 * (t1 * A[2] + t2);
+        F[4] = coef1 * (temp + 3.0 * A[2] * A[2] * AkJk0 / A[5]);
+        t1 = 3.0 * A[3] * AkJk0 / A[5];
+        F[5] = coef1 * (t1 * A[2] - t4);
+        F[6] = coef1 * (t1 * A[1] + t3);
+        F[7] = coef1 * (t1 * A[2] + t4);
+        F[8] = coef1 * (temp + t1 * A[3]);
+        for (int l = 0; l < kNSG; ++l) {
+            double fv = q->constfm1secval[a][l * kNSG + j];
+            for (int k = 0; k < 9; ++k)
+                st[k] -= F[k] * fv;
+        }
+        double fl = q->constfm1lineval[a][j];
+        for (int k = 0; k < 9; ++k)
+            st[k] += F[k] * fl;
+    }
+    double* u = v + 9;
+    double* a1 = v + 15;
+    double* a2 = v + 24;
+    for (int row = lane; row < 4 * kNSPW2; row += 64) {
+        int a = row / kNSPW2, k = row - a * kNSPW2;
+        for (int l = 0; l < kNSPWG2; ++l) {
+            double s1 = q->secconstpospw[a][0][k][l], s2 = q->secconstpospw[a][1][k][l];
+            double J = Jacobi(g.cc, s1, s2);
+            double nor[3], p[3];
+            NormalAt(g.cc, s1, s2, nor);
+            FromLocal(g.cc, s1, s2, p);
+            dev::Dist d;
+            dev::MakeDist(node, p, d);
+            double U[9];
+            dev::DynaU(m, 0.0, d, U);
+            double temp = q->secconstvalpw[a][k][l] * J;
+            u[0] += U[0] * temp;
+            u[1] += U[1] * temp;
+            u[2] += U[2] * temp;
+            u[3] += U[4] * temp;
+            u[4] += U[5] * temp;
+            u[5] += U[8] * temp;
+            double SR[4] = {d.R[1], d.RI[1], d.RI[2], d.RI[3]};
+            double T1[9], T2[9], ST[9];
+            dev::DynaT(m, 1, 0.0, d, nor, T1);
+            dev::DynaT(m, 2, 1.0, d, nor, T2);
+            dev::StaticT(m, SR, nor, ST);
+            for (int c = 0; c < 9; ++c) {
+                a1[c] += (T1[c] - ST[c]) * temp;
+                a2[c] += (T2[c] - ST[c]) * temp;
+            }
+        }
+    }
+    for (int k = 0; k < kVals; ++k)
+        red[k][lane] = v[k];
+    __syncthreads();
+    for (int w = 32; w > 0; w >>= 1) {
+        if (lane < w)
+            for (int k = 0; k < kVals; ++k)
+                red[k][lane] += red[k][lane + w];
+        __syncthreads();
+    }
+    if (lane == 0) {
+        double* uo = u1o + 9 * (long)e;
+        uo[0] = red[9][0];
+        uo[1] = red[10][0];
+        uo[2] = red[11][0];
+        uo[3] = red[10][0];
+        uo[4] = red[12][0];
+        uo[5] = red[13][0];
+        uo[6] = red[11][0];
+        uo[7] = red[13][0];
+        uo[8] = red[14][0];
+        double* to1 = t1o + 9 * (long)e;
+        double* to2 = t2o + 9 * (long)e;
+        for (int c = 0; c < 9; ++c) {
+            to1[c] = red[15 + c][0] + red[c][0];
+            to2[c] = red[24 + c][0] + red[c][0];
+        }
+        to1[0] += 0.5;
+        to1[4] += 0.5;
+        to1[8] += 0.5;
+    }
+}
+
+void LaunchSingular(const ElemGeo* geo, const int* dom, const QuadTables* q, double* u1, double* t1, double* t2,
+                    int ne, bool exact, hipStream_t s)
+{
+    if (ne <= 0)
+        return;
+    if (exact)
+        k_singular<<<(ne + 63) / 64, 64, 0, s>>>(geo, dom, q, u1, t1, t2, ne);
+    else
+        k_singular_wave<<<ne, 64, 0, s>>>(geo, dom, q, u1, t1, t2, ne);
 }
 
 // ---------------------------------------------------------------------------

@@ -54,8 +54,10 @@ __device__ __forceinline__ void DecodePairOf(const PairGrid& g, long p, int& s, 
 void UploadDynaMats(const DynaMat* mats, int nd, hipStream_t s);
 
 void LaunchPoints(const ElemGeo* geo, const QuadTables* q, ElemPoints* pts, int ne, hipStream_t s);
+// Singular self tables; exact = one lane per element in the reference's
+// summation order, else one wavefront per element (same sums, tree order).
 void LaunchSingular(const ElemGeo* geo, const int* dom, const QuadTables* q, double* u1, double* t1, double* t2,
-                    int ne, hipStream_t s);
+                    int ne, bool exact, hipStream_t s);
 
 // Causality cull: band [kb, kb+len) of each pair and near-field flag.
 void LaunchCull(const ElemGeo* geo, const int* dom, PairGrid grid, long npairs, int maxn, int* kb, int* len,

@@ -192,10 +192,23 @@ void Case::Create(long npoints, const double* points, long nele, const int* ele_
     d_t2.alloc(9 * (size_t)ne);
     double t0 = Now();
     LaunchPoints(d_geo.p, d_tables.p, d_pts.p, (int)ne, stream);
-    LaunchSingular(d_geo.p, d_dom.p, d_tables.p, d_u1.p, d_t1.p, d_t2.p, (int)ne, stream);
+    LaunchSingular(d_geo.p, d_dom.p, d_tables.p, d_u1.p, d_t1.p, d_t2.p, (int)ne, !quadStaged, stream);
+    selfMode = quadStaged ? 1 : 0;
     Check(hipStreamSynchronize(stream), "setup kernels");
     timing[0] = Now() - t0;
     Check(hipGetLastError(), "setup launch");
+}
+
+void Case::EnsureSelfTables()
+{
+    int want = quadStaged ? 1 : 0;
+    if (selfMode == want)
+        return;
+    double t0 = Now();
+    LaunchSingular(d_geo.p, d_dom.p, d_tables.p, d_u1.p, d_t1.p, d_t2.p, (int)ne, !quadStaged, stream);
+    Check(hipStreamSynchronize(stream), "self tables");
+    selfMode = want;
+    timing[0] = Now() - t0;
 }
 
 void Case::Cull(int maxn)
@@ -304,6 +317,7 @@ void Case::FreezeStructure(bool on)
 // AddBlock builders on the host.
 void Case::ComputeStep0(CCSRMat& Aout, CCSRMat& Kout)
 {
+    EnsureSelfTables();
     std::vector<long> list = NearPairs();
     DevBuf<long> d_list;
     d_list.upload(list.data(), list.size(), stream);
@@ -421,6 +435,7 @@ void Case::AssembleHistory(int maxn_in)
         throw std::runtime_error("maxn must be >= 1");
     if (frozen && maxn_in != maxn)
         throw std::runtime_error("the frozen structure holds the bands of the last history assembly");
+    EnsureSelfTables();
     maxn = maxn_in;
     h_mts.clear();
     h_mgs.clear();

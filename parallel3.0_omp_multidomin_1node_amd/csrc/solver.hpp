@@ -83,6 +83,9 @@ public:
     // assembled again before a solve).
     void GetMaterials(double* evr) const;
     void FreezeStructure(bool on);
+    // (Re)computes the singular self tables when they are missing or were made
+    // by the other kernel than quadStaged asks for (reference order when false).
+    void EnsureSelfTables();
     void SetMaterials(const double* evr);
     void ExportMatrix(int which, int step, std::vector<int>& rp, std::vector<int>& col, std::vector<double>& val);
 
@@ -103,6 +106,7 @@ public:
     double timing[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     bool convTiled = true;   // time-blocked convolution (false: one full pass per step)
     int convTile = 16;       // steps per far-part tile (8 or 16); fixes the near lists built by the cull
+    int selfMode = -1;       // kernel of the current self tables: 1 wave per element, 0 reference order
     bool quadStaged = true;  // wave-per-pair quadrature with LDS-staged point geometry (false: lane per entry)
     int mixed = 0;  // for the next history assembly: 0 FP64, 1 FP32 blocks, 2 FP32 + bf16 residual blocks
     bool historyMixed = false, historySplit = false;

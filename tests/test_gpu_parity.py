@@ -48,10 +48,24 @@ def test_dofmap_bitexact(rod_pair):
         assert np.array_equal(od[k], gd[k]), k
 
 
-def test_self_tables_bitexact(rod_pair):
+def test_self_tables(oracle_mod, rod_pair, rod_fixture):
+    """Singular self tables: the default wave-per-element kernel (tree-ordered
+    sums) within 1e-13 of the table's largest entry; the lane-per-element
+    kernel (quad_staged 0, reference summation order) bit for bit.  The T2
+    table of a flat element is zero up to rounding (entries of 1e-16 whose
+    digits depend on the summation order), so it is compared against the T1
+    table's scale."""
     oc, gc, f = rod_pair
-    u, t1, t2 = gc.self_tables()
     ref = np.array([oc.self_tables(e) for e in range(len(f["ele"]))])
+    scale = {"U": np.abs(ref[:, 0]).max(), "T1": np.abs(ref[:, 1]).max(), "T2": np.abs(ref[:, 1]).max()}
+    for name, got, want in zip(("U", "T1", "T2"), gc.self_tables(), (ref[:, 0], ref[:, 1], ref[:, 2])):
+        err = np.abs(got - want).max()
+        print(name, "largest difference", err, "scale", scale[name], flush=True)
+        assert err <= 1e-13 * scale[name]
+    _, gx, _ = _rod_cases(oracle_mod, rod_fixture)
+    gx.set_option("quad_staged", 0)
+    u, t1, t2 = gx.self_tables()
+    gx.close()
     assert np.array_equal(u, ref[:, 0]) and np.array_equal(t1, ref[:, 1]) and np.array_equal(t2, ref[:, 2])
 
 
@@ -78,7 +92,7 @@ def _same_ccsr(a, b, rtol):
 def test_step0_matrices(rod_pair):
     oc, gc, f = rod_pair
     for which in (0, 1):
-        _same_ccsr(gc.ccsr(which), oc.ccsr(which), 1e-15)
+        _same_ccsr(gc.ccsr(which), oc.ccsr(which), 1e-12)
 
 
 def test_history_blocks_all_steps(rod_pair):
@@ -150,7 +164,7 @@ def test_generated_cases_match_oracle(oracle_mod, which):
     gc.assemble_history(w.nstep)
     oc.assemble(w.nstep)
     for which_m in (0, 1):
-        _same_ccsr(gc.ccsr(which_m), oc.ccsr(which_m), 1e-14)
+        _same_ccsr(gc.ccsr(which_m), oc.ccsr(which_m), 1e-12)
     for k in range(1, w.nstep + 1):
         for which_m in (2, 3):
             _bands_close(gc.ccsr(which_m, k), oc.ccsr(which_m, k))

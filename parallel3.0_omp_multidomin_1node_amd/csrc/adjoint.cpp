@@ -244,20 +244,42 @@ double Case::Adjoint(int nstep, const double* known_lu, const int* recv, int nre
     d_dj.upload(dJ.data(), dJ.size(), stream);
     PrepareGmres(restart);
     GmresOp<double> opT{dAt.rp.p, dAt.col.p, dAt.val.p, nullptr, dMt.rp.p, dMt.col.p, dMt.val.p};
-    for (int K = nstep; K >= 1; --K) {
-        CheckA(hipMemcpyAsync(d_y.p, d_dj.p + (size_t)K * n3, sizeof(double) * (size_t)n3, hipMemcpyDeviceToDevice,
-                              stream),
-               "adjoint rhs");
-        if (K < nstep) {
-            LaunchConvT(grid, d_refs.p, d_off.p, d_kb.p, d_len.p, d_mts.p, d_mgs.p, d_lam.p, n3, nstep, K, (int)ne,
-                        d_z.p, stream);
-            LaunchStateT(d_refs.p, d_geo.p, d_cuPtr.p, d_cuMem.p, d_ctPtr.p, d_ctMem.p, d_z.p, (int)ne, d_y.p, stream);
+    // time-blocked transposed history term: per block of kConvTTile backward
+    // steps the lags > kConvTTile in one pass over the bands, the shorter lags
+    // per step
+    const int TT = convTiled ? kConvTTile : 0;
+    const int pstride = nstep + kConvTTile + 2;
+    DevBuf<double> d_phi, d_zf;
+    if (TT) {
+        d_phi.alloc((size_t)ne * (size_t)pstride * 3);
+        LaunchZero(d_phi.p, (long)d_phi.n, stream);
+        d_zf.alloc((size_t)TT * 6 * (size_t)ne);
+    }
+    for (int m1 = nstep; m1 >= 1; m1 -= std::max(TT, 1)) {
+        int mlo = TT ? std::max(1, m1 - TT + 1) : m1;
+        if (TT) {
+            LaunchPhi(d_lam.p, n3, (int)ne, nstep, m1 + 2, std::min(m1 + TT + 1, nstep + 1), pstride, d_phi.p,
+                      stream);
+            LaunchConvTFar(grid, d_refs.p, d_off.p, d_kb.p, d_len.p, d_mts.p, d_mgs.p, d_phi.p, pstride, nstep, mlo,
+                           (int)ne, d_zf.p, stream);
         }
-        int it = 0;
-        int flag = GmresCore<double>(d_y.p, d_lam.p + (size_t)K * n3, restart, tol, &it, opT, d_V.p, d_W.p, d_u.p,
-                                     d_u2.p);
-        if (flag != 0)
-            throw std::runtime_error("adjoint GMRES did not converge at step " + std::to_string(K));
+        for (int K = m1; K >= mlo; --K) {
+            CheckA(hipMemcpyAsync(d_y.p, d_dj.p + (size_t)K * n3, sizeof(double) * (size_t)n3,
+                                  hipMemcpyDeviceToDevice, stream),
+                   "adjoint rhs");
+            if (K < nstep) {
+                LaunchConvT(grid, d_refs.p, d_off.p, d_kb.p, d_len.p, d_mts.p, d_mgs.p, d_lam.p, n3, nstep, K,
+                            TT ? TT : nstep + 1, TT ? d_zf.p + (size_t)(K - mlo) * 6 * (size_t)ne : nullptr,
+                            (int)ne, d_z.p, stream);
+                LaunchStateT(d_refs.p, d_geo.p, d_cuPtr.p, d_cuMem.p, d_ctPtr.p, d_ctMem.p, d_z.p, (int)ne, d_y.p,
+                             stream);
+            }
+            int it = 0;
+            int flag = GmresCore<double>(d_y.p, d_lam.p + (size_t)K * n3, restart, tol, &it, opT, d_V.p, d_W.p,
+                                         d_u.p, d_u2.p);
+            if (flag != 0)
+                throw std::runtime_error("adjoint GMRES did not converge at step " + std::to_string(K));
+        }
     }
 
     // 5. gradient: central differences of the operators, contracted with lambda,

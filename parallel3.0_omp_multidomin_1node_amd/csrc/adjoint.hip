@@ -23,7 +23,8 @@ __global__ void __launch_bounds__(256) k_conv_t(PairGrid grid, const NodeRefs* _
                                                 const long* __restrict__ off, const int* __restrict__ kb,
                                                 const int* __restrict__ len, const double* __restrict__ mts,
                                                 const double* __restrict__ mgs, const double* __restrict__ lam,
-                                                long n3, int N, int m, double* __restrict__ z)
+                                                long n3, int N, int m, int jmax, const double* __restrict__ zadd,
+                                                double* __restrict__ z)
 {
     int f = blockIdx.x;
     int d = refs[f].domain;
@@ -41,7 +42,7 @@ __global__ void __launch_bounds__(256) k_conv_t(PairGrid grid, const NodeRefs* _
         long base = off[p];
         for (int t = lane; t < L; t += 64) {
             int j = k0 + t;
-            if (m + j - 1 > N)
+            if (m + j - 1 > N || j > jmax)
                 break;
             double lt0 = 0.0, lt1 = 0.0, lt2 = 0.0;
             for (int zz = 0; zz < 3; ++zz) {
@@ -77,14 +78,126 @@ __global__ void __launch_bounds__(256) k_conv_t(PairGrid grid, const NodeRefs* _
     }
     if (threadIdx.x == 0)
         for (int k = 0; k < 6; ++k)
-            z[6 * (long)f + k] = red[k][0];
+            z[6 * (long)f + k] = red[k][0] + (zadd ? zadd[6 * (long)f + k] : 0.0);
 }
 
 void LaunchConvT(PairGrid grid, const NodeRefs* refs, const long* off, const int* kb, const int* len,
-                 const double* mts, const double* mgs, const double* lam, long n3, int N, int m, int ne, double* z,
-                 hipStream_t s)
+                 const double* mts, const double* mgs, const double* lam, long n3, int N, int m, int jmax,
+                 const double* zadd, int ne, double* z, hipStream_t s)
 {
-    k_conv_t<<<ne, 256, 0, s>>>(grid, refs, off, kb, len, mts, mgs, lam, n3, N, m, z);
+    k_conv_t<<<ne, 256, 0, s>>>(grid, refs, off, kb, len, mts, mgs, lam, n3, N, m, jmax, zadd, z);
+}
+
+// Filtered adjoint series of every source row: Phi[s][K] = lambda_{K-1} +
+// 2 lambda_K + lambda_{K+1} (lambda_K = 0 outside 1 .. N) for K in [K0, K1],
+// so the transposed history term of lag j at step m reads Phi[s][m + j]
+// whenever m + j - 1 > m (every lag j >= 2).
+__global__ void __launch_bounds__(256) k_phi(const double* __restrict__ lam, long n3, int ne, int N, int K0, int K1,
+                                             int stride, double* __restrict__ phi)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int nk = K1 - K0 + 1;
+    if (i >= (long)ne * nk * 3)
+        return;
+    int s = (int)(i / ((long)nk * 3));
+    int r = (int)(i - (long)s * nk * 3);
+    int K = K0 + r / 3, c = r % 3;
+    double v = 0.0;
+    if (K - 1 >= 1 && K - 1 <= N)
+        v += lam[(long)(K - 1) * n3 + 3 * (long)s + c];
+    if (K >= 1 && K <= N)
+        v += 2.0 * lam[(long)K * n3 + 3 * (long)s + c];
+    if (K + 1 >= 1 && K + 1 <= N)
+        v += lam[(long)(K + 1) * n3 + 3 * (long)s + c];
+    phi[((long)s * stride + K) * 3 + c] = v;
+}
+
+void LaunchPhi(const double* lam, long n3, int ne, int N, int K0, int K1, int stride, double* phi, hipStream_t s)
+{
+    long n = (long)ne * (K1 - K0 + 1) * 3;
+    if (n > 0)
+        k_phi<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(lam, n3, ne, N, K0, K1, stride, phi);
+}
+
+// Far part of the transposed history term for a block of T backward steps
+// m = mlo .. mlo + T - 1: every lag j > T of every pair reads
+// Phi[s][m + j] >= m + T + 1 > the block, all known when the block starts.
+// zf[i][6 f .. 6 f + 5] = ( -sum MTS^T Phi , sum MGS Phi ) for m = mlo + i.
+// One workgroup per field element, four wavefronts over its sources, one lag
+// per lane; each band entry is read once per block.
+template <int T>
+__global__ void __launch_bounds__(256) k_conv_t_far(PairGrid grid, const NodeRefs* __restrict__ refs,
+                                                    const long* __restrict__ off, const int* __restrict__ kb,
+                                                    const int* __restrict__ len, const double* __restrict__ mts,
+                                                    const double* __restrict__ mgs, const double* __restrict__ phi,
+                                                    int stride, int N, int mlo, long ne, double* __restrict__ zf)
+{
+    int f = blockIdx.x;
+    int d = refs[f].domain;
+    int nd = grid.eleCount[d], eb = grid.eleBegin[d];
+    int fl = f - eb;
+    int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double a[T][6];
+#pragma unroll
+    for (int i = 0; i < T; ++i)
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+            a[i][k] = 0.0;
+    for (int q = wave; q < nd; q += 4) {
+        long p = grid.pairBase[d] + (long)q * nd + fl;
+        int L = len[p];
+        if (L == 0)
+            continue;
+        int k0 = kb[p];
+        long base = off[p];
+        const double* ph = phi + (long)(eb + q) * stride * 3;
+        int t = lane + (k0 > T ? 0 : T + 1 - k0);
+        for (; t < L; t += 64) {
+            int j = k0 + t;
+            if (mlo + j > N + 1)
+                break;
+            long e = base + t;
+            const double* M = mts + 9 * e;
+            const double* G = mgs + 6 * e;
+            double m0 = M[0], m1 = M[1], m2 = M[2], m3 = M[3], m4 = M[4], m5 = M[5], m6 = M[6], m7 = M[7], m8 = M[8];
+            double g0 = G[0], g1 = G[1], g2 = G[2], g3 = G[3], g4 = G[4], g5 = G[5];
+            const double* pw = ph + (long)(mlo + j) * 3;
+#pragma unroll
+            for (int i = 0; i < T; ++i) {
+                double l0 = pw[3 * i], l1 = pw[3 * i + 1], l2 = pw[3 * i + 2];
+                a[i][0] -= fma(m0, l0, fma(m3, l1, m6 * l2));
+                a[i][1] -= fma(m1, l0, fma(m4, l1, m7 * l2));
+                a[i][2] -= fma(m2, l0, fma(m5, l1, m8 * l2));
+                a[i][3] += fma(g0, l0, fma(g1, l1, g2 * l2));
+                a[i][4] += fma(g1, l0, fma(g3, l1, g4 * l2));
+                a[i][5] += fma(g2, l0, fma(g4, l1, g5 * l2));
+            }
+        }
+    }
+    __shared__ double red[4][T * 6];
+#pragma unroll
+    for (int i = 0; i < T; ++i)
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            double v = a[i][k];
+            for (int w = 32; w > 0; w >>= 1)
+                v += __shfl_xor(v, w);
+            if (lane == 0)
+                red[wave][i * 6 + k] = v;
+        }
+    __syncthreads();
+    if ((int)threadIdx.x < T * 6) {
+        int i = threadIdx.x / 6, k = threadIdx.x - 6 * i;
+        zf[(long)i * 6 * ne + 6 * (long)f + k] =
+            red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+    }
+}
+
+void LaunchConvTFar(PairGrid grid, const NodeRefs* refs, const long* off, const int* kb, const int* len,
+                    const double* mts, const double* mgs, const double* phi, int stride, int N, int mlo, int ne,
+                    double* zf, hipStream_t s)
+{
+    k_conv_t_far<kConvTTile><<<ne, 256, 0, s>>>(grid, refs, off, kb, len, mts, mgs, phi, stride, N, mlo, ne, zf);
 }
 
 // Transposed state map: the column sums of z go back through

@@ -5,11 +5,23 @@
 
 namespace tdbem {
 
-// Transposed history convolution for the adjoint state of step m; lam holds
-// lambda_K at lam + K * n3 (K = 1 .. N); z receives 6 values per element.
+// Steps per block of the time-blocked transposed history convolution.
+constexpr int kConvTTile = 8;
+
+// Transposed history convolution for the adjoint state of step m over the
+// lags j <= jmax; lam holds lambda_K at lam + K * n3 (K = 1 .. N); z receives
+// 6 values per element (plus zadd when given).
 void LaunchConvT(PairGrid grid, const NodeRefs* refs, const long* off, const int* kb, const int* len,
-                 const double* mts, const double* mgs, const double* lam, long n3, int N, int m, int ne, double* z,
-                 hipStream_t s);
+                 const double* mts, const double* mgs, const double* lam, long n3, int N, int m, int jmax,
+                 const double* zadd, int ne, double* z, hipStream_t s);
+// Phi[s][K] = lambda_{K-1} + 2 lambda_K + lambda_{K+1} for K in [K0, K1]
+// (stride values of K per row).
+void LaunchPhi(const double* lam, long n3, int ne, int N, int K0, int K1, int stride, double* phi, hipStream_t s);
+// Far part (lags > kConvTTile) of the transposed history term for the steps
+// mlo .. mlo + kConvTTile - 1, into zf[i][6 ne].
+void LaunchConvTFar(PairGrid grid, const NodeRefs* refs, const long* off, const int* kb, const int* len,
+                    const double* mts, const double* mgs, const double* phi, int stride, int N, int mlo, int ne,
+                    double* zf, hipStream_t s);
 // Transposed state map into the unknown vector y (y += ...); colU / colT are
 // CSR lists of the elements reading each history column.
 void LaunchStateT(const NodeRefs* refs, const ElemGeo* geo, const long* colU, const int* colUf, const long* colT,

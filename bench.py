@@ -43,6 +43,12 @@ HBM_PEAK_GBS = 8000.0
 # profiles/k_conv_far16_traffic_layered2x2x4_h0.1_fp64.md.
 MEASURED_TRAFFIC = {("layered_2x2x4_h0.1", 400, 16): 70341037698.0}
 FP64_PEAK_TFLOPS = 78.6
+# FP64 lane-operations of one history quadrature (k_history_pw + k_history_reg)
+# of the default workload: 64 x (ADD + MUL + TRANS + 2 FMA) F64 VALU wave
+# instructions from the SQ_INSTS_VALU_*_F64 counters (every lane of an issued
+# instruction counted, so an upper bound of the useful FLOPs); the summary with
+# its command is kept in profiles/k_history_fp64_valu_layered2x2x4_h0.1.md.
+MEASURED_QUAD_FLOPS = {("layered_2x2x4_h0.1", 400): 1.8535e13}
 
 
 def log(msg):
@@ -299,6 +305,9 @@ def main():
                    "conv_tile": args.conv_tile},
         "breakdown_s": {k: round(float(v), 4) for k, v in tm.items() if k != "conv_far_launches"},
         "time_loop_steps_per_s": round(w.nstep / max(loop_time, 1e-9), 3),
+        # every band entry holds one MTS and one MGS block (computed; the
+        # 1e-14 drop rule may leave some empty)
+        "assembled_blocks_per_s": round(2.0 * float(last["entries"]) / max(tm["history"], 1e-9), 1),
         "gmres_inner_iterations_mean": float(np.mean(last["iters"])),
         "roofline": {"kernel": f"k_conv_far (time-blocked history convolution, {args.conv_tile} steps per launch)",
                      "bound": "hbm", "achieved": round(conv_gbs, 2),
@@ -307,6 +316,13 @@ def main():
                      "avg_launch_ms": round(1000 * conv_avg, 4),
                      "bytes_per_launch": round(bytes_launch)},
     }
+    qflops = MEASURED_QUAD_FLOPS.get((w.name, w.nstep))
+    if qflops and tm.get("quadrature", 0.0) > 0.0:
+        qrate = qflops / tm["quadrature"] / 1e12
+        out["quadrature_roofline"] = {"kernels": "k_history_pw + k_history_reg (history quadrature)",
+                                      "bound": "fp64-valu", "achieved": round(qrate, 2), "peak": FP64_PEAK_TFLOPS,
+                                      "unit": "TFLOP/s", "frac": round(qrate / FP64_PEAK_TFLOPS, 4),
+                                      "flops_per_assembly": qflops, "kernel_s": round(tm["quadrature"], 4)}
     c.close()
     if not args.no_batched:
         out["batched"] = batched_report(tdbem, w, value)

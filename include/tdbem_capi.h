@@ -121,6 +121,10 @@ int tdbem_dyna_gmres_solver(tdbem_ctx* ctx, long iterations, double error, long 
  * quadrature, t[3] step 0, t[4] convolution (sum over steps), t[5] GMRES
  * (sum over steps), t[6] far-part convolution kernels, t[7] their count. */
 int tdbem_timings(tdbem_ctx* ctx, double* t8);
+/* The same timings and more, n values: t[8] the history quadrature kernels
+ * alone (device events; t[2] also holds the cull and the wave packing).
+ * Values past the known ones are 0. */
+int tdbem_timings_n(tdbem_ctx* ctx, int n, double* t);
 
 /* Number of stored history band entries per lag j = 1..maxn
  * (counts[j-1]) and the pair count, for byte accounting of the convolution.

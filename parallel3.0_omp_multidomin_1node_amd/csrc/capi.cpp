@@ -275,6 +275,15 @@ int tdbem_timings(tdbem_ctx* ctx, double* t8)
     return 0;
 }
 
+int tdbem_timings_n(tdbem_ctx* ctx, int n, double* t)
+{
+    if (!ctx || !t || n < 0)
+        return Fail("tdbem_timings_n: invalid arguments");
+    for (int i = 0; i < n; ++i)
+        t[i] = i < 9 ? ctx->c.timing[i] : 0.0;
+    return 0;
+}
+
 int tdbem_history_stats(tdbem_ctx* ctx, long* counts, long* npairs)
 {
     if (!ctx || !counts || !npairs)

@@ -1322,6 +1322,61 @@ void LaunchPwCount(const ElemGeo* geo, PairGrid grid, const int* kb, const int* 
     k_pwcount<<<(unsigned)((npairs + 255) / 256), 256, 0, s>>>(geo, grid, kb, len, off, cls, npairs, cnt);
 }
 
+// Greedy packing of one field element's patch-wise lags into PwWaves: its
+// sources in order, up to kPwSlots pairs and 64 lanes per wave, a pair that
+// does not fit continuing in the next wave.  One thread per field element;
+// the count pass (waves == nullptr) writes nw[f], the write pass fills the
+// waves from woff[f] on.
+__global__ void __launch_bounds__(256) k_pwpack(PairGrid grid, const NodeRefs* __restrict__ refs,
+                                                const int* __restrict__ cnt, int ne, const long* __restrict__ woff,
+                                                long* __restrict__ nw, PwWave* __restrict__ waves)
+{
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= ne)
+        return;
+    int d = refs[f].domain;
+    long nd = grid.eleCount[d];
+    long fl = f - grid.eleBegin[d];
+    PwWave cur{};
+    int lanes = 0;
+    long w = waves ? woff[f] : 0, count = 0;
+    for (long sl = 0; sl < nd; ++sl) {
+        long p = grid.pairBase[d] + sl * nd + fl;
+        int c = cnt[p], r0 = 0;
+        while (c > 0) {
+            if (cur.nseg == kPwSlots || lanes == 64) {
+                if (waves)
+                    waves[w + count] = cur;
+                ++count;
+                cur = PwWave{};
+                lanes = 0;
+            }
+            int take = c < 64 - lanes ? c : 64 - lanes;
+            cur.pair[cur.nseg] = p;
+            cur.rank0[cur.nseg] = r0;
+            cur.cnt[cur.nseg] = take;
+            cur.nseg++;
+            lanes += take;
+            c -= take;
+            r0 += take;
+        }
+    }
+    if (cur.nseg > 0) {
+        if (waves)
+            waves[w + count] = cur;
+        ++count;
+    }
+    if (!waves)
+        nw[f] = count;
+}
+
+void LaunchPwPack(PairGrid grid, const NodeRefs* refs, const int* cnt, int ne, const long* woff, long* nw,
+                  PwWave* waves, hipStream_t s)
+{
+    if (ne > 0)
+        k_pwpack<<<(ne + 255) / 256, 256, 0, s>>>(grid, refs, cnt, ne, woff, nw, waves);
+}
+
 // Judges of every band entry (f2 | f1 << 2) with the current materials.
 __global__ void __launch_bounds__(256) k_classify(const ElemGeo* __restrict__ geo, PairGrid grid,
                                                   const int* __restrict__ kb, const int* __restrict__ len,
@@ -1419,7 +1474,7 @@ __global__ void __launch_bounds__(64) k_history_reg(
 // picking the regular path) are added here too, so each entry is finished by
 // exactly one kernel.
 template <typename TB>
-__global__ void __launch_bounds__(64) k_history_pw(
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_history_pw(
     const ElemGeo* __restrict__ geo, const ElemPoints* __restrict__ pts, const NodeRefs* __restrict__ refs,
     const double* __restrict__ selfT2, PairGrid grid, const PwWave* __restrict__ waves, long w0, long w1,
     const long* __restrict__ off, const int* __restrict__ kb, const int* __restrict__ len,

@@ -83,6 +83,10 @@ struct PwWave {
 };
 // cls (nullable): frozen judges of every band entry (f2 | f1 << 2) used
 // instead of judging with the current materials.
+// PwWave packing per field element: count pass (waves null) -> nw[f], write
+// pass -> waves[woff[f] ..].
+void LaunchPwPack(PairGrid grid, const NodeRefs* refs, const int* cnt, int ne, const long* woff, long* nw,
+                  PwWave* waves, hipStream_t s);
 void LaunchPwCount(const ElemGeo* geo, PairGrid grid, const int* kb, const int* len, const long* off,
                    const unsigned char* cls, long npairs, int* cnt, hipStream_t s);
 void LaunchClassify(const ElemGeo* geo, PairGrid grid, const int* kb, const int* len, const long* off, long npairs,

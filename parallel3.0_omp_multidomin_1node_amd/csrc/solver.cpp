@@ -293,6 +293,7 @@ std::vector<long> Case::NearPairs()
 // materials keep the same bands and take the same patch-wise / regular paths.
 void Case::FreezeStructure(bool on)
 {
+    pwPacked = false;
     if (!on) {
         frozen = false;
         d_cls.free();
@@ -509,61 +510,56 @@ void Case::StagedQuadrature(double* mts, double* mgs, float* mts32, float* mgs32
     // patch-wise lags per pair, then waves that pack the patch-wise lags
     // of consecutive sources with the same field element
     DevBuf<int> d_pwcnt;
-    d_pwcnt.alloc((size_t)npairs);
     const unsigned char* cls = frozen ? d_cls.p : nullptr;
-    LaunchPwCount(d_geo.p, grid, d_kb.p, d_len.p, d_off.p, cls, npairs, d_pwcnt.p, stream);
-    std::vector<int> pwcnt((size_t)npairs);
-    Check(hipMemcpyAsync(pwcnt.data(), d_pwcnt.p, sizeof(int) * (size_t)npairs, hipMemcpyDeviceToHost, stream),
-          "pw counts");
-    Check(hipStreamSynchronize(stream), "pw counts");
-    std::vector<PwWave> waves;
-    PwWave cur{};
-    int lanes = 0;
-    auto flush = [&]() {
-        if (cur.nseg > 0)
-            waves.push_back(cur);
-        cur = PwWave{};
-        lanes = 0;
-    };
-    for (int d = 0; d < grid.nd; ++d) {
-        long nd = grid.eleCount[d];
-        for (long fl = 0; fl < nd; ++fl) {
-            for (long sl = 0; sl < nd; ++sl) {
-                long p = grid.pairBase[d] + sl * nd + fl;
-                int c = pwcnt[(size_t)p], r0 = 0;
-                while (c > 0) {
-                    if (cur.nseg == kPwSlots || lanes == 64)
-                        flush();
-                    int take = std::min(c, 64 - lanes);
-                    cur.pair[cur.nseg] = p;
-                    cur.rank0[cur.nseg] = r0;
-                    cur.cnt[cur.nseg] = take;
-                    cur.nseg++;
-                    lanes += take;
-                    c -= take;
-                    r0 += take;
-                }
-            }
-            flush();
-        }
+    if (!(frozen && pwPacked)) {
+        d_pwcnt.alloc((size_t)npairs);
+        LaunchPwCount(d_geo.p, grid, d_kb.p, d_len.p, d_off.p, cls, npairs, d_pwcnt.p, stream);
     }
-    pwWaves = (long)waves.size();
-    DevBuf<PwWave> d_waves;
-    d_waves.upload(waves.data(), waves.size(), stream);
+    if (!(frozen && pwPacked)) {
+        DevBuf<long> d_nw;
+        d_nw.alloc((size_t)ne);
+        LaunchPwPack(grid, d_refs.p, d_pwcnt.p, (int)ne, nullptr, d_nw.p, nullptr, stream);
+        std::vector<long> nw((size_t)ne), woff((size_t)ne);
+        Check(hipMemcpyAsync(nw.data(), d_nw.p, sizeof(long) * (size_t)ne, hipMemcpyDeviceToHost, stream), "pw pack");
+        Check(hipStreamSynchronize(stream), "pw pack");
+        // waves in field-element order (domains are contiguous element ranges)
+        long total = 0;
+        for (long f = 0; f < ne; ++f) {
+            woff[(size_t)f] = total;
+            total += nw[(size_t)f];
+        }
+        pwWaves = total;
+        DevBuf<long> d_woff;
+        d_woff.upload(woff.data(), woff.size(), stream);
+        d_pwWaves.alloc((size_t)std::max<long>(1, pwWaves));
+        LaunchPwPack(grid, d_refs.p, d_pwcnt.p, (int)ne, d_woff.p, nullptr, d_pwWaves.p, stream);
+        Check(hipStreamSynchronize(stream), "pw pack");
+        pwPacked = frozen;
+    }
+    hipEvent_t qa, qb;
+    Check(hipEventCreate(&qa), "event");
+    Check(hipEventCreate(&qb), "event");
+    Check(hipEventRecord(qa, stream), "event");
     const long step = 1L << 20;  // wavefronts per launch
     auto launch = [&](auto* mts, auto* mgs) {
         for (long c0 = 0; c0 < chunks; c0 += step)
             LaunchHistoryReg(d_geo.p, d_pts.p, d_refs.p, d_t2.p, grid, d_coff.p, d_off.p, d_kb.p, d_len.p, cls,
                              npairs, c0, std::min(chunks, c0 + step), mts, mgs, d_mtsLo.p, d_mgsLo.p, alive, stream);
         for (long w0 = 0; w0 < pwWaves; w0 += step)
-            LaunchHistoryPw(d_geo.p, d_pts.p, d_refs.p, d_t2.p, grid, d_waves.p, w0, std::min(pwWaves, w0 + step),
+            LaunchHistoryPw(d_geo.p, d_pts.p, d_refs.p, d_t2.p, grid, d_pwWaves.p, w0, std::min(pwWaves, w0 + step),
                             d_off.p, d_kb.p, d_len.p, cls, mts, mgs, d_mtsLo.p, d_mgsLo.p, alive, stream);
     };
     if (mts32)
         launch(mts32, mgs32);
     else
         launch(mts, mgs);
-    Check(hipStreamSynchronize(stream), "history");
+    Check(hipEventRecord(qb, stream), "event");
+    Check(hipEventSynchronize(qb), "history");
+    float ms = 0.f;
+    Check(hipEventElapsedTime(&ms, qa, qb), "elapsed");
+    timing[8] = ms * 1e-3;
+    (void)hipEventDestroy(qa);
+    (void)hipEventDestroy(qb);
 }
 
 void Case::HistoryStats(long* counts) const

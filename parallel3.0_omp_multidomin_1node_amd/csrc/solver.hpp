@@ -103,7 +103,9 @@ public:
     int maxn = 0;
     CCSRMat A, KnownM;
     LeafPreconditioner pre;
-    double timing[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    // 0 singular, 1 cull, 2 history, 3 step 0, 4 convolution, 5 GMRES, 6 far
+    // convolution kernels, 7 their launches, 8 history quadrature kernels
+    double timing[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     bool convTiled = true;   // time-blocked convolution (false: one full pass per step)
     int convTile = 16;       // steps per far-part tile (8 or 16); fixes the near lists built by the cull
     int selfMode = -1;       // kernel of the current self tables: 1 wave per element, 0 reference order
@@ -146,6 +148,8 @@ private:
     DevBuf<long> d_off, d_coff;
     long chunks = 0;
     long pwWaves = 0;  // patch-wise waves of the last history assembly
+    DevBuf<PwWave> d_pwWaves;
+    bool pwPacked = false;  // d_pwWaves holds the packing of the frozen structure
     DevBuf<long> d_nearPtr, d_nearPair;
     DevBuf<int> d_nearField;
     int nearTile = 0;  // tile the near lists were built for

@@ -24,7 +24,7 @@ EXPORTED = [
     "tdbem_last_error", "tdbem_create", "tdbem_destroy", "tdbem_dofmap_export", "tdbem_assemble_step0",
     "tdbem_assemble_history", "tdbem_ccsr_export", "tdbem_self_tables", "tdbem_solve", "tdbem_solve_batched",
     "tdbem_adjoint", "tdbem_get_materials", "tdbem_set_materials", "tdbem_dyna_gmres_solver",
-    "tdbem_timings", "tdbem_history_stats", "tdbem_device_synchronize",
+    "tdbem_timings", "tdbem_timings_n", "tdbem_history_stats", "tdbem_device_synchronize",
     "tdbem_set_option",
 ]
 
@@ -65,6 +65,7 @@ def lib():
         L.tdbem_dyna_gmres_solver.argtypes = [C.c_void_p, C.c_long, C.c_double, C.c_long, C.c_int, _dp, _dp, _dp,
                                               C.POINTER(C.c_int)]
         L.tdbem_timings.argtypes = [C.c_void_p, _dp]
+        L.tdbem_timings_n.argtypes = [C.c_void_p, C.c_int, _dp]
         L.tdbem_history_stats.argtypes = [C.c_void_p, _lp, C.POINTER(C.c_long)]
         L.tdbem_device_synchronize.argtypes = []
         L.tdbem_set_option.argtypes = [C.c_void_p, C.c_char_p, C.c_double]
@@ -226,7 +227,7 @@ class Case:
         return counts, npairs.value
 
     def timings(self):
-        t = np.zeros(8)
-        _check(lib().tdbem_timings(self.h, t), "tdbem_timings")
+        t = np.zeros(9)
+        _check(lib().tdbem_timings_n(self.h, 9, t), "tdbem_timings_n")
         return dict(singular=t[0], cull=t[1], history=t[2], step0=t[3], convolution=t[4], gmres=t[5],
-                    conv_far=t[6], conv_far_launches=int(t[7]))
+                    conv_far=t[6], conv_far_launches=int(t[7]), quadrature=t[8])

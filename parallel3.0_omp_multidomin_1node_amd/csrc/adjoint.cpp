@@ -268,9 +268,12 @@ double Case::Adjoint(int nstep, const double* known_lu, const int* recv, int nre
                                   hipMemcpyDeviceToDevice, stream),
                    "adjoint rhs");
             if (K < nstep) {
-                LaunchConvT(grid, d_refs.p, d_off.p, d_kb.p, d_len.p, d_mts.p, d_mgs.p, d_lam.p, n3, nstep, K,
-                            TT ? TT : nstep + 1, TT ? d_zf.p + (size_t)(K - mlo) * 6 * (size_t)ne : nullptr,
-                            (int)ne, d_z.p, stream);
+                if (TT)
+                    LaunchConvTNear(grid, d_refs.p, d_off.p, d_kb.p, d_len.p, d_mts.p, d_mgs.p, d_lam.p, n3, nstep,
+                                    K, TT, d_zf.p + (size_t)(K - mlo) * 6 * (size_t)ne, (int)ne, d_z.p, stream);
+                else
+                    LaunchConvT(grid, d_refs.p, d_off.p, d_kb.p, d_len.p, d_mts.p, d_mgs.p, d_lam.p, n3, nstep, K,
+                                nstep + 1, nullptr, (int)ne, d_z.p, stream);
                 LaunchStateT(d_refs.p, d_geo.p, d_cuPtr.p, d_cuMem.p, d_ctPtr.p, d_ctMem.p, d_z.p, (int)ne, d_y.p,
                              stream);
             }

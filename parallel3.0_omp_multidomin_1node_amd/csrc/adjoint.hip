@@ -88,6 +88,76 @@ void LaunchConvT(PairGrid grid, const NodeRefs* refs, const long* off, const int
     k_conv_t<<<ne, 256, 0, s>>>(grid, refs, off, kb, len, mts, mgs, lam, n3, N, m, jmax, zadd, z);
 }
 
+// Short-lag part (lags j <= jmax) of the transposed history term of step m,
+// plus zadd: one workgroup per field element, one source pair per thread
+// (each pair has at most jmax short lags).
+__global__ void __launch_bounds__(256) k_conv_t_near(PairGrid grid, const NodeRefs* __restrict__ refs,
+                                                     const long* __restrict__ off, const int* __restrict__ kb,
+                                                     const int* __restrict__ len, const double* __restrict__ mts,
+                                                     const double* __restrict__ mgs, const double* __restrict__ lam,
+                                                     long n3, int N, int m, int jmax,
+                                                     const double* __restrict__ zadd, double* __restrict__ z)
+{
+    int f = blockIdx.x;
+    int d = refs[f].domain;
+    int nd = grid.eleCount[d], eb = grid.eleBegin[d];
+    int fl = f - eb;
+    double a[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int q = threadIdx.x; q < nd; q += 256) {
+        long p = grid.pairBase[d] + (long)q * nd + fl;
+        int L = len[p], k0 = kb[p];
+        int j1 = k0 + L - 1;
+        if (j1 > jmax)
+            j1 = jmax;
+        if (j1 > N - m + 1)
+            j1 = N - m + 1;
+        long s = eb + q;
+        for (int j = k0; j <= j1; ++j) {
+            double lt0 = 0.0, lt1 = 0.0, lt2 = 0.0;
+            for (int zz = 0; zz < 3; ++zz) {
+                int Kp = m + j - 1 + zz;
+                if (Kp >= m + 1 && Kp <= N) {
+                    double w = zz == 1 ? 2.0 : 1.0;
+                    const double* l = lam + (long)Kp * n3 + 3 * s;
+                    lt0 += w * l[0];
+                    lt1 += w * l[1];
+                    lt2 += w * l[2];
+                }
+            }
+            long e = off[p] + (j - k0);
+            const double* M = mts + 9 * e;
+            const double* G = mgs + 6 * e;
+            a[0] -= M[0] * lt0 + M[3] * lt1 + M[6] * lt2;
+            a[1] -= M[1] * lt0 + M[4] * lt1 + M[7] * lt2;
+            a[2] -= M[2] * lt0 + M[5] * lt1 + M[8] * lt2;
+            a[3] += G[0] * lt0 + G[1] * lt1 + G[2] * lt2;
+            a[4] += G[1] * lt0 + G[3] * lt1 + G[4] * lt2;
+            a[5] += G[2] * lt0 + G[4] * lt1 + G[5] * lt2;
+        }
+    }
+    __shared__ double red[4][6];
+    int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int k = 0; k < 6; ++k) {
+        double v = a[k];
+        for (int w = 32; w > 0; w >>= 1)
+            v += __shfl_xor(v, w);
+        if (lane == 0)
+            red[wave][k] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        int k = threadIdx.x;
+        z[6 * (long)f + k] = ((red[0][k] + red[1][k]) + red[2][k]) + red[3][k] + (zadd ? zadd[6 * (long)f + k] : 0.0);
+    }
+}
+
+void LaunchConvTNear(PairGrid grid, const NodeRefs* refs, const long* off, const int* kb, const int* len,
+                     const double* mts, const double* mgs, const double* lam, long n3, int N, int m, int jmax,
+                     const double* zadd, int ne, double* z, hipStream_t s)
+{
+    k_conv_t_near<<<ne, 256, 0, s>>>(grid, refs, off, kb, len, mts, mgs, lam, n3, N, m, jmax, zadd, z);
+}
+
 // Filtered adjoint series of every source row: Phi[s][K] = lambda_{K-1} +
 // 2 lambda_K + lambda_{K+1} (lambda_K = 0 outside 1 .. N) for K in [K0, K1],
 // so the transposed history term of lag j at step m reads Phi[s][m + j]

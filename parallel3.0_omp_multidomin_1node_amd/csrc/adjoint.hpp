@@ -14,6 +14,11 @@ constexpr int kConvTTile = 8;
 void LaunchConvT(PairGrid grid, const NodeRefs* refs, const long* off, const int* kb, const int* len,
                  const double* mts, const double* mgs, const double* lam, long n3, int N, int m, int jmax,
                  const double* zadd, int ne, double* z, hipStream_t s);
+// The lags j <= jmax of the same term, one source pair per thread (for the
+// short lags of the time-blocked sweep).
+void LaunchConvTNear(PairGrid grid, const NodeRefs* refs, const long* off, const int* kb, const int* len,
+                     const double* mts, const double* mgs, const double* lam, long n3, int N, int m, int jmax,
+                     const double* zadd, int ne, double* z, hipStream_t s);
 // Phi[s][K] = lambda_{K-1} + 2 lambda_K + lambda_{K+1} for K in [K0, K1]
 // (stride values of K per row).
 void LaunchPhi(const double* lam, long n3, int ne, int N, int K0, int K1, int stride, double* phi, hipStream_t s);

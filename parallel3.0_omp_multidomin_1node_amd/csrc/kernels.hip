@@ -2122,7 +2122,8 @@ __global__ void __launch_bounds__(256) k_conv_far(PairGrid grid, const NodeRefs*
 
 // Near part of step K in the tile starting at K0 plus the far part:
 // rhs_K += hfar[K-K0] + sum over near pairs (band start <= K-K0+1) of the
-// taps with K0 <= m <= K-1.  One wavefront per source row.
+// taps with K0 <= m <= K-1.  One wavefront per source row, one near pair per
+// lane (each pair has at most K-K0+1 near lags).
 template <typename TB>
 __global__ void __launch_bounds__(64) k_conv_near(const NodeRefs* __restrict__ refs, const long* __restrict__ nearPtr,
                                                   const long* __restrict__ nearPair, const int* __restrict__ nearField,
@@ -2140,17 +2141,18 @@ __global__ void __launch_bounds__(64) k_conv_near(const NodeRefs* __restrict__ r
     if (jlim > maxn)
         jlim = maxn;
     double y[3] = {0.0, 0.0, 0.0};
-    for (long q = nearPtr[s]; q < nearPtr[s + 1]; ++q) {
+    for (long q = nearPtr[s] + lane; q < nearPtr[s + 1]; q += 64) {
         long p = nearPair[q];
         int k0 = kb[p], L = len[p];
+        int jmax = k0 + L - 1 < jlim ? k0 + L - 1 : jlim;
+        if (jmax < k0)
+            continue;
         int f = nearField[q];
         const double* su = gU + (long)refs[f].hu_block * nstore * 3;
         const double* st = gT + (long)refs[f].ht_block * nstore * 3;
-        for (int t = lane; t < L; t += 64) {
-            int j = k0 + t;
-            if (j > jlim)
-                break;
-            long e = off[p] + t;
+        long e0 = off[p] - k0;
+        for (int j = k0; j <= jmax; ++j) {
+            long e = e0 + j;
             double M[9], G[6];
             for (int i = 0; i < 9; ++i)
                 M[i] = (double)mts[9 * e + i] + (mtsLo ? dev::Bf16ToDouble(mtsLo[9 * e + i]) : 0.0);

@@ -136,6 +136,9 @@ int tdbem_history_stats(tdbem_ctx* ctx, long* counts, long* npairs);
  *   over the history bands per step (the reference's per-step summation);
  *   "conv_tile" 16 (default) or 8 steps per far-part tile of the
  *   time-blocked convolution;
+ *   "conv_filtered" 1 (default) 16-step far tiles read a pre-filtered state
+ *   series and prefetch the next (pair, lags) item, 0 each item stages and
+ *   filters the raw states itself (same sums in the same order);
  *   "mixed" 0 (default) FP64 throughout; 1 or 2 mixed precision for the
  *   next tdbem_assemble_history and the solves that use it: blocks computed
  *   in FP64 and stored in FP32 (1) or as FP32 plus a bf16 residual (2, about

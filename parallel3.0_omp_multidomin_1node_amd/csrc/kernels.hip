@@ -2120,6 +2120,216 @@ __global__ void __launch_bounds__(256) k_conv_far(PairGrid grid, const NodeRefs*
     }
 }
 
+// Filtered state series of the tile starting at K0, per field element:
+// F[f][x] = (u~, t~), u~ = (gU_{x-1} + 2 gU_x) + gU_{x+1} over the solved
+// steps 1 .. K0-1 (other states count as zero), for x in [x0, x1].
+__global__ void __launch_bounds__(256) k_filter_tile(const NodeRefs* __restrict__ refs, const double* __restrict__ gU,
+                                                     const double* __restrict__ gT, int nstore, int ne, int K0,
+                                                     int x0, int x1, double* __restrict__ F)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int nx = x1 - x0 + 1;
+    if (i >= (long)ne * nx)
+        return;
+    int f = (int)(i / nx);
+    int x = x0 + (int)(i - (long)f * nx);
+    const double* su = gU + (long)refs[f].hu_block * nstore * 3;
+    const double* st = gT + (long)refs[f].ht_block * nstore * 3;
+    double v[6];
+    for (int c = 0; c < 3; ++c) {
+        double a = (x - 1 >= 1 && x - 1 <= K0 - 1) ? su[3 * (long)(x - 1) + c] : 0.0;
+        double b = (x >= 1 && x <= K0 - 1) ? su[3 * (long)x + c] : 0.0;
+        double d = (x + 1 >= 1 && x + 1 <= K0 - 1) ? su[3 * (long)(x + 1) + c] : 0.0;
+        v[c] = (a + 2.0 * b) + d;
+        a = (x - 1 >= 1 && x - 1 <= K0 - 1) ? st[3 * (long)(x - 1) + c] : 0.0;
+        b = (x >= 1 && x <= K0 - 1) ? st[3 * (long)x + c] : 0.0;
+        d = (x + 1 >= 1 && x + 1 <= K0 - 1) ? st[3 * (long)(x + 1) + c] : 0.0;
+        v[3 + c] = (a + 2.0 * b) + d;
+    }
+    double* o = F + ((long)f * (nstore + 1) + x) * 6;
+    for (int c = 0; c < 6; ++c)
+        o[c] = v[c];
+}
+
+void LaunchFilterTile(const NodeRefs* refs, const double* gU, const double* gT, int nstore, int ne, int K0, int x0,
+                      int x1, double* F, hipStream_t s)
+{
+    long n = (long)ne * (x1 - x0 + 1);
+    if (n > 0)
+        k_filter_tile<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(refs, gU, gT, nstore, ne, K0, x0, x1, F);
+}
+
+namespace dev {
+
+// One (pair, chunk of up to 64 lags) work item of the far convolution with
+// its band blocks and filtered-state rows held in registers, so the next
+// item's loads are in flight while the current one is computed.
+struct FarItem {
+    int q, t0, Lj, k0, nl, xlo, nx;
+    long base;
+    bool valid;
+    double M[9], G[6], r0[6], r1[6];
+};
+
+}  // namespace dev
+
+// The far part reading a pre-filtered state series (k_filter_tile) and
+// prefetching the next work item: rows of F for the chunk's x range go
+// straight into LDS, and each wavefront loads item n+1 while it computes
+// item n.
+template <int T, typename TB>
+__global__ void __launch_bounds__(256) k_conv_far_p(PairGrid grid, const NodeRefs* __restrict__ refs,
+                                                    const long* __restrict__ off, const int* __restrict__ kb,
+                                                    const int* __restrict__ len, const TB* __restrict__ mts,
+                                                    const TB* __restrict__ mgs,
+                                                    const unsigned short* __restrict__ mtsLo,
+                                                    const unsigned short* __restrict__ mgsLo,
+                                                    const double* __restrict__ F, int fstride, int K0, int Tn,
+                                                    int maxn, int ne, double* __restrict__ hfar)
+{
+    constexpr int kRows = 64 + T;
+    __shared__ double fwin[4][kRows * 6];
+    int s = blockIdx.x;
+    int d = refs[s].domain;
+    int nd = grid.eleCount[d];
+    int eb = grid.eleBegin[d];
+    long rowStart = grid.pairBase[d] + (long)(s - eb) * nd;
+    int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    int lane = threadIdx.x & 63;
+    int Kmax = K0 + Tn - 1;
+    int jmax = Kmax < maxn ? Kmax : maxn;
+    double* fw = fwin[wave];
+    double acc[T][3];
+#pragma unroll
+    for (int i = 0; i < T; ++i)
+        acc[i][0] = acc[i][1] = acc[i][2] = 0.0;
+    // next item after (q, t0) of this wave: the next 64 lags of the pair, or
+    // the first chunk of the next pair with lags <= jmax
+    auto advance = [&](dev::FarItem& it, int q, int t0, int Lj, int k0, long base) {
+        t0 += 64;
+        while (q < nd && t0 >= Lj) {
+            q += 4;
+            t0 = 0;
+            if (q < nd) {
+                long p = rowStart + q;
+                int L = len[p];
+                k0 = kb[p];
+                base = off[p];
+                Lj = jmax - k0 + 1 < L ? jmax - k0 + 1 : L;
+            }
+        }
+        it.valid = q < nd;
+        it.q = q;
+        it.t0 = t0;
+        it.Lj = Lj;
+        it.k0 = k0;
+        it.base = base;
+        if (!it.valid)
+            return;
+        it.nl = Lj - t0 < 64 ? Lj - t0 : 64;
+        it.xlo = K0 - (k0 + t0 + it.nl - 1);
+        it.nx = Kmax - (k0 + t0) - it.xlo + 1;
+        bool own = lane < it.nl;
+        long e = base + (own ? t0 + lane : 0);
+#pragma unroll
+        for (int i = 0; i < 9; ++i)
+            it.M[i] = (double)mts[9 * e + i];
+#pragma unroll
+        for (int i = 0; i < 6; ++i)
+            it.G[i] = (double)mgs[6 * e + i];
+        if (mtsLo) {
+#pragma unroll
+            for (int i = 0; i < 9; ++i)
+                it.M[i] += dev::Bf16ToDouble(mtsLo[9 * e + i]);
+#pragma unroll
+            for (int i = 0; i < 6; ++i)
+                it.G[i] += dev::Bf16ToDouble(mgsLo[6 * e + i]);
+        }
+        const double* Fb = F + (long)(eb + q) * fstride * 6;
+        int x = it.xlo + lane;
+        bool in0 = lane < it.nx && x >= 0 && x <= K0;
+        const double* a = Fb + 6 * (long)(in0 ? x : 0);
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+            it.r0[c] = in0 ? a[c] : 0.0;
+        x += 64;
+        bool in1 = lane + 64 < it.nx && x >= 0 && x <= K0;
+        const double* b = Fb + 6 * (long)(in1 ? x : 0);
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+            it.r1[c] = in1 ? b[c] : 0.0;
+    };
+    auto body = [&](dev::FarItem& cur, dev::FarItem& nxt) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+            fw[6 * lane + c] = cur.r0[c];
+        if (lane + 64 < cur.nx)
+#pragma unroll
+            for (int c = 0; c < 6; ++c)
+                fw[6 * (lane + 64) + c] = cur.r1[c];
+        dev::WaveSync();
+        advance(nxt, cur.q, cur.t0, cur.Lj, cur.k0, cur.base);
+        if (lane < cur.nl) {
+            int x0 = K0 - (cur.k0 + cur.t0 + lane) - cur.xlo;
+            const double* M = cur.M;
+            const double* G = cur.G;
+#pragma unroll
+            for (int i = 0; i < T; ++i) {
+                const double* w = fw + 6 * (x0 + i);
+                double u0 = w[0], u1 = w[1], u2 = w[2], t0v = w[3], t1v = w[4], t2v = w[5];
+                double a0 = acc[i][0], a1 = acc[i][1], a2 = acc[i][2];
+                a0 = fma(G[0], t0v, a0); a0 = fma(G[1], t1v, a0); a0 = fma(G[2], t2v, a0);
+                a1 = fma(G[1], t0v, a1); a1 = fma(G[3], t1v, a1); a1 = fma(G[4], t2v, a1);
+                a2 = fma(G[2], t0v, a2); a2 = fma(G[4], t1v, a2); a2 = fma(G[5], t2v, a2);
+                a0 = fma(-M[0], u0, a0); a0 = fma(-M[1], u1, a0); a0 = fma(-M[2], u2, a0);
+                a1 = fma(-M[3], u0, a1); a1 = fma(-M[4], u1, a1); a1 = fma(-M[5], u2, a1);
+                a2 = fma(-M[6], u0, a2); a2 = fma(-M[7], u1, a2); a2 = fma(-M[8], u2, a2);
+                acc[i][0] = a0;
+                acc[i][1] = a1;
+                acc[i][2] = a2;
+            }
+        }
+        dev::WaveSync();
+    };
+    dev::FarItem A, B;
+    advance(A, wave - 4, 0, 0, 0, 0);
+    while (A.valid) {
+        body(A, B);
+        if (!B.valid)
+            break;
+        body(B, A);
+    }
+    __shared__ double red[4][T * 3];
+#pragma unroll
+    for (int i = 0; i < T; ++i)
+        for (int c = 0; c < 3; ++c) {
+            double v = dev::WaveSum(acc[i][c]);
+            if (lane == 0)
+                red[wave][i * 3 + c] = v;
+        }
+    __syncthreads();
+    for (int x = threadIdx.x; x < Tn * 3; x += blockDim.x) {
+        double v = ((red[0][x] + red[1][x]) + red[2][x]) + red[3][x];
+        int i = x / 3, c = x % 3;
+        hfar[(long)i * 3 * ne + 3 * (long)s + c] = v;
+    }
+}
+
+template <typename TB>
+void LaunchConvFarF(PairGrid grid, const NodeRefs* refs, const long* off, const int* kb, const int* len,
+                    const TB* mts, const TB* mgs, const unsigned short* mtsLo, const unsigned short* mgsLo,
+                    const double* F, int fstride, int K0, int Tn, int maxn, int ne, double* hfar, hipStream_t s)
+{
+    k_conv_far_p<16, TB><<<ne, 256, 0, s>>>(grid, refs, off, kb, len, mts, mgs, mtsLo, mgsLo, F, fstride, K0, Tn,
+                                            maxn, ne, hfar);
+}
+template void LaunchConvFarF<double>(PairGrid, const NodeRefs*, const long*, const int*, const int*, const double*,
+                                     const double*, const unsigned short*, const unsigned short*, const double*, int,
+                                     int, int, int, int, double*, hipStream_t);
+template void LaunchConvFarF<float>(PairGrid, const NodeRefs*, const long*, const int*, const int*, const float*,
+                                    const float*, const unsigned short*, const unsigned short*, const double*, int,
+                                    int, int, int, int, double*, hipStream_t);
+
 // Near part of step K in the tile starting at K0 plus the far part:
 // rhs_K += hfar[K-K0] + sum over near pairs (band start <= K-K0+1) of the
 // taps with K0 <= m <= K-1.  One wavefront per source row, one near pair per

@@ -122,6 +122,16 @@ void LaunchConvolution(PairGrid grid, const NodeRefs* refs, const long* off, con
 // into rhs.
 constexpr int kConvTileMax = 16;
 // mtsLo / mgsLo (nullable): bf16 residuals of the split FP32 storage.
+// Filtered state series of the tile starting at K0 for x in [x0, x1]
+// (F[f][x][6], x = 0 .. nstore).
+void LaunchFilterTile(const NodeRefs* refs, const double* gU, const double* gT, int nstore, int ne, int K0, int x0,
+                      int x1, double* F, hipStream_t s);
+// Far part of a 16-step tile from the filtered series, next work item
+// prefetched.
+template <typename TB>
+void LaunchConvFarF(PairGrid grid, const NodeRefs* refs, const long* off, const int* kb, const int* len,
+                    const TB* mts, const TB* mgs, const unsigned short* mtsLo, const unsigned short* mgsLo,
+                    const double* F, int fstride, int K0, int Tn, int maxn, int ne, double* hfar, hipStream_t s);
 template <typename TB>
 void LaunchConvFar(PairGrid grid, const NodeRefs* refs, const long* off, const int* kb, const int* len, const TB* mts,
                    const TB* mgs, const unsigned short* mtsLo, const unsigned short* mgsLo, const double* gU,

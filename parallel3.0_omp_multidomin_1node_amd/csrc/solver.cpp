@@ -880,6 +880,12 @@ void Case::Solve(int nstep, const double* known_lu, long restart, double tol, do
     d_rhs.alloc((size_t)n3);
     d_x.alloc((size_t)n3);
     d_hfar.alloc((size_t)kConvTileMax * (size_t)n3);
+    DevBuf<double> d_F;
+    const bool farF = convTiled && convFiltered && convTile == 16;
+    if (farF) {
+        d_F.alloc((size_t)ne * (size_t)(nstore + 1) * 6);
+        LaunchZero(d_F.p, (long)d_F.n, stream);
+    }
     LaunchZero(d_gU.p, (long)d_gU.n, stream);
     LaunchZero(d_gT.p, (long)d_gT.n, stream);
     LaunchZero(d_known.p, (long)d_known.n, stream);
@@ -905,7 +911,19 @@ void Case::Solve(int nstep, const double* known_lu, long restart, double tol, do
                     K0 = K;
                     int Tn = std::min(convTile, nstep - K0 + 1);
                     Check(hipEventRecord(f0, stream), "event");
-                    if (historyMixed)
+                    if (farF) {
+                        LaunchFilterTile(d_refs.p, d_gU.p, d_gT.p, nstore, (int)ne, K0, std::max(0, K0 - convTile - 1),
+                                         K0, d_F.p, stream);
+                        if (historyMixed)
+                            LaunchConvFarF(grid, d_refs.p, d_off.p, d_kb.p, d_len.p, (const float*)d_mts32.p,
+                                           (const float*)d_mgs32.p, (const unsigned short*)d_mtsLo.p,
+                                           (const unsigned short*)d_mgsLo.p, d_F.p, nstore + 1, K0, Tn, maxn,
+                                           (int)ne, d_hfar.p, stream);
+                        else
+                            LaunchConvFarF(grid, d_refs.p, d_off.p, d_kb.p, d_len.p, (const double*)d_mts.p,
+                                           (const double*)d_mgs.p, nullptr, nullptr, d_F.p, nstore + 1, K0, Tn, maxn,
+                                           (int)ne, d_hfar.p, stream);
+                    } else if (historyMixed)
                         LaunchConvFar(grid, d_refs.p, d_off.p, d_kb.p, d_len.p, (const float*)d_mts32.p,
                                       (const float*)d_mgs32.p, (const unsigned short*)d_mtsLo.p,
                                       (const unsigned short*)d_mgsLo.p, d_gU.p, d_gT.p, nstore, K0, Tn, convTile,

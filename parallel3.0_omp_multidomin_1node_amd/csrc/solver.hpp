@@ -108,6 +108,7 @@ public:
     double timing[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     bool convTiled = true;   // time-blocked convolution (false: one full pass per step)
     int convTile = 16;       // steps per far-part tile (8 or 16); fixes the near lists built by the cull
+    bool convFiltered = true;  // 16-step far tiles read a pre-filtered state series and prefetch work items
     int selfMode = -1;       // kernel of the current self tables: 1 wave per element, 0 reference order
     bool quadStaged = true;  // wave-per-pair quadrature with LDS-staged point geometry (false: lane per entry)
     int mixed = 0;  // for the next history assembly: 0 FP64, 1 FP32 blocks, 2 FP32 + bf16 residual blocks

@@ -236,7 +236,8 @@ def test_full_size_properties():
 def test_tiled_convolution_matches_per_step_sum(oracle_mod):
     """The time-blocked convolution (far part per 8- or 16-step tile + near
     part per step) reassociates the reference's per-step history sum; both give
-    the same solution to rounding."""
+    the same solution to rounding.  The two 16-step far kernels agree bit for
+    bit."""
     tdbem = _tdbem()
     from tdbem import mesh
     w = mesh.layered_config2(h=0.5, nstep=45)
@@ -249,6 +250,13 @@ def test_tiled_convolution_matches_per_step_sum(oracle_mod):
     for tile in (16, 8):
         gc.set_option("conv_tile", tile)
         tiled[tile] = gc.solve(w.known, w.nstep, tol=1e-13)[:2]
+    # 16-step tiles: the pre-filtered, prefetching far kernel (default) and the
+    # one that stages and filters raw states add the same terms in the same order
+    gc.set_option("conv_tile", 16)
+    gc.set_option("conv_filtered", 0)
+    Ur, Tr = gc.solve(w.known, w.nstep, tol=1e-13)[:2]
+    assert np.array_equal(Ur, tiled[16][0]) and np.array_equal(Tr, tiled[16][1])
+    gc.set_option("conv_filtered", 1)
     gc.set_option("conv_tiled", 0)
     U0, T0, *_ = gc.solve(w.known, w.nstep, tol=1e-13)
     for tile, (U1, T1) in tiled.items():

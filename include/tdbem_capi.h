@@ -156,6 +156,41 @@ int tdbem_history_stats(tdbem_ctx* ctx, long* counts, long* npairs);
  *   band entry in the reference's operation order (bit-identical to it). */
 int tdbem_set_option(tdbem_ctx* ctx, const char* name, double value);
 
+/* Reference input files (host only, no device work).  binary_mode follows the
+ * datacard's InputBinaryMode (DBEM1/DBEM.cpp:686-700,828-848,1023-1040):
+ * 0 the text file at path, 1 the DBEMBIN file path + ".bin" (required),
+ * 2 the DBEMBIN file when it exists, else the text file.
+ *   tdbem_model_counts / tdbem_read_model: the .model mesh (text:
+ *   DBEM1/DBEM.cpp:849-900, 1-based ids; binary: ReadModelBinary,
+ *   DBEM1/DBEM.cpp:133-225); ele_pid[8 ne] and inf_pid[4 ninf] (may be null)
+ *   come back 0-based;
+ *   tdbem_read_bd: BCIDs and (nstep + 1) x ne triplets of the .bd file (text:
+ *   DBEM1/DBEM.cpp:1010-1085; binary: ReadBoundaryBinary, DBEM1/DBEM.cpp:
+ *   227-291, whose header must carry the same ne and nstep); expected_bcid
+ *   (may be null) must match, as for the reference's batch boundary files. */
+int tdbem_model_counts(const char* path, int binary_mode, long* npoints, long* ne, long* ninf);
+int tdbem_read_model(const char* path, int binary_mode, double* points, int* ele_pid, long* inf_pid);
+int tdbem_read_bd(const char* path, int binary_mode, long ne, int nstep, const int* expected_bcid, int* bcid,
+                  double* lu);
+
+/* DBEMBIN v1 writers: the layout convert_inputs_to_binary.py writes and
+ * DBEM1/DBEM.cpp:89-330 reads (magic "DBEMBIN\0", version 1, kind 1 model /
+ * 2 boundary, endian mark 0x01020304, reserved 0; little-endian int64 counts,
+ * float64 points, int64 0-based ids; int32 BCIDs and float64 triplets). */
+int tdbem_write_model_bin(const char* path, long npoints, const double* points, long ne, const int* ele_pid,
+                          long ninf, const long* inf_pid);
+int tdbem_write_bd_bin(const char* path, long ne, int nstep, const int* bcid, const double* lu);
+
+/* validation_state.csv and validation_metrics.txt into directory dir
+ * (WriteValidationOutputs, DBEM1/DBEM.cpp:434-561, and
+ * WriteMultiDomainValidationMetrics, DBEM1/multidomain.cpp:388-416) for a
+ * solve of this context: lu, lt [(nstep+1)][ne][3] in the global frame as
+ * tdbem_solve returns them, its iters / flags and solved count, the
+ * datacard's solver number.  The average MTS / MGS block counts are those of
+ * the assembled CCSR history matrices (the bands are copied to the host). */
+int tdbem_write_validation(tdbem_ctx* ctx, const char* dir, int nstep, const double* lu, const double* lt,
+                           const int* iters, const int* flags, int solved, int solver);
+
 /* Wait for all device work of this process (hipDeviceSynchronize). */
 int tdbem_device_synchronize(void);
 

@@ -1,6 +1,7 @@
 // extern "C" boundary of the MI355X TDBEM path (declared in include/tdbem_capi.h).
 #include "../../include/tdbem_capi.h"
 
+#include "io.hpp"
 #include "solver.hpp"
 
 #include <cstring>
@@ -335,6 +336,131 @@ int tdbem_set_option(tdbem_ctx* ctx, const char* name, double value)
         return 0;
     }
     return Fail("tdbem_set_option: unknown option");
+}
+
+int tdbem_model_counts(const char* path, int binary_mode, long* npoints, long* ne, long* ninf)
+{
+    if (!path || !npoints || !ne || !ninf)
+        return Fail("tdbem_model_counts: invalid arguments");
+    try {
+        tdbem::ModelData m = tdbem::ReadModelFile(path, binary_mode);
+        *npoints = (long)(m.points.size() / 3);
+        *ne = (long)(m.elePid.size() / 8);
+        *ninf = (long)(m.infPid.size() / 4);
+    } catch (const std::exception& e) {
+        return Fail("tdbem_model_counts", e);
+    }
+    return 0;
+}
+
+int tdbem_read_model(const char* path, int binary_mode, double* points, int* ele_pid, long* inf_pid)
+{
+    if (!path || !points || !ele_pid)
+        return Fail("tdbem_read_model: invalid arguments");
+    try {
+        tdbem::ModelData m = tdbem::ReadModelFile(path, binary_mode);
+        std::memcpy(points, m.points.data(), sizeof(double) * m.points.size());
+        for (size_t i = 0; i < m.elePid.size(); ++i)
+            ele_pid[i] = (int)m.elePid[i];
+        if (inf_pid)
+            for (size_t i = 0; i < m.infPid.size(); ++i)
+                inf_pid[i] = m.infPid[i];
+    } catch (const std::exception& e) {
+        return Fail("tdbem_read_model", e);
+    }
+    return 0;
+}
+
+int tdbem_read_bd(const char* path, int binary_mode, long ne, int nstep, const int* expected_bcid, int* bcid,
+                  double* lu)
+{
+    if (!path || !bcid || !lu || ne <= 0 || nstep < 0)
+        return Fail("tdbem_read_bd: invalid arguments");
+    try {
+        tdbem::ReadBdFile(path, binary_mode, ne, nstep, expected_bcid, bcid, lu);
+    } catch (const std::exception& e) {
+        return Fail("tdbem_read_bd", e);
+    }
+    return 0;
+}
+
+int tdbem_write_model_bin(const char* path, long npoints, const double* points, long ne, const int* ele_pid,
+                          long ninf, const long* inf_pid)
+{
+    if (!path || !points || !ele_pid || npoints <= 0 || ne <= 0 || ninf < 0 || (ninf > 0 && !inf_pid))
+        return Fail("tdbem_write_model_bin: invalid arguments");
+    try {
+        std::vector<long> ids(ele_pid, ele_pid + 8 * (size_t)ne);
+        tdbem::WriteModelBin(path, npoints, points, ne, ids.data(), ninf, inf_pid);
+    } catch (const std::exception& e) {
+        return Fail("tdbem_write_model_bin", e);
+    }
+    return 0;
+}
+
+int tdbem_write_bd_bin(const char* path, long ne, int nstep, const int* bcid, const double* lu)
+{
+    if (!path || !bcid || !lu || ne <= 0 || nstep < 0)
+        return Fail("tdbem_write_bd_bin: invalid arguments");
+    try {
+        tdbem::WriteBdBin(path, ne, nstep, bcid, lu);
+    } catch (const std::exception& e) {
+        return Fail("tdbem_write_bd_bin", e);
+    }
+    return 0;
+}
+
+int tdbem_write_validation(tdbem_ctx* ctx, const char* dir, int nstep, const double* lu, const double* lt,
+                           const int* iters, const int* flags, int solved, int solver)
+{
+    if (!ctx || !dir || !lu || !lt || nstep < 0 || solved < 0 || solved > nstep || (solved > 0 && (!iters || !flags)))
+        return Fail("tdbem_write_validation: invalid arguments");
+    try {
+        tdbem::Case& c = ctx->c;
+        tdbem::ValidationInput in;
+        in.dir = dir;
+        in.ne = c.ne;
+        in.nstep = nstep;
+        in.solved = solved;
+        in.dt = c.dt;
+        in.solver = solver;
+        in.lu = lu;
+        in.lt = lt;
+        in.iters = iters;
+        in.flags = flags;
+        std::vector<double> nodes(3 * (size_t)c.ne);
+        for (long e = 0; e < c.ne; ++e)
+            for (int k = 0; k < 3; ++k)
+                nodes[3 * (size_t)e + k] = c.geo[(size_t)e].center[k];
+        in.nodes = nodes.data();
+        in.domainId.resize((size_t)c.ne);
+        for (long e = 0; e < c.ne; ++e)
+            in.domainId[(size_t)e] = c.model.domains[(size_t)c.domOf[(size_t)e]].id;
+        in.surface = c.surface;
+        in.bcid = c.bcid;
+        in.model = &c.model;
+        in.dof = &c.dof;
+        in.A = &c.A;
+        in.assemblyTime = c.timing[1] + c.timing[2] + c.timing[3];
+        in.gmresTime = c.timing[5];
+        if (c.HistoryReady() && c.maxn > 0) {
+            double mts = 0.0, mgs = 0.0;
+            std::vector<int> rp, col;
+            std::vector<double> val;
+            for (int k = 1; k <= c.maxn; ++k) {
+                c.ExportMatrix(2, k, rp, col, val);
+                mts += (double)col.size();
+                c.ExportMatrix(3, k, rp, col, val);
+                mgs += (double)col.size();
+            }
+            in.avgMTSBlocks = mts / c.maxn;
+            in.avgMGSBlocks = mgs / c.maxn;
+        }
+        tdbem::WriteValidationFiles(in);
+    } catch (const std::exception& e) {
+        return Fail("tdbem_write_validation", e);
+    }
+    return 0;
 }
 
 int tdbem_device_synchronize(void)

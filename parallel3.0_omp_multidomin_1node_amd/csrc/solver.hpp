@@ -88,6 +88,7 @@ public:
     void EnsureSelfTables();
     void SetMaterials(const double* evr);
     void ExportMatrix(int which, int step, std::vector<int>& rp, std::vector<int>& col, std::vector<double>& val);
+    bool HistoryReady() const { return historyDone; }
 
     long ne = 0;
     double dt = 0.0;

@@ -25,7 +25,8 @@ EXPORTED = [
     "tdbem_assemble_history", "tdbem_ccsr_export", "tdbem_self_tables", "tdbem_solve", "tdbem_solve_batched",
     "tdbem_adjoint", "tdbem_get_materials", "tdbem_set_materials", "tdbem_dyna_gmres_solver",
     "tdbem_timings", "tdbem_timings_n", "tdbem_history_stats", "tdbem_device_synchronize",
-    "tdbem_set_option",
+    "tdbem_set_option", "tdbem_model_counts", "tdbem_read_model", "tdbem_read_bd", "tdbem_write_model_bin",
+    "tdbem_write_bd_bin", "tdbem_write_validation",
 ]
 
 
@@ -69,6 +70,13 @@ def lib():
         L.tdbem_history_stats.argtypes = [C.c_void_p, _lp, C.POINTER(C.c_long)]
         L.tdbem_device_synchronize.argtypes = []
         L.tdbem_set_option.argtypes = [C.c_void_p, C.c_char_p, C.c_double]
+        L.tdbem_model_counts.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_long), C.POINTER(C.c_long),
+                                         C.POINTER(C.c_long)]
+        L.tdbem_read_model.argtypes = [C.c_char_p, C.c_int, _dp, _ip, C.c_void_p]
+        L.tdbem_read_bd.argtypes = [C.c_char_p, C.c_int, C.c_long, C.c_int, C.c_void_p, _ip, _dp]
+        L.tdbem_write_model_bin.argtypes = [C.c_char_p, C.c_long, _dp, C.c_long, _ip, C.c_long, C.c_void_p]
+        L.tdbem_write_bd_bin.argtypes = [C.c_char_p, C.c_long, C.c_int, _ip, _dp]
+        L.tdbem_write_validation.argtypes = [C.c_void_p, C.c_char_p, C.c_int, _dp, _dp, _ip, _ip, C.c_int, C.c_int]
         _lib = L
     return _lib
 
@@ -225,6 +233,19 @@ class Case:
         npairs = C.c_long()
         _check(lib().tdbem_history_stats(self.h, counts, C.byref(npairs)), "tdbem_history_stats")
         return counts, npairs.value
+
+    def write_validation(self, directory, U, T, iters, flags, solved, solver=2):
+        """validation_state.csv and validation_metrics.txt of a solve (U, T as
+        solve() returns them) into directory."""
+        nstep = len(iters)
+        U = np.ascontiguousarray(U, np.float64).reshape(-1)
+        T = np.ascontiguousarray(T, np.float64).reshape(-1)
+        if U.size != (nstep + 1) * self.ne * 3 or T.size != U.size:
+            raise ValueError("U / T must be (nstep+1, ne, 3)")
+        _check(lib().tdbem_write_validation(self.h, str(directory).encode(), nstep, U, T,
+                                            np.ascontiguousarray(iters, np.int32),
+                                            np.ascontiguousarray(flags, np.int32), int(solved), int(solver)),
+               "tdbem_write_validation")
 
     def timings(self):
         t = np.zeros(9)

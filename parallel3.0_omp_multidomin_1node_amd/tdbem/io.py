@@ -31,6 +31,7 @@ class DataCard:
     iterations: int = 200
     error: float = 1e-5
     premaxleaf: int = 10
+    input_binary_mode: int = 0
     domain_count: int = 0
     materials: List[Tuple[int, float, float, float]] = field(default_factory=list)
     interfaces: Optional[List[Tuple[int, int, int, int, int]]] = None
@@ -61,7 +62,8 @@ def read_datacard(path: Path) -> DataCard:
     in_mode = int(nxt())
     batch = int(nxt())
     if in_mode not in (0, 1, 2) or batch not in (0,):
-        raise ValueError("binary / batch boundary inputs are not supported")
+        raise ValueError("batch boundary inputs are not supported")
+    card.input_binary_mode = in_mode
     dc = nxt()
     if dc is None:
         return card
@@ -164,3 +166,62 @@ def reference_dt(points: np.ndarray, ele: np.ndarray, beta: float, E: float, v: 
     G = E / 2.0 / (1 + v)
     c1 = math.sqrt((2.0 - 2.0 * v) / (1.0 - 2.0 * v) * G / rho)
     return beta * ave / c1
+
+
+# ---------------------------------------------------------------------------
+# Text or DBEMBIN inputs through the C-ABI readers (mode 0 text, 1 binary
+# "<path>.bin" required, 2 binary when it exists; DBEM1/DBEM.cpp:828-848,
+# 1023-1040) and the DBEMBIN writers (the layout of the reference's
+# convert_inputs_to_binary.py).  These touch no device.
+# ---------------------------------------------------------------------------
+def _lib():
+    from . import lib
+    return lib()
+
+
+def _ck(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"{what} failed ({rc}): {_lib().tdbem_last_error().decode()}")
+
+
+def load_model(path, mode: int = 0):
+    """points (np,3), element ids (ne,8) 0-based int32, infinite-element ids
+    (ninf,4) 0-based int64."""
+    import ctypes as C
+    L = _lib()
+    n_p, n_e, n_i = C.c_long(), C.c_long(), C.c_long()
+    _ck(L.tdbem_model_counts(str(path).encode(), mode, C.byref(n_p), C.byref(n_e), C.byref(n_i)),
+        "tdbem_model_counts")
+    pts = np.zeros(3 * n_p.value)
+    ele = np.zeros(8 * n_e.value, np.int32)
+    inf = np.zeros(max(1, 4 * n_i.value), np.int64)
+    _ck(L.tdbem_read_model(str(path).encode(), mode, pts, ele, inf.ctypes.data), "tdbem_read_model")
+    return pts.reshape(-1, 3), ele.reshape(-1, 8), inf[:4 * n_i.value].reshape(-1, 4)
+
+
+def load_bd(path, ne: int, nstep: int, mode: int = 0, expected_bcid=None):
+    """bcid (ne,) int32 and given triplets (nstep+1, ne, 3)."""
+    L = _lib()
+    bcid = np.zeros(ne, np.int32)
+    lu = np.zeros((nstep + 1) * ne * 3)
+    exp = None if expected_bcid is None else np.ascontiguousarray(expected_bcid, np.int32)
+    _ck(L.tdbem_read_bd(str(path).encode(), mode, ne, nstep, None if exp is None else exp.ctypes.data, bcid, lu),
+        "tdbem_read_bd")
+    return bcid, lu.reshape(nstep + 1, ne, 3)
+
+
+def write_model_bin(path, points, ele, inf=None):
+    L = _lib()
+    pts = np.ascontiguousarray(points, np.float64).reshape(-1)
+    e = np.ascontiguousarray(ele, np.int32).reshape(-1)
+    i = np.zeros(0, np.int64) if inf is None else np.ascontiguousarray(inf, np.int64).reshape(-1)
+    _ck(L.tdbem_write_model_bin(str(path).encode(), pts.size // 3, pts, e.size // 8, e, i.size // 4,
+                                i.ctypes.data if i.size else None), "tdbem_write_model_bin")
+
+
+def write_bd_bin(path, bcid, lu):
+    L = _lib()
+    b = np.ascontiguousarray(bcid, np.int32)
+    v = np.ascontiguousarray(lu, np.float64)
+    nstep = v.size // (3 * b.size) - 1
+    _ck(L.tdbem_write_bd_bin(str(path).encode(), b.size, nstep, b, v.reshape(-1)), "tdbem_write_bd_bin")

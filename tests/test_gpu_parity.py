@@ -515,3 +515,67 @@ def test_autograd_function_returns_the_adjoint_gradient(tmp_path):
     r = json.loads(out.stdout.strip().splitlines()[-1])
     assert r["J"] == r["J2"]
     assert np.allclose(np.array(r["grad"]), 2.0 * np.array(r["g2"]), rtol=1e-12, atol=0.0)
+
+
+def test_validation_outputs_of_a_solve(tmp_path):
+    """validation_state.csv / validation_metrics.txt in the reference's layout
+    (DBEM1/DBEM.cpp:434-561, DBEM1/multidomain.cpp:388-416): the CSV holds the
+    solved global-frame displacements and tractions exactly (%.17g), and the
+    metrics agree with what this test recomputes from the solution, the
+    exported matrices and the materials."""
+    tdbem = _tdbem()
+    from tdbem import mesh
+    w = mesh.layered_config2(h=0.5, nstep=20)
+    m = w.mesh
+    print("case:", w.name, "validation outputs", flush=True)
+    gc = tdbem.Case(m.points, m.ele, m.bcid, w.dt, w.gap, domain_count=4, mat_rows=w.mats, itf_rows=m.interfaces)
+    gc.assemble_step0(10)
+    gc.assemble_history(w.nstep)
+    U, T, it, fl, solved, _ = gc.solve(w.known, w.nstep, tol=1e-10)
+    gc.write_validation(tmp_path, U, T, it, fl, solved)
+    A = gc.ccsr(0)
+    mts = [len(gc.ccsr(2, k)[1]) for k in range(1, w.nstep + 1)]
+    mgs = [len(gc.ccsr(3, k)[1]) for k in range(1, w.nstep + 1)]
+    gc.close()
+    ne = len(m.ele)
+    lines = (tmp_path / "validation_state.csv").read_text().splitlines()
+    assert lines[0] == "step,element,localNode,nodeId,domain,surfaceType,bcid,x,y,z,ux,uy,uz,tx,ty,tz"
+    rows = np.array([[float(x) for x in ln.split(",")] for ln in lines[1:]])
+    assert rows.shape == ((w.nstep + 1) * ne, 16)
+    assert np.array_equal(rows[:, 0], np.repeat(np.arange(w.nstep + 1), ne))
+    assert np.array_equal(rows[:, 1], np.tile(np.arange(ne), w.nstep + 1)) and np.array_equal(rows[:, 1], rows[:, 3])
+    assert np.array_equal(rows[:, 10:13], U.reshape(-1, 3)) and np.array_equal(rows[:, 13:16], T.reshape(-1, 3))
+    assert np.array_equal(rows[:ne, 6], m.bcid)
+    itf = np.asarray(m.interfaces)
+    surf = np.zeros(ne)
+    surf[itf[:, 1] - 1] = 1
+    surf[itf[:, 3] - 1] = 1
+    assert np.array_equal(rows[:ne, 5], surf)
+    assert np.array_equal(rows[:ne, 4], np.repeat(np.arange(4), ne // 4))
+    kv = dict(ln.split("=", 1) for ln in (tmp_path / "validation_metrics.txt").read_text().splitlines())
+    assert kv["NStep"] == str(w.nstep) and kv["ElementCount"] == str(ne) and kv["DomainCount"] == "4"
+    assert float(kv["Dt"]) == w.dt and kv["InterfaceCount"] == str(len(itf))
+    for d in range(4):
+        E, v, rho = w.mats[d, 1:]
+        G = E / 2.0 / (1.0 + v)
+        c1 = np.sqrt((2.0 - 2.0 * v) / (1.0 - 2.0 * v) * G / rho)
+        vals = kv["DomainMaterial[%d]" % d].split()
+        assert abs(float(vals[1]) - c1) <= 1e-15 * c1 and abs(float(vals[5]) - c1 * w.dt) <= 1e-15 * c1 * w.dt
+    ea, eb, sgn = itf[:, 1] - 1, itf[:, 3] - 1, itf[:, 4]
+    for k in (0, 7, w.nstep):
+        du = np.abs(U[k, ea] - U[k, eb]).max()
+        tb = np.abs(T[k, eb] - sgn[:, None] * T[k, ea]).max()
+        assert float(kv["InterfaceStep[%d].MaxAbsU" % k]) == du
+        assert float(kv["InterfaceStep[%d].MaxAbsTBalance" % k]) == tb
+    assert kv["MultiDomainMetricsValid"] == "1" and kv["MultiDomainFinalFlag"] == "0"
+    assert kv["MultiDomainSolvedSteps"] == str(w.nstep)
+    assert kv["MultiDomainTotalInnerIterations"] == str(int(np.sum(it)))
+    assert kv["MultiDomainTotalOuterIterations"] == str(int(np.sum(np.asarray(it) > 0)))
+    assert kv["MultiDomainMTS0NonZeroBlocks"] == str(len(A[1]))
+    assert abs(float(kv["MultiDomainMTSAverageBlocks"]) - np.mean(mts)) <= 1e-6
+    assert abs(float(kv["MultiDomainMGSAverageBlocks"]) - np.mean(mgs)) <= 1e-6
+    rp, col = A[0], A[1]
+    diag = sum(int(np.sum(col[rp[r]:rp[r + 1]] == r)) for r in range(len(rp) - 1))
+    assert kv["MultiDomainMatrixStructureValid"] == "1" and kv["MultiDomainMatrixZeroRows"] == "0"
+    assert kv["MultiDomainMatrixDiagonalBlocks"] == str(diag)
+    assert kv["MultiDomainInterfaceDofInvariantValid"] == "1"

@@ -191,6 +191,19 @@ int tdbem_write_bd_bin(const char* path, long ne, int nstep, const int* bcid, co
 int tdbem_write_validation(tdbem_ctx* ctx, const char* dir, int nstep, const double* lu, const double* lt,
                            const int* iters, const int* flags, int solved, int solver);
 
+/* History-matrix persistence (the analogue of the reference's CCSR matrix
+ * cache, DBEM1/pre_gmres.cpp:16-27,199-544, and of its in-process reuse
+ * across load cases): tdbem_history_save writes the assembled history bands
+ * (per pair: first lag and band length; per entry: the MTS / MGS blocks in the
+ * storage precision and the presence bits) with the element / pair counts,
+ * dt, gap and per-domain materials; tdbem_history_load checks those against
+ * the context, re-runs the causality cull and requires the same bands, then
+ * uploads the blocks, so tdbem_solve / tdbem_solve_batched run without a
+ * history assembly (*maxn, may be null, receives the stored step count).
+ * Within a process every solve of a context already reuses its bands. */
+int tdbem_history_save(tdbem_ctx* ctx, const char* path);
+int tdbem_history_load(tdbem_ctx* ctx, const char* path, int* maxn);
+
 /* Wait for all device work of this process (hipDeviceSynchronize). */
 int tdbem_device_synchronize(void);
 

@@ -463,6 +463,32 @@ int tdbem_write_validation(tdbem_ctx* ctx, const char* dir, int nstep, const dou
     return 0;
 }
 
+int tdbem_history_save(tdbem_ctx* ctx, const char* path)
+{
+    if (!ctx || !path)
+        return Fail("tdbem_history_save: invalid arguments");
+    try {
+        ctx->c.SaveHistory(path);
+    } catch (const std::exception& e) {
+        return Fail("tdbem_history_save", e);
+    }
+    return 0;
+}
+
+int tdbem_history_load(tdbem_ctx* ctx, const char* path, int* maxn)
+{
+    if (!ctx || !path)
+        return Fail("tdbem_history_load: invalid arguments");
+    try {
+        ctx->c.LoadHistory(path);
+        if (maxn)
+            *maxn = ctx->c.maxn;
+    } catch (const std::exception& e) {
+        return Fail("tdbem_history_load", e);
+    }
+    return 0;
+}
+
 int tdbem_device_synchronize(void)
 {
     hipError_t e = hipDeviceSynchronize();

@@ -65,6 +65,8 @@ template struct DevBuf<ElemPoints>;
 template struct DevBuf<NodeRefs>;
 template struct DevBuf<QuadTables>;
 template struct DevBuf<PwWave>;
+template struct DevBuf<float>;
+template struct DevBuf<unsigned short>;
 
 Case::~Case()
 {

@@ -89,6 +89,11 @@ public:
     void SetMaterials(const double* evr);
     void ExportMatrix(int which, int step, std::vector<int>& rp, std::vector<int>& col, std::vector<double>& val);
     bool HistoryReady() const { return historyDone; }
+    // Assembled history bands to a file / from a file written for the same
+    // model (same cull bands, materials, dt and gap); the loaded history
+    // replaces any assembled one.
+    void SaveHistory(const std::string& path);
+    void LoadHistory(const std::string& path);
 
     long ne = 0;
     double dt = 0.0;

@@ -26,7 +26,7 @@ EXPORTED = [
     "tdbem_adjoint", "tdbem_get_materials", "tdbem_set_materials", "tdbem_dyna_gmres_solver",
     "tdbem_timings", "tdbem_timings_n", "tdbem_history_stats", "tdbem_device_synchronize",
     "tdbem_set_option", "tdbem_model_counts", "tdbem_read_model", "tdbem_read_bd", "tdbem_write_model_bin",
-    "tdbem_write_bd_bin", "tdbem_write_validation",
+    "tdbem_write_bd_bin", "tdbem_write_validation", "tdbem_history_save", "tdbem_history_load",
 ]
 
 
@@ -76,6 +76,8 @@ def lib():
         L.tdbem_read_bd.argtypes = [C.c_char_p, C.c_int, C.c_long, C.c_int, C.c_void_p, _ip, _dp]
         L.tdbem_write_model_bin.argtypes = [C.c_char_p, C.c_long, _dp, C.c_long, _ip, C.c_long, C.c_void_p]
         L.tdbem_write_bd_bin.argtypes = [C.c_char_p, C.c_long, C.c_int, _ip, _dp]
+        L.tdbem_history_save.argtypes = [C.c_void_p, C.c_char_p]
+        L.tdbem_history_load.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_int)]
         L.tdbem_write_validation.argtypes = [C.c_void_p, C.c_char_p, C.c_int, _dp, _dp, _ip, _ip, C.c_int, C.c_int]
         _lib = L
     return _lib
@@ -233,6 +235,16 @@ class Case:
         npairs = C.c_long()
         _check(lib().tdbem_history_stats(self.h, counts, C.byref(npairs)), "tdbem_history_stats")
         return counts, npairs.value
+
+    def save_history(self, path):
+        """Write the assembled history bands to path."""
+        _check(lib().tdbem_history_save(self.h, str(path).encode()), "tdbem_history_save")
+
+    def load_history(self, path):
+        """Load history bands written for this model; returns the step count."""
+        n = C.c_int()
+        _check(lib().tdbem_history_load(self.h, str(path).encode(), C.byref(n)), "tdbem_history_load")
+        return n.value
 
     def write_validation(self, directory, U, T, iters, flags, solved, solver=2):
         """validation_state.csv and validation_metrics.txt of a solve (U, T as

@@ -579,3 +579,50 @@ def test_validation_outputs_of_a_solve(tmp_path):
     assert kv["MultiDomainMatrixStructureValid"] == "1" and kv["MultiDomainMatrixZeroRows"] == "0"
     assert kv["MultiDomainMatrixDiagonalBlocks"] == str(diag)
     assert kv["MultiDomainInterfaceDofInvariantValid"] == "1"
+
+
+def test_history_persistence(tmp_path):
+    """Saved history bands loaded into a fresh case of the same model give the
+    same MTS / MGS matrices and the same solution bit for bit (FP64 and the
+    FP32 + bf16 mode); a case with other materials or another mesh refuses the
+    file."""
+    tdbem = _tdbem()
+    from tdbem import mesh
+    w = mesh.layered_config2(h=0.5, nstep=30)
+    m = w.mesh
+    print("case:", w.name, "history save / load", flush=True)
+
+    def case(mats=w.mats, mm=m, mixed=0):
+        c = tdbem.Case(mm.points, mm.ele, mm.bcid, w.dt, w.gap, domain_count=4, mat_rows=mats,
+                       itf_rows=mm.interfaces)
+        c.set_option("mixed", mixed)
+        c.assemble_step0(10)
+        return c
+
+    for mixed in (0, 2):
+        path = tmp_path / ("hist%d.bin" % mixed)
+        a = case(mixed=mixed)
+        a.assemble_history(w.nstep)
+        U0, T0, it0, *_ = a.solve(w.known, w.nstep, tol=1e-12)
+        a.save_history(path)
+        ref = [(a.ccsr(2, k), a.ccsr(3, k)) for k in (1, 9, w.nstep)]
+        a.close()
+        b = case()
+        assert b.load_history(path) == w.nstep
+        for (x2, x3), k in zip(ref, (1, 9, w.nstep)):
+            for x, y in ((x2, b.ccsr(2, k)), (x3, b.ccsr(3, k))):
+                assert all(np.array_equal(u, v) for u, v in zip(x, y)), (mixed, k)
+        U1, T1, it1, *_ = b.solve(w.known, w.nstep, tol=1e-12)
+        b.close()
+        assert np.array_equal(U0, U1) and np.array_equal(T0, T1) and np.array_equal(it0, it1), mixed
+    other = w.mats.copy()
+    other[0, 1] *= 1.01
+    c = case(mats=other)
+    with pytest.raises(RuntimeError, match="other materials"):
+        c.load_history(tmp_path / "hist0.bin")
+    c.close()
+    m2 = mesh.layered_config2(h=0.25, nstep=30).mesh
+    c = case(mm=m2)
+    with pytest.raises(RuntimeError, match="another model"):
+        c.load_history(tmp_path / "hist0.bin")
+    c.close()

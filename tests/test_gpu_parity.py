@@ -626,3 +626,61 @@ def test_history_persistence(tmp_path):
     with pytest.raises(RuntimeError, match="another model"):
         c.load_history(tmp_path / "hist0.bin")
     c.close()
+
+
+def test_single_domain_gates():
+    """The reference's D = 1 gates (tools/run_multidomain_validation.py:
+    996-1030): (i) a single-domain model read without the multi-domain
+    datacard tail and the same model with an explicit one-domain tail give the
+    same solution bit for bit; (ii) a homogeneous bar solved as one domain and
+    as two domains with an interface agree on the shared outer elements
+    (reference tolerances 1e-7 absolute / 1e-5 relative; here 1e-9 of the
+    largest displacement) over steps whose time windows end well before the
+    load's P wave can reach the interface."""
+    tdbem = _tdbem()
+    from tdbem import mesh
+    E, v, rho = 10.0, 0.25, 1.0
+    c1, _ = mesh.wave_speeds(E, v, rho)
+    h = 0.25
+    dt = 0.5 * h / c1
+    nstep = 6
+    # the P front of the loaded face (x = 2) stays far from the interface at
+    # x = 0 over every step's time window, numerical precursors included
+    assert c1 * dt * (nstep + 1) <= 1.0
+    one = mesh.box_mesh(1, 4.0, 1.0, 1.0, 16, 4, 4)
+    two = mesh.box_mesh(2, 4.0, 1.0, 1.0, 8, 4, 4)
+    print("case: single-domain gates", len(one.ele), len(two.ele), "elements", flush=True)
+
+    def run(mm, domain_count, mats, itf):
+        known = mesh.normal_load_history(len(mm.ele), nstep, dt, mm.top, np.r_[0.0, np.ones(nstep)],
+                                         direction=(0.0, 0.0, 1.0))
+        c = tdbem.Case(mm.points, mm.ele, mm.bcid, dt, 0.4, domain_count=domain_count, mat_rows=mats,
+                       itf_rows=itf, default_mat=(E, v, rho))
+        c.assemble_step0(10)
+        c.assemble_history(nstep)
+        U, T, it, fl, solved, _ = c.solve(known, nstep, tol=1e-12)
+        c.close()
+        assert solved == nstep and not fl.any()
+        return U, T
+
+    U0, T0 = run(one, 0, None, None)
+    U1, T1 = run(one, 1, np.array([[1, E, v, rho]]), np.zeros((0, 5), np.int64))
+    assert np.array_equal(U0, U1) and np.array_equal(T0, T1)
+    U2, T2 = run(two, 2, np.array([[1, E, v, rho], [2, E, v, rho]]), two.interfaces)
+
+    def centres(mm):
+        return mm.points[mm.ele[:, :4]].mean(axis=1)
+
+    c_one, c_two = centres(one), centres(two)
+    itf = set((two.interfaces[:, 1] - 1).tolist()) | set((two.interfaces[:, 3] - 1).tolist())
+    idx = {tuple(np.round(c, 9)): i for i, c in enumerate(c_one)}
+    pairs = [(idx[tuple(np.round(c, 9))], j) for j, c in enumerate(c_two) if j not in itf]
+    assert len(pairs) == len(one.ele)
+    a = np.array([p[0] for p in pairs])
+    b = np.array([p[1] for p in pairs])
+    scale = np.abs(U0).max()
+    assert scale > 0
+    diff = np.abs(U0[:, a] - U2[:, b]).max()
+    print("outer displacement difference", diff, "of", scale, flush=True)
+    assert diff <= 1e-9 * scale
+    assert np.abs(T0[:, a] - T2[:, b]).max() <= 1e-9 * np.abs(T0).max()

@@ -2524,13 +2524,25 @@ __global__ void __launch_bounds__(256) k_multidot(const TV* __restrict__ V, long
     }
 }
 
+// h[k] = sum of the block partials in block order; the loads go out eight at
+// a time ahead of the adds.
 __global__ void k_finish(const double* __restrict__ partial, int nblocks, int nv, double* __restrict__ h, int doSqrt)
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nv)
         return;
     double a = 0.0;
-    for (int b = 0; b < nblocks; ++b)
+    int b = 0;
+    for (; b + 8 <= nblocks; b += 8) {
+        double t[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            t[i] = partial[(long)(b + i) * nv + k];
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            a += t[i];
+    }
+    for (; b < nblocks; ++b)
         a += partial[(long)b * nv + k];
     h[k] = doSqrt ? sqrt(a) : a;
 }

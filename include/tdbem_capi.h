@@ -85,8 +85,10 @@ int tdbem_solve(tdbem_ctx* ctx, int nstep, const double* known_lu, long restart,
  * (E, v, rho): grad[3*d + p].  The forward solve, the backward sweep
  * (A^T solves with the transposed leaf preconditioner and the transposed
  * history convolution) and the gradient contraction all run on the device;
- * the operator derivatives are central differences over the materials with
- * relative step fd_step (band structure held fixed).  Needs FP64 history
+ * the history blocks' material derivatives are analytic (option
+ * "adjoint_analytic"), those of the step-0 blocks and the singular self
+ * tables central differences with relative step fd_step, all with the band
+ * structure and judges held fixed.  Needs FP64 history
  * blocks assembled for at least nstep steps; lu (may be null) receives the
  * forward displacements [(nstep+1)][ne][3]. */
 int tdbem_adjoint(tdbem_ctx* ctx, int nstep, const double* known_lu, int nrec, const int* receivers,
@@ -136,6 +138,12 @@ int tdbem_history_stats(tdbem_ctx* ctx, long* counts, long* npairs);
  *   over the history bands per step (the reference's per-step summation);
  *   "conv_tile" 16 (default) or 8 steps per far-part tile of the
  *   time-blocked convolution;
+ *   "adjoint_analytic" 1 (default) tdbem_adjoint takes the history blocks'
+ *   material derivatives analytically (the derivative of the closed-form
+ *   quadrature with the judges, patch masks and every point's front / window
+ *   case held at the current materials), 0 by central differences of two
+ *   quadratures per parameter; step-0 blocks and the singular self tables
+ *   are differenced in both modes;
  *   "conv_filtered" 1 (default) 16-step far tiles read a pre-filtered state
  *   series and prefetch the next (pair, lags) item, 0 each item stages and
  *   filters the raw states itself (same sums in the same order);

@@ -285,23 +285,25 @@ double Case::Adjoint(int nstep, const double* known_lu, const int* recv, int nre
         }
     }
 
-    // 5. gradient: central differences of the operators, contracted with lambda,
-    // with the bands and judges frozen at the current materials
+    // 5. gradient, with the bands and judges frozen at the current materials.
+    // History blocks: analytic material derivatives contracted with the
+    // per-entry correlation of lambda and the filtered states (or, with
+    // adjointAnalytic off, central differences of two quadratures per
+    // parameter).  Step-0 blocks and the singular self tables: central
+    // differences (few blocks, reference-order kernels).
+    const bool analytic = adjointAnalytic;
     bool wasFrozen = frozen;
     if (!wasFrozen)
         FreezeStructure(true);
     std::vector<double> theta((size_t)nd * 3);
     GetMaterials(theta.data());
-    DevBuf<double> d_dM, d_dG, d_acc, d_v, d_t1, d_t2, d_kn;
+    DevBuf<double> d_dM, d_dG, d_acc, d_v, d_y1, d_y2, d_kn, d_C;
     DevBuf<unsigned char> d_aliveS;
-    d_dM.alloc(9 * (size_t)entries);
-    d_dG.alloc(6 * (size_t)entries);
-    d_aliveS.alloc((size_t)entries);
     d_acc.alloc(3 * (size_t)ne);
     LaunchZero(d_acc.p, (long)d_acc.n, stream);
     d_v.alloc((size_t)n3);
-    d_t1.alloc((size_t)n3);
-    d_t2.alloc((size_t)n3);
+    d_y1.alloc((size_t)n3);
+    d_y2.alloc((size_t)n3);
     d_kn.alloc(3 * (size_t)std::max<long>(1, dof.knownBlockCount));
     LaunchZero(d_kn.p, (long)d_kn.n, stream);
     std::vector<double> delta((size_t)nd), inv2row((size_t)ne), scaleDom((size_t)nd);
@@ -312,8 +314,20 @@ double Case::Adjoint(int nstep, const double* known_lu, const int* recv, int nre
     d_FT.alloc(std::max<size_t>(1, (size_t)dof.historyTBlockCount * 3 * (size_t)nstore));
     LaunchFilterSeries(kp.gU.p, dof.historyUBlockCount, nstore, nstep, d_FU.p, stream);
     LaunchFilterSeries(kp.gT.p, dof.historyTBlockCount, nstore, nstep, d_FT.p, stream);
-    d_part.alloc(std::max<size_t>(1, (size_t)chunks));
-    d_scale.alloc((size_t)nd);
+    std::vector<double> dT2;
+    if (analytic) {
+        d_C.alloc(15 * (size_t)std::max<long>(1, entries));
+        LaunchCorrStore(grid, d_refs.p, d_coff.p, d_off.p, d_kb.p, d_len.p, npairs, chunks, d_lam.p, n3, d_FU.p,
+                        d_FT.p, kp.gU.p, kp.gT.p, nstore, nstep, d_C.p, stream);
+        dT2.assign(27 * (size_t)ne, 0.0);
+    } else {
+        d_dM.alloc(9 * (size_t)entries);
+        d_dG.alloc(6 * (size_t)entries);
+        d_aliveS.alloc((size_t)entries);
+        d_part.alloc(std::max<size_t>(1, (size_t)chunks));
+        d_scale.alloc((size_t)nd);
+    }
+    std::vector<double> t2p, t2m;
     for (int p = 0; p < 3; ++p) {
         std::vector<double> tp = theta, tm = theta;
         for (int d = 0; d < nd; ++d) {
@@ -327,41 +341,89 @@ double Case::Adjoint(int nstep, const double* known_lu, const int* recv, int nre
         CCSRMat Ap, Kp, Am, Km;
         SetMaterials(tp.data());
         ComputeStep0(Ap, Kp);
-        StagedQuadrature(d_dM.p, d_dG.p, nullptr, nullptr, d_aliveS.p);
+        if (analytic) {
+            t2p.resize(9 * (size_t)ne);
+            CheckA(hipMemcpy(t2p.data(), d_t2.p, sizeof(double) * t2p.size(),
+                             hipMemcpyDeviceToHost),
+                   "self tables");
+        } else {
+            StagedQuadrature(d_dM.p, d_dG.p, nullptr, nullptr, d_aliveS.p);
+        }
         SetMaterials(tm.data());
         ComputeStep0(Am, Km);
-        StagedQuadrature(d_mts.p, d_mgs.p, nullptr, nullptr, d_aliveS.p);
-        // history blocks: sum_K lambda_K . h_K(dM), entry by entry
-        for (int d = 0; d < nd; ++d)
-            scaleDom[(size_t)d] = 1.0 / (2.0 * delta[(size_t)d]);
-        d_scale.upload(scaleDom.data(), scaleDom.size(), stream);
-        LaunchCorrFd(grid, d_refs.p, d_coff.p, d_off.p, d_kb.p, d_len.p, npairs, chunks, d_dM.p, d_dG.p, d_mts.p,
-                     d_mgs.p, d_scale.p, d_lam.p, n3, d_FU.p, d_FT.p, kp.gU.p, kp.gT.p, nstore, nstep, d_part.p,
-                     stream);
-        LaunchRowChunks(grid, d_refs.p, d_coff.p, d_part.p, (int)ne, d_acc.p + (size_t)p * ne, stream);
+        if (analytic) {
+            t2m.resize(9 * (size_t)ne);
+            CheckA(hipMemcpy(t2m.data(), d_t2.p, sizeof(double) * t2m.size(), hipMemcpyDeviceToHost),
+                   "self tables");
+            for (long f = 0; f < ne; ++f)
+                for (int i = 0; i < 9; ++i)
+                    dT2[27 * (size_t)f + 9 * (size_t)p + i] = (t2p[9 * (size_t)f + i] - t2m[9 * (size_t)f + i]) *
+                                                               inv2row[(size_t)f];
+        } else {
+            StagedQuadrature(d_mts.p, d_mgs.p, nullptr, nullptr, d_aliveS.p);
+            // history blocks: sum_K lambda_K . h_K(dM), entry by entry
+            for (int d = 0; d < nd; ++d)
+                scaleDom[(size_t)d] = 1.0 / (2.0 * delta[(size_t)d]);
+            d_scale.upload(scaleDom.data(), scaleDom.size(), stream);
+            LaunchCorrFd(grid, d_refs.p, d_coff.p, d_off.p, d_kb.p, d_len.p, npairs, chunks, d_dM.p, d_dG.p, d_mts.p,
+                         d_mgs.p, d_scale.p, d_lam.p, n3, d_FU.p, d_FT.p, kp.gU.p, kp.gT.p, nstore, nstep, d_part.p,
+                         stream);
+            LaunchRowChunks(grid, d_refs.p, d_coff.p, d_part.p, (int)ne, d_acc.p + (size_t)p * ne, stream);
+        }
         // step-0 blocks: lambda_K . (dKnownM k_K - dA x_K)
         dA.Upload(DiffOnPattern(A, Ap, Am, inv2row), stream);
         dK.Upload(DiffOnPattern(KnownM, Kp, Km, inv2row), stream);
         for (int K = 1; K <= nstep; ++K) {
             LaunchZero(d_v.p, n3, stream);
             LaunchKnown(d_refs.p, kp.bd.p + (size_t)K * n3, (int)ne, d_kn.p, dof.knownBlockCount, stream);
-            LaunchSpmv(ne, dK.rp.p, dK.col.p, (const double*)dK.val.p, (const double*)d_kn.p, d_t1.p, stream);
+            LaunchSpmv(ne, dK.rp.p, dK.col.p, (const double*)dK.val.p, (const double*)d_kn.p, d_y1.p, stream);
             LaunchSpmv(ne, dA.rp.p, dA.col.p, (const double*)dA.val.p, (const double*)(kp.X.p + (size_t)K * n3),
-                       d_t2.p, stream);
-            LaunchAddSub(d_v.p, d_t1.p, d_t2.p, n3, stream);
+                       d_y2.p, stream);
+            LaunchAddSub(d_v.p, d_y1.p, d_y2.p, n3, stream);
             LaunchRowDotAcc(d_lam.p + (size_t)K * n3, d_v.p, ne, d_acc.p + (size_t)p * ne, stream);
         }
         CheckA(hipStreamSynchronize(stream), "gradient");
     }
-    // restore the materials, self tables and FP64 bands (same kernels, same inputs)
+    // restore the materials and self tables (and, after differences, the bands)
     SetMaterials(theta.data());
-    StagedQuadrature(d_mts.p, d_mgs.p, nullptr, nullptr, d_alive.p);
+    std::vector<double> hsum((size_t)nd * 3, 0.0);
+    if (analytic) {
+        // d(C1, C2, rho) / d(E, v, rho) per domain
+        std::vector<double> J((size_t)nd * 9, 0.0);
+        for (int d = 0; d < nd; ++d) {
+            const Domain& D = model.domains[(size_t)d];
+            double E = D.E, v = D.v, rho = D.Rou, C1 = D.mat.C1, C2 = D.mat.C2;
+            double* j = &J[(size_t)d * 9];
+            j[0] = C1 / (2.0 * E);
+            j[1] = 0.5 * C1 * (1.0 / ((1.0 - 2.0 * v) * (1.0 - v)) - 1.0 / (1.0 + v));
+            j[2] = -C1 / (2.0 * rho);
+            j[3] = C2 / (2.0 * E);
+            j[4] = -C2 / (2.0 * (1.0 + v));
+            j[5] = -C2 / (2.0 * rho);
+            j[8] = 1.0;
+        }
+        DevBuf<double> d_J, d_dT2, d_gpart, d_gsum;
+        d_J.upload(J.data(), J.size(), stream);
+        d_dT2.upload(dT2.data(), dT2.size(), stream);
+        PackPwWaves();
+        long nw = chunks + pwWaves;
+        d_gpart.alloc(4 * (size_t)std::max<long>(1, nw));
+        d_gsum.alloc((size_t)nd * 3);
+        LaunchZero(d_gsum.p, (long)d_gsum.n, stream);
+        LaunchHistoryGrad(d_geo.p, d_pts.p, d_refs.p, grid, d_coff.p, d_off.p, d_kb.p, d_len.p, d_cls.p, npairs,
+                          chunks, d_pwWaves.p, pwWaves, d_alive.p, d_C.p, d_J.p, d_dT2.p, d_gpart.p, stream);
+        LaunchDomainSums(d_gpart.p, nw, nd, d_gsum.p, stream);
+        CheckA(hipMemcpy(hsum.data(), d_gsum.p, sizeof(double) * hsum.size(), hipMemcpyDeviceToHost),
+               "history gradient");
+    } else {
+        StagedQuadrature(d_mts.p, d_mgs.p, nullptr, nullptr, d_alive.p);
+    }
     if (!wasFrozen)
         FreezeStructure(false);
     std::vector<double> acc(3 * (size_t)ne);
     CheckA(hipMemcpy(acc.data(), d_acc.p, sizeof(double) * acc.size(), hipMemcpyDeviceToHost), "gradient");
     for (int i = 0; i < 3 * nd; ++i)
-        grad[i] = 0.0;
+        grad[i] = hsum[(size_t)i];
     for (int p = 0; p < 3; ++p)
         for (long s = 0; s < ne; ++s)
             grad[3 * domOf[(size_t)s] + p] += acc[(size_t)p * ne + s];

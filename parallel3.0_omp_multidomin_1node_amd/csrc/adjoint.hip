@@ -500,6 +500,99 @@ void LaunchCorrFd(PairGrid grid, const NodeRefs* refs, const long* coff, const l
     }
 }
 
+// Per band entry e = (s, f, j) the correlation of the adjoint with the
+// filtered states the entry multiplies in the forward history term:
+//   CU[r][c] = sum_{K >= j} lambda_K[s][r] cU_{K-j}[c],  CT likewise,
+// stored as C[15 e ..] = (CU transposed: C[3a+b] = CU[b][a], then the
+// symmetric sums of CT: 00, 01+10, 02+20, 11, 12+21, 22), so that
+// sum_K lambda_K . h_K(dM) = sum_e (dMGS_e . CT_e - dMTS_e : CU_e).
+__global__ void __launch_bounds__(64) k_corr_store(PairGrid grid, const NodeRefs* __restrict__ refs,
+                                                   const long* __restrict__ coff, const long* __restrict__ off,
+                                                   const int* __restrict__ kb, const int* __restrict__ len,
+                                                   long npairs, long c0, long c1, const double* __restrict__ lam,
+                                                   long n3, const double* __restrict__ FU,
+                                                   const double* __restrict__ FT, const double* __restrict__ gU,
+                                                   const double* __restrict__ gT, int nstore, int N,
+                                                   double* __restrict__ C)
+{
+    long wv = c0 + (long)blockIdx.x;
+    if (wv >= c1)
+        return;
+    int lane = threadIdx.x;
+    long lo = 0, hi = npairs;
+    while (hi - lo > 1) {
+        long mid = (lo + hi) >> 1;
+        if (coff[mid] <= wv)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    long p = lo;
+    int s, f, d;
+    DecodePairOf(grid, p, s, f, d);
+    int t0 = (int)(wv - coff[p]) * 64;
+    int t = t0 + lane;
+    bool in = t < len[p];
+    int j = kb[p] + t, jlo = kb[p] + t0;
+    const double* fu = FU + (long)refs[f].hu_block * (N + 1) * 3;
+    const double* ft = FT + (long)refs[f].ht_block * (N + 1) * 3;
+    double cu[9], ct[9];
+    for (int i = 0; i < 9; ++i) {
+        cu[i] = 0.0;
+        ct[i] = 0.0;
+    }
+    for (int K = jlo; K <= N; ++K) {
+        const double* l = lam + (long)K * n3 + 3 * (long)s;
+        double l0 = l[0], l1 = l[1], l2 = l[2];
+        if (in && K >= j) {
+            int tt = K - j;
+            double u[3] = {fu[3 * tt], fu[3 * tt + 1], fu[3 * tt + 2]};
+            double v[3] = {ft[3 * tt], ft[3 * tt + 1], ft[3 * tt + 2]};
+            if (j == 1) {
+                const double* gu = gU + ((long)refs[f].hu_block * nstore + K) * 3;
+                const double* gt = gT + ((long)refs[f].ht_block * nstore + K) * 3;
+                for (int c = 0; c < 3; ++c) {
+                    u[c] -= gu[c];
+                    v[c] -= gt[c];
+                }
+            }
+            for (int c = 0; c < 3; ++c) {
+                cu[c] = fma(l0, u[c], cu[c]);
+                cu[3 + c] = fma(l1, u[c], cu[3 + c]);
+                cu[6 + c] = fma(l2, u[c], cu[6 + c]);
+                ct[c] = fma(l0, v[c], ct[c]);
+                ct[3 + c] = fma(l1, v[c], ct[3 + c]);
+                ct[6 + c] = fma(l2, v[c], ct[6 + c]);
+            }
+        }
+    }
+    if (!in)
+        return;
+    double* o = C + 15 * (off[p] + t);
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+            o[3 * a + b] = cu[3 * b + a];
+    o[9] = ct[0];
+    o[10] = ct[1] + ct[3];
+    o[11] = ct[2] + ct[6];
+    o[12] = ct[4];
+    o[13] = ct[5] + ct[7];
+    o[14] = ct[8];
+}
+
+void LaunchCorrStore(PairGrid grid, const NodeRefs* refs, const long* coff, const long* off, const int* kb,
+                     const int* len, long npairs, long chunks, const double* lam, long n3, const double* FU,
+                     const double* FT, const double* gU, const double* gT, int nstore, int N, double* C,
+                     hipStream_t s)
+{
+    const long step = 1L << 20;
+    for (long c0 = 0; c0 < chunks; c0 += step) {
+        long c1 = std::min(chunks, c0 + step);
+        k_corr_store<<<(unsigned)(c1 - c0), 64, 0, s>>>(grid, refs, coff, off, kb, len, npairs, c0, c1, lam, n3, FU,
+                                                        FT, gU, gT, nstore, N, C);
+    }
+}
+
 // acc[s] += sum of the chunk partials of row s (its pairs are contiguous).
 __global__ void __launch_bounds__(256) k_row_chunks(PairGrid grid, const NodeRefs* __restrict__ refs,
                                                     const long* __restrict__ coff, const double* __restrict__ part,

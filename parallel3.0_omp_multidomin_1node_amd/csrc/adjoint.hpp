@@ -43,6 +43,11 @@ void LaunchCorrFd(PairGrid grid, const NodeRefs* refs, const long* coff, const l
                   const double* gm, const double* scale, const double* lam, long n3, const double* FU,
                   const double* FT, const double* gU, const double* gT, int nstore, int N, double* part,
                   hipStream_t s);
+// Per band entry: (CU transposed, symmetric sums of CT), 15 doubles.
+void LaunchCorrStore(PairGrid grid, const NodeRefs* refs, const long* coff, const long* off, const int* kb,
+                     const int* len, long npairs, long chunks, const double* lam, long n3, const double* FU,
+                     const double* FT, const double* gU, const double* gT, int nstore, int N, double* C,
+                     hipStream_t s);
 void LaunchRowChunks(PairGrid grid, const NodeRefs* refs, const long* coff, const double* part, int ne, double* acc,
                      hipStream_t s);
 

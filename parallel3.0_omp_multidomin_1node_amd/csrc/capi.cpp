@@ -313,6 +313,10 @@ int tdbem_set_option(tdbem_ctx* ctx, const char* name, double value)
         ctx->c.convTile = (int)value;  // near lists are rebuilt by the next solve
         return 0;
     }
+    if (n == "adjoint_analytic") {
+        ctx->c.adjointAnalytic = value != 0.0;
+        return 0;
+    }
     if (n == "conv_filtered") {
         ctx->c.convFiltered = value != 0.0;
         return 0;

@@ -1663,6 +1663,536 @@ TDBEM_HIST_INSTANTIATE(float)
 #undef TDBEM_HIST_INSTANTIATE
 
 // ---------------------------------------------------------------------------
+// Material derivatives of the history blocks, contracted on the fly.
+// For the adjoint gradient only sum_e (dMGS_e . CT_e - dMTS_e : CU_e) is
+// needed (CU, CT: the correlation of the adjoint with the entry's filtered
+// states, k_corr_store).  A block is sum over points of cX X + cQ Q + cB B
+// (and uX X + uD I for U), X, Q, B material-free, so its derivative
+// contracted with CU is sum over points of dcX (r.CU^T r) + dcQ (n.CU^T r) +
+// dcB (r.CU^T n + (r.n) tr CU): three scalars per point and lane.  The
+// derivatives are those of the closed forms of the staged quadrature with the
+// band structure, the judges, the patch masks and every point's front / window
+// case held fixed (the piece of the piecewise-smooth block the current
+// materials sit on), taken with respect to q = (C1, C2, rho) and mapped to
+// (E, v, rho) per domain.
+// ---------------------------------------------------------------------------
+namespace dev {
+
+constexpr int kRecD = 100;  // doubles per staged point of the derivative pass
+enum RecDGeo : int { gIR2h = 0, gIR3t = 1, gRk = 2, gR = 3, gN = 6, gRn = 9, gX = 10 };
+enum RecDMat : int {
+    mRc1 = 0, mRc2, mPc1, mPc2, mSXd, mSQd, mSBd, mSXe, mSQe, mSBe, mSXf, mSQBf, mSXg, mSQg, mSXh, mSBh,
+    mUP, mUS1, mUS0, mUC1, mUC0
+};
+// value, d/dC1, d/dC2, d/drho of material scalar i
+__device__ __forceinline__ int MS(int i) { return 16 + 4 * i; }
+
+__device__ void StageRecordD(const DynaMat& m, const double* src, const double* pos, const double* nor, double w,
+                             double* g)
+{
+    double r[3] = {pos[0] - src[0], pos[1] - src[1], pos[2] - src[2]};
+    double R = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+    double iR = 1.0 / R;
+    double iR2 = iR * iR, iR3 = iR2 * iR, iR4 = iR2 * iR2, iR5 = iR4 * iR;
+    double rn = r[0] * nor[0] + r[1] * nor[1] + r[2] * nor[2];
+    double C1 = m.C1, C2 = m.C2, rho = m.Rou;
+    g[gIR2h] = 0.5 * iR2;
+    g[gIR3t] = iR3 * (1.0 / 3.0);
+    g[gRk] = kOneOver4Pi * (R / m.Dt);
+    for (int a = 0; a < 3; ++a) {
+        g[gR + a] = r[a];
+        g[gN + a] = nor[a];
+    }
+    g[gRn] = rn;
+    g[gX + 0] = r[0] * r[0];
+    g[gX + 1] = r[0] * r[1];
+    g[gX + 2] = r[0] * r[2];
+    g[gX + 3] = r[1] * r[1];
+    g[gX + 4] = r[1] * r[2];
+    g[gX + 5] = r[2] * r[2];
+    auto put = [&](int i, double v, double d1, double d2, double dr) {
+        double* o = g + MS(i);
+        o[0] = v;
+        o[1] = d1;
+        o[2] = d2;
+        o[3] = dr;
+    };
+    double rc1 = R / C1, rc2 = R / C2;
+    put(mRc1, rc1, -rc1 / C1, 0.0, 0.0);
+    put(mRc2, rc2, 0.0, -rc2 / C2, 0.0);
+    double pc1 = kOneOver4Pi * (R / m.C1Dt), pc2 = kOneOver4Pi * (R / m.C2Dt);
+    put(mPc1, pc1, -pc1 / C1, 0.0, 0.0);
+    put(mPc2, pc2, 0.0, -pc2 / C2, 0.0);
+    double c = m.C2_2OverC1_2;
+    double dc1 = -2.0 * c / C1, dc2 = 2.0 * c / C2;  // dc/dC1, dc/dC2
+    double tg = kOneOver4Pi / m.Dt * w;
+    double sXd = -(c * 12.0 * rn * iR5) * w;
+    put(mSXd, sXd, sXd / c * dc1, sXd / c * dc2, 0.0);
+    double sQd = (4.0 * c - 1.0) * iR3 * w;
+    put(mSQd, sQd, 4.0 * iR3 * w * dc1, 4.0 * iR3 * w * dc2, 0.0);
+    double sBd = c * 2.0 * iR3 * w;
+    put(mSBd, sBd, 2.0 * iR3 * w * dc1, 2.0 * iR3 * w * dc2, 0.0);
+    put(mSXe, 12.0 * rn * iR5 * w, 0.0, 0.0, 0.0);
+    put(mSQe, -2.0 * iR3 * w, 0.0, 0.0, 0.0);
+    put(mSBe, -3.0 * iR3 * w, 0.0, 0.0, 0.0);
+    double sXf = -30.0 * m.C2_2 * rn * iR5 * w;
+    put(mSXf, sXf, 0.0, 2.0 * sXf / C2, 0.0);
+    double sQBf = 6.0 * m.C2_2 * iR3 * w;
+    put(mSQBf, sQBf, 0.0, 2.0 * sQBf / C2, 0.0);
+    double sXg = -2.0 * m.C2_2OverC1_3 * rn * iR4 * tg;
+    put(mSXg, sXg, -3.0 * sXg / C1, 2.0 * sXg / C2, 0.0);
+    double sQg = (2.0 * c - 1.0) / C1 * iR2 * tg;
+    double C1_2 = C1 * C1;
+    put(mSQg, sQg, (-6.0 * c / C1_2 + 1.0 / C1_2) * iR2 * tg, 4.0 * C2 / (C1_2 * C1) * iR2 * tg, 0.0);
+    double sXh = 2.0 / C2 * rn * iR4 * tg;
+    put(mSXh, sXh, 0.0, -sXh / C2, 0.0);
+    double sBh = -(iR2 / C2) * tg;
+    put(mSBh, sBh, 0.0, -sBh / C2, 0.0);
+    double tu = m.OneOverFourPaiRou * w;
+    double uP = m.OneOverC1_2 * iR3 * tu;
+    put(mUP, uP, -2.0 * uP / C1, 0.0, -uP / rho);
+    double uS1 = -(m.OneOverC2_2 * iR3) * tu;
+    put(mUS1, uS1, 0.0, -2.0 * uS1 / C2, -uS1 / rho);
+    double uS0 = m.OneOverC2_2 * iR * tu;
+    put(mUS0, uS0, 0.0, -2.0 * uS0 / C2, -uS0 / rho);
+    double uC1 = 3.0 * iR3 * tu;
+    put(mUC1, uC1, 0.0, 0.0, -uC1 / rho);
+    double uC0 = -iR * tu;
+    put(mUC0, uC0, 0.0, 0.0, -uC0 / rho);
+}
+
+// Derivatives (q = C1, C2, rho) of the basis scalars of one span at one
+// staged point, the piece (front flags, window bounds) held at its current
+// values; for TYPE 1 also those of the U scalars.
+struct SpanDer {
+    double dX[3], dQ[3], dB[3], duX[3], duD[3];
+};
+
+template <int TYPE>
+__device__ __forceinline__ void MixDerAt(const double* g, const Span& s, bool on, SpanDer& o)
+{
+    const double* R1 = g + MS(mRc1);
+    const double* R2 = g + MS(mRc2);
+    double rc1 = R1[0], rc2 = R2[0];
+    bool P = on && rc1 >= s.ndt && rc1 < s.np1;
+    bool S = on && rc2 >= s.ndt && rc2 < s.np1;
+    double lo = fmax(s.ndt, rc1), hi = fmin(s.np1, rc2);
+    bool win = on && hi - lo > 0.0;
+    double d = win ? hi - lo : 0.0;
+    double sm = hi + lo;
+    double ri = d * sm * g[gIR2h];
+    double phi = fma(hi, sm, lo * lo);
+    double rj = d * phi * g[gIR3t];
+    double rk = g[gRk];
+    double gam = TYPE == 1 ? fma(s.nfac, ri, -(rk * rj)) : fma(s.nfac, ri, rk * rj);
+    const double* Pc1 = g + MS(mPc1);
+    const double* Pc2 = g + MS(mPc2);
+    double aP = P ? (TYPE == 1 ? s.nfac - Pc1[0] : s.nfac + Pc1[0]) : 0.0;
+    double aS = S ? (TYPE == 1 ? s.nfac - Pc2[0] : s.nfac + Pc2[0]) : 0.0;
+    double bP = P ? (TYPE == 1 ? 1.0 : -1.0) : 0.0;
+    double bS = S ? (TYPE == 1 ? 1.0 : -1.0) : 0.0;
+    bool loR = rc1 > s.ndt, hiR = rc2 < s.np1;
+    const double* sXd = g + MS(mSXd);
+    const double* sQd = g + MS(mSQd);
+    const double* sBd = g + MS(mSBd);
+    const double* sXe = g + MS(mSXe);
+    const double* sQe = g + MS(mSQe);
+    const double* sBe = g + MS(mSBe);
+    const double* sXf = g + MS(mSXf);
+    const double* sQBf = g + MS(mSQBf);
+    const double* sXg = g + MS(mSXg);
+    const double* sQg = g + MS(mSQg);
+    const double* sXh = g + MS(mSXh);
+    const double* sBh = g + MS(mSBh);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        double dlo = loR ? R1[1 + q] : 0.0;
+        double dhi = hiR ? R2[1 + q] : 0.0;
+        double dd = win ? dhi - dlo : 0.0;
+        double dsm = dhi + dlo;
+        double dri = (dd * sm + d * dsm) * g[gIR2h];
+        double dphi = dhi * sm + hi * dsm + 2.0 * lo * dlo;
+        double drj = (dd * phi + d * dphi) * g[gIR3t];
+        double dgam = TYPE == 1 ? s.nfac * dri - rk * drj : s.nfac * dri + rk * drj;
+        double daP = P ? (TYPE == 1 ? -Pc1[1 + q] : Pc1[1 + q]) : 0.0;
+        double daS = S ? (TYPE == 1 ? -Pc2[1 + q] : Pc2[1 + q]) : 0.0;
+        o.dX[q] = daP * sXd[0] + aP * sXd[1 + q] + daS * sXe[0] + aS * sXe[1 + q] + bP * sXg[1 + q] +
+                  bS * sXh[1 + q] + dgam * sXf[0] + gam * sXf[1 + q];
+        o.dQ[q] = daP * sQd[0] + aP * sQd[1 + q] + daS * sQe[0] + aS * sQe[1 + q] + bP * sQg[1 + q] +
+                  dgam * sQBf[0] + gam * sQBf[1 + q];
+        o.dB[q] = daP * sBd[0] + aP * sBd[1 + q] + daS * sBe[0] + aS * sBe[1 + q] + bS * sBh[1 + q] +
+                  dgam * sQBf[0] + gam * sQBf[1 + q];
+        if (TYPE == 1) {
+            const double* uC1 = g + MS(mUC1);
+            const double* uC0 = g + MS(mUC0);
+            const double* uP = g + MS(mUP);
+            const double* uS1 = g + MS(mUS1);
+            const double* uS0 = g + MS(mUS0);
+            o.duX[q] = dri * uC1[0] + ri * uC1[1 + q] + (P ? uP[1 + q] : 0.0) + (S ? uS1[1 + q] : 0.0);
+            o.duD[q] = dri * uC0[0] + ri * uC0[1 + q] + (S ? uS0[1 + q] : 0.0);
+        } else {
+            o.duX[q] = 0.0;
+            o.duD[q] = 0.0;
+        }
+    }
+}
+
+__device__ __forceinline__ double WaveTotal(double v)
+{
+    for (int w = 32; w > 0; w >>= 1)
+        v += __shfl_xor(v, w);
+    return v;
+}
+
+// The lane's correlation of one entry: CU transposed (9) and the symmetric
+// sums of CT (6).
+struct EntryCorr {
+    double cut[9], cts[6], trU, trT;
+};
+
+__device__ __forceinline__ void LoadCorr(const double* C, long e, bool valid, EntryCorr& k)
+{
+    const double* c = C + 15 * (valid ? e : 0);
+    for (int i = 0; i < 9; ++i)
+        k.cut[i] = valid ? c[i] : 0.0;
+    for (int i = 0; i < 6; ++i)
+        k.cts[i] = valid ? c[9 + i] : 0.0;
+    k.trU = k.cut[0] + k.cut[4] + k.cut[8];
+    k.trT = k.cts[0] + k.cts[3] + k.cts[5];
+}
+
+// Contractions of the basis tensors at one point with the lane's correlation.
+__device__ __forceinline__ void PointContract(const double* g, const EntryCorr& k, double& xc, double& qc,
+                                              double& bc, double& xt)
+{
+    const double* r = g + gR;
+    const double* n = g + gN;
+    double w1[3], w2[3];
+    for (int a = 0; a < 3; ++a) {
+        w1[a] = k.cut[3 * a] * r[0] + k.cut[3 * a + 1] * r[1] + k.cut[3 * a + 2] * r[2];
+        w2[a] = k.cut[3 * a] * n[0] + k.cut[3 * a + 1] * n[1] + k.cut[3 * a + 2] * n[2];
+    }
+    xc = r[0] * w1[0] + r[1] * w1[1] + r[2] * w1[2];
+    qc = n[0] * w1[0] + n[1] * w1[1] + n[2] * w1[2];
+    bc = r[0] * w2[0] + r[1] * w2[1] + r[2] * w2[2] + g[gRn] * k.trU;
+    const double* X = g + gX;
+    xt = X[0] * k.cts[0] + X[1] * k.cts[1] + X[2] * k.cts[2] + X[3] * k.cts[3] + X[4] * k.cts[4] + X[5] * k.cts[5];
+}
+
+// Entry finish: the contracted derivatives with the AddBlock signs and the
+// stored presence bits, mapped from q = (C1, C2, rho) to (E, v, rho); self
+// entries take the self table's derivative for their T2 block.
+__device__ __forceinline__ void FinishEntryGrad(const EntryCorr& k, const double* gM, const double* gU,
+                                                unsigned char bits, double hs, double ts, bool self1,
+                                                const double* dT2self, const double* J, double* out)
+{
+    double gq[3];
+    for (int q = 0; q < 3; ++q)
+        gq[q] = ((bits & 1) ? -hs * gM[q] : 0.0) + ((bits & 2) ? ts * gU[q] : 0.0);
+    for (int p = 0; p < 3; ++p) {
+        double v = J[p] * gq[0] + J[3 + p] * gq[1] + J[6 + p] * gq[2];
+        if (self1 && (bits & 1)) {
+            const double* dt = dT2self + 9 * p;
+            double sdot = 0.0;
+            for (int i = 0; i < 9; ++i)
+                sdot += dt[i] * k.cut[i];
+            v -= hs * sdot;
+        }
+        out[p] = v;
+    }
+}
+
+}  // namespace dev
+
+// Derivative pass of the regular-path entries (layout of k_history_reg);
+// part[4 wv + p] = the wave's sum of d/d(E, v, rho)_p, part[4 wv + 3] its
+// domain.  J: per domain the
+// 3 x 3 matrix d(C1, C2, rho)/d(E, v, rho) (row q); dT2: per element the
+// derivative of the T2 self table, 27 values (p-major).
+__global__ void __launch_bounds__(64) k_hgrad_reg(
+    const ElemGeo* __restrict__ geo, const ElemPoints* __restrict__ pts, const NodeRefs* __restrict__ refs,
+    PairGrid grid, const long* __restrict__ coff, const long* __restrict__ off, const int* __restrict__ kb,
+    const int* __restrict__ len, const unsigned char* __restrict__ cls, long npairs, long c0, long c1,
+    const unsigned char* __restrict__ alive, const double* __restrict__ C, const double* __restrict__ J,
+    const double* __restrict__ dT2, double* __restrict__ part)
+{
+    __shared__ double sgeo[kNG2 * dev::kRecD];
+    using namespace dev;
+    long wv = c0 + (long)blockIdx.x;
+    if (wv >= c1)
+        return;
+    int lane = threadIdx.x;
+    long lo = 0, hi = npairs;
+    while (hi - lo > 1) {
+        long mid = (lo + hi) >> 1;
+        if (coff[mid] <= wv)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    long p = lo;
+    int s, f, d;
+    DecodePair(grid, p, s, f, d);
+    int t = (int)(wv - coff[p]) * 64 + lane;
+    const DynaMat& m = c_mats[d];
+    const ElemGeo& fe = geo[f];
+    const ElemPoints& P = pts[f];
+    const double* src = geo[s].center;
+    int k = kb[p] + t;
+    double n = (double)k;
+    bool inBand = t < len[p];
+    long e = off[p] + t;
+    EntryFlags fl = JudgeAt(m, fe.radius, Dist3(fe.center, src), k, s == f, inBand ? cls : nullptr, e);
+    bool mine = inBand && !fl.patchwise();
+    EntryCorr kc;
+    LoadCorr(C, e, mine, kc);
+    double gM[3] = {0.0, 0.0, 0.0}, gU[3] = {0.0, 0.0, 0.0};
+    bool rt = mine && (fl.f2 == 2 || fl.f1 == 2);
+    double fr = (mine && fl.f2 == 2 ? 1.0 : 0.0) + (mine && fl.f1 == 2 ? 1.0 : 0.0);
+    double fu = mine && fl.f1 == 2 ? 1.0 : 0.0;
+    if (WaveAny(rt)) {
+        if (lane < kNG2)
+            StageRecordD(m, src, P.rg_pos[lane], P.rg_nor[lane], P.rg_w[lane], sgeo + lane * kRecD);
+        __syncthreads();
+        Span s1 = MakeSpan(m, 1, n), s2 = MakeSpan(m, 2, n);
+#pragma unroll 1
+        for (int i = 0; i < kNG2; ++i) {
+            const double* g = sgeo + i * kRecD;
+            SpanDer a, b;
+            MixDerAt<1>(g, s1, rt, a);
+            MixDerAt<2>(g, s2, rt, b);
+            double xc, qc, bc, xt;
+            PointContract(g, kc, xc, qc, bc, xt);
+            for (int q = 0; q < 3; ++q) {
+                gM[q] += fr * ((a.dX[q] + b.dX[q]) * xc + (a.dQ[q] + b.dQ[q]) * qc + (a.dB[q] + b.dB[q]) * bc);
+                gU[q] += fu * (a.duX[q] * xt + a.duD[q] * kc.trT);
+            }
+        }
+    }
+    double out[3] = {0.0, 0.0, 0.0};
+    if (mine)
+        FinishEntryGrad(kc, gM, gU, alive[e], refs[f].hu_sign, refs[f].ht_sign, fl.self1, dT2 + 27 * (long)f,
+                        J + 9 * d, out);
+    for (int q = 0; q < 3; ++q) {
+        double v = WaveTotal(out[q]);
+        if (lane == 0)
+            part[4 * wv + q] = v;
+    }
+    if (lane == 0)
+        part[4 * wv + 3] = (double)d;
+}
+
+// Derivative pass of the patch-wise entries (layout of k_history_pw).
+__global__ void __launch_bounds__(64) k_hgrad_pw(
+    const ElemGeo* __restrict__ geo, const ElemPoints* __restrict__ pts, const NodeRefs* __restrict__ refs,
+    PairGrid grid, const PwWave* __restrict__ waves, long w0, long w1, const long* __restrict__ off,
+    const int* __restrict__ kb, const int* __restrict__ len, const unsigned char* __restrict__ cls,
+    const unsigned char* __restrict__ alive, const double* __restrict__ C, const double* __restrict__ J,
+    const double* __restrict__ dT2, double* __restrict__ part)
+{
+    __shared__ double sgeo[kPwSlots * kNPWG2 * dev::kRecD];
+    __shared__ double spr[kPwSlots * 2 * kNPW2];
+    __shared__ int ssrc[kPwSlots];
+    using namespace dev;
+    long wi = w0 + (long)blockIdx.x;
+    if (wi >= w1)
+        return;
+    int lane = threadIdx.x;
+    const PwWave& W = waves[wi];
+    int nseg = W.nseg;
+    int s0, f, d;
+    DecodePair(grid, W.pair[0], s0, f, d);
+    const DynaMat& m = c_mats[d];
+    const ElemGeo& fe = geo[f];
+    const ElemPoints& P = pts[f];
+    if (lane < nseg) {
+        int ss, ff, dd;
+        DecodePair(grid, W.pair[lane], ss, ff, dd);
+        ssrc[lane] = ss;
+    }
+    int seg = -1, rank = 0, base = 0;
+    for (int i = 0; i < kPwSlots; ++i) {
+        int c = i < nseg ? W.cnt[i] : 0;
+        if (seg < 0 && lane >= base && lane < base + c) {
+            seg = i;
+            rank = W.rank0[i] + (lane - base);
+        }
+        base += c;
+    }
+    __syncthreads();
+    for (int idx = lane; idx < nseg * kNPW2; idx += 64) {
+        int slot = idx / kNPW2, i = idx - slot * kNPW2;
+        double r = Dist3(fe.rcenpw[i], geo[ssrc[slot]].center);
+        spr[slot * 2 * kNPW2 + 2 * i] = r + fe.radiusPW;
+        spr[slot * 2 * kNPW2 + 2 * i + 1] = r - fe.radiusPW;
+    }
+    bool active = seg >= 0;
+    int myseg = active ? seg : 0;
+    long p = W.pair[myseg];
+    int s = ssrc[myseg];
+    const double* src = geo[s].center;
+    double r0 = Dist3(fe.center, src);
+    int t = -1, k = 0;
+    EntryFlags fl{false, 0, 0};
+    if (active) {
+        int k0 = kb[p], L = len[p], c = 0;
+        long eb0 = off[p];
+        for (int tt = 0; tt < L; ++tt) {
+            EntryFlags e = JudgeAt(m, fe.radius, r0, k0 + tt, s == f, cls, eb0 + tt);
+            if (e.patchwise()) {
+                if (c == rank) {
+                    t = tt;
+                    fl = e;
+                    break;
+                }
+                ++c;
+            }
+        }
+        k = k0 + t;
+        if (t < 0)
+            active = false;
+    }
+    double n = (double)k;
+    long e = active ? off[p] + t : 0;
+    __syncthreads();
+    unsigned long long m2 = 0, m1 = 0;
+    {
+        double nn2 = n - 1, nn1 = n;
+        double lo2 = nn2 * m.C2Dt, hi2 = (nn2 + 1) * m.C1Dt;
+        double lo1 = nn1 * m.C2Dt, hi1 = (nn1 + 1) * m.C1Dt;
+        bool pw2 = active && fl.f2 == 1, pw1 = active && fl.f1 == 1;
+        const double* sp = spr + myseg * 2 * kNPW2;
+        for (int i = 0; i < kNPW2; ++i) {
+            double rpr = sp[2 * i], rmr = sp[2 * i + 1];
+            if (pw2 && !(rmr >= hi2 || rpr < lo2))
+                m2 |= 1ull << i;
+            if (pw1 && !(rmr >= hi1 || rpr < lo1))
+                m1 |= 1ull << i;
+        }
+    }
+    unsigned long long need = 0;
+    for (int i = 0; i < kNPW2; ++i)
+        if (WaveAny(((m2 | m1) >> i) & 1ull))
+            need |= 1ull << i;
+    EntryCorr kc;
+    LoadCorr(C, e, active, kc);
+    double gM[3] = {0.0, 0.0, 0.0}, gU[3] = {0.0, 0.0, 0.0};
+    Span s1 = MakeSpan(m, 1, n), s2 = MakeSpan(m, 2, n);
+    bool r2 = active && fl.f2 == 2, r1 = active && fl.f1 == 2, rt = r2 || r1;
+    double fr = (r2 ? 1.0 : 0.0) + (r1 ? 1.0 : 0.0);
+    double fu = r1 ? 1.0 : 0.0;
+    if (WaveAny(rt)) {
+        int rgSlot = lane >> 3, rgJ = lane & 7;
+        for (int half = 0; half < 2; ++half) {
+            if (rgSlot < nseg) {
+                int q = half * 8 + rgJ;
+                StageRecordD(m, geo[ssrc[rgSlot]].center, P.rg_pos[q], P.rg_nor[q], P.rg_w[q], sgeo + lane * kRecD);
+            }
+            __syncthreads();
+            const double* gs = sgeo + myseg * 8 * kRecD;
+#pragma unroll 1
+            for (int j = 0; j < 8; ++j) {
+                const double* g = gs + j * kRecD;
+                SpanDer a, b;
+                MixDerAt<1>(g, s1, rt, a);
+                MixDerAt<2>(g, s2, rt, b);
+                double xc, qc, bc, xt;
+                PointContract(g, kc, xc, qc, bc, xt);
+                for (int q = 0; q < 3; ++q) {
+                    gM[q] += fr * ((a.dX[q] + b.dX[q]) * xc + (a.dQ[q] + b.dQ[q]) * qc + (a.dB[q] + b.dB[q]) * bc);
+                    gU[q] += fu * (a.duX[q] * xt + a.duD[q] * kc.trT);
+                }
+            }
+            __syncthreads();
+        }
+    }
+    int stSlot = lane / kNPWG2, stJ = lane - stSlot * kNPWG2;
+    while (need) {
+        int i = __builtin_ctzll(need);
+        need &= need - 1;
+        if (stSlot < nseg) {
+            int q = i * kNPWG2 + stJ;
+            StageRecordD(m, geo[ssrc[stSlot]].center, P.pw_pos[q], P.pw_nor[q], P.pw_w[q], sgeo + lane * kRecD);
+        }
+        __syncthreads();
+        bool u2 = (m2 >> i) & 1ull, u1 = (m1 >> i) & 1ull;
+        const double* gs = sgeo + myseg * kNPWG2 * kRecD;
+#pragma unroll 1
+        for (int j = 0; j < kNPWG2; ++j) {
+            const double* g = gs + j * kRecD;
+            SpanDer a, b;
+            MixDerAt<2>(g, s2, u2, a);
+            MixDerAt<1>(g, s1, u1, b);
+            double xc, qc, bc, xt;
+            PointContract(g, kc, xc, qc, bc, xt);
+            for (int q = 0; q < 3; ++q) {
+                gM[q] += (a.dX[q] + b.dX[q]) * xc + (a.dQ[q] + b.dQ[q]) * qc + (a.dB[q] + b.dB[q]) * bc;
+                gU[q] += b.duX[q] * xt + b.duD[q] * kc.trT;
+            }
+        }
+        __syncthreads();
+    }
+    double out[3] = {0.0, 0.0, 0.0};
+    if (active)
+        FinishEntryGrad(kc, gM, gU, alive[e], refs[f].hu_sign, refs[f].ht_sign, fl.self1, dT2 + 27 * (long)f,
+                        J + 9 * d, out);
+    for (int q = 0; q < 3; ++q) {
+        double v = WaveTotal(out[q]);
+        if (lane == 0)
+            part[4 * wi + q] = v;
+    }
+    if (lane == 0)
+        part[4 * wi + 3] = (double)d;
+}
+
+// sums[3 d + p] += sum of the wave partials part[4 w + p] of domain d
+// (part[4 w + 3]); one block per (domain, parameter), fixed summation order.
+__global__ void __launch_bounds__(256) k_domain_sums(const double* __restrict__ part, long nw,
+                                                     double* __restrict__ sums)
+{
+    int d = blockIdx.x / 3, q = blockIdx.x % 3;
+    double a = 0.0;
+    for (long w = threadIdx.x; w < nw; w += blockDim.x)
+        if ((int)part[4 * w + 3] == d)
+            a += part[4 * w + q];
+    __shared__ double red[256];
+    red[threadIdx.x] = a;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        sums[blockIdx.x] += red[0];
+}
+
+void LaunchHistoryGrad(const ElemGeo* geo, const ElemPoints* pts, const NodeRefs* refs, PairGrid grid,
+                       const long* coff, const long* off, const int* kb, const int* len, const unsigned char* cls,
+                       long npairs, long chunks, const PwWave* waves, long nwaves, const unsigned char* alive,
+                       const double* C, const double* J, const double* dT2, double* part, hipStream_t s)
+{
+    const long step = 1L << 20;
+    for (long c0 = 0; c0 < chunks; c0 += step) {
+        long c1 = std::min(chunks, c0 + step);
+        k_hgrad_reg<<<(unsigned)(c1 - c0), 64, 0, s>>>(geo, pts, refs, grid, coff, off, kb, len, cls, npairs, c0, c1,
+                                                       alive, C, J, dT2, part);
+    }
+    for (long w0 = 0; w0 < nwaves; w0 += step) {
+        long w1 = std::min(nwaves, w0 + step);
+        k_hgrad_pw<<<(unsigned)(w1 - w0), 64, 0, s>>>(geo, pts, refs, grid, waves, w0, w1, off, kb, len, cls, alive,
+                                                      C, J, dT2, part + 4 * chunks);
+    }
+}
+
+void LaunchDomainSums(const double* part, long nw, int nd, double* sums, hipStream_t s)
+{
+    if (nw > 0)
+        k_domain_sums<<<3 * nd, 256, 0, s>>>(part, nw, sums);
+}
+
+
+// ---------------------------------------------------------------------------
 // Step-0 coefficient blocks (DBEM1/dsquareelement.cpp:2756-2868,3281-3394,
 // 4215-4550,5157-5491,5922-6078,6476-6632) for FlagDynaMethod 2:
 // U: 4 U(0) + U(1);  T: 4 T1(0) + T2(1) + T1(1);  then +-K * R^T.

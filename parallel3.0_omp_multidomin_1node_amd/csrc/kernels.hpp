@@ -83,6 +83,16 @@ struct PwWave {
 };
 // cls (nullable): frozen judges of every band entry (f2 | f1 << 2) used
 // instead of judging with the current materials.
+// Material-derivative pass of the history blocks contracted with the
+// per-entry correlation C (15 per entry): part[4 w + p] (and the domain in
+// part[4 w + 3]), w over the regular chunks then the patch-wise waves; J[9 d + 3 q + p] = d q / d (E, v, rho)_p
+// with q = (C1, C2, rho); dT2[27 f + 9 p + i] the self table derivatives.
+void LaunchHistoryGrad(const ElemGeo* geo, const ElemPoints* pts, const NodeRefs* refs, PairGrid grid,
+                       const long* coff, const long* off, const int* kb, const int* len, const unsigned char* cls,
+                       long npairs, long chunks, const PwWave* waves, long nwaves, const unsigned char* alive,
+                       const double* C, const double* J, const double* dT2, double* part, hipStream_t s);
+// sums[3 d + p] += the partials of the waves of domain d.
+void LaunchDomainSums(const double* part, long nw, int nd, double* sums, hipStream_t s);
 // PwWave packing per field element: count pass (waves null) -> nw[f], write
 // pass -> waves[woff[f] ..].
 void LaunchPwPack(PairGrid grid, const NodeRefs* refs, const int* cnt, int ne, const long* woff, long* nw,

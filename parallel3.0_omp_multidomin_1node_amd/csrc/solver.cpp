@@ -504,6 +504,35 @@ void Case::AssembleHistory(int maxn_in)
     historyDone = true;
 }
 
+void Case::PackPwWaves()
+{
+    if (frozen && pwPacked)
+        return;
+    const unsigned char* cls = frozen ? d_cls.p : nullptr;
+    DevBuf<int> d_pwcnt;
+    d_pwcnt.alloc((size_t)npairs);
+    LaunchPwCount(d_geo.p, grid, d_kb.p, d_len.p, d_off.p, cls, npairs, d_pwcnt.p, stream);
+    DevBuf<long> d_nw;
+    d_nw.alloc((size_t)ne);
+    LaunchPwPack(grid, d_refs.p, d_pwcnt.p, (int)ne, nullptr, d_nw.p, nullptr, stream);
+    std::vector<long> nw((size_t)ne), woff((size_t)ne);
+    Check(hipMemcpyAsync(nw.data(), d_nw.p, sizeof(long) * (size_t)ne, hipMemcpyDeviceToHost, stream), "pw pack");
+    Check(hipStreamSynchronize(stream), "pw pack");
+    // waves in field-element order (domains are contiguous element ranges)
+    long total = 0;
+    for (long f = 0; f < ne; ++f) {
+        woff[(size_t)f] = total;
+        total += nw[(size_t)f];
+    }
+    pwWaves = total;
+    DevBuf<long> d_woff;
+    d_woff.upload(woff.data(), woff.size(), stream);
+    d_pwWaves.alloc((size_t)std::max<long>(1, pwWaves));
+    LaunchPwPack(grid, d_refs.p, d_pwcnt.p, (int)ne, d_woff.p, nullptr, d_pwWaves.p, stream);
+    Check(hipStreamSynchronize(stream), "pw pack");
+    pwPacked = frozen;
+}
+
 // Staged history quadrature of the current bands with the current materials
 // into mts / mgs (FP64) or mts32 / mgs32 (FP32, plus the bf16 residuals of the
 // split mode); alive receives the presence bits.
@@ -511,33 +540,8 @@ void Case::StagedQuadrature(double* mts, double* mgs, float* mts32, float* mgs32
 {
     // patch-wise lags per pair, then waves that pack the patch-wise lags
     // of consecutive sources with the same field element
-    DevBuf<int> d_pwcnt;
     const unsigned char* cls = frozen ? d_cls.p : nullptr;
-    if (!(frozen && pwPacked)) {
-        d_pwcnt.alloc((size_t)npairs);
-        LaunchPwCount(d_geo.p, grid, d_kb.p, d_len.p, d_off.p, cls, npairs, d_pwcnt.p, stream);
-    }
-    if (!(frozen && pwPacked)) {
-        DevBuf<long> d_nw;
-        d_nw.alloc((size_t)ne);
-        LaunchPwPack(grid, d_refs.p, d_pwcnt.p, (int)ne, nullptr, d_nw.p, nullptr, stream);
-        std::vector<long> nw((size_t)ne), woff((size_t)ne);
-        Check(hipMemcpyAsync(nw.data(), d_nw.p, sizeof(long) * (size_t)ne, hipMemcpyDeviceToHost, stream), "pw pack");
-        Check(hipStreamSynchronize(stream), "pw pack");
-        // waves in field-element order (domains are contiguous element ranges)
-        long total = 0;
-        for (long f = 0; f < ne; ++f) {
-            woff[(size_t)f] = total;
-            total += nw[(size_t)f];
-        }
-        pwWaves = total;
-        DevBuf<long> d_woff;
-        d_woff.upload(woff.data(), woff.size(), stream);
-        d_pwWaves.alloc((size_t)std::max<long>(1, pwWaves));
-        LaunchPwPack(grid, d_refs.p, d_pwcnt.p, (int)ne, d_woff.p, nullptr, d_pwWaves.p, stream);
-        Check(hipStreamSynchronize(stream), "pw pack");
-        pwPacked = frozen;
-    }
+    PackPwWaves();
     hipEvent_t qa, qb;
     Check(hipEventCreate(&qa), "event");
     Check(hipEventCreate(&qb), "event");

@@ -87,6 +87,8 @@ public:
     // by the other kernel than quadStaged asks for (reference order when false).
     void EnsureSelfTables();
     void SetMaterials(const double* evr);
+    // Patch-wise waves of the current (or frozen) judges (cached while frozen).
+    void PackPwWaves();
     void ExportMatrix(int which, int step, std::vector<int>& rp, std::vector<int>& col, std::vector<double>& val);
     bool HistoryReady() const { return historyDone; }
     // Assembled history bands to a file / from a file written for the same
@@ -114,6 +116,7 @@ public:
     double timing[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     bool convTiled = true;   // time-blocked convolution (false: one full pass per step)
     int convTile = 16;       // steps per far-part tile (8 or 16); fixes the near lists built by the cull
+    bool adjointAnalytic = true;  // history-block derivatives analytic (false: operator differences)
     bool convFiltered = true;  // 16-step far tiles read a pre-filtered state series and prefetch work items
     int selfMode = -1;       // kernel of the current self tables: 1 wave per element, 0 reference order
     bool quadStaged = true;  // wave-per-pair quadrature with LDS-staged point geometry (false: lane per entry)

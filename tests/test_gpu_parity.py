@@ -440,8 +440,46 @@ def test_adjoint_time_blocked_transposed_term():
     assert np.abs(g1 - g0).max() <= 1e-9 * np.abs(g0).max()
 
 
+def test_adjoint_analytic_gradient_matches_small_step_differences():
+    """Analytic history-block derivatives (default): every (E, v, rho)
+    component against central differences of the misfit at a relative step of
+    1e-7 on the frozen structure (no quadrature point changes its front or
+    window case within such a step): agreement to 1e-6 of the largest
+    component (observed ~1e-8); and against the operator-difference mode at
+    1e-6 to 1e-5 (observed ~3e-7)."""
+    w, recv, obs, case = _adjoint_setup()
+    print("case:", w.name, "analytic adjoint vs small-step differences", flush=True)
+    c = case(w.mats)
+    J, ga, _ = c.adjoint(w.known, w.nstep, recv, obs, tol=1e-12, fd_step=1e-6)
+    c.set_option("adjoint_analytic", 0)
+    J2, gf, _ = c.adjoint(w.known, w.nstep, recv, obs, tol=1e-12, fd_step=1e-6)
+    c.set_option("adjoint_analytic", 1)
+    assert J == J2
+    evr = c.materials()
+    c.set_option("freeze_structure", 1)
+    gfd = np.zeros((4, 3))
+    for d in range(4):
+        for p in range(3):
+            dl = 1e-7 * max(abs(evr[d, p]), 0.1)
+            Js = []
+            for sgn in (1, -1):
+                e = evr.copy()
+                e[d, p] += sgn * dl
+                c.set_materials(e)
+                c.assemble_step0(10)
+                c.assemble_history(w.nstep)
+                U, *_ = c.solve(w.known, w.nstep, tol=1e-13)
+                Js.append(float(((U[1:, recv, :] - obs[1:]) ** 2).sum()))
+            gfd[d, p] = (Js[0] - Js[1]) / (2 * dl)
+    c.close()
+    print("analytic\n", ga, "\noperator differences\n", gf, "\nmisfit differences\n", gfd, flush=True)
+    assert np.abs(ga - gfd).max() <= 1e-6 * np.abs(gfd).max()
+    assert np.abs(ga - gf).max() <= 1e-5 * np.abs(gf).max()
+
+
 def test_adjoint_gradient_matches_same_step_differences():
-    """All (E, v, rho) components: the reference's time-integrated kernels jump
+    """Operator-difference mode (adjoint_analytic 0), all (E, v, rho)
+    components: the reference's time-integrated kernels jump
     where a quadrature point crosses a front window, so the misfit is only
     piecewise smooth in the wave speeds and its derivative depends on the
     difference step.  With the operators differenced at the same relative
@@ -449,6 +487,7 @@ def test_adjoint_gradient_matches_same_step_differences():
     central differences to second order (1e-2 of the largest component)."""
     w, recv, obs, case = _adjoint_setup()
     c = case(w.mats)
+    c.set_option("adjoint_analytic", 0)
     J, g, lu = c.adjoint(w.known, w.nstep, recv, obs, tol=1e-12, fd_step=1e-3)
     evr = c.materials()
     # the misfit's differences on the same frozen structure (bands and judges

@@ -20,7 +20,7 @@ class MisfitProblem:
     ids, time step, load history, receivers and observations."""
 
     def __init__(self, points, ele, bcid, dt, gap, domain_ids, itf_rows, known, nstep, receivers, obs,
-                 tol=1e-12, fd_step=1e-3, restart=200):
+                 tol=1e-12, fd_step=1e-6, restart=200):
         self.points, self.ele, self.bcid = points, ele, bcid
         self.dt, self.gap = float(dt), float(gap)
         self.domain_ids = np.asarray(domain_ids, np.float64)

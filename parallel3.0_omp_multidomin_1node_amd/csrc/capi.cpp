@@ -321,6 +321,10 @@ int tdbem_set_option(tdbem_ctx* ctx, const char* name, double value)
         ctx->c.convFiltered = value != 0.0;
         return 0;
     }
+    if (n == "conv_packed") {
+        ctx->c.convPacked = value != 0.0;
+        return 0;
+    }
     if (n == "mixed") {
         if (value != 0.0 && value != 1.0 && value != 2.0)
             return Fail("mixed must be 0, 1 or 2");

@@ -2845,20 +2845,278 @@ __global__ void __launch_bounds__(256) k_conv_far_p(PairGrid grid, const NodeRef
     }
 }
 
+namespace dev {
+
+constexpr int kFarGroup = 32;  // pairs per group table of a wavefront
+constexpr int kFarPairs = 4;   // pairs one packed item may span
+
+// One packed work item of the far convolution: up to 64 consecutive band
+// entries of a wavefront's group of pairs, one entry per lane, crossing pair
+// boundaries (at most kFarPairs pairs).  The item's pairs own consecutive
+// windows of nI + T - 1 filtered-state rows in LDS (nI = the pair's entries
+// in the item), ascending in time; lane l stages window slots l and l + 64.
+struct FarPack {
+    bool own;  // the lane holds an entry
+    int row;   // LDS row of the lane's step-K0 state
+    double M[9], G[6], r0[6], r1[6];
+};
+
+__device__ __forceinline__ int WaveInclusiveScan(int v, int lane)
+{
+    for (int o = 1; o < 64; o <<= 1) {
+        int u = __shfl_up(v, o, 64);
+        if (lane >= o)
+            v += u;
+    }
+    return v;
+}
+
+__device__ __forceinline__ int Uniform(int v)
+{
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+}  // namespace dev
+
+// The far part with entries packed across pairs: every wavefront takes
+// groups of 32 consecutive pairs of the row (groups wave, wave+4, ...), lists
+// the entries with lag <= jmax of the group in pair order and computes them
+// 64 at a time, so a lane idles only at the end of a group instead of at the
+// end of every band.  Per entry the arithmetic is the one of k_conv_far_p
+// (sixteen steps of MGS t~ - MTS u~ fused into the step accumulators); only
+// the assignment of entries to lanes differs, and with it the order of the
+// final sums.
+template <int T, typename TB>
+__global__ void __launch_bounds__(64) k_conv_far_q(PairGrid grid, const NodeRefs* __restrict__ refs,
+                                                   const long* __restrict__ off, const int* __restrict__ kb,
+                                                   const int* __restrict__ len, const TB* __restrict__ mts,
+                                                   const TB* __restrict__ mgs,
+                                                   const unsigned short* __restrict__ mtsLo,
+                                                   const unsigned short* __restrict__ mgsLo,
+                                                   const double* __restrict__ F, int fstride, int K0, int Tn,
+                                                   int maxn, int ne, double* __restrict__ hfar, long partStride)
+{
+    constexpr int P = dev::kFarPairs;
+    constexpr int kRows = 64 + P * (T - 1);
+    static_assert(kRows <= 128, "two window slots per lane");
+    __shared__ double fw[kRows * 6];
+    __shared__ int ex[dev::kFarGroup + 1];  // exclusive prefix of the clipped band lengths, then the total
+    __shared__ int gk[dev::kFarGroup];
+    __shared__ long go[dev::kFarGroup];
+    int s = blockIdx.x >> 2;      // source row
+    int wave = blockIdx.x & 3;    // quarter of the row: pair groups wave, wave + 4, ...
+    int d = refs[s].domain;
+    int nd = grid.eleCount[d];
+    int eb = grid.eleBegin[d];
+    long rowStart = grid.pairBase[d] + (long)(s - eb) * nd;
+    int lane = threadIdx.x;
+    int Kmax = K0 + Tn - 1;
+    int jmax = Kmax < maxn ? Kmax : maxn;
+    double acc[T][3];
+#pragma unroll
+    for (int i = 0; i < T; ++i)
+        acc[i][0] = acc[i][1] = acc[i][2] = 0.0;
+    // wavefront-uniform cursor: group g, next entry E of Gt, first pair i0 not yet finished
+    int g = wave - 4, E = 0, Gt = 0, i0 = 0;
+    auto loadGroup = [&]() -> bool {
+        for (;;) {
+            g += 4;
+            if (g * dev::kFarGroup >= nd)
+                return false;
+            int q = g * dev::kFarGroup + lane;
+            int L = 0, k0 = 0;
+            long o = 0;
+            if (lane < dev::kFarGroup && q < nd) {
+                long p = rowStart + q;
+                k0 = kb[p];
+                o = off[p];
+                int l = len[p];
+                int lj = jmax - k0 + 1;
+                L = lj < l ? lj : l;
+                L = L > 0 ? L : 0;
+            }
+            int incl = dev::WaveInclusiveScan(L, lane);
+            if (lane < dev::kFarGroup) {
+                ex[lane] = incl - L;
+                gk[lane] = k0;
+                go[lane] = o;
+            }
+            if (lane == dev::kFarGroup - 1)
+                ex[dev::kFarGroup] = incl;
+            Gt = dev::Uniform(__shfl(incl, dev::kFarGroup - 1, 64));
+            dev::WaveSync();
+            E = 0;
+            i0 = 0;
+            if (Gt > 0)
+                return true;
+        }
+    };
+    auto advance = [&](dev::FarPack& it) -> bool {
+        if (E >= Gt && !loadGroup())
+            return false;
+        while (ex[i0 + 1] <= E)
+            ++i0;
+        // the item's candidate pairs i0 .. i0+P-1, wavefront-uniform
+        int cEx[P + 1], cKb[P];
+        long cOff[P];
+#pragma unroll
+        for (int k = 0; k <= P; ++k) {
+            int c = i0 + k;
+            cEx[k] = dev::Uniform(ex[c <= dev::kFarGroup ? c : dev::kFarGroup]);
+        }
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            int c = i0 + k < dev::kFarGroup ? i0 + k : dev::kFarGroup - 1;
+            cKb[k] = dev::Uniform(gk[c]);
+            long o = go[c];
+            cOff[k] = ((long)(unsigned)dev::Uniform((int)(o >> 32)) << 32) | (unsigned)dev::Uniform((int)o);
+        }
+        int end = E + 64 < cEx[P] ? E + 64 : cEx[P];
+        // windows: pair k holds entries [first, last] of the item and rows wb[k] .. wb[k] + n + T - 2
+        int wb[P], tb[P], xl[P];
+        int rows = 0;
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            int first = cEx[k] > E ? cEx[k] : E;
+            int last = (cEx[k + 1] < end ? cEx[k + 1] : end) - 1;
+            int n = last - first + 1;
+            wb[k] = rows;
+            tb[k] = last - cEx[k];
+            xl[k] = K0 - cKb[k] - tb[k];
+            rows += n > 0 ? n + T - 1 : 0;
+        }
+        // the lane's entry e: the last pair starting at or before e
+        int e = E + lane;
+        it.own = e < end;
+        int ke = 0;
+#pragma unroll
+        for (int k = 1; k < P; ++k)
+            if (cEx[k] <= e)
+                ke = k;
+        int wbE = wb[0], tbE = tb[0], exE = cEx[0];
+        long offE = cOff[0];
+#pragma unroll
+        for (int k = 1; k < P; ++k)
+            if (ke == k) {
+                wbE = wb[k];
+                tbE = tb[k];
+                exE = cEx[k];
+                offE = cOff[k];
+            }
+        int t = e - exE;
+        it.row = wbE + tbE - t;
+        long en = it.own ? offE + t : 0;  // lanes without an entry read entry 0 (never used)
+#pragma unroll
+        for (int c = 0; c < 9; ++c)
+            it.M[c] = (double)mts[9 * en + c];
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+            it.G[c] = (double)mgs[6 * en + c];
+        if (mtsLo) {
+#pragma unroll
+            for (int c = 0; c < 9; ++c)
+                it.M[c] += dev::Bf16ToDouble(mtsLo[9 * en + c]);
+#pragma unroll
+            for (int c = 0; c < 6; ++c)
+                it.G[c] += dev::Bf16ToDouble(mgsLo[6 * en + c]);
+        }
+        // window slots lane and lane + 64: the last pair whose window starts at or before the slot
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            int sl = lane + 64 * h;
+            int ks = 0;
+#pragma unroll
+            for (int k = 1; k < P; ++k)
+                if (wb[k] <= sl)
+                    ks = k;
+            int wbS = wb[0], xlS = xl[0];
+#pragma unroll
+            for (int k = 1; k < P; ++k)
+                if (ks == k) {
+                    wbS = wb[k];
+                    xlS = xl[k];
+                }
+            int x = xlS + (sl - wbS);
+            bool in = sl < rows && x >= 0 && x <= K0;
+            const double* a = F + ((long)(in ? eb + g * dev::kFarGroup + i0 + ks : 0) * fstride + (in ? x : 0)) * 6;
+            double* r = h == 0 ? it.r0 : it.r1;
+#pragma unroll
+            for (int c = 0; c < 6; ++c)
+                r[c] = in ? a[c] : 0.0;
+        }
+        E = end;
+        return true;
+    };
+    auto body = [&](dev::FarPack& cur, dev::FarPack& nxt) -> bool {
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+            fw[6 * lane + c] = cur.r0[c];
+        if (lane + 64 < kRows)
+#pragma unroll
+            for (int c = 0; c < 6; ++c)
+                fw[6 * (lane + 64) + c] = cur.r1[c];
+        dev::WaveSync();
+        bool more = advance(nxt);
+        if (cur.own) {
+            const double* M = cur.M;
+            const double* G = cur.G;
+            const double* wr = fw + 6 * cur.row;
+#pragma unroll
+            for (int i = 0; i < T; ++i) {
+                const double* w = wr + 6 * i;
+                double u0 = w[0], u1 = w[1], u2 = w[2], t0v = w[3], t1v = w[4], t2v = w[5];
+                double a0 = acc[i][0], a1 = acc[i][1], a2 = acc[i][2];
+                a0 = fma(G[0], t0v, a0); a0 = fma(G[1], t1v, a0); a0 = fma(G[2], t2v, a0);
+                a1 = fma(G[1], t0v, a1); a1 = fma(G[3], t1v, a1); a1 = fma(G[4], t2v, a1);
+                a2 = fma(G[2], t0v, a2); a2 = fma(G[4], t1v, a2); a2 = fma(G[5], t2v, a2);
+                a0 = fma(-M[0], u0, a0); a0 = fma(-M[1], u1, a0); a0 = fma(-M[2], u2, a0);
+                a1 = fma(-M[3], u0, a1); a1 = fma(-M[4], u1, a1); a1 = fma(-M[5], u2, a1);
+                a2 = fma(-M[6], u0, a2); a2 = fma(-M[7], u1, a2); a2 = fma(-M[8], u2, a2);
+                acc[i][0] = a0;
+                acc[i][1] = a1;
+                acc[i][2] = a2;
+            }
+        }
+        dev::WaveSync();
+        return more;
+    };
+    dev::FarPack A, B;
+    bool more = advance(A);
+    while (more) {
+        if (!body(A, B))
+            break;
+        more = body(B, A);
+    }
+    // the quarter's partial sums; the near kernel adds the four quarters in order
+    double* out = hfar + wave * partStride + 3 * (long)s;
+#pragma unroll
+    for (int i = 0; i < T; ++i)
+        for (int c = 0; c < 3; ++c) {
+            double v = dev::WaveSum(acc[i][c]);
+            if (lane == 0 && i < Tn)
+                out[(long)i * 3 * ne + c] = v;
+        }
+}
+
 template <typename TB>
 void LaunchConvFarF(PairGrid grid, const NodeRefs* refs, const long* off, const int* kb, const int* len,
                     const TB* mts, const TB* mgs, const unsigned short* mtsLo, const unsigned short* mgsLo,
-                    const double* F, int fstride, int K0, int Tn, int maxn, int ne, double* hfar, hipStream_t s)
+                    const double* F, int fstride, int K0, int Tn, int maxn, int ne, double* hfar, bool packed,
+                    hipStream_t s)
 {
-    k_conv_far_p<16, TB><<<ne, 256, 0, s>>>(grid, refs, off, kb, len, mts, mgs, mtsLo, mgsLo, F, fstride, K0, Tn,
-                                            maxn, ne, hfar);
+    if (packed)
+        k_conv_far_q<16, TB><<<4 * ne, 64, 0, s>>>(grid, refs, off, kb, len, mts, mgs, mtsLo, mgsLo, F, fstride, K0,
+                                                   Tn, maxn, ne, hfar, (long)kConvTileMax * 3 * ne);
+    else
+        k_conv_far_p<16, TB><<<ne, 256, 0, s>>>(grid, refs, off, kb, len, mts, mgs, mtsLo, mgsLo, F, fstride, K0,
+                                                Tn, maxn, ne, hfar);
 }
 template void LaunchConvFarF<double>(PairGrid, const NodeRefs*, const long*, const int*, const int*, const double*,
                                      const double*, const unsigned short*, const unsigned short*, const double*, int,
-                                     int, int, int, int, double*, hipStream_t);
+                                     int, int, int, int, double*, bool, hipStream_t);
 template void LaunchConvFarF<float>(PairGrid, const NodeRefs*, const long*, const int*, const int*, const float*,
                                     const float*, const unsigned short*, const unsigned short*, const double*, int,
-                                    int, int, int, int, double*, hipStream_t);
+                                    int, int, int, int, double*, bool, hipStream_t);
 
 // Near part of step K in the tile starting at K0 plus the far part:
 // rhs_K += hfar[K-K0] + sum over near pairs (band start <= K-K0+1) of the
@@ -2873,7 +3131,8 @@ __global__ void __launch_bounds__(64) k_conv_near(const NodeRefs* __restrict__ r
                                                   const unsigned short* __restrict__ mgsLo,
                                                   const double* __restrict__ gU,
                                                   const double* __restrict__ gT, int nstore, int K, int K0, int maxn,
-                                                  int ne, const double* __restrict__ hfar, double* __restrict__ rhs)
+                                                  int ne, const double* __restrict__ hfar, int parts,
+                                                  long partStride, double* __restrict__ rhs)
 {
     int s = blockIdx.x;
     int lane = threadIdx.x;
@@ -2921,8 +3180,12 @@ __global__ void __launch_bounds__(64) k_conv_near(const NodeRefs* __restrict__ r
         y[c] = dev::WaveSum(y[c]);
     if (lane == 0) {
         const double* hf = hfar + (long)(K - K0) * 3 * ne + 3 * (long)s;
-        for (int c = 0; c < 3; ++c)
-            rhs[3 * (long)s + c] += hf[c] + y[c];
+        for (int c = 0; c < 3; ++c) {
+            double h = hf[c];
+            for (int q = 1; q < parts; ++q)
+                h += hf[q * partStride + c];
+            rhs[3 * (long)s + c] += h + y[c];
+        }
     }
 }
 
@@ -2943,10 +3206,11 @@ template <typename TB>
 void LaunchConvNear(const NodeRefs* refs, const long* nearPtr, const long* nearPair, const int* nearField,
                     const long* off, const int* kb, const int* len, const TB* mts, const TB* mgs,
                     const unsigned short* mtsLo, const unsigned short* mgsLo, const double* gU, const double* gT,
-                    int nstore, int K, int K0, int maxn, int ne, const double* hfar, double* rhs, hipStream_t s)
+                    int nstore, int K, int K0, int maxn, int ne, const double* hfar, int parts, double* rhs,
+                    hipStream_t s)
 {
     k_conv_near<TB><<<ne, 64, 0, s>>>(refs, nearPtr, nearPair, nearField, off, kb, len, mts, mgs, mtsLo, mgsLo, gU, gT,
-                                      nstore, K, K0, maxn, ne, hfar, rhs);
+                                      nstore, K, K0, maxn, ne, hfar, parts, (long)kConvTileMax * 3 * ne, rhs);
 }
 
 #define TDBEM_CONV_INSTANTIATE(TB)                                                                                   \
@@ -2955,8 +3219,8 @@ void LaunchConvNear(const NodeRefs* refs, const long* nearPtr, const long* nearP
                                     const double*, int, int, int, int, int, int, double*, hipStream_t);               \
     template void LaunchConvNear<TB>(const NodeRefs*, const long*, const long*, const int*, const long*, const int*,  \
                                      const int*, const TB*, const TB*, const unsigned short*, const unsigned short*,  \
-                                     const double*, const double*, int, int, int, int, int, const double*, double*,   \
-                                     hipStream_t);
+                                     const double*, const double*, int, int, int, int, int, const double*, int,       \
+                                     double*, hipStream_t);
 TDBEM_CONV_INSTANTIATE(double)
 TDBEM_CONV_INSTANTIATE(float)
 #undef TDBEM_CONV_INSTANTIATE

@@ -137,11 +137,15 @@ constexpr int kConvTileMax = 16;
 void LaunchFilterTile(const NodeRefs* refs, const double* gU, const double* gT, int nstore, int ne, int K0, int x0,
                       int x1, double* F, hipStream_t s);
 // Far part of a 16-step tile from the filtered series, next work item
-// prefetched.
+// prefetched; packed: 64 band entries per item across pair boundaries, one
+// wavefront per quarter of a row, four partial sums per row (hfar holds four
+// parts of kConvTileMax * 3 * ne values); otherwise 64 lags of one pair per
+// item and one sum per row.
 template <typename TB>
 void LaunchConvFarF(PairGrid grid, const NodeRefs* refs, const long* off, const int* kb, const int* len,
                     const TB* mts, const TB* mgs, const unsigned short* mtsLo, const unsigned short* mgsLo,
-                    const double* F, int fstride, int K0, int Tn, int maxn, int ne, double* hfar, hipStream_t s);
+                    const double* F, int fstride, int K0, int Tn, int maxn, int ne, double* hfar, bool packed,
+                    hipStream_t s);
 template <typename TB>
 void LaunchConvFar(PairGrid grid, const NodeRefs* refs, const long* off, const int* kb, const int* len, const TB* mts,
                    const TB* mgs, const unsigned short* mtsLo, const unsigned short* mgsLo, const double* gU,
@@ -151,7 +155,8 @@ template <typename TB>
 void LaunchConvNear(const NodeRefs* refs, const long* nearPtr, const long* nearPair, const int* nearField,
                     const long* off, const int* kb, const int* len, const TB* mts, const TB* mgs,
                     const unsigned short* mtsLo, const unsigned short* mgsLo, const double* gU, const double* gT,
-                    int nstore, int K, int K0, int maxn, int ne, const double* hfar, double* rhs, hipStream_t s);
+                    int nstore, int K, int K0, int maxn, int ne, const double* hfar, int parts, double* rhs,
+                    hipStream_t s);
 
 // GMRES linear algebra.  TA: matrix / inverse storage, TX / TY: vector
 // storage (double, or float for the inner solves of the mixed mode); every

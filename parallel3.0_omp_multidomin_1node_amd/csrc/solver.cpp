@@ -885,9 +885,10 @@ void Case::Solve(int nstep, const double* known_lu, long restart, double tol, do
     d_known.alloc(3 * (size_t)std::max<long>(1, dof.knownBlockCount));
     d_rhs.alloc((size_t)n3);
     d_x.alloc((size_t)n3);
-    d_hfar.alloc((size_t)kConvTileMax * (size_t)n3);
+    d_hfar.alloc(4 * (size_t)kConvTileMax * (size_t)n3);  // four partial sums per row (packed far part)
     DevBuf<double> d_F;
     const bool farF = convTiled && convFiltered && convTile == 16;
+    const int hfarParts = farF && convPacked ? 4 : 1;
     if (farF) {
         d_F.alloc((size_t)ne * (size_t)(nstore + 1) * 6);
         LaunchZero(d_F.p, (long)d_F.n, stream);
@@ -924,11 +925,11 @@ void Case::Solve(int nstep, const double* known_lu, long restart, double tol, do
                             LaunchConvFarF(grid, d_refs.p, d_off.p, d_kb.p, d_len.p, (const float*)d_mts32.p,
                                            (const float*)d_mgs32.p, (const unsigned short*)d_mtsLo.p,
                                            (const unsigned short*)d_mgsLo.p, d_F.p, nstore + 1, K0, Tn, maxn,
-                                           (int)ne, d_hfar.p, stream);
+                                           (int)ne, d_hfar.p, convPacked, stream);
                         else
                             LaunchConvFarF(grid, d_refs.p, d_off.p, d_kb.p, d_len.p, (const double*)d_mts.p,
                                            (const double*)d_mgs.p, nullptr, nullptr, d_F.p, nstore + 1, K0, Tn, maxn,
-                                           (int)ne, d_hfar.p, stream);
+                                           (int)ne, d_hfar.p, convPacked, stream);
                     } else if (historyMixed)
                         LaunchConvFar(grid, d_refs.p, d_off.p, d_kb.p, d_len.p, (const float*)d_mts32.p,
                                       (const float*)d_mgs32.p, (const unsigned short*)d_mtsLo.p,
@@ -949,11 +950,11 @@ void Case::Solve(int nstep, const double* known_lu, long restart, double tol, do
                     LaunchConvNear(d_refs.p, d_nearPtr.p, d_nearPair.p, d_nearField.p, d_off.p, d_kb.p, d_len.p,
                                    (const float*)d_mts32.p, (const float*)d_mgs32.p, (const unsigned short*)d_mtsLo.p,
                                    (const unsigned short*)d_mgsLo.p, d_gU.p, d_gT.p, nstore, K, K0, maxn, (int)ne,
-                                   d_hfar.p, d_rhs.p, stream);
+                                   d_hfar.p, hfarParts, d_rhs.p, stream);
                 else
                     LaunchConvNear(d_refs.p, d_nearPtr.p, d_nearPair.p, d_nearField.p, d_off.p, d_kb.p, d_len.p,
                                    (const double*)d_mts.p, (const double*)d_mgs.p, nullptr, nullptr, d_gU.p, d_gT.p,
-                                   nstore, K, K0, maxn, (int)ne, d_hfar.p, d_rhs.p, stream);
+                                   nstore, K, K0, maxn, (int)ne, d_hfar.p, hfarParts, d_rhs.p, stream);
             } else {
                 LaunchConvolution(grid, d_refs.p, d_off.p, d_kb.p, d_len.p, d_mts.p, d_mgs.p, d_gU.p, d_gT.p, nstore,
                                   K, maxn, (int)ne, d_rhs.p, stream);

@@ -118,6 +118,7 @@ public:
     int convTile = 16;       // steps per far-part tile (8 or 16); fixes the near lists built by the cull
     bool adjointAnalytic = true;  // history-block derivatives analytic (false: operator differences)
     bool convFiltered = true;  // 16-step far tiles read a pre-filtered state series and prefetch work items
+    bool convPacked = true;    // filtered far tiles pack 64 band entries per work item across pairs
     int selfMode = -1;       // kernel of the current self tables: 1 wave per element, 0 reference order
     bool quadStaged = true;  // wave-per-pair quadrature with LDS-staged point geometry (false: lane per entry)
     int mixed = 0;  // for the next history assembly: 0 FP64, 1 FP32 blocks, 2 FP32 + bf16 residual blocks

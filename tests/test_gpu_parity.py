@@ -236,8 +236,9 @@ def test_full_size_properties():
 def test_tiled_convolution_matches_per_step_sum(oracle_mod):
     """The time-blocked convolution (far part per 8- or 16-step tile + near
     part per step) reassociates the reference's per-step history sum; both give
-    the same solution to rounding.  The two 16-step far kernels agree bit for
-    bit."""
+    the same solution to rounding.  The two one-pair-per-item 16-step far
+    kernels agree bit for bit; the packed one (default) only reorders their
+    final sums."""
     tdbem = _tdbem()
     from tdbem import mesh
     w = mesh.layered_config2(h=0.5, nstep=45)
@@ -253,15 +254,43 @@ def test_tiled_convolution_matches_per_step_sum(oracle_mod):
     # 16-step tiles: the pre-filtered, prefetching far kernel (default) and the
     # one that stages and filters raw states add the same terms in the same order
     gc.set_option("conv_tile", 16)
+    gc.set_option("conv_packed", 0)
+    Up, Tp = gc.solve(w.known, w.nstep, tol=1e-13)[:2]
     gc.set_option("conv_filtered", 0)
     Ur, Tr = gc.solve(w.known, w.nstep, tol=1e-13)[:2]
-    assert np.array_equal(Ur, tiled[16][0]) and np.array_equal(Tr, tiled[16][1])
+    assert np.array_equal(Ur, Up) and np.array_equal(Tr, Tp)
+    assert np.abs(tiled[16][0] - Up).max() <= 1e-13 * np.abs(Up).max()
+    assert np.abs(tiled[16][1] - Tp).max() <= 1e-13 * np.abs(Tp).max()
     gc.set_option("conv_filtered", 1)
+    gc.set_option("conv_packed", 1)
     gc.set_option("conv_tiled", 0)
     U0, T0, *_ = gc.solve(w.known, w.nstep, tol=1e-13)
     for tile, (U1, T1) in tiled.items():
         assert np.abs(U1 - U0).max() <= 1e-11 * np.abs(U0).max(), tile
         assert np.abs(T1 - T0).max() <= 1e-11 * np.abs(T0).max(), tile
+
+
+def test_packed_far_convolution_on_long_rows():
+    """Rows of 256 pairs (eight groups of 32 per row, two per wavefront) and
+    bands up to the full history: the packed far kernel and the one-pair-per-
+    item kernel give the same solution to rounding, FP64 and FP32 + bf16."""
+    tdbem = _tdbem()
+    from tdbem import mesh
+    w = mesh.layered_config2(h=0.25, nstep=70)
+    m = w.mesh
+    print("case:", w.name, "packed vs one pair per item", flush=True)
+    for mixed in (0, 2):
+        gc = tdbem.Case(m.points, m.ele, m.bcid, w.dt, w.gap, domain_count=4, mat_rows=w.mats,
+                        itf_rows=m.interfaces)
+        gc.set_option("mixed", mixed)
+        gc.assemble_step0(10)
+        gc.assemble_history(w.nstep)
+        U1, T1, it1, *_ = gc.solve(w.known, w.nstep, tol=1e-13)
+        gc.set_option("conv_packed", 0)
+        U0, T0, it0, *_ = gc.solve(w.known, w.nstep, tol=1e-13)
+        gc.close()
+        assert np.abs(U1 - U0).max() <= 1e-12 * np.abs(U0).max(), mixed
+        assert np.abs(T1 - T0).max() <= 1e-12 * np.abs(T0).max(), mixed
 
 
 @pytest.mark.parametrize("which", ["regular_path_probe", "layered_small"])

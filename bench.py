@@ -61,10 +61,11 @@ def make_case(tdbem, w):
                       itf_rows=m.interfaces)
 
 
-def run_once(tdbem, w, restart=200, tol=1e-5, keep=False, tile=16):
+def run_once(tdbem, w, restart=200, tol=1e-5, keep=False, tile=16, gmres_device=1):
     t0 = time.perf_counter()
     c = make_case(tdbem, w)
     c.set_option("conv_tile", tile)
+    c.set_option("gmres_device", gmres_device)
     c.assemble_step0(10)
     entries = c.assemble_history(w.nstep)
     U, T, iters, flags, solved, times = c.solve(w.known, w.nstep, restart=restart, tol=tol)
@@ -243,6 +244,9 @@ def main():
     ap.add_argument("--h", type=float, default=0.1)
     ap.add_argument("--nstep", type=int, default=400)
     ap.add_argument("--conv-tile", type=int, default=16, choices=(8, 16), help="steps per far-part tile")
+    ap.add_argument("--gmres-device", type=int, default=1, choices=(0, 1, 2),
+                    help="GMRES driver: 0 host loop, 1 device state + hipGraphs (default), 2 device state + direct "
+                         "launches (for profiler runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the config-3 mixed-precision report")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -268,13 +272,14 @@ def main():
         copy_calibration(args.copy_calibration)
     log(f"workload {w.name}: {len(w.mesh.ele)} elements, {w.nstep} steps, dt={w.dt:.6g}")
     for i in range(args.warmup):
-        r = run_once(tdbem, w, tile=args.conv_tile)
+        r = run_once(tdbem, w, tile=args.conv_tile, gmres_device=args.gmres_device)
         log(f"warmup {i}: {r['wall']:.2f} s, entries {r['entries']}")
     tdbem.synchronize()
     t0 = time.perf_counter()
     runs = []
     for i in range(args.steps):
-        runs.append(run_once(tdbem, w, keep=(i == args.steps - 1), tile=args.conv_tile))
+        runs.append(run_once(tdbem, w, keep=(i == args.steps - 1), tile=args.conv_tile,
+                             gmres_device=args.gmres_device))
         log(f"step {i}: {runs[-1]['wall']:.2f} s")
     tdbem.synchronize()
     elapsed = time.perf_counter() - t0

@@ -325,6 +325,12 @@ int tdbem_set_option(tdbem_ctx* ctx, const char* name, double value)
         ctx->c.convPacked = value != 0.0;
         return 0;
     }
+    if (n == "gmres_device") {
+        if (value != 0.0 && value != 1.0 && value != 2.0)
+            return Fail("gmres_device must be 0, 1 or 2");
+        ctx->c.gmresDevice = (int)value;
+        return 0;
+    }
     if (n == "mixed") {
         if (value != 0.0 && value != 1.0 && value != 2.0)
             return Fail("mixed must be 0, 1 or 2");

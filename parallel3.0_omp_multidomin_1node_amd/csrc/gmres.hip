@@ -1,0 +1,371 @@
+// Device-driven GMRES of one time step (DBEM1/pre_gmres.cpp:1369-1605 with
+// maxit = 1 and x0 = 0), FP64.  The iteration counter, the convergence test
+// and the stopping flags live in device memory (GmState), so one iteration is
+// a fixed sequence of kernels with fixed arguments: it is captured once in a
+// hipGraph and replayed, and the host reads the state back once per time
+// step.  Every kernel returns at once when the state says the solve is done,
+// so iterations launched past convergence do no work.  The arithmetic, the
+// reduction partition and the summation orders are those of the host-driven
+// kernels in kernels.hip, so both drivers give the same iterates bit for bit.
+#include "gmres.hpp"
+
+namespace tdbem {
+
+namespace {
+
+// a null state never reads as done (the kernels that open a solve)
+__device__ __forceinline__ bool Done(const GmState* st)
+{
+    return st && __builtin_amdgcn_readfirstlane(st->done) != 0;
+}
+
+// the next iteration's column count j (1-based)
+__device__ __forceinline__ int NextJ(const GmState* st)
+{
+    return __builtin_amdgcn_readfirstlane(st->j) + 1;
+}
+
+// sum of the block partials of column k in block order, loads eight at a time
+// (the order of k_finish)
+__device__ double SumPartials(const double* __restrict__ partial, int nblocks, int nv, int k)
+{
+    double a = 0.0;
+    int b = 0;
+    for (; b + 8 <= nblocks; b += 8) {
+        double t[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            t[i] = partial[(long)(b + i) * nv + k];
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            a += t[i];
+    }
+    for (; b < nblocks; ++b)
+        a += partial[(long)b * nv + k];
+    return a;
+}
+
+}  // namespace
+
+// bnorm = ||b|| from the partials of b.b; x0 = 0 so r = b and ||r|| = bnorm.
+// Done at once for b = 0 or when bnorm <= tol * bnorm.
+__global__ void k_gm_begin(const double* __restrict__ partial, int nblocks, double tol, GmState* st, int* stag)
+{
+    if (threadIdx.x != 0)
+        return;
+    double bn = sqrt(SumPartials(partial, nblocks, 1, 0));
+    st->bnorm = bn;
+    st->normr = bn;
+    st->tolb = tol * bn;
+    st->j = 0;
+    st->flag = 0;
+    st->done = (bn == 0.0 || bn <= tol * bn) ? 1 : 0;
+    stag[0] = 0;
+}
+
+__global__ void __launch_bounds__(256) k_gm_fill(const GmState* st, double* __restrict__ a, long n, double* __restrict__ b,
+                                                 long m)
+{
+    if (Done(st))
+        return;
+    long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n)
+        a[r] = 0.0;
+    if (r < m)
+        b[r] = 0.0;
+}
+
+// y = A x with x = V(:, st->j + colOff) (colOff < 0: x itself), or the
+// residual y = b - A x when b is given
+__global__ void __launch_bounds__(256) k_gm_spmv(const GmState* st, long nrows, const int* __restrict__ rp,
+                                                 const int* __restrict__ col, const double* __restrict__ val,
+                                                 const double* __restrict__ x, long ldx, int colOff,
+                                                 const double* __restrict__ b, double* __restrict__ y)
+{
+    if (Done(st))
+        return;
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nrows)
+        return;
+    if (colOff >= 0)
+        x += (long)(NextJ(st) - 1 + colOff) * ldx;
+    double b0 = 0.0, b1 = 0.0, b2 = 0.0;
+    for (int j = rp[i]; j < rp[i + 1]; ++j) {
+        const double* v = val + 9 * (long)j;
+        const double* xc = x + 3 * (long)col[j];
+        double x0 = xc[0], x1 = xc[1], x2 = xc[2];
+        b0 += v[0] * x0 + v[1] * x1 + v[2] * x2;
+        b1 += v[3] * x0 + v[4] * x1 + v[5] * x2;
+        b2 += v[6] * x0 + v[7] * x1 + v[8] * x2;
+    }
+    if (b) {
+        b0 = b[3 * i] - b0;
+        b1 = b[3 * i + 1] - b1;
+        b2 = b[3 * i + 2] - b2;
+    }
+    y[3 * i] = b0;
+    y[3 * i + 1] = b1;
+    y[3 * i + 2] = b2;
+}
+
+__global__ void __launch_bounds__(256) k_gm_precond(const GmState* st, long n, const long* __restrict__ posLeaf,
+                                                    const long* __restrict__ leafBegin,
+                                                    const long* __restrict__ leafSize, const long* __restrict__ matOff,
+                                                    const double* __restrict__ inv, const long* __restrict__ repid,
+                                                    const long* __restrict__ outBlock, const double* __restrict__ x,
+                                                    double* __restrict__ y)
+{
+    if (Done(st))
+        return;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 3 * n)
+        return;
+    long i = t / 3;
+    int mrow = (int)(t % 3);
+    long leaf = posLeaf[i];
+    long b = leafBegin[leaf], c = leafSize[leaf], jl = i - b, n3 = 3 * c;
+    const double* P = inv + matOff[leaf] + (3 * jl + mrow) * n3;
+    double acc = 0.0;
+    for (long k = 0; k < c; ++k) {
+        const double* xc = x + 3 * repid[b + k];
+        acc += P[3 * k] * xc[0];
+        acc += P[3 * k + 1] * xc[1];
+        acc += P[3 * k + 2] * xc[2];
+    }
+    y[3 * outBlock[i] + mrow] = acc;
+}
+
+// block partials of V(:, 0 .. nv-1) . u with nv = j (all) or 1 (V = u: norm)
+__global__ void __launch_bounds__(256) k_gm_multidot(const GmState* st, const double* __restrict__ V, long ldv,
+                                                     int all, const double* __restrict__ u, long n,
+                                                     double* __restrict__ partial)
+{
+    if (Done(st))
+        return;
+    int nv = all ? NextJ(st) : 1;
+    __shared__ double red[256];
+    for (int k = 0; k < nv; ++k) {
+        double a = 0.0;
+        for (long r = (long)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (long)gridDim.x * blockDim.x)
+            a += V[k * ldv + r] * u[r];
+        red[threadIdx.x] = a;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w)
+                red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0)
+            partial[(long)blockIdx.x * nv + k] = red[0];
+        __syncthreads();
+    }
+}
+
+// h[off + k] = sum of the partials of column k, k < j (all) or k = 0 with
+// the square root written to h[st->j + off] (norm)
+__global__ void k_gm_finish(const GmState* st, const double* __restrict__ partial, int nblocks, int all,
+                            double* __restrict__ h, int off)
+{
+    if (Done(st))
+        return;
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (all) {
+        int nv = NextJ(st);
+        if (k < nv)
+            h[off + k] = SumPartials(partial, nblocks, nv, k);
+    } else if (k == 0) {
+        h[NextJ(st) - 1 + off] = sqrt(SumPartials(partial, nblocks, 1, 0));
+    }
+}
+
+// u -= V(:, 0 .. j-1) h(0 .. j-1); htot[0 .. j-1] += h when given
+__global__ void __launch_bounds__(256) k_gm_multiaxpy(const GmState* st, const double* __restrict__ V, long ldv,
+                                                      const double* __restrict__ h, double* __restrict__ u, long n,
+                                                      double* htot)
+{
+    if (Done(st))
+        return;
+    int nv = NextJ(st);
+    long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (htot && r < nv)
+        htot[r] += h[r];
+    if (r >= n)
+        return;
+    double a = u[r];
+    for (int k = 0; k < nv; ++k)
+        a -= h[k] * V[k * ldv + r];
+    u[r] = a;
+}
+
+// V(:, j - 1 + colOff) = u / h[j - 1 + denOff] (tiny denominators give 0)
+__global__ void __launch_bounds__(256) k_gm_scale(const GmState* st, const double* __restrict__ u,
+                                                  const double* __restrict__ h, int denOff, double* __restrict__ V,
+                                                  int colOff, long n)
+{
+    if (Done(st))
+        return;
+    long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n)
+        return;
+    int j = NextJ(st);
+    double s = h[j - 1 + denOff];
+    V[(long)(j - 1 + colOff) * n + r] = s <= 1.0e-12 ? 0.0 : u[r] / s;
+}
+
+// QT(1,1) = 1, phibar = h[1] (start of the Givens sequence)
+__global__ void k_gm_start(const GmState* st, double* QT, const double* h, double* phibar)
+{
+    if (Done(st) || threadIdx.x != 0)
+        return;
+    QT[0] = 1.0;
+    phibar[0] = h[1];
+}
+
+// Givens rotation in QT form for column j (DBEM1/pre_gmres.cpp:1492-1533)
+__global__ void __launch_bounds__(256) k_gm_givens(const GmState* st, double* h, double* QT, double* R, double* f,
+                                                   double* phibar, int* stag, int restart)
+{
+    if (Done(st))
+        return;
+    const int j = NextJ(st);
+    const int lq = restart + 1;
+#define QQ(a, b) QT[((a)-1) * lq + ((b)-1)]
+#define RR(a, b) R[((a)-1) * restart + ((b)-1)]
+    for (int k = 1 + (int)threadIdx.x; k <= j; k += blockDim.x) {
+        double v = 0.0;
+        for (int l = 1; l <= j; ++l)
+            v += QQ(k, l) * h[l];
+        RR(k, j) = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double rt = RR(j, j), c, s, temp;
+        double hj1 = h[j + 1];
+        if (hj1 == 0.0) {
+            c = 1.0;
+            s = 0.0;
+        } else if (hj1 * hj1 > rt * rt) {
+            temp = rt / hj1;
+            s = 1.0 / sqrt(1.0 + fabs(temp) * fabs(temp));
+            c = -temp * s;
+        } else {
+            temp = hj1 / rt;
+            c = 1.0 / sqrt(1.0 + fabs(temp) * fabs(temp));
+            s = -temp * c;
+        }
+        RR(j, j) = c * rt - s * hj1;
+        for (int k = 1; k <= j; ++k) {
+            double qk = QQ(j, k);
+            QQ(j, k) = c * qk;
+            QQ(j + 1, k) = s * qk;
+        }
+        QQ(j, j + 1) = -s;
+        QQ(j + 1, j + 1) = c;
+        f[j] = c * phibar[0];
+        phibar[0] = s * phibar[0];
+        if (j < restart && f[j] == 0.0)
+            stag[0] = 1;
+    }
+#undef QQ
+#undef RR
+}
+
+// W(:,j) = (V(:,j) - sum_{l<j} W(:,l) R(l,j)) / R(j,j);  x += f_j W(:,j)
+// (1-based columns), together with the next Krylov vector
+// V(:, j+1) = u / h[j+1] (the scale of k_gm_scale)
+__global__ void __launch_bounds__(256) k_gm_wupdate(const GmState* st, double* __restrict__ V,
+                                                    double* __restrict__ W, long n, const double* __restrict__ R,
+                                                    int restart, const double* __restrict__ f, double* __restrict__ x,
+                                                    const double* __restrict__ u, const double* __restrict__ h)
+{
+    if (Done(st))
+        return;
+    long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n)
+        return;
+    int j = NextJ(st);
+    double sc = h[j + 1];
+    V[(long)j * n + k] = sc <= 1.0e-12 ? 0.0 : u[k] / sc;
+    double rjj = R[(j - 1) * restart + (j - 1)];
+    double w = V[(long)(j - 1) * n + k] / rjj;
+    for (int l = 1; l <= j - 1; ++l)
+        w -= W[(long)(l - 1) * n + k] * R[(l - 1) * restart + (j - 1)] / rjj;
+    W[(long)(j - 1) * n + k] = w;
+    x[k] += f[j] * w;
+}
+
+// ||r|| of iteration j from the partials of r.r, then the reference's tests:
+// converged (flag 0), stagnation (3) or the restart length reached (1)
+__global__ void k_gm_check(GmState* st, const double* __restrict__ partial, int nblocks, const int* stag,
+                           int restart)
+{
+    if (Done(st) || threadIdx.x != 0)
+        return;
+    int j = st->j + 1;
+    double nr = sqrt(SumPartials(partial, nblocks, 1, 0));
+    st->normr = nr;
+    st->j = j;
+    if (nr <= st->tolb) {
+        st->flag = 0;
+        st->done = 1;
+    } else if (stag[0]) {
+        st->flag = 3;
+        st->done = 1;
+    } else if (j >= restart) {
+        st->flag = 1;
+        st->done = 1;
+    }
+}
+
+namespace {
+
+unsigned Blocks(long n)
+{
+    return (unsigned)((n + 255) / 256);
+}
+
+}  // namespace
+
+void LaunchGmresBegin(const GmresDevArgs& a, const double* b, double tol, hipStream_t s)
+{
+    const long n = 3 * a.ne;
+    // ||b|| partials (the partition of LaunchNorm), x = 0
+    k_gm_multidot<<<a.nblocks, 256, 0, s>>>(nullptr, b, n, 0, b, n, a.part);
+    k_gm_fill<<<Blocks(n), 256, 0, s>>>(nullptr, a.x, n, a.x, 0);
+    k_gm_begin<<<1, 64, 0, s>>>(a.part, a.nblocks, tol, a.st, a.stag);
+    k_gm_fill<<<Blocks(std::max(a.nQT, a.nR)), 256, 0, s>>>(a.st, a.QT, a.nQT, a.R, a.nR);
+    // u = M^-1 r with r = b; h[1] = ||u||; V(:,0) = u / h[1]
+    k_gm_precond<<<Blocks(n), 256, 0, s>>>(a.st, a.ne, a.posLeaf, a.leafBegin, a.leafSize, a.matOff, a.inv, a.repid,
+                                           a.outBlock, b, a.u);
+    k_gm_multidot<<<a.nblocks, 256, 0, s>>>(a.st, a.u, n, 0, a.u, n, a.part);
+    k_gm_finish<<<1, 64, 0, s>>>(a.st, a.part, a.nblocks, 0, a.h, 1);
+    k_gm_scale<<<Blocks(n), 256, 0, s>>>(a.st, a.u, a.h, 1, a.V, 0, n);
+    k_gm_start<<<1, 64, 0, s>>>(a.st, a.QT, a.h, a.phibar);
+}
+
+void LaunchGmresIteration(const GmresDevArgs& a, const double* b, hipStream_t s)
+{
+    const long n = 3 * a.ne;
+    const unsigned fin = (unsigned)((a.restart + 63) / 64);
+    k_gm_spmv<<<Blocks(a.ne), 256, 0, s>>>(a.st, a.ne, a.rp, a.col, a.val, a.V, n, 0, nullptr, a.u2);
+    k_gm_precond<<<Blocks(n), 256, 0, s>>>(a.st, a.ne, a.posLeaf, a.leafBegin, a.leafSize, a.matOff, a.inv, a.repid,
+                                           a.outBlock, a.u2, a.u);
+    // classical Gram-Schmidt twice
+    k_gm_multidot<<<a.nblocks, 256, 0, s>>>(a.st, a.V, n, 1, a.u, n, a.part);
+    k_gm_finish<<<fin, 64, 0, s>>>(a.st, a.part, a.nblocks, 1, a.h, 1);
+    k_gm_multiaxpy<<<Blocks(std::max(n, a.restart)), 256, 0, s>>>(a.st, a.V, n, a.h + 1, a.u, n, nullptr);
+    k_gm_multidot<<<a.nblocks, 256, 0, s>>>(a.st, a.V, n, 1, a.u, n, a.part);
+    k_gm_finish<<<fin, 64, 0, s>>>(a.st, a.part, a.nblocks, 1, a.h2, 0);
+    k_gm_multiaxpy<<<Blocks(std::max(n, a.restart)), 256, 0, s>>>(a.st, a.V, n, a.h2, a.u, n, a.h + 1);
+    // h[j+1] = ||u||; Givens; V(:,j) = u / h[j+1] with the W update and x
+    k_gm_multidot<<<a.nblocks, 256, 0, s>>>(a.st, a.u, n, 0, a.u, n, a.part);
+    k_gm_finish<<<1, 64, 0, s>>>(a.st, a.part, a.nblocks, 0, a.h, 2);
+    k_gm_givens<<<1, 256, 0, s>>>(a.st, a.h, a.QT, a.R, a.f, a.phibar, a.stag, (int)a.restart);
+    k_gm_wupdate<<<Blocks(n), 256, 0, s>>>(a.st, a.V, a.W, n, a.R, (int)a.restart, a.f, a.x, a.u, a.h);
+    // true residual b - A x and its norm
+    k_gm_spmv<<<Blocks(a.ne), 256, 0, s>>>(a.st, a.ne, a.rp, a.col, a.val, a.x, n, -1, b, a.vh);
+    k_gm_multidot<<<a.nblocks, 256, 0, s>>>(a.st, a.vh, n, 0, a.vh, n, a.part);
+    k_gm_check<<<1, 64, 0, s>>>(a.st, a.part, a.nblocks, a.stag, (int)a.restart);
+}
+
+}  // namespace tdbem

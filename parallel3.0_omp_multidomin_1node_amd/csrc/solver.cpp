@@ -4,6 +4,7 @@
 #include "solver.hpp"
 
 #include <algorithm>
+#include <array>
 #include <type_traits>
 #include <chrono>
 #include <cmath>
@@ -26,6 +27,38 @@ double Now()
 {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
+
+namespace {
+
+// Events of one solve: created on demand, timed after the stream has drained,
+// destroyed with the pool.
+class EventPool {
+public:
+    ~EventPool()
+    {
+        for (hipEvent_t e : all)
+            (void)hipEventDestroy(e);
+    }
+    hipEvent_t Get()
+    {
+        hipEvent_t e = nullptr;
+        Check(hipEventCreate(&e), "event");
+        all.push_back(e);
+        return e;
+    }
+    double Elapsed(hipEvent_t a, hipEvent_t b)
+    {
+        float ms = 0.f;
+        Check(hipEventSynchronize(b), "event");
+        Check(hipEventElapsedTime(&ms, a, b), "elapsed");
+        return ms * 1e-3;
+    }
+
+private:
+    std::vector<hipEvent_t> all;
+};
+
+}  // namespace
 
 }  // namespace
 
@@ -70,6 +103,9 @@ template struct DevBuf<unsigned short>;
 
 Case::~Case()
 {
+    ReleaseGraphs();
+    if (h_gst)
+        (void)hipHostFree(h_gst);
     if (h_pin)
         (void)hipHostFree(h_pin);
     if (stream)
@@ -694,7 +730,8 @@ void Case::PrepareGmres(long restart)
     d_f.alloc((size_t)restart + 2);
     d_scal.alloc(4);
     d_stag.alloc(1);
-    gm_blocks = (int)std::min<long>(128, std::max<long>(1, (n + 255) / 256));
+    // reduction partition: few blocks keep the in-order finish short
+    gm_blocks = (int)std::min<long>(32, std::max<long>(1, (n + 255) / 256));
     d_part.alloc((size_t)gm_blocks * (size_t)(restart + 1));
     if (!h_pin)
         Check(hipHostMalloc((void**)&h_pin, 4 * sizeof(double), hipHostMallocDefault), "pinned");
@@ -809,6 +846,108 @@ int Case::Gmres(const double* d_b, double* d_x, long restart, double tol, int* i
     return GmresCore<double>(d_b, d_x, restart, tol, iters, op, d_V.p, d_W.p, d_u.p, d_u2.p);
 }
 
+// The same solve driven from the device (gmres.hip): the opening kernels and
+// one iteration are captured once as hipGraphs (gmresDevice 1) or launched
+// directly (2); iterations are queued ahead, as many as the previous step
+// needed, and the host reads the device state once they have run, queueing
+// two more at a time until the state says done.
+int Case::GmresDevice(const double* d_b, double* d_x, long restart, double tol, int* iters)
+{
+    if (!d_gst.p) {
+        d_gst.alloc((sizeof(GmState) + sizeof(double) - 1) / sizeof(double));
+        Check(hipHostMalloc((void**)&h_gst, sizeof(GmState), hipHostMallocDefault), "pinned");
+    }
+    GmresDevArgs a{};
+    a.ne = ne;
+    a.restart = restart;
+    a.nblocks = gm_blocks;
+    a.rp = d_Arp.p;
+    a.col = d_Acol.p;
+    a.val = d_Aval.p;
+    a.posLeaf = d_posLeaf.p;
+    a.leafBegin = d_leafBegin.p;
+    a.leafSize = d_leafSize.p;
+    a.matOff = d_matOff.p;
+    a.repid = d_repid.p;
+    a.outBlock = d_outBlock.p;
+    a.inv = d_inv.p;
+    a.V = d_V.p;
+    a.W = d_W.p;
+    a.u = d_u.p;
+    a.u2 = d_u2.p;
+    a.vh = d_vh.p;
+    a.h = d_h.p;
+    a.h2 = d_h2.p;
+    a.QT = d_QT.p;
+    a.R = d_R.p;
+    a.f = d_f.p;
+    a.phibar = d_scal.p + 2;
+    a.part = d_part.p;
+    a.x = d_x;
+    a.nQT = (long)d_QT.n;
+    a.nR = (long)d_R.n;
+    a.stag = d_stag.p;
+    a.st = reinterpret_cast<GmState*>(d_gst.p);
+    const bool useGraph = gmresDevice == 1;
+    if (useGraph && (graphB != d_b || graphX != d_x || graphTol != tol || graphRestart != restart ||
+                     graphV != d_V.p)) {
+        ReleaseGraphs();
+        hipGraph_t g = nullptr;
+        Check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "capture");
+        LaunchGmresBegin(a, d_b, tol, stream);
+        Check(hipStreamEndCapture(stream, &g), "capture");
+        Check(hipGraphInstantiate(&graphBegin, g, nullptr, nullptr, 0), "instantiate");
+        Check(hipGraphDestroy(g), "graph");
+        Check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "capture");
+        LaunchGmresIteration(a, d_b, stream);
+        Check(hipStreamEndCapture(stream, &g), "capture");
+        Check(hipGraphInstantiate(&graphIter, g, nullptr, nullptr, 0), "instantiate");
+        Check(hipGraphDestroy(g), "graph");
+        graphB = d_b;
+        graphX = d_x;
+        graphTol = tol;
+        graphRestart = restart;
+        graphV = d_V.p;
+    }
+    if (useGraph)
+        Check(hipGraphLaunch(graphBegin, stream), "graph launch");
+    else
+        LaunchGmresBegin(a, d_b, tol, stream);
+    long queued = 0;
+    long ahead = std::max(1, gm_lastIters);
+    for (;;) {
+        long k = std::min(ahead, restart - queued);
+        for (long i = 0; i < k; ++i) {
+            if (useGraph)
+                Check(hipGraphLaunch(graphIter, stream), "graph launch");
+            else
+                LaunchGmresIteration(a, d_b, stream);
+        }
+        queued += k;
+        Check(hipMemcpyAsync(h_gst, d_gst.p, sizeof(GmState), hipMemcpyDeviceToHost, stream), "d2h");
+        Check(hipStreamSynchronize(stream), "gmres");
+        if (h_gst->done || queued >= restart)
+            break;
+        ahead = 2;
+    }
+    if (!h_gst->done)
+        throw std::runtime_error("device GMRES ended without a stopping state");
+    *iters = h_gst->j;
+    gm_lastIters = h_gst->j;
+    return h_gst->flag;
+}
+
+void Case::ReleaseGraphs()
+{
+    if (graphBegin)
+        (void)hipGraphExecDestroy(graphBegin);
+    if (graphIter)
+        (void)hipGraphExecDestroy(graphIter);
+    graphBegin = graphIter = nullptr;
+    graphB = nullptr;
+    graphX = nullptr;
+}
+
 // Mixed precision: FP32-stored inner GMRES (GmresCore<float>) on A d = r
 // under an FP64 iterative-refinement residual r = b - A x with the FP64 A,
 // until ||r|| <= tol ||b|| (the reference's stopping test,
@@ -862,6 +1001,8 @@ void Case::Solve(int nstep, const double* known_lu, long restart, double tol, do
     if (!historyDone || maxn < 1)
         throw std::runtime_error("history not assembled");
     PrepareGmres(restart);
+    ReleaseGraphs();  // captured per solve: the operands may have moved since the last one
+    gm_lastIters = 1;
     if (historyMixed)
         PrepareMixed(restart);
     if (historyMixed && !convTiled)
@@ -898,26 +1039,26 @@ void Case::Solve(int nstep, const double* known_lu, long restart, double tol, do
     LaunchZero(d_known.p, (long)d_known.n, stream);
     LaunchZero(d_lu.p, (long)d_lu.n, stream);
     LaunchZero(d_lt.p, (long)d_lt.n, stream);
-    hipEvent_t c0, c1;
-    Check(hipEventCreate(&c0), "event");
-    Check(hipEventCreate(&c1), "event");
-    double convTime = 0.0, gmresTime = 0.0, farTime = 0.0;
-    int farLaunches = 0;
-    hipEvent_t f0, f1;
-    Check(hipEventCreate(&f0), "event");
-    Check(hipEventCreate(&f1), "event");
+    // per-step events (convolution start, GMRES start, GMRES end) and per-tile
+    // far-kernel events, read once after the loop
+    EventPool ev;
+    std::vector<std::array<hipEvent_t, 3>> stepEv;
+    std::vector<std::array<hipEvent_t, 2>> farEv;
+    stepEv.reserve((size_t)nstep);
     *solved = 0;
     int K0 = 2;
     for (int K = 1; K <= nstep; ++K) {
         LaunchKnown(d_refs.p, d_bd.p + (size_t)K * (size_t)n3, (int)ne, d_known.p, dof.knownBlockCount, stream);
         LaunchSpmv(ne, d_Krp.p, d_Kcol.p, d_Kval.p, d_known.p, d_rhs.p, stream);
+        std::array<hipEvent_t, 3> se{ev.Get(), ev.Get(), ev.Get()};
+        Check(hipEventRecord(se[0], stream), "event");
         if (K > 1) {
-            Check(hipEventRecord(c0, stream), "event");
             if (convTiled) {
                 if (K == 2 || K - K0 >= convTile) {
                     K0 = K;
                     int Tn = std::min(convTile, nstep - K0 + 1);
-                    Check(hipEventRecord(f0, stream), "event");
+                    std::array<hipEvent_t, 2> fe{ev.Get(), ev.Get()};
+                    Check(hipEventRecord(fe[0], stream), "event");
                     if (farF) {
                         LaunchFilterTile(d_refs.p, d_gU.p, d_gT.p, nstore, (int)ne, K0, std::max(0, K0 - convTile - 1),
                                          K0, d_F.p, stream);
@@ -939,12 +1080,8 @@ void Case::Solve(int nstep, const double* known_lu, long restart, double tol, do
                         LaunchConvFar(grid, d_refs.p, d_off.p, d_kb.p, d_len.p, (const double*)d_mts.p,
                                       (const double*)d_mgs.p, nullptr, nullptr, d_gU.p, d_gT.p, nstore, K0, Tn,
                                       convTile, maxn, (int)ne, d_hfar.p, stream);
-                    Check(hipEventRecord(f1, stream), "event");
-                    Check(hipEventSynchronize(f1), "far");
-                    float fms = 0.f;
-                    Check(hipEventElapsedTime(&fms, f0, f1), "elapsed");
-                    farTime += fms * 1e-3;
-                    ++farLaunches;
+                    Check(hipEventRecord(fe[1], stream), "event");
+                    farEv.push_back(fe);
                 }
                 if (historyMixed)
                     LaunchConvNear(d_refs.p, d_nearPtr.p, d_nearPair.p, d_nearField.p, d_off.p, d_kb.p, d_len.p,
@@ -959,17 +1096,14 @@ void Case::Solve(int nstep, const double* known_lu, long restart, double tol, do
                 LaunchConvolution(grid, d_refs.p, d_off.p, d_kb.p, d_len.p, d_mts.p, d_mgs.p, d_gU.p, d_gT.p, nstore,
                                   K, maxn, (int)ne, d_rhs.p, stream);
             }
-            Check(hipEventRecord(c1, stream), "event");
-            Check(hipEventSynchronize(c1), "conv");
-            float ms = 0.f;
-            Check(hipEventElapsedTime(&ms, c0, c1), "elapsed");
-            convTime += ms * 1e-3;
         }
-        double g0 = Now();
+        Check(hipEventRecord(se[1], stream), "event");
         int it = 0;
         int flag = historyMixed ? GmresMixed(d_rhs.p, d_x.p, restart, tol, &it)
-                                : Gmres(d_rhs.p, d_x.p, restart, tol, &it);
-        gmresTime += Now() - g0;
+                   : gmresDevice ? GmresDevice(d_rhs.p, d_x.p, restart, tol, &it)
+                                 : Gmres(d_rhs.p, d_x.p, restart, tol, &it);
+        Check(hipEventRecord(se[2], stream), "event");
+        stepEv.push_back(se);
         iters[K - 1] = it;
         flags[K - 1] = flag;
         *solved = K;
@@ -1015,14 +1149,17 @@ void Case::Solve(int nstep, const double* known_lu, long restart, double tol, do
         lt[3 * i + 1] = F[1] * t[0] + F[4] * t[1] + F[7] * t[2];
         lt[3 * i + 2] = F[2] * t[0] + F[5] * t[1] + F[8] * t[2];
     }
-    (void)hipEventDestroy(c0);
-    (void)hipEventDestroy(c1);
-    (void)hipEventDestroy(f0);
-    (void)hipEventDestroy(f1);
+    double convTime = 0.0, gmresTime = 0.0, farTime = 0.0;
+    for (const auto& e : stepEv) {
+        convTime += ev.Elapsed(e[0], e[1]);
+        gmresTime += ev.Elapsed(e[1], e[2]);
+    }
+    for (const auto& e : farEv)
+        farTime += ev.Elapsed(e[0], e[1]);
     timing[4] = convTime;
     timing[5] = gmresTime;
     timing[6] = farTime;
-    timing[7] = farLaunches;
+    timing[7] = (double)farEv.size();
 }
 
 }  // namespace tdbem

@@ -1,6 +1,7 @@
 // One multi-domain case on one GPU: host model and device buffers.
 #pragma once
 
+#include "gmres.hpp"
 #include "kernels.hpp"
 #include "model.hpp"
 #include "precond.hpp"
@@ -119,6 +120,7 @@ public:
     bool adjointAnalytic = true;  // history-block derivatives analytic (false: operator differences)
     bool convFiltered = true;  // 16-step far tiles read a pre-filtered state series and prefetch work items
     bool convPacked = true;    // filtered far tiles pack 64 band entries per work item across pairs
+    int gmresDevice = 1;       // FP64 GMRES: 0 host-driven, 1 device-driven hipGraphs, 2 device-driven launches
     int selfMode = -1;       // kernel of the current self tables: 1 wave per element, 0 reference order
     bool quadStaged = true;  // wave-per-pair quadrature with LDS-staged point geometry (false: lane per entry)
     int mixed = 0;  // for the next history assembly: 0 FP64, 1 FP32 blocks, 2 FP32 + bf16 residual blocks
@@ -144,6 +146,8 @@ private:
     int GmresCore(const double* d_b, double* d_x, long restart, double tol, int* iters, const GmresOp<T>& op, T* V,
                   T* W, T* u, T* u2);
     int GmresMixed(const double* d_b, double* d_x, long restart, double tol, int* iters);
+    int GmresDevice(const double* d_b, double* d_x, long restart, double tol, int* iters);
+    void ReleaseGraphs();
     void GmresBatched(int B, const double* rhs, double* x, long restart, double tol, int* its, int* flg);
     BatchWork bw;
 
@@ -185,6 +189,17 @@ private:
     DevBuf<float> d_V32, d_W32, d_u32, d_u232, d_Aval32, d_inv32;
     DevBuf<double> d_r, d_dx;
     double* h_pin = nullptr;
+    // device-driven GMRES: state, its pinned copy, captured graphs and the
+    // operands they were captured for
+    DevBuf<double> d_gst;
+    GmState* h_gst = nullptr;
+    hipGraphExec_t graphBegin = nullptr, graphIter = nullptr;
+    const double* graphB = nullptr;
+    const double* graphX = nullptr;
+    const double* graphV = nullptr;
+    double graphTol = -1.0;
+    long graphRestart = -1;
+    int gm_lastIters = 1;
 };
 
 }  // namespace tdbem

@@ -270,6 +270,36 @@ def test_tiled_convolution_matches_per_step_sum(oracle_mod):
         assert np.abs(T1 - T0).max() <= 1e-11 * np.abs(T0).max(), tile
 
 
+def test_device_driven_gmres_matches_host_driven():
+    """The device-driven GMRES (state, convergence test and stopping flags in
+    device memory; iterations replayed from hipGraphs or launched directly)
+    gives the host-driven solve's iterates bit for bit: same solution, same
+    iteration counts and flags, also when a step stops at the restart length
+    (flag 1) and the solve ends there."""
+    tdbem = _tdbem()
+    from tdbem import mesh
+    w = mesh.layered_config2(h=0.5, nstep=30)
+    m = w.mesh
+    print("case:", w.name, "device vs host GMRES", flush=True)
+    gc = tdbem.Case(m.points, m.ele, m.bcid, w.dt, w.gap, domain_count=4, mat_rows=w.mats, itf_rows=m.interfaces)
+    gc.assemble_step0(10)
+    gc.assemble_history(w.nstep)
+    for restart, tol in ((200, 1e-13), (200, 1e-5), (2, 1e-13)):
+        out = {}
+        for mode in (0, 1, 2):
+            gc.set_option("gmres_device", mode)
+            out[mode] = gc.solve(w.known, w.nstep, restart=restart, tol=tol)
+        for mode in (1, 2):
+            for a, b in zip(out[0][:5], out[mode][:5]):
+                assert np.array_equal(np.asarray(a), np.asarray(b)), (restart, tol, mode)
+        if restart == 2:
+            assert out[0][4] < w.nstep and out[0][3][out[0][4] - 1] == 1
+        else:
+            assert out[0][4] == w.nstep and not out[0][3].any()
+    gc.set_option("gmres_device", 1)
+    gc.close()
+
+
 def test_packed_far_convolution_on_long_rows():
     """Rows of 256 pairs (eight groups of 32 per row, two per wavefront) and
     bands up to the full history: the packed far kernel and the one-pair-per-

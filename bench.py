@@ -36,12 +36,12 @@ sys.path.insert(0, str(REPO))
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per far-convolution launch (k_conv_far_p<16>) on the default
+# HBM bytes per far-convolution launch (k_conv_far_q<16>) on the default
 # workload (layered block, h = 0.1, 400 steps, FP64), from FETCH_SIZE and
 # WRITE_SIZE counter passes calibrated with a 1 GiB device copy (fetch x2.000,
 # write x1.000); the summary with its commands is kept in
 # profiles/k_conv_far16_traffic_layered2x2x4_h0.1_fp64.md.
-MEASURED_TRAFFIC = {("layered_2x2x4_h0.1", 400, 16): 65162307000.0}
+MEASURED_TRAFFIC = {("layered_2x2x4_h0.1", 400, 16): 56922168320.0}
 FP64_PEAK_TFLOPS = 78.6
 # FP64 lane-operations of one history quadrature (k_history_pw + k_history_reg)
 # of the default workload: 64 x (ADD + MUL + TRANS + 2 FMA) F64 VALU wave
@@ -309,14 +309,16 @@ def main():
         "data": "synthetic (seeded layered block, Ricker load)",
         "config": {"workload": w.name, "elements": int(len(w.mesh.ele)), "domains": 4, "nstep": w.nstep,
                    "dt": w.dt, "gmres_restart": 200, "gmres_tol": 1e-5, "history_band_entries": int(last["entries"]),
-                   "conv_tile": args.conv_tile},
+                   "conv_tile": args.conv_tile,
+                   "gmres_driver": ("host loop", "device state, hipGraph replay", "device state, direct launches")[
+                       args.gmres_device]},
         "breakdown_s": {k: round(float(v), 4) for k, v in tm.items() if k != "conv_far_launches"},
         "time_loop_steps_per_s": round(w.nstep / max(loop_time, 1e-9), 3),
         # every band entry holds one MTS and one MGS block (computed; the
         # 1e-14 drop rule may leave some empty)
         "assembled_blocks_per_s": round(2.0 * float(last["entries"]) / max(tm["history"], 1e-9), 1),
         "gmres_inner_iterations_mean": float(np.mean(last["iters"])),
-        "roofline": {"kernel": f"k_conv_far_p (time-blocked history convolution, {args.conv_tile} steps per launch)",
+        "roofline": {"kernel": f"k_conv_far_q (time-blocked history convolution, {args.conv_tile} steps per launch)",
                      "bound": "hbm", "achieved": round(conv_gbs, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(conv_gbs / HBM_PEAK_GBS, 4),
                      "traffic": MEASURED_TRAFFIC.get((w.name, w.nstep, args.conv_tile)),

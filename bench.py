@@ -199,6 +199,43 @@ def batched_report(tdbem, w, single_steps_per_s, nrhs=64):
                             "gmres": round(float(times[1]), 3)}}
 
 
+def models_report(tdbem):
+    """16 independent single-domain cube models of 500-6000 elements (seeded
+    sizes, materials and loads, dt = 0.5 h / c1 each, 200 steps): every
+    model's whole job one after another, then all of them through
+    tdbem_run_models (one stream and host thread per model, history bands
+    held at once within 80 % of the free device memory)."""
+    from tdbem import mesh
+    ws = mesh.models_config4ii()
+
+    def case(w):
+        m = w.mesh
+        return tdbem.Case(m.points, m.ele, m.bcid, w.dt, w.gap, domain_count=1, mat_rows=w.mats,
+                          itf_rows=m.interfaces)
+
+    steps = sum(w.nstep for w in ws)
+    t0 = time.perf_counter()
+    for w in ws:
+        c = case(w)
+        c.assemble_step0(10)
+        c.assemble_history(w.nstep)
+        c.solve(w.known, w.nstep)
+        c.close()
+    tdbem.synchronize()
+    t1 = time.perf_counter()
+    cs = [case(w) for w in ws]
+    res = tdbem.run_models(cs, [w.known for w in ws], [w.nstep for w in ws])
+    tdbem.synchronize()
+    t2 = time.perf_counter()
+    for c in cs:
+        c.close()
+    log(f"models: one after another {t1 - t0:.2f} s, together {t2 - t1:.2f} s")
+    return {"models": len(ws), "elements": [int(len(w.mesh.ele)) for w in ws], "nstep": ws[0].nstep,
+            "model_steps_per_s_sequential": round(steps / (t1 - t0), 2),
+            "model_steps_per_s_together": round(steps / (t2 - t1), 2),
+            "speedup": round((t1 - t0) / (t2 - t1), 3), "all_solved": all(r[4] == w.nstep for r, w in zip(res, ws))}
+
+
 def adjoint_report(tdbem):
     """Reverse-time adjoint on the 1x1x4 block (3456 elements, 4 domains, 300
     steps): misfit over 32 receiver elements against observations from
@@ -253,6 +290,8 @@ def main():
                     help="also run one device copy of BYTES (profiler counter calibration)")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-load-case batched report")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-gradient report")
+    ap.add_argument("--models", action="store_true",
+                    help="also run the 16-model report (about 1.3 TB of history bands in total; a few minutes)")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
     args = ap.parse_args()
     if args.gpus != 1:
@@ -339,6 +378,8 @@ def main():
         out["mixed_precision"] = mixed_precision_report(tdbem)
     if not args.no_adjoint:
         out["adjoint"] = adjoint_report(tdbem)
+    if args.models:
+        out["models"] = models_report(tdbem)
     if not args.no_cpu_baseline:
         log(f"cpu baseline on {args.cpu_threads} threads ...")
         cb = cpu_baseline(args.cpu_threads)

@@ -112,6 +112,24 @@ int tdbem_set_materials(tdbem_ctx* ctx, const double* evr);
 int tdbem_solve_batched(tdbem_ctx* ctx, int nstep, int nrhs, const double* known_lu, long restart, double tol,
                         double* lu, double* lt, int* iters, int* flags, int* solved, double* times);
 
+/* Independent models run together: n contexts, each on its own stream and
+ * driven by its own host thread, do their whole job concurrently -- step-0
+ * assembly (maxleaf), history assembly for nstep[i] steps and the time loop
+ * (known_lu[i], outputs lu[i], lt[i], iters[i], flags[i], solved[i] as for
+ * tdbem_solve).  The device interleaves the models' kernels, which keeps it
+ * busy where one model's launches are too small to fill it.  History bands
+ * held at once stay within budget_bytes (<= 0: 80 % of the free device
+ * memory): a model waits, after its cull, until its bands fit next to those
+ * of the running models, and releases them after its time loop (the
+ * contexts keep their step-0 matrices, not their bands).  Each model's
+ * results equal those of its own tdbem_assemble_step0 / _history / _solve.
+ * No counterpart in the reference (one model per process there); the
+ * per-model operations are those of DBEM1/multidomain.cpp:3664-3926.  On
+ * failure the message names the first failing model. */
+int tdbem_run_models(int n, tdbem_ctx* const* ctx, long maxleaf, const int* nstep, const double* const* known_lu,
+                     long restart, double tol, double budget_bytes, double* const* lu, double* const* lt,
+                     int* const* iters, int* const* flags, int* solved);
+
 /* Convenience: full DynaGMRESSolverMultiDomainCCSR (DBEM1/multidomain.cpp:
  * 3664-3926): step 0, history for maxn = nstep, time loop.  Returns 1 when
  * the final GMRES flag is 0 and -1 otherwise through *result. */
@@ -211,6 +229,13 @@ int tdbem_write_validation(tdbem_ctx* ctx, const char* dir, int nstep, const dou
  * Within a process every solve of a context already reuses its bands. */
 int tdbem_history_save(tdbem_ctx* ctx, const char* path);
 int tdbem_history_load(tdbem_ctx* ctx, const char* path, int* maxn);
+
+/* Freed device buffers of 1 GB or more (history bands and the like) are
+ * kept by the library and reused by the next allocation they fit (mapping
+ * hundreds of GB costs seconds per call); a failed allocation returns them
+ * first.  This returns every cached buffer to the device now.  No
+ * counterpart in the reference (host memory there). */
+int tdbem_release_cached_memory(void);
 
 /* Wait for all device work of this process (hipDeviceSynchronize). */
 int tdbem_device_synchronize(void);

@@ -111,8 +111,9 @@ void Case::SetMaterials(const double* evr)
         D.mat = BuildDynaMat(D.v, G, D.Rou, dt);
         mats.push_back(D.mat);
     }
-    UploadDynaMats(mats.data(), (int)mats.size(), stream);
-    LaunchSingular(d_geo.p, d_dom.p, d_tables.p, d_u1.p, d_t1.p, d_t2.p, (int)ne, !quadStaged, stream);
+    d_mats.upload(mats.data(), mats.size(), stream);
+    grid.mats = d_mats.p;
+    LaunchSingular(d_geo.p, d_dom.p, d_mats.p, d_tables.p, d_u1.p, d_t1.p, d_t2.p, (int)ne, !quadStaged, stream);
     selfMode = quadStaged ? 1 : 0;
     CheckA(hipStreamSynchronize(stream), "materials");
 }
@@ -176,7 +177,7 @@ double Case::Adjoint(int nstep, const double* known_lu, const int* recv, int nre
     // 3. A^T and the transposed leaf preconditioner
     CCSRMat At = TransposeBlocks(A, ne);
     std::vector<long> outBlock((size_t)ne);
-    CheckA(hipMemcpy(outBlock.data(), d_outBlock.p, sizeof(long) * (size_t)ne, hipMemcpyDeviceToHost), "outBlock");
+    CheckA(CopyOnStream(outBlock.data(), d_outBlock.p, sizeof(long) * (size_t)ne, hipMemcpyDeviceToHost, stream), "outBlock");
     struct Tri {
         int r, c;
         double v[9];
@@ -343,8 +344,8 @@ double Case::Adjoint(int nstep, const double* known_lu, const int* recv, int nre
         ComputeStep0(Ap, Kp);
         if (analytic) {
             t2p.resize(9 * (size_t)ne);
-            CheckA(hipMemcpy(t2p.data(), d_t2.p, sizeof(double) * t2p.size(),
-                             hipMemcpyDeviceToHost),
+            CheckA(CopyOnStream(t2p.data(), d_t2.p, sizeof(double) * t2p.size(),
+                             hipMemcpyDeviceToHost, stream),
                    "self tables");
         } else {
             StagedQuadrature(d_dM.p, d_dG.p, nullptr, nullptr, d_aliveS.p);
@@ -353,7 +354,7 @@ double Case::Adjoint(int nstep, const double* known_lu, const int* recv, int nre
         ComputeStep0(Am, Km);
         if (analytic) {
             t2m.resize(9 * (size_t)ne);
-            CheckA(hipMemcpy(t2m.data(), d_t2.p, sizeof(double) * t2m.size(), hipMemcpyDeviceToHost),
+            CheckA(CopyOnStream(t2m.data(), d_t2.p, sizeof(double) * t2m.size(), hipMemcpyDeviceToHost, stream),
                    "self tables");
             for (long f = 0; f < ne; ++f)
                 for (int i = 0; i < 9; ++i)
@@ -413,7 +414,7 @@ double Case::Adjoint(int nstep, const double* known_lu, const int* recv, int nre
         LaunchHistoryGrad(d_geo.p, d_pts.p, d_refs.p, grid, d_coff.p, d_off.p, d_kb.p, d_len.p, d_cls.p, npairs,
                           chunks, d_pwWaves.p, pwWaves, d_alive.p, d_C.p, d_J.p, d_dT2.p, d_gpart.p, stream);
         LaunchDomainSums(d_gpart.p, nw, nd, d_gsum.p, stream);
-        CheckA(hipMemcpy(hsum.data(), d_gsum.p, sizeof(double) * hsum.size(), hipMemcpyDeviceToHost),
+        CheckA(CopyOnStream(hsum.data(), d_gsum.p, sizeof(double) * hsum.size(), hipMemcpyDeviceToHost, stream),
                "history gradient");
     } else {
         StagedQuadrature(d_mts.p, d_mgs.p, nullptr, nullptr, d_alive.p);
@@ -421,7 +422,7 @@ double Case::Adjoint(int nstep, const double* known_lu, const int* recv, int nre
     if (!wasFrozen)
         FreezeStructure(false);
     std::vector<double> acc(3 * (size_t)ne);
-    CheckA(hipMemcpy(acc.data(), d_acc.p, sizeof(double) * acc.size(), hipMemcpyDeviceToHost), "gradient");
+    CheckA(CopyOnStream(acc.data(), d_acc.p, sizeof(double) * acc.size(), hipMemcpyDeviceToHost, stream), "gradient");
     for (int i = 0; i < 3 * nd; ++i)
         grad[i] = hsum[(size_t)i];
     for (int p = 0; p < 3; ++p)

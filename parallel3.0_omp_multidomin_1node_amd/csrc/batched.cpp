@@ -229,8 +229,8 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
     }
     CheckB(hipStreamSynchronize(stream), "batched solve");
     CheckB(hipGetLastError(), "batched solve launch");
-    CheckB(hipMemcpy(lu, d_lu.p, sizeof(double) * d_lu.n, hipMemcpyDeviceToHost), "lu");
-    CheckB(hipMemcpy(lt, d_lt.p, sizeof(double) * d_lt.n, hipMemcpyDeviceToHost), "lt");
+    CheckB(CopyOnStream(lu, d_lu.p, sizeof(double) * d_lu.n, hipMemcpyDeviceToHost, stream), "lu");
+    CheckB(CopyOnStream(lt, d_lt.p, sizeof(double) * d_lt.n, hipMemcpyDeviceToHost, stream), "lt");
     // a case's steps after its GMRES failure are left zero, as a single solve leaves them
     for (int b = 0; b < B; ++b)
         for (int K = solved[b] + 1; K <= nstep; ++K) {

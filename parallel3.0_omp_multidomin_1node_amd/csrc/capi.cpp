@@ -8,6 +8,10 @@
 #include <exception>
 #include <memory>
 #include <string>
+#include <condition_variable>
+#include <mutex>
+#include <thread>
+#include <vector>
 
 struct tdbem_ctx {
     tdbem::Case c;
@@ -26,6 +30,11 @@ int Fail(const char* msg)
 {
     g_err = msg;
     return -2;
+}
+
+int Fail(const std::string& msg)
+{
+    return Fail(msg.c_str());
 }
 }  // namespace
 
@@ -167,9 +176,10 @@ int tdbem_self_tables(tdbem_ctx* ctx, double* u1, double* t1, double* t2)
         return Fail("tdbem_self_tables", e);
     }
     size_t n = 9 * (size_t)ctx->c.ne;
-    if (hipMemcpy(u1, ctx->c.d_u1.p, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(t1, ctx->c.d_t1.p, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(t2, ctx->c.d_t2.p, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+    hipStream_t s = ctx->c.stream;
+    if (tdbem::CopyOnStream(u1, ctx->c.d_u1.p, n * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        tdbem::CopyOnStream(t1, ctx->c.d_t1.p, n * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        tdbem::CopyOnStream(t2, ctx->c.d_t2.p, n * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess)
         return Fail("tdbem_self_tables: device copy failed");
     return 0;
 }
@@ -189,6 +199,86 @@ int tdbem_solve(tdbem_ctx* ctx, int nstep, const double* known_lu, long restart,
         times[1] = ctx->c.timing[5];
         times[2] = ctx->c.timing[4] + ctx->c.timing[5];
     }
+    return 0;
+}
+
+int tdbem_run_models(int n, tdbem_ctx* const* ctx, long maxleaf, const int* nstep, const double* const* known_lu,
+                     long restart, double tol, double budget_bytes, double* const* lu, double* const* lt,
+                     int* const* iters, int* const* flags, int* solved)
+{
+    if (n < 1 || !ctx || maxleaf <= 0 || !nstep || !known_lu || !lu || !lt || !iters || !flags || !solved ||
+        restart < 1 || !(tol > 0.0))
+        return Fail("tdbem_run_models: invalid arguments");
+    for (int i = 0; i < n; ++i) {
+        if (!ctx[i] || nstep[i] < 1 || !known_lu[i] || !lu[i] || !lt[i] || !iters[i] || !flags[i])
+            return Fail("tdbem_run_models: invalid arguments for model " + std::to_string(i));
+        for (int k = 0; k < i; ++k)
+            if (ctx[k] == ctx[i])
+                return Fail("tdbem_run_models: model " + std::to_string(i) + " repeats a context");
+    }
+    if (!(budget_bytes > 0.0)) {
+        size_t freeB = 0, totalB = 0;
+        if (hipMemGetInfo(&freeB, &totalB) != hipSuccess)
+            return Fail("tdbem_run_models: hipMemGetInfo failed");
+        budget_bytes = 0.8 * (double)(freeB + tdbem::PooledBytesCached());
+    }
+    // history bands held at once stay within the budget: a model whose bands
+    // do not fit next to the running ones waits for them to finish (a model
+    // larger than the budget runs alone)
+    std::mutex mu;
+    std::condition_variable cv;
+    double held = 0.0;
+    std::vector<std::string> errs((size_t)n);
+    std::vector<char> restoreGraphs((size_t)n, 0);
+    std::vector<std::thread> pool;
+    pool.reserve((size_t)n);
+    for (int i = 0; i < n; ++i)
+        pool.emplace_back([&, i] {
+            double need = 0.0;
+            bool holding = false;
+            try {
+                tdbem::Case& c = ctx[i]->c;
+                // device-driven GMRES with direct launches: no stream capture
+                // while other threads allocate and free
+                restoreGraphs[(size_t)i] = c.gmresDevice == 1;
+                if (restoreGraphs[(size_t)i])
+                    c.gmresDevice = 2;
+                c.AssembleStep0(maxleaf);
+                need = c.PrepareHistory(nstep[i]);
+                {
+                    std::unique_lock<std::mutex> lk(mu);
+                    cv.wait(lk, [&] { return held == 0.0 || held + need <= budget_bytes; });
+                    held += need;
+                    holding = true;
+                }
+                c.AssembleHistory(nstep[i]);
+                c.Solve(nstep[i], known_lu[i], restart, tol, lu[i], lt[i], iters[i], flags[i], &solved[i]);
+                c.ReleaseHistory();
+            } catch (const std::exception& e) {
+                errs[(size_t)i] = e.what();
+                try {
+                    ctx[i]->c.ReleaseHistory();
+                } catch (...) {
+                }
+            }
+            if (restoreGraphs[(size_t)i])
+                ctx[i]->c.gmresDevice = 1;
+            if (holding) {
+                std::lock_guard<std::mutex> lk(mu);
+                held -= need;
+                if (held < 1.0)
+                    held = 0.0;
+            }
+            cv.notify_all();
+        });
+    for (std::thread& t : pool)
+        t.join();
+    for (int i = 0; i < n; ++i)
+        if (!errs[(size_t)i].empty())
+        {
+            g_err = "tdbem_run_models: model " + std::to_string(i) + ": " + errs[(size_t)i];
+            return -1;
+        }
     return 0;
 }
 
@@ -500,6 +590,12 @@ int tdbem_history_load(tdbem_ctx* ctx, const char* path, int* maxn)
     } catch (const std::exception& e) {
         return Fail("tdbem_history_load", e);
     }
+    return 0;
+}
+
+int tdbem_release_cached_memory(void)
+{
+    tdbem::ReleaseCachedDeviceMemory();
     return 0;
 }
 

@@ -10,13 +10,6 @@
 
 namespace tdbem {
 
-__constant__ DynaMat c_mats[kMaxDomains];
-
-void UploadDynaMats(const DynaMat* mats, int nd, hipStream_t s)
-{
-    (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(c_mats), mats, sizeof(DynaMat) * (size_t)nd, 0, hipMemcpyHostToDevice,
-                                 s);
-}
 
 // ---------------------------------------------------------------------------
 // Time-integrated elastodynamic kernels (DBEM1/dsquareelement.h:631-903,
@@ -557,14 +550,14 @@ void LaunchPoints(const ElemGeo* geo, const QuadTables* q, ElemPoints* pts, int 
 // (DBEM1/dsquareelement.cpp:2292-2427, DBEM1/dsquareelementdyna.cpp:239-490).
 // One lane per element keeps the reference's summation order.
 __global__ void __launch_bounds__(64) k_singular(const ElemGeo* __restrict__ geo, const int* __restrict__ dom,
-                                                 const QuadTables* __restrict__ q, double* u1o, double* t1o,
+                                                 const DynaMat* __restrict__ mats, const QuadTables* __restrict__ q, double* u1o, double* t1o,
                                                  double* t2o, int ne)
 {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= ne)
         return;
     const ElemGeo& g = geo[e];
-    const DynaMat& m = c_mats[dom[e]];
+    const DynaMat& m = mats[dom[e]];
     double node[3];
     FromLocal(g.cc, 0.0, 0.0, node);
     double st[9];
@@ -702,6 +695,7 @@ __global__ void __launch_bounds__(64) k_singular(const ElemGeo* __restrict__ geo
 // and the partial sums are added in a fixed tree order (deterministic; equal
 // to the lane-per-element order up to rounding).
 __global__ void __launch_bounds__(64) k_singular_wave(const ElemGeo* __restrict__ geo, const int* __restrict__ dom,
+                                                      const DynaMat* __restrict__ mats,
                                                       const QuadTables* __restrict__ q, double* u1o, double* t1o,
                                                       double* t2o, int ne)
 {
@@ -710,7 +704,7 @@ __global__ void __launch_bounds__(64) k_singular_wave(const ElemGeo* __restrict_
     int e = blockIdx.x;
     int lane = threadIdx.x;
     const ElemGeo& g = geo[e];
-    const DynaMat& m = c_mats[dom[e]];
+    const DynaMat& m = mats[dom[e]];
     double node[3];
     FromLocal(g.cc, 0.0, 0.0, node);
     double v[kVals];
@@ -844,15 +838,15 @@ __global__ void __launch_bounds__(64) k_singular_wave(const ElemGeo* __restrict_
     }
 }
 
-void LaunchSingular(const ElemGeo* geo, const int* dom, const QuadTables* q, double* u1, double* t1, double* t2,
-                    int ne, bool exact, hipStream_t s)
+void LaunchSingular(const ElemGeo* geo, const int* dom, const DynaMat* mats, const QuadTables* q, double* u1,
+                    double* t1, double* t2, int ne, bool exact, hipStream_t s)
 {
     if (ne <= 0)
         return;
     if (exact)
-        k_singular<<<(ne + 63) / 64, 64, 0, s>>>(geo, dom, q, u1, t1, t2, ne);
+        k_singular<<<(ne + 63) / 64, 64, 0, s>>>(geo, dom, mats, q, u1, t1, t2, ne);
     else
-        k_singular_wave<<<ne, 64, 0, s>>>(geo, dom, q, u1, t1, t2, ne);
+        k_singular_wave<<<ne, 64, 0, s>>>(geo, dom, mats, q, u1, t1, t2, ne);
 }
 
 // ---------------------------------------------------------------------------
@@ -867,7 +861,7 @@ __global__ void __launch_bounds__(256) k_cull(const ElemGeo* __restrict__ geo, c
         return;
     int s, f, d;
     dev::DecodePair(grid, p, s, f, d);
-    const DynaMat& m = c_mats[d];
+    const DynaMat& m = grid.mats[d];
     CullBounds b = PairBounds(m, geo[f], geo[s].center);
     long cap = (long)maxn + 2;
     long k1 = LastActive(m, b.a2, cap);
@@ -918,7 +912,7 @@ __global__ void __launch_bounds__(256) k_history(const ElemGeo* __restrict__ geo
     int s, f, d;
     dev::DecodePair(grid, p, s, f, d);
     int k = kb[p] + (int)(e - off[p]);
-    const DynaMat& m = c_mats[d];
+    const DynaMat& m = grid.mats[d];
     const ElemGeo& fe = geo[f];
     const ElemPoints& P = pts[f];
     const double* src = geo[s].center;
@@ -1304,7 +1298,7 @@ __global__ void __launch_bounds__(256) k_pwcount(const ElemGeo* __restrict__ geo
         return;
     int s, f, d;
     dev::DecodePair(grid, p, s, f, d);
-    const DynaMat& m = c_mats[d];
+    const DynaMat& m = grid.mats[d];
     double r = Dist3(geo[f].center, geo[s].center);
     double radius = geo[f].radius;
     int c = 0, k0 = kb[p], L = len[p];
@@ -1388,7 +1382,7 @@ __global__ void __launch_bounds__(256) k_classify(const ElemGeo* __restrict__ ge
         return;
     int s, f, d;
     dev::DecodePair(grid, p, s, f, d);
-    const DynaMat& m = c_mats[d];
+    const DynaMat& m = grid.mats[d];
     double r = Dist3(geo[f].center, geo[s].center);
     double radius = geo[f].radius;
     int k0 = kb[p], L = len[p];
@@ -1435,7 +1429,7 @@ __global__ void __launch_bounds__(64) k_history_reg(
     int s, f, d;
     DecodePair(grid, p, s, f, d);
     int t = (int)(wv - coff[p]) * 64 + lane;
-    const DynaMat& m = c_mats[d];
+    const DynaMat& m = grid.mats[d];
     const ElemGeo& fe = geo[f];
     const ElemPoints& P = pts[f];
     const double* src = geo[s].center;
@@ -1494,7 +1488,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
     // field element and domain (the same for every slot)
     int s0, f, d;
     DecodePair(grid, W.pair[0], s0, f, d);
-    const DynaMat& m = c_mats[d];
+    const DynaMat& m = grid.mats[d];
     const ElemGeo& fe = geo[f];
     const ElemPoints& P = pts[f];
     if (lane < nseg) {
@@ -1934,7 +1928,7 @@ __global__ void __launch_bounds__(64) k_hgrad_reg(
     int s, f, d;
     DecodePair(grid, p, s, f, d);
     int t = (int)(wv - coff[p]) * 64 + lane;
-    const DynaMat& m = c_mats[d];
+    const DynaMat& m = grid.mats[d];
     const ElemGeo& fe = geo[f];
     const ElemPoints& P = pts[f];
     const double* src = geo[s].center;
@@ -2002,7 +1996,7 @@ __global__ void __launch_bounds__(64) k_hgrad_pw(
     int nseg = W.nseg;
     int s0, f, d;
     DecodePair(grid, W.pair[0], s0, f, d);
-    const DynaMat& m = c_mats[d];
+    const DynaMat& m = grid.mats[d];
     const ElemGeo& fe = geo[f];
     const ElemPoints& P = pts[f];
     if (lane < nseg) {
@@ -2310,7 +2304,7 @@ __global__ void __launch_bounds__(64) k_step0(const ElemGeo* __restrict__ geo, c
         return;
     int s, f, d;
     dev::DecodePair(grid, pairIdx[i], s, f, d);
-    const DynaMat& m = c_mats[d];
+    const DynaMat& m = grid.mats[d];
     const ElemGeo& fe = geo[f];
     const ElemPoints& P = pts[f];
     const double* src = geo[s].center;
@@ -2351,7 +2345,7 @@ __global__ void __launch_bounds__(256) k_classify0(const ElemGeo* __restrict__ g
         return;
     int s, f, d;
     dev::DecodePair(grid, pairIdx[i], s, f, d);
-    const DynaMat& m = c_mats[d];
+    const DynaMat& m = grid.mats[d];
     const double* src = geo[s].center;
     int f0 = dev::NeedCal(m, geo[f], 1, 0.0, src);
     int f1 = dev::NeedCal(m, geo[f], 1, 1.0, src);

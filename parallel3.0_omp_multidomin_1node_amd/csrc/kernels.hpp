@@ -36,6 +36,7 @@ struct PairGrid {
     long pairBase[kMaxDomains + 1];
     int eleBegin[kMaxDomains];
     int eleCount[kMaxDomains];
+    const DynaMat* mats;  // per-domain materials of the case (device memory)
 };
 
 // Pair p of the grid -> source s, field f, domain d (pairs of a domain are
@@ -51,13 +52,11 @@ __device__ __forceinline__ void DecodePairOf(const PairGrid& g, long p, int& s, 
     f = g.eleBegin[d] + (int)(local % nd);
 }
 
-void UploadDynaMats(const DynaMat* mats, int nd, hipStream_t s);
-
 void LaunchPoints(const ElemGeo* geo, const QuadTables* q, ElemPoints* pts, int ne, hipStream_t s);
 // Singular self tables; exact = one lane per element in the reference's
 // summation order, else one wavefront per element (same sums, tree order).
-void LaunchSingular(const ElemGeo* geo, const int* dom, const QuadTables* q, double* u1, double* t1, double* t2,
-                    int ne, bool exact, hipStream_t s);
+void LaunchSingular(const ElemGeo* geo, const int* dom, const DynaMat* mats, const QuadTables* q, double* u1,
+                    double* t1, double* t2, int ne, bool exact, hipStream_t s);
 
 // Causality cull: band [kb, kb+len) of each pair and near-field flag.
 void LaunchCull(const ElemGeo* geo, const int* dom, PairGrid grid, long npairs, int maxn, int* kb, int* len,

@@ -44,24 +44,26 @@ void Get(FILE* f, void* p, size_t bytes, const std::string& path)
 }
 
 // device array -> file through one host staging buffer
-void PutDevice(FILE* f, const void* dev, size_t bytes, std::vector<char>& stage, const std::string& path)
+void PutDevice(FILE* f, const void* dev, size_t bytes, std::vector<char>& stage, const std::string& path,
+               hipStream_t stream)
 {
     const char* d = (const char*)dev;
     for (size_t o = 0; o < bytes; o += kStage) {
         size_t n = std::min(kStage, bytes - o);
-        if (hipMemcpy(stage.data(), d + o, n, hipMemcpyDeviceToHost) != hipSuccess)
+        if (CopyOnStream(stage.data(), d + o, n, hipMemcpyDeviceToHost, stream) != hipSuccess)
             throw std::runtime_error("device copy failed while saving the history");
         Put(f, stage.data(), n, path);
     }
 }
 
-void GetDevice(FILE* f, void* dev, size_t bytes, std::vector<char>& stage, const std::string& path)
+void GetDevice(FILE* f, void* dev, size_t bytes, std::vector<char>& stage, const std::string& path,
+               hipStream_t stream)
 {
     char* d = (char*)dev;
     for (size_t o = 0; o < bytes; o += kStage) {
         size_t n = std::min(kStage, bytes - o);
         Get(f, stage.data(), n, path);
-        if (hipMemcpy(d + o, stage.data(), n, hipMemcpyHostToDevice) != hipSuccess)
+        if (CopyOnStream(d + o, stage.data(), n, hipMemcpyHostToDevice, stream) != hipSuccess)
             throw std::runtime_error("device copy failed while loading the history");
     }
 }
@@ -93,17 +95,17 @@ void Case::SaveHistory(const std::string& path)
     std::vector<char> stage(std::min(kStage, (size_t)std::max<long>(1, entries) * 72));
     size_t e = (size_t)entries;
     if (historyMixed) {
-        PutDevice(f.get(), d_mts32.p, 9 * e * sizeof(float), stage, path);
-        PutDevice(f.get(), d_mgs32.p, 6 * e * sizeof(float), stage, path);
+        PutDevice(f.get(), d_mts32.p, 9 * e * sizeof(float), stage, path, stream);
+        PutDevice(f.get(), d_mgs32.p, 6 * e * sizeof(float), stage, path, stream);
         if (historySplit) {
-            PutDevice(f.get(), d_mtsLo.p, 9 * e * sizeof(unsigned short), stage, path);
-            PutDevice(f.get(), d_mgsLo.p, 6 * e * sizeof(unsigned short), stage, path);
+            PutDevice(f.get(), d_mtsLo.p, 9 * e * sizeof(unsigned short), stage, path, stream);
+            PutDevice(f.get(), d_mgsLo.p, 6 * e * sizeof(unsigned short), stage, path, stream);
         }
     } else {
-        PutDevice(f.get(), d_mts.p, 9 * e * sizeof(double), stage, path);
-        PutDevice(f.get(), d_mgs.p, 6 * e * sizeof(double), stage, path);
+        PutDevice(f.get(), d_mts.p, 9 * e * sizeof(double), stage, path, stream);
+        PutDevice(f.get(), d_mgs.p, 6 * e * sizeof(double), stage, path, stream);
     }
-    PutDevice(f.get(), d_alive.p, e, stage, path);
+    PutDevice(f.get(), d_alive.p, e, stage, path, stream);
     if (std::fflush(f.get()) != 0)
         throw std::runtime_error("write error on " + path);
 }
@@ -169,22 +171,22 @@ void Case::LoadHistory(const std::string& path)
     if (historyMixed) {
         d_mts32.alloc(9 * e);
         d_mgs32.alloc(6 * e);
-        GetDevice(f.get(), d_mts32.p, 9 * e * sizeof(float), stage, path);
-        GetDevice(f.get(), d_mgs32.p, 6 * e * sizeof(float), stage, path);
+        GetDevice(f.get(), d_mts32.p, 9 * e * sizeof(float), stage, path, stream);
+        GetDevice(f.get(), d_mgs32.p, 6 * e * sizeof(float), stage, path, stream);
         if (historySplit) {
             d_mtsLo.alloc(9 * e);
             d_mgsLo.alloc(6 * e);
-            GetDevice(f.get(), d_mtsLo.p, 9 * e * sizeof(unsigned short), stage, path);
-            GetDevice(f.get(), d_mgsLo.p, 6 * e * sizeof(unsigned short), stage, path);
+            GetDevice(f.get(), d_mtsLo.p, 9 * e * sizeof(unsigned short), stage, path, stream);
+            GetDevice(f.get(), d_mgsLo.p, 6 * e * sizeof(unsigned short), stage, path, stream);
         }
     } else {
         d_mts.alloc(9 * e);
         d_mgs.alloc(6 * e);
-        GetDevice(f.get(), d_mts.p, 9 * e * sizeof(double), stage, path);
-        GetDevice(f.get(), d_mgs.p, 6 * e * sizeof(double), stage, path);
+        GetDevice(f.get(), d_mts.p, 9 * e * sizeof(double), stage, path, stream);
+        GetDevice(f.get(), d_mgs.p, 6 * e * sizeof(double), stage, path, stream);
     }
     d_alive.alloc(std::max<size_t>(1, e));
-    GetDevice(f.get(), d_alive.p, e, stage, path);
+    GetDevice(f.get(), d_alive.p, e, stage, path, stream);
     char extra;
     if (std::fread(&extra, 1, 1, f.get()) == 1)
         throw std::runtime_error("history file " + path + " is longer than its header says");

@@ -5,6 +5,9 @@
 
 #include <algorithm>
 #include <array>
+#include <map>
+#include <mutex>
+#include <unordered_map>
 #include <type_traits>
 #include <chrono>
 #include <cmath>
@@ -62,20 +65,109 @@ private:
 
 }  // namespace
 
+// Large device buffers (history bands and their like) are kept when freed
+// and handed to the next request they fit (up to twice its size): mapping
+// and unmapping hundreds of GB costs seconds per call on this device, and a
+// process that runs many cases or models in turn would pay it every time.  A
+// failed allocation first returns the cached blocks to the device.
+namespace {
+
+constexpr size_t kPoolMin = (size_t)1 << 30;
+std::mutex g_poolMu;
+std::multimap<size_t, void*> g_pool;          // cached blocks by capacity
+std::unordered_map<void*, size_t> g_pooled;   // every pooled block and its capacity
+size_t g_poolCached = 0;
+
+void FlushPoolLocked()
+{
+    for (auto& b : g_pool) {
+        g_pooled.erase(b.second);
+        (void)hipFree(b.second);
+    }
+    g_pool.clear();
+    g_poolCached = 0;
+}
+
+hipError_t MallocOrFlush(void** p, size_t bytes)
+{
+    hipError_t e = hipMalloc(p, bytes);
+    if (e == hipSuccess)
+        return e;
+    (void)hipGetLastError();
+    std::lock_guard<std::mutex> lk(g_poolMu);
+    FlushPoolLocked();
+    return hipMalloc(p, bytes);
+}
+
+void* PoolAcquire(size_t bytes)
+{
+    {
+        std::lock_guard<std::mutex> lk(g_poolMu);
+        auto it = g_pool.lower_bound(bytes);
+        if (it != g_pool.end() && it->first / 2 <= bytes) {
+            void* p = it->second;
+            g_poolCached -= it->first;
+            g_pool.erase(it);
+            return p;
+        }
+    }
+    void* p = nullptr;
+    Check(MallocOrFlush(&p, bytes), "hipMalloc");
+    std::lock_guard<std::mutex> lk(g_poolMu);
+    g_pooled[p] = bytes;
+    return p;
+}
+
+// true when p is a pooled block (now cached); the device is drained first so
+// no queued kernel still reads the block when it is handed out again
+bool PoolRelease(void* p)
+{
+    {
+        std::lock_guard<std::mutex> lk(g_poolMu);
+        if (!g_pooled.count(p))
+            return false;
+    }
+    (void)hipDeviceSynchronize();
+    std::lock_guard<std::mutex> lk(g_poolMu);
+    size_t cap = g_pooled[p];
+    g_pool.emplace(cap, p);
+    g_poolCached += cap;
+    return true;
+}
+
+}  // namespace
+
+size_t PooledBytesCached()
+{
+    std::lock_guard<std::mutex> lk(g_poolMu);
+    return g_poolCached;
+}
+
+void ReleaseCachedDeviceMemory()
+{
+    (void)hipDeviceSynchronize();
+    std::lock_guard<std::mutex> lk(g_poolMu);
+    FlushPoolLocked();
+}
+
 template <typename T>
 void DevBuf<T>::alloc(size_t count)
 {
     free();
     if (count == 0)
         count = 1;
-    Check(hipMalloc(&p, count * sizeof(T)), "hipMalloc");
+    size_t bytes = count * sizeof(T);
+    if (bytes >= kPoolMin)
+        p = (T*)PoolAcquire(bytes);
+    else
+        Check(MallocOrFlush((void**)&p, bytes), "hipMalloc");
     n = count;
 }
 
 template <typename T>
 void DevBuf<T>::free()
 {
-    if (p)
+    if (p && !PoolRelease(p))
         (void)hipFree(p);
     p = nullptr;
     n = 0;
@@ -99,6 +191,7 @@ template struct DevBuf<NodeRefs>;
 template struct DevBuf<QuadTables>;
 template struct DevBuf<PwWave>;
 template struct DevBuf<float>;
+template struct DevBuf<DynaMat>;
 template struct DevBuf<unsigned short>;
 
 Case::~Case()
@@ -120,7 +213,9 @@ void Case::Create(long npoints, const double* points, long nele, const int* ele_
         throw std::runtime_error("no elements");
     ne = nele;
     dt = dt_in;
-    Check(hipStreamCreate(&stream), "hipStreamCreate");
+    // non-blocking: work of other contexts on the null stream never orders
+    // against this one (every copy of the case is ordered on its own stream)
+    Check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
     if (!BuildMultiDomainModel(cfg, ne, ne, E, v, rho, dt, model))
         throw std::runtime_error("BuildMultiDomainModel failed");
     if (model.DomainCount() > kMaxDomains)
@@ -219,7 +314,8 @@ void Case::Create(long npoints, const double* points, long nele, const int* ele_
     std::vector<DynaMat> mats;
     for (const Domain& d : model.domains)
         mats.push_back(d.mat);
-    UploadDynaMats(mats.data(), (int)mats.size(), stream);
+    d_mats.upload(mats.data(), mats.size(), stream);
+    grid.mats = d_mats.p;
     d_tables.upload(&tables, 1, stream);
     d_geo.upload(geo.data(), geo.size(), stream);
     d_dom.upload(domOf.data(), domOf.size(), stream);
@@ -230,7 +326,7 @@ void Case::Create(long npoints, const double* points, long nele, const int* ele_
     d_t2.alloc(9 * (size_t)ne);
     double t0 = Now();
     LaunchPoints(d_geo.p, d_tables.p, d_pts.p, (int)ne, stream);
-    LaunchSingular(d_geo.p, d_dom.p, d_tables.p, d_u1.p, d_t1.p, d_t2.p, (int)ne, !quadStaged, stream);
+    LaunchSingular(d_geo.p, d_dom.p, d_mats.p, d_tables.p, d_u1.p, d_t1.p, d_t2.p, (int)ne, !quadStaged, stream);
     selfMode = quadStaged ? 1 : 0;
     Check(hipStreamSynchronize(stream), "setup kernels");
     timing[0] = Now() - t0;
@@ -243,7 +339,7 @@ void Case::EnsureSelfTables()
     if (selfMode == want)
         return;
     double t0 = Now();
-    LaunchSingular(d_geo.p, d_dom.p, d_tables.p, d_u1.p, d_t1.p, d_t2.p, (int)ne, !quadStaged, stream);
+    LaunchSingular(d_geo.p, d_dom.p, d_mats.p, d_tables.p, d_u1.p, d_t1.p, d_t2.p, (int)ne, !quadStaged, stream);
     Check(hipStreamSynchronize(stream), "self tables");
     selfMode = want;
     timing[0] = Now() - t0;
@@ -260,8 +356,8 @@ void Case::Cull(int maxn)
     Check(hipGetLastError(), "cull launch");
     h_len.resize((size_t)npairs);
     h_kb.resize((size_t)npairs);
-    Check(hipMemcpy(h_len.data(), d_len.p, sizeof(int) * (size_t)npairs, hipMemcpyDeviceToHost), "len");
-    Check(hipMemcpy(h_kb.data(), d_kb.p, sizeof(int) * (size_t)npairs, hipMemcpyDeviceToHost), "kb");
+    Check(CopyOnStream(h_len.data(), d_len.p, sizeof(int) * (size_t)npairs, hipMemcpyDeviceToHost, stream), "len");
+    Check(CopyOnStream(h_kb.data(), d_kb.p, sizeof(int) * (size_t)npairs, hipMemcpyDeviceToHost, stream), "kb");
     h_off.resize((size_t)npairs + 1);
     long acc = 0;
     for (long p = 0; p < npairs; ++p) {
@@ -318,7 +414,7 @@ void Case::BuildNearLists(int tile)
 std::vector<long> Case::NearPairs()
 {
     std::vector<unsigned char> nearFlag((size_t)npairs);
-    Check(hipMemcpy(nearFlag.data(), d_near.p, (size_t)npairs, hipMemcpyDeviceToHost), "near");
+    Check(CopyOnStream(nearFlag.data(), d_near.p, (size_t)npairs, hipMemcpyDeviceToHost, stream), "near");
     std::vector<long> list;
     for (long p = 0; p < npairs; ++p)
         if (nearFlag[(size_t)p])
@@ -371,8 +467,8 @@ void Case::ComputeStep0(CCSRMat& Aout, CCSRMat& Kout)
     Check(hipStreamSynchronize(stream), "step0");
     Check(hipGetLastError(), "step0 launch");
     if (!list.empty()) {
-        Check(hipMemcpy(out.data(), d_out.p, sizeof(double) * out.size(), hipMemcpyDeviceToHost), "step0 out");
-        Check(hipMemcpy(flags.data(), d_flags.p, sizeof(int) * flags.size(), hipMemcpyDeviceToHost), "step0 flags");
+        Check(CopyOnStream(out.data(), d_out.p, sizeof(double) * out.size(), hipMemcpyDeviceToHost, stream), "step0 out");
+        Check(CopyOnStream(flags.data(), d_flags.p, sizeof(int) * flags.size(), hipMemcpyDeviceToHost, stream), "step0 flags");
     }
     // AddBlock into the unknown / known builders (DBEM1/multidomain.cpp:1907-1937,2286-2321)
     MultiDomainCCSRBuilder bA(dof.equationBlockCount, dof.unknownBlockCount);
@@ -468,6 +564,37 @@ bool Case::StructureValid() const
     return true;
 }
 
+// The causality cull for a history assembly of maxn steps ahead of it, and
+// the device bytes that assembly will allocate for the bands.
+double Case::PrepareHistory(int maxn_in)
+{
+    if (maxn_in < 1)
+        throw std::runtime_error("maxn must be >= 1");
+    if (frozen)
+        throw std::runtime_error("the frozen structure holds the bands of the last history assembly");
+    EnsureSelfTables();
+    Cull(maxn_in);
+    precull = true;
+    const double vbytes = mixed != 0 ? (mixed == 2 ? 6.0 : 4.0) : 8.0;
+    return (double)entries * (15.0 * vbytes + 1.0);
+}
+
+// Frees the history bands; the step-0 matrices and the cull stay.
+void Case::ReleaseHistory()
+{
+    h_mts.clear();
+    h_mgs.clear();
+    h_alive.clear();
+    d_mts.free();
+    d_mgs.free();
+    d_mts32.free();
+    d_mgs32.free();
+    d_mtsLo.free();
+    d_mgsLo.free();
+    d_alive.free();
+    historyDone = false;
+}
+
 void Case::AssembleHistory(int maxn_in)
 {
     if (maxn_in < 1)
@@ -491,10 +618,12 @@ void Case::AssembleHistory(int maxn_in)
     historySplit = mixed == 2;
     if (historyMixed && !quadStaged)
         throw std::runtime_error("the mixed-precision mode needs the staged history quadrature");
-    if (!frozen)
+    if (!frozen && !(precull && culledFor == maxn))
         Cull(maxn);
+    precull = false;
     size_t freeB = 0, totalB = 0;
     Check(hipMemGetInfo(&freeB, &totalB), "hipMemGetInfo");
+    freeB += PooledBytesCached();
     const double vbytes = historyMixed ? (historySplit ? 6.0 : 4.0) : 8.0;
     double need = (double)entries * (15.0 * vbytes + 1.0);
     if (need > 0.92 * (double)freeB)
@@ -636,14 +765,14 @@ void Case::ExportMatrix(int which, int step, std::vector<int>& rp, std::vector<i
         h_alive.resize((size_t)entries);
         if (historyMixed) {
             std::vector<float> a(h_mts.size()), b(h_mgs.size());
-            Check(hipMemcpy(a.data(), d_mts32.p, sizeof(float) * a.size(), hipMemcpyDeviceToHost), "mts");
-            Check(hipMemcpy(b.data(), d_mgs32.p, sizeof(float) * b.size(), hipMemcpyDeviceToHost), "mgs");
+            Check(CopyOnStream(a.data(), d_mts32.p, sizeof(float) * a.size(), hipMemcpyDeviceToHost, stream), "mts");
+            Check(CopyOnStream(b.data(), d_mgs32.p, sizeof(float) * b.size(), hipMemcpyDeviceToHost, stream), "mgs");
             std::copy(a.begin(), a.end(), h_mts.begin());
             std::copy(b.begin(), b.end(), h_mgs.begin());
             if (historySplit) {  // value = hi + bf16 residual
                 std::vector<unsigned short> la(h_mts.size()), lb(h_mgs.size());
-                Check(hipMemcpy(la.data(), d_mtsLo.p, 2 * la.size(), hipMemcpyDeviceToHost), "mts lo");
-                Check(hipMemcpy(lb.data(), d_mgsLo.p, 2 * lb.size(), hipMemcpyDeviceToHost), "mgs lo");
+                Check(CopyOnStream(la.data(), d_mtsLo.p, 2 * la.size(), hipMemcpyDeviceToHost, stream), "mts lo");
+                Check(CopyOnStream(lb.data(), d_mgsLo.p, 2 * lb.size(), hipMemcpyDeviceToHost, stream), "mgs lo");
                 auto lo = [](unsigned short v) {
                     unsigned int bits = (unsigned int)v << 16;
                     float f;
@@ -656,10 +785,10 @@ void Case::ExportMatrix(int which, int step, std::vector<int>& rp, std::vector<i
                     h_mgs[i] += lo(lb[i]);
             }
         } else {
-            Check(hipMemcpy(h_mts.data(), d_mts.p, sizeof(double) * h_mts.size(), hipMemcpyDeviceToHost), "mts");
-            Check(hipMemcpy(h_mgs.data(), d_mgs.p, sizeof(double) * h_mgs.size(), hipMemcpyDeviceToHost), "mgs");
+            Check(CopyOnStream(h_mts.data(), d_mts.p, sizeof(double) * h_mts.size(), hipMemcpyDeviceToHost, stream), "mts");
+            Check(CopyOnStream(h_mgs.data(), d_mgs.p, sizeof(double) * h_mgs.size(), hipMemcpyDeviceToHost, stream), "mgs");
         }
-        Check(hipMemcpy(h_alive.data(), d_alive.p, h_alive.size(), hipMemcpyDeviceToHost), "alive");
+        Check(CopyOnStream(h_alive.data(), d_alive.p, h_alive.size(), hipMemcpyDeviceToHost, stream), "alive");
     }
     struct Item {
         long row, col;
@@ -1128,8 +1257,8 @@ void Case::Solve(int nstep, const double* known_lu, long restart, double tol, do
         std::swap(keep->bd.n, d_bd.n);
     }
     Check(hipGetLastError(), "solve launch");
-    Check(hipMemcpy(lu, d_lu.p, sizeof(double) * d_lu.n, hipMemcpyDeviceToHost), "lu");
-    Check(hipMemcpy(lt, d_lt.p, sizeof(double) * d_lt.n, hipMemcpyDeviceToHost), "lt");
+    Check(CopyOnStream(lu, d_lu.p, sizeof(double) * d_lu.n, hipMemcpyDeviceToHost, stream), "lu");
+    Check(CopyOnStream(lt, d_lt.p, sizeof(double) * d_lt.n, hipMemcpyDeviceToHost, stream), "lt");
     // step 0: lt = 0, Convert (lu <-> lt for BCID 456), then LocToAbs
     // (DBEM1/multidomain.cpp:3819-3822)
     for (long i = 0; i < ne; ++i) {

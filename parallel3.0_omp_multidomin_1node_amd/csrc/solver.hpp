@@ -11,6 +11,21 @@
 
 namespace tdbem {
 
+// Device bytes held by the cache of large freed buffers, and its release.
+size_t PooledBytesCached();
+void ReleaseCachedDeviceMemory();
+
+// A copy ordered after the work already queued on stream s, complete on
+// return (no use of the null stream, so contexts on other streams and
+// threads are not synchronized by it).
+inline hipError_t CopyOnStream(void* dst, const void* src, size_t bytes, hipMemcpyKind kind, hipStream_t s)
+{
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, s);
+    if (e != hipSuccess)
+        return e;
+    return hipStreamSynchronize(s);
+}
+
 template <typename T>
 struct DevBuf {
     T* p = nullptr;
@@ -64,6 +79,8 @@ public:
                 const MultiDomainInputConfig& cfg, double E, double v, double rho, double dt, double gap);
     void AssembleStep0(long maxleaf);
     void AssembleHistory(int maxn);
+    double PrepareHistory(int maxn);  // cull ahead of AssembleHistory; returns the band bytes it will allocate
+    void ReleaseHistory();
     void Solve(int nstep, const double* known_lu, long restart, double tol, double* lu, double* lt, int* iters,
                int* flags, int* solved);
     // B load cases over the same bands: known_lu, lu, lt are [b][NStep+1][3N],
@@ -148,12 +165,14 @@ private:
     int GmresMixed(const double* d_b, double* d_x, long restart, double tol, int* iters);
     int GmresDevice(const double* d_b, double* d_x, long restart, double tol, int* iters);
     void ReleaseGraphs();
+    bool precull = false;  // the last cull was run for the next history assembly
     void GmresBatched(int B, const double* rhs, double* x, long restart, double tol, int* its, int* flg);
     BatchWork bw;
 
     bool step0Done = false, historyDone = false;
     int culledFor = 0;
     DevBuf<QuadTables> d_tables;
+    DevBuf<DynaMat> d_mats;  // per-domain materials, read through PairGrid::mats
     DevBuf<ElemGeo> d_geo;
     DevBuf<ElemPoints> d_pts;
     DevBuf<int> d_dom;

@@ -27,6 +27,7 @@ EXPORTED = [
     "tdbem_timings", "tdbem_timings_n", "tdbem_history_stats", "tdbem_device_synchronize",
     "tdbem_set_option", "tdbem_model_counts", "tdbem_read_model", "tdbem_read_bd", "tdbem_write_model_bin",
     "tdbem_write_bd_bin", "tdbem_write_validation", "tdbem_history_save", "tdbem_history_load",
+    "tdbem_run_models", "tdbem_release_cached_memory",
 ]
 
 
@@ -79,6 +80,9 @@ def lib():
         L.tdbem_history_save.argtypes = [C.c_void_p, C.c_char_p]
         L.tdbem_history_load.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_int)]
         L.tdbem_write_validation.argtypes = [C.c_void_p, C.c_char_p, C.c_int, _dp, _dp, _ip, _ip, C.c_int, C.c_int]
+        _pp = C.POINTER(C.c_void_p)
+        L.tdbem_run_models.argtypes = [C.c_int, _pp, C.c_long, _ip, _pp, C.c_long, C.c_double, C.c_double, _pp, _pp,
+                                       _pp, _pp, _ip]
         _lib = L
     return _lib
 
@@ -91,6 +95,45 @@ def _check(rc: int, what: str) -> None:
 
 def synchronize() -> None:
     _check(lib().tdbem_device_synchronize(), "tdbem_device_synchronize")
+
+
+def release_cached_memory() -> None:
+    """Return the library's cached large device buffers to the device."""
+    _check(lib().tdbem_release_cached_memory(), "tdbem_release_cached_memory")
+
+
+def run_models(cases, knowns, nsteps, maxleaf=10, restart=200, tol=1e-5, budget_bytes=0.0):
+    """Whole jobs of independent cases together (tdbem_run_models): step 0,
+    history for nsteps[i] steps and the time loop of every case, each on its
+    own stream and host thread, with the history bands held at once kept
+    within budget_bytes (0: 80 % of the free device memory; the bands are
+    released after each case's time loop).  Returns per case (lu, lt, iters,
+    flags, solved) as Case.solve does."""
+    n = len(cases)
+    if len(knowns) != n or len(nsteps) != n:
+        raise ValueError("one load history and one step count per case")
+    keep, outs = [], []
+    for c, k, ns in zip(cases, knowns, nsteps):
+        k = np.ascontiguousarray(k, np.float64).reshape(-1)
+        if k.size != (ns + 1) * c.ne * 3:
+            raise ValueError("known_lu must hold (nstep+1) x ne x 3 values")
+        o = (np.zeros_like(k), np.zeros_like(k), np.zeros(ns, np.int32), np.zeros(ns, np.int32))
+        keep.append(k)
+        outs.append(o)
+
+    def ptrs(arrs):
+        return (C.c_void_p * n)(*[a.ctypes.data for a in arrs])
+
+    ctx = (C.c_void_p * n)(*[c.h.value for c in cases])
+    solved = np.zeros(n, np.int32)
+    _check(lib().tdbem_run_models(n, ctx, int(maxleaf), np.asarray(nsteps, np.int32), ptrs(keep), int(restart),
+                                  float(tol), float(budget_bytes), ptrs([o[0] for o in outs]), ptrs([o[1] for o in outs]),
+                                  ptrs([o[2] for o in outs]), ptrs([o[3] for o in outs]), solved),
+           "tdbem_run_models")
+    res = []
+    for c, ns, o, sv in zip(cases, nsteps, outs, solved):
+        res.append((o[0].reshape(ns + 1, c.ne, 3), o[1].reshape(ns + 1, c.ne, 3), o[2], o[3], int(sv)))
+    return res
 
 
 class Case:

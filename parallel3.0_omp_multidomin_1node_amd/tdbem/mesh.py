@@ -201,6 +201,31 @@ def mixed_config3(h: float = 0.125, nstep: int = 600, seed: int = 2) -> Workload
     return Workload(f"stacked_2x2x8_h{h:g}", m, mats, dt, nstep, known)
 
 
+def models_config4ii(n_models: int = 16, nstep: int = 200, seed: int = 4, lo: int = 500,
+                     hi: int = 6000) -> List[Workload]:
+    """Independent single-domain unit-cube models: element counts drawn uniformly from [lo, hi] and
+    rounded to the nearest cube mesh (6 n^2 elements, h = 1/n), seeded E in [1,10], v in [0.2,0.35],
+    rho in [1,3], a Heaviside or Ricker normal traction on the top face, bottom fixed,
+    dt = 0.5 h / c1 per model."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_models):
+        target = int(rng.integers(lo, hi + 1))
+        n = max(2, int(round(math.sqrt(target / 6.0))))
+        m = box_mesh(1, 1.0, 1.0, 1.0, n, n, n)
+        E, v, rho = rng.uniform(1.0, 10.0), rng.uniform(0.2, 0.35), rng.uniform(1.0, 3.0)
+        dt = 0.5 / n / wave_speeds(E, v, rho)[0]
+        if rng.random() < 0.5:
+            sig = _heaviside(nstep, rng.uniform(0.5, 2.0))
+        else:
+            sig = rng.uniform(0.5, 2.0) * ricker(np.arange(nstep + 1) * dt, rng.uniform(0.5, 1.5) / (40 * dt))
+            sig[0] = 0.0
+        known = normal_load_history(len(m.ele), nstep, dt, m.top, sig)
+        mats = np.array([[1, E, v, rho]])
+        out.append(Workload(f"cube_model{i}_n{n}", m, mats, dt, nstep, known))
+    return out
+
+
 def load_cases(w: Workload, nrhs: int = 64, seed: int = 4) -> np.ndarray:
     """(nrhs, nstep+1, ne, 3) seeded load cases on the workload's mesh: each a
     Ricker (centre frequency in [0.5, 1.5] / (40 dt)) or Heaviside pulse on a

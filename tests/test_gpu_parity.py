@@ -300,6 +300,47 @@ def test_device_driven_gmres_matches_host_driven():
     gc.close()
 
 
+def test_models_run_together_match_single_jobs():
+    """Independent models with their own sizes, materials and time steps run
+    together (tdbem_run_models: one host thread and stream per model, all
+    contexts alive at once) give each model's own job bit for bit; so does a
+    run whose memory budget admits one model's bands at a time."""
+    tdbem = _tdbem()
+    from tdbem import mesh
+    ws = []
+    for i, (n, E, v, rho) in enumerate(((4, 3.0, 0.25, 2.0), (6, 8.0, 0.3, 1.2), (5, 1.5, 0.22, 2.7))):
+        m = mesh.box_mesh(1, 1.0, 1.0, 1.0, n, n, n)
+        dt = 0.5 / n / mesh.wave_speeds(E, v, rho)[0]
+        nstep = 20 + 5 * i
+        sig = mesh.ricker(np.arange(nstep + 1) * dt, 0.8 / (20 * dt))
+        sig[0] = 0.0
+        known = mesh.normal_load_history(len(m.ele), nstep, dt, m.top, sig)
+        ws.append(mesh.Workload(f"cube{n}", m, np.array([[1, E, v, rho]]), dt, nstep, known))
+    print("case: three cube models run together", flush=True)
+
+    def case(w):
+        m = w.mesh
+        return tdbem.Case(m.points, m.ele, m.bcid, w.dt, w.gap, domain_count=1, mat_rows=w.mats,
+                          itf_rows=m.interfaces)
+
+    alone = []
+    for w in ws:
+        c = case(w)
+        c.assemble_step0(10)
+        c.assemble_history(w.nstep)
+        alone.append(c.solve(w.known, w.nstep, tol=1e-10)[:5])
+        c.close()
+    for budget in (0.0, 1.0):
+        cs = [case(w) for w in ws]
+        got = tdbem.run_models(cs, [w.known for w in ws], [w.nstep for w in ws], tol=1e-10, budget_bytes=budget)
+        for c in cs:
+            c.close()
+        for w, a, g in zip(ws, alone, got):
+            assert g[4] == w.nstep
+            for x, y in zip(a, g):
+                assert np.array_equal(np.asarray(x), np.asarray(y)), (w.name, budget)
+
+
 def test_packed_far_convolution_on_long_rows():
     """Rows of 256 pairs (eight groups of 32 per row, two per wavefront) and
     bands up to the full history: the packed far kernel and the one-pair-per-

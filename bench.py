@@ -264,13 +264,15 @@ def adjoint_report(tdbem):
     J, g, _ = c.adjoint(w.known, w.nstep, a.receivers, obs, tol=1e-12, fd_step=1e-6)
     tdbem.synchronize()
     t2 = time.perf_counter()
+    tm = c.timings()
     c.close()
     log(f"adjoint: assembly {t1 - t0:.2f} s, gradient {t2 - t1:.2f} s")
     return {"workload": w.name, "elements": int(len(m.ele)), "domains": 4, "nstep": w.nstep,
             "receivers": int(len(a.receivers)), "gradients_per_s": round(1.0 / (t2 - t1), 4),
             "gradient_s": round(t2 - t1, 3), "assembly_s": round(t1 - t0, 3), "misfit": J,
             "gradient_E_v_rho": np.asarray(g).round(12).tolist(),
-            "history_derivatives": "analytic", "step0_difference_step": 1e-6}
+            "history_derivatives": "analytic", "step0_difference_step": 1e-6,
+            "breakdown_s": {k[8:]: round(float(v), 3) for k, v in tm.items() if k.startswith("adjoint_")}}
 
 
 def main():

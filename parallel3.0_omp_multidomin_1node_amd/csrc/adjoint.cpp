@@ -10,6 +10,8 @@
 // held fixed (every block of domain d depends only on domain d's material,
 // so all domains are perturbed at once, one parameter at a time).
 #include "adjoint.hpp"
+
+#include <chrono>
 #include "solver.hpp"
 
 #include <algorithm>
@@ -20,6 +22,11 @@
 namespace tdbem {
 
 namespace {
+
+double NowA()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
 
 void CheckA(hipError_t e, const char* what)
 {
@@ -135,6 +142,7 @@ double Case::Adjoint(int nstep, const double* known_lu, const int* recv, int nre
     const int nd = grid.nd;
 
     // 1. forward solve, keeping x_K, the state columns and the known inputs
+    double tA = NowA();
     AdjointKeep kp;
     std::vector<double> lu((size_t)nstore * n3), lt((size_t)nstore * n3);
     std::vector<int> its((size_t)nstep), fls((size_t)nstep);
@@ -149,6 +157,8 @@ double Case::Adjoint(int nstep, const double* known_lu, const int* recv, int nre
     keep = nullptr;
     if (solved != nstep || fls[(size_t)nstep - 1] != 0)
         throw std::runtime_error("forward solve did not converge");
+    timing[9] = NowA() - tA;
+    tA = NowA();
 
     // 2. misfit and dJ/dx_K through lu = F^T (u_sign T x_u)
     double J = 0.0;
@@ -286,6 +296,9 @@ double Case::Adjoint(int nstep, const double* known_lu, const int* recv, int nre
         }
     }
 
+    CheckA(hipStreamSynchronize(stream), "backward sweep");
+    timing[10] = NowA() - tA;
+    tA = NowA();
     // 5. gradient, with the bands and judges frozen at the current materials.
     // History blocks: analytic material derivatives contracted with the
     // per-entry correlation of lambda and the filtered states (or, with
@@ -387,6 +400,8 @@ double Case::Adjoint(int nstep, const double* known_lu, const int* recv, int nre
     }
     // restore the materials and self tables (and, after differences, the bands)
     SetMaterials(theta.data());
+    timing[11] = NowA() - tA;
+    tA = NowA();
     std::vector<double> hsum((size_t)nd * 3, 0.0);
     if (analytic) {
         // d(C1, C2, rho) / d(E, v, rho) per domain
@@ -421,6 +436,7 @@ double Case::Adjoint(int nstep, const double* known_lu, const int* recv, int nre
     }
     if (!wasFrozen)
         FreezeStructure(false);
+    timing[12] = NowA() - tA;
     std::vector<double> acc(3 * (size_t)ne);
     CheckA(CopyOnStream(acc.data(), d_acc.p, sizeof(double) * acc.size(), hipMemcpyDeviceToHost, stream), "gradient");
     for (int i = 0; i < 3 * nd; ++i)

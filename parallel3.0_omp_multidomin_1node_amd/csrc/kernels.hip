@@ -1672,14 +1672,43 @@ TDBEM_HIST_INSTANTIATE(float)
 // ---------------------------------------------------------------------------
 namespace dev {
 
-constexpr int kRecD = 100;  // doubles per staged point of the derivative pass
-enum RecDGeo : int { gIR2h = 0, gIR3t = 1, gRk = 2, gR = 3, gN = 6, gRn = 9, gX = 10 };
-enum RecDMat : int {
-    mRc1 = 0, mRc2, mPc1, mPc2, mSXd, mSQd, mSBd, mSXe, mSQe, mSBe, mSXf, mSQBf, mSXg, mSQg, mSXh, mSBh,
-    mUP, mUS1, mUS0, mUC1, mUC0
+constexpr int kRecD = 26;  // doubles per staged point of the derivative pass
+// geometry of the point (as in the forward record) and the point scalars the
+// material-dependent terms are built from: R/C1, R/C2, R/(4 pi C1 dt),
+// R/(4 pi C2 dt) and b3 = w/R^3, b5 = w (r.n)/R^5, b4 = tg (r.n)/R^4,
+// b2 = tg/R^2, u3 = tu/R^3, u1 = tu/R with tg = w/(4 pi dt), tu = w/(4 pi rho)
+enum RecDGeo : int {
+    gIR2h = 0, gIR3t = 1, gRk = 2, gR = 3, gN = 6, gRn = 9, gX = 10,
+    gRc1 = 16, gRc2, gPc1, gPc2, gB3, gB5, gB4, gB2, gU3, gU1
 };
-// value, d/dC1, d/dC2, d/drho of material scalar i
-__device__ __forceinline__ int MS(int i) { return 16 + 4 * i; }
+
+// Material constants of the derivative pass (one domain): the tensor and U
+// scalars are (constant) x (point scalar), and so are their derivatives with
+// respect to q = (C1, C2, rho).
+struct MatD {
+    double iC1, iC2, iRho, c, dc1, dc2, C2sq, twoC2, kXg, kQg, kQg1, kQg2, iC1sq, iC2sq;
+};
+
+__device__ __forceinline__ MatD MakeMatD(const DynaMat& m)
+{
+    MatD k;
+    k.iC1 = 1.0 / m.C1;
+    k.iC2 = 1.0 / m.C2;
+    k.iRho = 1.0 / m.Rou;
+    k.c = m.C2_2OverC1_2;
+    k.dc1 = -2.0 * k.c / m.C1;
+    k.dc2 = 2.0 * k.c / m.C2;
+    k.C2sq = m.C2_2;
+    k.twoC2 = 2.0 * m.C2;
+    k.kXg = -2.0 * m.C2_2OverC1_3;
+    k.kQg = (2.0 * k.c - 1.0) / m.C1;
+    double C1_2 = m.C1 * m.C1;
+    k.kQg1 = -6.0 * k.c / C1_2 + 1.0 / C1_2;
+    k.kQg2 = 4.0 * m.C2 / (C1_2 * m.C1);
+    k.iC1sq = m.OneOverC1_2;
+    k.iC2sq = m.OneOverC2_2;
+    return k;
+}
 
 __device__ void StageRecordD(const DynaMat& m, const double* src, const double* pos, const double* nor, double w,
                              double* g)
@@ -1689,7 +1718,6 @@ __device__ void StageRecordD(const DynaMat& m, const double* src, const double* 
     double iR = 1.0 / R;
     double iR2 = iR * iR, iR3 = iR2 * iR, iR4 = iR2 * iR2, iR5 = iR4 * iR;
     double rn = r[0] * nor[0] + r[1] * nor[1] + r[2] * nor[2];
-    double C1 = m.C1, C2 = m.C2, rho = m.Rou;
     g[gIR2h] = 0.5 * iR2;
     g[gIR3t] = iR3 * (1.0 / 3.0);
     g[gRk] = kOneOver4Pi * (R / m.Dt);
@@ -1704,70 +1732,33 @@ __device__ void StageRecordD(const DynaMat& m, const double* src, const double* 
     g[gX + 3] = r[1] * r[1];
     g[gX + 4] = r[1] * r[2];
     g[gX + 5] = r[2] * r[2];
-    auto put = [&](int i, double v, double d1, double d2, double dr) {
-        double* o = g + MS(i);
-        o[0] = v;
-        o[1] = d1;
-        o[2] = d2;
-        o[3] = dr;
-    };
-    double rc1 = R / C1, rc2 = R / C2;
-    put(mRc1, rc1, -rc1 / C1, 0.0, 0.0);
-    put(mRc2, rc2, 0.0, -rc2 / C2, 0.0);
-    double pc1 = kOneOver4Pi * (R / m.C1Dt), pc2 = kOneOver4Pi * (R / m.C2Dt);
-    put(mPc1, pc1, -pc1 / C1, 0.0, 0.0);
-    put(mPc2, pc2, 0.0, -pc2 / C2, 0.0);
-    double c = m.C2_2OverC1_2;
-    double dc1 = -2.0 * c / C1, dc2 = 2.0 * c / C2;  // dc/dC1, dc/dC2
+    g[gRc1] = R / m.C1;
+    g[gRc2] = R / m.C2;
+    g[gPc1] = kOneOver4Pi * (R / m.C1Dt);
+    g[gPc2] = kOneOver4Pi * (R / m.C2Dt);
     double tg = kOneOver4Pi / m.Dt * w;
-    double sXd = -(c * 12.0 * rn * iR5) * w;
-    put(mSXd, sXd, sXd / c * dc1, sXd / c * dc2, 0.0);
-    double sQd = (4.0 * c - 1.0) * iR3 * w;
-    put(mSQd, sQd, 4.0 * iR3 * w * dc1, 4.0 * iR3 * w * dc2, 0.0);
-    double sBd = c * 2.0 * iR3 * w;
-    put(mSBd, sBd, 2.0 * iR3 * w * dc1, 2.0 * iR3 * w * dc2, 0.0);
-    put(mSXe, 12.0 * rn * iR5 * w, 0.0, 0.0, 0.0);
-    put(mSQe, -2.0 * iR3 * w, 0.0, 0.0, 0.0);
-    put(mSBe, -3.0 * iR3 * w, 0.0, 0.0, 0.0);
-    double sXf = -30.0 * m.C2_2 * rn * iR5 * w;
-    put(mSXf, sXf, 0.0, 2.0 * sXf / C2, 0.0);
-    double sQBf = 6.0 * m.C2_2 * iR3 * w;
-    put(mSQBf, sQBf, 0.0, 2.0 * sQBf / C2, 0.0);
-    double sXg = -2.0 * m.C2_2OverC1_3 * rn * iR4 * tg;
-    put(mSXg, sXg, -3.0 * sXg / C1, 2.0 * sXg / C2, 0.0);
-    double sQg = (2.0 * c - 1.0) / C1 * iR2 * tg;
-    double C1_2 = C1 * C1;
-    put(mSQg, sQg, (-6.0 * c / C1_2 + 1.0 / C1_2) * iR2 * tg, 4.0 * C2 / (C1_2 * C1) * iR2 * tg, 0.0);
-    double sXh = 2.0 / C2 * rn * iR4 * tg;
-    put(mSXh, sXh, 0.0, -sXh / C2, 0.0);
-    double sBh = -(iR2 / C2) * tg;
-    put(mSBh, sBh, 0.0, -sBh / C2, 0.0);
     double tu = m.OneOverFourPaiRou * w;
-    double uP = m.OneOverC1_2 * iR3 * tu;
-    put(mUP, uP, -2.0 * uP / C1, 0.0, -uP / rho);
-    double uS1 = -(m.OneOverC2_2 * iR3) * tu;
-    put(mUS1, uS1, 0.0, -2.0 * uS1 / C2, -uS1 / rho);
-    double uS0 = m.OneOverC2_2 * iR * tu;
-    put(mUS0, uS0, 0.0, -2.0 * uS0 / C2, -uS0 / rho);
-    double uC1 = 3.0 * iR3 * tu;
-    put(mUC1, uC1, 0.0, 0.0, -uC1 / rho);
-    double uC0 = -iR * tu;
-    put(mUC0, uC0, 0.0, 0.0, -uC0 / rho);
+    g[gB3] = iR3 * w;
+    g[gB5] = rn * iR5 * w;
+    g[gB4] = rn * iR4 * tg;
+    g[gB2] = iR2 * tg;
+    g[gU3] = iR3 * tu;
+    g[gU1] = iR * tu;
 }
 
 // Derivatives (q = C1, C2, rho) of the basis scalars of one span at one
 // staged point, the piece (front flags, window bounds) held at its current
-// values; for TYPE 1 also those of the U scalars.
+// values; for TYPE 1 also those of the U scalars.  The tensor scalars do not
+// depend on rho, so their q = 2 derivatives are zero; every term is formed
+// from the point scalars and the material constants.
 struct SpanDer {
     double dX[3], dQ[3], dB[3], duX[3], duD[3];
 };
 
 template <int TYPE>
-__device__ __forceinline__ void MixDerAt(const double* g, const Span& s, bool on, SpanDer& o)
+__device__ __forceinline__ void MixDerAt(const double* g, const Span& s, bool on, const MatD& mc, SpanDer& o)
 {
-    const double* R1 = g + MS(mRc1);
-    const double* R2 = g + MS(mRc2);
-    double rc1 = R1[0], rc2 = R2[0];
+    double rc1 = g[gRc1], rc2 = g[gRc2];
     bool P = on && rc1 >= s.ndt && rc1 < s.np1;
     bool S = on && rc2 >= s.ndt && rc2 < s.np1;
     double lo = fmax(s.ndt, rc1), hi = fmin(s.np1, rc2);
@@ -1779,55 +1770,65 @@ __device__ __forceinline__ void MixDerAt(const double* g, const Span& s, bool on
     double rj = d * phi * g[gIR3t];
     double rk = g[gRk];
     double gam = TYPE == 1 ? fma(s.nfac, ri, -(rk * rj)) : fma(s.nfac, ri, rk * rj);
-    const double* Pc1 = g + MS(mPc1);
-    const double* Pc2 = g + MS(mPc2);
-    double aP = P ? (TYPE == 1 ? s.nfac - Pc1[0] : s.nfac + Pc1[0]) : 0.0;
-    double aS = S ? (TYPE == 1 ? s.nfac - Pc2[0] : s.nfac + Pc2[0]) : 0.0;
+    double pc1 = g[gPc1], pc2 = g[gPc2];
+    double aP = P ? (TYPE == 1 ? s.nfac - pc1 : s.nfac + pc1) : 0.0;
     double bP = P ? (TYPE == 1 ? 1.0 : -1.0) : 0.0;
     double bS = S ? (TYPE == 1 ? 1.0 : -1.0) : 0.0;
     bool loR = rc1 > s.ndt, hiR = rc2 < s.np1;
-    const double* sXd = g + MS(mSXd);
-    const double* sQd = g + MS(mSQd);
-    const double* sBd = g + MS(mSBd);
-    const double* sXe = g + MS(mSXe);
-    const double* sQe = g + MS(mSQe);
-    const double* sBe = g + MS(mSBe);
-    const double* sXf = g + MS(mSXf);
-    const double* sQBf = g + MS(mSQBf);
-    const double* sXg = g + MS(mSXg);
-    const double* sQg = g + MS(mSQg);
-    const double* sXh = g + MS(mSXh);
-    const double* sBh = g + MS(mSBh);
+    double b3 = g[gB3], b5 = g[gB5], b4 = g[gB4], b2 = g[gB2];
+    // values needed with the span derivatives
+    double sXd = -12.0 * mc.c * b5, sQd = (4.0 * mc.c - 1.0) * b3, sBd = 2.0 * mc.c * b3;
+    double sXe = 12.0 * b5, sQe = -2.0 * b3, sBe = -3.0 * b3;
+    double sXf = -30.0 * mc.C2sq * b5, sQBf = 6.0 * mc.C2sq * b3;
+    double sXg = mc.kXg * b4, sXh = 2.0 * mc.iC2 * b4, sBh = -(mc.iC2 * b2);
+    // d(lo), d(hi) and the window terms for q = C1 (0) and C2 (1)
+    double dlo[2] = {loR ? -rc1 * mc.iC1 : 0.0, 0.0};
+    double dhi[2] = {0.0, hiR ? -rc2 * mc.iC2 : 0.0};
+    double dri[2], dgam[2];
 #pragma unroll
-    for (int q = 0; q < 3; ++q) {
-        double dlo = loR ? R1[1 + q] : 0.0;
-        double dhi = hiR ? R2[1 + q] : 0.0;
-        double dd = win ? dhi - dlo : 0.0;
-        double dsm = dhi + dlo;
-        double dri = (dd * sm + d * dsm) * g[gIR2h];
-        double dphi = dhi * sm + hi * dsm + 2.0 * lo * dlo;
+    for (int q = 0; q < 2; ++q) {
+        double dd = win ? dhi[q] - dlo[q] : 0.0;
+        double dsm = dhi[q] + dlo[q];
+        dri[q] = (dd * sm + d * dsm) * g[gIR2h];
+        double dphi = dhi[q] * sm + hi * dsm + 2.0 * lo * dlo[q];
         double drj = (dd * phi + d * dphi) * g[gIR3t];
-        double dgam = TYPE == 1 ? s.nfac * dri - rk * drj : s.nfac * dri + rk * drj;
-        double daP = P ? (TYPE == 1 ? -Pc1[1 + q] : Pc1[1 + q]) : 0.0;
-        double daS = S ? (TYPE == 1 ? -Pc2[1 + q] : Pc2[1 + q]) : 0.0;
-        o.dX[q] = daP * sXd[0] + aP * sXd[1 + q] + daS * sXe[0] + aS * sXe[1 + q] + bP * sXg[1 + q] +
-                  bS * sXh[1 + q] + dgam * sXf[0] + gam * sXf[1 + q];
-        o.dQ[q] = daP * sQd[0] + aP * sQd[1 + q] + daS * sQe[0] + aS * sQe[1 + q] + bP * sQg[1 + q] +
-                  dgam * sQBf[0] + gam * sQBf[1 + q];
-        o.dB[q] = daP * sBd[0] + aP * sBd[1 + q] + daS * sBe[0] + aS * sBe[1 + q] + bS * sBh[1 + q] +
-                  dgam * sQBf[0] + gam * sQBf[1 + q];
-        if (TYPE == 1) {
-            const double* uC1 = g + MS(mUC1);
-            const double* uC0 = g + MS(mUC0);
-            const double* uP = g + MS(mUP);
-            const double* uS1 = g + MS(mUS1);
-            const double* uS0 = g + MS(mUS0);
-            o.duX[q] = dri * uC1[0] + ri * uC1[1 + q] + (P ? uP[1 + q] : 0.0) + (S ? uS1[1 + q] : 0.0);
-            o.duD[q] = dri * uC0[0] + ri * uC0[1 + q] + (S ? uS0[1 + q] : 0.0);
-        } else {
-            o.duX[q] = 0.0;
-            o.duD[q] = 0.0;
-        }
+        dgam[q] = TYPE == 1 ? s.nfac * dri[q] - rk * drj : s.nfac * dri[q] + rk * drj;
+    }
+    // q = C1: aP and the c-dependent scalars move
+    {
+        double daP = P ? (TYPE == 1 ? pc1 * mc.iC1 : -pc1 * mc.iC1) : 0.0;
+        double dsXd = -12.0 * mc.dc1 * b5, dsQd = 4.0 * mc.dc1 * b3, dsBd = 2.0 * mc.dc1 * b3;
+        double dsXg = -3.0 * sXg * mc.iC1, dsQg = mc.kQg1 * b2;
+        o.dX[0] = daP * sXd + aP * dsXd + bP * dsXg + dgam[0] * sXf;
+        o.dQ[0] = daP * sQd + aP * dsQd + bP * dsQg + dgam[0] * sQBf;
+        o.dB[0] = daP * sBd + aP * dsBd + dgam[0] * sQBf;
+    }
+    // q = C2: aS, c and C2^2 move
+    {
+        double daS = S ? (TYPE == 1 ? pc2 * mc.iC2 : -pc2 * mc.iC2) : 0.0;
+        double dsXd = -12.0 * mc.dc2 * b5, dsQd = 4.0 * mc.dc2 * b3, dsBd = 2.0 * mc.dc2 * b3;
+        double dsXf = -30.0 * mc.twoC2 * b5, dsQBf = 6.0 * mc.twoC2 * b3;
+        double dsXg = 2.0 * sXg * mc.iC2, dsQg = mc.kQg2 * b2;
+        double dsXh = -sXh * mc.iC2, dsBh = -sBh * mc.iC2;
+        o.dX[1] = aP * dsXd + daS * sXe + bP * dsXg + bS * dsXh + dgam[1] * sXf + gam * dsXf;
+        o.dQ[1] = aP * dsQd + daS * sQe + bP * dsQg + dgam[1] * sQBf + gam * dsQBf;
+        o.dB[1] = aP * dsBd + daS * sBe + bS * dsBh + dgam[1] * sQBf + gam * dsQBf;
+    }
+    o.dX[2] = o.dQ[2] = o.dB[2] = 0.0;
+    if (TYPE == 1) {
+        double u3 = g[gU3], u1 = g[gU1];
+        double uC1 = 3.0 * u3, uC0 = -u1;
+        double uP = mc.iC1sq * u3, uS1 = -(mc.iC2sq * u3), uS0 = mc.iC2sq * u1;
+        // C1: uP; C2: uS1, uS0; rho: every U scalar scales with 1/rho
+        o.duX[0] = dri[0] * uC1 + (P ? -2.0 * uP * mc.iC1 : 0.0);
+        o.duD[0] = dri[0] * uC0;
+        o.duX[1] = dri[1] * uC1 + (S ? -2.0 * uS1 * mc.iC2 : 0.0);
+        o.duD[1] = dri[1] * uC0 + (S ? -2.0 * uS0 * mc.iC2 : 0.0);
+        o.duX[2] = -mc.iRho * (ri * uC1 + (P ? uP : 0.0) + (S ? uS1 : 0.0));
+        o.duD[2] = -mc.iRho * (ri * uC0 + (S ? uS0 : 0.0));
+    } else {
+        for (int q = 0; q < 3; ++q)
+            o.duX[q] = o.duD[q] = 0.0;
     }
 }
 
@@ -1929,6 +1930,7 @@ __global__ void __launch_bounds__(64) k_hgrad_reg(
     DecodePair(grid, p, s, f, d);
     int t = (int)(wv - coff[p]) * 64 + lane;
     const DynaMat& m = grid.mats[d];
+    const dev::MatD mc = dev::MakeMatD(m);
     const ElemGeo& fe = geo[f];
     const ElemPoints& P = pts[f];
     const double* src = geo[s].center;
@@ -1953,8 +1955,8 @@ __global__ void __launch_bounds__(64) k_hgrad_reg(
         for (int i = 0; i < kNG2; ++i) {
             const double* g = sgeo + i * kRecD;
             SpanDer a, b;
-            MixDerAt<1>(g, s1, rt, a);
-            MixDerAt<2>(g, s2, rt, b);
+            MixDerAt<1>(g, s1, rt, mc, a);
+            MixDerAt<2>(g, s2, rt, mc, b);
             double xc, qc, bc, xt;
             PointContract(g, kc, xc, qc, bc, xt);
             for (int q = 0; q < 3; ++q) {
@@ -1997,6 +1999,7 @@ __global__ void __launch_bounds__(64) k_hgrad_pw(
     int s0, f, d;
     DecodePair(grid, W.pair[0], s0, f, d);
     const DynaMat& m = grid.mats[d];
+    const dev::MatD mc = dev::MakeMatD(m);
     const ElemGeo& fe = geo[f];
     const ElemPoints& P = pts[f];
     if (lane < nseg) {
@@ -2088,8 +2091,8 @@ __global__ void __launch_bounds__(64) k_hgrad_pw(
             for (int j = 0; j < 8; ++j) {
                 const double* g = gs + j * kRecD;
                 SpanDer a, b;
-                MixDerAt<1>(g, s1, rt, a);
-                MixDerAt<2>(g, s2, rt, b);
+                MixDerAt<1>(g, s1, rt, mc, a);
+                MixDerAt<2>(g, s2, rt, mc, b);
                 double xc, qc, bc, xt;
                 PointContract(g, kc, xc, qc, bc, xt);
                 for (int q = 0; q < 3; ++q) {
@@ -2115,8 +2118,8 @@ __global__ void __launch_bounds__(64) k_hgrad_pw(
         for (int j = 0; j < kNPWG2; ++j) {
             const double* g = gs + j * kRecD;
             SpanDer a, b;
-            MixDerAt<2>(g, s2, u2, a);
-            MixDerAt<1>(g, s1, u1, b);
+            MixDerAt<2>(g, s2, u2, mc, a);
+            MixDerAt<1>(g, s1, u1, mc, b);
             double xc, qc, bc, xt;
             PointContract(g, kc, xc, qc, bc, xt);
             for (int q = 0; q < 3; ++q) {

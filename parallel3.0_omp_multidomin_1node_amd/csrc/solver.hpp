@@ -130,8 +130,10 @@ public:
     CCSRMat A, KnownM;
     LeafPreconditioner pre;
     // 0 singular, 1 cull, 2 history, 3 step 0, 4 convolution, 5 GMRES, 6 far
-    // convolution kernels, 7 their launches, 8 history quadrature kernels
-    double timing[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    // convolution kernels, 7 their launches, 8 history quadrature kernels;
+    // adjoint: 9 forward solve, 10 backward sweep, 11 step-0 and self-table
+    // differences with the correlation store, 12 history-block derivatives
+    double timing[13] = {0};
     bool convTiled = true;   // time-blocked convolution (false: one full pass per step)
     int convTile = 16;       // steps per far-part tile (8 or 16); fixes the near lists built by the cull
     bool adjointAnalytic = true;  // history-block derivatives analytic (false: operator differences)

@@ -303,7 +303,8 @@ class Case:
                "tdbem_write_validation")
 
     def timings(self):
-        t = np.zeros(9)
-        _check(lib().tdbem_timings_n(self.h, 9, t), "tdbem_timings_n")
+        t = np.zeros(13)
+        _check(lib().tdbem_timings_n(self.h, 13, t), "tdbem_timings_n")
         return dict(singular=t[0], cull=t[1], history=t[2], step0=t[3], convolution=t[4], gmres=t[5],
-                    conv_far=t[6], conv_far_launches=int(t[7]), quadrature=t[8])
+                    conv_far=t[6], conv_far_launches=int(t[7]), quadrature=t[8], adjoint_forward=t[9],
+                    adjoint_backward=t[10], adjoint_step0_differences=t[11], adjoint_history_derivatives=t[12])

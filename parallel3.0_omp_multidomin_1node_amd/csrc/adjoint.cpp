@@ -247,8 +247,9 @@ double Case::Adjoint(int nstep, const double* known_lu, const int* recv, int nre
     d_ctMem.upload(ctMem.data(), ctMem.size(), stream);
 
     // 4. backward sweep
-    DevBuf<double> d_lam, d_y, d_z, d_dj;
+    DevBuf<double> d_lam, d_y, d_z, d_dj, d_xs;
     d_lam.alloc((size_t)nstore * n3);
+    d_xs.alloc((size_t)n3);
     LaunchZero(d_lam.p, (long)d_lam.n, stream);
     d_y.alloc((size_t)n3);
     d_z.alloc(6 * (size_t)ne);
@@ -289,8 +290,16 @@ double Case::Adjoint(int nstep, const double* known_lu, const int* recv, int nre
                              stream);
             }
             int it = 0;
-            int flag = GmresCore<double>(d_y.p, d_lam.p + (size_t)K * n3, restart, tol, &it, opT, d_V.p, d_W.p,
+            int flag;
+            if (gmresDevice) {  // device-driven; solved into a fixed buffer so the captured graphs stay valid
+                flag = GmresDevice(d_y.p, d_xs.p, restart, tol, &it, &opT);
+                CheckA(hipMemcpyAsync(d_lam.p + (size_t)K * n3, d_xs.p, sizeof(double) * (size_t)n3,
+                                      hipMemcpyDeviceToDevice, stream),
+                       "adjoint solution");
+            } else {
+                flag = GmresCore<double>(d_y.p, d_lam.p + (size_t)K * n3, restart, tol, &it, opT, d_V.p, d_W.p,
                                          d_u.p, d_u2.p);
+            }
             if (flag != 0)
                 throw std::runtime_error("adjoint GMRES did not converge at step " + std::to_string(K));
         }

@@ -324,6 +324,16 @@ unsigned Blocks(long n)
     return (unsigned)((n + 255) / 256);
 }
 
+// y = M^-1 x: the leaf inverses, or the explicit preconditioner matrix
+void LaunchPre(const GmresDevArgs& a, const double* x, double* y, hipStream_t s)
+{
+    if (a.prp)
+        k_gm_spmv<<<Blocks(a.ne), 256, 0, s>>>(a.st, a.ne, a.prp, a.pcol, a.pval, x, 3 * a.ne, -1, nullptr, y);
+    else
+        k_gm_precond<<<Blocks(3 * a.ne), 256, 0, s>>>(a.st, a.ne, a.posLeaf, a.leafBegin, a.leafSize, a.matOff,
+                                                      a.inv, a.repid, a.outBlock, x, y);
+}
+
 }  // namespace
 
 void LaunchGmresBegin(const GmresDevArgs& a, const double* b, double tol, hipStream_t s)
@@ -335,8 +345,7 @@ void LaunchGmresBegin(const GmresDevArgs& a, const double* b, double tol, hipStr
     k_gm_begin<<<1, 64, 0, s>>>(a.part, a.nblocks, tol, a.st, a.stag);
     k_gm_fill<<<Blocks(std::max(a.nQT, a.nR)), 256, 0, s>>>(a.st, a.QT, a.nQT, a.R, a.nR);
     // u = M^-1 r with r = b; h[1] = ||u||; V(:,0) = u / h[1]
-    k_gm_precond<<<Blocks(n), 256, 0, s>>>(a.st, a.ne, a.posLeaf, a.leafBegin, a.leafSize, a.matOff, a.inv, a.repid,
-                                           a.outBlock, b, a.u);
+    LaunchPre(a, b, a.u, s);
     k_gm_multidot<<<a.nblocks, 256, 0, s>>>(a.st, a.u, n, 0, a.u, n, a.part);
     k_gm_finish<<<1, 64, 0, s>>>(a.st, a.part, a.nblocks, 0, a.h, 1);
     k_gm_scale<<<Blocks(n), 256, 0, s>>>(a.st, a.u, a.h, 1, a.V, 0, n);
@@ -348,8 +357,7 @@ void LaunchGmresIteration(const GmresDevArgs& a, const double* b, hipStream_t s)
     const long n = 3 * a.ne;
     const unsigned fin = (unsigned)((a.restart + 63) / 64);
     k_gm_spmv<<<Blocks(a.ne), 256, 0, s>>>(a.st, a.ne, a.rp, a.col, a.val, a.V, n, 0, nullptr, a.u2);
-    k_gm_precond<<<Blocks(n), 256, 0, s>>>(a.st, a.ne, a.posLeaf, a.leafBegin, a.leafSize, a.matOff, a.inv, a.repid,
-                                           a.outBlock, a.u2, a.u);
+    LaunchPre(a, a.u2, a.u, s);
     // classical Gram-Schmidt twice
     k_gm_multidot<<<a.nblocks, 256, 0, s>>>(a.st, a.V, n, 1, a.u, n, a.part);
     k_gm_finish<<<fin, 64, 0, s>>>(a.st, a.part, a.nblocks, 1, a.h, 1);

@@ -24,6 +24,9 @@ struct GmresDevArgs {
     const double* val;
     const long *posLeaf, *leafBegin, *leafSize, *matOff, *repid, *outBlock;
     const double* inv;
+    // explicit block CCSR preconditioner (used instead of the leaf inverses when prp is set)
+    const int *prp, *pcol;
+    const double* pval;
     double *V, *W, *u, *u2, *vh, *h, *h2, *QT, *R, *f, *phibar, *part, *x;
     long nQT, nR;
     int* stag;

@@ -980,7 +980,8 @@ int Case::Gmres(const double* d_b, double* d_x, long restart, double tol, int* i
 // directly (2); iterations are queued ahead, as many as the previous step
 // needed, and the host reads the device state once they have run, queueing
 // two more at a time until the state says done.
-int Case::GmresDevice(const double* d_b, double* d_x, long restart, double tol, int* iters)
+int Case::GmresDevice(const double* d_b, double* d_x, long restart, double tol, int* iters,
+                      const GmresOp<double>* op)
 {
     if (!d_gst.p) {
         d_gst.alloc((sizeof(GmState) + sizeof(double) - 1) / sizeof(double));
@@ -990,16 +991,19 @@ int Case::GmresDevice(const double* d_b, double* d_x, long restart, double tol, 
     a.ne = ne;
     a.restart = restart;
     a.nblocks = gm_blocks;
-    a.rp = d_Arp.p;
-    a.col = d_Acol.p;
-    a.val = d_Aval.p;
+    a.rp = op ? op->rp : d_Arp.p;
+    a.col = op ? op->col : d_Acol.p;
+    a.val = op ? op->val : d_Aval.p;
+    a.prp = op ? op->prp : nullptr;
+    a.pcol = op ? op->pcol : nullptr;
+    a.pval = op ? op->pval : nullptr;
     a.posLeaf = d_posLeaf.p;
     a.leafBegin = d_leafBegin.p;
     a.leafSize = d_leafSize.p;
     a.matOff = d_matOff.p;
     a.repid = d_repid.p;
     a.outBlock = d_outBlock.p;
-    a.inv = d_inv.p;
+    a.inv = op && op->inv ? op->inv : d_inv.p;
     a.V = d_V.p;
     a.W = d_W.p;
     a.u = d_u.p;
@@ -1018,8 +1022,9 @@ int Case::GmresDevice(const double* d_b, double* d_x, long restart, double tol, 
     a.stag = d_stag.p;
     a.st = reinterpret_cast<GmState*>(d_gst.p);
     const bool useGraph = gmresDevice == 1;
+    const void* opKey = a.prp ? (const void*)a.pval : (const void*)a.inv;
     if (useGraph && (graphB != d_b || graphX != d_x || graphTol != tol || graphRestart != restart ||
-                     graphV != d_V.p)) {
+                     graphV != d_V.p || graphA != a.val || graphPre != opKey)) {
         ReleaseGraphs();
         hipGraph_t g = nullptr;
         Check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "capture");
@@ -1037,6 +1042,8 @@ int Case::GmresDevice(const double* d_b, double* d_x, long restart, double tol, 
         graphTol = tol;
         graphRestart = restart;
         graphV = d_V.p;
+        graphA = a.val;
+        graphPre = opKey;
     }
     if (useGraph)
         Check(hipGraphLaunch(graphBegin, stream), "graph launch");

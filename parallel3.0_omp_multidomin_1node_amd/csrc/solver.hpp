@@ -165,7 +165,8 @@ private:
     int GmresCore(const double* d_b, double* d_x, long restart, double tol, int* iters, const GmresOp<T>& op, T* V,
                   T* W, T* u, T* u2);
     int GmresMixed(const double* d_b, double* d_x, long restart, double tol, int* iters);
-    int GmresDevice(const double* d_b, double* d_x, long restart, double tol, int* iters);
+    int GmresDevice(const double* d_b, double* d_x, long restart, double tol, int* iters,
+                    const GmresOp<double>* op = nullptr);
     void ReleaseGraphs();
     bool precull = false;  // the last cull was run for the next history assembly
     void GmresBatched(int B, const double* rhs, double* x, long restart, double tol, int* its, int* flg);
@@ -218,6 +219,8 @@ private:
     const double* graphB = nullptr;
     const double* graphX = nullptr;
     const double* graphV = nullptr;
+    const double* graphA = nullptr;
+    const void* graphPre = nullptr;
     double graphTol = -1.0;
     long graphRestart = -1;
     int gm_lastIters = 1;

@@ -540,6 +540,21 @@ def test_adjoint_time_blocked_transposed_term():
     assert np.abs(g1 - g0).max() <= 1e-9 * np.abs(g0).max()
 
 
+def test_adjoint_device_driven_gmres_is_bit_identical():
+    """The forward and backward GMRES driven from the device (hipGraph
+    replay, default) give the host-driven adjoint's misfit and gradient bit
+    for bit."""
+    w, recv, obs, case = _adjoint_setup()
+    print("case:", w.name, "adjoint device vs host GMRES", flush=True)
+    c = case(w.mats)
+    J1, g1, u1 = c.adjoint(w.known, w.nstep, recv, obs, tol=1e-12, fd_step=1e-6)
+    c.set_option("gmres_device", 0)
+    J0, g0, u0 = c.adjoint(w.known, w.nstep, recv, obs, tol=1e-12, fd_step=1e-6)
+    c.close()
+    assert J1 == J0
+    assert np.array_equal(g1, g0) and np.array_equal(u1, u0)
+
+
 def test_adjoint_analytic_gradient_matches_small_step_differences():
     """Analytic history-block derivatives (default): every (E, v, rho)
     component against central differences of the misfit at a relative step of

@@ -230,6 +230,20 @@ int tdbem_write_validation(tdbem_ctx* ctx, const char* dir, int nstep, const dou
 int tdbem_history_save(tdbem_ctx* ctx, const char* path);
 int tdbem_history_load(tdbem_ctx* ctx, const char* path, int* maxn);
 
+/* Launch-shape autotuner (after tdbem_assemble_history): times, with device
+ * events, the far part of the time-blocked convolution on the last 16-step
+ * tile with 2, 4 or 8 wavefronts per source row or one pair's lags per work
+ * item, and the GMRES dot-product reductions with 8 .. 128 blocks, then
+ * applies the fastest of each to the context (options "conv_packed",
+ * "far_parts", "gmres_blocks").  With cache_path (may be null) a choice
+ * stored for the same case shape (elements, pairs, band entries, steps,
+ * domains, storage precision) is applied without timing, and a new one is
+ * appended.  Outputs (may be null): wavefronts per row (0 = one pair per
+ * item), reduction blocks, time of the chosen far launch in ms (0 when
+ * cached).  The shapes reassociate sums, so results change only by rounding.
+ * No counterpart in the reference. */
+int tdbem_autotune(tdbem_ctx* ctx, const char* cache_path, int* far_parts, int* gmres_blocks, double* far_ms);
+
 /* Freed device buffers of 1 GB or more (history bands and the like) are
  * kept by the library and reused by the next allocation they fit (mapping
  * hundreds of GB costs seconds per call); a failed allocation returns them

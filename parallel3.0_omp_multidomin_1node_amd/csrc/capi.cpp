@@ -415,6 +415,18 @@ int tdbem_set_option(tdbem_ctx* ctx, const char* name, double value)
         ctx->c.convPacked = value != 0.0;
         return 0;
     }
+    if (n == "far_parts") {
+        if (value < 1.0 || value > (double)tdbem::kFarPartsMax || value != (double)(int)value)
+            return Fail("far_parts must be a whole number from 1 to 8");
+        ctx->c.farParts = (int)value;
+        return 0;
+    }
+    if (n == "gmres_blocks") {
+        if (value < 1.0 || value > (double)tdbem::kGmBlocksMax || value != (double)(int)value)
+            return Fail("gmres_blocks must be a whole number from 1 to 128");
+        ctx->c.gmBlocks = (int)value;
+        return 0;
+    }
     if (n == "gmres_device") {
         if (value != 0.0 && value != 1.0 && value != 2.0)
             return Fail("gmres_device must be 0, 1 or 2");
@@ -589,6 +601,24 @@ int tdbem_history_load(tdbem_ctx* ctx, const char* path, int* maxn)
             *maxn = ctx->c.maxn;
     } catch (const std::exception& e) {
         return Fail("tdbem_history_load", e);
+    }
+    return 0;
+}
+
+int tdbem_autotune(tdbem_ctx* ctx, const char* cache_path, int* far_parts, int* gmres_blocks, double* far_ms)
+{
+    if (!ctx)
+        return Fail("tdbem_autotune: invalid arguments");
+    try {
+        tdbem::TuneChoice c = ctx->c.Autotune(cache_path ? std::string(cache_path) : std::string());
+        if (far_parts)
+            *far_parts = c.farParts;
+        if (gmres_blocks)
+            *gmres_blocks = c.gmBlocks;
+        if (far_ms)
+            *far_ms = c.farMs;
+    } catch (const std::exception& e) {
+        return Fail("tdbem_autotune", e);
     }
     return 0;
 }

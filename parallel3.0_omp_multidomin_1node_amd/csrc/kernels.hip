@@ -2891,7 +2891,8 @@ __global__ void __launch_bounds__(64) k_conv_far_q(PairGrid grid, const NodeRefs
                                                    const unsigned short* __restrict__ mtsLo,
                                                    const unsigned short* __restrict__ mgsLo,
                                                    const double* __restrict__ F, int fstride, int K0, int Tn,
-                                                   int maxn, int ne, double* __restrict__ hfar, long partStride)
+                                                   int maxn, int ne, double* __restrict__ hfar, long partStride,
+                                                   int parts)
 {
     constexpr int P = dev::kFarPairs;
     constexpr int kRows = 64 + P * (T - 1);
@@ -2900,8 +2901,8 @@ __global__ void __launch_bounds__(64) k_conv_far_q(PairGrid grid, const NodeRefs
     __shared__ int ex[dev::kFarGroup + 1];  // exclusive prefix of the clipped band lengths, then the total
     __shared__ int gk[dev::kFarGroup];
     __shared__ long go[dev::kFarGroup];
-    int s = blockIdx.x >> 2;      // source row
-    int wave = blockIdx.x & 3;    // quarter of the row: pair groups wave, wave + 4, ...
+    int s = blockIdx.x / parts;     // source row
+    int wave = blockIdx.x % parts;  // part of the row: pair groups wave, wave + parts, ...
     int d = refs[s].domain;
     int nd = grid.eleCount[d];
     int eb = grid.eleBegin[d];
@@ -2914,10 +2915,10 @@ __global__ void __launch_bounds__(64) k_conv_far_q(PairGrid grid, const NodeRefs
     for (int i = 0; i < T; ++i)
         acc[i][0] = acc[i][1] = acc[i][2] = 0.0;
     // wavefront-uniform cursor: group g, next entry E of Gt, first pair i0 not yet finished
-    int g = wave - 4, E = 0, Gt = 0, i0 = 0;
+    int g = wave - parts, E = 0, Gt = 0, i0 = 0;
     auto loadGroup = [&]() -> bool {
         for (;;) {
-            g += 4;
+            g += parts;
             if (g * dev::kFarGroup >= nd)
                 return false;
             int q = g * dev::kFarGroup + lane;
@@ -3084,7 +3085,7 @@ __global__ void __launch_bounds__(64) k_conv_far_q(PairGrid grid, const NodeRefs
             break;
         more = body(B, A);
     }
-    // the quarter's partial sums; the near kernel adds the four quarters in order
+    // the part's partial sums; the near kernel adds the parts in order
     double* out = hfar + wave * partStride + 3 * (long)s;
 #pragma unroll
     for (int i = 0; i < T; ++i)
@@ -3098,22 +3099,23 @@ __global__ void __launch_bounds__(64) k_conv_far_q(PairGrid grid, const NodeRefs
 template <typename TB>
 void LaunchConvFarF(PairGrid grid, const NodeRefs* refs, const long* off, const int* kb, const int* len,
                     const TB* mts, const TB* mgs, const unsigned short* mtsLo, const unsigned short* mgsLo,
-                    const double* F, int fstride, int K0, int Tn, int maxn, int ne, double* hfar, bool packed,
+                    const double* F, int fstride, int K0, int Tn, int maxn, int ne, double* hfar, int parts,
                     hipStream_t s)
 {
-    if (packed)
-        k_conv_far_q<16, TB><<<4 * ne, 64, 0, s>>>(grid, refs, off, kb, len, mts, mgs, mtsLo, mgsLo, F, fstride, K0,
-                                                   Tn, maxn, ne, hfar, (long)kConvTileMax * 3 * ne);
+    if (parts > 0)
+        k_conv_far_q<16, TB><<<(unsigned)(parts * ne), 64, 0, s>>>(grid, refs, off, kb, len, mts, mgs, mtsLo, mgsLo,
+                                                                   F, fstride, K0, Tn, maxn, ne, hfar,
+                                                                   (long)kConvTileMax * 3 * ne, parts);
     else
         k_conv_far_p<16, TB><<<ne, 256, 0, s>>>(grid, refs, off, kb, len, mts, mgs, mtsLo, mgsLo, F, fstride, K0,
                                                 Tn, maxn, ne, hfar);
 }
 template void LaunchConvFarF<double>(PairGrid, const NodeRefs*, const long*, const int*, const int*, const double*,
                                      const double*, const unsigned short*, const unsigned short*, const double*, int,
-                                     int, int, int, int, double*, bool, hipStream_t);
+                                     int, int, int, int, double*, int, hipStream_t);
 template void LaunchConvFarF<float>(PairGrid, const NodeRefs*, const long*, const int*, const int*, const float*,
                                     const float*, const unsigned short*, const unsigned short*, const double*, int,
-                                    int, int, int, int, double*, bool, hipStream_t);
+                                    int, int, int, int, double*, int, hipStream_t);
 
 // Near part of step K in the tile starting at K0 plus the far part:
 // rhs_K += hfar[K-K0] + sum over near pairs (band start <= K-K0+1) of the

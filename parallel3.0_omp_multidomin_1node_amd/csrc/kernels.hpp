@@ -136,14 +136,15 @@ constexpr int kConvTileMax = 16;
 void LaunchFilterTile(const NodeRefs* refs, const double* gU, const double* gT, int nstore, int ne, int K0, int x0,
                       int x1, double* F, hipStream_t s);
 // Far part of a 16-step tile from the filtered series, next work item
-// prefetched; packed: 64 band entries per item across pair boundaries, one
-// wavefront per quarter of a row, four partial sums per row (hfar holds four
-// parts of kConvTileMax * 3 * ne values); otherwise 64 lags of one pair per
-// item and one sum per row.
+// prefetched.  parts > 0: 64 band entries per item across pair boundaries,
+// parts wavefronts per row, one partial sum each (hfar holds parts blocks of
+// kConvTileMax * 3 * ne values, at most kFarPartsMax); parts = 0: 64 lags of
+// one pair per item and one sum per row.
+constexpr int kFarPartsMax = 8;
 template <typename TB>
 void LaunchConvFarF(PairGrid grid, const NodeRefs* refs, const long* off, const int* kb, const int* len,
                     const TB* mts, const TB* mgs, const unsigned short* mtsLo, const unsigned short* mgsLo,
-                    const double* F, int fstride, int K0, int Tn, int maxn, int ne, double* hfar, bool packed,
+                    const double* F, int fstride, int K0, int Tn, int maxn, int ne, double* hfar, int parts,
                     hipStream_t s);
 template <typename TB>
 void LaunchConvFar(PairGrid grid, const NodeRefs* refs, const long* off, const int* kb, const int* len, const TB* mts,

@@ -843,6 +843,10 @@ void Case::ExportMatrix(int which, int step, std::vector<int>& rp, std::vector<i
 void Case::PrepareGmres(long restart)
 {
     long n = 3 * ne;
+    if (gmBlocks < 1 || gmBlocks > kGmBlocksMax)
+        throw std::runtime_error("GMRES reduction blocks must be 1 .. " + std::to_string(kGmBlocksMax));
+    // reduction partition: few blocks keep the in-order finish short
+    gm_blocks = (int)std::min<long>(gmBlocks, std::max<long>(1, (n + 255) / 256));
     if (gm_restart == restart && gm_n == n)
         return;
     gm_restart = restart;
@@ -859,9 +863,7 @@ void Case::PrepareGmres(long restart)
     d_f.alloc((size_t)restart + 2);
     d_scal.alloc(4);
     d_stag.alloc(1);
-    // reduction partition: few blocks keep the in-order finish short
-    gm_blocks = (int)std::min<long>(32, std::max<long>(1, (n + 255) / 256));
-    d_part.alloc((size_t)gm_blocks * (size_t)(restart + 1));
+    d_part.alloc((size_t)kGmBlocksMax * (size_t)(restart + 1));
     if (!h_pin)
         Check(hipHostMalloc((void**)&h_pin, 4 * sizeof(double), hipHostMallocDefault), "pinned");
     gm_mixed = false;
@@ -1024,7 +1026,7 @@ int Case::GmresDevice(const double* d_b, double* d_x, long restart, double tol, 
     const bool useGraph = gmresDevice == 1;
     const void* opKey = a.prp ? (const void*)a.pval : (const void*)a.inv;
     if (useGraph && (graphB != d_b || graphX != d_x || graphTol != tol || graphRestart != restart ||
-                     graphV != d_V.p || graphA != a.val || graphPre != opKey)) {
+                     graphV != d_V.p || graphA != a.val || graphPre != opKey || graphBlocks != gm_blocks)) {
         ReleaseGraphs();
         hipGraph_t g = nullptr;
         Check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "capture");
@@ -1044,6 +1046,7 @@ int Case::GmresDevice(const double* d_b, double* d_x, long restart, double tol, 
         graphV = d_V.p;
         graphA = a.val;
         graphPre = opKey;
+        graphBlocks = gm_blocks;
     }
     if (useGraph)
         Check(hipGraphLaunch(graphBegin, stream), "graph launch");
@@ -1162,10 +1165,12 @@ void Case::Solve(int nstep, const double* known_lu, long restart, double tol, do
     d_known.alloc(3 * (size_t)std::max<long>(1, dof.knownBlockCount));
     d_rhs.alloc((size_t)n3);
     d_x.alloc((size_t)n3);
-    d_hfar.alloc(4 * (size_t)kConvTileMax * (size_t)n3);  // four partial sums per row (packed far part)
+    d_hfar.alloc((size_t)kFarPartsMax * (size_t)kConvTileMax * (size_t)n3);  // partial sums per row (packed far part)
     DevBuf<double> d_F;
     const bool farF = convTiled && convFiltered && convTile == 16;
-    const int hfarParts = farF && convPacked ? 4 : 1;
+    if (farParts < 1 || farParts > kFarPartsMax)
+        throw std::runtime_error("far-part wavefronts per row must be 1 .. " + std::to_string(kFarPartsMax));
+    const int hfarParts = farF && convPacked ? farParts : 1;
     if (farF) {
         d_F.alloc((size_t)ne * (size_t)(nstore + 1) * 6);
         LaunchZero(d_F.p, (long)d_F.n, stream);
@@ -1202,11 +1207,11 @@ void Case::Solve(int nstep, const double* known_lu, long restart, double tol, do
                             LaunchConvFarF(grid, d_refs.p, d_off.p, d_kb.p, d_len.p, (const float*)d_mts32.p,
                                            (const float*)d_mgs32.p, (const unsigned short*)d_mtsLo.p,
                                            (const unsigned short*)d_mgsLo.p, d_F.p, nstore + 1, K0, Tn, maxn,
-                                           (int)ne, d_hfar.p, convPacked, stream);
+                                           (int)ne, d_hfar.p, convPacked ? farParts : 0, stream);
                         else
                             LaunchConvFarF(grid, d_refs.p, d_off.p, d_kb.p, d_len.p, (const double*)d_mts.p,
                                            (const double*)d_mgs.p, nullptr, nullptr, d_F.p, nstore + 1, K0, Tn, maxn,
-                                           (int)ne, d_hfar.p, convPacked, stream);
+                                           (int)ne, d_hfar.p, convPacked ? farParts : 0, stream);
                     } else if (historyMixed)
                         LaunchConvFar(grid, d_refs.p, d_off.p, d_kb.p, d_len.p, (const float*)d_mts32.p,
                                       (const float*)d_mgs32.p, (const unsigned short*)d_mtsLo.p,

@@ -72,6 +72,17 @@ struct BatchWork {
     }
 };
 
+constexpr int kGmBlocksMax = 128;
+
+// Launch shapes picked by Case::Autotune: far-part wavefronts per row (0 =
+// one pair's lags per item) and GMRES reduction blocks, with the timings of
+// the chosen shapes (0 when taken from the cache).
+struct TuneChoice {
+    int farParts, gmBlocks;
+    double farMs, gmUs;
+    bool cached;
+};
+
 class Case {
 public:
     ~Case();
@@ -80,6 +91,8 @@ public:
     void AssembleStep0(long maxleaf);
     void AssembleHistory(int maxn);
     double PrepareHistory(int maxn);  // cull ahead of AssembleHistory; returns the band bytes it will allocate
+    TuneChoice Autotune(const std::string& cachePath);
+    std::string TuneKey() const;
     void ReleaseHistory();
     void Solve(int nstep, const double* known_lu, long restart, double tol, double* lu, double* lt, int* iters,
                int* flags, int* solved);
@@ -139,6 +152,8 @@ public:
     bool adjointAnalytic = true;  // history-block derivatives analytic (false: operator differences)
     bool convFiltered = true;  // 16-step far tiles read a pre-filtered state series and prefetch work items
     bool convPacked = true;    // filtered far tiles pack 64 band entries per work item across pairs
+    int farParts = 4;          // wavefronts (and partial sums) per source row of the packed far part
+    int gmBlocks = 32;         // reduction blocks of the GMRES dot products (at most; <= kGmBlocksMax)
     int gmresDevice = 1;       // FP64 GMRES: 0 host-driven, 1 device-driven hipGraphs, 2 device-driven launches
     int selfMode = -1;       // kernel of the current self tables: 1 wave per element, 0 reference order
     bool quadStaged = true;  // wave-per-pair quadrature with LDS-staged point geometry (false: lane per entry)
@@ -221,6 +236,7 @@ private:
     const double* graphV = nullptr;
     const double* graphA = nullptr;
     const void* graphPre = nullptr;
+    int graphBlocks = 0;
     double graphTol = -1.0;
     long graphRestart = -1;
     int gm_lastIters = 1;

@@ -27,7 +27,7 @@ EXPORTED = [
     "tdbem_timings", "tdbem_timings_n", "tdbem_history_stats", "tdbem_device_synchronize",
     "tdbem_set_option", "tdbem_model_counts", "tdbem_read_model", "tdbem_read_bd", "tdbem_write_model_bin",
     "tdbem_write_bd_bin", "tdbem_write_validation", "tdbem_history_save", "tdbem_history_load",
-    "tdbem_run_models", "tdbem_release_cached_memory",
+    "tdbem_run_models", "tdbem_release_cached_memory", "tdbem_autotune",
 ]
 
 
@@ -83,6 +83,8 @@ def lib():
         _pp = C.POINTER(C.c_void_p)
         L.tdbem_run_models.argtypes = [C.c_int, _pp, C.c_long, _ip, _pp, C.c_long, C.c_double, C.c_double, _pp, _pp,
                                        _pp, _pp, _ip]
+        L.tdbem_autotune.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                     C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -301,6 +303,14 @@ class Case:
                                             np.ascontiguousarray(iters, np.int32),
                                             np.ascontiguousarray(flags, np.int32), int(solved), int(solver)),
                "tdbem_write_validation")
+
+    def autotune(self, cache_path=None):
+        """Pick the far-convolution and GMRES-reduction launch shapes by timing
+        (tdbem_autotune); returns (far_parts, gmres_blocks, far_ms)."""
+        fp, gb, ms = C.c_int(), C.c_int(), C.c_double()
+        _check(lib().tdbem_autotune(self.h, cache_path.encode() if cache_path else None, C.byref(fp), C.byref(gb),
+                                    C.byref(ms)), "tdbem_autotune")
+        return fp.value, gb.value, ms.value
 
     def timings(self):
         t = np.zeros(13)

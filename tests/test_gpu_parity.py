@@ -341,6 +341,45 @@ def test_models_run_together_match_single_jobs():
                 assert np.array_equal(np.asarray(x), np.asarray(y)), (w.name, budget)
 
 
+def test_autotuner_choices_and_cache(tmp_path):
+    """The autotuner picks one of the candidate launch shapes, the solve after
+    tuning agrees with the default solve to rounding, every far-part shape
+    and reduction partition does too, and a second context of the same shape
+    takes the stored choice from the cache without timing."""
+    tdbem = _tdbem()
+    from tdbem import mesh
+    w = mesh.layered_config2(h=0.25, nstep=60)
+    m = w.mesh
+    print("case:", w.name, "autotuner", flush=True)
+
+    def case():
+        c = tdbem.Case(m.points, m.ele, m.bcid, w.dt, w.gap, domain_count=4, mat_rows=w.mats,
+                       itf_rows=m.interfaces)
+        c.assemble_step0(10)
+        c.assemble_history(w.nstep)
+        return c
+
+    c = case()
+    U0, T0, *_ = c.solve(w.known, w.nstep, tol=1e-13)
+    cache = str(tmp_path / "tune.txt")
+    fp, gb, ms = c.autotune(cache)
+    assert fp in (0, 2, 4, 8) and gb in (8, 16, 32, 64, 128) and ms > 0.0
+    U1, T1, *_ = c.solve(w.known, w.nstep, tol=1e-13)
+    assert np.abs(U1 - U0).max() <= 1e-12 * np.abs(U0).max()
+    for parts, blocks in ((2, 8), (8, 128), (1, 16)):
+        c.set_option("conv_packed", 1)
+        c.set_option("far_parts", parts)
+        c.set_option("gmres_blocks", blocks)
+        U2, T2, *_ = c.solve(w.known, w.nstep, tol=1e-13)
+        assert np.abs(U2 - U0).max() <= 1e-12 * np.abs(U0).max(), (parts, blocks)
+        assert np.abs(T2 - T0).max() <= 1e-12 * np.abs(T0).max(), (parts, blocks)
+    c.close()
+    c2 = case()
+    assert c2.autotune(cache) == (fp, gb, 0.0)
+    c2.close()
+    assert len(open(cache).read().strip().splitlines()) == 1
+
+
 def test_packed_far_convolution_on_long_rows():
     """Rows of 256 pairs (eight groups of 32 per row, two per wavefront) and
     bands up to the full history: the packed far kernel and the one-pair-per-

@@ -36,13 +36,14 @@ sys.path.insert(0, str(REPO))
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per far-convolution launch (k_conv_far_q<16>, four wavefronts per
-# source row) on the default
-# workload (layered block, h = 0.1, 400 steps, FP64), from FETCH_SIZE and
-# WRITE_SIZE counter passes calibrated with a 1 GiB device copy (fetch x2.000,
-# write x1.000); the summary with its commands is kept in
-# profiles/k_conv_far16_traffic_layered2x2x4_h0.1_fp64.md.
-MEASURED_TRAFFIC = {("layered_2x2x4_h0.1", 400, 16, 4): 56922168320.0}
+# HBM bytes per far-convolution launch (k_conv_far_q<16> with 2, 4 or 8
+# wavefronts per source row) on the default workload (layered block, h = 0.1,
+# 400 steps, FP64), from FETCH_SIZE and WRITE_SIZE counter passes calibrated
+# with a 1 GiB device copy (fetch x2.000, write x1.000); the summary with its
+# commands is kept in profiles/k_conv_far16_traffic_layered2x2x4_h0.1_fp64.md.
+MEASURED_TRAFFIC = {("layered_2x2x4_h0.1", 400, 16, 2): 61661732702.0,
+                    ("layered_2x2x4_h0.1", 400, 16, 4): 56922168320.0,
+                    ("layered_2x2x4_h0.1", 400, 16, 8): 52886495483.0}
 FP64_PEAK_TFLOPS = 78.6
 # FP64 lane-operations of one history quadrature (k_history_pw + k_history_reg)
 # of the default workload: 64 x (ADD + MUL + TRANS + 2 FMA) F64 VALU wave
@@ -62,11 +63,13 @@ def make_case(tdbem, w):
                       itf_rows=m.interfaces)
 
 
-def run_once(tdbem, w, restart=200, tol=1e-5, keep=False, tile=16, gmres_device=1, tune_cache=None):
+def run_once(tdbem, w, restart=200, tol=1e-5, keep=False, tile=16, gmres_device=1, tune_cache=None, far_parts=0):
     t0 = time.perf_counter()
     c = make_case(tdbem, w)
     c.set_option("conv_tile", tile)
     c.set_option("gmres_device", gmres_device)
+    if far_parts:
+        c.set_option("far_parts", far_parts)
     c.assemble_step0(10)
     entries = c.assemble_history(w.nstep)
     tuned = c.autotune(tune_cache) if tune_cache else None
@@ -289,6 +292,8 @@ def main():
                     help="GMRES driver: 0 host loop, 1 device state + hipGraphs (default), 2 device state + direct "
                          "launches (for profiler runs)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--far-parts", type=int, default=0, choices=(0, 1, 2, 4, 8),
+                    help="far-convolution wavefronts per row (fixes the shape; 0: autotuned or the default 4)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the config-3 mixed-precision report")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -318,20 +323,21 @@ def main():
     # launch shapes: tuned in the first warm-up run (tdbem_autotune, timed on the device), stored in a cache
     # file of this process, applied from the cache in the timed runs
     tune_cache = None
-    if args.warmup >= 1 and not args.no_autotune:
+    if args.warmup >= 1 and not args.no_autotune and not args.far_parts:
         import tempfile
         tune_cache = os.path.join(tempfile.gettempdir(), f"tdbem_tune_{os.getpid()}.txt")
         if os.path.exists(tune_cache):
             os.remove(tune_cache)
     for i in range(args.warmup):
-        r = run_once(tdbem, w, tile=args.conv_tile, gmres_device=args.gmres_device, tune_cache=tune_cache)
+        r = run_once(tdbem, w, tile=args.conv_tile, gmres_device=args.gmres_device, tune_cache=tune_cache,
+                     far_parts=args.far_parts)
         log(f"warmup {i}: {r['wall']:.2f} s, entries {r['entries']}, launch shapes {r['tuned']}")
     tdbem.synchronize()
     t0 = time.perf_counter()
     runs = []
     for i in range(args.steps):
         runs.append(run_once(tdbem, w, keep=(i == args.steps - 1), tile=args.conv_tile,
-                             gmres_device=args.gmres_device, tune_cache=tune_cache))
+                             gmres_device=args.gmres_device, tune_cache=tune_cache, far_parts=args.far_parts))
         log(f"step {i}: {runs[-1]['wall']:.2f} s")
     tdbem.synchronize()
     elapsed = time.perf_counter() - t0
@@ -346,7 +352,7 @@ def main():
     bytes_launch = conv_far_bytes_per_launch(counts, npairs, c.ne, nhist, w.nstep, args.conv_tile)
     conv_gbs = bytes_launch / conv_avg / 1e9
     loop_time = tm["convolution"] + tm["gmres"]
-    far_parts = last["tuned"][0] if last["tuned"] else 4
+    far_parts = last["tuned"][0] if last["tuned"] else (args.far_parts or 4)
     far_kernel = f"k_conv_far_q, {far_parts} wavefronts per row" if far_parts else "k_conv_far_p"
     out = {
         "metric": "time steps/s (whole job: setup + step-0 + history assembly + time loop), FP64",
@@ -369,7 +375,8 @@ def main():
                    "launch_shapes": ({"far_wavefronts_per_row": last["tuned"][0], "gmres_reduction_blocks":
                                       last["tuned"][1], "picked_by": "autotuner (warm-up run)"}
                                      if last["tuned"] else "defaults")},
-        "breakdown_s": {k: round(float(v), 4) for k, v in tm.items() if k != "conv_far_launches"},
+        "breakdown_s": {k: round(float(v), 4) for k, v in tm.items()
+                        if k != "conv_far_launches" and not k.startswith("adjoint_")},
         "time_loop_steps_per_s": round(w.nstep / max(loop_time, 1e-9), 3),
         # every band entry holds one MTS and one MGS block (computed; the
         # 1e-14 drop rule may leave some empty)

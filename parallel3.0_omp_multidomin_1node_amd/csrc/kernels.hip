@@ -885,6 +885,175 @@ void LaunchCull(const ElemGeo* geo, const int* dom, PairGrid grid, long npairs, 
 }
 
 // ---------------------------------------------------------------------------
+// Band bookkeeping on the device: exclusive prefix sums of the band lengths
+// (entry offsets) and of their 64-lag chunk counts over all pairs, and the
+// per-row lists of near pairs.  Three-phase scan: per-block totals of 4096
+// consecutive values, an in-order scan of those totals, then each block
+// writes its values' offsets.
+// ---------------------------------------------------------------------------
+constexpr int kScanPer = 16;                  // values per thread
+constexpr int kScanBlock = 256 * kScanPer;    // values per block
+
+__device__ __forceinline__ long BlockExclusiveScan(long v, long* sh, long* total)
+{
+    int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+        long u = t >= o ? sh[t - o] : 0;
+        __syncthreads();
+        sh[t] += u;
+        __syncthreads();
+    }
+    long incl = sh[t];
+    *total = sh[255];
+    __syncthreads();
+    return incl - v;
+}
+
+__device__ __forceinline__ long ChunkCount(int l) { return (l + 63) / 64; }
+
+__global__ void __launch_bounds__(256) k_scan_reduce(const int* __restrict__ len, long n, long* __restrict__ bsum,
+                                                     long* __restrict__ bcsum)
+{
+    __shared__ long sh[256];
+    long base = (long)blockIdx.x * kScanBlock + (long)threadIdx.x * kScanPer;
+    long a = 0, c = 0;
+    for (int i = 0; i < kScanPer; ++i) {
+        long p = base + i;
+        int l = p < n ? len[p] : 0;
+        a += l;
+        c += ChunkCount(l);
+    }
+    long ta, tc;
+    BlockExclusiveScan(a, sh, &ta);
+    BlockExclusiveScan(c, sh, &tc);
+    if (threadIdx.x == 0) {
+        bsum[blockIdx.x] = ta;
+        bcsum[blockIdx.x] = tc;
+    }
+}
+
+// in-place exclusive scan of the nb block totals (one block); the grand
+// totals go to bsum[nb], bcsum[nb]
+__global__ void __launch_bounds__(256) k_scan_top(long* __restrict__ bsum, long* __restrict__ bcsum, long nb)
+{
+    __shared__ long sh[256];
+    long per = (nb + 255) / 256;
+    long b0 = (long)threadIdx.x * per;
+    long a = 0, c = 0;
+    for (long i = 0; i < per; ++i)
+        if (b0 + i < nb) {
+            a += bsum[b0 + i];
+            c += bcsum[b0 + i];
+        }
+    long ta, tc;
+    long ea = BlockExclusiveScan(a, sh, &ta);
+    long ec = BlockExclusiveScan(c, sh, &tc);
+    for (long i = 0; i < per; ++i)
+        if (b0 + i < nb) {
+            long va = bsum[b0 + i], vc = bcsum[b0 + i];
+            bsum[b0 + i] = ea;
+            bcsum[b0 + i] = ec;
+            ea += va;
+            ec += vc;
+        }
+    if (threadIdx.x == 0) {
+        bsum[nb] = ta;
+        bcsum[nb] = tc;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_scan_apply(const int* __restrict__ len, long n,
+                                                    const long* __restrict__ bsum, const long* __restrict__ bcsum,
+                                                    long nb, long* __restrict__ off, long* __restrict__ coff)
+{
+    __shared__ long sh[256];
+    long base = (long)blockIdx.x * kScanBlock + (long)threadIdx.x * kScanPer;
+    long a = 0, c = 0;
+    for (int i = 0; i < kScanPer; ++i) {
+        long p = base + i;
+        int l = p < n ? len[p] : 0;
+        a += l;
+        c += ChunkCount(l);
+    }
+    long ta, tc;
+    long ea = BlockExclusiveScan(a, sh, &ta) + bsum[blockIdx.x];
+    long ec = BlockExclusiveScan(c, sh, &tc) + bcsum[blockIdx.x];
+    for (int i = 0; i < kScanPer; ++i) {
+        long p = base + i;
+        if (p >= n)
+            break;
+        int l = len[p];
+        off[p] = ea;
+        coff[p] = ec;
+        ea += l;
+        ec += ChunkCount(l);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        off[n] = bsum[nb];
+        coff[n] = bcsum[nb];
+    }
+}
+
+void LaunchBandOffsets(const int* len, long n, long* work, long* off, long* coff, hipStream_t s)
+{
+    long nb = (n + kScanBlock - 1) / kScanBlock;
+    if (nb < 1)
+        nb = 1;
+    long* bsum = work;
+    long* bcsum = work + nb + 1;
+    k_scan_reduce<<<(unsigned)nb, 256, 0, s>>>(len, n, bsum, bcsum);
+    k_scan_top<<<1, 256, 0, s>>>(bsum, bcsum, nb);
+    k_scan_apply<<<(unsigned)nb, 256, 0, s>>>(len, n, bsum, bcsum, nb, off, coff);
+}
+
+long BandOffsetsWork(long n)
+{
+    long nb = (n + kScanBlock - 1) / kScanBlock;
+    return 2 * (std::max(nb, 1L) + 1);
+}
+
+// Near pairs of each source row (band not empty, first lag <= tile + 1), in
+// field order: one wavefront per row counts them (cnt) and, in the second
+// pass, writes them from the row's offset ptr.
+__global__ void __launch_bounds__(64) k_near_lists(PairGrid grid, const NodeRefs* __restrict__ refs,
+                                                   const int* __restrict__ kb, const int* __restrict__ len, int tile,
+                                                   const long* __restrict__ ptr, int* __restrict__ cnt,
+                                                   long* __restrict__ pairOut, int* __restrict__ fieldOut)
+{
+    int s = blockIdx.x;
+    int lane = threadIdx.x;
+    int d = refs[s].domain;
+    int nd = grid.eleCount[d];
+    int eb = grid.eleBegin[d];
+    long rowStart = grid.pairBase[d] + (long)(s - eb) * nd;
+    long w = ptr ? ptr[s] : 0;
+    int total = 0;
+    for (int q0 = 0; q0 < nd; q0 += 64) {
+        int q = q0 + lane;
+        long p = rowStart + q;
+        bool near = q < nd && len[p] > 0 && kb[p] <= tile + 1;
+        unsigned long long m = __ballot(near);
+        if (ptr && near) {
+            int rank = __popcll(m & ((1ull << lane) - 1ull));
+            pairOut[w + rank] = p;
+            fieldOut[w + rank] = eb + q;
+        }
+        w += __popcll(m);
+        total += __popcll(m);
+    }
+    if (!ptr && lane == 0)
+        cnt[s] = total;
+}
+
+void LaunchNearLists(PairGrid grid, const NodeRefs* refs, const int* kb, const int* len, int tile, int ne,
+                     const long* ptr, int* cnt, long* pairOut, int* fieldOut, hipStream_t s)
+{
+    k_near_lists<<<ne, 64, 0, s>>>(grid, refs, kb, len, tile, ptr, cnt, pairOut, fieldOut);
+}
+
+// ---------------------------------------------------------------------------
 // History quadrature: one lane per (pair, lag) band entry.
 // MTS entry = AddBlock(T2) then AddBlock(T1); MGS entry = AddBlock(U)
 // (DBEM1/multidomain.cpp:2017-2078).

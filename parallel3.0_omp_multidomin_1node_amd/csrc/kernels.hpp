@@ -59,6 +59,15 @@ void LaunchSingular(const ElemGeo* geo, const int* dom, const DynaMat* mats, con
                     double* t1, double* t2, int ne, bool exact, hipStream_t s);
 
 // Causality cull: band [kb, kb+len) of each pair and near-field flag.
+// off[p] / coff[p] = exclusive prefix sums of len and of its 64-lag chunk
+// counts over n pairs, with the totals at off[n] / coff[n]; work holds
+// BandOffsetsWork(n) values.
+void LaunchBandOffsets(const int* len, long n, long* work, long* off, long* coff, hipStream_t s);
+long BandOffsetsWork(long n);
+// Near pairs per source row: with ptr null, cnt[s] = their number; else they
+// are written from pairOut[ptr[s]] (pair) and fieldOut[ptr[s]] (field element).
+void LaunchNearLists(PairGrid grid, const NodeRefs* refs, const int* kb, const int* len, int tile, int ne,
+                     const long* ptr, int* cnt, long* pairOut, int* fieldOut, hipStream_t s);
 void LaunchCull(const ElemGeo* geo, const int* dom, PairGrid grid, long npairs, int maxn, int* kb, int* len,
                 unsigned char* nearFlag, hipStream_t s);
 

@@ -90,6 +90,7 @@ void Case::SaveHistory(const std::string& path)
         double evr[3] = {d.E, d.v, d.Rou};
         Put(f.get(), evr, sizeof(evr), path);
     }
+    HostBands();
     Put(f.get(), h_kb.data(), sizeof(int) * (size_t)npairs, path);
     Put(f.get(), h_len.data(), sizeof(int) * (size_t)npairs, path);
     std::vector<char> stage(std::min(kStage, (size_t)std::max<long>(1, entries) * 72));
@@ -149,6 +150,7 @@ void Case::LoadHistory(const std::string& path)
     Get(f.get(), len.data(), sizeof(int) * len.size(), path);
     // the loading case's own cull must give the same bands
     Cull(fileMaxn);
+    HostBands();
     if (kb != h_kb || len != h_len || entries != counts[2])
         throw std::runtime_error("history file bands differ from this model's causality cull");
     EnsureSelfTables();

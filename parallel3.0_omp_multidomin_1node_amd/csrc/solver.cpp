@@ -352,60 +352,58 @@ void Case::Cull(int maxn)
     d_near.alloc((size_t)npairs);
     double t0 = Now();
     LaunchCull(d_geo.p, d_dom.p, grid, npairs, maxn, d_kb.p, d_len.p, d_near.p, stream);
-    Check(hipStreamSynchronize(stream), "cull");
     Check(hipGetLastError(), "cull launch");
-    h_len.resize((size_t)npairs);
-    h_kb.resize((size_t)npairs);
-    Check(CopyOnStream(h_len.data(), d_len.p, sizeof(int) * (size_t)npairs, hipMemcpyDeviceToHost, stream), "len");
-    Check(CopyOnStream(h_kb.data(), d_kb.p, sizeof(int) * (size_t)npairs, hipMemcpyDeviceToHost, stream), "kb");
-    h_off.resize((size_t)npairs + 1);
-    long acc = 0;
-    for (long p = 0; p < npairs; ++p) {
-        h_off[(size_t)p] = acc;
-        acc += h_len[(size_t)p];
-    }
-    h_off[(size_t)npairs] = acc;
-    entries = acc;
-    d_off.upload(h_off.data(), h_off.size(), stream);
-    {
-        std::vector<long> coff((size_t)npairs + 1);
-        long c = 0;
-        for (long p = 0; p < npairs; ++p) {
-            coff[(size_t)p] = c;
-            c += (h_len[(size_t)p] + 63) / 64;
-        }
-        coff[(size_t)npairs] = c;
-        chunks = c;
-        d_coff.upload(coff.data(), coff.size(), stream);
-    }
+    // entry offsets and 64-lag chunk offsets of the bands, on the device
+    d_off.alloc((size_t)npairs + 1);
+    d_coff.alloc((size_t)npairs + 1);
+    DevBuf<long> work;
+    work.alloc((size_t)BandOffsetsWork(npairs));
+    LaunchBandOffsets(d_len.p, npairs, work.p, d_off.p, d_coff.p, stream);
+    long tot[2] = {0, 0};
+    Check(CopyOnStream(&tot[0], d_off.p + npairs, sizeof(long), hipMemcpyDeviceToHost, stream), "entries");
+    Check(CopyOnStream(&tot[1], d_coff.p + npairs, sizeof(long), hipMemcpyDeviceToHost, stream), "chunks");
+    entries = tot[0];
+    chunks = tot[1];
+    hostBands = false;
     BuildNearLists(convTile);
     culledFor = maxn;
     timing[1] = Now() - t0;
 }
 
+// Host copies of the band starts, lengths and offsets (for export, statistics
+// and persistence), fetched when first needed after a cull.
+void Case::HostBands() const
+{
+    if (hostBands)
+        return;
+    h_len.resize((size_t)npairs);
+    h_kb.resize((size_t)npairs);
+    h_off.resize((size_t)npairs + 1);
+    Check(CopyOnStream(h_len.data(), d_len.p, sizeof(int) * (size_t)npairs, hipMemcpyDeviceToHost, stream), "len");
+    Check(CopyOnStream(h_kb.data(), d_kb.p, sizeof(int) * (size_t)npairs, hipMemcpyDeviceToHost, stream), "kb");
+    Check(CopyOnStream(h_off.data(), d_off.p, sizeof(long) * ((size_t)npairs + 1), hipMemcpyDeviceToHost, stream),
+          "off");
+    hostBands = true;
+}
+
 // Per source row, the pairs whose band starts at a lag <= tile + 1: the only
-// ones the near part of the time-blocked convolution touches.
+// ones the near part of the time-blocked convolution touches (field order).
 void Case::BuildNearLists(int tile)
 {
-    std::vector<long> nearPtr((size_t)ne + 1, 0), nearPair;
-    std::vector<int> nearField;
-    for (int d = 0; d < grid.nd; ++d) {
-        long n = grid.eleCount[d];
-        for (long sl = 0; sl < n; ++sl) {
-            long srow = grid.eleBegin[d] + sl;
-            for (long fl = 0; fl < n; ++fl) {
-                long p = grid.pairBase[d] + sl * n + fl;
-                if (h_len[(size_t)p] > 0 && h_kb[(size_t)p] <= tile + 1) {
-                    nearPair.push_back(p);
-                    nearField.push_back((int)(grid.eleBegin[d] + fl));
-                }
-            }
-            nearPtr[(size_t)srow + 1] = (long)nearPair.size();
-        }
-    }
-    d_nearPtr.upload(nearPtr.data(), nearPtr.size(), stream);
-    d_nearPair.upload(nearPair.data(), nearPair.size(), stream);
-    d_nearField.upload(nearField.data(), nearField.size(), stream);
+    DevBuf<int> cnt;
+    cnt.alloc((size_t)ne);
+    LaunchNearLists(grid, d_refs.p, d_kb.p, d_len.p, tile, (int)ne, nullptr, cnt.p, nullptr, nullptr, stream);
+    d_nearPtr.alloc((size_t)ne + 1);
+    DevBuf<long> work, unused;
+    work.alloc((size_t)BandOffsetsWork(ne));
+    unused.alloc((size_t)ne + 1);
+    LaunchBandOffsets(cnt.p, ne, work.p, d_nearPtr.p, unused.p, stream);
+    long total = 0;
+    Check(CopyOnStream(&total, d_nearPtr.p + ne, sizeof(long), hipMemcpyDeviceToHost, stream), "near count");
+    d_nearPair.alloc((size_t)std::max<long>(1, total));
+    d_nearField.alloc((size_t)std::max<long>(1, total));
+    LaunchNearLists(grid, d_refs.p, d_kb.p, d_len.p, tile, (int)ne, d_nearPtr.p, nullptr, d_nearPair.p,
+                    d_nearField.p, stream);
     Check(hipStreamSynchronize(stream), "near lists");
     nearTile = tile;
 }
@@ -737,6 +735,7 @@ void Case::HistoryStats(long* counts) const
 {
     if (!historyDone)
         throw std::runtime_error("history not assembled");
+    HostBands();
     for (int j = 0; j < maxn; ++j)
         counts[j] = 0;
     for (long p = 0; p < npairs; ++p) {
@@ -790,6 +789,7 @@ void Case::ExportMatrix(int which, int step, std::vector<int>& rp, std::vector<i
         }
         Check(CopyOnStream(h_alive.data(), d_alive.p, h_alive.size(), hipMemcpyDeviceToHost, stream), "alive");
     }
+    HostBands();
     struct Item {
         long row, col;
         long e;

@@ -205,8 +205,10 @@ private:
     DevBuf<long> d_nearPtr, d_nearPair;
     DevBuf<int> d_nearField;
     int nearTile = 0;  // tile the near lists were built for
-    std::vector<int> h_kb, h_len;
-    std::vector<long> h_off;
+    void HostBands() const;
+    mutable bool hostBands = false;  // h_kb, h_len, h_off hold the current cull
+    mutable std::vector<int> h_kb, h_len;
+    mutable std::vector<long> h_off;
     DevBuf<double> d_mts, d_mgs;
     DevBuf<float> d_mts32, d_mgs32;  // mixed precision: FP32 band blocks
     DevBuf<unsigned short> d_mtsLo, d_mgsLo;  // and their bf16 residuals (mixed == 2)

@@ -415,6 +415,10 @@ int tdbem_set_option(tdbem_ctx* ctx, const char* name, double value)
         ctx->c.convPacked = value != 0.0;
         return 0;
     }
+    if (n == "near_per_entry") {
+        ctx->c.nearPerEntry = value != 0.0;
+        return 0;
+    }
     if (n == "far_parts") {
         if (value < 1.0 || value > (double)tdbem::kFarPartsMax || value != (double)(int)value)
             return Fail("far_parts must be a whole number from 1 to 8");

@@ -3357,6 +3357,91 @@ __global__ void __launch_bounds__(64) k_conv_near(const NodeRefs* __restrict__ r
     }
 }
 
+// The same near part with one (pair, lag) entry per lane: four wavefronts per
+// source row take its near pairs in turn, a wavefront holds 64 / L pairs at a
+// time with L = the power of two >= the step's lag limit, and the rows' four
+// wavefront sums are added in order with the far parts.
+template <typename TB>
+__global__ void __launch_bounds__(256) k_conv_near_e(const NodeRefs* __restrict__ refs,
+                                                     const long* __restrict__ nearPtr,
+                                                     const long* __restrict__ nearPair,
+                                                     const int* __restrict__ nearField, const long* __restrict__ off,
+                                                     const int* __restrict__ kb, const int* __restrict__ len,
+                                                     const TB* __restrict__ mts, const TB* __restrict__ mgs,
+                                                     const unsigned short* __restrict__ mtsLo,
+                                                     const unsigned short* __restrict__ mgsLo,
+                                                     const double* __restrict__ gU, const double* __restrict__ gT,
+                                                     int nstore, int K, int K0, int maxn, int ne,
+                                                     const double* __restrict__ hfar, int parts, long partStride,
+                                                     double* __restrict__ rhs)
+{
+    __shared__ double red[4][3];
+    int s = blockIdx.x;
+    int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int jlim = K - K0 + 1;
+    if (jlim > maxn)
+        jlim = maxn;
+    int L = 1;
+    while (L < jlim)
+        L <<= 1;
+    const int per = 64 / L;  // pairs per wavefront pass
+    int slot = lane / L, t = lane - slot * L;
+    double y[3] = {0.0, 0.0, 0.0};
+    long q0 = nearPtr[s], q1 = nearPtr[s + 1];
+    for (long qb = q0 + (long)wave * per; qb < q1; qb += 4L * per) {
+        long q = qb + slot;
+        if (q >= q1)
+            continue;
+        long p = nearPair[q];
+        int k0 = kb[p], Lp = len[p];
+        int j = k0 + t;
+        if (j > jlim || t >= Lp)
+            continue;
+        int f = nearField[q];
+        const double* su = gU + (long)refs[f].hu_block * nstore * 3;
+        const double* st = gT + (long)refs[f].ht_block * nstore * 3;
+        long e = off[p] + t;
+        double M[9], G[6];
+        for (int i = 0; i < 9; ++i)
+            M[i] = (double)mts[9 * e + i] + (mtsLo ? dev::Bf16ToDouble(mtsLo[9 * e + i]) : 0.0);
+        for (int i = 0; i < 6; ++i)
+            G[i] = (double)mgs[6 * e + i] + (mgsLo ? dev::Bf16ToDouble(mgsLo[6 * e + i]) : 0.0);
+        const int mm[3] = {K - j + 1, K - j, K - j - 1};
+        const double co[3] = {1.0, 2.0, 1.0};
+        double cu[3] = {0.0, 0.0, 0.0}, ct[3] = {0.0, 0.0, 0.0};
+        for (int z = 0; z < 3; ++z) {
+            int m = mm[z];
+            if (m >= K0 && m <= K - 1 && m >= 1) {
+                for (int c = 0; c < 3; ++c) {
+                    cu[c] += co[z] * su[3 * (long)m + c];
+                    ct[c] += co[z] * st[3 * (long)m + c];
+                }
+            }
+        }
+        double v[3];
+        dev::BlockApply(M, G, cu, ct, v);
+        y[0] += v[0];
+        y[1] += v[1];
+        y[2] += v[2];
+    }
+    for (int c = 0; c < 3; ++c) {
+        double w = dev::WaveSum(y[c]);
+        if (lane == 0)
+            red[wave][c] = w;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double* hf = hfar + (long)(K - K0) * 3 * ne + 3 * (long)s;
+        for (int c = 0; c < 3; ++c) {
+            double h = hf[c];
+            for (int qq = 1; qq < parts; ++qq)
+                h += hf[qq * partStride + c];
+            double yn = ((red[0][c] + red[1][c]) + red[2][c]) + red[3][c];
+            rhs[3 * (long)s + c] += h + yn;
+        }
+    }
+}
+
 template <typename TB>
 void LaunchConvFar(PairGrid grid, const NodeRefs* refs, const long* off, const int* kb, const int* len, const TB* mts,
                    const TB* mgs, const unsigned short* mtsLo, const unsigned short* mgsLo, const double* gU,
@@ -3375,10 +3460,15 @@ void LaunchConvNear(const NodeRefs* refs, const long* nearPtr, const long* nearP
                     const long* off, const int* kb, const int* len, const TB* mts, const TB* mgs,
                     const unsigned short* mtsLo, const unsigned short* mgsLo, const double* gU, const double* gT,
                     int nstore, int K, int K0, int maxn, int ne, const double* hfar, int parts, double* rhs,
-                    hipStream_t s)
+                    bool perEntry, hipStream_t s)
 {
-    k_conv_near<TB><<<ne, 64, 0, s>>>(refs, nearPtr, nearPair, nearField, off, kb, len, mts, mgs, mtsLo, mgsLo, gU, gT,
-                                      nstore, K, K0, maxn, ne, hfar, parts, (long)kConvTileMax * 3 * ne, rhs);
+    if (perEntry)
+        k_conv_near_e<TB><<<ne, 256, 0, s>>>(refs, nearPtr, nearPair, nearField, off, kb, len, mts, mgs, mtsLo, mgsLo,
+                                             gU, gT, nstore, K, K0, maxn, ne, hfar, parts,
+                                             (long)kConvTileMax * 3 * ne, rhs);
+    else
+        k_conv_near<TB><<<ne, 64, 0, s>>>(refs, nearPtr, nearPair, nearField, off, kb, len, mts, mgs, mtsLo, mgsLo, gU,
+                                          gT, nstore, K, K0, maxn, ne, hfar, parts, (long)kConvTileMax * 3 * ne, rhs);
 }
 
 #define TDBEM_CONV_INSTANTIATE(TB)                                                                                   \
@@ -3388,7 +3478,7 @@ void LaunchConvNear(const NodeRefs* refs, const long* nearPtr, const long* nearP
     template void LaunchConvNear<TB>(const NodeRefs*, const long*, const long*, const int*, const long*, const int*,  \
                                      const int*, const TB*, const TB*, const unsigned short*, const unsigned short*,  \
                                      const double*, const double*, int, int, int, int, int, const double*, int,       \
-                                     double*, hipStream_t);
+                                     double*, bool, hipStream_t);
 TDBEM_CONV_INSTANTIATE(double)
 TDBEM_CONV_INSTANTIATE(float)
 #undef TDBEM_CONV_INSTANTIATE

@@ -165,7 +165,7 @@ void LaunchConvNear(const NodeRefs* refs, const long* nearPtr, const long* nearP
                     const long* off, const int* kb, const int* len, const TB* mts, const TB* mgs,
                     const unsigned short* mtsLo, const unsigned short* mgsLo, const double* gU, const double* gT,
                     int nstore, int K, int K0, int maxn, int ne, const double* hfar, int parts, double* rhs,
-                    hipStream_t s);
+                    bool perEntry, hipStream_t s);
 
 // GMRES linear algebra.  TA: matrix / inverse storage, TX / TY: vector
 // storage (double, or float for the inner solves of the mixed mode); every

@@ -1228,11 +1228,11 @@ void Case::Solve(int nstep, const double* known_lu, long restart, double tol, do
                     LaunchConvNear(d_refs.p, d_nearPtr.p, d_nearPair.p, d_nearField.p, d_off.p, d_kb.p, d_len.p,
                                    (const float*)d_mts32.p, (const float*)d_mgs32.p, (const unsigned short*)d_mtsLo.p,
                                    (const unsigned short*)d_mgsLo.p, d_gU.p, d_gT.p, nstore, K, K0, maxn, (int)ne,
-                                   d_hfar.p, hfarParts, d_rhs.p, stream);
+                                   d_hfar.p, hfarParts, d_rhs.p, nearPerEntry, stream);
                 else
                     LaunchConvNear(d_refs.p, d_nearPtr.p, d_nearPair.p, d_nearField.p, d_off.p, d_kb.p, d_len.p,
                                    (const double*)d_mts.p, (const double*)d_mgs.p, nullptr, nullptr, d_gU.p, d_gT.p,
-                                   nstore, K, K0, maxn, (int)ne, d_hfar.p, hfarParts, d_rhs.p, stream);
+                                   nstore, K, K0, maxn, (int)ne, d_hfar.p, hfarParts, d_rhs.p, nearPerEntry, stream);
             } else {
                 LaunchConvolution(grid, d_refs.p, d_off.p, d_kb.p, d_len.p, d_mts.p, d_mgs.p, d_gU.p, d_gT.p, nstore,
                                   K, maxn, (int)ne, d_rhs.p, stream);

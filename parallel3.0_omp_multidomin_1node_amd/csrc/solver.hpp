@@ -153,6 +153,7 @@ public:
     bool convFiltered = true;  // 16-step far tiles read a pre-filtered state series and prefetch work items
     bool convPacked = true;    // filtered far tiles pack 64 band entries per work item across pairs
     int farParts = 4;          // wavefronts (and partial sums) per source row of the packed far part
+    bool nearPerEntry = true;  // near part: one (pair, lag) entry per lane (false: one pair per lane)
     int gmBlocks = 32;         // reduction blocks of the GMRES dot products (at most; <= kGmBlocksMax)
     int gmresDevice = 1;       // FP64 GMRES: 0 host-driven, 1 device-driven hipGraphs, 2 device-driven launches
     int selfMode = -1;       // kernel of the current self tables: 1 wave per element, 0 reference order

@@ -291,7 +291,8 @@ double Case::Adjoint(int nstep, const double* known_lu, const int* recv, int nre
             }
             int it = 0;
             int flag;
-            if (gmresDevice) {  // device-driven; solved into a fixed buffer so the captured graphs stay valid
+            if (gmresDevice && restart <= kGmDeviceMaxRestart) {
+                // device-driven; solved into a fixed buffer so the captured graphs stay valid
                 flag = GmresDevice(d_y.p, d_xs.p, restart, tol, &it, &opT);
                 CheckA(hipMemcpyAsync(d_lam.p + (size_t)K * n3, d_xs.p, sizeof(double) * (size_t)n3,
                                       hipMemcpyDeviceToDevice, stream),

@@ -161,34 +161,30 @@ __global__ void __launch_bounds__(256) k_gm_multidot(const GmState* st, const do
     }
 }
 
-// h[off + k] = sum of the partials of column k, k < j (all) or k = 0 with
-// the square root written to h[st->j + off] (norm)
-__global__ void k_gm_finish(const GmState* st, const double* __restrict__ partial, int nblocks, int all,
-                            double* __restrict__ h, int off)
-{
-    if (Done(st))
-        return;
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (all) {
-        int nv = NextJ(st);
-        if (k < nv)
-            h[off + k] = SumPartials(partial, nblocks, nv, k);
-    } else if (k == 0) {
-        h[NextJ(st) - 1 + off] = sqrt(SumPartials(partial, nblocks, 1, 0));
-    }
-}
-
-// u -= V(:, 0 .. j-1) h(0 .. j-1); htot[0 .. j-1] += h when given
+// u -= V(:, 0 .. j-1) h(0 .. j-1), the coefficients h(k) summed from the
+// multi-dot partials by every block (the order of k_finish); block 0 stores
+// them at hout and, when given, adds them to htot
 __global__ void __launch_bounds__(256) k_gm_multiaxpy(const GmState* st, const double* __restrict__ V, long ldv,
-                                                      const double* __restrict__ h, double* __restrict__ u, long n,
+                                                      const double* __restrict__ partial, int nblocks,
+                                                      double* __restrict__ hout, double* __restrict__ u, long n,
                                                       double* htot)
 {
     if (Done(st))
         return;
+    __shared__ double h[256];
     int nv = NextJ(st);
-    long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (htot && r < nv)
-        htot[r] += h[r];
+    int tid = threadIdx.x;
+    if (tid < nv) {
+        double v = SumPartials(partial, nblocks, nv, tid);
+        h[tid] = v;
+        if (blockIdx.x == 0) {
+            hout[tid] = v;
+            if (htot)
+                htot[tid] += v;
+        }
+    }
+    __syncthreads();
+    long r = (long)blockIdx.x * blockDim.x + tid;
     if (r >= n)
         return;
     double a = u[r];
@@ -197,37 +193,42 @@ __global__ void __launch_bounds__(256) k_gm_multiaxpy(const GmState* st, const d
     u[r] = a;
 }
 
-// V(:, j - 1 + colOff) = u / h[j - 1 + denOff] (tiny denominators give 0)
-__global__ void __launch_bounds__(256) k_gm_scale(const GmState* st, const double* __restrict__ u,
-                                                  const double* __restrict__ h, int denOff, double* __restrict__ V,
-                                                  int colOff, long n)
+// h[1] = ||u|| from the partials (every block, the order of k_finish),
+// V(:,0) = u / h[1]; block 0 also starts the Givens sequence: QT(1,1) = 1,
+// phibar = h[1]
+__global__ void __launch_bounds__(256) k_gm_open(const GmState* st, const double* __restrict__ partial, int nblocks,
+                                                 const double* __restrict__ u, double* __restrict__ V, long n,
+                                                 double* h, double* QT, double* phibar)
 {
     if (Done(st))
         return;
+    __shared__ double hs;
+    if (threadIdx.x == 0)
+        hs = sqrt(SumPartials(partial, nblocks, 1, 0));
+    __syncthreads();
+    double s = hs;
     long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n)
-        return;
-    int j = NextJ(st);
-    double s = h[j - 1 + denOff];
-    V[(long)(j - 1 + colOff) * n + r] = s <= 1.0e-12 ? 0.0 : u[r] / s;
-}
-
-// QT(1,1) = 1, phibar = h[1] (start of the Givens sequence)
-__global__ void k_gm_start(const GmState* st, double* QT, const double* h, double* phibar)
-{
-    if (Done(st) || threadIdx.x != 0)
-        return;
-    QT[0] = 1.0;
-    phibar[0] = h[1];
+    if (r < n)
+        V[r] = s <= 1.0e-12 ? 0.0 : u[r] / s;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        h[1] = s;
+        QT[0] = 1.0;
+        phibar[0] = s;
+    }
 }
 
 // Givens rotation in QT form for column j (DBEM1/pre_gmres.cpp:1492-1533)
 __global__ void __launch_bounds__(256) k_gm_givens(const GmState* st, double* h, double* QT, double* R, double* f,
-                                                   double* phibar, int* stag, int restart)
+                                                   double* phibar, int* stag, int restart,
+                                                   const double* __restrict__ partial, int nblocks)
 {
     if (Done(st))
         return;
     const int j = NextJ(st);
+    // h[j+1] = ||u|| from the partials of u.u
+    if (threadIdx.x == 0)
+        h[j + 1] = sqrt(SumPartials(partial, nblocks, 1, 0));
+    __syncthreads();
     const int lq = restart + 1;
 #define QQ(a, b) QT[((a)-1) * lq + ((b)-1)]
 #define RR(a, b) R[((a)-1) * restart + ((b)-1)]
@@ -272,7 +273,7 @@ __global__ void __launch_bounds__(256) k_gm_givens(const GmState* st, double* h,
 
 // W(:,j) = (V(:,j) - sum_{l<j} W(:,l) R(l,j)) / R(j,j);  x += f_j W(:,j)
 // (1-based columns), together with the next Krylov vector
-// V(:, j+1) = u / h[j+1] (the scale of k_gm_scale)
+// V(:, j+1) = u / h[j+1] (tiny denominators give 0, as the opening's scale)
 __global__ void __launch_bounds__(256) k_gm_wupdate(const GmState* st, double* __restrict__ V,
                                                     double* __restrict__ W, long n, const double* __restrict__ R,
                                                     int restart, const double* __restrict__ f, double* __restrict__ x,
@@ -347,28 +348,22 @@ void LaunchGmresBegin(const GmresDevArgs& a, const double* b, double tol, hipStr
     // u = M^-1 r with r = b; h[1] = ||u||; V(:,0) = u / h[1]
     LaunchPre(a, b, a.u, s);
     k_gm_multidot<<<a.nblocks, 256, 0, s>>>(a.st, a.u, n, 0, a.u, n, a.part);
-    k_gm_finish<<<1, 64, 0, s>>>(a.st, a.part, a.nblocks, 0, a.h, 1);
-    k_gm_scale<<<Blocks(n), 256, 0, s>>>(a.st, a.u, a.h, 1, a.V, 0, n);
-    k_gm_start<<<1, 64, 0, s>>>(a.st, a.QT, a.h, a.phibar);
+    k_gm_open<<<Blocks(n), 256, 0, s>>>(a.st, a.part, a.nblocks, a.u, a.V, n, a.h, a.QT, a.phibar);
 }
 
 void LaunchGmresIteration(const GmresDevArgs& a, const double* b, hipStream_t s)
 {
     const long n = 3 * a.ne;
-    const unsigned fin = (unsigned)((a.restart + 63) / 64);
     k_gm_spmv<<<Blocks(a.ne), 256, 0, s>>>(a.st, a.ne, a.rp, a.col, a.val, a.V, n, 0, nullptr, a.u2);
     LaunchPre(a, a.u2, a.u, s);
     // classical Gram-Schmidt twice
     k_gm_multidot<<<a.nblocks, 256, 0, s>>>(a.st, a.V, n, 1, a.u, n, a.part);
-    k_gm_finish<<<fin, 64, 0, s>>>(a.st, a.part, a.nblocks, 1, a.h, 1);
-    k_gm_multiaxpy<<<Blocks(std::max(n, a.restart)), 256, 0, s>>>(a.st, a.V, n, a.h + 1, a.u, n, nullptr);
+    k_gm_multiaxpy<<<Blocks(n), 256, 0, s>>>(a.st, a.V, n, a.part, a.nblocks, a.h + 1, a.u, n, nullptr);
     k_gm_multidot<<<a.nblocks, 256, 0, s>>>(a.st, a.V, n, 1, a.u, n, a.part);
-    k_gm_finish<<<fin, 64, 0, s>>>(a.st, a.part, a.nblocks, 1, a.h2, 0);
-    k_gm_multiaxpy<<<Blocks(std::max(n, a.restart)), 256, 0, s>>>(a.st, a.V, n, a.h2, a.u, n, a.h + 1);
-    // h[j+1] = ||u||; Givens; V(:,j) = u / h[j+1] with the W update and x
+    k_gm_multiaxpy<<<Blocks(n), 256, 0, s>>>(a.st, a.V, n, a.part, a.nblocks, a.h2, a.u, n, a.h + 1);
+    // h[j+1] = ||u|| with the Givens rotation; V(:,j) = u / h[j+1] with the W update and x
     k_gm_multidot<<<a.nblocks, 256, 0, s>>>(a.st, a.u, n, 0, a.u, n, a.part);
-    k_gm_finish<<<1, 64, 0, s>>>(a.st, a.part, a.nblocks, 0, a.h, 2);
-    k_gm_givens<<<1, 256, 0, s>>>(a.st, a.h, a.QT, a.R, a.f, a.phibar, a.stag, (int)a.restart);
+    k_gm_givens<<<1, 256, 0, s>>>(a.st, a.h, a.QT, a.R, a.f, a.phibar, a.stag, (int)a.restart, a.part, a.nblocks);
     k_gm_wupdate<<<Blocks(n), 256, 0, s>>>(a.st, a.V, a.W, n, a.R, (int)a.restart, a.f, a.x, a.u, a.h);
     // true residual b - A x and its norm
     k_gm_spmv<<<Blocks(a.ne), 256, 0, s>>>(a.st, a.ne, a.rp, a.col, a.val, a.x, n, -1, b, a.vh);

@@ -16,6 +16,10 @@ struct GmState {
     int pad;
 };
 
+// Longest restart the device-driven solve takes (coefficients of one Arnoldi
+// step are held in LDS); longer restarts use the host-driven loop.
+constexpr long kGmDeviceMaxRestart = 256;
+
 // Operands and workspace of one solve (all device pointers, 1-based h).
 struct GmresDevArgs {
     long ne, restart;

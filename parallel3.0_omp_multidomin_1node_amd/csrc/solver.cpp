@@ -1241,8 +1241,8 @@ void Case::Solve(int nstep, const double* known_lu, long restart, double tol, do
         Check(hipEventRecord(se[1], stream), "event");
         int it = 0;
         int flag = historyMixed ? GmresMixed(d_rhs.p, d_x.p, restart, tol, &it)
-                   : gmresDevice ? GmresDevice(d_rhs.p, d_x.p, restart, tol, &it)
-                                 : Gmres(d_rhs.p, d_x.p, restart, tol, &it);
+                   : gmresDevice && restart <= kGmDeviceMaxRestart ? GmresDevice(d_rhs.p, d_x.p, restart, tol, &it)
+                                                                   : Gmres(d_rhs.p, d_x.p, restart, tol, &it);
         Check(hipEventRecord(se[2], stream), "event");
         stepEv.push_back(se);
         iters[K - 1] = it;

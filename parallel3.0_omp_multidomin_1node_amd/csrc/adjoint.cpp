@@ -59,31 +59,6 @@ CCSRMat TransposeBlocks(const CCSRMat& M, long n)
     return T;
 }
 
-// Block (row, col) of M, or null.
-const double* FindBlock(const CCSRMat& M, long row, int col)
-{
-    auto b = M.col_index.begin() + M.row_ptr[(size_t)row];
-    auto e = M.col_index.begin() + M.row_ptr[(size_t)row + 1];
-    auto it = std::lower_bound(b, e, col);
-    if (it == e || *it != col)
-        return nullptr;
-    return &M.value[9 * (size_t)(it - M.col_index.begin())];
-}
-
-// Central difference (P - M) / (2 delta_row) on the pattern of C.
-CCSRMat DiffOnPattern(const CCSRMat& C, const CCSRMat& P, const CCSRMat& Mm, const std::vector<double>& inv2delta)
-{
-    CCSRMat D = C;
-    for (long r = 0; r < (long)C.row_ptr.size() - 1; ++r)
-        for (int k = C.row_ptr[(size_t)r]; k < C.row_ptr[(size_t)r + 1]; ++k) {
-            const double* a = FindBlock(P, r, C.col_index[(size_t)k]);
-            const double* b = FindBlock(Mm, r, C.col_index[(size_t)k]);
-            for (int q = 0; q < 9; ++q)
-                D.value[9 * (size_t)k + q] = ((a ? a[q] : 0.0) - (b ? b[q] : 0.0)) * inv2delta[(size_t)r];
-        }
-    return D;
-}
-
 struct DevCCSR {
     DevBuf<int> rp, col;
     DevBuf<double> val;
@@ -321,17 +296,11 @@ double Case::Adjoint(int nstep, const double* known_lu, const int* recv, int nre
         FreezeStructure(true);
     std::vector<double> theta((size_t)nd * 3);
     GetMaterials(theta.data());
-    DevBuf<double> d_dM, d_dG, d_acc, d_v, d_y1, d_y2, d_kn, d_C;
+    DevBuf<double> d_dM, d_dG, d_acc, d_C;
     DevBuf<unsigned char> d_aliveS;
     d_acc.alloc(3 * (size_t)ne);
     LaunchZero(d_acc.p, (long)d_acc.n, stream);
-    d_v.alloc((size_t)n3);
-    d_y1.alloc((size_t)n3);
-    d_y2.alloc((size_t)n3);
-    d_kn.alloc(3 * (size_t)std::max<long>(1, dof.knownBlockCount));
-    LaunchZero(d_kn.p, (long)d_kn.n, stream);
     std::vector<double> delta((size_t)nd), inv2row((size_t)ne), scaleDom((size_t)nd);
-    DevCCSR dA, dK;
     // filtered state series for the entry-wise history contraction
     DevBuf<double> d_FU, d_FT, d_part, d_scale;
     d_FU.alloc(std::max<size_t>(1, (size_t)dof.historyUBlockCount * 3 * (size_t)nstore));
@@ -351,6 +320,42 @@ double Case::Adjoint(int nstep, const double* known_lu, const int* recv, int nre
         d_part.alloc(std::max<size_t>(1, (size_t)chunks));
         d_scale.alloc((size_t)nd);
     }
+    // step-0 passes at the perturbed materials over the near pairs of the
+    // frozen structure, contracted on the device (LaunchStep0Grad)
+    std::vector<long> nlist = NearPairs();
+    const long nnear = (long)nlist.size();
+    std::vector<long> nearRow((size_t)ne + 1, 0);
+    for (long i = 0; i < nnear; ++i) {
+        int s0, f0, d0;
+        long p0 = nlist[(size_t)i];
+        d0 = 0;
+        while (d0 + 1 < grid.nd && p0 >= grid.pairBase[d0 + 1])
+            ++d0;
+        s0 = grid.eleBegin[d0] + (int)((p0 - grid.pairBase[d0]) / grid.eleCount[d0]);
+        (void)f0;
+        nearRow[(size_t)s0 + 1]++;
+    }
+    for (long r = 0; r < ne; ++r)
+        nearRow[(size_t)r + 1] += nearRow[(size_t)r];
+    DevBuf<long> d_nlist, d_nrow;
+    DevBuf<double> d_outP, d_outM, d_gpair, d_inv2, d_knAll;
+    DevBuf<int> d_flagP, d_flagM;
+    d_nlist.upload(nlist.data(), std::max<size_t>(1, nlist.size()), stream);
+    d_nrow.upload(nearRow.data(), nearRow.size(), stream);
+    d_outP.alloc(18 * (size_t)std::max<long>(1, nnear));
+    d_outM.alloc(18 * (size_t)std::max<long>(1, nnear));
+    d_flagP.alloc(2 * (size_t)std::max<long>(1, nnear));
+    d_flagM.alloc(2 * (size_t)std::max<long>(1, nnear));
+    d_gpair.alloc((size_t)std::max<long>(1, nnear));
+    const long nk3 = 3 * std::max<long>(1, dof.knownBlockCount);
+    d_knAll.alloc((size_t)nstore * (size_t)nk3);
+    LaunchZero(d_knAll.p, (long)d_knAll.n, stream);
+    LaunchKnownAll(d_refs.p, kp.bd.p, (int)ne, nstore, nk3, d_knAll.p, stream);
+    auto step0Pass = [&](double* out, int* flags) {
+        EnsureSelfTables();
+        LaunchStep0(d_geo.p, d_pts.p, d_dom.p, d_refs.p, d_u1.p, d_t1.p, d_t2.p, grid, d_nlist.p, nnear,
+                    frozen ? d_cls0.p : nullptr, out, flags, stream);
+    };
     std::vector<double> t2p, t2m;
     for (int p = 0; p < 3; ++p) {
         std::vector<double> tp = theta, tm = theta;
@@ -362,9 +367,8 @@ double Case::Adjoint(int nstep, const double* known_lu, const int* recv, int nre
         }
         for (long s = 0; s < ne; ++s)
             inv2row[(size_t)s] = 1.0 / (2.0 * delta[(size_t)domOf[(size_t)s]]);
-        CCSRMat Ap, Kp, Am, Km;
         SetMaterials(tp.data());
-        ComputeStep0(Ap, Kp);
+        step0Pass(d_outP.p, d_flagP.p);
         if (analytic) {
             t2p.resize(9 * (size_t)ne);
             CheckA(CopyOnStream(t2p.data(), d_t2.p, sizeof(double) * t2p.size(),
@@ -374,7 +378,7 @@ double Case::Adjoint(int nstep, const double* known_lu, const int* recv, int nre
             StagedQuadrature(d_dM.p, d_dG.p, nullptr, nullptr, d_aliveS.p);
         }
         SetMaterials(tm.data());
-        ComputeStep0(Am, Km);
+        step0Pass(d_outM.p, d_flagM.p);
         if (analytic) {
             t2m.resize(9 * (size_t)ne);
             CheckA(CopyOnStream(t2m.data(), d_t2.p, sizeof(double) * t2m.size(), hipMemcpyDeviceToHost, stream),
@@ -394,18 +398,11 @@ double Case::Adjoint(int nstep, const double* known_lu, const int* recv, int nre
                          stream);
             LaunchRowChunks(grid, d_refs.p, d_coff.p, d_part.p, (int)ne, d_acc.p + (size_t)p * ne, stream);
         }
-        // step-0 blocks: lambda_K . (dKnownM k_K - dA x_K)
-        dA.Upload(DiffOnPattern(A, Ap, Am, inv2row), stream);
-        dK.Upload(DiffOnPattern(KnownM, Kp, Km, inv2row), stream);
-        for (int K = 1; K <= nstep; ++K) {
-            LaunchZero(d_v.p, n3, stream);
-            LaunchKnown(d_refs.p, kp.bd.p + (size_t)K * n3, (int)ne, d_kn.p, dof.knownBlockCount, stream);
-            LaunchSpmv(ne, dK.rp.p, dK.col.p, (const double*)dK.val.p, (const double*)d_kn.p, d_y1.p, stream);
-            LaunchSpmv(ne, dA.rp.p, dA.col.p, (const double*)dA.val.p, (const double*)(kp.X.p + (size_t)K * n3),
-                       d_y2.p, stream);
-            LaunchAddSub(d_v.p, d_y1.p, d_y2.p, n3, stream);
-            LaunchRowDotAcc(d_lam.p + (size_t)K * n3, d_v.p, ne, d_acc.p + (size_t)p * ne, stream);
-        }
+        // step-0 blocks: lambda_K . (dKnownM k_K - dA x_K), summed over K per near pair
+        d_inv2.upload(inv2row.data(), inv2row.size(), stream);
+        LaunchStep0Grad(d_nlist.p, nnear, grid, d_refs.p, d_outP.p, d_flagP.p, d_outM.p, d_flagM.p, d_inv2.p,
+                        d_lam.p, kp.X.p, n3, d_knAll.p, nk3, nstep, d_gpair.p, d_nrow.p, (int)ne,
+                        d_acc.p + (size_t)p * ne, stream);
         CheckA(hipStreamSynchronize(stream), "gradient");
     }
     // restore the materials and self tables (and, after differences, the bands)

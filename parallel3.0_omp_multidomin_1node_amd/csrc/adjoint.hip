@@ -593,6 +593,126 @@ void LaunchCorrStore(PairGrid grid, const NodeRefs* refs, const long* coff, cons
     }
 }
 
+// Known vectors of every step: kn[K][3 blk + c] = sign * bd[K][3 i + c]
+// (the map of k_known for all K at once).
+__global__ void __launch_bounds__(256) k_known_all(const NodeRefs* __restrict__ refs, const double* __restrict__ bd,
+                                                   int ne, int nstore, long nk3, double* __restrict__ kn)
+{
+    long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)ne * nstore)
+        return;
+    int K = (int)(idx / ne), i = (int)(idx - (long)K * ne);
+    const NodeRefs& r = refs[i];
+    if (r.surface == 1)
+        return;
+    int blk = r.bcid == 123 ? r.u_block : r.t_block;
+    int known = r.bcid == 123 ? r.u_known : r.t_known;
+    double sg = r.bcid == 123 ? r.u_sign : r.t_sign;
+    if (!known)
+        return;
+    for (int c = 0; c < 3; ++c)
+        kn[(long)K * nk3 + 3 * (long)blk + c] = sg * bd[((long)K * ne + i) * 3 + c];
+}
+
+void LaunchKnownAll(const NodeRefs* refs, const double* bd, int ne, int nstore, long nk3, double* kn, hipStream_t s)
+{
+    long n = (long)ne * nstore;
+    k_known_all<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(refs, bd, ne, nstore, nk3, kn);
+}
+
+// Step-0 part of one parameter's gradient per near pair i: the difference of
+// the two perturbed step-0 passes (outP - outM, scaled by inv2[s]) placed as
+// the unknown / known builders place the blocks (stored transposed, times the
+// DOF sign; DBEM1/multidomain.cpp:1907-1937,2286-2321), contracted with
+// sum_K lambda_K[s] (x) x_K[col] (unknown columns, -dA x_K) or k_K[col]
+// (known columns, +dKnownM k_K).  A block counts when either pass has it.
+__global__ void __launch_bounds__(256) k_step0_grad(const long* __restrict__ list, long n, PairGrid grid,
+                                                    const NodeRefs* __restrict__ refs,
+                                                    const double* __restrict__ outP, const int* __restrict__ flagP,
+                                                    const double* __restrict__ outM, const int* __restrict__ flagM,
+                                                    const double* __restrict__ inv2, const double* __restrict__ lam,
+                                                    const double* __restrict__ X, long n3,
+                                                    const double* __restrict__ kn, long nk3, int N,
+                                                    double* __restrict__ gpair)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    int s, f, d;
+    DecodePairOf(grid, list[i], s, f, d);
+    const NodeRefs& r = refs[f];
+    double g = 0.0;
+    for (int which = 0; which < 2; ++which) {
+        if (!(flagP[2 * i + which] || flagM[2 * i + which]))
+            continue;
+        int col;
+        double sgn;
+        bool unknownCol;
+        if (r.surface) {
+            col = which == 0 ? r.u_block : r.t_block;
+            sgn = which == 0 ? r.u_sign : r.t_sign;
+            unknownCol = true;
+        } else if (which == 0) {
+            bool useT = r.bcid == 123;
+            if (useT ? r.t_known : r.u_known)
+                continue;
+            col = useT ? r.t_block : r.u_block;
+            sgn = useT ? r.t_sign : r.u_sign;
+            unknownCol = true;
+        } else {
+            bool useU = r.bcid == 123;
+            if (!(useU ? r.u_known : r.t_known))
+                continue;
+            col = useU ? r.u_block : r.t_block;
+            sgn = useU ? r.u_sign : r.t_sign;
+            unknownCol = false;
+        }
+        double db[9];
+        for (int q = 0; q < 9; ++q)
+            db[q] = (outP[18 * i + 9 * which + q] - outM[18 * i + 9 * which + q]) * inv2[s];
+        // C[r][c] = sum_K lambda_K[s][r] v_K[col][c]
+        double C[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int K = 1; K <= N; ++K) {
+            const double* l = lam + (long)K * n3 + 3 * (long)s;
+            const double* v = unknownCol ? X + (long)K * n3 + 3 * (long)col : kn + (long)K * nk3 + 3 * (long)col;
+            for (int a = 0; a < 3; ++a)
+                for (int c = 0; c < 3; ++c)
+                    C[3 * a + c] = fma(l[a], v[c], C[3 * a + c]);
+        }
+        // stored block M[a][c] = sgn * db[3 c + a]
+        double t = 0.0;
+        for (int a = 0; a < 3; ++a)
+            for (int c = 0; c < 3; ++c)
+                t += db[3 * c + a] * C[3 * a + c];
+        g += (unknownCol ? -sgn : sgn) * t;
+    }
+    gpair[i] = g;
+}
+
+// acc[s] += sum of gpair over the near pairs of row s (rowPtr, in list order)
+__global__ void __launch_bounds__(256) k_pair_rows(const double* __restrict__ gpair, const long* __restrict__ rowPtr,
+                                                   int ne, double* __restrict__ acc)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ne)
+        return;
+    double a = 0.0;
+    for (long k = rowPtr[s]; k < rowPtr[s + 1]; ++k)
+        a += gpair[k];
+    acc[s] += a;
+}
+
+void LaunchStep0Grad(const long* list, long n, PairGrid grid, const NodeRefs* refs, const double* outP,
+                     const int* flagP, const double* outM, const int* flagM, const double* inv2, const double* lam,
+                     const double* X, long n3, const double* kn, long nk3, int N, double* gpair,
+                     const long* rowPtr, int ne, double* acc, hipStream_t s)
+{
+    if (n > 0)
+        k_step0_grad<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(list, n, grid, refs, outP, flagP, outM, flagM, inv2,
+                                                                  lam, X, n3, kn, nk3, N, gpair);
+    k_pair_rows<<<(ne + 255) / 256, 256, 0, s>>>(gpair, rowPtr, ne, acc);
+}
+
 // acc[s] += sum of the chunk partials of row s (its pairs are contiguous).
 __global__ void __launch_bounds__(256) k_row_chunks(PairGrid grid, const NodeRefs* __restrict__ refs,
                                                     const long* __restrict__ coff, const double* __restrict__ part,

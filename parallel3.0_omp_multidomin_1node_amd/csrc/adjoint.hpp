@@ -50,5 +50,14 @@ void LaunchCorrStore(PairGrid grid, const NodeRefs* refs, const long* coff, cons
                      hipStream_t s);
 void LaunchRowChunks(PairGrid grid, const NodeRefs* refs, const long* coff, const double* part, int ne, double* acc,
                      hipStream_t s);
+// kn[K][...] = the known vector of step K for K = 0 .. nstore - 1 (nk3 values each).
+void LaunchKnownAll(const NodeRefs* refs, const double* bd, int ne, int nstore, long nk3, double* kn, hipStream_t s);
+// Step-0 part of one parameter's gradient from the two perturbed step-0
+// passes over the near pairs in list: acc[s] += per row s (rowPtr) the
+// contraction with the adjoint and forward histories.
+void LaunchStep0Grad(const long* list, long n, PairGrid grid, const NodeRefs* refs, const double* outP,
+                     const int* flagP, const double* outM, const int* flagM, const double* inv2, const double* lam,
+                     const double* X, long n3, const double* kn, long nk3, int N, double* gpair,
+                     const long* rowPtr, int ne, double* acc, hipStream_t s);
 
 }  // namespace tdbem

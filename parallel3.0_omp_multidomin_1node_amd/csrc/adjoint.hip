@@ -515,6 +515,12 @@ __global__ void __launch_bounds__(64) k_corr_store(PairGrid grid, const NodeRefs
                                                    const double* __restrict__ gT, int nstore, int N,
                                                    double* __restrict__ C)
 {
+    // steps K are taken kCorrSeg at a time: lambda_K[s] of the segment and
+    // the filtered-state rows its lags read are staged in LDS
+    constexpr int kSeg = 256;
+    constexpr int kRowsMax = kSeg + 63;
+    __shared__ double sl[kSeg * 3];
+    __shared__ double sx[kRowsMax * 6];
     long wv = c0 + (long)blockIdx.x;
     if (wv >= c1)
         return;
@@ -531,40 +537,58 @@ __global__ void __launch_bounds__(64) k_corr_store(PairGrid grid, const NodeRefs
     int s, f, d;
     DecodePairOf(grid, p, s, f, d);
     int t0 = (int)(wv - coff[p]) * 64;
+    int L = len[p];
     int t = t0 + lane;
-    bool in = t < len[p];
+    bool in = t < L;
     int j = kb[p] + t, jlo = kb[p] + t0;
+    int jhi = kb[p] + (t0 + 63 < L - 1 ? t0 + 63 : L - 1);
     const double* fu = FU + (long)refs[f].hu_block * (N + 1) * 3;
     const double* ft = FT + (long)refs[f].ht_block * (N + 1) * 3;
+    const double* gu1 = gU + (long)refs[f].hu_block * nstore * 3;
+    const double* gt1 = gT + (long)refs[f].ht_block * nstore * 3;
     double cu[9], ct[9];
     for (int i = 0; i < 9; ++i) {
         cu[i] = 0.0;
         ct[i] = 0.0;
     }
-    for (int K = jlo; K <= N; ++K) {
-        const double* l = lam + (long)K * n3 + 3 * (long)s;
-        double l0 = l[0], l1 = l[1], l2 = l[2];
-        if (in && K >= j) {
-            int tt = K - j;
-            double u[3] = {fu[3 * tt], fu[3 * tt + 1], fu[3 * tt + 2]};
-            double v[3] = {ft[3 * tt], ft[3 * tt + 1], ft[3 * tt + 2]};
-            if (j == 1) {
-                const double* gu = gU + ((long)refs[f].hu_block * nstore + K) * 3;
-                const double* gt = gT + ((long)refs[f].ht_block * nstore + K) * 3;
+    for (int K1 = jlo; K1 <= N; K1 += kSeg) {
+        int K2 = K1 + kSeg - 1 < N ? K1 + kSeg - 1 : N;
+        // lambda_K[s], K = K1 .. K2
+        for (int idx = lane; idx < (K2 - K1 + 1) * 3; idx += 64)
+            sl[idx] = lam[(long)(K1 + idx / 3) * n3 + 3 * (long)s + idx % 3];
+        // rows tt = K - j of the chunk's lags: rlo = K1 - jhi .. K2 - jlo, clipped to >= 0
+        int rlo = K1 - jhi, rhi = K2 - jlo;
+        int r0 = rlo > 0 ? rlo : 0;
+        for (int idx = lane; idx < (rhi - r0 + 1) * 6; idx += 64) {
+            int rr = r0 + idx / 6, c = idx % 6;
+            sx[idx] = c < 3 ? fu[3 * (long)rr + c] : ft[3 * (long)rr + (c - 3)];
+        }
+        __syncthreads();
+        if (in) {
+            int Kb = j > K1 ? j : K1;
+            for (int K = Kb; K <= K2; ++K) {
+                const double* l = sl + 3 * (K - K1);
+                const double* x = sx + 6 * (K - j - r0);
+                double l0 = l[0], l1 = l[1], l2 = l[2];
+                double u[3] = {x[0], x[1], x[2]};
+                double v[3] = {x[3], x[4], x[5]};
+                if (j == 1) {
+                    for (int c = 0; c < 3; ++c) {
+                        u[c] -= gu1[3 * (long)K + c];
+                        v[c] -= gt1[3 * (long)K + c];
+                    }
+                }
                 for (int c = 0; c < 3; ++c) {
-                    u[c] -= gu[c];
-                    v[c] -= gt[c];
+                    cu[c] = fma(l0, u[c], cu[c]);
+                    cu[3 + c] = fma(l1, u[c], cu[3 + c]);
+                    cu[6 + c] = fma(l2, u[c], cu[6 + c]);
+                    ct[c] = fma(l0, v[c], ct[c]);
+                    ct[3 + c] = fma(l1, v[c], ct[3 + c]);
+                    ct[6 + c] = fma(l2, v[c], ct[6 + c]);
                 }
             }
-            for (int c = 0; c < 3; ++c) {
-                cu[c] = fma(l0, u[c], cu[c]);
-                cu[3 + c] = fma(l1, u[c], cu[3 + c]);
-                cu[6 + c] = fma(l2, u[c], cu[6 + c]);
-                ct[c] = fma(l0, v[c], ct[c]);
-                ct[3 + c] = fma(l1, v[c], ct[3 + c]);
-                ct[6 + c] = fma(l2, v[c], ct[6 + c]);
-            }
         }
+        __syncthreads();
     }
     if (!in)
         return;

@@ -179,6 +179,47 @@ def mixed_precision_report(tdbem, h=0.125, nstep=600):
     return out
 
 
+def batched_mixed_report(tdbem, single_fp64_steps_per_s, h=0.125, nstep=600, nrhs=64):
+    """The mixed-precision plus batched form on the 8-domain stacked box
+    (8192 elements, 600 steps): 64 seeded load cases as one batch over FP32 +
+    bf16 bands (mode 2), whole job timed, against the single FP64 whole job of
+    the same box (mixed_precision report); displacement error of the batch's
+    first case against its own FP64 solve, both at GMRES tol 1e-12."""
+    from tdbem import mesh
+    w = mesh.mixed_config3(h=h, nstep=nstep)
+    m = w.mesh
+    cases = mesh.load_cases(w, nrhs)
+
+    def case(mixed):
+        c = tdbem.Case(m.points, m.ele, m.bcid, w.dt, w.gap, domain_count=m.domain_count, mat_rows=w.mats,
+                       itf_rows=m.interfaces)
+        c.set_option("mixed", mixed)
+        c.assemble_step0(10)
+        c.assemble_history(w.nstep)
+        return c
+
+    t0 = time.perf_counter()
+    c = case(2)
+    BU, BT, it, fl, solved, times = c.solve_batched(cases, w.nstep, tol=1e-5)
+    tdbem.synchronize()
+    t1 = time.perf_counter()
+    c.close()
+    v = nrhs * w.nstep / (t1 - t0)
+    # accuracy of the mixed batch: case 0 again at tol 1e-12, mixed batch of one vs FP64
+    c = case(2)
+    U2 = c.solve_batched(cases[:1], w.nstep, tol=1e-12)[0][0]
+    c.close()
+    c = case(0)
+    U0 = c.solve(cases[0], w.nstep, tol=1e-12)[0]
+    c.close()
+    err = float(np.abs(U2 - U0).max() / np.abs(U0).max())
+    log(f"batched mixed {nrhs}: {t1 - t0:.2f} s, error {err:.2e}")
+    return {"workload": w.name, "load_cases": nrhs, "mixed": "FP32 + bf16 bands (mode 2)",
+            "case_steps_per_s": round(v, 2), "vs_single_fp64_whole_job": round(v / single_fp64_steps_per_s, 2),
+            "north_star_target": 10.0, "max_rel_err_u_vs_fp64": err, "all_solved": bool((solved == w.nstep).all()),
+            "breakdown_s": {"convolution": round(float(times[0]), 3), "gmres": round(float(times[1]), 3)}}
+
+
 def batched_report(tdbem, w, single_steps_per_s, nrhs=64):
     """64 seeded load cases on the layered block solved as one batch (shared
     bands, lane-per-case convolution, lockstep GMRES), whole job timed."""
@@ -401,6 +442,8 @@ def main():
         out["batched"] = batched_report(tdbem, w, value)
     if not args.no_mixed:
         out["mixed_precision"] = mixed_precision_report(tdbem)
+        if not args.no_batched:
+            out["batched_mixed"] = batched_mixed_report(tdbem, out["mixed_precision"]["fp64"]["steps_per_s"])
     if not args.no_adjoint:
         out["adjoint"] = adjoint_report(tdbem)
     if args.models:

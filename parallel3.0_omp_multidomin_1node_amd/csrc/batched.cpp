@@ -164,7 +164,8 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
     d_known.alloc(3 * (size_t)std::max<long>(1, dof.knownBlockCount) * B);
     d_rhs.alloc((size_t)n * B);
     d_x.alloc((size_t)n * B);
-    d_hfar.alloc((size_t)kBatchTile * (size_t)n * B);
+    const int farParts = batchedMfma ? batchedFarParts : 0;
+    d_hfar.alloc((size_t)std::max(1, farParts) * kBatchTile * (size_t)n * B);
     for (DevBuf<double>* z : {&d_lu, &d_lt, &d_gU, &d_gT, &d_FU, &d_FT, &d_known})
         LaunchZero(z->p, (long)z->n, stream);
     const unsigned short* mLo = historySplit ? d_mtsLo.p : nullptr;
@@ -190,20 +191,20 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
                 if (historyMixed)
                     LaunchConvFarB(grid, d_refs.p, d_off.p, d_kb.p, d_len.p, (const float*)d_mts32.p,
                                    (const float*)d_mgs32.p, mLo, gLo, d_FU.p, d_FT.p, nstore, K0, Tn, maxn, (int)ne, B,
-                                   d_hfar.p, stream);
+                                   d_hfar.p, farParts, stream);
                 else
                     LaunchConvFarB(grid, d_refs.p, d_off.p, d_kb.p, d_len.p, (const double*)d_mts.p,
                                    (const double*)d_mgs.p, nullptr, nullptr, d_FU.p, d_FT.p, nstore, K0, Tn, maxn,
-                                   (int)ne, B, d_hfar.p, stream);
+                                   (int)ne, B, d_hfar.p, farParts, stream);
             }
             if (historyMixed)
                 LaunchConvNearB(d_refs.p, d_nearPtr.p, d_nearPair.p, d_nearField.p, d_off.p, d_kb.p, d_len.p,
                                 (const float*)d_mts32.p, (const float*)d_mgs32.p, mLo, gLo, d_gU.p, d_gT.p, nstore, K,
-                                K0, maxn, (int)ne, B, d_hfar.p, d_rhs.p, stream);
+                                K0, maxn, (int)ne, B, d_hfar.p, farParts, d_rhs.p, stream);
             else
                 LaunchConvNearB(d_refs.p, d_nearPtr.p, d_nearPair.p, d_nearField.p, d_off.p, d_kb.p, d_len.p,
                                 (const double*)d_mts.p, (const double*)d_mgs.p, nullptr, nullptr, d_gU.p, d_gT.p,
-                                nstore, K, K0, maxn, (int)ne, B, d_hfar.p, d_rhs.p, stream);
+                                nstore, K, K0, maxn, (int)ne, B, d_hfar.p, farParts, d_rhs.p, stream);
             CheckB(hipStreamSynchronize(stream), "batched convolution");
             convTime += NowB() - c0;
         }

@@ -195,7 +195,8 @@ void LaunchSpmmB(long nrows, const int* rp, const int* col, const double* val, c
 // back as wave-uniform broadcasts.  Each lane keeps the filtered states u~, t~
 // of x = K0 - j .. K0 - j + T - 1 in a register window whose slots rotate with
 // the lag (the slot of x is fixed per lag within a group of T lags), so a new
-// lag needs one state (6 values, loaded one lag ahead) and applies its block
+// lag needs one state (6 values, loaded two lags ahead: the loads are latency-
+// bound, and a second lag in flight took 14 % off the launch) and applies its block
 // to all T steps (18 FMA per step).
 // ---------------------------------------------------------------------------
 template <int T, typename TB>
@@ -256,10 +257,11 @@ __global__ void __launch_bounds__(256) k_conv_far_b(PairGrid grid, const NodeRef
                     w[k] = 0.0;
             }
         };
-        double win[T][6], tmp[6];
+        double win[T][6], tmp[6], tmp2[6];
 #pragma unroll
         for (int qq = 0; qq < T; ++qq)
             loadx(K0 - k0 + qq, win[qq]);
+        loadx(K0 - k0 - 1, tmp);  // the state lag k0 + 1 adds
         for (int c0 = k0; c0 <= jend; c0 += CH) {
             int c1 = c0 + CH - 1 < jend ? c0 + CH - 1 : jend;
             int nlag = c1 - c0 + 1;
@@ -288,10 +290,12 @@ __global__ void __launch_bounds__(256) k_conv_far_b(PairGrid grid, const NodeRef
                     if (j <= c1) {
                         if (u > 0 || jg > k0) {
 #pragma unroll
-                            for (int k = 0; k < 6; ++k)
+                            for (int k = 0; k < 6; ++k) {
                                 win[(T - u) % T][k] = tmp[k];
+                                tmp[k] = tmp2[k];
+                            }
                         }
-                        loadx(K0 - j - 1, tmp);  // the state lag j + 1 adds
+                        loadx(K0 - j - 2, tmp2);  // the state lag j + 2 adds (two lags ahead)
                         const double* bl = sb + (j - c0) * 15;
                         double M[9], G[6];
 #pragma unroll
@@ -336,12 +340,241 @@ __global__ void __launch_bounds__(256) k_conv_far_b(PairGrid grid, const NodeRef
     }
 }
 
+// ---------------------------------------------------------------------------
+// The same far part on the FP64 matrix cores.  For one field element f and
+// kMfRows consecutive source rows s of its domain, the tile's far sums are
+//   h[s, i, c, b] += sum_x sum_c' A_i[(s, c), (x, c')] F[f, x, c', b],
+//   A_i[(s, c), (x, c')] = block of pair (s, f) at lag j = K0 + i - x,
+// with the state x fixed per MFMA k-step (four consecutive states) and the
+// step i selecting which lags of the staged window form the A operand.  So
+// each state value loaded for a k-step feeds T steps x 3 components x
+// kMfRows rows of 16 load cases, and each lag block is read from LDS by
+// address arithmetic alone (no Toeplitz expansion in memory).  M = 3 x
+// kMfRows = 15 of the 16 rows of v_mfma_f64_16x16x4f64 (row 15 reads a zero
+// LDS row), N = 16 load cases per wavefront, K = 4 states of one component.
+// Lane layout of v_mfma_f64_16x16x4f64 (64 lanes): A (16 x 4) lane l holds
+// A[l % 16][l / 16]; B (4 x 16) lane l holds B[l / 16][l % 16]; D (16 x 16)
+// lane l, register r holds D[l / 16 + 4 r][l % 16].
+// One workgroup = one row group x 64 load cases (a wavefront per 16 cases);
+// the lag window of each k-step is staged once for the four wavefronts, in
+// two LDS buffers (the next window is fetched into registers while the
+// current k-step runs).  The sums differ from k_conv_far_b by rounding only.
+// ---------------------------------------------------------------------------
+constexpr int kMfRows = 5;
+
+typedef double MfD4 __attribute__((ext_vector_type(4)));
+
+template <int T, typename TB>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_conv_far_bm(
+    PairGrid grid, const NodeRefs* __restrict__ refs, const long* __restrict__ off, const int* __restrict__ kb,
+    const int* __restrict__ len, const TB* __restrict__ mts, const TB* __restrict__ mgs,
+    const unsigned short* __restrict__ mtsLo, const unsigned short* __restrict__ mgsLo,
+    const double* __restrict__ FU, const double* __restrict__ FT, int nstore, int K0, int Tn, int maxn, int ne,
+    int B, int parts, long partStride, double* __restrict__ hfar)
+{
+    static_assert(T == 8, "two groups of four steps per k-step");
+    constexpr int W = T + 3;        // lags of one k-step's window
+    constexpr int VL = 18;          // staged values per lag: -M (9), G as a full 3 x 3 (9)
+    constexpr int RS = W * VL + 3;  // LDS doubles per row of a window (A reads at most 2-way bank conflicts)
+    constexpr int TPR = 256 / kMfRows;   // staging threads per row
+    constexpr int PER = (W * VL + TPR - 1) / TPR;  // staged values per thread
+    __shared__ double sw[2][(kMfRows + 1) * RS];
+    // workgroup -> (row group, part of the columns); row group -> domain, first local row
+    const int part = (int)blockIdx.x % parts;
+    int g = (int)blockIdx.x / parts, d = 0;
+    for (;;) {
+        int ng = (grid.eleCount[d] + kMfRows - 1) / kMfRows;
+        if (g < ng || d + 1 >= grid.nd)
+            break;
+        g -= ng;
+        ++d;
+    }
+    const int nd = grid.eleCount[d], eb = grid.eleBegin[d];
+    const int r0 = g * kMfRows;
+    if (r0 >= nd)
+        return;
+    const int nrows = nd - r0 < kMfRows ? nd - r0 : kMfRows;
+    const int tid = threadIdx.x, lane = tid & 63, wave = WaveIndex();
+    const int kq = lane >> 4;
+    const int b = blockIdx.y * 64 + wave * 16 + (lane & 15);
+    const bool live = b < B;
+    const bool waveLive = blockIdx.y * 64 + wave * 16 < B;
+    const int bb = live ? b : 0;
+    // this lane's A row m = 3 sl + c (row 15 -> the zero row kMfRows)
+    const int m = lane & 15, sl = m / 3, c = m - 3 * (m / 3);
+    const int abase = sl * RS + kq * VL + 3 * c;  // + component offset + VL * step
+    const int Kmax = K0 + Tn - 1;
+    const int jmax = Kmax < maxn ? Kmax : maxn;
+    MfD4 acc[T];
+#pragma unroll
+    for (int i = 0; i < T; ++i)
+        acc[i] = MfD4{0.0, 0.0, 0.0, 0.0};
+    for (int idx = tid; idx < RS; idx += 256) {
+        sw[0][kMfRows * RS + idx] = 0.0;
+        sw[1][kMfRows * RS + idx] = 0.0;
+    }
+    const long pair0 = grid.pairBase[d] + (long)r0 * nd;  // pair (row r0, column 0)
+    const int sr = tid / TPR, st = tid - TPR * (tid / TPR);  // staging row (kMfRows: idle) and slot
+    constexpr int kG9 = 0 | 1 << 3 | 2 << 6 | 1 << 9 | 3 << 12 | 4 << 15 | 2 << 18 | 4 << 21 | 5 << 24;
+    const int q0 = (int)((long)nd * part / parts), q1 = (int)((long)nd * (part + 1) / parts);
+    for (int q = q0; q < q1; ++q) {
+        // band union of the group's rows (the same in every wavefront)
+        int jlo = 0x7fffffff, jhi = -1;
+        for (int l = 0; l < nrows; ++l) {
+            long p = pair0 + (long)l * nd + q;
+            int L = len[p], k0 = kb[p];
+            int je = k0 + L - 1 < jmax ? k0 + L - 1 : jmax;
+            if (L > 0 && k0 <= je) {
+                jlo = k0 < jlo ? k0 : jlo;
+                jhi = je > jhi ? je : jhi;
+            }
+        }
+        if (jhi < 0)
+            continue;
+        // this thread stages values t, t + TPR, ... (window lag u = t / VL,
+        // value k = t % VL) of row sr: first lag rk0, last lag rje, entry roff
+        int rk0 = 1, rje = 0;
+        long roff = 0;
+        if (sr < nrows) {
+            long p = pair0 + (long)sr * nd + q;
+            int L = len[p];
+            rk0 = kb[p];
+            rje = L > 0 ? (rk0 + L - 1 < jmax ? rk0 + L - 1 : jmax) : rk0 - 1;
+            roff = off[p];
+        }
+        TB raw[PER];
+        unsigned short rawLo[PER];
+        auto fetch = [&](int jw) {
+#pragma unroll
+            for (int v = 0; v < PER; ++v) {
+                int t = st + TPR * v;
+                int u = t / VL, k = t - VL * (t / VL);
+                int j = jw + u;
+                bool ok = j >= rk0 && j <= rje;
+                long e = ok ? roff + (j - rk0) : 0;
+                // G (00 01 02 11 12 22) expanded row by row: 0 1 2 / 1 3 4 / 2 4 5
+                long at = k < 9 ? 9 * e + k : 6 * e + ((kG9 >> (3 * (k - 9))) & 7);
+                raw[v] = (k < 9 ? mts : mgs)[at];
+                if (mtsLo)
+                    rawLo[v] = (k < 9 ? mtsLo : mgsLo)[at];
+            }
+        };
+        auto store = [&](int buf, int jw) {
+#pragma unroll
+            for (int v = 0; v < PER; ++v) {
+                int t = st + TPR * v;
+                int u = t / VL, k = t - VL * (t / VL);
+                int j = jw + u;
+                bool ok = j >= rk0 && j <= rje;
+                double x = (double)raw[v];
+                if (mtsLo)
+                    x += Bf16Lo(rawLo[v]);
+                x = ok ? (k < 9 ? -x : x) : 0.0;
+                if (sr < kMfRows && t < W * VL)
+                    sw[buf][sr * RS + t] = x;
+            }
+        };
+        const int f = eb + q;
+        const double* fu = FU + (long)refs[f].hu_block * nstore * 3 * B + bb;
+        const double* ft = FT + (long)refs[f].ht_block * nstore * 3 * B + bb;
+        // k-steps: states xb - 3 .. xb, xb = xTop, xTop - 4, ... while xb >= K0 - jhi
+        const int xTop = K0 + Tn - 1 - jlo;
+        const int nk = (xTop - (K0 - jhi)) / 4 + 1;
+        // B operand of the k-step with top state xb: F[x = xb - kq][c'][b]
+        auto loadB = [&](int xb, double* bv) {
+            const int x = xb - kq;
+            const bool xin = live && x >= 0 && x < nstore;
+            const long xo = xin ? (long)x * 3 * B : 0;
+#pragma unroll
+            for (int cp = 0; cp < 6; ++cp) {
+                double v = (cp < 3 ? fu : ft)[xo + (long)(cp % 3) * B];
+                bv[cp] = xin ? v : 0.0;
+            }
+        };
+        fetch(K0 - xTop);
+        double bv[6];
+        loadB(xTop, bv);
+        store(0, K0 - xTop);
+        __syncthreads();
+        for (int ks = 0; ks < nk; ++ks) {
+            const int xb = xTop - 4 * ks;
+            const int jw = K0 - xb;  // window lags jw .. jw + W - 1
+            const bool more = ks + 1 < nk;
+            if (more)
+                fetch(jw + 4);
+            double bn[6];
+            if (more)
+                loadB(xb - 4, bn);
+            if (waveLive) {
+                // steps i with a lag of the union in this k-step: [ia, ib]
+                const bool lo4 = jlo - 3 - jw <= 3;  // some i in 0..3
+                const bool hi4 = jhi - jw >= 4;      // some i in 4..7
+                const double* A = sw[ks & 1] + abase;
+#pragma unroll
+                for (int cp = 0; cp < 6; ++cp) {
+                    const int co = cp < 3 ? cp : cp + 6;
+                    if (lo4) {
+                        double a[4];
+#pragma unroll
+                        for (int i = 0; i < 4; ++i)
+                            a[i] = A[co + VL * i];
+#pragma unroll
+                        for (int i = 0; i < 4; ++i)
+                            acc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], bv[cp], acc[i], 0, 0, 0);
+                    }
+                    if (hi4) {
+                        double a[4];
+#pragma unroll
+                        for (int i = 0; i < 4; ++i)
+                            a[i] = A[co + VL * (i + 4)];
+#pragma unroll
+                        for (int i = 0; i < 4; ++i)
+                            acc[i + 4] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], bv[cp], acc[i + 4], 0, 0, 0);
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            }
+            if (more) {
+                store((ks + 1) & 1, jw + 4);
+#pragma unroll
+                for (int cp = 0; cp < 6; ++cp)
+                    bv[cp] = bn[cp];
+            }
+            __syncthreads();
+        }
+    }
+    if (!live)
+        return;
+#pragma unroll
+    for (int i = 0; i < T; ++i) {
+        if (i >= Tn)
+            break;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            int row = kq + 4 * r;
+            int rs = row / 3, rc = row - 3 * (row / 3);
+            if (row < 3 * kMfRows && rs < nrows)
+                hfar[part * partStride + ((long)i * 3 * ne + 3 * (long)(eb + r0 + rs) + rc) * B + b] = acc[i][r];
+        }
+    }
+}
+
 template <typename TB>
 void LaunchConvFarB(PairGrid grid, const NodeRefs* refs, const long* off, const int* kb, const int* len,
                     const TB* mts, const TB* mgs, const unsigned short* mtsLo, const unsigned short* mgsLo,
                     const double* FU, const double* FT, int nstore, int K0, int Tn, int maxn, int ne, int B,
-                    double* hfar, hipStream_t s)
+                    double* hfar, int parts, hipStream_t s)
 {
+    if (parts > 0) {
+        long groups = 0;
+        for (int d = 0; d < grid.nd; ++d)
+            groups += (grid.eleCount[d] + kMfRows - 1) / kMfRows;
+        dim3 g((unsigned)(groups * parts), (unsigned)((B + 63) / 64));
+        k_conv_far_bm<kBatchTile, TB><<<g, 256, 0, s>>>(grid, refs, off, kb, len, mts, mgs, mtsLo, mgsLo, FU, FT,
+                                                        nstore, K0, Tn, maxn, ne, B, parts,
+                                                        (long)kBatchTile * 3 * ne * B, hfar);
+        return;
+    }
     dim3 g((unsigned)ne, (unsigned)((B + 63) / 64));
     k_conv_far_b<kBatchTile, TB><<<g, 256, 0, s>>>(grid, refs, off, kb, len, mts, mgs, mtsLo, mgsLo, FU, FT, nstore,
                                                    K0, Tn, maxn, ne, B, hfar);
@@ -361,7 +594,8 @@ __global__ void __launch_bounds__(64) k_conv_near_b(const NodeRefs* __restrict__
                                                     const unsigned short* __restrict__ mgsLo,
                                                     const double* __restrict__ gU, const double* __restrict__ gT,
                                                     int nstore, int K, int K0, int maxn, int ne, int B,
-                                                    const double* __restrict__ hfar, double* __restrict__ rhs)
+                                                    const double* __restrict__ hfar, int parts, long partStride,
+                                                    double* __restrict__ rhs)
 {
     int s = blockIdx.x;
     int b = blockIdx.y * 64 + threadIdx.x;
@@ -407,30 +641,35 @@ __global__ void __launch_bounds__(64) k_conv_near_b(const NodeRefs* __restrict__
     if (!live)
         return;
     const double* hf = hfar + ((long)(K - K0) * 3 * ne + 3 * (long)s) * B + b;
-    for (int c = 0; c < 3; ++c)
-        rhs[(3 * (long)s + c) * B + b] += hf[(long)c * B] + y[c];
+    for (int c = 0; c < 3; ++c) {
+        double v = hf[(long)c * B];
+        for (int p = 1; p < parts; ++p)
+            v += hf[p * partStride + (long)c * B];
+        rhs[(3 * (long)s + c) * B + b] += v + y[c];
+    }
 }
 
 template <typename TB>
 void LaunchConvNearB(const NodeRefs* refs, const long* nearPtr, const long* nearPair, const int* nearField,
                      const long* off, const int* kb, const int* len, const TB* mts, const TB* mgs,
                      const unsigned short* mtsLo, const unsigned short* mgsLo, const double* gU, const double* gT,
-                     int nstore, int K, int K0, int maxn, int ne, int B, const double* hfar, double* rhs,
-                     hipStream_t s)
+                     int nstore, int K, int K0, int maxn, int ne, int B, const double* hfar, int parts,
+                     double* rhs, hipStream_t s)
 {
     dim3 g((unsigned)ne, (unsigned)((B + 63) / 64));
     k_conv_near_b<TB><<<g, 64, 0, s>>>(refs, nearPtr, nearPair, nearField, off, kb, len, mts, mgs, mtsLo, mgsLo, gU,
-                                       gT, nstore, K, K0, maxn, ne, B, hfar, rhs);
+                                       gT, nstore, K, K0, maxn, ne, B, hfar, parts < 1 ? 1 : parts,
+                                       (long)kBatchTile * 3 * ne * B, rhs);
 }
 
 #define TDBEM_BATCH_INSTANTIATE(TB)                                                                                  \
     template void LaunchConvFarB<TB>(PairGrid, const NodeRefs*, const long*, const int*, const int*, const TB*,       \
                                      const TB*, const unsigned short*, const unsigned short*, const double*,          \
-                                     const double*, int, int, int, int, int, int, double*, hipStream_t);              \
+                                     const double*, int, int, int, int, int, int, double*, int, hipStream_t);         \
     template void LaunchConvNearB<TB>(const NodeRefs*, const long*, const long*, const int*, const long*, const int*, \
                                       const int*, const TB*, const TB*, const unsigned short*,                        \
                                       const unsigned short*, const double*, const double*, int, int, int, int, int,   \
-                                      int, const double*, double*, hipStream_t);
+                                      int, const double*, int, double*, hipStream_t);
 TDBEM_BATCH_INSTANTIATE(double)
 TDBEM_BATCH_INSTANTIATE(float)
 #undef TDBEM_BATCH_INSTANTIATE

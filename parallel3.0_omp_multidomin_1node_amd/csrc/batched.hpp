@@ -10,6 +10,7 @@
 namespace tdbem {
 
 constexpr int kBatchTile = 8;  // steps per far-part tile of the batched convolution
+constexpr int kBatchFarPartsMax = 16;  // column parts of the matrix-core far part
 
 // Per-step plumbing.  bd / lu / lt are [b][K][3 ne] (host layout).
 void LaunchKnownB(const NodeRefs* refs, const double* bd, int ne, int nstore, int K, int B, double* known,
@@ -27,19 +28,23 @@ void LaunchFilterB(const double* g, long cols, int nstore, int B, int x0, int x1
 void LaunchSpmmB(long nrows, const int* rp, const int* col, const double* val, const double* x, double* y, int B,
                  hipStream_t s);
 
-// Far part of the tile [K0, K0+Tn) for B load cases into hfar[(i*3ne + r)*B + b].
+// Far part of the tile [K0, K0+Tn) for B load cases into hfar[(i*3ne + r)*B + b]:
+// on the FP64 matrix cores with the columns of each row split into parts > 0
+// partial sums (part p at hfar + p * kBatchTile * 3ne * B), or with vector
+// FMAs in one sum (parts = 0).
 template <typename TB>
 void LaunchConvFarB(PairGrid grid, const NodeRefs* refs, const long* off, const int* kb, const int* len,
                     const TB* mts, const TB* mgs, const unsigned short* mtsLo, const unsigned short* mgsLo,
                     const double* FU, const double* FT, int nstore, int K0, int Tn, int maxn, int ne, int B,
-                    double* hfar, hipStream_t s);
-// Near part of step K plus the tile's far part into rhs.
+                    double* hfar, int parts, hipStream_t s);
+// Near part of step K plus the tile's far part (the sum of its parts, in
+// order) into rhs.
 template <typename TB>
 void LaunchConvNearB(const NodeRefs* refs, const long* nearPtr, const long* nearPair, const int* nearField,
                      const long* off, const int* kb, const int* len, const TB* mts, const TB* mgs,
                      const unsigned short* mtsLo, const unsigned short* mgsLo, const double* gU, const double* gT,
-                     int nstore, int K, int K0, int maxn, int ne, int B, const double* hfar, double* rhs,
-                     hipStream_t s);
+                     int nstore, int K, int K0, int maxn, int ne, int B, const double* hfar, int parts,
+                     double* rhs, hipStream_t s);
 
 // Batched GMRES linear algebra (every reduction per load case, FP64).
 void LaunchPrecondB(long n, const long* posLeaf, const long* leafBegin, const long* leafSize, const long* matOff,

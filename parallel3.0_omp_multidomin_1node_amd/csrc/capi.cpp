@@ -1,6 +1,7 @@
 // extern "C" boundary of the MI355X TDBEM path (declared in include/tdbem_capi.h).
 #include "../../include/tdbem_capi.h"
 
+#include "batched.hpp"
 #include "io.hpp"
 #include "solver.hpp"
 
@@ -417,6 +418,16 @@ int tdbem_set_option(tdbem_ctx* ctx, const char* name, double value)
     }
     if (n == "near_per_entry") {
         ctx->c.nearPerEntry = value != 0.0;
+        return 0;
+    }
+    if (n == "batched_mfma") {
+        ctx->c.batchedMfma = value != 0.0;
+        return 0;
+    }
+    if (n == "batched_far_parts") {
+        if (value < 1.0 || value > (double)tdbem::kBatchFarPartsMax || value != (double)(int)value)
+            return Fail("batched_far_parts must be a whole number from 1 to 16");
+        ctx->c.batchedFarParts = (int)value;
         return 0;
     }
     if (n == "far_parts") {

@@ -154,6 +154,8 @@ public:
     bool convPacked = true;    // filtered far tiles pack 64 band entries per work item across pairs
     int farParts = 4;          // wavefronts (and partial sums) per source row of the packed far part
     bool nearPerEntry = true;  // near part: one (pair, lag) entry per lane (false: one pair per lane)
+    bool batchedMfma = false;  // batched far part on the FP64 matrix cores (false: vector FMAs)
+    int batchedFarParts = 16;  // column parts (partial sums) of the matrix-core batched far part
     int gmBlocks = 32;         // reduction blocks of the GMRES dot products (at most; <= kGmBlocksMax)
     int gmresDevice = 1;       // FP64 GMRES: 0 host-driven, 1 device-driven hipGraphs, 2 device-driven launches
     int selfMode = -1;       // kernel of the current self tables: 1 wave per element, 0 reference order

@@ -505,6 +505,40 @@ def test_batched_load_cases_match_single_solves(nrhs, mixed):
         assert np.abs(bit[b] - it).max() <= 1, b
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("mixed", [0, 2])
+def test_batched_far_matrix_cores_match_vector_fmas(mixed):
+    """The batched far part on the FP64 matrix cores (row groups of five
+    sources, the state fixed per k-step; columns split into 4, 1 or 3 partial
+    sums) and with vector FMAs (one source row per workgroup, a register
+    window of states) give the same batch to rounding: 1e-10 relative on
+    every case's displacements and tractions, on a mesh whose domains end
+    with partial row groups and 70 load cases (a partial second group of
+    64)."""
+    tdbem = _tdbem()
+    from tdbem import mesh
+    w = mesh.layered_config2(h=0.25, nstep=40)
+    m = w.mesh
+    cases = mesh.load_cases(w, 70)
+    print("case:", w.name, "elements", len(m.ele), "mixed", mixed, flush=True)
+    gc = tdbem.Case(m.points, m.ele, m.bcid, w.dt, w.gap, domain_count=4, mat_rows=w.mats, itf_rows=m.interfaces)
+    gc.set_option("mixed", mixed)
+    gc.assemble_step0(10)
+    gc.assemble_history(w.nstep)
+    out = []
+    for mf, parts in ((0, 1), (1, 4), (1, 1), (1, 3)):
+        gc.set_option("batched_mfma", mf)
+        gc.set_option("batched_far_parts", parts)
+        BU, BT, bit, bfl, bsolved, _ = gc.solve_batched(cases, w.nstep, tol=1e-12)
+        assert (bsolved == w.nstep).all() and not bfl.any()
+        out.append((BU, BT))
+    gc.close()
+    U0, T0 = out[0]
+    for U1, T1 in out[1:]:
+        assert np.abs(U1 - U0).max() <= 1e-10 * np.abs(U0).max()
+        assert np.abs(T1 - T0).max() <= 1e-10 * np.abs(T0).max()
+
+
 def _adjoint_setup():
     tdbem = _tdbem()
     from tdbem import mesh

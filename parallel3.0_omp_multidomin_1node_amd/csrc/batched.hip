@@ -4,6 +4,8 @@
 // (scalar loads, SGPR operands) and each lane streams only its own vectors.
 #include "batched.hpp"
 
+#include <type_traits>
+
 #pragma clang fp contract(off)
 
 namespace tdbem {
@@ -200,7 +202,7 @@ void LaunchSpmmB(long nrows, const int* rp, const int* col, const double* val, c
 // to all T steps (18 FMA per step).
 // ---------------------------------------------------------------------------
 template <int T, typename TB>
-__global__ void __launch_bounds__(256) k_conv_far_b(PairGrid grid, const NodeRefs* __restrict__ refs,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_conv_far_b(PairGrid grid, const NodeRefs* __restrict__ refs,
                                                     const long* __restrict__ off, const int* __restrict__ kb,
                                                     const int* __restrict__ len, const TB* __restrict__ mts,
                                                     const TB* __restrict__ mgs,
@@ -210,8 +212,9 @@ __global__ void __launch_bounds__(256) k_conv_far_b(PairGrid grid, const NodeRef
                                                     int nstore, int K0, int Tn, int maxn, int ne, int B,
                                                     double* __restrict__ hfar)
 {
-    constexpr int CH = 64;  // lags per staged chunk (a multiple of T)
-    static_assert(CH % T == 0, "chunk must hold whole lag groups");
+    constexpr int D = 3;          // lags between a state's load and its first use
+    constexpr int R = T + D - 1;  // state ring slots
+    constexpr int CH = 6 * R;     // lags per staged chunk (a multiple of R)
     __shared__ double sblk[4][CH * 15];
     __shared__ double red[4][T * 3][64];
     int wave = WaveIndex();
@@ -257,45 +260,64 @@ __global__ void __launch_bounds__(256) k_conv_far_b(PairGrid grid, const NodeRef
                     w[k] = 0.0;
             }
         };
-        double win[T][6], tmp[6], tmp2[6];
+        // ring of R = T + D - 1 states: slot (x - (K0 - k0)) mod R holds the
+        // state of x; at lag j (rotation u = (j - k0) mod R) the steps read
+        // slots (i - u) mod R and the state lag j + D adds is loaded into the
+        // slot step T - 1 has just used, D lags before its first use, with
+        // no register copies
+        double ring[R][6];
 #pragma unroll
         for (int qq = 0; qq < T; ++qq)
-            loadx(K0 - k0 + qq, win[qq]);
-        loadx(K0 - k0 - 1, tmp);  // the state lag k0 + 1 adds
+            loadx(K0 - k0 + qq, ring[qq]);
+#pragma unroll
+        for (int dd = 1; dd < D; ++dd)
+            loadx(K0 - k0 - dd, ring[R - dd]);
         for (int c0 = k0; c0 <= jend; c0 += CH) {
             int c1 = c0 + CH - 1 < jend ? c0 + CH - 1 : jend;
             int nlag = c1 - c0 + 1;
             long e0 = base + (c0 - k0);
-            for (int idx = lane; idx < nlag * 9; idx += 64) {
-                int r = idx / 9, k = idx - 9 * r;
-                double v = (double)mts[9 * e0 + idx];
-                if (mtsLo)
-                    v += Bf16Lo(mtsLo[9 * e0 + idx]);
-                sb[r * 15 + k] = v;
-            }
-            for (int idx = lane; idx < nlag * 6; idx += 64) {
-                int r = idx / 6, k = idx - 6 * r;
-                double v = (double)mgs[6 * e0 + idx];
-                if (mgsLo)
-                    v += Bf16Lo(mgsLo[6 * e0 + idx]);
-                sb[r * 15 + 9 + k] = v;
-            }
+            // the chunk's loads issued together per block type (9 M values,
+            // then 6 G values per lane), each lane's values converted and
+            // stored after its loads; indexes past the chunk read its first
+            // entry and are not stored
+            auto stage = [&](const TB* src, const unsigned short* lo, int nv, int off9, auto nper) {
+                constexpr int NP = decltype(nper)::value;
+                TB rv[NP];
+                unsigned short rl[NP];
+#pragma unroll
+                for (int v = 0; v < NP; ++v) {
+                    int idx = lane + 64 * v;
+                    rv[v] = src[(long)nv * e0 + (idx < nlag * nv ? idx : 0)];
+                }
+                if (lo) {
+#pragma unroll
+                    for (int v = 0; v < NP; ++v) {
+                        int idx = lane + 64 * v;
+                        rl[v] = lo[(long)nv * e0 + (idx < nlag * nv ? idx : 0)];
+                    }
+                }
+#pragma unroll
+                for (int v = 0; v < NP; ++v) {
+                    int idx = lane + 64 * v;
+                    if (idx < nlag * nv) {
+                        int r = idx / nv, k = idx - nv * r;
+                        double x = (double)rv[v];
+                        if (lo)
+                            x += Bf16Lo(rl[v]);
+                        sb[r * 15 + off9 + k] = x;
+                    }
+                }
+            };
+            stage(mts, mtsLo, 9, 0, std::integral_constant<int, (CH * 9 + 63) / 64>{});
+            stage(mgs, mgsLo, 6, 9, std::integral_constant<int, (CH * 6 + 63) / 64>{});
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            for (int jg = c0; jg <= c1; jg += T) {
+            for (int jg = c0; jg <= c1; jg += R) {
 #pragma unroll
-                for (int u = 0; u < T; ++u) {
+                for (int u = 0; u < R; ++u) {
                     int j = jg + u;
                     if (j <= c1) {
-                        if (u > 0 || jg > k0) {
-#pragma unroll
-                            for (int k = 0; k < 6; ++k) {
-                                win[(T - u) % T][k] = tmp[k];
-                                tmp[k] = tmp2[k];
-                            }
-                        }
-                        loadx(K0 - j - 2, tmp2);  // the state lag j + 2 adds (two lags ahead)
                         const double* bl = sb + (j - c0) * 15;
                         double M[9], G[6];
 #pragma unroll
@@ -307,7 +329,7 @@ __global__ void __launch_bounds__(256) k_conv_far_b(PairGrid grid, const NodeRef
                         // all T steps (one basic block; steps past Tn are never stored)
 #pragma unroll
                         for (int i = 0; i < T; ++i) {
-                            const double* w = win[(i - u + T) % T];
+                            const double* w = ring[(i - u + R) % R];
                             double a0 = acc[i][0], a1 = acc[i][1], a2 = acc[i][2];
                             a0 = fma(G[0], w[3], a0); a0 = fma(G[1], w[4], a0); a0 = fma(G[2], w[5], a0);
                             a1 = fma(G[1], w[3], a1); a1 = fma(G[3], w[4], a1); a1 = fma(G[4], w[5], a1);
@@ -319,6 +341,7 @@ __global__ void __launch_bounds__(256) k_conv_far_b(PairGrid grid, const NodeRef
                             acc[i][1] = a1;
                             acc[i][2] = a2;
                         }
+                        loadx(K0 - j - D, ring[(T - 1 - u + R) % R]);  // the state lag j + D adds
                     }
                 }
             }

@@ -143,7 +143,7 @@ def copy_calibration(nbytes):
     log(f"calibration copy of {nbytes} bytes done")
 
 
-def mixed_precision_report(tdbem, h=0.125, nstep=600):
+def mixed_precision_report(tdbem, h=0.125, nstep=600, gmres_device=1):
     """Mixed precision on the 8-domain stacked box (8192 elements, 600 steps):
     whole-job steps/s of the FP64 run and of both mixed modes, each solved to
     GMRES tol 1e-12, and the max relative displacement / traction error of the
@@ -157,6 +157,7 @@ def mixed_precision_report(tdbem, h=0.125, nstep=600):
         c = tdbem.Case(m.points, m.ele, m.bcid, w.dt, w.gap, domain_count=m.domain_count, mat_rows=w.mats,
                        itf_rows=m.interfaces)
         c.set_option("mixed", mixed)
+        c.set_option("gmres_device", gmres_device)
         c.assemble_step0(10)
         c.assemble_history(w.nstep)
         U, T, it, fl, solved, _ = c.solve(w.known, w.nstep, tol=1e-12)
@@ -179,7 +180,7 @@ def mixed_precision_report(tdbem, h=0.125, nstep=600):
     return out
 
 
-def batched_mixed_report(tdbem, single_fp64_steps_per_s, h=0.125, nstep=600, nrhs=64):
+def batched_mixed_report(tdbem, single_fp64_steps_per_s, h=0.125, nstep=600, nrhs=64, gmres_device=1):
     """The mixed-precision plus batched form on the 8-domain stacked box
     (8192 elements, 600 steps): 64 seeded load cases as one batch over FP32 +
     bf16 bands (mode 2), whole job timed, against the single FP64 whole job of
@@ -194,6 +195,7 @@ def batched_mixed_report(tdbem, single_fp64_steps_per_s, h=0.125, nstep=600, nrh
         c = tdbem.Case(m.points, m.ele, m.bcid, w.dt, w.gap, domain_count=m.domain_count, mat_rows=w.mats,
                        itf_rows=m.interfaces)
         c.set_option("mixed", mixed)
+        c.set_option("gmres_device", gmres_device)
         c.assemble_step0(10)
         c.assemble_history(w.nstep)
         return c
@@ -441,9 +443,10 @@ def main():
     if not args.no_batched:
         out["batched"] = batched_report(tdbem, w, value)
     if not args.no_mixed:
-        out["mixed_precision"] = mixed_precision_report(tdbem)
+        out["mixed_precision"] = mixed_precision_report(tdbem, gmres_device=args.gmres_device)
         if not args.no_batched:
-            out["batched_mixed"] = batched_mixed_report(tdbem, out["mixed_precision"]["fp64"]["steps_per_s"])
+            out["batched_mixed"] = batched_mixed_report(tdbem, out["mixed_precision"]["fp64"]["steps_per_s"],
+                                                        gmres_device=args.gmres_device)
     if not args.no_adjoint:
         out["adjoint"] = adjoint_report(tdbem)
     if args.models:

@@ -179,7 +179,11 @@ int tdbem_history_stats(tdbem_ctx* ctx, long* counts, long* npairs);
  *   "quad_staged" 1 (default) wave-per-pair history quadrature with the
  *   point terms staged in LDS as basis tensors (same band pattern, blocks
  *   within 1e-10 relative of the reference's arithmetic), 0 one lane per
- *   band entry in the reference's operation order (bit-identical to it). */
+ *   band entry in the reference's operation order (bit-identical to it);
+ *   "batched_mfma" 0 (default) the batched far convolution of
+ *   tdbem_solve_batched on FP64 vector FMAs, 1 on the FP64 matrix cores
+ *   (same sums to rounding), with "batched_far_parts" 1-16 (default 16)
+ *   column parts summed in order by the near part. */
 int tdbem_set_option(tdbem_ctx* ctx, const char* name, double value);
 
 /* Reference input files (host only, no device work).  binary_mode follows the

@@ -3066,7 +3066,9 @@ __global__ void __launch_bounds__(64) k_conv_far_q(PairGrid grid, const NodeRefs
     constexpr int P = dev::kFarPairs;
     constexpr int kRows = 64 + P * (T - 1);
     static_assert(kRows <= 128, "two window slots per lane");
-    __shared__ double fw[kRows * 6];
+    // window rows, then T zero rows read by the lanes without an entry (their
+    // FMAs then add exact zeros, so the FMA block needs no branch)
+    __shared__ double fw[(kRows + T) * 6];
     __shared__ int ex[dev::kFarGroup + 1];  // exclusive prefix of the clipped band lengths, then the total
     __shared__ int gk[dev::kFarGroup];
     __shared__ long go[dev::kFarGroup];
@@ -3079,6 +3081,8 @@ __global__ void __launch_bounds__(64) k_conv_far_q(PairGrid grid, const NodeRefs
     int lane = threadIdx.x;
     int Kmax = K0 + Tn - 1;
     int jmax = Kmax < maxn ? Kmax : maxn;
+    for (int z = lane; z < T * 6; z += 64)
+        fw[kRows * 6 + z] = 0.0;
     double acc[T][3];
 #pragma unroll
     for (int i = 0; i < T; ++i)
@@ -3118,11 +3122,16 @@ __global__ void __launch_bounds__(64) k_conv_far_q(PairGrid grid, const NodeRefs
                 return true;
         }
     };
+    // Issues the same loads whether or not work is left (with no work left the
+    // item is empty: no lane owns an entry, every window slot reads the zero
+    // row), so the number of loads in flight after an advance is the same on
+    // every path and the wait before the current item's FMAs covers only the
+    // loads of the item before it.
     auto advance = [&](dev::FarPack& it) -> bool {
-        if (E >= Gt && !loadGroup())
-            return false;
-        while (ex[i0 + 1] <= E)
-            ++i0;
+        const bool live = !(E >= Gt && !loadGroup());
+        if (live)
+            while (ex[i0 + 1] <= E)
+                ++i0;
         // the item's candidate pairs i0 .. i0+P-1, wavefront-uniform
         int cEx[P + 1], cKb[P];
         long cOff[P];
@@ -3138,7 +3147,7 @@ __global__ void __launch_bounds__(64) k_conv_far_q(PairGrid grid, const NodeRefs
             long o = go[c];
             cOff[k] = ((long)(unsigned)dev::Uniform((int)(o >> 32)) << 32) | (unsigned)dev::Uniform((int)o);
         }
-        int end = E + 64 < cEx[P] ? E + 64 : cEx[P];
+        int end = live ? (E + 64 < cEx[P] ? E + 64 : cEx[P]) : E;
         // windows: pair k holds entries [first, last] of the item and rows wb[k] .. wb[k] + n + T - 2
         int wb[P], tb[P], xl[P];
         int rows = 0;
@@ -3171,8 +3180,8 @@ __global__ void __launch_bounds__(64) k_conv_far_q(PairGrid grid, const NodeRefs
                 offE = cOff[k];
             }
         int t = e - exE;
-        it.row = wbE + tbE - t;
-        long en = it.own ? offE + t : 0;  // lanes without an entry read entry 0 (never used)
+        it.row = it.own ? wbE + tbE - t : kRows;  // lanes without an entry: the zero rows
+        long en = it.own ? offE + t : 0;  // lanes without an entry read entry 0 (times zero states)
 #pragma unroll
         for (int c = 0; c < 9; ++c)
             it.M[c] = (double)mts[9 * en + c];
@@ -3205,14 +3214,19 @@ __global__ void __launch_bounds__(64) k_conv_far_q(PairGrid grid, const NodeRefs
                 }
             int x = xlS + (sl - wbS);
             bool in = sl < rows && x >= 0 && x <= K0;
-            const double* a = F + ((long)(in ? eb + g * dev::kFarGroup + i0 + ks : 0) * fstride + (in ? x : 0)) * 6;
+            // slots outside the windows read row fstride - 1 of element 0,
+            // which the filter never writes (zero): the loads stay
+            // unconditional, so the wait for them falls at their first use
+            // in the next body and not before the current item's FMAs
+            const double* a =
+                F + ((long)(in ? eb + g * dev::kFarGroup + i0 + ks : 0) * fstride + (in ? x : fstride - 1)) * 6;
             double* r = h == 0 ? it.r0 : it.r1;
 #pragma unroll
             for (int c = 0; c < 6; ++c)
-                r[c] = in ? a[c] : 0.0;
+                r[c] = a[c];
         }
         E = end;
-        return true;
+        return live;
     };
     auto body = [&](dev::FarPack& cur, dev::FarPack& nxt) -> bool {
 #pragma unroll
@@ -3224,7 +3238,7 @@ __global__ void __launch_bounds__(64) k_conv_far_q(PairGrid grid, const NodeRefs
                 fw[6 * (lane + 64) + c] = cur.r1[c];
         dev::WaveSync();
         bool more = advance(nxt);
-        if (cur.own) {
+        {
             const double* M = cur.M;
             const double* G = cur.G;
             const double* wr = fw + 6 * cur.row;

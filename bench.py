@@ -284,7 +284,7 @@ def models_report(tdbem):
             "speedup": round((t1 - t0) / (t2 - t1), 3), "all_solved": all(r[4] == w.nstep for r, w in zip(res, ws))}
 
 
-def adjoint_report(tdbem):
+def adjoint_report(tdbem, gmres_device=1):
     """Reverse-time adjoint on the 1x1x4 block (3456 elements, 4 domains, 300
     steps): misfit over 32 receiver elements against observations from
     materials perturbed by +-10 %, gradient with respect to each domain's
@@ -298,6 +298,7 @@ def adjoint_report(tdbem):
 
     def case(mats):
         c = tdbem.Case(m.points, m.ele, m.bcid, w.dt, w.gap, domain_count=4, mat_rows=mats, itf_rows=m.interfaces)
+        c.set_option("gmres_device", gmres_device)
         c.assemble_step0(10)
         c.assemble_history(w.nstep)
         return c
@@ -448,7 +449,7 @@ def main():
             out["batched_mixed"] = batched_mixed_report(tdbem, out["mixed_precision"]["fp64"]["steps_per_s"],
                                                         gmres_device=args.gmres_device)
     if not args.no_adjoint:
-        out["adjoint"] = adjoint_report(tdbem)
+        out["adjoint"] = adjoint_report(tdbem, gmres_device=args.gmres_device)
     if args.models:
         out["models"] = models_report(tdbem)
     if not args.no_cpu_baseline:

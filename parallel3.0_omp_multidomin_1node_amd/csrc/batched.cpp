@@ -34,7 +34,9 @@ double NowB()
 void Case::GmresBatched(int B, const double* rhs, double* x, long restart, double tol, int* its, int* flg)
 {
     const long n = 3 * ne;
-    const int nblk = (int)std::min<long>(64, std::max<long>(1, (n + 1023) / 1024));
+    // reduction blocks of the dot products: four wavefronts of 64 load cases
+    // per block, about 32 rows per wavefront (enough blocks to fill the device)
+    const int nblk = (int)std::min<long>(kBatchRedBlocks, std::max<long>(1, (n + 127) / 128));
     BatchWork& w = bw;
     std::vector<double> bn((size_t)B), nr((size_t)B);
     std::vector<int> act((size_t)B), stag((size_t)B);
@@ -152,7 +154,7 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
     w.scal.alloc(2 * (size_t)B);
     w.stag.alloc((size_t)B);
     w.active.alloc((size_t)B);
-    w.part.alloc((size_t)64 * (size_t)(restart + 1) * B);
+    w.part.alloc((size_t)kBatchRedBlocks * (size_t)(restart + 1) * B);
     DevBuf<double> d_bd, d_lu, d_lt, d_gU, d_gT, d_FU, d_FT, d_known, d_rhs, d_x, d_hfar;
     d_bd.upload(known_lu, (size_t)B * nstore * (size_t)n, stream);
     d_lu.alloc((size_t)B * nstore * (size_t)n);

@@ -767,8 +767,10 @@ __global__ void k_finish_b(const double* __restrict__ partial, int nblocks, int 
     if (idx >= (long)nv * B)
         return;
     double a = 0.0;
+    const long st = (long)nv * B;
+#pragma unroll 8
     for (int x = 0; x < nblocks; ++x)
-        a += partial[(long)x * nv * B + idx];
+        a += partial[(long)x * st + idx];
     h[idx] = doSqrt ? sqrt(a) : a;
 }
 

@@ -11,6 +11,7 @@ namespace tdbem {
 
 constexpr int kBatchTile = 8;  // steps per far-part tile of the batched convolution
 constexpr int kBatchFarPartsMax = 16;  // column parts of the matrix-core far part
+constexpr int kBatchRedBlocks = 256;   // most reduction blocks of the batched dot products
 
 // Per-step plumbing.  bd / lu / lt are [b][K][3 ne] (host layout).
 void LaunchKnownB(const NodeRefs* refs, const double* bd, int ne, int nstore, int K, int B, double* known,

@@ -41,9 +41,9 @@ HBM_PEAK_GBS = 8000.0
 # 400 steps, FP64), from FETCH_SIZE and WRITE_SIZE counter passes calibrated
 # with a 1 GiB device copy (fetch x2.000, write x1.000); the summary with its
 # commands is kept in profiles/k_conv_far16_traffic_layered2x2x4_h0.1_fp64.md.
-MEASURED_TRAFFIC = {("layered_2x2x4_h0.1", 400, 16, 2): 61661732702.0,
-                    ("layered_2x2x4_h0.1", 400, 16, 4): 56922168320.0,
-                    ("layered_2x2x4_h0.1", 400, 16, 8): 52886495483.0}
+MEASURED_TRAFFIC = {("layered_2x2x4_h0.1", 400, 16, 2): 61703231633.9,
+                    ("layered_2x2x4_h0.1", 400, 16, 4): 56938840066.6,
+                    ("layered_2x2x4_h0.1", 400, 16, 8): 52872278677.8}
 FP64_PEAK_TFLOPS = 78.6
 # FP64 lane-operations of one history quadrature (k_history_pw + k_history_reg)
 # of the default workload: 64 x (ADD + MUL + TRANS + 2 FMA) F64 VALU wave

@@ -161,8 +161,9 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
     d_lt.alloc((size_t)B * nstore * (size_t)n);
     d_gU.alloc((size_t)histU * 3 * nstore * B);
     d_gT.alloc((size_t)histT * 3 * nstore * B);
-    d_FU.alloc(d_gU.n);
-    d_FT.alloc(d_gT.n);
+    // filtered series: nstore + 1 rows per column, the last never written (zero)
+    d_FU.alloc((size_t)histU * 3 * (nstore + 1) * B);
+    d_FT.alloc((size_t)histT * 3 * (nstore + 1) * B);
     d_known.alloc(3 * (size_t)std::max<long>(1, dof.knownBlockCount) * B);
     d_rhs.alloc((size_t)n * B);
     d_x.alloc((size_t)n * B);

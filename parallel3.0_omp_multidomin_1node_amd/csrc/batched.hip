@@ -140,7 +140,7 @@ __global__ void __launch_bounds__(256) k_filter_b(const double* __restrict__ g, 
     auto raw = [&](int m) {
         return (m >= 1 && m <= K0 - 1 && m < nstore) ? g[(col * nstore + m) * 3 * B + cb] : 0.0;
     };
-    F[(col * nstore + x) * 3 * B + cb] = (raw(x - 1) + 2.0 * raw(x)) + raw(x + 1);
+    F[(col * (nstore + 1) + x) * 3 * B + cb] = (raw(x - 1) + 2.0 * raw(x)) + raw(x + 1);
 }
 
 void LaunchFilterB(const double* g, long cols, int nstore, int B, int x0, int x1, int K0, double* F, hipStream_t s)
@@ -242,23 +242,22 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
         if (L == 0 || k0 > jend)
             continue;
         int f = eb + q;
-        const double* fu = FU + (long)refs[f].hu_block * nstore * 3 * B + bb;
-        const double* ft = FT + (long)refs[f].ht_block * nstore * 3 * B + bb;
+        const double* fu = FU + (long)refs[f].hu_block * (nstore + 1) * 3 * B + bb;
+        const double* ft = FT + (long)refs[f].ht_block * (nstore + 1) * 3 * B + bb;
         long base = off[p];
+        // the state of x; x outside 0 .. nstore-1 reads row nstore of the
+        // column, which the filter never writes (zero): a wave-uniform row
+        // choice, the loads issued on every path and no per-value select
         auto loadx = [&](int x, double* w) {
-            if (x >= 0 && x < nstore) {
-                const double* a = fu + (long)x * 3 * B;
-                const double* c = ft + (long)x * 3 * B;
-                w[0] = a[0];
-                w[1] = a[B];
-                w[2] = a[2 * B];
-                w[3] = c[0];
-                w[4] = c[B];
-                w[5] = c[2 * B];
-            } else {
-                for (int k = 0; k < 6; ++k)
-                    w[k] = 0.0;
-            }
+            const int xr = x >= 0 && x < nstore ? x : nstore;
+            const double* a = fu + (long)xr * 3 * B;
+            const double* c = ft + (long)xr * 3 * B;
+            w[0] = a[0];
+            w[1] = a[B];
+            w[2] = a[2 * B];
+            w[3] = c[0];
+            w[4] = c[B];
+            w[5] = c[2 * B];
         };
         // ring of R = T + D - 1 states: slot (x - (K0 - k0)) mod R holds the
         // state of x; at lag j (rotation u = (j - k0) mod R) the steps read
@@ -498,8 +497,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
             }
         };
         const int f = eb + q;
-        const double* fu = FU + (long)refs[f].hu_block * nstore * 3 * B + bb;
-        const double* ft = FT + (long)refs[f].ht_block * nstore * 3 * B + bb;
+        const double* fu = FU + (long)refs[f].hu_block * (nstore + 1) * 3 * B + bb;
+        const double* ft = FT + (long)refs[f].ht_block * (nstore + 1) * 3 * B + bb;
         // k-steps: states xb - 3 .. xb, xb = xTop, xTop - 4, ... while xb >= K0 - jhi
         const int xTop = K0 + Tn - 1 - jlo;
         const int nk = (xTop - (K0 - jhi)) / 4 + 1;

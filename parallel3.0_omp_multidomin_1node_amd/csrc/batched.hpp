@@ -22,7 +22,8 @@ void LaunchScatterB(const NodeRefs* refs, const ElemGeo* geo, const double* bd, 
 void LaunchStateB(const NodeRefs* refs, const double* lu, const double* lt, int ne, int nstore, int K, int B,
                   double* gU, double* gT, hipStream_t s);
 // 1-2-1 filtered state series for x in [x0, x1] with the solved states m in
-// [1, K0 - 1]: F[x] = (g[x-1] + 2 g[x]) + g[x+1].
+// [1, K0 - 1]: F[x] = (g[x-1] + 2 g[x]) + g[x+1], F[((col * (nstore + 1) + x)
+// * 3 + c) * B + b] (row nstore of each column is left zero).
 void LaunchFilterB(const double* g, long cols, int nstore, int B, int x0, int x1, int K0, double* F, hipStream_t s);
 
 // y = A x for B vectors (block CCSR, row-major 3x3 blocks).

@@ -603,10 +603,10 @@ void LaunchConvFarB(PairGrid grid, const NodeRefs* refs, const long* off, const 
 }
 
 // Near part of step K (states K0 .. K-1 inside the tile) plus the tile's far
-// part: one wavefront per (source row, 64 load cases), the near pairs'
-// lags summed per lane in order.
+// part: one workgroup of four wavefronts per (source row, 64 load cases),
+// each wavefront summing every fourth near pair's lags per lane in order.
 template <typename TB>
-__global__ void __launch_bounds__(64) k_conv_near_b(const NodeRefs* __restrict__ refs,
+__global__ void __launch_bounds__(256) k_conv_near_b(const NodeRefs* __restrict__ refs,
                                                     const long* __restrict__ nearPtr,
                                                     const long* __restrict__ nearPair,
                                                     const int* __restrict__ nearField, const long* __restrict__ off,
@@ -619,15 +619,19 @@ __global__ void __launch_bounds__(64) k_conv_near_b(const NodeRefs* __restrict__
                                                     const double* __restrict__ hfar, int parts, long partStride,
                                                     double* __restrict__ rhs)
 {
+    __shared__ double red[4][3][64];
     int s = blockIdx.x;
-    int b = blockIdx.y * 64 + threadIdx.x;
+    int lane = threadIdx.x & 63, wave = WaveIndex();
+    int b = blockIdx.y * 64 + lane;
     bool live = b < B;
     int bb = live ? b : B - 1;
     int jlim = K - K0 + 1;
     if (jlim > maxn)
         jlim = maxn;
     double y[3] = {0.0, 0.0, 0.0};
-    for (long q = nearPtr[s]; q < nearPtr[s + 1]; ++q) {
+    // four wavefronts per source row, each every fourth near pair; partial
+    // sums added in wavefront order
+    for (long q = nearPtr[s] + wave; q < nearPtr[s + 1]; q += 4) {
         long p = nearPair[q];
         int k0 = kb[p], L = len[p];
         int f = nearField[q];
@@ -660,8 +664,13 @@ __global__ void __launch_bounds__(64) k_conv_near_b(const NodeRefs* __restrict__
             y[2] += (G[2] * ct[0] + G[4] * ct[1] + G[5] * ct[2]) - (M[6] * cu[0] + M[7] * cu[1] + M[8] * cu[2]);
         }
     }
-    if (!live)
+    for (int c = 0; c < 3; ++c)
+        red[wave][c][lane] = y[c];
+    __syncthreads();
+    if (wave != 0 || !live)
         return;
+    for (int c = 0; c < 3; ++c)
+        y[c] = ((red[0][c][lane] + red[1][c][lane]) + red[2][c][lane]) + red[3][c][lane];
     const double* hf = hfar + ((long)(K - K0) * 3 * ne + 3 * (long)s) * B + b;
     for (int c = 0; c < 3; ++c) {
         double v = hf[(long)c * B];
@@ -679,7 +688,7 @@ void LaunchConvNearB(const NodeRefs* refs, const long* nearPtr, const long* near
                      double* rhs, hipStream_t s)
 {
     dim3 g((unsigned)ne, (unsigned)((B + 63) / 64));
-    k_conv_near_b<TB><<<g, 64, 0, s>>>(refs, nearPtr, nearPair, nearField, off, kb, len, mts, mgs, mtsLo, mgsLo, gU,
+    k_conv_near_b<TB><<<g, 256, 0, s>>>(refs, nearPtr, nearPair, nearField, off, kb, len, mts, mgs, mtsLo, mgsLo, gU,
                                        gT, nstore, K, K0, maxn, ne, B, hfar, parts < 1 ? 1 : parts,
                                        (long)kBatchTile * 3 * ne * B, rhs);
 }

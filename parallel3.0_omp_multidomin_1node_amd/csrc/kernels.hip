@@ -1644,8 +1644,12 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
     const unsigned char* __restrict__ cls, TB* __restrict__ mts, TB* __restrict__ mgs,
     unsigned short* __restrict__ mtsLo, unsigned short* __restrict__ mgsLo, unsigned char* __restrict__ alive)
 {
+    // the patch radii are read only by the patch-mask pass, before the first
+    // point record is staged, so they share the record buffer (12.1 KB of LDS
+    // per wavefront: twelve wavefronts, three per SIMD, fit the 160 KB of a CU)
+    static_assert(kPwSlots * 2 * kNPW2 <= kPwSlots * kNPWG2 * dev::kRec, "radii inside the record buffer");
     __shared__ double sgeo[kPwSlots * kNPWG2 * dev::kRec];
-    __shared__ double spr[kPwSlots * 2 * kNPW2];
+    double* const spr = sgeo;
     __shared__ int ssrc[kPwSlots];
     using namespace dev;
     long wi = w0 + (long)blockIdx.x;

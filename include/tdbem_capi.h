@@ -45,6 +45,38 @@ int tdbem_dofmap_export(tdbem_ctx* ctx, long* counts, long* u_block, double* u_s
                         long* t_block, double* t_sign, int* t_known, long* hu_block, double* hu_sign,
                         long* ht_block, double* ht_sign, long* preferred);
 
+/* GlobalDofMap::Build on the host alone (DBEM1/multidomain.cpp:777-1098):
+ * the same map tdbem_create builds, from the element BCIDs and the
+ * multi-domain tail (arguments as for tdbem_create), without a context or a
+ * device.  Outputs as for tdbem_dofmap_export; the per-node arrays may be
+ * null (counts only).  Fails, as the reference's Build returns false, for a
+ * BCID other than 123 / 456 or a mismatch of unknown and equation blocks. */
+int tdbem_dofmap_build(long ne, const int* bcid, int domain_count, const double* mat_rows, int interface_count,
+                       const long* itf_rows, long* counts, long* u_block, double* u_sign, int* u_known,
+                       long* t_block, double* t_sign, int* t_known, long* hu_block, double* hu_sign,
+                       long* ht_block, double* ht_sign, long* preferred);
+
+/* MultiDomainCCSRBuilder (DBEM1/multidomain.h:139-160, DBEM1/multidomain.cpp:
+ * 1101-1303) on the host: add_block stores the transpose of a column-major
+ * 3x3 block times sign, skips null / zero-sign / all-zero / max|.| <= 1e-14
+ * blocks and non-finite ones (counted, the first 20 reported on stderr) and
+ * drops an accumulated entry that falls to max|.| <= 1e-14; merge adds another
+ * builder's entries in (row, col) order (MergeFrom); build writes block CCSR
+ * with rows and columns ascending and row-major blocks (Build), build_sym the
+ * upper triangle (b00 b01 b02 b11 b12 b22) with *symmetric = 0 when a block
+ * misses the mixed tolerance max(tolerance, 1e-14 + 1e-10 * max|block|)
+ * (BuildSym).  row_ptr needs block_rows + 1 ints, col / val the nnz that
+ * tdbem_builder_info reports. */
+typedef struct tdbem_builder tdbem_builder;
+int tdbem_builder_create(tdbem_builder** out, long block_rows, long block_cols);
+void tdbem_builder_destroy(tdbem_builder* b);
+int tdbem_builder_add_block(tdbem_builder* b, long row, long col, double sign, const double* block9);
+int tdbem_builder_merge(tdbem_builder* b, const tdbem_builder* other);
+int tdbem_builder_info(const tdbem_builder* b, long* nnz, long* skipped_nonfinite);
+int tdbem_builder_build(const tdbem_builder* b, int* row_ptr, int* col, double* val);
+int tdbem_builder_build_sym(const tdbem_builder* b, double tolerance, int* row_ptr, int* col, double* val6,
+                            int* symmetric);
+
 /* Step-0 assembly of A and KnownM plus the leaf preconditioner
  * (AssembleMultiDomainStep0, DBEM1/multidomain.cpp:2256-2327,2658-2693;
  * BuildMappedLeafPreConditioner 3493-3565). */

@@ -43,6 +43,7 @@ def lib():
         L.orc_dynamat.argtypes = [C.c_void_p, C.c_int, _dp]
         L.orc_dofmap.argtypes = [C.c_void_p, _lp, _lp, _dp, _ip, _lp, _dp, _ip, _lp, _dp, _lp, _dp, _lp]
         L.orc_kernel.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp]
+        L.orc_regular_table.argtypes = [C.c_void_p, _dp, _dp, _dp]
         L.orc_needcal.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double]
         L.orc_quadrature.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, _dp]
         L.orc_assemble.argtypes = [C.c_void_p, C.c_int]
@@ -115,6 +116,11 @@ class Case:
         f = lib().orc_kernel(self.h, d, which, float(n), np.ascontiguousarray(ri, np.float64),
                              np.ascontiguousarray(nor, np.float64), out)
         return f, out
+
+    def regular_table(self):
+        pos, ra, nrgv0 = np.zeros(32), np.zeros(16), np.zeros(16)
+        lib().orc_regular_table(self.h, pos, ra, nrgv0)
+        return pos.reshape(16, 2), ra, nrgv0
 
     def needcal(self, field, source, typ, n):
         return lib().orc_needcal(self.h, field, source, typ, float(n))

@@ -1674,6 +1674,23 @@ int orc_kernel(orc_case* c, int d, int which, double n, const double* ri, const 
     return dyna_t2(&c->mats[d], out9, n, R, RI, nor);
 }
 
+/* Regular-path table probe: the 16 points (s1, s2) of the 4x4 rule, their
+ * plain weights m_RGV and the Q1 weights m_NRGV[0] = N0(s1, s2; gap) * w
+ * (DBEM1/quadinfo.cpp:83-97). */
+int orc_regular_table(const orc_case* c, double* pos32, double* ra16, double* nrgv0_16)
+{
+    int i;
+    if (!c)
+        return -1;
+    for (i = 0; i < NG2; ++i) {
+        pos32[2 * i] = c->q.rp[i][0];
+        pos32[2 * i + 1] = c->q.rp[i][1];
+        ra16[i] = c->q.ra[i];
+        nrgv0_16[i] = c->q.nrgv0[i];
+    }
+    return 0;
+}
+
 int orc_needcal(orc_case* c, int field, int source, int type, double n)
 {
     const elem* f;

@@ -50,6 +50,9 @@ int orc_dofmap(orc_case* c, long* counts, long* u_block, double* u_sign, int* u_
 /* Kernel probes (a5): evaluate GetDynaUij / GetDynaT1ij / GetDynaT2ij of
  * domain d at distance vector ri[3] with normal nor[3] and step n. */
 int orc_kernel(orc_case* c, int d, int which, double n, const double* ri, const double* nor, double* out9);
+/* Regular-path table (a3, quirk Q1): pos32 = 16 (s1, s2) points of the 4x4
+ * rule, ra16 their weights, nrgv0_16 = N0(s1, s2; gap) * weight. */
+int orc_regular_table(const orc_case* c, double* pos32, double* ra16, double* nrgv0_16);
 /* NeedCal (a4) of field element f seen from the node of element s. */
 int orc_needcal(orc_case* c, int field, int source, int type, double n);
 /* Judged quadrature (a6-a8) of field f, source node s, step n:

@@ -37,21 +37,11 @@ int Fail(const std::string& msg)
 {
     return Fail(msg.c_str());
 }
-}  // namespace
 
-extern "C" {
-
-const char* tdbem_last_error(void) { return g_err.c_str(); }
-
-int tdbem_create(tdbem_ctx** out, long npoints, const double* points, long ne, const int* ele_pid, const int* bcid,
-                 int domain_count, const double* mat_rows, int interface_count, const long* itf_rows,
-                 double default_E, double default_v, double default_rho, double dt, double gap)
+// The multi-domain datacard tail from the C-ABI's flat rows.
+tdbem::MultiDomainInputConfig MakeConfig(int domain_count, const double* mat_rows, int interface_count,
+                                         const long* itf_rows)
 {
-    if (!out || !points || !ele_pid || !bcid || ne <= 0 || npoints <= 0)
-        return Fail("tdbem_create: invalid arguments");
-    for (long i = 0; i < 8 * ne; ++i)
-        if (ele_pid[i] < 0 || ele_pid[i] >= npoints)
-            return Fail("tdbem_create: element point id out of range");
     tdbem::MultiDomainInputConfig cfg;
     if (domain_count > 0) {
         cfg.enabled = true;
@@ -72,6 +62,63 @@ int tdbem_create(tdbem_ctx** out, long npoints, const double* points, long ne, c
             }
         }
     }
+    return cfg;
+}
+
+void ExportDofMap(const tdbem::GlobalDofMap& d, const std::vector<int>& domOf, long ne, long* counts, long* u_block,
+                  double* u_sign, int* u_known, long* t_block, double* t_sign, int* t_known, long* hu_block,
+                  double* hu_sign, long* ht_block, double* ht_sign, long* preferred)
+{
+    counts[0] = d.unknownBlockCount;
+    counts[1] = d.knownBlockCount;
+    counts[2] = d.equationBlockCount;
+    counts[3] = d.outerDisplacementUnknownBlocks;
+    counts[4] = d.outerTractionUnknownBlocks;
+    counts[5] = d.interfaceDisplacementUnknownBlocks;
+    counts[6] = d.interfaceTractionUnknownBlocks;
+    counts[7] = d.historyUBlockCount;
+    counts[8] = d.historyTBlockCount;
+    for (long i = 0; i < ne; ++i) {
+        int dm = domOf[(size_t)i];
+        tdbem::VariableRef u = d.GetU(dm, i), t = d.GetT(dm, i), hu = d.GetHistoryU(dm, i), ht = d.GetHistoryT(dm, i);
+        if (u_block) {
+            u_block[i] = u.globalBlock;
+            u_sign[i] = u.sign;
+            u_known[i] = u.known;
+            t_block[i] = t.globalBlock;
+            t_sign[i] = t.sign;
+            t_known[i] = t.known;
+            hu_block[i] = hu.globalBlock;
+            hu_sign[i] = hu.sign;
+            ht_block[i] = ht.globalBlock;
+            ht_sign[i] = ht.sign;
+        }
+        if (preferred)
+            preferred[i] = d.GetPreferredUnknownBlockForRow(i);
+    }
+}
+}  // namespace
+
+struct tdbem_builder {
+    tdbem::MultiDomainCCSRBuilder b;
+};
+
+extern "C" {
+
+const char* tdbem_last_error(void) { return g_err.c_str(); }
+
+int tdbem_create(tdbem_ctx** out, long npoints, const double* points, long ne, const int* ele_pid, const int* bcid,
+                 int domain_count, const double* mat_rows, int interface_count, const long* itf_rows,
+                 double default_E, double default_v, double default_rho, double dt, double gap)
+{
+    if (!out || !points || !ele_pid || !bcid || ne <= 0 || npoints <= 0)
+        return Fail("tdbem_create: invalid arguments");
+    for (long i = 0; i < 8 * ne; ++i)
+        if (ele_pid[i] < 0 || ele_pid[i] >= npoints)
+            return Fail("tdbem_create: element point id out of range");
+    if ((domain_count > 0 && !mat_rows) || (domain_count > 0 && interface_count > 0 && !itf_rows))
+        return Fail("tdbem_create: invalid arguments");
+    tdbem::MultiDomainInputConfig cfg = MakeConfig(domain_count, mat_rows, interface_count, itf_rows);
     std::unique_ptr<tdbem_ctx> ctx(new tdbem_ctx());
     try {
         ctx->c.Create(npoints, points, ne, ele_pid, bcid, cfg, default_E, default_v, default_rho, dt, gap);
@@ -88,33 +135,117 @@ int tdbem_dofmap_export(tdbem_ctx* ctx, long* counts, long* u_block, double* u_s
                         double* t_sign, int* t_known, long* hu_block, double* hu_sign, long* ht_block,
                         double* ht_sign, long* preferred)
 {
-    if (!ctx || !counts)
+    if (!ctx || !counts || (u_block && (!u_sign || !u_known || !t_block || !t_sign || !t_known || !hu_block ||
+                                        !hu_sign || !ht_block || !ht_sign)))
         return Fail("tdbem_dofmap_export: invalid arguments");
-    const tdbem::GlobalDofMap& d = ctx->c.dof;
-    counts[0] = d.unknownBlockCount;
-    counts[1] = d.knownBlockCount;
-    counts[2] = d.equationBlockCount;
-    counts[3] = d.outerDisplacementUnknownBlocks;
-    counts[4] = d.outerTractionUnknownBlocks;
-    counts[5] = d.interfaceDisplacementUnknownBlocks;
-    counts[6] = d.interfaceTractionUnknownBlocks;
-    counts[7] = d.historyUBlockCount;
-    counts[8] = d.historyTBlockCount;
-    for (long i = 0; i < ctx->c.ne; ++i) {
-        int dm = ctx->c.domOf[(size_t)i];
-        tdbem::VariableRef u = d.GetU(dm, i), t = d.GetT(dm, i), hu = d.GetHistoryU(dm, i), ht = d.GetHistoryT(dm, i);
-        u_block[i] = u.globalBlock;
-        u_sign[i] = u.sign;
-        u_known[i] = u.known;
-        t_block[i] = t.globalBlock;
-        t_sign[i] = t.sign;
-        t_known[i] = t.known;
-        hu_block[i] = hu.globalBlock;
-        hu_sign[i] = hu.sign;
-        ht_block[i] = ht.globalBlock;
-        ht_sign[i] = ht.sign;
-        preferred[i] = d.GetPreferredUnknownBlockForRow(i);
+    ExportDofMap(ctx->c.dof, ctx->c.domOf, ctx->c.ne, counts, u_block, u_sign, u_known, t_block, t_sign, t_known,
+                 hu_block, hu_sign, ht_block, ht_sign, preferred);
+    return 0;
+}
+
+int tdbem_dofmap_build(long ne, const int* bcid, int domain_count, const double* mat_rows, int interface_count,
+                       const long* itf_rows, long* counts, long* u_block, double* u_sign, int* u_known,
+                       long* t_block, double* t_sign, int* t_known, long* hu_block, double* hu_sign,
+                       long* ht_block, double* ht_sign, long* preferred)
+{
+    if (ne <= 0 || !bcid || !counts || (domain_count > 0 && !mat_rows) ||
+        (domain_count > 0 && interface_count > 0 && !itf_rows) ||
+        (u_block && (!u_sign || !u_known || !t_block || !t_sign || !t_known || !hu_block || !hu_sign || !ht_block ||
+                     !ht_sign)))
+        return Fail("tdbem_dofmap_build: invalid arguments");
+    try {
+        tdbem::MultiDomainInputConfig cfg = MakeConfig(domain_count, mat_rows, interface_count, itf_rows);
+        tdbem::MultiDomainModel model;
+        // the materials do not enter the DOF map; unit values stand in for the global line
+        if (!tdbem::BuildMultiDomainModel(cfg, ne, ne, 1.0, 0.0, 1.0, 1.0, model))
+            return Fail("tdbem_dofmap_build: BuildMultiDomainModel failed");
+        tdbem::GlobalDofMap dof;
+        if (!dof.Build(model, bcid, ne))
+            return Fail("tdbem_dofmap_build: GlobalDofMap::Build failed");
+        std::vector<int> domOf((size_t)ne, 0);
+        for (const tdbem::Domain& d : model.domains)
+            for (long e : d.elementIds)
+                domOf[(size_t)e] = d.id;
+        ExportDofMap(dof, domOf, ne, counts, u_block, u_sign, u_known, t_block, t_sign, t_known, hu_block, hu_sign,
+                     ht_block, ht_sign, preferred);
+    } catch (const std::exception& e) {
+        return Fail("tdbem_dofmap_build", e);
     }
+    return 0;
+}
+
+int tdbem_builder_create(tdbem_builder** out, long block_rows, long block_cols)
+{
+    if (!out || block_rows < 0 || block_cols < 0)
+        return Fail("tdbem_builder_create: invalid arguments");
+    try {
+        tdbem_builder* b = new tdbem_builder();
+        b->b.Reset(block_rows, block_cols);
+        *out = b;
+    } catch (const std::exception& e) {
+        return Fail("tdbem_builder_create", e);
+    }
+    return 0;
+}
+
+void tdbem_builder_destroy(tdbem_builder* b) { delete b; }
+
+int tdbem_builder_add_block(tdbem_builder* b, long row, long col, double sign, const double* block9)
+{
+    if (!b)
+        return Fail("tdbem_builder_add_block: invalid arguments");
+    b->b.AddBlock(row, col, sign, block9);
+    return 0;
+}
+
+int tdbem_builder_merge(tdbem_builder* b, const tdbem_builder* other)
+{
+    if (!b || !other)
+        return Fail("tdbem_builder_merge: invalid arguments");
+    b->b.MergeFrom(other->b);
+    return 0;
+}
+
+int tdbem_builder_info(const tdbem_builder* b, long* nnz, long* skipped_nonfinite)
+{
+    if (!b)
+        return Fail("tdbem_builder_info: invalid arguments");
+    if (nnz)
+        *nnz = b->b.NonZeroBlocks();
+    if (skipped_nonfinite)
+        *skipped_nonfinite = b->b.SkippedNonFiniteBlocks();
+    return 0;
+}
+
+int tdbem_builder_build(const tdbem_builder* b, int* row_ptr, int* col, double* val)
+{
+    if (!b || !row_ptr)
+        return Fail("tdbem_builder_build: invalid arguments");
+    tdbem::CCSRMat m;
+    b->b.Build(m);
+    std::memcpy(row_ptr, m.row_ptr.data(), sizeof(int) * m.row_ptr.size());
+    if (col && !m.col_index.empty())
+        std::memcpy(col, m.col_index.data(), sizeof(int) * m.col_index.size());
+    if (val && !m.value.empty())
+        std::memcpy(val, m.value.data(), sizeof(double) * m.value.size());
+    return 0;
+}
+
+int tdbem_builder_build_sym(const tdbem_builder* b, double tolerance, int* row_ptr, int* col, double* val6,
+                            int* symmetric)
+{
+    if (!b || !row_ptr)
+        return Fail("tdbem_builder_build_sym: invalid arguments");
+    tdbem::SymCCSRMat m;
+    bool sym = false;
+    b->b.BuildSym(m, tolerance, &sym);
+    std::memcpy(row_ptr, m.row_ptr.data(), sizeof(int) * m.row_ptr.size());
+    if (col && !m.col_index.empty())
+        std::memcpy(col, m.col_index.data(), sizeof(int) * m.col_index.size());
+    if (val6 && !m.value.empty())
+        std::memcpy(val6, m.value.data(), sizeof(double) * m.value.size());
+    if (symmetric)
+        *symmetric = sym ? 1 : 0;
     return 0;
 }
 
@@ -343,7 +474,7 @@ int tdbem_set_materials(tdbem_ctx* ctx, const double* evr)
 int tdbem_dyna_gmres_solver(tdbem_ctx* ctx, long iterations, double error, long maxleafpointnum, int nstep,
                             const double* known_lu, double* lu, double* lt, int* result)
 {
-    if (!ctx || !result || nstep < 1)
+    if (!ctx || !result || !known_lu || !lu || !lt || nstep < 1 || iterations < 1 || maxleafpointnum < 1)
         return Fail("tdbem_dyna_gmres_solver: invalid arguments");
     std::vector<int> it((size_t)nstep), fl((size_t)nstep);
     int solved = 0;

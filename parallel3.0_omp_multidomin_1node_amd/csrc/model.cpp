@@ -283,20 +283,28 @@ void MultiDomainCCSRBuilder::Reset(long blockRows, long blockCols)
     m_blockRows = blockRows;
     m_blockCols = blockCols;
     m_blocks.clear();
+    m_skippedNonFinite = 0;
 }
 
 void MultiDomainCCSRBuilder::AddBlock(long row, long col, double sign, const double block9[9])
 {
     if (row < 0 || row >= m_blockRows || col < 0 || col >= m_blockCols)
         return;
+    if (!block9 || sign == 0.0)
+        return;
     bool zero = true;
     for (int i = 0; i < 9; ++i)
         if (block9[i] != 0.0)
             zero = false;
-    if (!block9 || sign == 0.0 || zero || BlockMaxAbs9(block9) <= kAbsTol)
+    if (zero || BlockMaxAbs9(block9) <= kAbsTol)
         return;
-    if (!IsFiniteBlock9(block9))
+    if (!IsFiniteBlock9(block9)) {
+        // skipped and reported, the first 20 per builder (DBEM1/multidomain.cpp:1126-1133)
+        if (m_skippedNonFinite < 20)
+            std::fprintf(stderr, "MultiDomain skipped non-finite block row=%ld col=%ld\n", row, col);
+        ++m_skippedNonFinite;
         return;
+    }
     Entry& e = m_blocks[Key(row, col)];
     if (!e.alive) {
         e.v.fill(0.0);

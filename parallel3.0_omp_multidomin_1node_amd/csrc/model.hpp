@@ -146,6 +146,8 @@ public:
     long NonZeroBlocks() const;
     long BlockRows() const { return m_blockRows; }
     long BlockCols() const { return m_blockCols; }
+    // blocks AddBlock skipped for a NaN or infinite value since the last Reset
+    long SkippedNonFiniteBlocks() const { return m_skippedNonFinite; }
 
 private:
     struct Entry {
@@ -153,6 +155,7 @@ private:
         bool alive;
     };
     long m_blockRows = 0, m_blockCols = 0;
+    long m_skippedNonFinite = 0;
     std::unordered_map<uint64_t, Entry> m_blocks;
     void Sorted(std::vector<std::pair<uint64_t, const Entry*>>& out) const;
 };

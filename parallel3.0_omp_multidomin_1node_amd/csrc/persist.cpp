@@ -148,49 +148,53 @@ void Case::LoadHistory(const std::string& path)
     std::vector<int> kb((size_t)npairs), len((size_t)npairs);
     Get(f.get(), kb.data(), sizeof(int) * kb.size(), path);
     Get(f.get(), len.data(), sizeof(int) * len.size(), path);
-    // the loading case's own cull must give the same bands
-    Cull(fileMaxn);
-    HostBands();
-    if (kb != h_kb || len != h_len || entries != counts[2])
-        throw std::runtime_error("history file bands differ from this model's causality cull");
-    EnsureSelfTables();
-    h_mts.clear();
-    h_mgs.clear();
-    h_alive.clear();
-    d_mts.free();
-    d_mgs.free();
-    d_mts32.free();
-    d_mgs32.free();
-    d_mtsLo.free();
-    d_mgsLo.free();
-    d_alive.free();
-    historyDone = false;
+    // The loading case's own cull must give the same bands.  The cull
+    // replaces the band offsets the resident blocks were indexed by, so from
+    // here on any refusal or read error also drops those blocks: the context
+    // is left with no history rather than with blocks of another band layout.
+    ReleaseHistory();
+    precull = false;
+    try {
+        Cull(fileMaxn);
+        HostBands();
+        if (kb != h_kb || len != h_len || entries != counts[2])
+            throw std::runtime_error("history file bands differ from this model's causality cull");
+        EnsureSelfTables();
+        LoadHistoryBlocks(f.get(), path, head[1], fileMaxn);
+    } catch (...) {
+        ReleaseHistory();
+        throw;
+    }
+}
+
+void Case::LoadHistoryBlocks(FILE* f, const std::string& path, uint32_t precision, int fileMaxn)
+{
     maxn = fileMaxn;
-    historyMixed = head[1] != 0;
-    historySplit = head[1] == 2;
+    historyMixed = precision != 0;
+    historySplit = precision == 2;
     size_t e = (size_t)entries;
     std::vector<char> stage(std::min(kStage, std::max<size_t>(1, e) * 72));
     if (historyMixed) {
         d_mts32.alloc(9 * e);
         d_mgs32.alloc(6 * e);
-        GetDevice(f.get(), d_mts32.p, 9 * e * sizeof(float), stage, path, stream);
-        GetDevice(f.get(), d_mgs32.p, 6 * e * sizeof(float), stage, path, stream);
+        GetDevice(f, d_mts32.p, 9 * e * sizeof(float), stage, path, stream);
+        GetDevice(f, d_mgs32.p, 6 * e * sizeof(float), stage, path, stream);
         if (historySplit) {
             d_mtsLo.alloc(9 * e);
             d_mgsLo.alloc(6 * e);
-            GetDevice(f.get(), d_mtsLo.p, 9 * e * sizeof(unsigned short), stage, path, stream);
-            GetDevice(f.get(), d_mgsLo.p, 6 * e * sizeof(unsigned short), stage, path, stream);
+            GetDevice(f, d_mtsLo.p, 9 * e * sizeof(unsigned short), stage, path, stream);
+            GetDevice(f, d_mgsLo.p, 6 * e * sizeof(unsigned short), stage, path, stream);
         }
     } else {
         d_mts.alloc(9 * e);
         d_mgs.alloc(6 * e);
-        GetDevice(f.get(), d_mts.p, 9 * e * sizeof(double), stage, path, stream);
-        GetDevice(f.get(), d_mgs.p, 6 * e * sizeof(double), stage, path, stream);
+        GetDevice(f, d_mts.p, 9 * e * sizeof(double), stage, path, stream);
+        GetDevice(f, d_mgs.p, 6 * e * sizeof(double), stage, path, stream);
     }
     d_alive.alloc(std::max<size_t>(1, e));
-    GetDevice(f.get(), d_alive.p, e, stage, path, stream);
+    GetDevice(f, d_alive.p, e, stage, path, stream);
     char extra;
-    if (std::fread(&extra, 1, 1, f.get()) == 1)
+    if (std::fread(&extra, 1, 1, f) == 1)
         throw std::runtime_error("history file " + path + " is longer than its header says");
     historyDone = true;
 }

@@ -127,6 +127,7 @@ public:
     // replaces any assembled one.
     void SaveHistory(const std::string& path);
     void LoadHistory(const std::string& path);
+    void LoadHistoryBlocks(FILE* f, const std::string& path, uint32_t precision, int fileMaxn);
 
     long ne = 0;
     double dt = 0.0;

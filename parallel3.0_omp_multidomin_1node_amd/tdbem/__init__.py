@@ -27,7 +27,9 @@ EXPORTED = [
     "tdbem_timings", "tdbem_timings_n", "tdbem_history_stats", "tdbem_device_synchronize",
     "tdbem_set_option", "tdbem_model_counts", "tdbem_read_model", "tdbem_read_bd", "tdbem_write_model_bin",
     "tdbem_write_bd_bin", "tdbem_write_validation", "tdbem_history_save", "tdbem_history_load",
-    "tdbem_run_models", "tdbem_release_cached_memory", "tdbem_autotune",
+    "tdbem_run_models", "tdbem_release_cached_memory", "tdbem_autotune", "tdbem_dofmap_build",
+    "tdbem_builder_create", "tdbem_builder_destroy", "tdbem_builder_add_block", "tdbem_builder_merge",
+    "tdbem_builder_info", "tdbem_builder_build", "tdbem_builder_build_sym",
 ]
 
 
@@ -85,8 +87,89 @@ def lib():
                                        _pp, _pp, _ip]
         L.tdbem_autotune.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                      C.POINTER(C.c_double)]
+        L.tdbem_dofmap_build.argtypes = [C.c_long, _ip, C.c_int, _dp, C.c_int, _lp, _lp, _lp, _dp, _ip, _lp, _dp,
+                                         _ip, _lp, _dp, _lp, _dp, _lp]
+        L.tdbem_builder_create.argtypes = [C.POINTER(C.c_void_p), C.c_long, C.c_long]
+        L.tdbem_builder_destroy.argtypes = [C.c_void_p]
+        L.tdbem_builder_add_block.argtypes = [C.c_void_p, C.c_long, C.c_long, C.c_double, C.c_void_p]
+        L.tdbem_builder_merge.argtypes = [C.c_void_p, C.c_void_p]
+        L.tdbem_builder_info.argtypes = [C.c_void_p, C.POINTER(C.c_long), C.POINTER(C.c_long)]
+        L.tdbem_builder_build.argtypes = [C.c_void_p, _ip, C.c_void_p, C.c_void_p]
+        L.tdbem_builder_build_sym.argtypes = [C.c_void_p, C.c_double, _ip, C.c_void_p, C.c_void_p,
+                                              C.POINTER(C.c_int)]
         _lib = L
     return _lib
+
+
+def dofmap_build(bcid, domain_count=0, mat_rows=None, itf_rows=None):
+    """GlobalDofMap::Build on the host (tdbem_dofmap_build): the arrays of
+    Case.dofmap() from the element BCIDs and the multi-domain tail, without a
+    device.  Raises on the reference's failures (BCIDs other than 123 / 456)."""
+    bcid = np.ascontiguousarray(bcid, np.int32)
+    n = len(bcid)
+    mats = np.ascontiguousarray(mat_rows if mat_rows is not None else np.zeros((1, 4)), np.float64).ravel()
+    itf = np.ascontiguousarray(itf_rows if itf_rows is not None else np.zeros((1, 5)), np.int64).ravel()
+    ni = -1 if itf_rows is None else len(np.asarray(itf_rows).reshape(-1, 5))
+    counts = np.zeros(9, np.int64)
+    arr = dict(u_block=np.zeros(n, np.int64), u_sign=np.zeros(n), u_known=np.zeros(n, np.int32),
+               t_block=np.zeros(n, np.int64), t_sign=np.zeros(n), t_known=np.zeros(n, np.int32),
+               hu_block=np.zeros(n, np.int64), hu_sign=np.zeros(n), ht_block=np.zeros(n, np.int64),
+               ht_sign=np.zeros(n), preferred=np.zeros(n, np.int64))
+    _check(lib().tdbem_dofmap_build(n, bcid, int(domain_count), mats, ni, itf, counts, arr["u_block"],
+                                    arr["u_sign"], arr["u_known"], arr["t_block"], arr["t_sign"], arr["t_known"],
+                                    arr["hu_block"], arr["hu_sign"], arr["ht_block"], arr["ht_sign"],
+                                    arr["preferred"]), "tdbem_dofmap_build")
+    arr["counts"] = counts
+    return arr
+
+
+class Builder:
+    """MultiDomainCCSRBuilder of the library (tdbem_builder_*), host only."""
+
+    def __init__(self, block_rows, block_cols):
+        h = C.c_void_p()
+        _check(lib().tdbem_builder_create(C.byref(h), block_rows, block_cols), "tdbem_builder_create")
+        self.h, self.rows = h, block_rows
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().tdbem_builder_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def add_block(self, row, col, sign, block9):
+        if block9 is None:
+            _check(lib().tdbem_builder_add_block(self.h, row, col, sign, None), "tdbem_builder_add_block")
+            return
+        b = np.ascontiguousarray(block9, np.float64).ravel()
+        _check(lib().tdbem_builder_add_block(self.h, row, col, sign, b.ctypes.data), "tdbem_builder_add_block")
+
+    def merge(self, other):
+        _check(lib().tdbem_builder_merge(self.h, other.h), "tdbem_builder_merge")
+
+    def info(self):
+        nnz, sk = C.c_long(), C.c_long()
+        _check(lib().tdbem_builder_info(self.h, C.byref(nnz), C.byref(sk)), "tdbem_builder_info")
+        return nnz.value, sk.value
+
+    def build(self):
+        nnz = self.info()[0]
+        rp = np.zeros(self.rows + 1, np.int32)
+        col = np.zeros(max(nnz, 1), np.int32)
+        val = np.zeros(max(nnz, 1) * 9)
+        _check(lib().tdbem_builder_build(self.h, rp, col.ctypes.data, val.ctypes.data), "tdbem_builder_build")
+        return rp, col[:nnz], val[:9 * nnz].reshape(-1, 3, 3)
+
+    def build_sym(self, tolerance):
+        nnz = self.info()[0]
+        rp = np.zeros(self.rows + 1, np.int32)
+        col = np.zeros(max(nnz, 1), np.int32)
+        val = np.zeros(max(nnz, 1) * 6)
+        sym = C.c_int()
+        _check(lib().tdbem_builder_build_sym(self.h, tolerance, rp, col.ctypes.data, val.ctypes.data,
+                                             C.byref(sym)), "tdbem_builder_build_sym")
+        return rp, col[:nnz], val[:6 * nnz].reshape(-1, 6), bool(sym.value)
 
 
 def _check(rc: int, what: str) -> None:
@@ -224,6 +307,21 @@ class Case:
                                  C.byref(solved), times), "tdbem_solve")
         return (lu.reshape(nstep + 1, self.ne, 3), lt.reshape(nstep + 1, self.ne, 3), iters, flags, solved.value,
                 times)
+
+    def dyna_gmres_solver(self, known_lu, nstep, iterations=200, error=1e-5, maxleaf=10):
+        """The whole DynaGMRESSolverMultiDomainCCSR (tdbem_dyna_gmres_solver):
+        step 0, history for nstep steps and the time loop.  Returns (result,
+        lu, lt) with result 1 when the last solved step's GMRES flag is 0 and
+        -1 otherwise."""
+        known = np.ascontiguousarray(known_lu, np.float64).reshape(-1)
+        if known.size != (nstep + 1) * self.ne * 3:
+            raise ValueError("known_lu must hold (nstep+1) x ne x 3 values")
+        lu = np.zeros_like(known)
+        lt = np.zeros_like(known)
+        result = C.c_int()
+        _check(lib().tdbem_dyna_gmres_solver(self.h, int(iterations), float(error), int(maxleaf), int(nstep), known,
+                                             lu, lt, C.byref(result)), "tdbem_dyna_gmres_solver")
+        return result.value, lu.reshape(nstep + 1, self.ne, 3), lt.reshape(nstep + 1, self.ne, 3)
 
     def solve_batched(self, known_lu, nstep, restart=200, tol=1e-5):
         """known_lu: (nrhs, nstep+1, ne, 3) load histories -> lu, lt of the same

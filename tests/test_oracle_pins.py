@@ -211,3 +211,159 @@ def test_t_kernels_telescope_to_static_traction(oracle_mod):
         Tst = k * (drdn * ((1 - 2 * v) * np.eye(3) + 3 * np.outer(rh, rh))
                    - (1 - 2 * v) * (np.outer(rh, nor) - np.outer(nor, rh)))
         np.testing.assert_allclose(S.reshape(3, 3), Tst.T, rtol=1e-9, atol=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# Regular-path quadrature (quirks Q1 and Q2).  The regular 4x4 rule
+# (IntDynaUij / IntDynaTij, DBEM1/dsquareelement.cpp:1955-2009,2233-2290) is
+# taken when a whole element stays between the P and S fronts for a whole step
+# (NeedCal = 2).  It weights every point by the 8-node serendipity shape
+# function N0(s1, s2; gap) of corner node 0 (m_NRGV[0], DBEM1/quadinfo.h:
+# 149-172, DBEM1/quadinfo.cpp:92-97) and sums T1(n) + T2(n) whatever typeT is.
+# The pins below restate that integral independently of the oracle: the
+# textbook serendipity function, the element map from the mesh corners, and
+# Gauss-Legendre rules from numpy, with only the point kernels (themselves
+# pinned by the telescoping identities above) taken from the oracle.
+# ---------------------------------------------------------------------------
+_GAP = 0.4
+
+
+def _serendipity(s1, s2, a):
+    """The eight textbook 8-node serendipity shape functions on [-a, a]^2,
+    nodes ordered corners (-,-), (+,-), (+,+), (-,+), then the edge midpoints
+    (0,-), (+,0), (0,+), (-,0) (DBEM1/quadinfo.h:80-107)."""
+    x, y = s1 / a, s2 / a
+    out = []
+    for xi, yi in ((-1, -1), (1, -1), (1, 1), (-1, 1)):
+        out.append(0.25 * (1 + x * xi) * (1 + y * yi) * (x * xi + y * yi - 1))
+    out.append(0.5 * (1 - x * x) * (1 - y))
+    out.append(0.5 * (1 + x) * (1 - y * y))
+    out.append(0.5 * (1 - x * x) * (1 + y))
+    out.append(0.5 * (1 - x) * (1 - y * y))
+    return np.array(out)
+
+
+def _regular_probe_case(O):
+    """Two 2x1x1 domains with v = 0.45 (c1/c2 = 3.3) and dt = 0.02: many
+    element / source / step triples lie wholly between the fronts."""
+    from tdbem import mesh
+    m = mesh.box_mesh(2, 4.0, 1.0, 1.0, 8, 2, 2)
+    mats = np.array([[10.0, 0.45, 1.0], [10.0, 0.45, 1.0]])
+    itf = m.interfaces.copy()
+    itf[:, [0, 1, 2, 3]] -= 1
+    return m, O.Case(m.points, m.ele, m.bcid, mats, itf.astype(np.int32), 0.02, _GAP)
+
+
+def _regular_triples(c, ne, nstep=30):
+    half = ne // 2
+    return [(f, s, k) for f in range(half) for s in range(0, half, 3) for k in range(1, nstep)
+            if c.needcal(f, s, 1, k) == 2]
+
+
+def _element_rule(m, c, f, s, k, which, x, w, weight):
+    """sum over the tensor rule (x, w) of weight(s1, s2) * J * K(n = k) on the
+    element's bilinear map from its mesh corners (flat square elements, so the
+    8-node geometry reduces to it); which 'U' -> U, 'T' -> T1 + T2, 'T1', 'T2'."""
+    cor = m.points[m.ele[f][:4]]
+    src = c.element_info(s)[0]
+    tot = np.zeros(9)
+    for i, s2 in enumerate(x):
+        for j, s1 in enumerate(x):
+            a, b = 0.5 * (1 + s1), 0.5 * (1 + s2)
+            p = cor[0] * (1 - a) * (1 - b) + cor[1] * a * (1 - b) + cor[2] * a * b + cor[3] * (1 - a) * b
+            d1 = 0.5 * (-(1 - b) * cor[0] + (1 - b) * cor[1] + b * cor[2] - b * cor[3])
+            d2 = 0.5 * (-(1 - a) * cor[0] - a * cor[1] + a * cor[2] + (1 - a) * cor[3])
+            nn = np.cross(d1, d2)
+            J = np.linalg.norm(nn)
+            nn = nn / J
+            K = np.zeros(9)
+            if which == "U":
+                K = c.kernel(0, 0, k, p - src, nn)[1]
+            if which in ("T", "T1"):
+                K = K + c.kernel(0, 1, k, p - src, nn)[1]
+            if which in ("T", "T2"):
+                K = K + c.kernel(0, 2, k, p - src, nn)[1]
+            tot += K * J * w[i] * w[j] * weight(s1, s2)
+    return tot
+
+
+def test_regular_weights_are_serendipity_n0(oracle_mod):
+    """Q1: the oracle's regular weights are N0(s1, s2; gap) times the 4x4
+    Gauss weight at the 16 Gauss points, with N0 the textbook serendipity
+    corner function (nodal at the eight nodes at +-gap, partition of unity).
+    Weight 1 (the patch-wise convention) differs by O(1)."""
+    m, c = _regular_probe_case(oracle_mod)
+    nodes = np.array([(-1, -1), (1, -1), (1, 1), (-1, 1), (0, -1), (1, 0), (0, 1), (-1, 0)], float) * _GAP
+    for j, (a, b) in enumerate(nodes):
+        np.testing.assert_allclose(_serendipity(a, b, _GAP), np.eye(8)[j], atol=1e-15)
+    rng = np.random.default_rng(7)
+    for a, b in rng.uniform(-1, 1, size=(20, 2)):
+        assert abs(_serendipity(a, b, _GAP).sum() - 1.0) < 1e-14
+    pos, ra, nrgv0 = c.regular_table()
+    x, w = np.polynomial.legendre.leggauss(4)
+    np.testing.assert_allclose(pos[:, 0], np.tile(x, 4), atol=1e-15)
+    np.testing.assert_allclose(pos[:, 1], np.repeat(x, 4), atol=1e-15)
+    np.testing.assert_allclose(ra, np.outer(w, w).ravel(), atol=1e-15)
+    n0 = np.array([_serendipity(a, b, _GAP)[0] for a, b in pos])
+    np.testing.assert_allclose(nrgv0, n0 * ra, rtol=1e-14, atol=1e-16)
+    assert _serendipity(0.0, 0.0, _GAP)[0] == -0.25
+    assert np.abs(nrgv0 - ra).max() > 0.1
+
+
+def test_regular_quadrature_matches_independent_rules(oracle_mod):
+    """On triples with NeedCal = 2 the oracle's judged U block equals the 4x4
+    rule of N0 * w * J * U(n) and its judged T block (typeT 1 and 2 alike)
+    the 4x4 rule of N0 * w * J * (T1(n) + T2(n)), restated here from the mesh
+    corners (<= 1e-13 of the block); both are within 3e-4 of a 24x24 rule of
+    the same integrand (the 4x4 rule's own error), and a weight of 1, T1 alone
+    or T2 alone would miss by more than 5 %."""
+    m, c = _regular_probe_case(oracle_mod)
+    trip = _regular_triples(c, len(m.ele))
+    assert len(trip) > 1000
+    rng = np.random.default_rng(11)
+    x4, w4 = np.polynomial.legendre.leggauss(4)
+    x24, w24 = np.polynomial.legendre.leggauss(24)
+    n0 = lambda a, b: _serendipity(a, b, _GAP)[0]
+    one = lambda a, b: 1.0
+    for t in rng.choice(len(trip), 5, replace=False):
+        f, s, k = trip[t]
+        flag, qu = c.quadrature(f, s, 0, k)
+        assert flag == 2
+        flag, qt = c.quadrature(f, s, 1, k)
+        assert flag == 2
+        su, st = np.abs(qu).max(), np.abs(qt).max()
+        assert np.abs(qu - _element_rule(m, c, f, s, k, "U", x4, w4, n0)).max() <= 1e-13 * su
+        assert np.abs(qt - _element_rule(m, c, f, s, k, "T", x4, w4, n0)).max() <= 1e-13 * st
+        if c.needcal(f, s, 2, k) == 2:
+            assert np.array_equal(c.quadrature(f, s, 2, k)[1], qt)
+        assert np.abs(qu - _element_rule(m, c, f, s, k, "U", x24, w24, n0)).max() <= 3e-4 * su
+        assert np.abs(qt - _element_rule(m, c, f, s, k, "T", x24, w24, n0)).max() <= 3e-4 * st
+        assert np.abs(qu - _element_rule(m, c, f, s, k, "U", x4, w4, one)).max() > 0.05 * su
+        assert np.abs(qt - _element_rule(m, c, f, s, k, "T", x4, w4, one)).max() > 0.05 * st
+        assert np.abs(qt - _element_rule(m, c, f, s, k, "T1", x4, w4, n0)).max() > 0.05 * st
+        assert np.abs(qt - _element_rule(m, c, f, s, k, "T2", x4, w4, n0)).max() > 0.05 * st
+
+
+def test_history_blocks_double_the_regular_t(oracle_mod):
+    """Q2 at the assembly level: where both T judges of a history entry take the
+    regular path, MTS[k] receives the typeT = 2 call and then the typeT = 1 call
+    (DBEM1/multidomain.cpp:2053-2064), each T1(k) + T2(k), so the stored block is
+    twice the regular T block (times the history DOF sign); MGS[k] holds the
+    regular U block (DBEM1/multidomain.cpp:1137-1141 transposes; both blocks are
+    symmetric here)."""
+    m, c = _regular_probe_case(oracle_mod)
+    trip = _regular_triples(c, len(m.ele))
+    both = [t for t in trip if c.needcal(t[0], t[1], 2, t[2]) == 2]
+    assert len(both) > 100
+    c.assemble(29)
+    d = c.dofmap()
+    rng = np.random.default_rng(13)
+    for t in rng.choice(len(both), 6, replace=False):
+        f, s, k = both[t]
+        for which_m, blk, sgn, which_q, factor in ((2, "hu_block", "hu_sign", 1, 2.0),
+                                                   (3, "ht_block", "ht_sign", 0, 1.0)):
+            rp, col, val = c.ccsr(which_m, k)
+            hit = [i for i in range(rp[s], rp[s + 1]) if col[i] == d[blk][f]]
+            assert len(hit) == 1
+            q = c.quadrature(f, s, which_q, k)[1].reshape(3, 3)
+            assert np.array_equal(val[hit[0]], factor * d[sgn][f] * q.T), (which_m, f, s, k)

@@ -95,6 +95,13 @@ int tdbem_assemble_history(tdbem_ctx* ctx, int maxn, long* entries);
 int tdbem_ccsr_export(tdbem_ctx* ctx, int which, int step, long* nrows, long* nnz, int* row_ptr, int* col,
                       double* val);
 
+/* The rows row0 .. row1-1 of MTS[step] (which 2) or MGS[step] (which 3) in
+ * the same form, copying only those rows' bands from the device (for checks
+ * of sampled rows of large cases); row_ptr has row1-row0+1 entries starting
+ * at 0.  Call with null arrays to get *nnz. */
+int tdbem_ccsr_export_rows(tdbem_ctx* ctx, int which, int step, long row0, long row1, long* nnz, int* row_ptr,
+                           int* col, double* val);
+
 /* Singular self tables of every element (U1(0), T1(0), T2(1); column-major
  * 9 values each), for checks (DBEM1/dsquareelementdyna.cpp:239-490). */
 int tdbem_self_tables(tdbem_ctx* ctx, double* u1, double* t1, double* t2);

@@ -49,6 +49,8 @@ def lib():
         L.orc_assemble.argtypes = [C.c_void_p, C.c_int]
         L.orc_ccsr.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_long), C.POINTER(C.c_long),
                                C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_history_rows.argtypes = [C.c_void_p, C.c_int, C.c_long, C.c_long, C.c_int, C.POINTER(C.c_long),
+                                       C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_precond_info.argtypes = [C.c_void_p, C.POINTER(C.c_long), C.c_void_p, C.c_void_p]
         L.orc_solve.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, C.c_long, C.c_double, C.c_long, _dp, _dp, _ip, _ip]
         L.orc_gmres_ccsr.argtypes = [C.c_long, _ip, _ip, _dp, _dp, C.c_long, C.c_double, _dp, _ip]
@@ -145,6 +147,19 @@ class Case:
         val = np.zeros(max(nnz.value, 1) * 9)
         lib().orc_ccsr(self.h, which, step, C.byref(nr), C.byref(nnz), rp.ctypes.data, col.ctypes.data,
                        val.ctypes.data)
+        return rp, col[:nnz.value], val[:9 * nnz.value].reshape(-1, 3, 3)
+
+    def history_rows(self, which, k, row0, row1):
+        """MTS[k] (which 2) or MGS[k] (which 3) restricted to source rows
+        row0 .. row1-1 without assembling the rest: (row_ptr, col, blocks)."""
+        nnz = C.c_long()
+        if lib().orc_history_rows(self.h, k, row0, row1, which, C.byref(nnz), None, None, None) < 0:
+            raise RuntimeError("orc_history_rows failed")
+        rp = np.zeros(row1 - row0 + 1, np.int32)
+        col = np.zeros(max(nnz.value, 1), np.int32)
+        val = np.zeros(max(nnz.value, 1) * 9)
+        lib().orc_history_rows(self.h, k, row0, row1, which, C.byref(nnz), rp.ctypes.data, col.ctypes.data,
+                               val.ctypes.data)
         return rp, col[:nnz.value], val[:9 * nnz.value].reshape(-1, 3, 3)
 
     def precond_info(self):

@@ -1915,7 +1915,7 @@ static void assemble_step0(orc_case* c, builder* A, builder* K)
 
 /* ComputeT2/T1/U BlockCurrentMat + AssembleMultiDomainHistoryStep
  * (DBEM1/multidomain.cpp:1849-1905,2017-2078,2329-2378) */
-static void assemble_history(orc_case* c, int k, builder* mts, builder* mgs)
+static void assemble_history_rows(orc_case* c, int k, long row0, long row1, builder* mts, builder* mgs)
 {
     int d;
     long s, f;
@@ -1924,6 +1924,8 @@ static void assemble_history(orc_case* c, int k, builder* mts, builder* mgs)
     for (d = 0; d < c->nd; ++d) {
         const dmat* m = &c->mats[d];
         for (s = c->dom_begin[d]; s < c->dom_begin[d] + c->dom_count[d]; ++s) {
+            if (s < row0 || s >= row1)
+                continue;
             const double* src = c->el[s].center;
             for (f = c->dom_begin[d]; f < c->dom_begin[d] + c->dom_count[d]; ++f) {
                 const elem* fe = &c->el[f];
@@ -1945,6 +1947,11 @@ static void assemble_history(orc_case* c, int k, builder* mts, builder* mgs)
             }
         }
     }
+}
+
+static void assemble_history(orc_case* c, int k, builder* mts, builder* mgs)
+{
+    assemble_history_rows(c, k, 0, c->ne, mts, mgs);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -2493,6 +2500,41 @@ int orc_ccsr(orc_case* c, int which, int step, long* nrows, long* nnz, int* row_
         memcpy(col, M->col, sizeof(int) * (size_t)M->nnz);
     if (val)
         memcpy(val, M->val, sizeof(double) * 9 * (size_t)M->nnz);
+    return 0;
+}
+
+int orc_history_rows(orc_case* c, int k, long row0, long row1, int which, long* nnz, int* row_ptr, int* col,
+                     double* val)
+{
+    builder b;
+    ccsr M;
+    long r;
+    if (!c || !c->dof_ok || k < 1 || row0 < 0 || row1 > c->ne || row0 >= row1 || (which != 2 && which != 3))
+        return -1;
+    {
+        builder bt, bg;
+        bld_init(&bt, c->counts[2], c->counts[7]);
+        bld_init(&bg, c->counts[2], c->counts[8]);
+        assemble_history_rows(c, k, row0, row1, &bt, &bg);
+        if (which == 2) {
+            b = bt;
+            bld_free(&bg);
+        } else {
+            b = bg;
+            bld_free(&bt);
+        }
+    }
+    bld_build(&b, &M);
+    bld_free(&b);
+    *nnz = M.row_ptr[row1] - M.row_ptr[row0];
+    if (row_ptr)
+        for (r = row0; r <= row1; ++r)
+            row_ptr[r - row0] = M.row_ptr[r] - M.row_ptr[row0];
+    if (col)
+        memcpy(col, M.col + M.row_ptr[row0], sizeof(int) * (size_t)*nnz);
+    if (val)
+        memcpy(val, M.val + 9 * (size_t)M.row_ptr[row0], sizeof(double) * 9 * (size_t)*nnz);
+    ccsr_free(&M);
     return 0;
 }
 

@@ -69,6 +69,13 @@ int orc_ccsr(orc_case* c, int which, int step, long* nrows, long* nnz,
              int* row_ptr, int* col, double* val);
 /* which: 0 = A, 1 = KnownM, 2 = MTS[step], 3 = MGS[step] (full 9 values) */
 
+/* History rows without a full assembly: MTS[k] (which 2) or MGS[k] (which 3,
+ * full 9 values) restricted to source rows row0 .. row1-1, assembled exactly
+ * as orc_assemble does for those rows.  row_ptr has row1-row0+1 entries
+ * (starting at 0); call with null arrays for *nnz. */
+int orc_history_rows(orc_case* c, int k, long row0, long row1, int which, long* nnz, int* row_ptr, int* col,
+                     double* val);
+
 /* Preconditioner export: leaf count, RePID order and leaf sizes. */
 int orc_precond_info(orc_case* c, long* leaf_count, long* leaf_size, long* repid);
 

@@ -298,6 +298,28 @@ int tdbem_ccsr_export(tdbem_ctx* ctx, int which, int step, long* nrows, long* nn
     return 0;
 }
 
+int tdbem_ccsr_export_rows(tdbem_ctx* ctx, int which, int step, long row0, long row1, long* nnz, int* row_ptr,
+                           int* col, double* val)
+{
+    if (!ctx || !nnz || which < 2 || which > 3)
+        return Fail("tdbem_ccsr_export_rows: invalid arguments");
+    std::vector<int> rp, c;
+    std::vector<double> v;
+    try {
+        ctx->c.ExportRows(which, step, row0, row1, rp, c, v);
+    } catch (const std::exception& e) {
+        return Fail("tdbem_ccsr_export_rows", e);
+    }
+    *nnz = (long)c.size();
+    if (row_ptr)
+        std::memcpy(row_ptr, rp.data(), sizeof(int) * rp.size());
+    if (col && !c.empty())
+        std::memcpy(col, c.data(), sizeof(int) * c.size());
+    if (val && !v.empty())
+        std::memcpy(val, v.data(), sizeof(double) * v.size());
+    return 0;
+}
+
 int tdbem_self_tables(tdbem_ctx* ctx, double* u1, double* t1, double* t2)
 {
     if (!ctx || !u1 || !t1 || !t2)

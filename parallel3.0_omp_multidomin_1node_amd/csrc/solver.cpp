@@ -833,6 +833,93 @@ void Case::ExportMatrix(int which, int step, std::vector<int>& rp, std::vector<i
         rp[(size_t)r + 1] += rp[(size_t)r];
 }
 
+// MTS[step] / MGS[step] restricted to source rows row0 .. row1-1: only the
+// band slice of those rows' pairs is copied from the device (a source row's
+// pairs, and so its band entries, are contiguous), so sampled rows of a
+// large case export without moving the whole history to the host.
+void Case::ExportRows(int which, int step, long row0, long row1, std::vector<int>& rp, std::vector<int>& col,
+                      std::vector<double>& val)
+{
+    if (which != 2 && which != 3)
+        throw std::runtime_error("row export covers the history matrices (2, 3)");
+    if (!historyDone || step < 1 || step > maxn)
+        throw std::runtime_error("history step not assembled");
+    if (row0 < 0 || row1 > ne || row0 >= row1)
+        throw std::runtime_error("row range out of bounds");
+    HostBands();
+    rp.assign((size_t)(row1 - row0) + 1, 0);
+    col.clear();
+    val.clear();
+    for (long s = row0; s < row1; ++s) {
+        int d = 0;
+        while (d + 1 < grid.nd && s >= grid.eleBegin[d + 1])
+            ++d;
+        const long n = grid.eleCount[d];
+        const long p0 = grid.pairBase[d] + (s - grid.eleBegin[d]) * n, p1 = p0 + n;
+        const long e0 = h_off[(size_t)p0], e1 = h_off[(size_t)p1], m = e1 - e0;
+        std::vector<double> mts(9 * (size_t)m), mgs(6 * (size_t)m);
+        std::vector<unsigned char> alive((size_t)m);
+        if (m > 0) {
+            if (historyMixed) {
+                std::vector<float> a(mts.size()), b(mgs.size());
+                Check(CopyOnStream(a.data(), d_mts32.p + 9 * e0, sizeof(float) * a.size(), hipMemcpyDeviceToHost,
+                                   stream), "mts rows");
+                Check(CopyOnStream(b.data(), d_mgs32.p + 6 * e0, sizeof(float) * b.size(), hipMemcpyDeviceToHost,
+                                   stream), "mgs rows");
+                std::copy(a.begin(), a.end(), mts.begin());
+                std::copy(b.begin(), b.end(), mgs.begin());
+                if (historySplit) {
+                    std::vector<unsigned short> la(mts.size()), lb(mgs.size());
+                    Check(CopyOnStream(la.data(), d_mtsLo.p + 9 * e0, 2 * la.size(), hipMemcpyDeviceToHost, stream),
+                          "mts lo rows");
+                    Check(CopyOnStream(lb.data(), d_mgsLo.p + 6 * e0, 2 * lb.size(), hipMemcpyDeviceToHost, stream),
+                          "mgs lo rows");
+                    auto lo = [](unsigned short v) {
+                        unsigned int bits = (unsigned int)v << 16;
+                        float f;
+                        std::memcpy(&f, &bits, 4);
+                        return (double)f;
+                    };
+                    for (size_t i = 0; i < la.size(); ++i)
+                        mts[i] += lo(la[i]);
+                    for (size_t i = 0; i < lb.size(); ++i)
+                        mgs[i] += lo(lb[i]);
+                }
+            } else {
+                Check(CopyOnStream(mts.data(), d_mts.p + 9 * e0, sizeof(double) * mts.size(), hipMemcpyDeviceToHost,
+                                   stream), "mts rows");
+                Check(CopyOnStream(mgs.data(), d_mgs.p + 6 * e0, sizeof(double) * mgs.size(), hipMemcpyDeviceToHost,
+                                   stream), "mgs rows");
+            }
+            Check(CopyOnStream(alive.data(), d_alive.p + e0, (size_t)m, hipMemcpyDeviceToHost, stream), "alive rows");
+        }
+        std::vector<std::pair<long, long>> items;  // (column block, entry - e0)
+        for (long p = p0; p < p1; ++p) {
+            const int L = h_len[(size_t)p], k0 = h_kb[(size_t)p];
+            if (step < k0 || step >= k0 + L)
+                continue;
+            const long e = h_off[(size_t)p] + (step - k0) - e0;
+            if (!(alive[(size_t)e] & (which == 2 ? 1 : 2)))
+                continue;
+            const long f = grid.eleBegin[d] + (p - p0);
+            items.emplace_back(which == 2 ? refs[(size_t)f].hu_block : refs[(size_t)f].ht_block, e);
+        }
+        std::sort(items.begin(), items.end());
+        for (const auto& it : items) {
+            col.push_back((int)it.first);
+            if (which == 2) {
+                for (int k = 0; k < 9; ++k)
+                    val.push_back(mts[9 * (size_t)it.second + (size_t)k]);
+            } else {
+                const double* g = &mgs[6 * (size_t)it.second];
+                const double full[9] = {g[0], g[1], g[2], g[1], g[3], g[4], g[2], g[4], g[5]};
+                val.insert(val.end(), full, full + 9);
+            }
+        }
+        rp[(size_t)(s - row0) + 1] = (int)col.size();
+    }
+}
+
 // ---------------------------------------------------------------------------
 // GMRES (DBEM1/pre_gmres.cpp:1369-1605) with maxit = 1 and x0 = 0.  The
 // Arnoldi step uses classical Gram-Schmidt applied twice (two fused

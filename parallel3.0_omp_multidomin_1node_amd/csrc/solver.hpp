@@ -121,6 +121,8 @@ public:
     // Patch-wise waves of the current (or frozen) judges (cached while frozen).
     void PackPwWaves();
     void ExportMatrix(int which, int step, std::vector<int>& rp, std::vector<int>& col, std::vector<double>& val);
+    void ExportRows(int which, int step, long row0, long row1, std::vector<int>& rp, std::vector<int>& col,
+                    std::vector<double>& val);
     bool HistoryReady() const { return historyDone; }
     // Assembled history bands to a file / from a file written for the same
     // model (same cull bands, materials, dt and gap); the loaded history

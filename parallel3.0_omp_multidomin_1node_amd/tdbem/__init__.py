@@ -29,7 +29,7 @@ EXPORTED = [
     "tdbem_write_bd_bin", "tdbem_write_validation", "tdbem_history_save", "tdbem_history_load",
     "tdbem_run_models", "tdbem_release_cached_memory", "tdbem_autotune", "tdbem_dofmap_build",
     "tdbem_builder_create", "tdbem_builder_destroy", "tdbem_builder_add_block", "tdbem_builder_merge",
-    "tdbem_builder_info", "tdbem_builder_build", "tdbem_builder_build_sym",
+    "tdbem_builder_info", "tdbem_builder_build", "tdbem_builder_build_sym", "tdbem_ccsr_export_rows",
 ]
 
 
@@ -87,6 +87,8 @@ def lib():
                                        _pp, _pp, _ip]
         L.tdbem_autotune.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                      C.POINTER(C.c_double)]
+        L.tdbem_ccsr_export_rows.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_long, C.c_long, C.POINTER(C.c_long),
+                                             C.c_void_p, C.c_void_p, C.c_void_p]
         L.tdbem_dofmap_build.argtypes = [C.c_long, _ip, C.c_int, _dp, C.c_int, _lp, _lp, _lp, _dp, _ip, _lp, _dp,
                                          _ip, _lp, _dp, _lp, _dp, _lp]
         L.tdbem_builder_create.argtypes = [C.POINTER(C.c_void_p), C.c_long, C.c_long]
@@ -285,6 +287,19 @@ class Case:
         val = np.zeros(max(nnz.value, 1) * 9)
         _check(lib().tdbem_ccsr_export(self.h, which, step, C.byref(nr), C.byref(nnz), rp.ctypes.data,
                                        col.ctypes.data, val.ctypes.data), "tdbem_ccsr_export")
+        return rp, col[:nnz.value], val[:9 * nnz.value].reshape(-1, 3, 3)
+
+    def ccsr_rows(self, which, step, row0, row1):
+        """Rows row0 .. row1-1 of MTS[step] (2) / MGS[step] (3) as (row_ptr,
+        col, blocks), copying only those rows' bands from the device."""
+        nnz = C.c_long()
+        _check(lib().tdbem_ccsr_export_rows(self.h, which, step, row0, row1, C.byref(nnz), None, None, None),
+               "tdbem_ccsr_export_rows")
+        rp = np.zeros(row1 - row0 + 1, np.int32)
+        col = np.zeros(max(nnz.value, 1), np.int32)
+        val = np.zeros(max(nnz.value, 1) * 9)
+        _check(lib().tdbem_ccsr_export_rows(self.h, which, step, row0, row1, C.byref(nnz), rp.ctypes.data,
+                                            col.ctypes.data, val.ctypes.data), "tdbem_ccsr_export_rows")
         return rp, col[:nnz.value], val[:9 * nnz.value].reshape(-1, 3, 3)
 
     def self_tables(self):

@@ -11,6 +11,8 @@
 #ifndef TDBEM_CAPI_H
 #define TDBEM_CAPI_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -150,6 +152,22 @@ int tdbem_set_materials(tdbem_ctx* ctx, const double* evr);
  * times[3] as for tdbem_solve. */
 int tdbem_solve_batched(tdbem_ctx* ctx, int nstep, int nrhs, const double* known_lu, long restart, double tol,
                         double* lu, double* lt, int* iters, int* flags, int* solved, double* times);
+
+/* tdbem_solve / tdbem_solve_batched with the load histories already in
+ * device memory and the outputs left there (no counterpart in the reference,
+ * whose arrays are host memory): d_known_lu, d_lu, d_lt are device pointers
+ * of the same layouts (for example from tdbem_device_alloc); iters, flags,
+ * solved and times stay host arrays.  Same results as the host-buffer calls. */
+int tdbem_solve_device(tdbem_ctx* ctx, int nstep, const double* d_known_lu, long restart, double tol, double* d_lu,
+                       double* d_lt, int* iters, int* flags, int* solved, double* times);
+int tdbem_solve_batched_device(tdbem_ctx* ctx, int nstep, int nrhs, const double* d_known_lu, long restart,
+                               double tol, double* d_lu, double* d_lt, int* iters, int* flags, int* solved,
+                               double* times);
+/* Device memory for the calls above: allocate, free, and copy (kind 0 host to
+ * device, 1 device to host, 2 device to device; synchronous). */
+int tdbem_device_alloc(size_t bytes, void** out);
+int tdbem_device_free(void* p);
+int tdbem_memcpy(void* dst, const void* src, size_t bytes, int kind);
 
 /* Independent models run together: n contexts, each on its own stream and
  * driven by its own host thread, do their whole job concurrently -- step-0

@@ -125,7 +125,7 @@ void Case::GmresBatched(int B, const double* rhs, double* x, long restart, doubl
 }
 
 void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, double tol, double* lu, double* lt,
-                        int* iters, int* flags, int* solved)
+                        int* iters, int* flags, int* solved, bool deviceIO)
 {
     if (B < 1)
         throw std::runtime_error("batch size must be >= 1");
@@ -156,9 +156,18 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
     w.active.alloc((size_t)B);
     w.part.alloc((size_t)kBatchRedBlocks * (size_t)(restart + 1) * B);
     DevBuf<double> d_bd, d_lu, d_lt, d_gU, d_gT, d_FU, d_FT, d_known, d_rhs, d_x, d_hfar;
-    d_bd.upload(known_lu, (size_t)B * nstore * (size_t)n, stream);
-    d_lu.alloc((size_t)B * nstore * (size_t)n);
-    d_lt.alloc((size_t)B * nstore * (size_t)n);
+    const size_t outN = (size_t)B * nstore * (size_t)n;
+    const double* bd = known_lu;
+    double* olu = lu;
+    double* olt = lt;
+    if (!deviceIO) {
+        d_bd.upload(known_lu, outN, stream);
+        d_lu.alloc(outN);
+        d_lt.alloc(outN);
+        bd = d_bd.p;
+        olu = d_lu.p;
+        olt = d_lt.p;
+    }
     d_gU.alloc((size_t)histU * 3 * nstore * B);
     d_gT.alloc((size_t)histT * 3 * nstore * B);
     // filtered series: nstore + 1 rows per column, the last never written (zero)
@@ -169,8 +178,10 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
     d_x.alloc((size_t)n * B);
     const int farParts = batchedMfma ? batchedFarParts : 0;
     d_hfar.alloc((size_t)std::max(1, farParts) * kBatchTile * (size_t)n * B);
-    for (DevBuf<double>* z : {&d_lu, &d_lt, &d_gU, &d_gT, &d_FU, &d_FT, &d_known})
+    for (DevBuf<double>* z : {&d_gU, &d_gT, &d_FU, &d_FT, &d_known})
         LaunchZero(z->p, (long)z->n, stream);
+    LaunchZero(olu, (long)outN, stream);
+    LaunchZero(olt, (long)outN, stream);
     const unsigned short* mLo = historySplit ? d_mtsLo.p : nullptr;
     const unsigned short* gLo = historySplit ? d_mgsLo.p : nullptr;
     std::vector<int> it((size_t)B), fl((size_t)B);
@@ -180,7 +191,7 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
     double convTime = 0.0, gmresTime = 0.0;
     int K0 = 0;
     for (int K = 1; K <= nstep; ++K) {
-        LaunchKnownB(d_refs.p, d_bd.p, (int)ne, nstore, K, B, d_known.p, stream);
+        LaunchKnownB(d_refs.p, bd, (int)ne, nstore, K, B, d_known.p, stream);
         LaunchSpmmB(ne, d_Krp.p, d_Kcol.p, d_Kval.p, d_known.p, d_rhs.p, B, stream);
         if (K > 1) {
             double c0 = NowB();
@@ -226,10 +237,29 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
             else
                 any = true;
         }
-        LaunchScatterB(d_refs.p, d_geo.p, d_bd.p, d_x.p, (int)ne, nstore, K, B, d_lu.p, d_lt.p, stream);
-        LaunchStateB(d_refs.p, d_lu.p, d_lt.p, (int)ne, nstore, K, B, d_gU.p, d_gT.p, stream);
+        LaunchScatterB(d_refs.p, d_geo.p, bd, d_x.p, (int)ne, nstore, K, B, olu, olt, stream);
+        LaunchStateB(d_refs.p, olu, olt, (int)ne, nstore, K, B, d_gU.p, d_gT.p, stream);
         if (!any)
             break;
+    }
+    if (deviceIO) {
+        // a case's steps after its GMRES failure are left zero; step 0 in the global frame
+        for (int b = 0; b < B; ++b)
+            if (solved[b] < nstep) {
+                size_t from = ((size_t)b * nstore + (size_t)solved[b] + 1) * (size_t)n;
+                size_t cnt = (size_t)(nstep - solved[b]) * (size_t)n;
+                LaunchZero(olu + from, (long)cnt, stream);
+                LaunchZero(olt + from, (long)cnt, stream);
+            }
+        LaunchStep0Out(d_refs.p, d_geo.p, bd, (int)ne, nstore, B, olu, olt, stream);
+        CheckB(hipStreamSynchronize(stream), "batched solve");
+        CheckB(hipGetLastError(), "batched solve launch");
+        timing[4] = convTime;
+        timing[5] = gmresTime;
+        timing[6] = 0.0;
+        timing[7] = 0.0;
+        bw.release();
+        return;
     }
     CheckB(hipStreamSynchronize(stream), "batched solve");
     CheckB(hipGetLastError(), "batched solve launch");

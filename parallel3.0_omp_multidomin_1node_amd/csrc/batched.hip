@@ -126,6 +126,42 @@ void LaunchStateB(const NodeRefs* refs, const double* lu, const double* lt, int 
     k_state_b<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(refs, lu, lt, ne, nstore, K, B, gU, gT);
 }
 
+// Step 0 of every case in the global frame (DBEM1/multidomain.cpp:3819-3822):
+// lt = 0, the given triplets moved to lt for BCID 456 (Convert), then
+// LocToAbs; the outputs of device-resident solves.
+__global__ void __launch_bounds__(256) k_step0_out(const NodeRefs* __restrict__ refs, const ElemGeo* __restrict__ geo,
+                                                   const double* __restrict__ bd, int ne, int nstore, int B,
+                                                   double* __restrict__ lu, double* __restrict__ lt)
+{
+    long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)ne * B)
+        return;
+    int b = (int)(idx / ne), i = (int)(idx % ne);
+    long slice = (long)b * nstore * 3 * ne + 3 * (long)i;
+    double u[3], t[3];
+    for (int k = 0; k < 3; ++k) {
+        u[k] = bd[slice + k];
+        t[k] = 0.0;
+    }
+    if (refs[i].bcid == 456)
+        for (int k = 0; k < 3; ++k) {
+            t[k] = u[k];
+            u[k] = 0.0;
+        }
+    const double* F = geo[i].trans;
+    for (int r = 0; r < 3; ++r) {
+        lu[slice + r] = F[r] * u[0] + F[3 + r] * u[1] + F[6 + r] * u[2];
+        lt[slice + r] = F[r] * t[0] + F[3 + r] * t[1] + F[6 + r] * t[2];
+    }
+}
+
+void LaunchStep0Out(const NodeRefs* refs, const ElemGeo* geo, const double* bd, int ne, int nstore, int B, double* lu,
+                    double* lt, hipStream_t s)
+{
+    long n = (long)ne * B;
+    k_step0_out<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(refs, geo, bd, ne, nstore, B, lu, lt);
+}
+
 __global__ void __launch_bounds__(256) k_filter_b(const double* __restrict__ g, long cols, int nstore, int B, int x0,
                                                   int nx, int K0, double* __restrict__ F)
 {

@@ -16,6 +16,8 @@ constexpr int kBatchRedBlocks = 256;   // most reduction blocks of the batched d
 // Per-step plumbing.  bd / lu / lt are [b][K][3 ne] (host layout).
 void LaunchKnownB(const NodeRefs* refs, const double* bd, int ne, int nstore, int K, int B, double* known,
                   hipStream_t s);
+void LaunchStep0Out(const NodeRefs* refs, const ElemGeo* geo, const double* bd, int ne, int nstore, int B, double* lu,
+                    double* lt, hipStream_t s);
 void LaunchScatterB(const NodeRefs* refs, const ElemGeo* geo, const double* bd, const double* x, int ne, int nstore,
                     int K, int B, double* lu, double* lt, hipStream_t s);
 // raw state series g[((col * nstore + K) * 3 + c) * B + b]

@@ -455,6 +455,77 @@ int tdbem_solve_batched(tdbem_ctx* ctx, int nstep, int nrhs, const double* known
     return 0;
 }
 
+int tdbem_solve_device(tdbem_ctx* ctx, int nstep, const double* d_known_lu, long restart, double tol, double* d_lu,
+                       double* d_lt, int* iters, int* flags, int* solved, double* times)
+{
+    if (!ctx || nstep < 1 || !d_known_lu || !d_lu || !d_lt || !iters || !flags || !solved || restart < 1 ||
+        !(tol > 0.0))
+        return Fail("tdbem_solve_device: invalid arguments");
+    try {
+        ctx->c.Solve(nstep, d_known_lu, restart, tol, d_lu, d_lt, iters, flags, solved, true);
+    } catch (const std::exception& e) {
+        return Fail("tdbem_solve_device", e);
+    }
+    if (times) {
+        times[0] = ctx->c.timing[4];
+        times[1] = ctx->c.timing[5];
+        times[2] = ctx->c.timing[4] + ctx->c.timing[5];
+    }
+    return 0;
+}
+
+int tdbem_solve_batched_device(tdbem_ctx* ctx, int nstep, int nrhs, const double* d_known_lu, long restart,
+                               double tol, double* d_lu, double* d_lt, int* iters, int* flags, int* solved,
+                               double* times)
+{
+    if (!ctx || nstep < 1 || nrhs < 1 || !d_known_lu || !d_lu || !d_lt || !iters || !flags || !solved ||
+        restart < 1 || !(tol > 0.0))
+        return Fail("tdbem_solve_batched_device: invalid arguments");
+    try {
+        ctx->c.SolveBatched(nstep, nrhs, d_known_lu, restart, tol, d_lu, d_lt, iters, flags, solved, true);
+    } catch (const std::exception& e) {
+        return Fail("tdbem_solve_batched_device", e);
+    }
+    if (times) {
+        times[0] = ctx->c.timing[4];
+        times[1] = ctx->c.timing[5];
+        times[2] = ctx->c.timing[4] + ctx->c.timing[5];
+    }
+    return 0;
+}
+
+int tdbem_device_alloc(size_t bytes, void** out)
+{
+    if (!out || bytes == 0)
+        return Fail("tdbem_device_alloc: invalid arguments");
+    *out = nullptr;
+    if (hipMalloc(out, bytes) != hipSuccess) {
+        *out = nullptr;
+        (void)hipGetLastError();
+        return Fail("tdbem_device_alloc: hipMalloc of " + std::to_string(bytes) + " bytes failed");
+    }
+    return 0;
+}
+
+int tdbem_device_free(void* p)
+{
+    if (p && hipFree(p) != hipSuccess)
+        return Fail("tdbem_device_free: hipFree failed");
+    return 0;
+}
+
+int tdbem_memcpy(void* dst, const void* src, size_t bytes, int kind)
+{
+    static const hipMemcpyKind kinds[3] = {hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice};
+    if ((!dst || !src) && bytes)
+        return Fail("tdbem_memcpy: invalid arguments");
+    if (kind < 0 || kind > 2)
+        return Fail("tdbem_memcpy: kind must be 0 (host to device), 1 (device to host) or 2 (device to device)");
+    if (bytes && hipMemcpy(dst, src, bytes, kinds[kind]) != hipSuccess)
+        return Fail("tdbem_memcpy: hipMemcpy failed");
+    return 0;
+}
+
 int tdbem_adjoint(tdbem_ctx* ctx, int nstep, const double* known_lu, int nrec, const int* receivers,
                   const double* obs, long restart, double tol, double fd_step, double* J, double* grad, double* lu)
 {

@@ -2,6 +2,7 @@
 // and the time loop.  This file is the host side of the drop-in
 // DynaGMRESSolverMultiDomainCCSR (DBEM1/multidomain.cpp:3664-3926).
 #include "solver.hpp"
+#include "batched.hpp"
 
 #include <algorithm>
 #include <array>
@@ -1220,8 +1221,10 @@ int Case::GmresMixed(const double* d_b, double* d_x, long restart, double tol, i
 }
 
 void Case::Solve(int nstep, const double* known_lu, long restart, double tol, double* lu, double* lt, int* iters,
-                 int* flags, int* solved)
+                 int* flags, int* solved, bool deviceIO)
 {
+    if (deviceIO && keep)
+        throw std::runtime_error("device-resident inputs are not taken by the adjoint's forward solve");
     if (!step0Done)
         AssembleStep0(10);
     if (!historyDone || maxn < 1)
@@ -1244,9 +1247,18 @@ void Case::Solve(int nstep, const double* known_lu, long restart, double tol, do
         LaunchZero(keep->X.p, (long)keep->X.n, stream);
     }
     DevBuf<double> d_bd, d_lu, d_lt, d_gU, d_gT, d_known, d_rhs, d_x, d_hfar;
-    d_bd.upload(known_lu, (size_t)nstore * (size_t)n3, stream);
-    d_lu.alloc((size_t)nstore * (size_t)n3);
-    d_lt.alloc((size_t)nstore * (size_t)n3);
+    const size_t outN = (size_t)nstore * (size_t)n3;
+    const double* bd = known_lu;
+    double* olu = lu;
+    double* olt = lt;
+    if (!deviceIO) {
+        d_bd.upload(known_lu, outN, stream);
+        d_lu.alloc(outN);
+        d_lt.alloc(outN);
+        bd = d_bd.p;
+        olu = d_lu.p;
+        olt = d_lt.p;
+    }
     d_gU.alloc((size_t)dof.historyUBlockCount * 3 * (size_t)nstore);
     d_gT.alloc((size_t)dof.historyTBlockCount * 3 * (size_t)nstore);
     d_known.alloc(3 * (size_t)std::max<long>(1, dof.knownBlockCount));
@@ -1265,8 +1277,8 @@ void Case::Solve(int nstep, const double* known_lu, long restart, double tol, do
     LaunchZero(d_gU.p, (long)d_gU.n, stream);
     LaunchZero(d_gT.p, (long)d_gT.n, stream);
     LaunchZero(d_known.p, (long)d_known.n, stream);
-    LaunchZero(d_lu.p, (long)d_lu.n, stream);
-    LaunchZero(d_lt.p, (long)d_lt.n, stream);
+    LaunchZero(olu, (long)outN, stream);
+    LaunchZero(olt, (long)outN, stream);
     // per-step events (convolution start, GMRES start, GMRES end) and per-tile
     // far-kernel events, read once after the loop
     EventPool ev;
@@ -1276,7 +1288,7 @@ void Case::Solve(int nstep, const double* known_lu, long restart, double tol, do
     *solved = 0;
     int K0 = 2;
     for (int K = 1; K <= nstep; ++K) {
-        LaunchKnown(d_refs.p, d_bd.p + (size_t)K * (size_t)n3, (int)ne, d_known.p, dof.knownBlockCount, stream);
+        LaunchKnown(d_refs.p, bd + (size_t)K * (size_t)n3, (int)ne, d_known.p, dof.knownBlockCount, stream);
         LaunchSpmv(ne, d_Krp.p, d_Kcol.p, d_Kval.p, d_known.p, d_rhs.p, stream);
         std::array<hipEvent_t, 3> se{ev.Get(), ev.Get(), ev.Get()};
         Check(hipEventRecord(se[0], stream), "event");
@@ -1337,9 +1349,9 @@ void Case::Solve(int nstep, const double* known_lu, long restart, double tol, do
         *solved = K;
         if (flag != 0)
             break;
-        double* slu = d_lu.p + (size_t)K * (size_t)n3;
-        double* slt = d_lt.p + (size_t)K * (size_t)n3;
-        LaunchScatter(d_refs.p, d_geo.p, d_bd.p + (size_t)K * (size_t)n3, d_x.p, (int)ne, slu, slt, stream);
+        double* slu = olu + (size_t)K * (size_t)n3;
+        double* slt = olt + (size_t)K * (size_t)n3;
+        LaunchScatter(d_refs.p, d_geo.p, bd + (size_t)K * (size_t)n3, d_x.p, (int)ne, slu, slt, stream);
         LaunchState(d_refs.p, slu, slt, (int)ne, K, nstore, d_gU.p, d_gT.p, stream);
         if (keep)
             Check(hipMemcpyAsync(keep->X.p + (size_t)K * (size_t)n3, d_x.p, sizeof(double) * (size_t)n3,
@@ -1356,11 +1368,16 @@ void Case::Solve(int nstep, const double* known_lu, long restart, double tol, do
         std::swap(keep->bd.n, d_bd.n);
     }
     Check(hipGetLastError(), "solve launch");
-    Check(CopyOnStream(lu, d_lu.p, sizeof(double) * d_lu.n, hipMemcpyDeviceToHost, stream), "lu");
-    Check(CopyOnStream(lt, d_lt.p, sizeof(double) * d_lt.n, hipMemcpyDeviceToHost, stream), "lt");
+    if (deviceIO) {
+        LaunchStep0Out(d_refs.p, d_geo.p, bd, (int)ne, nstore, 1, olu, olt, stream);
+        Check(hipStreamSynchronize(stream), "solve");
+    } else {
+        Check(CopyOnStream(lu, d_lu.p, sizeof(double) * d_lu.n, hipMemcpyDeviceToHost, stream), "lu");
+        Check(CopyOnStream(lt, d_lt.p, sizeof(double) * d_lt.n, hipMemcpyDeviceToHost, stream), "lt");
+    }
     // step 0: lt = 0, Convert (lu <-> lt for BCID 456), then LocToAbs
     // (DBEM1/multidomain.cpp:3819-3822)
-    for (long i = 0; i < ne; ++i) {
+    for (long i = 0; i < ne && !deviceIO; ++i) {
         double u[3], t[3];
         for (int k = 0; k < 3; ++k) {
             u[k] = known_lu[3 * i + k];

@@ -94,12 +94,14 @@ public:
     TuneChoice Autotune(const std::string& cachePath);
     std::string TuneKey() const;
     void ReleaseHistory();
+    // deviceIO: known_lu, lu and lt are device pointers (inputs already in HBM,
+    // outputs left there); iters / flags / solved stay host arrays.
     void Solve(int nstep, const double* known_lu, long restart, double tol, double* lu, double* lt, int* iters,
-               int* flags, int* solved);
+               int* flags, int* solved, bool deviceIO = false);
     // B load cases over the same bands: known_lu, lu, lt are [b][NStep+1][3N],
     // iters / flags [b][NStep], solved [b].
     void SolveBatched(int nstep, int B, const double* known_lu, long restart, double tol, double* lu, double* lt,
-                      int* iters, int* flags, int* solved);
+                      int* iters, int* flags, int* solved, bool deviceIO = false);
     void HistoryStats(long* counts) const;
     // Reverse-time adjoint of the FP64 time loop for the misfit
     // J = sum_{K=1..NStep} sum_r |lu_K[r] - obs[K][r]|^2 over the receiver

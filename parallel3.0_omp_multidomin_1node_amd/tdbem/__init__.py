@@ -29,7 +29,8 @@ EXPORTED = [
     "tdbem_write_bd_bin", "tdbem_write_validation", "tdbem_history_save", "tdbem_history_load",
     "tdbem_run_models", "tdbem_release_cached_memory", "tdbem_autotune", "tdbem_dofmap_build",
     "tdbem_builder_create", "tdbem_builder_destroy", "tdbem_builder_add_block", "tdbem_builder_merge",
-    "tdbem_builder_info", "tdbem_builder_build", "tdbem_builder_build_sym", "tdbem_ccsr_export_rows",
+    "tdbem_builder_info", "tdbem_builder_build", "tdbem_builder_build_sym", "tdbem_ccsr_export_rows", "tdbem_solve_device", "tdbem_solve_batched_device",
+    "tdbem_device_alloc", "tdbem_device_free", "tdbem_memcpy",
 ]
 
 
@@ -87,6 +88,13 @@ def lib():
                                        _pp, _pp, _ip]
         L.tdbem_autotune.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                      C.POINTER(C.c_double)]
+        L.tdbem_solve_device.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_long, C.c_double, C.c_void_p,
+                                         C.c_void_p, _ip, _ip, C.POINTER(C.c_int), _dp]
+        L.tdbem_solve_batched_device.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_long, C.c_double,
+                                                 C.c_void_p, C.c_void_p, _ip, _ip, _ip, _dp]
+        L.tdbem_device_alloc.argtypes = [C.c_size_t, C.POINTER(C.c_void_p)]
+        L.tdbem_device_free.argtypes = [C.c_void_p]
+        L.tdbem_memcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
         L.tdbem_ccsr_export_rows.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_long, C.c_long, C.POINTER(C.c_long),
                                              C.c_void_p, C.c_void_p, C.c_void_p]
         L.tdbem_dofmap_build.argtypes = [C.c_long, _ip, C.c_int, _dp, C.c_int, _lp, _lp, _lp, _dp, _ip, _lp, _dp,
@@ -123,6 +131,43 @@ def dofmap_build(bcid, domain_count=0, mat_rows=None, itf_rows=None):
                                     arr["preferred"]), "tdbem_dofmap_build")
     arr["counts"] = counts
     return arr
+
+
+class DeviceArray:
+    """A float64 array in device memory (tdbem_device_alloc), for the
+    device-resident solves: upload() from numpy, download() to numpy."""
+
+    def __init__(self, shape):
+        self.shape = tuple(int(x) for x in np.atleast_1d(shape))
+        self.nbytes = int(np.prod(self.shape)) * 8
+        p = C.c_void_p()
+        _check(lib().tdbem_device_alloc(max(self.nbytes, 8), C.byref(p)), "tdbem_device_alloc")
+        self.ptr = p
+
+    @classmethod
+    def from_host(cls, a):
+        a = np.ascontiguousarray(a, np.float64)
+        d = cls(a.shape)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, np.float64)
+        if a.nbytes != self.nbytes:
+            raise ValueError("size mismatch")
+        _check(lib().tdbem_memcpy(self.ptr, a.ctypes.data, self.nbytes, 0), "tdbem_memcpy")
+
+    def download(self):
+        a = np.empty(self.shape)
+        _check(lib().tdbem_memcpy(a.ctypes.data, self.ptr, self.nbytes, 1), "tdbem_memcpy")
+        return a
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            lib().tdbem_device_free(self.ptr)
+            self.ptr = None
+
+    __del__ = close
 
 
 class Builder:
@@ -337,6 +382,29 @@ class Case:
         _check(lib().tdbem_dyna_gmres_solver(self.h, int(iterations), float(error), int(maxleaf), int(nstep), known,
                                              lu, lt, C.byref(result)), "tdbem_dyna_gmres_solver")
         return result.value, lu.reshape(nstep + 1, self.ne, 3), lt.reshape(nstep + 1, self.ne, 3)
+
+    def solve_device(self, d_known, nstep, d_lu, d_lt, restart=200, tol=1e-5):
+        """tdbem_solve_device: DeviceArray inputs / outputs of (nstep+1, ne, 3);
+        returns (iters, flags, solved, times)."""
+        iters = np.zeros(nstep, np.int32)
+        flags = np.zeros(nstep, np.int32)
+        solved = C.c_int()
+        times = np.zeros(3)
+        _check(lib().tdbem_solve_device(self.h, int(nstep), d_known.ptr, int(restart), float(tol), d_lu.ptr, d_lt.ptr,
+                                        iters, flags, C.byref(solved), times), "tdbem_solve_device")
+        return iters, flags, solved.value, times
+
+    def solve_batched_device(self, d_known, nrhs, nstep, d_lu, d_lt, restart=200, tol=1e-5):
+        """tdbem_solve_batched_device: DeviceArray inputs / outputs of
+        (nrhs, nstep+1, ne, 3); returns (iters, flags, solved, times)."""
+        iters = np.zeros(nrhs * nstep, np.int32)
+        flags = np.zeros(nrhs * nstep, np.int32)
+        solved = np.zeros(nrhs, np.int32)
+        times = np.zeros(3)
+        _check(lib().tdbem_solve_batched_device(self.h, int(nstep), int(nrhs), d_known.ptr, int(restart), float(tol),
+                                                d_lu.ptr, d_lt.ptr, iters, flags, solved, times),
+               "tdbem_solve_batched_device")
+        return iters.reshape(nrhs, nstep), flags.reshape(nrhs, nstep), solved, times
 
     def solve_batched(self, known_lu, nstep, restart=200, tol=1e-5):
         """known_lu: (nrhs, nstep+1, ne, 3) load histories -> lu, lt of the same

@@ -114,3 +114,33 @@ def test_history_file_refused_after_the_cull_drops_the_bands(tmp_path):
     U, *_ = b.solve(w.known, w.nstep, tol=1e-10)
     assert np.isfinite(U).all()
     b.close()
+
+
+def test_device_resident_solves_equal_the_host_buffer_solves():
+    """tdbem_solve_device / tdbem_solve_batched_device (inputs already in
+    device memory, outputs left there) give the host-buffer calls' results
+    bit for bit, step 0 and a GMRES-stopped case included."""
+    tdbem = _tdbem()
+    from tdbem import mesh
+    w = mesh.layered_config2(h=0.5, nstep=20)
+    m = w.mesh
+    print("case:", w.name, "device-resident solves", flush=True)
+    c = tdbem.Case(m.points, m.ele, m.bcid, w.dt, w.gap, domain_count=4, mat_rows=w.mats, itf_rows=m.interfaces)
+    c.assemble_step0(10)
+    c.assemble_history(w.nstep)
+    U, T, it, fl, solved, _ = c.solve(w.known, w.nstep, tol=1e-10)
+    dk = tdbem.DeviceArray.from_host(w.known)
+    du, dt_ = tdbem.DeviceArray(w.known.shape), tdbem.DeviceArray(w.known.shape)
+    it2, fl2, solved2, _ = c.solve_device(dk, w.nstep, du, dt_, tol=1e-10)
+    assert solved2 == solved and np.array_equal(it2, it) and np.array_equal(fl2, fl)
+    assert np.array_equal(du.download(), U) and np.array_equal(dt_.download(), T)
+    cases = mesh.load_cases(w, 5)
+    dk = tdbem.DeviceArray.from_host(cases)
+    du, dt_ = tdbem.DeviceArray(cases.shape), tdbem.DeviceArray(cases.shape)
+    for restart, tol in ((200, 1e-10), (1, 1e-12)):   # the second stops every case at step 1
+        BU, BT, bit, bfl, bsolved, _ = c.solve_batched(cases, w.nstep, restart=restart, tol=tol)
+        bit2, bfl2, bsolved2, _ = c.solve_batched_device(dk, 5, w.nstep, du, dt_, restart=restart, tol=tol)
+        assert np.array_equal(bsolved2, bsolved) and np.array_equal(bit2, bit) and np.array_equal(bfl2, bfl)
+        assert np.array_equal(du.download(), BU) and np.array_equal(dt_.download(), BT)
+    assert (bsolved == 1).all()
+    c.close()

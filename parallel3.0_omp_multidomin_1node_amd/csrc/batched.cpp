@@ -188,7 +188,19 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
     std::vector<char> alive((size_t)B, 1);
     for (int b = 0; b < B; ++b)
         solved[b] = 0;
-    double convTime = 0.0, gmresTime = 0.0;
+    double convTime = 0.0, gmresTime = 0.0, farTime = 0.0;
+    int farLaunches = 0;
+    hipEvent_t fa = nullptr, fb = nullptr;
+    CheckB(hipEventCreate(&fa), "event");
+    CheckB(hipEventCreate(&fb), "event");
+    struct EvGuard {
+        hipEvent_t a, b;
+        ~EvGuard()
+        {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+        }
+    } evg{fa, fb};
     int K0 = 0;
     for (int K = 1; K <= nstep; ++K) {
         LaunchKnownB(d_refs.p, bd, (int)ne, nstore, K, B, d_known.p, stream);
@@ -202,6 +214,7 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
                 LaunchFilterB(d_gU.p, histU, nstore, B, x0, K0, K0, d_FU.p, stream);
                 LaunchFilterB(d_gT.p, histT, nstore, B, x0, K0, K0, d_FT.p, stream);
                 int Tn = std::min(kBatchTile, nstep - K0 + 1);
+                CheckB(hipEventRecord(fa, stream), "event");
                 if (historyMixed)
                     LaunchConvFarB(grid, d_refs.p, d_off.p, d_kb.p, d_len.p, (const float*)d_mts32.p,
                                    (const float*)d_mgs32.p, mLo, gLo, d_FU.p, d_FT.p, nstore, K0, Tn, maxn, (int)ne, B,
@@ -210,6 +223,12 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
                     LaunchConvFarB(grid, d_refs.p, d_off.p, d_kb.p, d_len.p, (const double*)d_mts.p,
                                    (const double*)d_mgs.p, nullptr, nullptr, d_FU.p, d_FT.p, nstore, K0, Tn, maxn,
                                    (int)ne, B, d_hfar.p, farParts, stream);
+                CheckB(hipEventRecord(fb, stream), "event");
+                CheckB(hipEventSynchronize(fb), "far part");
+                float ms = 0.f;
+                CheckB(hipEventElapsedTime(&ms, fa, fb), "elapsed");
+                farTime += ms * 1e-3;
+                ++farLaunches;
             }
             if (historyMixed)
                 LaunchConvNearB(d_refs.p, d_nearPtr.p, d_nearPair.p, d_nearField.p, d_off.p, d_kb.p, d_len.p,
@@ -256,8 +275,8 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
         CheckB(hipGetLastError(), "batched solve launch");
         timing[4] = convTime;
         timing[5] = gmresTime;
-        timing[6] = 0.0;
-        timing[7] = 0.0;
+        timing[6] = farTime;
+        timing[7] = farLaunches;
         bw.release();
         return;
     }
@@ -294,8 +313,8 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
     }
     timing[4] = convTime;
     timing[5] = gmresTime;
-    timing[6] = 0.0;
-    timing[7] = 0.0;
+    timing[6] = farTime;
+    timing[7] = farLaunches;
     (void)t0;
     bw.release();
 }

@@ -154,7 +154,6 @@ def cpu_baseline(threads, gpu_tdbem, tol):
     workload family: the stacked 2x2x8 box at h = 0.5 (512 elements, 8
     domains), 60 steps, one load case, whole job; with `threads` threads and
     with one, and the GPU's FP64 whole job on the same sample."""
-    from oracle import oracle as O
     from tdbem import mesh
     w = mesh.mixed_config3(h=0.5, nstep=60)
     known = mesh.load_cases(w, 1)[0]

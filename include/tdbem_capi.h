@@ -163,6 +163,19 @@ int tdbem_solve_device(tdbem_ctx* ctx, int nstep, const double* d_known_lu, long
 int tdbem_solve_batched_device(tdbem_ctx* ctx, int nstep, int nrhs, const double* d_known_lu, long restart,
                                double tol, double* d_lu, double* d_lt, int* iters, int* flags, int* solved,
                                double* times);
+/* tdbem_solve_device without any device allocation (no counterpart in the
+ * reference): for callers that cannot allocate, such as code under HIP graph
+ * capture and tight serving loops.  Every working buffer of the time loop
+ * (state histories, right-hand sides, the GMRES workspace) is carved from a
+ * file-scope device array of 64 MiB per GPU, held until the process ends; the
+ * call makes no hipMalloc / hipFree, creates no event and no graph (GMRES is
+ * device-driven with direct launches).  Needs FP64 history bands assembled
+ * (mixed 0) and the time-blocked convolution; a shape whose buffers exceed
+ * the array is refused with an error naming the bytes it needs.  Calls that
+ * overlap on one GPU share the array and race; the array keeps the last
+ * call's contents (the solve clears what it needs). */
+int tdbem_solve_device_noalloc(tdbem_ctx* ctx, int nstep, const double* d_known_lu, long restart, double tol,
+                               double* d_lu, double* d_lt, int* iters, int* flags, int* solved);
 /* Device memory for the calls above: allocate, free, and copy (kind 0 host to
  * device, 1 device to host, 2 device to device; synchronous). */
 int tdbem_device_alloc(size_t bytes, void** out);

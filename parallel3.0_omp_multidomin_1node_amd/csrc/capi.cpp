@@ -494,6 +494,20 @@ int tdbem_solve_batched_device(tdbem_ctx* ctx, int nstep, int nrhs, const double
     return 0;
 }
 
+int tdbem_solve_device_noalloc(tdbem_ctx* ctx, int nstep, const double* d_known_lu, long restart, double tol,
+                               double* d_lu, double* d_lt, int* iters, int* flags, int* solved)
+{
+    if (!ctx || nstep < 1 || !d_known_lu || !d_lu || !d_lt || !iters || !flags || !solved || restart < 1 ||
+        !(tol > 0.0))
+        return Fail("tdbem_solve_device_noalloc: invalid arguments");
+    try {
+        ctx->c.SolveNoAlloc(nstep, d_known_lu, restart, tol, d_lu, d_lt, iters, flags, solved);
+    } catch (const std::exception& e) {
+        return Fail("tdbem_solve_device_noalloc", e);
+    }
+    return 0;
+}
+
 int tdbem_device_alloc(size_t bytes, void** out)
 {
     if (!out || bytes == 0)

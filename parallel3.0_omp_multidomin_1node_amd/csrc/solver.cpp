@@ -151,6 +151,12 @@ void ReleaseCachedDeviceMemory()
     FlushPoolLocked();
 }
 
+namespace {
+thread_local ScratchArena* g_arena = nullptr;
+}
+
+void SetScratchArena(ScratchArena* a) { g_arena = a; }
+
 template <typename T>
 void DevBuf<T>::alloc(size_t count)
 {
@@ -158,6 +164,16 @@ void DevBuf<T>::alloc(size_t count)
     if (count == 0)
         count = 1;
     size_t bytes = count * sizeof(T);
+    if (g_arena) {  // carve from the caller's device memory; nothing is allocated
+        size_t at = (g_arena->used + 255) & ~(size_t)255;
+        if (at + bytes > g_arena->size)
+            throw std::runtime_error("scratch of " + std::to_string(g_arena->size) + " bytes exhausted (" +
+                                     std::to_string(at + bytes) + " needed)");
+        p = reinterpret_cast<T*>(g_arena->base + at);
+        g_arena->used = at + bytes;
+        n = count;
+        return;
+    }
     if (bytes >= kPoolMin)
         p = (T*)PoolAcquire(bytes);
     else
@@ -168,6 +184,11 @@ void DevBuf<T>::alloc(size_t count)
 template <typename T>
 void DevBuf<T>::free()
 {
+    if (p && g_arena && (char*)p >= g_arena->base && (char*)p < g_arena->base + g_arena->size) {
+        p = nullptr;  // a view into the arena: nothing to return
+        n = 0;
+        return;
+    }
     if (p && !PoolRelease(p))
         (void)hipFree(p);
     p = nullptr;
@@ -321,6 +342,14 @@ void Case::Create(long npoints, const double* points, long nele, const int* ele_
     d_geo.upload(geo.data(), geo.size(), stream);
     d_dom.upload(domOf.data(), domOf.size(), stream);
     d_refs.upload(refs.data(), refs.size(), stream);
+    (void)ScratchOnDevice();  // the no-allocation solve's scratch address on this device
+    // GMRES host-visible state, set up once per context (no solve allocates it)
+    if (!h_pin)
+        Check(hipHostMalloc((void**)&h_pin, 4 * sizeof(double), hipHostMallocDefault), "pinned");
+    if (!d_gst.p) {
+        d_gst.alloc((sizeof(GmState) + sizeof(double) - 1) / sizeof(double));
+        Check(hipHostMalloc((void**)&h_gst, sizeof(GmState), hipHostMallocDefault), "pinned");
+    }
     d_pts.alloc((size_t)ne);
     d_u1.alloc(9 * (size_t)ne);
     d_t1.alloc(9 * (size_t)ne);
@@ -1230,7 +1259,8 @@ void Case::Solve(int nstep, const double* known_lu, long restart, double tol, do
     if (!historyDone || maxn < 1)
         throw std::runtime_error("history not assembled");
     PrepareGmres(restart);
-    ReleaseGraphs();  // captured per solve: the operands may have moved since the last one
+    if (!noAllocMode)
+        ReleaseGraphs();  // captured per solve: the operands may have moved since the last one
     gm_lastIters = 1;
     if (historyMixed)
         PrepareMixed(restart);
@@ -1290,15 +1320,22 @@ void Case::Solve(int nstep, const double* known_lu, long restart, double tol, do
     for (int K = 1; K <= nstep; ++K) {
         LaunchKnown(d_refs.p, bd + (size_t)K * (size_t)n3, (int)ne, d_known.p, dof.knownBlockCount, stream);
         LaunchSpmv(ne, d_Krp.p, d_Kcol.p, d_Kval.p, d_known.p, d_rhs.p, stream);
-        std::array<hipEvent_t, 3> se{ev.Get(), ev.Get(), ev.Get()};
-        Check(hipEventRecord(se[0], stream), "event");
+        // no events without allocation: the no-allocation variant keeps no timings
+        std::array<hipEvent_t, 3> se{};
+        if (!noAllocMode) {
+            se = {ev.Get(), ev.Get(), ev.Get()};
+            Check(hipEventRecord(se[0], stream), "event");
+        }
         if (K > 1) {
             if (convTiled) {
                 if (K == 2 || K - K0 >= convTile) {
                     K0 = K;
                     int Tn = std::min(convTile, nstep - K0 + 1);
-                    std::array<hipEvent_t, 2> fe{ev.Get(), ev.Get()};
-                    Check(hipEventRecord(fe[0], stream), "event");
+                    std::array<hipEvent_t, 2> fe{};
+                    if (!noAllocMode) {
+                        fe = {ev.Get(), ev.Get()};
+                        Check(hipEventRecord(fe[0], stream), "event");
+                    }
                     if (farF) {
                         LaunchFilterTile(d_refs.p, d_gU.p, d_gT.p, nstore, (int)ne, K0, std::max(0, K0 - convTile - 1),
                                          K0, d_F.p, stream);
@@ -1320,8 +1357,10 @@ void Case::Solve(int nstep, const double* known_lu, long restart, double tol, do
                         LaunchConvFar(grid, d_refs.p, d_off.p, d_kb.p, d_len.p, (const double*)d_mts.p,
                                       (const double*)d_mgs.p, nullptr, nullptr, d_gU.p, d_gT.p, nstore, K0, Tn,
                                       convTile, maxn, (int)ne, d_hfar.p, stream);
-                    Check(hipEventRecord(fe[1], stream), "event");
-                    farEv.push_back(fe);
+                    if (!noAllocMode) {
+                        Check(hipEventRecord(fe[1], stream), "event");
+                        farEv.push_back(fe);
+                    }
                 }
                 if (historyMixed)
                     LaunchConvNear(d_refs.p, d_nearPtr.p, d_nearPair.p, d_nearField.p, d_off.p, d_kb.p, d_len.p,
@@ -1337,13 +1376,16 @@ void Case::Solve(int nstep, const double* known_lu, long restart, double tol, do
                                   K, maxn, (int)ne, d_rhs.p, stream);
             }
         }
-        Check(hipEventRecord(se[1], stream), "event");
+        if (!noAllocMode)
+            Check(hipEventRecord(se[1], stream), "event");
         int it = 0;
         int flag = historyMixed ? GmresMixed(d_rhs.p, d_x.p, restart, tol, &it)
                    : gmresDevice && restart <= kGmDeviceMaxRestart ? GmresDevice(d_rhs.p, d_x.p, restart, tol, &it)
                                                                    : Gmres(d_rhs.p, d_x.p, restart, tol, &it);
-        Check(hipEventRecord(se[2], stream), "event");
-        stepEv.push_back(se);
+        if (!noAllocMode) {
+            Check(hipEventRecord(se[2], stream), "event");
+            stepEv.push_back(se);
+        }
         iters[K - 1] = it;
         flags[K - 1] = flag;
         *solved = K;

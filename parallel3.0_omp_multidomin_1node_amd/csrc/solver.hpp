@@ -26,6 +26,20 @@ inline hipError_t CopyOnStream(void* dst, const void* src, size_t bytes, hipMemc
     return hipStreamSynchronize(s);
 }
 
+// Caller-provided device memory that DevBuf::alloc carves from (256-byte
+// aligned bump allocation) while it is set on the calling thread; DevBuf::free
+// of memory inside it only forgets the pointer.
+struct ScratchArena {
+    char* base;
+    size_t size;
+    size_t used;
+};
+void SetScratchArena(ScratchArena* a);  // nullptr ends it
+// Address of the no-allocation solve's scratch array on the current device
+// (looked up once per device) and its size.
+void* ScratchOnDevice();
+size_t ScratchBytes();
+
 template <typename T>
 struct DevBuf {
     T* p = nullptr;
@@ -98,6 +112,11 @@ public:
     // outputs left there); iters / flags / solved stay host arrays.
     void Solve(int nstep, const double* known_lu, long restart, double tol, double* lu, double* lt, int* iters,
                int* flags, int* solved, bool deviceIO = false);
+    // Solve with device-resident I/O whose every device buffer is carved from
+    // the file-scope scratch array of noalloc.hip (no allocation, no event, no
+    // graph); FP64 bands only.  Sizes above the array are refused.
+    void SolveNoAlloc(int nstep, const double* d_known, long restart, double tol, double* d_lu, double* d_lt,
+                      int* iters, int* flags, int* solved);
     // B load cases over the same bands: known_lu, lu, lt are [b][NStep+1][3N],
     // iters / flags [b][NStep], solved [b].
     void SolveBatched(int nstep, int B, const double* known_lu, long restart, double tol, double* lu, double* lt,
@@ -162,6 +181,7 @@ public:
     bool batchedMfma = false;  // batched far part on the FP64 matrix cores (false: vector FMAs)
     int batchedFarParts = 16;  // column parts (partial sums) of the matrix-core batched far part
     int gmBlocks = 32;         // reduction blocks of the GMRES dot products (at most; <= kGmBlocksMax)
+    bool noAllocMode = false;  // SolveNoAlloc in progress
     int gmresDevice = 1;       // FP64 GMRES: 0 host-driven, 1 device-driven hipGraphs, 2 device-driven launches
     int selfMode = -1;       // kernel of the current self tables: 1 wave per element, 0 reference order
     bool quadStaged = true;  // wave-per-pair quadrature with LDS-staged point geometry (false: lane per entry)

@@ -30,7 +30,7 @@ EXPORTED = [
     "tdbem_run_models", "tdbem_release_cached_memory", "tdbem_autotune", "tdbem_dofmap_build",
     "tdbem_builder_create", "tdbem_builder_destroy", "tdbem_builder_add_block", "tdbem_builder_merge",
     "tdbem_builder_info", "tdbem_builder_build", "tdbem_builder_build_sym", "tdbem_ccsr_export_rows", "tdbem_solve_device", "tdbem_solve_batched_device",
-    "tdbem_device_alloc", "tdbem_device_free", "tdbem_memcpy",
+    "tdbem_device_alloc", "tdbem_device_free", "tdbem_memcpy", "tdbem_solve_device_noalloc",
 ]
 
 
@@ -92,6 +92,8 @@ def lib():
                                          C.c_void_p, _ip, _ip, C.POINTER(C.c_int), _dp]
         L.tdbem_solve_batched_device.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_long, C.c_double,
                                                  C.c_void_p, C.c_void_p, _ip, _ip, _ip, _dp]
+        L.tdbem_solve_device_noalloc.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_long, C.c_double, C.c_void_p,
+                                                 C.c_void_p, _ip, _ip, C.POINTER(C.c_int)]
         L.tdbem_device_alloc.argtypes = [C.c_size_t, C.POINTER(C.c_void_p)]
         L.tdbem_device_free.argtypes = [C.c_void_p]
         L.tdbem_memcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
@@ -393,6 +395,17 @@ class Case:
         _check(lib().tdbem_solve_device(self.h, int(nstep), d_known.ptr, int(restart), float(tol), d_lu.ptr, d_lt.ptr,
                                         iters, flags, C.byref(solved), times), "tdbem_solve_device")
         return iters, flags, solved.value, times
+
+    def solve_device_noalloc(self, d_known, nstep, d_lu, d_lt, restart=200, tol=1e-5):
+        """tdbem_solve_device_noalloc: as solve_device, with every working
+        buffer in the library's fixed scratch array; returns (iters, flags,
+        solved)."""
+        iters = np.zeros(nstep, np.int32)
+        flags = np.zeros(nstep, np.int32)
+        solved = C.c_int()
+        _check(lib().tdbem_solve_device_noalloc(self.h, int(nstep), d_known.ptr, int(restart), float(tol), d_lu.ptr,
+                                                d_lt.ptr, iters, flags, C.byref(solved)), "tdbem_solve_device_noalloc")
+        return iters, flags, solved.value
 
     def solve_batched_device(self, d_known, nrhs, nstep, d_lu, d_lt, restart=200, tol=1e-5):
         """tdbem_solve_batched_device: DeviceArray inputs / outputs of

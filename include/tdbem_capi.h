@@ -212,8 +212,13 @@ int tdbem_dyna_gmres_solver(tdbem_ctx* ctx, long iterations, double error, long 
  * (sum over steps), t[6] far-part convolution kernels, t[7] their count. */
 int tdbem_timings(tdbem_ctx* ctx, double* t8);
 /* The same timings and more, n values: t[8] the history quadrature kernels
- * alone (device events; t[2] also holds the cull and the wave packing).
- * Values past the known ones are 0. */
+ * alone (device events; t[2] also holds the cull and the wave packing);
+ * t[9..12] the adjoint's parts (forward, backward, step-0 differences,
+ * history derivatives); for tdbem_solve_batched t[13] the spectral far
+ * part's kernels (window spectra and products), t[14] their launches and
+ * t[15] 1 when the spectral form ran (t[6], t[7] then cover the lag-by-lag
+ * far kernel of the lags <= 16) and t[16] its set-up (item lists and filter
+ * spectra, once per solve).  Values past the known ones are 0. */
 int tdbem_timings_n(tdbem_ctx* ctx, int n, double* t);
 
 /* Number of stored history band entries per lag j = 1..maxn

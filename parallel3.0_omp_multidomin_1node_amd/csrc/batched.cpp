@@ -133,11 +133,97 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
         AssembleStep0(10);
     if (!historyDone || maxn < 1)
         throw std::runtime_error("history not assembled");
+    // far part: the lag-by-lag kernels (8-step tiles) take every lag, or, in
+    // the spectral form (panels of 16 or more load cases unless the option
+    // says otherwise, and when its filter spectra fit the device), the raw
+    // lags <= 16 while batched_fft.hip takes the rest over 16-step tiles
     if (nearTile != kBatchTile)
         BuildNearLists(kBatchTile);
+    bool fft = batchedFft > 0 || (batchedFft < 0 && B >= 16);
     const long n = 3 * ne;
     const int nstore = nstep + 1;
     const long histU = dof.historyUBlockCount, histT = dof.historyTBlockCount;
+    // spectral part: row groups, their (field, sub-filter) items and filter
+    // spectra, and the window spectra (a ring over the longest band's sub-filters)
+    DevBuf<int> d_grpDom, d_grpRow0, d_grpRows;
+    DevBuf<long> d_itemOff, d_hOff, d_items, d_cnt;
+    DevBuf<double> d_H, d_WU, d_WT, d_Y, d_hfar16;
+    int ngroups = 0, nwr = 1;
+    long strideU = 0, strideT = 0;
+    timing[16] = 0.0;
+    if (fft) {
+        const double fs0 = NowB();
+        std::vector<int> gd, g0, gr;
+        for (int dd = 0; dd < grid.nd; ++dd)
+            for (int r0 = 0; r0 < grid.eleCount[dd]; r0 += kFftRows) {
+                gd.push_back(dd);
+                g0.push_back(grid.eleBegin[dd] + r0);
+                gr.push_back(std::min(kFftRows, grid.eleCount[dd] - r0));
+            }
+        ngroups = (int)gd.size();
+        d_grpDom.upload(gd.data(), gd.size(), stream);
+        d_grpRow0.upload(g0.data(), g0.size(), stream);
+        d_grpRows.upload(gr.data(), gr.size(), stream);
+        d_cnt.alloc(2 * (size_t)ngroups);
+        LaunchFftItems(grid, d_refs.p, d_kb.p, d_len.p, maxn, ngroups, d_grpDom.p, d_grpRow0.p, d_grpRows.p, nullptr,
+                       nullptr, d_cnt.p, d_cnt.p + ngroups, nullptr, stream);
+        std::vector<long> cnt(2 * (size_t)ngroups), ioff((size_t)ngroups + 1, 0), hoff((size_t)ngroups + 1, 0);
+        CheckB(CopyOnStream(cnt.data(), d_cnt.p, sizeof(long) * cnt.size(), hipMemcpyDeviceToHost, stream), "items");
+        for (int gi = 0; gi < ngroups; ++gi) {
+            ioff[(size_t)gi + 1] = ioff[(size_t)gi] + cnt[(size_t)gi];
+            hoff[(size_t)gi + 1] = hoff[(size_t)gi] + cnt[(size_t)ngroups + gi] * 16 * 30;
+        }
+        const long nitems = ioff.back(), hvals = hoff.back();
+        HostBands();
+        int qmax = 1;
+        for (long pp = 0; pp < npairs; ++pp) {
+            const int L = h_len[(size_t)pp], k0 = h_kb[(size_t)pp];
+            const int jhi = std::min(k0 + L - 1, maxn);
+            if (L > 0 && std::max(k0, kFftLagMin) <= jhi)
+                qmax = std::max(qmax, (jhi + 1) / kFftTile);
+        }
+        nwr = qmax;
+        strideU = histU * 96 * (long)B;
+        strideT = histT * 96 * (long)B;
+        // the spectral part's buffers, and what the time loop allocates after them
+        const double nB = (double)n * B, ns = (double)nstep + 1;
+        const double bytes = 8.0 * ((double)hvals + (double)(strideU + strideT) * nwr + 16.0 * nB + (double)ne * 96 * B) +
+                             32.0 * (double)nitems;
+        const double rest = 8.0 * (nB * (2.0 * restart + 6) + (double)(histU + histT) * 3 * B * (2 * ns + 1) +
+                                   3.0 * std::max<long>(1, dof.knownBlockCount) * B +
+                                   std::max(1, batchedMfma ? batchedFarParts : 0) * kBatchTile * nB +
+                                   (deviceIO ? 0.0 : 3.0 * B * ns * n));
+        size_t freeB = 0, totalB = 0;
+        CheckB(hipMemGetInfo(&freeB, &totalB), "hipMemGetInfo");
+        freeB += PooledBytesCached();
+        if (bytes + rest > 0.92 * (double)freeB) {
+            fft = false;  // the filter spectra do not fit: every lag lag-by-lag
+        } else {
+            d_itemOff.upload(ioff.data(), ioff.size(), stream);
+            d_hOff.upload(hoff.data(), hoff.size(), stream);
+            d_items.alloc(4 * (size_t)std::max<long>(1, nitems));  // 32 bytes per item
+            FftItem* items = reinterpret_cast<FftItem*>(d_items.p);
+            LaunchFftItems(grid, d_refs.p, d_kb.p, d_len.p, maxn, ngroups, d_grpDom.p, d_grpRow0.p, d_grpRows.p,
+                           d_itemOff.p, d_hOff.p, nullptr, nullptr, items, stream);
+            d_H.alloc((size_t)hvals + kFftHPad);
+            LaunchZero(d_H.p + hvals, kFftHPad, stream);
+            if (historyMixed)
+                LaunchFftFilters(grid, d_off.p, d_kb.p, d_len.p, (const float*)d_mts32.p, (const float*)d_mgs32.p,
+                                 historySplit ? d_mtsLo.p : nullptr, historySplit ? d_mgsLo.p : nullptr, ngroups,
+                                 d_grpDom.p, d_grpRow0.p, d_itemOff.p, nitems, items, maxn, d_H.p, stream);
+            else
+                LaunchFftFilters(grid, d_off.p, d_kb.p, d_len.p, (const double*)d_mts.p, (const double*)d_mgs.p,
+                                 nullptr, nullptr, ngroups, d_grpDom.p, d_grpRow0.p, d_itemOff.p, nitems, items, maxn,
+                                 d_H.p, stream);
+            d_WU.alloc((size_t)strideU * nwr);
+            d_WT.alloc((size_t)strideT * nwr);
+            d_Y.alloc((size_t)ne * 96 * B);
+            d_hfar16.alloc((size_t)kFftTile * (size_t)n * B);
+            CheckB(hipStreamSynchronize(stream), "spectral set-up");
+        }
+        timing[16] = NowB() - fs0;
+    }
+    const int lagMax = fft ? std::min(maxn, kFftLagMin - 1) : maxn;  // the lag-by-lag kernels' last lag
     double t0 = NowB();
     BatchWork& w = bw;
     w.V.alloc((size_t)n * (size_t)(restart + 1) * (size_t)B);
@@ -188,8 +274,8 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
     std::vector<char> alive((size_t)B, 1);
     for (int b = 0; b < B; ++b)
         solved[b] = 0;
-    double convTime = 0.0, gmresTime = 0.0, farTime = 0.0;
-    int farLaunches = 0;
+    double convTime = 0.0, gmresTime = 0.0, farTime = 0.0, fftTime = 0.0;
+    int farLaunches = 0, fftLaunches = 0, K016 = 0;
     hipEvent_t fa = nullptr, fb = nullptr;
     CheckB(hipEventCreate(&fa), "event");
     CheckB(hipEventCreate(&fb), "event");
@@ -207,6 +293,24 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
         LaunchSpmmB(ne, d_Krp.p, d_Kcol.p, d_Kval.p, d_known.p, d_rhs.p, B, stream);
         if (K > 1) {
             double c0 = NowB();
+            if (fft && (K == 2 || K - K016 >= kFftTile)) {
+                K016 = K;
+                const int Tn = std::min(kFftTile, nstep - K016 + 1);
+                const int t = (K016 - 2) / kFftTile;
+                CheckB(hipEventRecord(fa, stream), "event");
+                LaunchFftWindows(d_gU.p, histU, nstore, B, t, false, d_WU.p + (size_t)(t % nwr) * strideU, stream);
+                LaunchFftWindows(d_gT.p, histT, nstore, B, t, false, d_WT.p + (size_t)(t % nwr) * strideT, stream);
+                LaunchConvFarFft(ngroups, d_grpRow0.p, d_grpRows.p, d_itemOff.p,
+                                 reinterpret_cast<const FftItem*>(d_items.p), d_H.p, d_WU.p, d_WT.p, strideU, strideT,
+                                 nwr, t, B, d_Y.p, stream);
+                LaunchFftInverse(d_Y.p, (int)ne, B, Tn, d_hfar16.p, stream);
+                CheckB(hipEventRecord(fb, stream), "event");
+                CheckB(hipEventSynchronize(fb), "far part");
+                float ms = 0.f;
+                CheckB(hipEventElapsedTime(&ms, fa, fb), "elapsed");
+                fftTime += ms * 1e-3;
+                ++fftLaunches;
+            }
             if (K == 2 || K - K0 >= kBatchTile) {
                 int prev = K0;
                 K0 = K;
@@ -217,11 +321,11 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
                 CheckB(hipEventRecord(fa, stream), "event");
                 if (historyMixed)
                     LaunchConvFarB(grid, d_refs.p, d_off.p, d_kb.p, d_len.p, (const float*)d_mts32.p,
-                                   (const float*)d_mgs32.p, mLo, gLo, d_FU.p, d_FT.p, nstore, K0, Tn, maxn, (int)ne, B,
-                                   d_hfar.p, farParts, stream);
+                                   (const float*)d_mgs32.p, mLo, gLo, d_FU.p, d_FT.p, nstore, K0, Tn, lagMax, (int)ne,
+                                   B, d_hfar.p, farParts, stream);
                 else
                     LaunchConvFarB(grid, d_refs.p, d_off.p, d_kb.p, d_len.p, (const double*)d_mts.p,
-                                   (const double*)d_mgs.p, nullptr, nullptr, d_FU.p, d_FT.p, nstore, K0, Tn, maxn,
+                                   (const double*)d_mgs.p, nullptr, nullptr, d_FU.p, d_FT.p, nstore, K0, Tn, lagMax,
                                    (int)ne, B, d_hfar.p, farParts, stream);
                 CheckB(hipEventRecord(fb, stream), "event");
                 CheckB(hipEventSynchronize(fb), "far part");
@@ -233,11 +337,13 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
             if (historyMixed)
                 LaunchConvNearB(d_refs.p, d_nearPtr.p, d_nearPair.p, d_nearField.p, d_off.p, d_kb.p, d_len.p,
                                 (const float*)d_mts32.p, (const float*)d_mgs32.p, mLo, gLo, d_gU.p, d_gT.p, nstore, K,
-                                K0, maxn, (int)ne, B, d_hfar.p, farParts, d_rhs.p, stream);
+                                K0, lagMax, (int)ne, B, d_hfar.p, farParts, d_rhs.p, stream,
+                                fft ? d_hfar16.p : nullptr, K016);
             else
                 LaunchConvNearB(d_refs.p, d_nearPtr.p, d_nearPair.p, d_nearField.p, d_off.p, d_kb.p, d_len.p,
                                 (const double*)d_mts.p, (const double*)d_mgs.p, nullptr, nullptr, d_gU.p, d_gT.p,
-                                nstore, K, K0, maxn, (int)ne, B, d_hfar.p, farParts, d_rhs.p, stream);
+                                nstore, K, K0, lagMax, (int)ne, B, d_hfar.p, farParts, d_rhs.p, stream,
+                                fft ? d_hfar16.p : nullptr, K016);
             CheckB(hipStreamSynchronize(stream), "batched convolution");
             convTime += NowB() - c0;
         }
@@ -277,6 +383,9 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
         timing[5] = gmresTime;
         timing[6] = farTime;
         timing[7] = farLaunches;
+        timing[13] = fftTime;
+        timing[14] = fftLaunches;
+        timing[15] = fft ? 1.0 : 0.0;
         bw.release();
         return;
     }
@@ -315,6 +424,9 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
     timing[5] = gmresTime;
     timing[6] = farTime;
     timing[7] = farLaunches;
+    timing[13] = fftTime;
+    timing[14] = fftLaunches;
+    timing[15] = fft ? 1.0 : 0.0;
     (void)t0;
     bw.release();
 }

@@ -653,7 +653,8 @@ __global__ void __launch_bounds__(256) k_conv_near_b(const NodeRefs* __restrict_
                                                     const double* __restrict__ gU, const double* __restrict__ gT,
                                                     int nstore, int K, int K0, int maxn, int ne, int B,
                                                     const double* __restrict__ hfar, int parts, long partStride,
-                                                    double* __restrict__ rhs)
+                                                    double* __restrict__ rhs, const double* __restrict__ hfar2,
+                                                    int K02)
 {
     __shared__ double red[4][3][64];
     int s = blockIdx.x;
@@ -712,6 +713,8 @@ __global__ void __launch_bounds__(256) k_conv_near_b(const NodeRefs* __restrict_
         double v = hf[(long)c * B];
         for (int p = 1; p < parts; ++p)
             v += hf[p * partStride + (long)c * B];
+        if (hfar2)
+            v += hfar2[((long)(K - K02) * 3 * ne + 3 * (long)s + c) * B + b];
         rhs[(3 * (long)s + c) * B + b] += v + y[c];
     }
 }
@@ -721,12 +724,12 @@ void LaunchConvNearB(const NodeRefs* refs, const long* nearPtr, const long* near
                      const long* off, const int* kb, const int* len, const TB* mts, const TB* mgs,
                      const unsigned short* mtsLo, const unsigned short* mgsLo, const double* gU, const double* gT,
                      int nstore, int K, int K0, int maxn, int ne, int B, const double* hfar, int parts,
-                     double* rhs, hipStream_t s)
+                     double* rhs, hipStream_t s, const double* hfar2, int K02)
 {
     dim3 g((unsigned)ne, (unsigned)((B + 63) / 64));
     k_conv_near_b<TB><<<g, 256, 0, s>>>(refs, nearPtr, nearPair, nearField, off, kb, len, mts, mgs, mtsLo, mgsLo, gU,
                                        gT, nstore, K, K0, maxn, ne, B, hfar, parts < 1 ? 1 : parts,
-                                       (long)kBatchTile * 3 * ne * B, rhs);
+                                       (long)kBatchTile * 3 * ne * B, rhs, hfar2, K02);
 }
 
 #define TDBEM_BATCH_INSTANTIATE(TB)                                                                                  \
@@ -736,7 +739,7 @@ void LaunchConvNearB(const NodeRefs* refs, const long* nearPtr, const long* near
     template void LaunchConvNearB<TB>(const NodeRefs*, const long*, const long*, const int*, const long*, const int*, \
                                       const int*, const TB*, const TB*, const unsigned short*,                        \
                                       const unsigned short*, const double*, const double*, int, int, int, int, int,   \
-                                      int, const double*, int, double*, hipStream_t);
+                                      int, const double*, int, double*, hipStream_t, const double*, int);
 TDBEM_BATCH_INSTANTIATE(double)
 TDBEM_BATCH_INSTANTIATE(float)
 #undef TDBEM_BATCH_INSTANTIATE

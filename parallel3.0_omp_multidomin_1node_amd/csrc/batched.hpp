@@ -12,6 +12,10 @@ namespace tdbem {
 constexpr int kBatchTile = 8;  // steps per far-part tile of the batched convolution
 constexpr int kBatchFarPartsMax = 16;  // column parts of the matrix-core far part
 constexpr int kBatchRedBlocks = 256;   // most reduction blocks of the batched dot products
+constexpr int kFftTile = 16;           // steps per tile (and lags per sub-filter) of the spectral far part
+constexpr int kFftRows = 64;           // source rows per group of the spectral far part (8 per wavefront)
+constexpr int kFftHPad = 256;          // values past the filter spectra that the products kernel may read
+constexpr int kFftLagMin = 17;         // the spectral part takes the raw lags >= 17, the lag-by-lag kernels the rest
 
 // Per-step plumbing.  bd / lu / lt are [b][K][3 ne] (host layout).
 void LaunchKnownB(const NodeRefs* refs, const double* bd, int ne, int nstore, int K, int B, double* known,
@@ -41,14 +45,53 @@ void LaunchConvFarB(PairGrid grid, const NodeRefs* refs, const long* off, const 
                     const TB* mts, const TB* mgs, const unsigned short* mtsLo, const unsigned short* mgsLo,
                     const double* FU, const double* FT, int nstore, int K0, int Tn, int maxn, int ne, int B,
                     double* hfar, int parts, hipStream_t s);
+// Spectral part (batched_fft.hip): raw lags >= kFftLagMin of every pair.  Row
+// groups: kFftRows consecutive source rows of one domain (grpDom, grpRow0,
+// grpRows).  Items: per group, the (field, sub-filter q >= 1) pairs with the
+// mask of rows whose band has that sub-filter and the offset of their filter
+// spectra, H[hoff + (slot * rows + k) * 30 + ch * 2 + ri] for the k-th row of
+// the mask (ch: the 9 values of -MTS, then the 6 of MGS).
+struct FftItem {
+    int f;                    // field element within the domain
+    int q;                    // sub-filter (lags 16 q .. 16 q + 15 of the filtered blocks)
+    int cu, ct;               // the field's displacement / traction history columns
+    unsigned long long mask;  // rows of the group (bit r: row grpRow0 + r)
+    long hoff;                // offset of the item's filter spectra in H
+};
+static_assert(sizeof(FftItem) == 32, "FftItem is 32 bytes");
+// Count mode (items null): count[g] items and rowCount[g] rows summed over them.
+void LaunchFftItems(PairGrid grid, const NodeRefs* refs, const int* kb, const int* len, int maxn, int ngroups,
+                    const int* grpDom, const int* grpRow0, const int* grpRows, const long* itemOff, const long* hOff,
+                    long* count, long* rowCount, FftItem* items, hipStream_t s);
+template <typename TB>
+void LaunchFftFilters(PairGrid grid, const long* off, const int* kb, const int* len, const TB* mts, const TB* mgs,
+                      const unsigned short* mtsLo, const unsigned short* mgsLo, int ngroups, const int* grpDom,
+                      const int* grpRow0, const long* itemOff, long nitems, const FftItem* items, int maxn,
+                      double* H, hipStream_t s);
+// Spectra of the state window beta (states 16 beta - 30 .. 16 beta + 1; the
+// second half zero with partial) of every column:
+// W[(((col * 16 + slot) * 3 + c) * 2 + ri) * B + b].
+void LaunchFftWindows(const double* g, long cols, int nstore, int B, int beta, bool partial, double* W, hipStream_t s);
+// Spectral products of tile t (steps 2 + 16 t + i) summed per row and slot
+// into Y[((row * 16 + slot) * 6 + c * 2 + ri) * B + b]; window beta at
+// WU / WT + (beta % nwr) * ringStride, every sub-filter q <= nwr.  H holds
+// kFftHPad values past its last item's spectra (read, never used).
+void LaunchConvFarFft(int ngroups, const int* grpRow0, const int* grpRows, const long* itemOff, const FftItem* items,
+                      const double* H, const double* WU, const double* WT, long ringStrideU, long ringStrideT,
+                      int nwr, int t, int B, double* Y, hipStream_t s);
+// Samples 16 .. 16 + Tn - 1 of the inverse transforms of Y into
+// hfar[(i*3ne + r)*B + b].
+void LaunchFftInverse(const double* Y, int ne, int B, int Tn, double* hfar, hipStream_t s);
+
 // Near part of step K plus the tile's far part (the sum of its parts, in
-// order) into rhs.
+// order) and, with hfar2, the spectral share of the 16-step tile starting at
+// K02, into rhs.
 template <typename TB>
 void LaunchConvNearB(const NodeRefs* refs, const long* nearPtr, const long* nearPair, const int* nearField,
                      const long* off, const int* kb, const int* len, const TB* mts, const TB* mgs,
                      const unsigned short* mtsLo, const unsigned short* mgsLo, const double* gU, const double* gT,
                      int nstore, int K, int K0, int maxn, int ne, int B, const double* hfar, int parts,
-                     double* rhs, hipStream_t s);
+                     double* rhs, hipStream_t s, const double* hfar2 = nullptr, int K02 = 0);
 
 // Batched GMRES linear algebra (every reduction per load case, FP64).
 void LaunchPrecondB(long n, const long* posLeaf, const long* leafBegin, const long* leafSize, const long* matOff,

@@ -505,6 +505,36 @@ def test_batched_load_cases_match_single_solves(nrhs, mixed):
         assert np.abs(bit[b] - it).max() <= 1, b
 
 
+@pytest.mark.parametrize("mixed,maxn", [(0, 60), (2, 60), (0, 37)])
+def test_batched_spectral_far_part_matches_lag_by_lag(mixed, maxn):
+    """The frequency-domain far part (filtered blocks in 16-lag sub-filters
+    times 32-point spectra of the state windows, 16-step tiles) against the
+    lag-by-lag far part (8-step tiles) on 20 load cases: the same solution to
+    1e-9 relative at GMRES tol 1e-12 (the two sum in another order); with the
+    history truncated at maxn < nstep the blocks past maxn stay out of both."""
+    tdbem = _tdbem()
+    from tdbem import mesh
+    w = mesh.layered_config2(h=0.25, nstep=60)
+    m = w.mesh
+    cases = mesh.load_cases(w, 20)
+    print("case:", w.name, "spectral far part, mixed", mixed, "maxn", maxn, flush=True)
+    gc = tdbem.Case(m.points, m.ele, m.bcid, w.dt, w.gap, domain_count=4, mat_rows=w.mats, itf_rows=m.interfaces)
+    gc.set_option("mixed", mixed)
+    gc.assemble_step0(10)
+    gc.assemble_history(maxn)
+    res = {}
+    for fft in (1, 0):
+        gc.set_option("batched_fft", fft)
+        BU, BT, bit, bfl, bsolved, _ = gc.solve_batched(cases, w.nstep, tol=1e-12)
+        assert (bsolved == w.nstep).all() and not bfl.any()
+        res[fft] = (BU, BT)
+    gc.close()
+    eu = np.abs(res[1][0] - res[0][0]).max() / np.abs(res[0][0]).max()
+    et = np.abs(res[1][1] - res[0][1]).max() / np.abs(res[0][1]).max()
+    print("spectral vs lag-by-lag: u", eu, "t", et, flush=True)
+    assert eu <= 1e-9 and et <= 1e-9
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("mixed", [0, 2])
 def test_batched_far_matrix_cores_match_vector_fmas(mixed):
@@ -523,6 +553,7 @@ def test_batched_far_matrix_cores_match_vector_fmas(mixed):
     print("case:", w.name, "elements", len(m.ele), "mixed", mixed, flush=True)
     gc = tdbem.Case(m.points, m.ele, m.bcid, w.dt, w.gap, domain_count=4, mat_rows=w.mats, itf_rows=m.interfaces)
     gc.set_option("mixed", mixed)
+    gc.set_option("batched_fft", 0)  # the lag-by-lag kernels
     gc.assemble_step0(10)
     gc.assemble_history(w.nstep)
     out = []

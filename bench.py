@@ -54,6 +54,9 @@ MEASURED_TRAFFIC = {("layered_2x2x4_h0.1", 400, 16, 2): 61703231633.9,
                     ("layered_2x2x4_h0.1", 400, 16, 4): 56938840066.6,
                     ("layered_2x2x4_h0.1", 400, 16, 8): 52872278677.8}
 HEADLINE_TRAFFIC = {}
+# timings entries that are counts, flags, flops or bytes rather than seconds
+NOT_SECONDS = {"conv_far_launches", "conv_spectral_launches", "conv_spectral_flops", "conv_spectral_bytes",
+               "spectral"}
 SECONDARY_BUDGET_S = 420.0  # secondary blocks run only while the bench has used less than this
 
 
@@ -241,7 +244,7 @@ def config2_report(tdbem, gmres_device, far_parts=0):
             "history_band_entries": int(entries),
             "assembled_blocks_per_s": round(2.0 * float(entries) / max(tm["history"], 1e-9), 1),
             "breakdown_s": {k: round(float(v), 4) for k, v in tm.items()
-                            if k != "conv_far_launches" and not k.startswith("adjoint_")},
+                            if k not in NOT_SECONDS and not k.startswith("adjoint_")},
             "roofline": {"kernel": f"k_conv_far_q ({fp} wavefronts per row, 16 steps per launch)",
                          "bound": "hbm", "achieved": round(bpl / avg / 1e9, 2), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(bpl / avg / 1e9 / HBM_PEAK_GBS, 4),
@@ -344,6 +347,14 @@ def main():
                     help="cases of the timed batch checked against their FP64 single solves")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true", help="skip the config-2 and adjoint blocks")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint block")
+    ap.add_argument("--skip-headline", action="store_true",
+                    help="run only the secondary blocks (profiling them; the JSON line then has no headline value)")
+    ap.add_argument("--far-parts", type=int, default=0,
+                    help="config-2 block: wavefronts per source row of the far convolution (0: autotuned)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="first copy BYTES device to device once (a known-size dispatch for calibrating the "
+                         "FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--models", action="store_true",
                     help="also run the 16-model report (about 1.3 TB of history bands in total; about a minute)")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
@@ -359,6 +370,25 @@ def main():
 
     import tdbem
     from tdbem import mesh
+
+    if args.copy_calibration > 0:
+        n = (args.copy_calibration + 7) // 8
+        src, dst = tdbem.DeviceArray((n,)), tdbem.DeviceArray((n,))
+        if tdbem.lib().tdbem_memcpy(dst.ptr, src.ptr, 8 * n, 2) != 0:
+            raise RuntimeError("calibration copy failed")
+        tdbem.synchronize()
+        del src, dst
+        log(f"calibration copy: {8 * n} bytes device to device")
+    if args.skip_headline:
+        out = {"metric": "secondary blocks only (--skip-headline)", "value": None, "n_gpus": 1}
+        if not args.no_cpu_baseline:
+            out["cpu_baseline"] = cpu_baseline(args.cpu_threads, tdbem, args.tol)
+        if not args.no_secondary:
+            out["config2_fp64"] = config2_report(tdbem, args.gmres_device, args.far_parts)
+            if not args.no_adjoint:
+                out["adjoint"] = adjoint_report(tdbem, gmres_device=args.gmres_device)
+        print(json.dumps(out), flush=True)
+        return
 
     w = mesh.mixed_config3(h=args.h, nstep=args.nstep)
     B = args.nrhs
@@ -385,8 +415,10 @@ def main():
     c = last["case"]
     counts, npairs = c.history_stats(w.nstep)
     c.close()
+    spectral = bool(tm.get("spectral"))
+    # with the spectral far part the lag-by-lag kernel takes the raw lags <= 16
     far_avg = tm["conv_far"] / max(tm["conv_far_launches"], 1)
-    flops = batched_far_flops_per_launch(counts, w.nstep, w.nstep, B)
+    flops = batched_far_flops_per_launch(counts, w.nstep, 16 if spectral else w.nstep, B)
     far_tflops = flops / far_avg / 1e12
     all_solved = bool((last["solved"] == w.nstep).all())
     # the timed batch's results, then the plain FP64 form of the same box
@@ -437,23 +469,38 @@ def main():
         "all_solved": all_solved,
         "gmres_inner_iterations_mean": float(np.mean(last["iters"])),
         "breakdown_s": {k: round(float(v), 4) for k, v in tm.items()
-                        if k != "conv_far_launches" and not k.startswith("adjoint_")},
-        "roofline": {"kernel": f"k_conv_far_b (batched far-part convolution, {BATCH_TILE} steps x {B} load cases "
-                               f"per launch)",
-                     "bound": "mfma", "pipe": "fp64 (vector FMA; MI355X FP64 vector and matrix peaks are both "
-                                              "78.6 TFLOP/s)",
-                     "achieved": round(far_tflops, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(far_tflops / FP64_PEAK_TFLOPS, 4),
-                     "traffic": HEADLINE_TRAFFIC.get((w.name, w.nstep, B, args.mixed)),
-                     "avg_launch_ms": round(1000 * far_avg, 4), "flops_per_launch": flops,
-                     "launches": int(tm["conv_far_launches"])},
+                        if k not in NOT_SECONDS and not k.startswith("adjoint_")},
     }
+    lag_roof = {"kernel": f"k_conv_far_b (lag-by-lag far part, {BATCH_TILE} steps x {B} load cases per launch"
+                          + (", raw lags <= 16)" if spectral else ")"),
+                "bound": "mfma", "pipe": "fp64 (vector FMA; MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s)",
+                "achieved": round(far_tflops, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(far_tflops / FP64_PEAK_TFLOPS, 4), "traffic": None,
+                "avg_launch_ms": round(1000 * far_avg, 4), "flops_per_launch": flops,
+                "launches": int(tm["conv_far_launches"])}
+    if spectral:
+        sl = max(int(tm["conv_spectral_launches"]), 1)
+        prod_avg = tm["conv_spectral_products"] / sl
+        sflops = tm["conv_spectral_flops"] / sl
+        out["roofline"] = {
+            "kernel": f"k_conv_far_fft (spectral products of the batched far part, raw lags >= 17: 16 steps x {B} "
+                      f"load cases per launch)",
+            "bound": "mfma", "pipe": "fp64 (vector FMA; MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s)",
+            "achieved": round(sflops / prod_avg / 1e12, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(sflops / prod_avg / 1e12 / FP64_PEAK_TFLOPS, 4),
+            "traffic": HEADLINE_TRAFFIC.get((w.name, w.nstep, B, args.mixed)),
+            "avg_launch_ms": round(1000 * prod_avg, 4), "flops_per_launch": round(sflops),
+            "algorithmic_bytes_per_launch": round(tm["conv_spectral_bytes"] / sl), "launches": sl}
+        out["lag_by_lag_far"] = lag_roof
+    else:
+        lag_roof["traffic"] = HEADLINE_TRAFFIC.get((w.name, w.nstep, B, args.mixed))
+        out["roofline"] = lag_roof
     if not args.no_cpu_baseline:
         log(f"cpu baseline on {args.cpu_threads} threads and on 1 ...")
         out["cpu_baseline"] = cpu_baseline(args.cpu_threads, tdbem, args.tol)
     if not args.no_secondary and time.perf_counter() - t_start < SECONDARY_BUDGET_S:
-        out["config2_fp64"] = config2_report(tdbem, args.gmres_device)
-    if not args.no_secondary and time.perf_counter() - t_start < SECONDARY_BUDGET_S:
+        out["config2_fp64"] = config2_report(tdbem, args.gmres_device, args.far_parts)
+    if not args.no_secondary and not args.no_adjoint and time.perf_counter() - t_start < SECONDARY_BUDGET_S:
         out["adjoint"] = adjoint_report(tdbem, gmres_device=args.gmres_device)
     if args.models:
         out["models"] = models_report(tdbem)

@@ -217,8 +217,10 @@ int tdbem_timings(tdbem_ctx* ctx, double* t8);
  * history derivatives); for tdbem_solve_batched t[13] the spectral far
  * part's kernels (window spectra and products), t[14] their launches and
  * t[15] 1 when the spectral form ran (t[6], t[7] then cover the lag-by-lag
- * far kernel of the lags <= 16) and t[16] its set-up (item lists and filter
- * spectra, once per solve).  Values past the known ones are 0. */
+ * far kernel of the lags <= 16), t[16] its set-up (item lists and filter
+ * spectra, once per solve), t[17] its products kernel alone (device events)
+ * with t[18] that kernel's algorithmic FP64 flops and t[19] its algorithmic
+ * bytes, summed over the launches.  Values past the known ones are 0. */
 int tdbem_timings_n(tdbem_ctx* ctx, int n, double* t);
 
 /* Number of stored history band entries per lag j = 1..maxn

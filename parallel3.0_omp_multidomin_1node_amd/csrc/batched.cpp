@@ -150,6 +150,7 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
     DevBuf<double> d_H, d_WU, d_WT, d_Y, d_hfar16;
     int ngroups = 0, nwr = 1;
     long strideU = 0, strideT = 0;
+    std::vector<long> qItemsUpTo(1, 0), qRowsUpTo(1, 0);
     timing[16] = 0.0;
     if (fft) {
         const double fs0 = NowB();
@@ -164,14 +165,26 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
         d_grpDom.upload(gd.data(), gd.size(), stream);
         d_grpRow0.upload(g0.data(), g0.size(), stream);
         d_grpRows.upload(gr.data(), gr.size(), stream);
-        d_cnt.alloc(2 * (size_t)ngroups);
+        // per group: items, rows; per sub-filter q <= qcap: items, rows
+        const int qcap = (maxn + 1) / kFftTile;
+        const size_t ncnt = 2 * (size_t)ngroups + 2 * (size_t)(qcap + 1);
+        d_cnt.alloc(ncnt);
+        LaunchZero(reinterpret_cast<double*>(d_cnt.p), (long)ncnt, stream);
         LaunchFftItems(grid, d_refs.p, d_kb.p, d_len.p, maxn, ngroups, d_grpDom.p, d_grpRow0.p, d_grpRows.p, nullptr,
-                       nullptr, d_cnt.p, d_cnt.p + ngroups, nullptr, stream);
-        std::vector<long> cnt(2 * (size_t)ngroups), ioff((size_t)ngroups + 1, 0), hoff((size_t)ngroups + 1, 0);
+                       nullptr, d_cnt.p, d_cnt.p + ngroups, d_cnt.p + 2 * ngroups, qcap, nullptr, stream);
+        std::vector<long> cnt(ncnt), ioff((size_t)ngroups + 1, 0), hoff((size_t)ngroups + 1, 0);
         CheckB(CopyOnStream(cnt.data(), d_cnt.p, sizeof(long) * cnt.size(), hipMemcpyDeviceToHost, stream), "items");
         for (int gi = 0; gi < ngroups; ++gi) {
             ioff[(size_t)gi + 1] = ioff[(size_t)gi] + cnt[(size_t)gi];
             hoff[(size_t)gi + 1] = hoff[(size_t)gi] + cnt[(size_t)ngroups + gi] * 16 * 30;
+        }
+        // items and rows with sub-filter <= q, for the per-launch work of tile t (q <= t + 1)
+        qItemsUpTo.assign((size_t)qcap + 1, 0);
+        qRowsUpTo.assign((size_t)qcap + 1, 0);
+        for (int q = 0; q <= qcap; ++q) {
+            const size_t i = 2 * (size_t)ngroups + (size_t)q, r = i + (size_t)qcap + 1;
+            qItemsUpTo[(size_t)q] = (q ? qItemsUpTo[(size_t)q - 1] : 0) + cnt[i];
+            qRowsUpTo[(size_t)q] = (q ? qRowsUpTo[(size_t)q - 1] : 0) + cnt[r];
         }
         const long nitems = ioff.back(), hvals = hoff.back();
         HostBands();
@@ -204,7 +217,7 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
             d_items.alloc(4 * (size_t)std::max<long>(1, nitems));  // 32 bytes per item
             FftItem* items = reinterpret_cast<FftItem*>(d_items.p);
             LaunchFftItems(grid, d_refs.p, d_kb.p, d_len.p, maxn, ngroups, d_grpDom.p, d_grpRow0.p, d_grpRows.p,
-                           d_itemOff.p, d_hOff.p, nullptr, nullptr, items, stream);
+                           d_itemOff.p, d_hOff.p, nullptr, nullptr, nullptr, 0, items, stream);
             d_H.alloc((size_t)hvals + kFftHPad);
             LaunchZero(d_H.p + hvals, kFftHPad, stream);
             if (historyMixed)
@@ -274,19 +287,21 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
     std::vector<char> alive((size_t)B, 1);
     for (int b = 0; b < B; ++b)
         solved[b] = 0;
-    double convTime = 0.0, gmresTime = 0.0, farTime = 0.0, fftTime = 0.0;
+    double convTime = 0.0, gmresTime = 0.0, farTime = 0.0, fftTime = 0.0, fftProdTime = 0.0, fftFlops = 0.0,
+           fftBytes = 0.0;
     int farLaunches = 0, fftLaunches = 0, K016 = 0;
-    hipEvent_t fa = nullptr, fb = nullptr;
-    CheckB(hipEventCreate(&fa), "event");
-    CheckB(hipEventCreate(&fb), "event");
     struct EvGuard {
-        hipEvent_t a, b;
+        hipEvent_t e[4] = {nullptr, nullptr, nullptr, nullptr};
         ~EvGuard()
         {
-            (void)hipEventDestroy(a);
-            (void)hipEventDestroy(b);
+            for (hipEvent_t x : e)
+                if (x)
+                    (void)hipEventDestroy(x);
         }
-    } evg{fa, fb};
+    } evg;
+    for (hipEvent_t& x : evg.e)
+        CheckB(hipEventCreate(&x), "event");
+    hipEvent_t fa = evg.e[0], fb = evg.e[1], fm = evg.e[2], fp = evg.e[3];
     int K0 = 0;
     for (int K = 1; K <= nstep; ++K) {
         LaunchKnownB(d_refs.p, bd, (int)ne, nstore, K, B, d_known.p, stream);
@@ -300,16 +315,30 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
                 CheckB(hipEventRecord(fa, stream), "event");
                 LaunchFftWindows(d_gU.p, histU, nstore, B, t, false, d_WU.p + (size_t)(t % nwr) * strideU, stream);
                 LaunchFftWindows(d_gT.p, histT, nstore, B, t, false, d_WT.p + (size_t)(t % nwr) * strideT, stream);
+                CheckB(hipEventRecord(fm, stream), "event");
                 LaunchConvFarFft(ngroups, d_grpRow0.p, d_grpRows.p, d_itemOff.p,
                                  reinterpret_cast<const FftItem*>(d_items.p), d_H.p, d_WU.p, d_WT.p, strideU, strideT,
-                                 nwr, t, B, d_Y.p, stream);
+                                 nwr, t, B, d_Y.p, batchedFftMfma != 0, stream);
+                CheckB(hipEventRecord(fp, stream), "event");
                 LaunchFftInverse(d_Y.p, (int)ne, B, Tn, d_hfar16.p, stream);
                 CheckB(hipEventRecord(fb, stream), "event");
                 CheckB(hipEventSynchronize(fb), "far part");
-                float ms = 0.f;
+                float ms = 0.f, msp = 0.f;
                 CheckB(hipEventElapsedTime(&ms, fa, fb), "elapsed");
+                CheckB(hipEventElapsedTime(&msp, fm, fp), "elapsed");
                 fftTime += ms * 1e-3;
+                fftProdTime += msp * 1e-3;
                 ++fftLaunches;
+                // algorithmic work of the products: per row of an active item
+                // (sub-filter <= t + 1) and load case, 15 slots of 18 complex
+                // multiply-adds (8 flops each, in 4 FMAs) and slot 0's 36 real
+                // ones; bytes: its filter spectra (16 x 30 values) once, the
+                // window spectra (16 slots x 12 values per load case) once per
+                // item and the summed spectra written once
+                const size_t qa = (size_t)std::min(t + 1, (int)qRowsUpTo.size() - 1);
+                const double rowsA = (double)qRowsUpTo[qa], itemsA = (double)qItemsUpTo[qa];
+                fftFlops += rowsA * (15.0 * 144.0 + 72.0) * B;
+                fftBytes += rowsA * 16 * 30 * 8 + itemsA * 16 * 12 * 8.0 * B + (double)ne * 96 * B * 8;
             }
             if (K == 2 || K - K0 >= kBatchTile) {
                 int prev = K0;
@@ -386,6 +415,9 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
         timing[13] = fftTime;
         timing[14] = fftLaunches;
         timing[15] = fft ? 1.0 : 0.0;
+        timing[17] = fftProdTime;
+        timing[18] = fftFlops;
+        timing[19] = fftBytes;
         bw.release();
         return;
     }
@@ -427,6 +459,9 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
     timing[13] = fftTime;
     timing[14] = fftLaunches;
     timing[15] = fft ? 1.0 : 0.0;
+    timing[17] = fftProdTime;
+    timing[18] = fftFlops;
+    timing[19] = fftBytes;
     (void)t0;
     bw.release();
 }

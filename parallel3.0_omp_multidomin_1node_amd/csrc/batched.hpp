@@ -14,6 +14,7 @@ constexpr int kBatchFarPartsMax = 16;  // column parts of the matrix-core far pa
 constexpr int kBatchRedBlocks = 256;   // most reduction blocks of the batched dot products
 constexpr int kFftTile = 16;           // steps per tile (and lags per sub-filter) of the spectral far part
 constexpr int kFftRows = 64;           // source rows per group of the spectral far part (8 per wavefront)
+constexpr int kFftPace = 16;           // items between the pacing barriers of the spectral products' wavefronts
 constexpr int kFftHPad = 256;          // values past the filter spectra that the products kernel may read
 constexpr int kFftLagMin = 17;         // the spectral part takes the raw lags >= 17, the lag-by-lag kernels the rest
 
@@ -59,10 +60,11 @@ struct FftItem {
     long hoff;                // offset of the item's filter spectra in H
 };
 static_assert(sizeof(FftItem) == 32, "FftItem is 32 bytes");
-// Count mode (items null): count[g] items and rowCount[g] rows summed over them.
+// Count mode (items null): count[g] items and rowCount[g] rows summed over
+// them; qHist[q] += items and qHist[qcap + 1 + q] += rows of sub-filter q.
 void LaunchFftItems(PairGrid grid, const NodeRefs* refs, const int* kb, const int* len, int maxn, int ngroups,
                     const int* grpDom, const int* grpRow0, const int* grpRows, const long* itemOff, const long* hOff,
-                    long* count, long* rowCount, FftItem* items, hipStream_t s);
+                    long* count, long* rowCount, long* qHist, int qcap, FftItem* items, hipStream_t s);
 template <typename TB>
 void LaunchFftFilters(PairGrid grid, const long* off, const int* kb, const int* len, const TB* mts, const TB* mgs,
                       const unsigned short* mtsLo, const unsigned short* mgsLo, int ngroups, const int* grpDom,
@@ -75,10 +77,11 @@ void LaunchFftWindows(const double* g, long cols, int nstore, int B, int beta, b
 // Spectral products of tile t (steps 2 + 16 t + i) summed per row and slot
 // into Y[((row * 16 + slot) * 6 + c * 2 + ri) * B + b]; window beta at
 // WU / WT + (beta % nwr) * ringStride, every sub-filter q <= nwr.  H holds
-// kFftHPad values past its last item's spectra (read, never used).
+// kFftHPad values past its last item's spectra (read, never used).  mfma:
+// on the FP64 matrix cores, else with vector FMAs.
 void LaunchConvFarFft(int ngroups, const int* grpRow0, const int* grpRows, const long* itemOff, const FftItem* items,
                       const double* H, const double* WU, const double* WT, long ringStrideU, long ringStrideT,
-                      int nwr, int t, int B, double* Y, hipStream_t s);
+                      int nwr, int t, int B, double* Y, bool mfma, hipStream_t s);
 // Samples 16 .. 16 + Tn - 1 of the inverse transforms of Y into
 // hfar[(i*3ne + r)*B + b].
 void LaunchFftInverse(const double* Y, int ne, int B, int Tn, double* hfar, hipStream_t s);

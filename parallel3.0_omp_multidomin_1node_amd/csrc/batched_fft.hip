@@ -94,14 +94,16 @@ __device__ __forceinline__ bool QRange(int k0, int L, int maxn, int& qlo, int& q
 // and rowCount[g] the sum of their row counts; else they are written from
 // items[itemOff[g]] in field order, sub-filters ascending, each with the
 // offset of its filter spectra (16 slots x its rows x 30 values) from
-// hOff[g] on.
+// hOff[g] on.  Count mode also adds the items and rows of each sub-filter q
+// to qHist[q] and qHist[qcap + 1 + q] (q <= qcap).
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(64) k_fft_items(PairGrid grid, const NodeRefs* __restrict__ refs,
                                                   const int* __restrict__ kb, const int* __restrict__ len, int maxn,
                                                   const int* __restrict__ grpDom, const int* __restrict__ grpRow0,
                                                   const int* __restrict__ grpRows, const long* __restrict__ itemOff,
                                                   const long* __restrict__ hOff, long* __restrict__ count,
-                                                  long* __restrict__ rowCount, FftItem* __restrict__ items)
+                                                  long* __restrict__ rowCount, long* __restrict__ qHist, int qcap,
+                                                  FftItem* __restrict__ items)
 {
     __shared__ long pre[65], prr[65];
     const int g = blockIdx.x, lane = threadIdx.x;
@@ -138,6 +140,10 @@ __global__ void __launch_bounds__(64) k_fft_items(PairGrid grid, const NodeRefs*
             const int pc = __popcll(maskOf(q));
             cnt += pc > 0;
             rows += pc;
+            if (!items && pc && q <= qcap) {
+                atomicAdd(reinterpret_cast<unsigned long long*>(qHist + q), 1ull);
+                atomicAdd(reinterpret_cast<unsigned long long*>(qHist + qcap + 1 + q), (unsigned long long)pc);
+            }
         }
         pre[lane + 1] = cnt;
         prr[lane + 1] = rows;
@@ -181,10 +187,10 @@ __global__ void __launch_bounds__(64) k_fft_items(PairGrid grid, const NodeRefs*
 
 void LaunchFftItems(PairGrid grid, const NodeRefs* refs, const int* kb, const int* len, int maxn, int ngroups,
                     const int* grpDom, const int* grpRow0, const int* grpRows, const long* itemOff, const long* hOff,
-                    long* count, long* rowCount, FftItem* items, hipStream_t s)
+                    long* count, long* rowCount, long* qHist, int qcap, FftItem* items, hipStream_t s)
 {
     k_fft_items<<<(unsigned)ngroups, 64, 0, s>>>(grid, refs, kb, len, maxn, grpDom, grpRow0, grpRows, itemOff, hOff,
-                                                 count, rowCount, items);
+                                                 count, rowCount, qHist, qcap, items);
 }
 
 // ---------------------------------------------------------------------------
@@ -411,9 +417,13 @@ __device__ __forceinline__ void FftProducts(const FftItem* __restrict__ items, l
             }
         const unsigned long long m = item.mask;
         const int pc = __popcll(m), below = __popcll(m & ((1ull << sh) - 1));
+        const int n2 = 15 * __popcll((m >> sh) & 0xffull);  // double2 values of the wavefront's rows
         const double2* hp = reinterpret_cast<const double2*>(H + item.hoff + ((long)slot * pc + below) * 30);
-        x.h0 = hp[lane];
-        x.h1 = hp[lane + 64];
+        // lanes past the rows keep what they held: the products never read those values
+        if (lane < n2)
+            x.h0 = hp[lane];
+        if (lane + 64 < n2)
+            x.h1 = hp[lane + 64];
     };
     // the item's filter spectra into the wavefront's LDS slice; the wave's
     // LDS accesses run in order, so the previous item's products have read
@@ -463,42 +473,63 @@ __device__ __forceinline__ void FftProducts(const FftItem* __restrict__ items, l
             }
         }
     };
+    // The wavefronts of the workgroup share each item's window spectra
+    // through L2: a barrier after every kFftPace items of the list keeps them
+    // within that many items of each other.  Every wavefront passes the same
+    // nPace barriers (those of the chunks it skips on its way, and the rest at
+    // the end); they order nothing in memory, so no fence goes with them.
+    const int nPace = (int)((i1 - i0 + kFftPace - 1) / kFftPace);
+    int paced = 0;
+    auto paceTo = [&](long it) {
+        const int chunk = (int)((it - i0) / kFftPace);
+        while (paced < chunk) {
+            __builtin_amdgcn_s_barrier();
+            ++paced;
+        }
+    };
     long cur = nextItem(i0);
-    if (cur >= i1)
-        return;
-    FftRegs x0, x1, x2;
-    long n1 = nextItem(cur + 1), n2;
-    load(cur, x0);
-    load(n1 < i1 ? n1 : cur, x1);
-    stage(x0);
-    // x0 / x1 / x2 rotate: the current item's set is staged, the next one's
-    // loads are in flight and the one after is loaded before the products;
-    // past the list's end the current item is loaded again
-    for (;;) {
-        n2 = n1 < i1 ? nextItem(n1 + 1) : i1;
-        load(n2 < i1 ? n2 : cur, x2);
-        mac(cur, x0);
-        if (n1 >= i1)
-            break;
-        stage(x1);
-        cur = n1;
-        n1 = n2;
-        n2 = n1 < i1 ? nextItem(n1 + 1) : i1;
-        load(n2 < i1 ? n2 : cur, x0);
-        mac(cur, x1);
-        if (n1 >= i1)
-            break;
-        stage(x2);
-        cur = n1;
-        n1 = n2;
-        n2 = n1 < i1 ? nextItem(n1 + 1) : i1;
-        load(n2 < i1 ? n2 : cur, x1);
-        mac(cur, x2);
-        if (n1 >= i1)
-            break;
+    if (cur < i1) {
+        FftRegs x0, x1, x2;
+        long n1 = nextItem(cur + 1), n2;
+        load(cur, x0);
+        load(n1 < i1 ? n1 : cur, x1);
         stage(x0);
-        cur = n1;
-        n1 = n2;
+        // x0 / x1 / x2 rotate: the current item's set is staged, the next
+        // one's loads are in flight and the one after is loaded before the
+        // products; past the list's end the current item is loaded again
+        for (;;) {
+            n2 = n1 < i1 ? nextItem(n1 + 1) : i1;
+            load(n2 < i1 ? n2 : cur, x2);
+            paceTo(cur);
+            mac(cur, x0);
+            if (n1 >= i1)
+                break;
+            stage(x1);
+            cur = n1;
+            n1 = n2;
+            n2 = n1 < i1 ? nextItem(n1 + 1) : i1;
+            load(n2 < i1 ? n2 : cur, x0);
+            paceTo(cur);
+            mac(cur, x1);
+            if (n1 >= i1)
+                break;
+            stage(x2);
+            cur = n1;
+            n1 = n2;
+            n2 = n1 < i1 ? nextItem(n1 + 1) : i1;
+            load(n2 < i1 ? n2 : cur, x1);
+            paceTo(cur);
+            mac(cur, x2);
+            if (n1 >= i1)
+                break;
+            stage(x0);
+            cur = n1;
+            n1 = n2;
+        }
+    }
+    while (paced < nPace) {
+        __builtin_amdgcn_s_barrier();
+        ++paced;
     }
 }
 
@@ -545,13 +576,257 @@ __global__ void __launch_bounds__(512) k_conv_far_fft(
     }
 }
 
+// ---------------------------------------------------------------------------
+// The same products on the FP64 matrix cores (v_mfma_f64_16x16x4f64; A (16 x
+// 4): lane l holds A[l % 16][l / 16]; B (4 x 16): lane l holds
+// B[l / 16][l % 16]; D (16 x 16): lane l, register q holds
+// D[l / 16 + 4 q][l % 16]).  Per wavefront and item, the complex products of
+// its 8 rows and 64 load cases are one real product
+//   D (48 x 64) += A (48 x 12) B (12 x 64)
+// with output o = 16 c + 8 ri + r (component c, real / imaginary ri, row r)
+// and k = 6 field + 2 cc + wri (field U / T, component cc of the window,
+// real / imaginary wri): A holds the filter values re / -im / im / re (slot
+// 0: its two real bins, re . re and im . im), zero for the rows the item does
+// not have; B the window spectra.  3 (c) x 3 (k steps) x 4 (16-case tiles)
+// MFMAs per item; the 12 accumulator tiles stay in registers.  Each lane
+// loads its own A entries (nine values of its row's filter spectra, two
+// items ahead) and B entries (one item ahead) into registers, so no LDS
+// stands between the loads and the MFMAs.  Items, windows and pacing as in
+// k_conv_far_fft.
+// ---------------------------------------------------------------------------
+typedef double FftD4 __attribute__((ext_vector_type(4)));
+
+template <bool PACKED>
+__device__ __forceinline__ void FftProductsMf(const FftItem* __restrict__ items, long i0, long i1, int t, int tRing,
+                                              int nwr, int sh, int slot, int lane, int B, int b0,
+                                              const double* __restrict__ H, const double* __restrict__ WU,
+                                              const double* __restrict__ WT, long ringStrideU, long ringStrideT,
+                                              FftD4 (&acc)[3][4])
+{
+    const int li = lane & 15, lj = lane >> 4;
+    const int ri = li >> 3, r = li & 7;
+    // this lane's A entries: row r, output (component c, ri), k = 4 kt + lj:
+    // the value at aoff[c][kt] of the row's filter spectra, negated for bit
+    // 3 c + kt of neg, zero for that bit of zero
+    int aoff[3][3], neg = 0, zero = 0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int kt = 0; kt < 3; ++kt) {
+            const int k = 4 * kt + lj, field = k / 6, cc = (k % 6) >> 1, wri = k & 1;
+            const int gi = c == cc ? (c == 0 ? 0 : (c == 1 ? 3 : 5)) : (c + cc == 1 ? 1 : (c + cc == 2 ? 2 : 4));
+            const int base = field == 0 ? 2 * (3 * c + cc) : 18 + 2 * gi;
+            const int bit = 1 << (3 * c + kt);
+            if (PACKED) {
+                aoff[c][kt] = base + ri;
+                zero |= ri == wri ? 0 : bit;
+            } else {
+                aoff[c][kt] = base + (ri ^ wri);
+                neg |= (ri == 0 && wri == 1) ? bit : 0;
+            }
+        }
+    // this lane's B entries: k = 4 kt + lj -> field, value row kk of the slot
+    long boff[3];
+    bool bT[3];
+#pragma unroll
+    for (int kt = 0; kt < 3; ++kt) {
+        const int k = 4 * kt + lj;
+        bT[kt] = k >= 6;
+        boff[kt] = ((long)slot * 6 + (k % 6)) * B;
+    }
+    auto nextItem = [&](long it) {
+        while (it < i1 && (items[it].q > t + 1 || ((items[it].mask >> sh) & 0xffull) == 0))
+            ++it;
+        return it;
+    };
+    // window spectra (B operands) of item `it`, one item ahead (L2 hits:
+    // the workgroup's wavefronts share them)
+    auto loadW = [&](long it, double (&bw)[3][4]) {
+        const FftItem item = items[it];
+        const int beta = tRing - item.q + (tRing < item.q ? nwr : 0);
+        const double* wu = WU + beta * ringStrideU + (long)item.cu * 96 * B;
+        const double* wt = WT + beta * ringStrideT + (long)item.ct * 96 * B;
+#pragma unroll
+        for (int kt = 0; kt < 3; ++kt) {
+            const double* w = (bT[kt] ? wt : wu) + boff[kt];
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) {
+                int bn = b0 + nt * 16 + li;
+                bn = bn < B ? bn : B - 1;
+                bw[kt][nt] = w[bn];
+            }
+        }
+    };
+    // the lane's row of the item's filter spectra (A operands), two items
+    // ahead (read once from HBM); a row the item lacks reads the wavefront's
+    // first row and its A entries are zeroed
+    auto loadH = [&](long it, double (&ha)[3][3]) {
+        const FftItem item = items[it];
+        const unsigned long long m = item.mask;
+        const unsigned sub = (unsigned)((m >> sh) & 0xffull);
+        const int pc = __popcll(m), below = __popcll(m & ((1ull << sh) - 1));
+        const int rank = ((sub >> r) & 1u) ? __popc(sub & ((1u << r) - 1u)) : 0;
+        const double* hr = H + item.hoff + ((long)slot * pc + below + rank) * 30;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int kt = 0; kt < 3; ++kt)
+                ha[c][kt] = hr[aoff[c][kt]];
+    };
+    auto mac = [&](long it, const double (&bw)[3][4], const double (&ha)[3][3]) {
+        const unsigned sub = (unsigned)((items[it].mask >> sh) & 0xffull);
+        const bool have = (sub >> r) & 1u;
+#pragma unroll
+        for (int kt = 0; kt < 3; ++kt) {
+            double a[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const int bit = 1 << (3 * c + kt);
+                const double v = ha[c][kt];
+                a[c] = (!have || (zero & bit)) ? 0.0 : ((neg & bit) ? -v : v);
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int nt = 0; nt < 4; ++nt)
+                    acc[c][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[c], bw[kt][nt], acc[c][nt], 0, 0, 0);
+        }
+    };
+    const int nPace = (int)((i1 - i0 + kFftPace - 1) / kFftPace);
+    int paced = 0;
+    auto paceTo = [&](long it) {
+        const int chunk = (int)((it - i0) / kFftPace);
+        while (paced < chunk) {
+            __builtin_amdgcn_s_barrier();
+            ++paced;
+        }
+    };
+    long cur = nextItem(i0);
+    if (cur < i1) {
+        // window spectra in two alternating sets (one item ahead), filter
+        // values in three rotating sets (two items ahead); past the list's
+        // end the current item is loaded again
+        double w0[3][4], w1[3][4], h0[3][3], h1[3][3], h2[3][3];
+        long n1 = nextItem(cur + 1), n2;
+        loadW(cur, w0);
+        loadH(cur, h0);
+        loadH(n1 < i1 ? n1 : cur, h1);
+        for (;;) {
+            n2 = n1 < i1 ? nextItem(n1 + 1) : i1;
+            loadW(n1 < i1 ? n1 : cur, w1);
+            loadH(n2 < i1 ? n2 : cur, h2);
+            paceTo(cur);
+            mac(cur, w0, h0);
+            if (n1 >= i1)
+                break;
+            cur = n1;
+            n1 = n2;
+            n2 = n1 < i1 ? nextItem(n1 + 1) : i1;
+            loadW(n1 < i1 ? n1 : cur, w0);
+            loadH(n2 < i1 ? n2 : cur, h0);
+            paceTo(cur);
+            mac(cur, w1, h1);
+            if (n1 >= i1)
+                break;
+            cur = n1;
+            n1 = n2;
+            n2 = n1 < i1 ? nextItem(n1 + 1) : i1;
+            loadW(n1 < i1 ? n1 : cur, w1);
+            loadH(n2 < i1 ? n2 : cur, h1);
+            paceTo(cur);
+            mac(cur, w0, h2);
+            if (n1 >= i1)
+                break;
+            cur = n1;
+            n1 = n2;
+            n2 = n1 < i1 ? nextItem(n1 + 1) : i1;
+            loadW(n1 < i1 ? n1 : cur, w0);
+            loadH(n2 < i1 ? n2 : cur, h2);
+            paceTo(cur);
+            mac(cur, w1, h0);
+            if (n1 >= i1)
+                break;
+            cur = n1;
+            n1 = n2;
+            n2 = n1 < i1 ? nextItem(n1 + 1) : i1;
+            loadW(n1 < i1 ? n1 : cur, w1);
+            loadH(n2 < i1 ? n2 : cur, h0);
+            paceTo(cur);
+            mac(cur, w0, h1);
+            if (n1 >= i1)
+                break;
+            cur = n1;
+            n1 = n2;
+            n2 = n1 < i1 ? nextItem(n1 + 1) : i1;
+            loadW(n1 < i1 ? n1 : cur, w0);
+            loadH(n2 < i1 ? n2 : cur, h1);
+            paceTo(cur);
+            mac(cur, w1, h2);
+            if (n1 >= i1)
+                break;
+            cur = n1;
+            n1 = n2;
+        }
+    }
+    while (paced < nPace) {
+        __builtin_amdgcn_s_barrier();
+        ++paced;
+    }
+}
+
+__global__ void __launch_bounds__(512) k_conv_far_fft_mf(
+    const int* __restrict__ grpRow0, const int* __restrict__ grpRows, const long* __restrict__ itemOff,
+    const FftItem* __restrict__ items, const double* __restrict__ H, const double* __restrict__ WU,
+    const double* __restrict__ WT, long ringStrideU, long ringStrideT, int nwr, int t, int B, double* __restrict__ Y)
+{
+    const int slot = blockIdx.x & 15, g = blockIdx.x >> 4;
+    const int wave = WaveIdx(), lane = threadIdx.x & 63;
+    const int sh = 8 * wave;
+    const int b0 = blockIdx.y * 64;
+    const int s0 = grpRow0[g], nr = grpRows[g];
+    const long i0 = itemOff[g], i1 = itemOff[g + 1];
+    const int tRing = (t + 1) % nwr;
+    FftD4 acc[3][4];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt)
+            acc[c][nt] = FftD4{0.0, 0.0, 0.0, 0.0};
+    if (slot == 0)
+        FftProductsMf<true>(items, i0, i1, t, tRing, nwr, sh, slot, lane, B, b0, H, WU, WT, ringStrideU, ringStrideT,
+                            acc);
+    else
+        FftProductsMf<false>(items, i0, i1, t, tRing, nwr, sh, slot, lane, B, b0, H, WU, WT, ringStrideU,
+                             ringStrideT, acc);
+    // D[o][n] with o = l / 16 + 4 q = 8 ri + r of component c, case 16 nt + l % 16
+    const int li = lane & 15, lj = lane >> 4;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) {
+            const int bn = b0 + nt * 16 + li;
+            if (bn >= B)
+                continue;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int o = lj + 4 * q, ri = o >> 3, r = o & 7;
+                if (sh + r < nr)
+                    Y[(((long)(s0 + sh + r) * 16 + slot) * 6 + c * 2 + ri) * B + bn] = acc[c][nt][q];
+            }
+        }
+}
+
 void LaunchConvFarFft(int ngroups, const int* grpRow0, const int* grpRows, const long* itemOff, const FftItem* items,
                       const double* H, const double* WU, const double* WT, long ringStrideU, long ringStrideT,
-                      int nwr, int t, int B, double* Y, hipStream_t s)
+                      int nwr, int t, int B, double* Y, bool mfma, hipStream_t s)
 {
     dim3 g((unsigned)ngroups * 16, (unsigned)((B + 63) / 64));
-    k_conv_far_fft<<<g, 512, 0, s>>>(grpRow0, grpRows, itemOff, items, H, WU, WT, ringStrideU, ringStrideT, nwr, t, B,
-                                     Y);
+    if (mfma)
+        k_conv_far_fft_mf<<<g, 512, 0, s>>>(grpRow0, grpRows, itemOff, items, H, WU, WT, ringStrideU, ringStrideT, nwr,
+                                            t, B, Y);
+    else
+        k_conv_far_fft<<<g, 512, 0, s>>>(grpRow0, grpRows, itemOff, items, H, WU, WT, ringStrideU, ringStrideT, nwr, t,
+                                         B, Y);
 }
 
 // ---------------------------------------------------------------------------

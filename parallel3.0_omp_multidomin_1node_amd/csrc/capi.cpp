@@ -610,7 +610,7 @@ int tdbem_timings_n(tdbem_ctx* ctx, int n, double* t)
     if (!ctx || !t || n < 0)
         return Fail("tdbem_timings_n: invalid arguments");
     for (int i = 0; i < n; ++i)
-        t[i] = i < 17 ? ctx->c.timing[i] : 0.0;
+        t[i] = i < 20 ? ctx->c.timing[i] : 0.0;
     return 0;
 }
 
@@ -656,6 +656,12 @@ int tdbem_set_option(tdbem_ctx* ctx, const char* name, double value)
     }
     if (n == "near_per_entry") {
         ctx->c.nearPerEntry = value != 0.0;
+        return 0;
+    }
+    if (n == "batched_fft_mfma") {
+        if (value != 0.0 && value != 1.0)
+            return Fail("tdbem_set_option: batched_fft_mfma must be 0 or 1");
+        ctx->c.batchedFftMfma = (int)value;
         return 0;
     }
     if (n == "batched_fft") {

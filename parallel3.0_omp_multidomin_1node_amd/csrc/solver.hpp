@@ -170,7 +170,7 @@ public:
     // convolution kernels, 7 their launches, 8 history quadrature kernels;
     // adjoint: 9 forward solve, 10 backward sweep, 11 step-0 and self-table
     // differences with the correlation store, 12 history-block derivatives
-    double timing[17] = {0};
+    double timing[20] = {0};
     bool convTiled = true;   // time-blocked convolution (false: one full pass per step)
     int convTile = 16;       // steps per far-part tile (8 or 16); fixes the near lists built by the cull
     bool adjointAnalytic = true;  // history-block derivatives analytic (false: operator differences)
@@ -178,6 +178,7 @@ public:
     bool convPacked = true;    // filtered far tiles pack 64 band entries per work item across pairs
     int farParts = 4;          // wavefronts (and partial sums) per source row of the packed far part
     bool nearPerEntry = true;  // near part: one (pair, lag) entry per lane (false: one pair per lane)
+    int batchedFftMfma = 1;  // the spectral products on the FP64 matrix cores (1) or with vector FMAs (0)
     int batchedFft = -1;  // batched far part in the frequency domain: 1 on, 0 off, -1 for 16 or more load cases
     bool batchedMfma = false;  // batched far part on the FP64 matrix cores (false: vector FMAs)
     int batchedFarParts = 16;  // column parts (partial sums) of the matrix-core batched far part

@@ -505,21 +505,24 @@ def test_batched_load_cases_match_single_solves(nrhs, mixed):
         assert np.abs(bit[b] - it).max() <= 1, b
 
 
-@pytest.mark.parametrize("mixed,maxn", [(0, 60), (2, 60), (0, 37)])
-def test_batched_spectral_far_part_matches_lag_by_lag(mixed, maxn):
+@pytest.mark.parametrize("mixed,maxn,mfma", [(0, 60, 1), (2, 60, 1), (0, 37, 1), (0, 60, 0), (2, 37, 0)])
+def test_batched_spectral_far_part_matches_lag_by_lag(mixed, maxn, mfma):
     """The frequency-domain far part (filtered blocks in 16-lag sub-filters
-    times 32-point spectra of the state windows, 16-step tiles) against the
-    lag-by-lag far part (8-step tiles) on 20 load cases: the same solution to
-    1e-9 relative at GMRES tol 1e-12 (the two sum in another order); with the
-    history truncated at maxn < nstep the blocks past maxn stay out of both."""
+    times 32-point spectra of the state windows, 16-step tiles; the spectral
+    products on the FP64 matrix cores or with vector FMAs) against the
+    lag-by-lag far part (8-step tiles) on 20 load cases (a partial 64-case
+    tile): the same solution to 1e-9 relative at GMRES tol 1e-12 (the two sum
+    in another order); with the history truncated at maxn < nstep the blocks
+    past maxn stay out of both."""
     tdbem = _tdbem()
     from tdbem import mesh
     w = mesh.layered_config2(h=0.25, nstep=60)
     m = w.mesh
     cases = mesh.load_cases(w, 20)
-    print("case:", w.name, "spectral far part, mixed", mixed, "maxn", maxn, flush=True)
+    print("case:", w.name, "spectral far part, mixed", mixed, "maxn", maxn, "mfma", mfma, flush=True)
     gc = tdbem.Case(m.points, m.ele, m.bcid, w.dt, w.gap, domain_count=4, mat_rows=w.mats, itf_rows=m.interfaces)
     gc.set_option("mixed", mixed)
+    gc.set_option("batched_fft_mfma", mfma)
     gc.assemble_step0(10)
     gc.assemble_history(maxn)
     res = {}

@@ -164,6 +164,22 @@ def rod_config1(nstep: int = 100) -> Workload:
     return Workload("rod_1x1x5_h0.5", m, mats, dt, nstep, known)
 
 
+def reference_rod_case(dt: float, domain_count: int = 5, length: float = 10.0, width: float = 1.0,
+                       height: float = 1.0, nx: int = 2, ny: int = 2, nz: int = 2, nstep: int = 100,
+                       amplitude: float = 1.0, E: float = 20.0, v: float = 0.0, rho: float = 20.0) -> Workload:
+    """The reference generator's multi-domain rod (tools/generate_multidomain_rod.py:148-474, 504-575):
+    domain_count stacked prisms of length/domain_count x width x height, faces subdivided nx / ny /
+    nz, the left end fixed (BCID 123), a constant normal traction `amplitude` on the right end from
+    step 1 (the generator's x-axis load in the element frame).  With the defaults this is the
+    reference's rod120_100 case (120 elements, 5 domains, 100 steps); dt is the caller's (the
+    reference derives it from the datacard's beta and the mesh)."""
+    m = box_mesh(domain_count, length, width, height, nx, ny, nz)
+    mats = np.array([[d + 1, E, v, rho] for d in range(domain_count)])
+    sig = _heaviside(nstep, amplitude)
+    known = normal_load_history(len(m.ele), nstep, dt, m.top, sig)
+    return Workload(f"rod{len(m.ele)}_{nstep}", m, mats, dt, nstep, known)
+
+
 def layered_config2(h: float = 0.1, nstep: int = 400, seed: int = 1) -> Workload:
     """2x2x4 box of 4 stacked 1x2x2 domains, seeded E in [1,10], v in [0.2,0.35], rho in [1,3];
     Ricker normal traction on the top face, bottom fixed, dt = 0.5 h / max c1."""

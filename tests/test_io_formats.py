@@ -103,3 +103,25 @@ def test_binary_mode_choice_and_errors(tmp_path):
     with pytest.raises(RuntimeError, match="header mismatch"):
         io.load_bd(bd, 2, 3, 1)
     assert tdbem.lib().tdbem_last_error()
+
+
+def test_generator_reproduces_the_reference_rod_case():
+    """tdbem.mesh's box generator against the reference generator's own output, the shipped
+    rod120_100 case (tools/generate_multidomain_rod.py:148-575 wrote it): the point table, the
+    8-node elements, the BCIDs, the interface rows of the datacard and the boundary history are
+    identical, bit for bit."""
+    from conftest import GOLDEN
+    from tdbem import io, mesh
+    G = GOLDEN / "rod120_100"
+    card = io.read_datacard(G / "BEM_DATACARD.DAT")
+    pts, ele = io.read_model(G / "rod120_100.model")
+    bcid, lu = io.read_bd(G / "rod120_100.bd", len(ele), card.nstep)
+    w = mesh.reference_rod_case(dt=1.0, nstep=card.nstep, amplitude=card.amplitude)
+    m = w.mesh
+    assert m.points.shape == pts.shape and np.array_equal(m.points, pts)
+    assert np.array_equal(m.ele, ele)
+    assert np.array_equal(m.bcid, bcid)
+    assert np.array_equal(np.asarray(m.interfaces), np.asarray(card.interfaces))
+    assert np.array_equal(w.known, lu)
+    assert card.domain_count == m.domain_count
+

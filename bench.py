@@ -56,7 +56,7 @@ MEASURED_TRAFFIC = {("layered_2x2x4_h0.1", 400, 16, 2): 61703231633.9,
 HEADLINE_TRAFFIC = {}
 # timings entries that are counts, flags, flops or bytes rather than seconds
 NOT_SECONDS = {"conv_far_launches", "conv_spectral_launches", "conv_spectral_flops", "conv_spectral_bytes",
-               "spectral"}
+               "spectral", "stream_slabs"}
 SECONDARY_BUDGET_S = 420.0  # secondary blocks run only while the bench has used less than this
 
 
@@ -252,6 +252,35 @@ def config2_report(tdbem, gmres_device, far_parts=0):
                          "avg_launch_ms": round(1000 * avg, 4), "bytes_per_launch": round(bpl)}}
 
 
+def streamed_report(tdbem, gmres_device, budget_gb):
+    """The layered 2x2x4 block (6400 elements, 400 steps, FP64) with its
+    history streamed under a device budget of budget_gb (the bands take 64 GB):
+    whole job, slab count, far part with the slabs' regeneration, and the
+    largest difference from the resident solve relative to its largest value."""
+    from tdbem import mesh
+    w = mesh.layered_config2(h=0.1, nstep=400)
+    t0 = time.perf_counter()
+    c = make_case(tdbem, w, gmres_device=gmres_device, history_stream=1, history_budget_gb=budget_gb)
+    c.assemble_step0(10)
+    c.assemble_history(w.nstep)
+    U, T, it, fl, solved, _ = c.solve(w.known, w.nstep, tol=1e-5)
+    tdbem.synchronize()
+    wall = time.perf_counter() - t0
+    tm = c.timings()
+    c.close()
+    r = make_case(tdbem, w, gmres_device=gmres_device)
+    r.assemble_step0(10)
+    r.assemble_history(w.nstep)
+    U0, T0, *_ = r.solve(w.known, w.nstep, tol=1e-5)
+    r.close()
+    log(f"streamed config 2: {wall:.1f} s, {tm['stream_slabs']} slabs")
+    return {"workload": w.name, "budget_gb": budget_gb, "slabs": tm["stream_slabs"], "whole_job_s": round(wall, 3),
+            "steps_per_s": round(w.nstep / wall, 3), "history_near_s": round(tm["history"], 3),
+            "far_with_regeneration_s": round(tm["stream_far"], 3), "gmres_s": round(tm["gmres"], 3),
+            "max_rel_diff_u_vs_resident": float(np.abs(U - U0).max() / np.abs(U0).max()),
+            "all_solved": bool(solved == w.nstep)}
+
+
 def adjoint_report(tdbem, gmres_device=1):
     """Reverse-time adjoint on the 1x1x4 block (3456 elements, 4 domains, 300
     steps): misfit over 32 receiver elements against observations from
@@ -355,6 +384,8 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first copy BYTES device to device once (a known-size dispatch for calibrating the "
                          "FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--streamed", type=float, default=0.0, metavar="GB",
+                    help="also run config 2 with its history streamed under this device budget (several minutes)")
     ap.add_argument("--models", action="store_true",
                     help="also run the 16-model report (about 1.3 TB of history bands in total; about a minute)")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
@@ -387,6 +418,8 @@ def main():
             out["config2_fp64"] = config2_report(tdbem, args.gmres_device, args.far_parts)
             if not args.no_adjoint:
                 out["adjoint"] = adjoint_report(tdbem, gmres_device=args.gmres_device)
+        if args.streamed > 0:
+            out["streamed"] = streamed_report(tdbem, args.gmres_device, args.streamed)
         print(json.dumps(out), flush=True)
         return
 
@@ -502,6 +535,8 @@ def main():
         out["config2_fp64"] = config2_report(tdbem, args.gmres_device, args.far_parts)
     if not args.no_secondary and not args.no_adjoint and time.perf_counter() - t_start < SECONDARY_BUDGET_S:
         out["adjoint"] = adjoint_report(tdbem, gmres_device=args.gmres_device)
+    if args.streamed > 0:
+        out["streamed"] = streamed_report(tdbem, args.gmres_device, args.streamed)
     if args.models:
         out["models"] = models_report(tdbem)
     print(json.dumps(out), flush=True)

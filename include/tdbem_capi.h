@@ -220,7 +220,9 @@ int tdbem_timings(tdbem_ctx* ctx, double* t8);
  * far kernel of the lags <= 16), t[16] its set-up (item lists and filter
  * spectra, once per solve), t[17] its products kernel alone (device events)
  * with t[18] that kernel's algorithmic FP64 flops and t[19] its algorithmic
- * bytes, summed over the launches.  Values past the known ones are 0. */
+ * bytes, summed over the launches; for streamed history bands t[20] the slab
+ * count and t[21] the far part with the slabs' regeneration.  Values past the
+ * known ones are 0. */
 int tdbem_timings_n(tdbem_ctx* ctx, int n, double* t);
 
 /* Number of stored history band entries per lag j = 1..maxn
@@ -260,7 +262,20 @@ int tdbem_history_stats(tdbem_ctx* ctx, long* counts, long* npairs);
  *   "batched_mfma" 0 (default) the batched far convolution of
  *   tdbem_solve_batched on FP64 vector FMAs, 1 on the FP64 matrix cores
  *   (same sums to rounding), with "batched_far_parts" 1-16 (default 16)
- *   column parts summed in order by the near part. */
+ *   column parts summed in order by the near part;
+ *   "batched_fft" -1 (default) tdbem_solve_batched takes the raw lags >= 17
+ *   of its far part in the frequency domain (16-lag sub-filters, 32-point
+ *   spectra, 16-step tiles) from 16 load cases on, 1 always, 0 never;
+ *   "batched_fft_mfma" 1 (default) its spectral products on the FP64
+ *   matrix cores, 0 with vector FMAs;
+ *   "history_stream" -1 (default) tdbem_assemble_history keeps the FP64
+ *   bands resident when they fit the budget and otherwise streams them:
+ *   the blocks of lags <= 16 stay resident and the far part of every
+ *   16-step tile regenerates the rest slab by slab (source-row ranges) with
+ *   the lane-per-entry quadrature; 1 always streams, 0 refuses bands that do
+ *   not fit; streamed cases solve with tdbem_solve / tdbem_solve_device and
+ *   the default convolution only; "history_budget_gb" the device memory the
+ *   history may take (0, the default: 92 % of the free memory). */
 int tdbem_set_option(tdbem_ctx* ctx, const char* name, double value);
 
 /* Reference input files (host only, no device work).  binary_mode follows the

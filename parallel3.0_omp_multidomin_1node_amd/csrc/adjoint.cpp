@@ -103,6 +103,7 @@ void Case::SetMaterials(const double* evr)
 double Case::Adjoint(int nstep, const double* known_lu, const int* recv, int nrec, const double* obs, long restart,
                      double tol, double fdStep, double* grad, double* lu_out)
 {
+    RefuseStreamed("the adjoint");
     if (!historyDone || maxn < nstep)
         throw std::runtime_error("history not assembled for the requested steps");
     if (historyMixed)

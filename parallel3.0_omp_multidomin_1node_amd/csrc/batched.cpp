@@ -131,6 +131,7 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
         throw std::runtime_error("batch size must be >= 1");
     if (!step0Done)
         AssembleStep0(10);
+    RefuseStreamed("the batched solve");
     if (!historyDone || maxn < 1)
         throw std::runtime_error("history not assembled");
     // far part: the lag-by-lag kernels (8-step tiles) take every lag, or, in

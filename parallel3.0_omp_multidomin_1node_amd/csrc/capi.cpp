@@ -610,7 +610,7 @@ int tdbem_timings_n(tdbem_ctx* ctx, int n, double* t)
     if (!ctx || !t || n < 0)
         return Fail("tdbem_timings_n: invalid arguments");
     for (int i = 0; i < n; ++i)
-        t[i] = i < 20 ? ctx->c.timing[i] : 0.0;
+        t[i] = i < 22 ? ctx->c.timing[i] : 0.0;
     return 0;
 }
 
@@ -656,6 +656,18 @@ int tdbem_set_option(tdbem_ctx* ctx, const char* name, double value)
     }
     if (n == "near_per_entry") {
         ctx->c.nearPerEntry = value != 0.0;
+        return 0;
+    }
+    if (n == "history_stream") {
+        if (value != -1.0 && value != 0.0 && value != 1.0)
+            return Fail("tdbem_set_option: history_stream must be -1, 0 or 1");
+        ctx->c.historyStream = (int)value;
+        return 0;
+    }
+    if (n == "history_budget_gb") {
+        if (!(value >= 0.0))
+            return Fail("tdbem_set_option: history_budget_gb must be >= 0");
+        ctx->c.historyBudget = value * 1e9;
         return 0;
     }
     if (n == "batched_fft_mfma") {

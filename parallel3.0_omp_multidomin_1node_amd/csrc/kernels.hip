@@ -6,6 +6,8 @@
 // decisions match the CPU restatement bit for bit.
 #include "kernels.hpp"
 
+#include <stdexcept>
+
 #pragma clang fp contract(off)
 
 namespace tdbem {
@@ -1062,7 +1064,8 @@ __global__ void __launch_bounds__(256) k_history(const ElemGeo* __restrict__ geo
                                                  const int* __restrict__ dom, const NodeRefs* __restrict__ refs,
                                                  const double* __restrict__ selfT2, PairGrid grid,
                                                  const long* __restrict__ off, const int* __restrict__ kb, long npairs,
-                                                 long e0, long e1, double* __restrict__ mts, double* __restrict__ mgs,
+                                                 long e0, long e1, long ebase, int maxLag,
+                                                 double* __restrict__ mts, double* __restrict__ mgs,
                                                  unsigned char* __restrict__ alive)
 {
     long e = e0 + (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1081,6 +1084,8 @@ __global__ void __launch_bounds__(256) k_history(const ElemGeo* __restrict__ geo
     int s, f, d;
     dev::DecodePair(grid, p, s, f, d);
     int k = kb[p] + (int)(e - off[p]);
+    if (k > maxLag)
+        return;
     const DynaMat& m = grid.mats[d];
     const ElemGeo& fe = geo[f];
     const ElemPoints& P = pts[f];
@@ -1098,30 +1103,65 @@ __global__ void __launch_bounds__(256) k_history(const ElemGeo* __restrict__ geo
     }
     if (dev::JudgeT(m, fe, P, 1, src, n, blk))
         acc.Add(su, blk);
-    double* out = mts + 9 * e;
+    const long eo = e - ebase;  // the entry's place in the output arrays
+    double* out = mts + 9 * eo;
     for (int i = 0; i < 9; ++i)
         out[i] = acc.alive ? acc.v[i] : 0.0;
     unsigned char bits = acc.alive ? 1 : 0;
     dev::BlockAcc accg;
     if (dev::JudgeU(m, fe, P, src, n, blk))
         accg.Add(refs[f].ht_sign, blk);
-    double* og = mgs + 6 * e;
+    double* og = mgs + 6 * eo;
     const int pick[6] = {0, 1, 2, 4, 5, 8};
     for (int i = 0; i < 6; ++i)
         og[i] = accg.alive ? accg.v[pick[i]] : 0.0;
     bits |= accg.alive ? 2 : 0;
-    alive[e] = bits;
+    alive[eo] = bits;
 }
 
 void LaunchHistory(const ElemGeo* geo, const ElemPoints* pts, const int* dom, const NodeRefs* refs,
                    const double* selfT2, PairGrid grid, const long* off, const int* kb, long npairs, long e0, long e1,
-                   double* mts, double* mgs, unsigned char* alive, hipStream_t s)
+                   double* mts, double* mgs, unsigned char* alive, hipStream_t s, long ebase, int maxLag)
 {
     long n = e1 - e0;
     if (n <= 0)
         return;
     k_history<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(geo, pts, dom, refs, selfT2, grid, off, kb, npairs, e0, e1,
-                                                          mts, mgs, alive);
+                                                          ebase, maxLag, mts, mgs, alive);
+}
+
+// Near blocks of a slab: one thread per pair of [p0, p1) with a near part
+// (nearOff[p + 1] > nearOff[p]); its first nearOff[p + 1] - nearOff[p] lags.
+__global__ void __launch_bounds__(256) k_near_gather(const long* __restrict__ off, const long* __restrict__ nearOff,
+                                                     long p0, long p1, long ebase, const double* __restrict__ mts,
+                                                     const double* __restrict__ mgs,
+                                                     const unsigned char* __restrict__ alive, double* __restrict__ nmts,
+                                                     double* __restrict__ nmgs, unsigned char* __restrict__ nalive)
+{
+    const long p = p0 + (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= p1)
+        return;
+    const long n0 = nearOff[p], cnt = nearOff[p + 1] - n0;
+    const long e0 = off[p] - ebase;
+    for (long i = 0; i < cnt; ++i) {
+        for (int v = 0; v < 9; ++v)
+            nmts[9 * (n0 + i) + v] = mts[9 * (e0 + i) + v];
+        for (int v = 0; v < 6; ++v)
+            nmgs[6 * (n0 + i) + v] = mgs[6 * (e0 + i) + v];
+        nalive[n0 + i] = alive[e0 + i];
+    }
+}
+
+void LaunchNearGather(const long* off, const int* kb, const long* nearOff, long p0, long p1, long ebase,
+                      const double* mts, const double* mgs, const unsigned char* alive, double* nmts, double* nmgs,
+                      unsigned char* nalive, hipStream_t s)
+{
+    (void)kb;
+    const long n = p1 - p0;
+    if (n <= 0)
+        return;
+    k_near_gather<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(off, nearOff, p0, p1, ebase, mts, mgs, alive, nmts, nmgs,
+                                                            nalive);
 }
 
 // ---------------------------------------------------------------------------
@@ -1492,7 +1532,8 @@ void LaunchPwCount(const ElemGeo* geo, PairGrid grid, const int* kb, const int* 
 // waves from woff[f] on.
 __global__ void __launch_bounds__(256) k_pwpack(PairGrid grid, const NodeRefs* __restrict__ refs,
                                                 const int* __restrict__ cnt, int ne, const long* __restrict__ woff,
-                                                long* __restrict__ nw, PwWave* __restrict__ waves)
+                                                long* __restrict__ nw, PwWave* __restrict__ waves, long rowBegin,
+                                                long rowEnd)
 {
     int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= ne)
@@ -1503,7 +1544,10 @@ __global__ void __launch_bounds__(256) k_pwpack(PairGrid grid, const NodeRefs* _
     PwWave cur{};
     int lanes = 0;
     long w = waves ? woff[f] : 0, count = 0;
-    for (long sl = 0; sl < nd; ++sl) {
+    // the sources of the domain inside [rowBegin, rowEnd)
+    const long sLo = rowBegin > grid.eleBegin[d] ? rowBegin - grid.eleBegin[d] : 0;
+    const long sHi = rowEnd - grid.eleBegin[d] < nd ? rowEnd - grid.eleBegin[d] : nd;
+    for (long sl = sLo; sl < sHi; ++sl) {
         long p = grid.pairBase[d] + sl * nd + fl;
         int c = cnt[p], r0 = 0;
         while (c > 0) {
@@ -1534,10 +1578,12 @@ __global__ void __launch_bounds__(256) k_pwpack(PairGrid grid, const NodeRefs* _
 }
 
 void LaunchPwPack(PairGrid grid, const NodeRefs* refs, const int* cnt, int ne, const long* woff, long* nw,
-                  PwWave* waves, hipStream_t s)
+                  PwWave* waves, hipStream_t s, long rowBegin, long rowEnd)
 {
+    if (rowEnd < 0)
+        rowEnd = ne;
     if (ne > 0)
-        k_pwpack<<<(ne + 255) / 256, 256, 0, s>>>(grid, refs, cnt, ne, woff, nw, waves);
+        k_pwpack<<<(ne + 255) / 256, 256, 0, s>>>(grid, refs, cnt, ne, woff, nw, waves, rowBegin, rowEnd);
 }
 
 // Judges of every band entry (f2 | f1 << 2) with the current materials.
@@ -1578,7 +1624,7 @@ __global__ void __launch_bounds__(64) k_history_reg(
     const double* __restrict__ selfT2, PairGrid grid, const long* __restrict__ coff, const long* __restrict__ off,
     const int* __restrict__ kb, const int* __restrict__ len, const unsigned char* __restrict__ cls, long npairs,
     long c0, long c1, TB* __restrict__ mts, TB* __restrict__ mgs, unsigned short* __restrict__ mtsLo,
-    unsigned short* __restrict__ mgsLo, unsigned char* __restrict__ alive)
+    unsigned short* __restrict__ mgsLo, unsigned char* __restrict__ alive, long ebase, int maxLag)
 {
     __shared__ double sgeo[kNG2 * dev::kRec];
     using namespace dev;
@@ -1598,6 +1644,8 @@ __global__ void __launch_bounds__(64) k_history_reg(
     int s, f, d;
     DecodePair(grid, p, s, f, d);
     int t = (int)(wv - coff[p]) * 64 + lane;
+    if (kb[p] + (int)(wv - coff[p]) * 64 > maxLag)
+        return;  // every lag of the chunk is past the lags asked for (wave-uniform)
     const DynaMat& m = grid.mats[d];
     const ElemGeo& fe = geo[f];
     const ElemPoints& P = pts[f];
@@ -1626,8 +1674,8 @@ __global__ void __launch_bounds__(64) k_history_reg(
     }
     if (!mine)
         return;
-    FinalizeEntry(fl, selfT2 + 9 * (long)f, refs[f].hu_sign, refs[f].ht_sign, accR, accR, accRU, off[p] + t, mts, mgs,
-                  mtsLo, mgsLo, alive);
+    FinalizeEntry(fl, selfT2 + 9 * (long)f, refs[f].hu_sign, refs[f].ht_sign, accR, accR, accRU, off[p] + t - ebase,
+                  mts, mgs, mtsLo, mgsLo, alive);
 }
 
 // Patch-wise entries: one wavefront per PwWave, whose lanes are the
@@ -1642,7 +1690,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
     const double* __restrict__ selfT2, PairGrid grid, const PwWave* __restrict__ waves, long w0, long w1,
     const long* __restrict__ off, const int* __restrict__ kb, const int* __restrict__ len,
     const unsigned char* __restrict__ cls, TB* __restrict__ mts, TB* __restrict__ mgs,
-    unsigned short* __restrict__ mtsLo, unsigned short* __restrict__ mgsLo, unsigned char* __restrict__ alive)
+    unsigned short* __restrict__ mtsLo, unsigned short* __restrict__ mgsLo, unsigned char* __restrict__ alive,
+    long ebase, int maxLag)
 {
     // the patch radii are read only by the patch-mask pass, before the first
     // point record is staged, so they share the record buffer (12.1 KB of LDS
@@ -1658,6 +1707,14 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
     int lane = threadIdx.x;
     const PwWave& W = waves[wi];
     int nseg = W.nseg;
+    {
+        // the wave's pairs all start past the lags asked for (wave-uniform)
+        int first = 1 << 30;
+        for (int i = 0; i < nseg; ++i)
+            first = kb[W.pair[i]] < first ? kb[W.pair[i]] : first;
+        if (first > maxLag)
+            return;
+    }
     // field element and domain (the same for every slot)
     int s0, f, d;
     DecodePair(grid, W.pair[0], s0, f, d);
@@ -1786,45 +1843,46 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
     }
     if (!active)
         return;
-    FinalizeEntry(fl, selfT2 + 9 * (long)f, refs[f].hu_sign, refs[f].ht_sign, accA, accB, accU, off[p] + t, mts, mgs,
-                  mtsLo, mgsLo, alive);
+    FinalizeEntry(fl, selfT2 + 9 * (long)f, refs[f].hu_sign, refs[f].ht_sign, accA, accB, accU, off[p] + t - ebase,
+                  mts, mgs, mtsLo, mgsLo, alive);
 }
 
 template <typename TB>
 void LaunchHistoryReg(const ElemGeo* geo, const ElemPoints* pts, const NodeRefs* refs, const double* selfT2,
                       PairGrid grid, const long* coff, const long* off, const int* kb, const int* len,
                       const unsigned char* cls, long npairs, long c0, long c1, TB* mts, TB* mgs,
-                      unsigned short* mtsLo, unsigned short* mgsLo, unsigned char* alive, hipStream_t s)
+                      unsigned short* mtsLo, unsigned short* mgsLo, unsigned char* alive, hipStream_t s, long ebase,
+                      int maxLag)
 {
     long waves = c1 - c0;
     if (waves <= 0)
         return;
     k_history_reg<TB><<<(unsigned)waves, 64, 0, s>>>(geo, pts, refs, selfT2, grid, coff, off, kb, len, cls, npairs,
-                                                     c0, c1, mts, mgs, mtsLo, mgsLo, alive);
+                                                     c0, c1, mts, mgs, mtsLo, mgsLo, alive, ebase, maxLag);
 }
 
 template <typename TB>
 void LaunchHistoryPw(const ElemGeo* geo, const ElemPoints* pts, const NodeRefs* refs, const double* selfT2,
                      PairGrid grid, const PwWave* waves, long w0, long w1, const long* off, const int* kb,
                      const int* len, const unsigned char* cls, TB* mts, TB* mgs, unsigned short* mtsLo,
-                     unsigned short* mgsLo, unsigned char* alive, hipStream_t s)
+                     unsigned short* mgsLo, unsigned char* alive, hipStream_t s, long ebase, int maxLag)
 {
     long n = w1 - w0;
     if (n <= 0)
         return;
     k_history_pw<TB><<<(unsigned)n, 64, 0, s>>>(geo, pts, refs, selfT2, grid, waves, w0, w1, off, kb, len, cls, mts,
-                                                mgs, mtsLo, mgsLo, alive);
+                                                mgs, mtsLo, mgsLo, alive, ebase, maxLag);
 }
 
 #define TDBEM_HIST_INSTANTIATE(TB)                                                                                   \
     template void LaunchHistoryReg<TB>(const ElemGeo*, const ElemPoints*, const NodeRefs*, const double*, PairGrid,   \
                                        const long*, const long*, const int*, const int*, const unsigned char*, long,   \
                                        long, long, TB*, TB*, unsigned short*, unsigned short*, unsigned char*,         \
-                                       hipStream_t);                                                                   \
+                                       hipStream_t, long, int);                                                        \
     template void LaunchHistoryPw<TB>(const ElemGeo*, const ElemPoints*, const NodeRefs*, const double*, PairGrid,    \
                                       const PwWave*, long, long, const long*, const int*, const int*,                  \
                                       const unsigned char*, TB*, TB*, unsigned short*, unsigned short*,                \
-                                      unsigned char*, hipStream_t);
+                                      unsigned char*, hipStream_t, long, int);
 TDBEM_HIST_INSTANTIATE(double)
 TDBEM_HIST_INSTANTIATE(float)
 #undef TDBEM_HIST_INSTANTIATE
@@ -3065,7 +3123,7 @@ __global__ void __launch_bounds__(64) k_conv_far_q(PairGrid grid, const NodeRefs
                                                    const unsigned short* __restrict__ mgsLo,
                                                    const double* __restrict__ F, int fstride, int K0, int Tn,
                                                    int maxn, int ne, double* __restrict__ hfar, long partStride,
-                                                   int parts)
+                                                   int parts, int rowBegin, long ebase)
 {
     constexpr int P = dev::kFarPairs;
     constexpr int kRows = 64 + P * (T - 1);
@@ -3076,8 +3134,8 @@ __global__ void __launch_bounds__(64) k_conv_far_q(PairGrid grid, const NodeRefs
     __shared__ int ex[dev::kFarGroup + 1];  // exclusive prefix of the clipped band lengths, then the total
     __shared__ int gk[dev::kFarGroup];
     __shared__ long go[dev::kFarGroup];
-    int s = blockIdx.x / parts;     // source row
-    int wave = blockIdx.x % parts;  // part of the row: pair groups wave, wave + parts, ...
+    int s = rowBegin + (int)blockIdx.x / parts;  // source row
+    int wave = blockIdx.x % parts;               // part of the row: pair groups wave, wave + parts, ...
     int d = refs[s].domain;
     int nd = grid.eleCount[d];
     int eb = grid.eleBegin[d];
@@ -3104,7 +3162,7 @@ __global__ void __launch_bounds__(64) k_conv_far_q(PairGrid grid, const NodeRefs
             if (lane < dev::kFarGroup && q < nd) {
                 long p = rowStart + q;
                 k0 = kb[p];
-                o = off[p];
+                o = off[p] - ebase;
                 int l = len[p];
                 int lj = jmax - k0 + 1;
                 L = lj < l ? lj : l;
@@ -3287,22 +3345,28 @@ template <typename TB>
 void LaunchConvFarF(PairGrid grid, const NodeRefs* refs, const long* off, const int* kb, const int* len,
                     const TB* mts, const TB* mgs, const unsigned short* mtsLo, const unsigned short* mgsLo,
                     const double* F, int fstride, int K0, int Tn, int maxn, int ne, double* hfar, int parts,
-                    hipStream_t s)
+                    hipStream_t s, int rowBegin, int rowCount, long ebase)
 {
+    if (rowCount < 0)
+        rowCount = ne - rowBegin;
+    if (rowCount <= 0)
+        return;
     if (parts > 0)
-        k_conv_far_q<16, TB><<<(unsigned)(parts * ne), 64, 0, s>>>(grid, refs, off, kb, len, mts, mgs, mtsLo, mgsLo,
-                                                                   F, fstride, K0, Tn, maxn, ne, hfar,
-                                                                   (long)kConvTileMax * 3 * ne, parts);
-    else
+        k_conv_far_q<16, TB><<<(unsigned)(parts * rowCount), 64, 0, s>>>(
+            grid, refs, off, kb, len, mts, mgs, mtsLo, mgsLo, F, fstride, K0, Tn, maxn, ne, hfar,
+            (long)kConvTileMax * 3 * ne, parts, rowBegin, ebase);
+    else if (rowBegin == 0 && rowCount == ne && ebase == 0)
         k_conv_far_p<16, TB><<<ne, 256, 0, s>>>(grid, refs, off, kb, len, mts, mgs, mtsLo, mgsLo, F, fstride, K0,
                                                 Tn, maxn, ne, hfar);
+    else
+        throw std::runtime_error("the one-pair-per-item far kernel runs over every row of resident bands only");
 }
 template void LaunchConvFarF<double>(PairGrid, const NodeRefs*, const long*, const int*, const int*, const double*,
                                      const double*, const unsigned short*, const unsigned short*, const double*, int,
-                                     int, int, int, int, double*, int, hipStream_t);
+                                     int, int, int, int, double*, int, hipStream_t, int, int, long);
 template void LaunchConvFarF<float>(PairGrid, const NodeRefs*, const long*, const int*, const int*, const float*,
                                     const float*, const unsigned short*, const unsigned short*, const double*, int,
-                                    int, int, int, int, double*, int, hipStream_t);
+                                    int, int, int, int, double*, int, hipStream_t, int, int, long);
 
 // Near part of step K in the tile starting at K0 plus the far part:
 // rhs_K += hfar[K-K0] + sum over near pairs (band start <= K-K0+1) of the

@@ -71,14 +71,25 @@ void LaunchNearLists(PairGrid grid, const NodeRefs* refs, const int* kb, const i
 void LaunchCull(const ElemGeo* geo, const int* dom, PairGrid grid, long npairs, int maxn, int* kb, int* len,
                 unsigned char* nearFlag, hipStream_t s);
 
-// History quadrature for all band entries [e0, e1).
+// History quadrature for all band entries [e0, e1); entry e lands at place
+// e - ebase of mts / mgs / alive (a slab of the bands when ebase > 0).
+// Entries of lags > maxLag are skipped (left as they were).
 void LaunchHistory(const ElemGeo* geo, const ElemPoints* pts, const int* dom, const NodeRefs* refs,
                    const double* selfT2, PairGrid grid, const long* off, const int* kb, long npairs, long e0, long e1,
-                   double* mts, double* mgs, unsigned char* alive, hipStream_t s);
+                   double* mts, double* mgs, unsigned char* alive, hipStream_t s, long ebase = 0,
+                   int maxLag = 1 << 30);
+// Copies the blocks of lags <= lagMax of the pairs [p0, p1) from a slab of the
+// bands (entry e at place e - ebase) to a compact array (pair p's lags from
+// kb[p] at nearOff[p]).
+void LaunchNearGather(const long* off, const int* kb, const long* nearOff, long p0, long p1, long ebase,
+                      const double* mts, const double* mgs, const unsigned char* alive, double* nmts, double* nmgs,
+                      unsigned char* nalive, hipStream_t s);
 
 // Staged history quadrature in two kernels that together write every band
-// entry once.  Regular entries: one wavefront per (pair, 64-lag chunk), chunk
-// offsets coff[p] (npairs+1 entries), chunks [c0, c1).  Patch-wise entries
+// entry once (entry e at place e - ebase of the outputs; the chunks / waves
+// whose lags all exceed maxLag are skipped, the rest computed whole).  Regular entries:
+// one wavefront per (pair, 64-lag chunk), chunk offsets coff[p] (npairs+1
+// entries), chunks [c0, c1).  Patch-wise entries
 // (a judge picking the 36-patch path): waves [w0, w1) of PwWave, each packing
 // the patch-wise lags of up to kPwSlots pairs with the same field element.
 constexpr int kPwSlots = 4;
@@ -103,8 +114,9 @@ void LaunchHistoryGrad(const ElemGeo* geo, const ElemPoints* pts, const NodeRefs
 void LaunchDomainSums(const double* part, long nw, int nd, double* sums, hipStream_t s);
 // PwWave packing per field element: count pass (waves null) -> nw[f], write
 // pass -> waves[woff[f] ..].
+// (waves of the sources in [rowBegin, rowEnd) only; rowEnd < 0: every source)
 void LaunchPwPack(PairGrid grid, const NodeRefs* refs, const int* cnt, int ne, const long* woff, long* nw,
-                  PwWave* waves, hipStream_t s);
+                  PwWave* waves, hipStream_t s, long rowBegin = 0, long rowEnd = -1);
 void LaunchPwCount(const ElemGeo* geo, PairGrid grid, const int* kb, const int* len, const long* off,
                    const unsigned char* cls, long npairs, int* cnt, hipStream_t s);
 void LaunchClassify(const ElemGeo* geo, PairGrid grid, const int* kb, const int* len, const long* off, long npairs,
@@ -113,12 +125,14 @@ template <typename TB>  // TB: stored block type (double, or float in the mixed-
 void LaunchHistoryReg(const ElemGeo* geo, const ElemPoints* pts, const NodeRefs* refs, const double* selfT2,
                       PairGrid grid, const long* coff, const long* off, const int* kb, const int* len,
                       const unsigned char* cls, long npairs, long c0, long c1, TB* mts, TB* mgs,
-                      unsigned short* mtsLo, unsigned short* mgsLo, unsigned char* alive, hipStream_t s);
+                      unsigned short* mtsLo, unsigned short* mgsLo, unsigned char* alive, hipStream_t s,
+                      long ebase = 0, int maxLag = 1 << 30);
 template <typename TB>
 void LaunchHistoryPw(const ElemGeo* geo, const ElemPoints* pts, const NodeRefs* refs, const double* selfT2,
                      PairGrid grid, const PwWave* waves, long w0, long w1, const long* off, const int* kb,
                      const int* len, const unsigned char* cls, TB* mts, TB* mgs, unsigned short* mtsLo,
-                     unsigned short* mgsLo, unsigned char* alive, hipStream_t s);
+                     unsigned short* mgsLo, unsigned char* alive, hipStream_t s, long ebase = 0,
+                     int maxLag = 1 << 30);
 
 // Step-0 blocks of the near pairs listed in pairIdx (column-major, ready for
 // AddBlock): out[2*i] and out[2*i+1] with flags.
@@ -148,13 +162,15 @@ void LaunchFilterTile(const NodeRefs* refs, const double* gU, const double* gT, 
 // prefetched.  parts > 0: 64 band entries per item across pair boundaries,
 // parts wavefronts per row, one partial sum each (hfar holds parts blocks of
 // kConvTileMax * 3 * ne values, at most kFarPartsMax); parts = 0: 64 lags of
-// one pair per item and one sum per row.
+// one pair per item and one sum per row.  With parts > 0 the rows may be
+// limited to [rowBegin, rowBegin + rowCount) and the bands to a slab whose
+// first entry is ebase (pair p's lags at off[p] - ebase).
 constexpr int kFarPartsMax = 8;
 template <typename TB>
 void LaunchConvFarF(PairGrid grid, const NodeRefs* refs, const long* off, const int* kb, const int* len,
                     const TB* mts, const TB* mgs, const unsigned short* mtsLo, const unsigned short* mgsLo,
                     const double* F, int fstride, int K0, int Tn, int maxn, int ne, double* hfar, int parts,
-                    hipStream_t s);
+                    hipStream_t s, int rowBegin = 0, int rowCount = -1, long ebase = 0);
 template <typename TB>
 void LaunchConvFar(PairGrid grid, const NodeRefs* refs, const long* off, const int* kb, const int* len, const TB* mts,
                    const TB* mgs, const unsigned short* mtsLo, const unsigned short* mgsLo, const double* gU,

@@ -79,6 +79,7 @@ void Case::SolveNoAlloc(int nstep, const double* d_known, long restart, double t
         throw std::runtime_error("step 0 and the history must be assembled before a solve without allocation");
     if (historyMixed)
         throw std::runtime_error("the solve without allocation takes FP64 history bands (mixed 0)");
+    RefuseStreamed("the solve without allocation");
     if (!convTiled || (nearTile != 8 && nearTile != 16))
         throw std::runtime_error("the solve without allocation needs the time-blocked convolution and its near lists");
     if (!d_gst.p || !h_gst || !h_pin)

@@ -72,6 +72,7 @@ void GetDevice(FILE* f, void* dev, size_t bytes, std::vector<char>& stage, const
 
 void Case::SaveHistory(const std::string& path)
 {
+    RefuseStreamed("saving the history");
     if (!historyDone)
         throw std::runtime_error("no assembled history to save");
     std::unique_ptr<FILE, Closer> f(std::fopen(path.c_str(), "wb"));

@@ -455,6 +455,8 @@ std::vector<long> Case::NearPairs()
 // materials keep the same bands and take the same patch-wise / regular paths.
 void Case::FreezeStructure(bool on)
 {
+    if (on)
+        RefuseStreamed("freezing the structure");
     pwPacked = false;
     if (!on) {
         frozen = false;
@@ -620,6 +622,7 @@ void Case::ReleaseHistory()
     d_mtsLo.free();
     d_mgsLo.free();
     d_alive.free();
+    ReleaseStreamed();
     historyDone = false;
 }
 
@@ -641,6 +644,7 @@ void Case::AssembleHistory(int maxn_in)
     d_mtsLo.free();
     d_mgsLo.free();
     d_alive.free();
+    ReleaseStreamed();
     historyDone = false;
     historyMixed = mixed != 0;
     historySplit = mixed == 2;
@@ -654,9 +658,15 @@ void Case::AssembleHistory(int maxn_in)
     freeB += PooledBytesCached();
     const double vbytes = historyMixed ? (historySplit ? 6.0 : 4.0) : 8.0;
     double need = (double)entries * (15.0 * vbytes + 1.0);
-    if (need > 0.92 * (double)freeB)
+    const double budget = historyBudget > 0.0 ? std::min(historyBudget, 0.92 * (double)freeB) : 0.92 * (double)freeB;
+    if (historyStream == 1 || (historyStream == -1 && need > budget)) {
+        AssembleStreamed(budget);
+        return;
+    }
+    if (need > budget)
         throw std::runtime_error("history bands need " + std::to_string(need / 1e9) + " GB but only " +
-                                 std::to_string((double)freeB / 1e9) + " GB of device memory are free");
+                                 std::to_string(budget / 1e9) + " GB of device memory are available to them "
+                                 "(option history_stream -1 or 1 regenerates them per tile instead)");
     if (historyMixed) {
         d_mts32.alloc(9 * (size_t)entries);
         d_mgs32.alloc(6 * (size_t)entries);
@@ -694,6 +704,182 @@ void Case::AssembleHistory(int maxn_in)
     timing[2] = ms * 1e-3;
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
+    historyDone = true;
+}
+
+// ---------------------------------------------------------------------------
+// Streamed history.  The reference holds every MTS[k] / MGS[k]
+// (DBEM1/multidomain.cpp:3718-3790) and reads them every step (3840-3893).
+// When the bands exceed the device budget they are regenerated instead: the
+// cull (first lag, length and offset of every pair) stays resident, and so do
+// the blocks of lags <= kStreamNearLag in a compact array, which the near part
+// of every step reads; the far part of each 16-step tile recomputes the bands
+// of one slab of source rows at a time (a slab's entries are contiguous,
+// since a row's pairs are) into one slab buffer and convolves those rows.
+// Regeneration runs the lane-per-entry quadrature, so the blocks equal the
+// resident ones of that kernel (option quad_staged 0) bit for bit.
+// ---------------------------------------------------------------------------
+constexpr int kStreamNearLag = kConvTileMax;  // the near part of a 16-step tile reads lags <= 16
+
+void Case::RefuseStreamed(const char* what) const
+{
+    if (historyStreamed)
+        throw std::runtime_error(std::string(what) + " needs resident history bands; this case streams them "
+                                 "(assemble with history_stream 0 and a budget that holds them)");
+}
+
+void Case::ReleaseStreamed()
+{
+    d_nearOff.free();
+    d_nmts.free();
+    d_nmgs.free();
+    d_nalive.free();
+    d_smts.free();
+    d_smgs.free();
+    d_salive.free();
+    slabRow0.clear();
+    slabRow1.clear();
+    slabE0.clear();
+    slabE1.clear();
+    slabC0.clear();
+    slabC1.clear();
+    slabW0.clear();
+    slabW1.clear();
+    d_slabWaves.free();
+    historyStreamed = false;
+    timing[20] = 0.0;
+}
+
+void Case::StreamSlab(size_t i, int maxLag)
+{
+    const long e0 = slabE0[i], e1 = slabE1[i];
+    if (streamStaged) {
+        const long step = 1L << 20;  // wavefronts per launch
+        for (long c = slabC0[i]; c < slabC1[i]; c += step)
+            LaunchHistoryReg(d_geo.p, d_pts.p, d_refs.p, d_t2.p, grid, d_coff.p, d_off.p, d_kb.p, d_len.p,
+                             (const unsigned char*)nullptr, npairs, c, std::min(slabC1[i], c + step), d_smts.p,
+                             d_smgs.p, (unsigned short*)nullptr, (unsigned short*)nullptr, d_salive.p, stream, e0,
+                             maxLag);
+        for (long w = slabW0[i]; w < slabW1[i]; w += step)
+            LaunchHistoryPw(d_geo.p, d_pts.p, d_refs.p, d_t2.p, grid, d_slabWaves.p, w, std::min(slabW1[i], w + step),
+                            d_off.p, d_kb.p, d_len.p, (const unsigned char*)nullptr, d_smts.p, d_smgs.p,
+                            (unsigned short*)nullptr, (unsigned short*)nullptr, d_salive.p, stream, e0, maxLag);
+        return;
+    }
+    const long chunk = 1L << 24;
+    for (long a = e0; a < e1; a += chunk)
+        LaunchHistory(d_geo.p, d_pts.p, d_dom.p, d_refs.p, d_t2.p, grid, d_off.p, d_kb.p, npairs, a,
+                      std::min(e1, a + chunk), d_smts.p, d_smgs.p, d_salive.p, stream, e0, maxLag);
+}
+
+void Case::AssembleStreamed(double budget)
+{
+    if (historyMixed)
+        throw std::runtime_error("streamed history bands are FP64 (mixed 0)");
+    if (frozen)
+        throw std::runtime_error("the frozen structure holds resident bands");
+    HostBands();
+    // near blocks: lags kb .. min(kb + len - 1, kStreamNearLag) of every pair
+    std::vector<long> nearOff((size_t)npairs + 1, 0);
+    for (long p = 0; p < npairs; ++p) {
+        const int k0 = h_kb[(size_t)p], L = h_len[(size_t)p];
+        const long c = (L > 0 && k0 <= kStreamNearLag) ? std::min<long>(L, kStreamNearLag - k0 + 1) : 0;
+        nearOff[(size_t)p + 1] = nearOff[(size_t)p] + c;
+    }
+    const long nearN = nearOff.back();
+    const double perEntry = 15.0 * 8.0 + 1.0;
+    // row slabs: greedy, each within what the budget leaves after the near blocks
+    std::vector<long> rowPair((size_t)ne + 1, npairs);
+    for (long r = 0; r < ne; ++r) {
+        const int d = refs[(size_t)r].domain;
+        rowPair[(size_t)r] = grid.pairBase[d] + (r - grid.eleBegin[d]) * (long)grid.eleCount[d];
+    }
+    long maxRow = 1;
+    for (long r = 0; r < ne; ++r)
+        maxRow = std::max(maxRow, h_off[(size_t)rowPair[(size_t)r + 1]] - h_off[(size_t)rowPair[(size_t)r]]);
+    const double room = budget - perEntry * (double)nearN - 8.0 * (double)(npairs + 1);
+    const long cap = std::max(maxRow, (long)std::min(room / perEntry, (double)entries));
+    if (room < perEntry * (double)maxRow)
+        throw std::runtime_error("the history budget does not hold the near blocks and one source row's bands (" +
+                                 std::to_string((perEntry * (double)(nearN + maxRow)) / 1e9) + " GB)");
+    for (long r = 0; r < ne;) {
+        const long e0 = h_off[(size_t)rowPair[(size_t)r]];
+        long r1 = r + 1;
+        while (r1 < ne && h_off[(size_t)rowPair[(size_t)r1 + 1]] - e0 <= cap)
+            ++r1;
+        slabRow0.push_back(r);
+        slabRow1.push_back(r1);
+        slabE0.push_back(e0);
+        slabE1.push_back(h_off[(size_t)rowPair[(size_t)r1]]);
+        r = r1;
+    }
+    long slabMax = 1;
+    for (size_t i = 0; i < slabE0.size(); ++i)
+        slabMax = std::max(slabMax, slabE1[i] - slabE0[i]);
+    streamStaged = quadStaged;
+    if (streamStaged) {
+        // regular chunks of a slab: its pairs' chunks, contiguous like the pairs
+        std::vector<long> coff((size_t)npairs + 1);
+        Check(CopyOnStream(coff.data(), d_coff.p, sizeof(long) * coff.size(), hipMemcpyDeviceToHost, stream), "coff");
+        // patch-wise waves of a slab: each field element's lags of the slab's sources only
+        DevBuf<int> d_pwcnt;
+        d_pwcnt.alloc((size_t)npairs);
+        LaunchPwCount(d_geo.p, grid, d_kb.p, d_len.p, d_off.p, nullptr, npairs, d_pwcnt.p, stream);
+        DevBuf<long> d_nw, d_woff;
+        d_nw.alloc((size_t)ne);
+        std::vector<long> nw((size_t)ne);
+        std::vector<std::vector<long>> woff(slabE0.size(), std::vector<long>((size_t)ne));
+        long total = 0;
+        for (size_t i = 0; i < slabE0.size(); ++i) {
+            slabC0.push_back(coff[(size_t)rowPair[(size_t)slabRow0[i]]]);
+            slabC1.push_back(coff[(size_t)rowPair[(size_t)slabRow1[i]]]);
+            LaunchPwPack(grid, d_refs.p, d_pwcnt.p, (int)ne, nullptr, d_nw.p, nullptr, stream, slabRow0[i],
+                         slabRow1[i]);
+            Check(CopyOnStream(nw.data(), d_nw.p, sizeof(long) * nw.size(), hipMemcpyDeviceToHost, stream), "waves");
+            slabW0.push_back(total);
+            for (long f = 0; f < ne; ++f) {
+                woff[i][(size_t)f] = total;
+                total += nw[(size_t)f];
+            }
+            slabW1.push_back(total);
+        }
+        d_slabWaves.alloc((size_t)std::max<long>(1, total));
+        d_woff.alloc((size_t)ne);
+        for (size_t i = 0; i < slabE0.size(); ++i) {
+            Check(hipMemcpyAsync(d_woff.p, woff[i].data(), sizeof(long) * (size_t)ne, hipMemcpyHostToDevice, stream),
+                  "waves");
+            LaunchPwPack(grid, d_refs.p, d_pwcnt.p, (int)ne, d_woff.p, nullptr, d_slabWaves.p, stream, slabRow0[i],
+                         slabRow1[i]);
+            Check(hipStreamSynchronize(stream), "waves");  // woff[i] is read before the next copy
+        }
+    }
+    d_nearOff.upload(nearOff.data(), nearOff.size(), stream);
+    d_nmts.alloc(9 * (size_t)std::max<long>(1, nearN));
+    d_nmgs.alloc(6 * (size_t)std::max<long>(1, nearN));
+    d_nalive.alloc((size_t)std::max<long>(1, nearN));
+    d_smts.alloc(9 * (size_t)slabMax);
+    d_smgs.alloc(6 * (size_t)slabMax);
+    d_salive.alloc((size_t)slabMax);
+    hipEvent_t a, b;
+    Check(hipEventCreate(&a), "event");
+    Check(hipEventCreate(&b), "event");
+    Check(hipEventRecord(a, stream), "event");
+    for (size_t i = 0; i < slabE0.size(); ++i) {
+        StreamSlab(i, kStreamNearLag);
+        LaunchNearGather(d_off.p, d_kb.p, d_nearOff.p, rowPair[(size_t)slabRow0[i]], rowPair[(size_t)slabRow1[i]],
+                         slabE0[i], d_smts.p, d_smgs.p, d_salive.p, d_nmts.p, d_nmgs.p, d_nalive.p, stream);
+    }
+    Check(hipEventRecord(b, stream), "event");
+    Check(hipEventSynchronize(b), "streamed history");
+    Check(hipGetLastError(), "streamed history launch");
+    float ms = 0.f;
+    Check(hipEventElapsedTime(&ms, a, b), "elapsed");
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    timing[2] = ms * 1e-3;
+    timing[8] = ms * 1e-3;
+    timing[20] = (double)slabE0.size();
+    historyStreamed = true;
     historyDone = true;
 }
 
@@ -786,6 +972,7 @@ void Case::ExportMatrix(int which, int step, std::vector<int>& rp, std::vector<i
         val = M.value;
         return;
     }
+    RefuseStreamed("exporting history blocks");
     if (!historyDone || step < 1 || step > maxn)
         throw std::runtime_error("history step not assembled");
     if (h_mts.empty()) {
@@ -872,6 +1059,8 @@ void Case::ExportRows(int which, int step, long row0, long row1, std::vector<int
 {
     if (which != 2 && which != 3)
         throw std::runtime_error("row export covers the history matrices (2, 3)");
+    if (which >= 2)
+        RefuseStreamed("exporting history blocks");
     if (!historyDone || step < 1 || step > maxn)
         throw std::runtime_error("history step not assembled");
     if (row0 < 0 || row1 > ne || row0 >= row1)
@@ -1094,6 +1283,19 @@ int Case::Gmres(const double* d_b, double* d_x, long restart, double tol, int* i
     return GmresCore<double>(d_b, d_x, restart, tol, iters, op, d_V.p, d_W.p, d_u.p, d_u2.p);
 }
 
+// Field by field (the struct has padding): equal when the captured kernels
+// would be given the same operands.
+static bool SameGmresArgs(const GmresDevArgs& p, const GmresDevArgs& q)
+{
+    return p.ne == q.ne && p.restart == q.restart && p.nblocks == q.nblocks && p.rp == q.rp && p.col == q.col &&
+           p.val == q.val && p.posLeaf == q.posLeaf && p.leafBegin == q.leafBegin && p.leafSize == q.leafSize &&
+           p.matOff == q.matOff && p.repid == q.repid && p.outBlock == q.outBlock && p.inv == q.inv &&
+           p.prp == q.prp && p.pcol == q.pcol && p.pval == q.pval && p.V == q.V && p.W == q.W && p.u == q.u &&
+           p.u2 == q.u2 && p.vh == q.vh && p.h == q.h && p.h2 == q.h2 && p.QT == q.QT && p.R == q.R && p.f == q.f &&
+           p.phibar == q.phibar && p.part == q.part && p.x == q.x && p.nQT == q.nQT && p.nR == q.nR &&
+           p.stag == q.stag && p.st == q.st;
+}
+
 // The same solve driven from the device (gmres.hip): the opening kernels and
 // one iteration are captured once as hipGraphs (gmresDevice 1) or launched
 // directly (2); iterations are queued ahead, as many as the previous step
@@ -1141,9 +1343,7 @@ int Case::GmresDevice(const double* d_b, double* d_x, long restart, double tol, 
     a.stag = d_stag.p;
     a.st = reinterpret_cast<GmState*>(d_gst.p);
     const bool useGraph = gmresDevice == 1;
-    const void* opKey = a.prp ? (const void*)a.pval : (const void*)a.inv;
-    if (useGraph && (graphB != d_b || graphX != d_x || graphTol != tol || graphRestart != restart ||
-                     graphV != d_V.p || graphA != a.val || graphPre != opKey || graphBlocks != gm_blocks)) {
+    if (useGraph && !(graphValid && graphB == d_b && graphTol == tol && SameGmresArgs(graphArgs, a))) {
         ReleaseGraphs();
         hipGraph_t g = nullptr;
         Check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "capture");
@@ -1157,13 +1357,9 @@ int Case::GmresDevice(const double* d_b, double* d_x, long restart, double tol, 
         Check(hipGraphInstantiate(&graphIter, g, nullptr, nullptr, 0), "instantiate");
         Check(hipGraphDestroy(g), "graph");
         graphB = d_b;
-        graphX = d_x;
         graphTol = tol;
-        graphRestart = restart;
-        graphV = d_V.p;
-        graphA = a.val;
-        graphPre = opKey;
-        graphBlocks = gm_blocks;
+        graphArgs = a;
+        graphValid = true;
     }
     if (useGraph)
         Check(hipGraphLaunch(graphBegin, stream), "graph launch");
@@ -1201,7 +1397,7 @@ void Case::ReleaseGraphs()
         (void)hipGraphExecDestroy(graphIter);
     graphBegin = graphIter = nullptr;
     graphB = nullptr;
-    graphX = nullptr;
+    graphValid = false;
 }
 
 // Mixed precision: FP32-stored inner GMRES (GmresCore<float>) on A d = r
@@ -1270,6 +1466,9 @@ void Case::Solve(int nstep, const double* known_lu, long restart, double tol, do
         throw std::runtime_error("convolution tile must be 8 or 16");
     if (convTiled && nearTile != convTile)
         BuildNearLists(convTile);
+    if (historyStreamed && (!convTiled || convTile != 16 || !convFiltered || !convPacked || keep))
+        throw std::runtime_error("streamed history bands need the packed, filtered 16-step convolution "
+                                 "(conv_tiled 1, conv_tile 16, conv_filtered 1, conv_packed 1) and no adjoint");
     const long n3 = 3 * ne;
     const int nstore = nstep + 1;
     if (keep) {
@@ -1339,7 +1538,15 @@ void Case::Solve(int nstep, const double* known_lu, long restart, double tol, do
                     if (farF) {
                         LaunchFilterTile(d_refs.p, d_gU.p, d_gT.p, nstore, (int)ne, K0, std::max(0, K0 - convTile - 1),
                                          K0, d_F.p, stream);
-                        if (historyMixed)
+                        if (historyStreamed)
+                            for (size_t i = 0; i < slabE0.size(); ++i) {
+                                StreamSlab(i, std::min(K0 + Tn - 1, maxn));  // the lags this tile reads
+                                LaunchConvFarF(grid, d_refs.p, d_off.p, d_kb.p, d_len.p, (const double*)d_smts.p,
+                                               (const double*)d_smgs.p, nullptr, nullptr, d_F.p, nstore + 1, K0, Tn,
+                                               maxn, (int)ne, d_hfar.p, farParts, stream, (int)slabRow0[i],
+                                               (int)(slabRow1[i] - slabRow0[i]), slabE0[i]);
+                            }
+                        else if (historyMixed)
                             LaunchConvFarF(grid, d_refs.p, d_off.p, d_kb.p, d_len.p, (const float*)d_mts32.p,
                                            (const float*)d_mgs32.p, (const unsigned short*)d_mtsLo.p,
                                            (const unsigned short*)d_mgsLo.p, d_F.p, nstore + 1, K0, Tn, maxn,
@@ -1367,6 +1574,10 @@ void Case::Solve(int nstep, const double* known_lu, long restart, double tol, do
                                    (const float*)d_mts32.p, (const float*)d_mgs32.p, (const unsigned short*)d_mtsLo.p,
                                    (const unsigned short*)d_mgsLo.p, d_gU.p, d_gT.p, nstore, K, K0, maxn, (int)ne,
                                    d_hfar.p, hfarParts, d_rhs.p, nearPerEntry, stream);
+                else if (historyStreamed)  // the resident near blocks, pair p's lags at nearOff[p]
+                    LaunchConvNear(d_refs.p, d_nearPtr.p, d_nearPair.p, d_nearField.p, d_nearOff.p, d_kb.p, d_len.p,
+                                   (const double*)d_nmts.p, (const double*)d_nmgs.p, nullptr, nullptr, d_gU.p, d_gT.p,
+                                   nstore, K, K0, maxn, (int)ne, d_hfar.p, hfarParts, d_rhs.p, nearPerEntry, stream);
                 else
                     LaunchConvNear(d_refs.p, d_nearPtr.p, d_nearPair.p, d_nearField.p, d_off.p, d_kb.p, d_len.p,
                                    (const double*)d_mts.p, (const double*)d_mgs.p, nullptr, nullptr, d_gU.p, d_gT.p,
@@ -1447,6 +1658,7 @@ void Case::Solve(int nstep, const double* known_lu, long restart, double tol, do
     timing[5] = gmresTime;
     timing[6] = farTime;
     timing[7] = (double)farEv.size();
+    timing[21] = historyStreamed ? farTime : 0.0;  // far part with the slabs' regeneration
 }
 
 }  // namespace tdbem

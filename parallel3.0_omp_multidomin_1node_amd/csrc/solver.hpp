@@ -170,7 +170,8 @@ public:
     // convolution kernels, 7 their launches, 8 history quadrature kernels;
     // adjoint: 9 forward solve, 10 backward sweep, 11 step-0 and self-table
     // differences with the correlation store, 12 history-block derivatives
-    double timing[20] = {0};
+    // streamed history: 20 band slabs, 21 their regeneration in the time loop (device events)
+    double timing[22] = {0};
     bool convTiled = true;   // time-blocked convolution (false: one full pass per step)
     int convTile = 16;       // steps per far-part tile (8 or 16); fixes the near lists built by the cull
     bool adjointAnalytic = true;  // history-block derivatives analytic (false: operator differences)
@@ -188,6 +189,11 @@ public:
     int selfMode = -1;       // kernel of the current self tables: 1 wave per element, 0 reference order
     bool quadStaged = true;  // wave-per-pair quadrature with LDS-staged point geometry (false: lane per entry)
     int mixed = 0;  // for the next history assembly: 0 FP64, 1 FP32 blocks, 2 FP32 + bf16 residual blocks
+    // Streamed history (bands larger than the device budget): 1 always, 0
+    // never (an assembly that does not fit is refused), -1 when they do not fit.
+    int historyStream = -1;
+    double historyBudget = 0.0;  // device bytes the history may take (0: 92 % of the free memory)
+    bool historyStreamed = false;  // the bands are regenerated per far tile, slab by slab
     bool historyMixed = false, historySplit = false;
     hipStream_t stream = nullptr;
 
@@ -236,6 +242,20 @@ private:
     DevBuf<int> d_nearField;
     int nearTile = 0;  // tile the near lists were built for
     void HostBands() const;
+    // streamed history: row slabs [slabRow0, slabRow1) with band entries
+    // [slabE0, slabE1); the blocks of lags <= kStreamNearLag resident in a
+    // compact array (pair p's at nearOff[p]); one slab buffer
+    void AssembleStreamed(double budget);
+    void StreamSlab(size_t i, int maxLag);  // the lags <= maxLag of slab i (and the rest of their chunks / waves)
+    void ReleaseStreamed();
+    void RefuseStreamed(const char* what) const;
+    std::vector<long> slabRow0, slabRow1, slabE0, slabE1;
+    std::vector<long> slabC0, slabC1, slabW0, slabW1;  // staged quadrature: regular chunks, patch-wise waves
+    bool streamStaged = true;  // the slabs are regenerated by the staged kernels (else lane per entry)
+    DevBuf<PwWave> d_slabWaves;
+    DevBuf<long> d_nearOff;
+    DevBuf<double> d_nmts, d_nmgs, d_smts, d_smgs;
+    DevBuf<unsigned char> d_nalive, d_salive;
     mutable bool hostBands = false;  // h_kb, h_len, h_off hold the current cull
     mutable std::vector<int> h_kb, h_len;
     mutable std::vector<long> h_off;
@@ -263,14 +283,12 @@ private:
     DevBuf<double> d_gst;
     GmState* h_gst = nullptr;
     hipGraphExec_t graphBegin = nullptr, graphIter = nullptr;
+    // what the captured kernels were given: every operand and size, the
+    // right-hand side and the tolerance (a difference in any re-captures)
+    GmresDevArgs graphArgs{};
     const double* graphB = nullptr;
-    const double* graphX = nullptr;
-    const double* graphV = nullptr;
-    const double* graphA = nullptr;
-    const void* graphPre = nullptr;
-    int graphBlocks = 0;
     double graphTol = -1.0;
-    long graphRestart = -1;
+    bool graphValid = false;
     int gm_lastIters = 1;
 };
 

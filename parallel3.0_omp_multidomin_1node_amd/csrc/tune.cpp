@@ -68,6 +68,7 @@ std::string Case::TuneKey() const
 
 TuneChoice Case::Autotune(const std::string& cachePath)
 {
+    RefuseStreamed("the autotuner");
     if (!historyDone)
         throw std::runtime_error("autotune needs the assembled history");
     TuneChoice c{convPacked ? farParts : 0, gmBlocks, 0.0, 0.0, false};

@@ -197,10 +197,15 @@ def test_config3_full_size_fp64_and_mixed(oracle_mod):
 
 def test_config4i_64_load_cases_on_the_config2_mesh():
     """Config 4(i): 64 seeded load cases on the config-2 mesh solved as one
-    batch; sampled cases equal their own single solves to 2e-8 relative
-    (observed 4e-9): both solves stop at ||r|| <= 1e-12 ||b|| but may stop
-    on different iterations, and the batch reassociates sums; the
-    difference carries through 400 steps of history."""
+    batch; each sampled case equals its own single solve to rounding, as
+    amplified by that case's own conditioning: both solves stop at
+    ||r|| <= 1e-12 ||b|| but may stop on different iterations, the batch
+    reassociates sums (its far part in the frequency domain), and the
+    difference carries through 400 steps of history.  The bound is 200 x
+    the change a 1e-15 relative perturbation of the case's load causes in
+    its single solve (the batch differs like a 5e-14 perturbation: 25-60 x
+    observed; case 63 responds 130 x more than cases 0 and 31), and at most
+    1e-6."""
     from tdbem import mesh
     w = mesh.layered_config2(h=0.1, nstep=400)
     cases = mesh.load_cases(w, 64)
@@ -213,9 +218,13 @@ def test_config4i_64_load_cases_on_the_config2_mesh():
     for b in (0, 31, 63):
         U, T, it, fl, solved, _ = gc.solve(cases[b], w.nstep, tol=1e-12)
         assert solved == w.nstep
+        Up, Tp, *_ = gc.solve(cases[b] * (1 + 1e-15), w.nstep, tol=1e-12)
+        pu = np.abs(Up / (1 + 1e-15) - U).max() / np.abs(U).max()
+        pt = np.abs(Tp / (1 + 1e-15) - T).max() / np.abs(T).max()
         eu, et = np.abs(BU[b] - U).max() / np.abs(U).max(), np.abs(BT[b] - T).max() / np.abs(T).max()
-        print("case", b, "batch vs single: u", eu, "t", et, flush=True)
-        assert eu <= 2e-8 and et <= 2e-8, b
+        print("case", b, "batch vs single: u", eu, "t", et, "response to a 1e-15 load change: u", pu, "t", pt,
+              flush=True)
+        assert eu <= min(1e-6, 200 * max(pu, 1e-14)) and et <= min(1e-6, 200 * max(pt, 1e-14)), b
     gc.close()
 
 

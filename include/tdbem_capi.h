@@ -267,7 +267,9 @@ int tdbem_history_stats(tdbem_ctx* ctx, long* counts, long* npairs);
  *   of its far part in the frequency domain (16-lag sub-filters, 32-point
  *   spectra, 16-step tiles) from 16 load cases on, 1 always, 0 never;
  *   "batched_fft_mfma" 1 (default) its spectral products on the FP64
- *   matrix cores, 0 with vector FMAs;
+ *   matrix cores, 0 with vector FMAs; "batched_fft_pace" n > 0 makes a
+ *   workgroup's wavefronts meet at a barrier every n items (0, the default:
+ *   never);
  *   "history_stream" -1 (default) tdbem_assemble_history keeps the FP64
  *   bands resident when they fit the budget and otherwise streams them:
  *   the blocks of lags <= 16 stay resident and the far part of every

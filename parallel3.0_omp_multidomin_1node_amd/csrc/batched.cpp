@@ -319,7 +319,7 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
                 CheckB(hipEventRecord(fm, stream), "event");
                 LaunchConvFarFft(ngroups, d_grpRow0.p, d_grpRows.p, d_itemOff.p,
                                  reinterpret_cast<const FftItem*>(d_items.p), d_H.p, d_WU.p, d_WT.p, strideU, strideT,
-                                 nwr, t, B, d_Y.p, batchedFftMfma != 0, stream);
+                                 nwr, t, B, d_Y.p, batchedFftMfma != 0, batchedFftPace, stream);
                 CheckB(hipEventRecord(fp, stream), "event");
                 LaunchFftInverse(d_Y.p, (int)ne, B, Tn, d_hfar16.p, stream);
                 CheckB(hipEventRecord(fb, stream), "event");

@@ -14,7 +14,6 @@ constexpr int kBatchFarPartsMax = 16;  // column parts of the matrix-core far pa
 constexpr int kBatchRedBlocks = 256;   // most reduction blocks of the batched dot products
 constexpr int kFftTile = 16;           // steps per tile (and lags per sub-filter) of the spectral far part
 constexpr int kFftRows = 64;           // source rows per group of the spectral far part (8 per wavefront)
-constexpr int kFftPace = 16;           // items between the pacing barriers of the spectral products' wavefronts
 constexpr int kFftHPad = 256;          // values past the filter spectra that the products kernel may read
 constexpr int kFftLagMin = 17;         // the spectral part takes the raw lags >= 17, the lag-by-lag kernels the rest
 
@@ -78,10 +77,11 @@ void LaunchFftWindows(const double* g, long cols, int nstore, int B, int beta, b
 // into Y[((row * 16 + slot) * 6 + c * 2 + ri) * B + b]; window beta at
 // WU / WT + (beta % nwr) * ringStride, every sub-filter q <= nwr.  H holds
 // kFftHPad values past its last item's spectra (read, never used).  mfma:
-// on the FP64 matrix cores, else with vector FMAs.
+// on the FP64 matrix cores, else with vector FMAs; pace > 0: a workgroup's
+// wavefronts meet at a barrier after every `pace` items of its list.
 void LaunchConvFarFft(int ngroups, const int* grpRow0, const int* grpRows, const long* itemOff, const FftItem* items,
                       const double* H, const double* WU, const double* WT, long ringStrideU, long ringStrideT,
-                      int nwr, int t, int B, double* Y, bool mfma, hipStream_t s);
+                      int nwr, int t, int B, double* Y, bool mfma, int pace, hipStream_t s);
 // Samples 16 .. 16 + Tn - 1 of the inverse transforms of Y into
 // hfar[(i*3ne + r)*B + b].
 void LaunchFftInverse(const double* Y, int ne, int B, int Tn, double* hfar, hipStream_t s);

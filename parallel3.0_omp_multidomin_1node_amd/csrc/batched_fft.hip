@@ -389,7 +389,7 @@ struct FftRegs {
 
 template <bool PACKED>
 __device__ __forceinline__ void FftProducts(const FftItem* __restrict__ items, long i0, long i1, int t, int tRing,
-                                            int nwr, int sh, int slot, int lane, int B, int bb,
+                                            int nwr, int pace, int sh, int slot, int lane, int B, int bb,
                                             const double* __restrict__ H, const double* __restrict__ WU,
                                             const double* __restrict__ WT, long ringStrideU, long ringStrideT,
                                             double2* hs2, double (&acc)[8][3][2])
@@ -474,14 +474,14 @@ __device__ __forceinline__ void FftProducts(const FftItem* __restrict__ items, l
         }
     };
     // The wavefronts of the workgroup share each item's window spectra
-    // through L2: a barrier after every kFftPace items of the list keeps them
+    // through L2: a barrier after every `pace` items of the list keeps them
     // within that many items of each other.  Every wavefront passes the same
     // nPace barriers (those of the chunks it skips on its way, and the rest at
     // the end); they order nothing in memory, so no fence goes with them.
-    const int nPace = (int)((i1 - i0 + kFftPace - 1) / kFftPace);
+    const int nPace = pace > 0 ? (int)((i1 - i0 + pace - 1) / pace) : 0;
     int paced = 0;
     auto paceTo = [&](long it) {
-        const int chunk = (int)((it - i0) / kFftPace);
+        const int chunk = pace > 0 ? (int)((it - i0) / pace) : 0;
         while (paced < chunk) {
             __builtin_amdgcn_s_barrier();
             ++paced;
@@ -536,7 +536,8 @@ __device__ __forceinline__ void FftProducts(const FftItem* __restrict__ items, l
 __global__ void __launch_bounds__(512) k_conv_far_fft(
     const int* __restrict__ grpRow0, const int* __restrict__ grpRows, const long* __restrict__ itemOff,
     const FftItem* __restrict__ items, const double* __restrict__ H, const double* __restrict__ WU,
-    const double* __restrict__ WT, long ringStrideU, long ringStrideT, int nwr, int t, int B, double* __restrict__ Y)
+    const double* __restrict__ WT, long ringStrideU, long ringStrideT, int nwr, int t, int B, int pace,
+    double* __restrict__ Y)
 {
     static_assert(kFftRows == 64, "eight rows per wavefront: the filter spectra of a wavefront fill at most 240 values");
     __shared__ __align__(16) double2 Hs[8][128];
@@ -556,10 +557,10 @@ __global__ void __launch_bounds__(512) k_conv_far_fft(
         for (int c = 0; c < 3; ++c)
             acc[r][c][0] = acc[r][c][1] = 0.0;
     if (slot == 0)
-        FftProducts<true>(items, i0, i1, t, tRing, nwr, sh, slot, lane, B, bb, H, WU, WT, ringStrideU, ringStrideT,
+        FftProducts<true>(items, i0, i1, t, tRing, nwr, pace, sh, slot, lane, B, bb, H, WU, WT, ringStrideU, ringStrideT,
                           Hs[wave], acc);
     else
-        FftProducts<false>(items, i0, i1, t, tRing, nwr, sh, slot, lane, B, bb, H, WU, WT, ringStrideU, ringStrideT,
+        FftProducts<false>(items, i0, i1, t, tRing, nwr, pace, sh, slot, lane, B, bb, H, WU, WT, ringStrideU, ringStrideT,
                            Hs[wave], acc);
     if (!live)
         return;
@@ -589,16 +590,16 @@ __global__ void __launch_bounds__(512) k_conv_far_fft(
 // 0: its two real bins, re . re and im . im), zero for the rows the item does
 // not have; B the window spectra.  3 (c) x 3 (k steps) x 4 (16-case tiles)
 // MFMAs per item; the 12 accumulator tiles stay in registers.  Each lane
-// loads its own A entries (nine values of its row's filter spectra, two
-// items ahead) and B entries (one item ahead) into registers, so no LDS
-// stands between the loads and the MFMAs.  Items, windows and pacing as in
+// loads its own A entries (nine values of its row's filter spectra) and B
+// entries one item ahead into registers, so no LDS stands between the loads
+// and the MFMAs.  Items, windows and pacing as in
 // k_conv_far_fft.
 // ---------------------------------------------------------------------------
 typedef double FftD4 __attribute__((ext_vector_type(4)));
 
 template <bool PACKED>
 __device__ __forceinline__ void FftProductsMf(const FftItem* __restrict__ items, long i0, long i1, int t, int tRing,
-                                              int nwr, int sh, int slot, int lane, int B, int b0,
+                                              int nwr, int pace, int sh, int slot, int lane, int B, int b0,
                                               const double* __restrict__ H, const double* __restrict__ WU,
                                               const double* __restrict__ WT, long ringStrideU, long ringStrideT,
                                               FftD4 (&acc)[3][4])
@@ -639,8 +640,8 @@ __device__ __forceinline__ void FftProductsMf(const FftItem* __restrict__ items,
             ++it;
         return it;
     };
-    // window spectra (B operands) of item `it`, one item ahead (L2 hits:
-    // the workgroup's wavefronts share them)
+    // window spectra (B operands) of item `it` (L2 hits: the workgroup's
+    // wavefronts share them)
     auto loadW = [&](long it, double (&bw)[3][4]) {
         const FftItem item = items[it];
         const int beta = tRing - item.q + (tRing < item.q ? nwr : 0);
@@ -657,9 +658,9 @@ __device__ __forceinline__ void FftProductsMf(const FftItem* __restrict__ items,
             }
         }
     };
-    // the lane's row of the item's filter spectra (A operands), two items
-    // ahead (read once from HBM); a row the item lacks reads the wavefront's
-    // first row and its A entries are zeroed
+    // the lane's row of the item's filter spectra (A operands, read once from
+    // HBM); a row the item lacks reads the wavefront's first row and its A
+    // entries are zeroed
     auto loadH = [&](long it, double (&ha)[3][3]) {
         const FftItem item = items[it];
         const unsigned long long m = item.mask;
@@ -692,10 +693,10 @@ __device__ __forceinline__ void FftProductsMf(const FftItem* __restrict__ items,
                     acc[c][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[c], bw[kt][nt], acc[c][nt], 0, 0, 0);
         }
     };
-    const int nPace = (int)((i1 - i0 + kFftPace - 1) / kFftPace);
+    const int nPace = pace > 0 ? (int)((i1 - i0 + pace - 1) / pace) : 0;
     int paced = 0;
     auto paceTo = [&](long it) {
-        const int chunk = (int)((it - i0) / kFftPace);
+        const int chunk = pace > 0 ? (int)((it - i0) / pace) : 0;
         while (paced < chunk) {
             __builtin_amdgcn_s_barrier();
             ++paced;
@@ -703,69 +704,30 @@ __device__ __forceinline__ void FftProductsMf(const FftItem* __restrict__ items,
     };
     long cur = nextItem(i0);
     if (cur < i1) {
-        // window spectra in two alternating sets (one item ahead), filter
-        // values in three rotating sets (two items ahead); past the list's
-        // end the current item is loaded again
-        double w0[3][4], w1[3][4], h0[3][3], h1[3][3], h2[3][3];
-        long n1 = nextItem(cur + 1), n2;
+        // two register sets: the next item's window spectra and filter values
+        // are in flight while the current item's products run (36 MFMAs of
+        // 64 cycles); a deeper ring does not fit the registers of two
+        // wavefronts per SIMD without spilling
+        double w0[3][4], w1[3][4], h0[3][3], h1[3][3];
         loadW(cur, w0);
         loadH(cur, h0);
-        loadH(n1 < i1 ? n1 : cur, h1);
         for (;;) {
-            n2 = n1 < i1 ? nextItem(n1 + 1) : i1;
+            long n1 = nextItem(cur + 1);
             loadW(n1 < i1 ? n1 : cur, w1);
-            loadH(n2 < i1 ? n2 : cur, h2);
+            loadH(n1 < i1 ? n1 : cur, h1);
             paceTo(cur);
             mac(cur, w0, h0);
             if (n1 >= i1)
                 break;
             cur = n1;
-            n1 = n2;
-            n2 = n1 < i1 ? nextItem(n1 + 1) : i1;
+            n1 = nextItem(cur + 1);
             loadW(n1 < i1 ? n1 : cur, w0);
-            loadH(n2 < i1 ? n2 : cur, h0);
+            loadH(n1 < i1 ? n1 : cur, h0);
             paceTo(cur);
             mac(cur, w1, h1);
             if (n1 >= i1)
                 break;
             cur = n1;
-            n1 = n2;
-            n2 = n1 < i1 ? nextItem(n1 + 1) : i1;
-            loadW(n1 < i1 ? n1 : cur, w1);
-            loadH(n2 < i1 ? n2 : cur, h1);
-            paceTo(cur);
-            mac(cur, w0, h2);
-            if (n1 >= i1)
-                break;
-            cur = n1;
-            n1 = n2;
-            n2 = n1 < i1 ? nextItem(n1 + 1) : i1;
-            loadW(n1 < i1 ? n1 : cur, w0);
-            loadH(n2 < i1 ? n2 : cur, h2);
-            paceTo(cur);
-            mac(cur, w1, h0);
-            if (n1 >= i1)
-                break;
-            cur = n1;
-            n1 = n2;
-            n2 = n1 < i1 ? nextItem(n1 + 1) : i1;
-            loadW(n1 < i1 ? n1 : cur, w1);
-            loadH(n2 < i1 ? n2 : cur, h0);
-            paceTo(cur);
-            mac(cur, w0, h1);
-            if (n1 >= i1)
-                break;
-            cur = n1;
-            n1 = n2;
-            n2 = n1 < i1 ? nextItem(n1 + 1) : i1;
-            loadW(n1 < i1 ? n1 : cur, w0);
-            loadH(n2 < i1 ? n2 : cur, h1);
-            paceTo(cur);
-            mac(cur, w1, h2);
-            if (n1 >= i1)
-                break;
-            cur = n1;
-            n1 = n2;
         }
     }
     while (paced < nPace) {
@@ -777,7 +739,8 @@ __device__ __forceinline__ void FftProductsMf(const FftItem* __restrict__ items,
 __global__ void __launch_bounds__(512) k_conv_far_fft_mf(
     const int* __restrict__ grpRow0, const int* __restrict__ grpRows, const long* __restrict__ itemOff,
     const FftItem* __restrict__ items, const double* __restrict__ H, const double* __restrict__ WU,
-    const double* __restrict__ WT, long ringStrideU, long ringStrideT, int nwr, int t, int B, double* __restrict__ Y)
+    const double* __restrict__ WT, long ringStrideU, long ringStrideT, int nwr, int t, int B, int pace,
+    double* __restrict__ Y)
 {
     const int slot = blockIdx.x & 15, g = blockIdx.x >> 4;
     const int wave = WaveIdx(), lane = threadIdx.x & 63;
@@ -793,10 +756,10 @@ __global__ void __launch_bounds__(512) k_conv_far_fft_mf(
         for (int nt = 0; nt < 4; ++nt)
             acc[c][nt] = FftD4{0.0, 0.0, 0.0, 0.0};
     if (slot == 0)
-        FftProductsMf<true>(items, i0, i1, t, tRing, nwr, sh, slot, lane, B, b0, H, WU, WT, ringStrideU, ringStrideT,
+        FftProductsMf<true>(items, i0, i1, t, tRing, nwr, pace, sh, slot, lane, B, b0, H, WU, WT, ringStrideU, ringStrideT,
                             acc);
     else
-        FftProductsMf<false>(items, i0, i1, t, tRing, nwr, sh, slot, lane, B, b0, H, WU, WT, ringStrideU,
+        FftProductsMf<false>(items, i0, i1, t, tRing, nwr, pace, sh, slot, lane, B, b0, H, WU, WT, ringStrideU,
                              ringStrideT, acc);
     // D[o][n] with o = l / 16 + 4 q = 8 ri + r of component c, case 16 nt + l % 16
     const int li = lane & 15, lj = lane >> 4;
@@ -818,15 +781,15 @@ __global__ void __launch_bounds__(512) k_conv_far_fft_mf(
 
 void LaunchConvFarFft(int ngroups, const int* grpRow0, const int* grpRows, const long* itemOff, const FftItem* items,
                       const double* H, const double* WU, const double* WT, long ringStrideU, long ringStrideT,
-                      int nwr, int t, int B, double* Y, bool mfma, hipStream_t s)
+                      int nwr, int t, int B, double* Y, bool mfma, int pace, hipStream_t s)
 {
     dim3 g((unsigned)ngroups * 16, (unsigned)((B + 63) / 64));
     if (mfma)
         k_conv_far_fft_mf<<<g, 512, 0, s>>>(grpRow0, grpRows, itemOff, items, H, WU, WT, ringStrideU, ringStrideT, nwr,
-                                            t, B, Y);
+                                            t, B, pace, Y);
     else
         k_conv_far_fft<<<g, 512, 0, s>>>(grpRow0, grpRows, itemOff, items, H, WU, WT, ringStrideU, ringStrideT, nwr, t,
-                                         B, Y);
+                                         B, pace, Y);
 }
 
 // ---------------------------------------------------------------------------

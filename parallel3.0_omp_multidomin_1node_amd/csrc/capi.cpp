@@ -670,6 +670,12 @@ int tdbem_set_option(tdbem_ctx* ctx, const char* name, double value)
         ctx->c.historyBudget = value * 1e9;
         return 0;
     }
+    if (n == "batched_fft_pace") {
+        if (!(value >= 0.0 && value <= 1e6))
+            return Fail("tdbem_set_option: batched_fft_pace must be 0 .. 1e6");
+        ctx->c.batchedFftPace = (int)value;
+        return 0;
+    }
     if (n == "batched_fft_mfma") {
         if (value != 0.0 && value != 1.0)
             return Fail("tdbem_set_option: batched_fft_mfma must be 0 or 1");

@@ -179,6 +179,7 @@ public:
     bool convPacked = true;    // filtered far tiles pack 64 band entries per work item across pairs
     int farParts = 4;          // wavefronts (and partial sums) per source row of the packed far part
     bool nearPerEntry = true;  // near part: one (pair, lag) entry per lane (false: one pair per lane)
+    int batchedFftPace = 0;  // items between the pacing barriers of the spectral products (0: none)
     int batchedFftMfma = 1;  // the spectral products on the FP64 matrix cores (1) or with vector FMAs (0)
     int batchedFft = -1;  // batched far part in the frequency domain: 1 on, 0 off, -1 for 16 or more load cases
     bool batchedMfma = false;  // batched far part on the FP64 matrix cores (false: vector FMAs)

@@ -146,7 +146,7 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
     const long histU = dof.historyUBlockCount, histT = dof.historyTBlockCount;
     // spectral part: row groups, their (field, sub-filter) items and filter
     // spectra, and the window spectra (a ring over the longest band's sub-filters)
-    DevBuf<int> d_grpDom, d_grpRow0, d_grpRows;
+    DevBuf<int> d_grpDom, d_grpRowList, d_grpRows;
     DevBuf<long> d_itemOff, d_hOff, d_items, d_cnt;
     DevBuf<double> d_H, d_WU, d_WT, d_Y, d_hfar16;
     int ngroups = 0, nwr = 1;
@@ -155,23 +155,49 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
     timing[16] = 0.0;
     if (fft) {
         const double fs0 = NowB();
-        std::vector<int> gd, g0, gr;
-        for (int dd = 0; dd < grid.nd; ++dd)
-            for (int r0 = 0; r0 < grid.eleCount[dd]; r0 += kFftRows) {
-                gd.push_back(dd);
-                g0.push_back(grid.eleBegin[dd] + r0);
-                gr.push_back(std::min(kFftRows, grid.eleCount[dd] - r0));
+        // groups of kFftRows source rows of one domain in the Z-order (Morton)
+        // of their element centres, so a group's rows lie close together and
+        // share more of their fields' sub-filters (fewer, fuller items)
+        std::vector<int> gd, gl, gr;
+        for (int dd = 0; dd < grid.nd; ++dd) {
+            const int eb = grid.eleBegin[dd], nd = grid.eleCount[dd];
+            double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+            for (int r = eb; r < eb + nd; ++r)
+                for (int a = 0; a < 3; ++a) {
+                    lo[a] = std::min(lo[a], geo[(size_t)r].center[a]);
+                    hi[a] = std::max(hi[a], geo[(size_t)r].center[a]);
+                }
+            std::vector<std::pair<unsigned long long, int>> key((size_t)nd);
+            for (int i = 0; i < nd; ++i) {
+                unsigned long long code = 0;
+                for (int a = 0; a < 3; ++a) {
+                    const double span = hi[a] - lo[a];
+                    const double x = span > 0.0 ? (geo[(size_t)(eb + i)].center[a] - lo[a]) / span : 0.0;
+                    const unsigned v = (unsigned)std::min(1023.0, x * 1023.0);
+                    for (int bit = 0; bit < 10; ++bit)
+                        code |= (unsigned long long)((v >> bit) & 1u) << (3 * bit + a);
+                }
+                key[(size_t)i] = {code, eb + i};
             }
+            std::sort(key.begin(), key.end());  // ties by element index: deterministic
+            for (int r0 = 0; r0 < nd; r0 += kFftRows) {
+                gd.push_back(dd);
+                const int cnt = std::min(kFftRows, nd - r0);
+                gr.push_back(cnt);
+                for (int r = 0; r < kFftRows; ++r)
+                    gl.push_back(r < cnt ? key[(size_t)(r0 + r)].second : key[(size_t)r0].second);
+            }
+        }
         ngroups = (int)gd.size();
         d_grpDom.upload(gd.data(), gd.size(), stream);
-        d_grpRow0.upload(g0.data(), g0.size(), stream);
+        d_grpRowList.upload(gl.data(), gl.size(), stream);
         d_grpRows.upload(gr.data(), gr.size(), stream);
         // per group: items, rows; per sub-filter q <= qcap: items, rows
         const int qcap = (maxn + 1) / kFftTile;
         const size_t ncnt = 2 * (size_t)ngroups + 2 * (size_t)(qcap + 1);
         d_cnt.alloc(ncnt);
         LaunchZero(reinterpret_cast<double*>(d_cnt.p), (long)ncnt, stream);
-        LaunchFftItems(grid, d_refs.p, d_kb.p, d_len.p, maxn, ngroups, d_grpDom.p, d_grpRow0.p, d_grpRows.p, nullptr,
+        LaunchFftItems(grid, d_refs.p, d_kb.p, d_len.p, maxn, ngroups, d_grpDom.p, d_grpRowList.p, d_grpRows.p, nullptr,
                        nullptr, d_cnt.p, d_cnt.p + ngroups, d_cnt.p + 2 * ngroups, qcap, nullptr, stream);
         std::vector<long> cnt(ncnt), ioff((size_t)ngroups + 1, 0), hoff((size_t)ngroups + 1, 0);
         CheckB(CopyOnStream(cnt.data(), d_cnt.p, sizeof(long) * cnt.size(), hipMemcpyDeviceToHost, stream), "items");
@@ -201,8 +227,9 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
         strideT = histT * 96 * (long)B;
         // the spectral part's buffers, and what the time loop allocates after them
         const double nB = (double)n * B, ns = (double)nstep + 1;
-        const double bytes = 8.0 * ((double)hvals + (double)(strideU + strideT) * nwr + 16.0 * nB + (double)ne * 96 * B) +
-                             32.0 * (double)nitems;
+        const double bytes =
+            8.0 * ((double)hvals + (double)(strideU + strideT) * nwr + 16.0 * nB + (double)ne * 96 * B) +
+            32.0 * (double)nitems;
         const double rest = 8.0 * (nB * (2.0 * restart + 6) + (double)(histU + histT) * 3 * B * (2 * ns + 1) +
                                    3.0 * std::max<long>(1, dof.knownBlockCount) * B +
                                    std::max(1, batchedMfma ? batchedFarParts : 0) * kBatchTile * nB +
@@ -217,18 +244,18 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
             d_hOff.upload(hoff.data(), hoff.size(), stream);
             d_items.alloc(4 * (size_t)std::max<long>(1, nitems));  // 32 bytes per item
             FftItem* items = reinterpret_cast<FftItem*>(d_items.p);
-            LaunchFftItems(grid, d_refs.p, d_kb.p, d_len.p, maxn, ngroups, d_grpDom.p, d_grpRow0.p, d_grpRows.p,
+            LaunchFftItems(grid, d_refs.p, d_kb.p, d_len.p, maxn, ngroups, d_grpDom.p, d_grpRowList.p, d_grpRows.p,
                            d_itemOff.p, d_hOff.p, nullptr, nullptr, nullptr, 0, items, stream);
             d_H.alloc((size_t)hvals + kFftHPad);
             LaunchZero(d_H.p + hvals, kFftHPad, stream);
             if (historyMixed)
                 LaunchFftFilters(grid, d_off.p, d_kb.p, d_len.p, (const float*)d_mts32.p, (const float*)d_mgs32.p,
                                  historySplit ? d_mtsLo.p : nullptr, historySplit ? d_mgsLo.p : nullptr, ngroups,
-                                 d_grpDom.p, d_grpRow0.p, d_itemOff.p, nitems, items, maxn, d_H.p, stream);
+                                 d_grpDom.p, d_grpRowList.p, d_itemOff.p, nitems, items, maxn, d_H.p, stream);
             else
                 LaunchFftFilters(grid, d_off.p, d_kb.p, d_len.p, (const double*)d_mts.p, (const double*)d_mgs.p,
-                                 nullptr, nullptr, ngroups, d_grpDom.p, d_grpRow0.p, d_itemOff.p, nitems, items, maxn,
-                                 d_H.p, stream);
+                                 nullptr, nullptr, ngroups, d_grpDom.p, d_grpRowList.p, d_itemOff.p, nitems, items,
+                                 maxn, d_H.p, stream);
             d_WU.alloc((size_t)strideU * nwr);
             d_WT.alloc((size_t)strideT * nwr);
             d_Y.alloc((size_t)ne * 96 * B);
@@ -317,7 +344,7 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
                 LaunchFftWindows(d_gU.p, histU, nstore, B, t, false, d_WU.p + (size_t)(t % nwr) * strideU, stream);
                 LaunchFftWindows(d_gT.p, histT, nstore, B, t, false, d_WT.p + (size_t)(t % nwr) * strideT, stream);
                 CheckB(hipEventRecord(fm, stream), "event");
-                LaunchConvFarFft(ngroups, d_grpRow0.p, d_grpRows.p, d_itemOff.p,
+                LaunchConvFarFft(ngroups, d_grpRowList.p, d_grpRows.p, d_itemOff.p,
                                  reinterpret_cast<const FftItem*>(d_items.p), d_H.p, d_WU.p, d_WT.p, strideU, strideT,
                                  nwr, t, B, d_Y.p, batchedFftMfma != 0, batchedFftPace, stream);
                 CheckB(hipEventRecord(fp, stream), "event");

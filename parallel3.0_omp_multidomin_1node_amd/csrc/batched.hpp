@@ -46,28 +46,29 @@ void LaunchConvFarB(PairGrid grid, const NodeRefs* refs, const long* off, const 
                     const double* FU, const double* FT, int nstore, int K0, int Tn, int maxn, int ne, int B,
                     double* hfar, int parts, hipStream_t s);
 // Spectral part (batched_fft.hip): raw lags >= kFftLagMin of every pair.  Row
-// groups: kFftRows consecutive source rows of one domain (grpDom, grpRow0,
-// grpRows).  Items: per group, the (field, sub-filter q >= 1) pairs with the
-// mask of rows whose band has that sub-filter and the offset of their filter
-// spectra, H[hoff + (slot * rows + k) * 30 + ch * 2 + ri] for the k-th row of
-// the mask (ch: the 9 values of -MTS, then the 6 of MGS).
+// groups: up to kFftRows source rows of one domain, near each other in space
+// (grpDom; rows grpRowList[g * kFftRows + r], r < grpRows[g]).  Items: per
+// group, the (field, sub-filter q >= 1) pairs with the mask of rows whose
+// band has that sub-filter and the offset of their filter spectra,
+// H[hoff + (slot * rows + k) * 30 + ch * 2 + ri] for the k-th row of the mask
+// (ch: the 9 values of -MTS, then the 6 of MGS).
 struct FftItem {
     int f;                    // field element within the domain
     int q;                    // sub-filter (lags 16 q .. 16 q + 15 of the filtered blocks)
     int cu, ct;               // the field's displacement / traction history columns
-    unsigned long long mask;  // rows of the group (bit r: row grpRow0 + r)
+    unsigned long long mask;  // rows of the group (bit r: row grpRowList[g * kFftRows + r])
     long hoff;                // offset of the item's filter spectra in H
 };
 static_assert(sizeof(FftItem) == 32, "FftItem is 32 bytes");
 // Count mode (items null): count[g] items and rowCount[g] rows summed over
 // them; qHist[q] += items and qHist[qcap + 1 + q] += rows of sub-filter q.
 void LaunchFftItems(PairGrid grid, const NodeRefs* refs, const int* kb, const int* len, int maxn, int ngroups,
-                    const int* grpDom, const int* grpRow0, const int* grpRows, const long* itemOff, const long* hOff,
+                    const int* grpDom, const int* grpRowList, const int* grpRows, const long* itemOff, const long* hOff,
                     long* count, long* rowCount, long* qHist, int qcap, FftItem* items, hipStream_t s);
 template <typename TB>
 void LaunchFftFilters(PairGrid grid, const long* off, const int* kb, const int* len, const TB* mts, const TB* mgs,
                       const unsigned short* mtsLo, const unsigned short* mgsLo, int ngroups, const int* grpDom,
-                      const int* grpRow0, const long* itemOff, long nitems, const FftItem* items, int maxn,
+                      const int* grpRowList, const long* itemOff, long nitems, const FftItem* items, int maxn,
                       double* H, hipStream_t s);
 // Spectra of the state window beta (states 16 beta - 30 .. 16 beta + 1; the
 // second half zero with partial) of every column:
@@ -79,7 +80,7 @@ void LaunchFftWindows(const double* g, long cols, int nstore, int B, int beta, b
 // kFftHPad values past its last item's spectra (read, never used).  mfma:
 // on the FP64 matrix cores, else with vector FMAs; pace > 0: a workgroup's
 // wavefronts meet at a barrier after every `pace` items of its list.
-void LaunchConvFarFft(int ngroups, const int* grpRow0, const int* grpRows, const long* itemOff, const FftItem* items,
+void LaunchConvFarFft(int ngroups, const int* grpRowList, const int* grpRows, const long* itemOff, const FftItem* items,
                       const double* H, const double* WU, const double* WT, long ringStrideU, long ringStrideT,
                       int nwr, int t, int B, double* Y, bool mfma, int pace, hipStream_t s);
 // Samples 16 .. 16 + Tn - 1 of the inverse transforms of Y into

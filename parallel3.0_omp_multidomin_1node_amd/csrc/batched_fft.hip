@@ -99,7 +99,7 @@ __device__ __forceinline__ bool QRange(int k0, int L, int maxn, int& qlo, int& q
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(64) k_fft_items(PairGrid grid, const NodeRefs* __restrict__ refs,
                                                   const int* __restrict__ kb, const int* __restrict__ len, int maxn,
-                                                  const int* __restrict__ grpDom, const int* __restrict__ grpRow0,
+                                                  const int* __restrict__ grpDom, const int* __restrict__ grpRowList,
                                                   const int* __restrict__ grpRows, const long* __restrict__ itemOff,
                                                   const long* __restrict__ hOff, long* __restrict__ count,
                                                   long* __restrict__ rowCount, long* __restrict__ qHist, int qcap,
@@ -107,9 +107,10 @@ __global__ void __launch_bounds__(64) k_fft_items(PairGrid grid, const NodeRefs*
 {
     __shared__ long pre[65], prr[65];
     const int g = blockIdx.x, lane = threadIdx.x;
-    const int d = grpDom[g], s0 = grpRow0[g], nr = grpRows[g];
+    const int d = grpDom[g], nr = grpRows[g];
+    const int* rows = grpRowList + (long)g * kFftRows;
     const int eb = grid.eleBegin[d], n = grid.eleCount[d];
-    const long rowPair0 = grid.pairBase[d] + (long)(s0 - eb) * n;
+    const long domPair0 = grid.pairBase[d];
     const long base = items ? itemOff[g] : 0, hbase = items ? hOff[g] : 0;
     long total = 0, rtotal = 0;
     for (int f0 = 0; f0 < n; f0 += 64) {
@@ -117,7 +118,7 @@ __global__ void __launch_bounds__(64) k_fft_items(PairGrid grid, const NodeRefs*
         int lo = 1 << 30, hi = -1;
         if (f < n)
             for (int r = 0; r < nr; ++r) {
-                const long p = rowPair0 + (long)r * n + f;
+                const long p = domPair0 + (long)(rows[r] - eb) * n + f;
                 int a, z;
                 if (QRange(kb[p], len[p], maxn, a, z)) {
                     lo = a < lo ? a : lo;
@@ -128,7 +129,7 @@ __global__ void __launch_bounds__(64) k_fft_items(PairGrid grid, const NodeRefs*
         auto maskOf = [&](int q) {
             unsigned long long m = 0;
             for (int r = 0; r < nr; ++r) {
-                const long p = rowPair0 + (long)r * n + f;
+                const long p = domPair0 + (long)(rows[r] - eb) * n + f;
                 int a, z;
                 if (QRange(kb[p], len[p], maxn, a, z) && q >= a && q <= z)
                     m |= 1ull << r;
@@ -186,10 +187,10 @@ __global__ void __launch_bounds__(64) k_fft_items(PairGrid grid, const NodeRefs*
 }
 
 void LaunchFftItems(PairGrid grid, const NodeRefs* refs, const int* kb, const int* len, int maxn, int ngroups,
-                    const int* grpDom, const int* grpRow0, const int* grpRows, const long* itemOff, const long* hOff,
+                    const int* grpDom, const int* grpRowList, const int* grpRows, const long* itemOff, const long* hOff,
                     long* count, long* rowCount, long* qHist, int qcap, FftItem* items, hipStream_t s)
 {
-    k_fft_items<<<(unsigned)ngroups, 64, 0, s>>>(grid, refs, kb, len, maxn, grpDom, grpRow0, grpRows, itemOff, hOff,
+    k_fft_items<<<(unsigned)ngroups, 64, 0, s>>>(grid, refs, kb, len, maxn, grpDom, grpRowList, grpRows, itemOff, hOff,
                                                  count, rowCount, qHist, qcap, items);
 }
 
@@ -260,7 +261,7 @@ template <typename TB>
 __global__ void __launch_bounds__(256) k_fft_filters(
     PairGrid grid, const long* __restrict__ off, const int* __restrict__ kb, const int* __restrict__ len,
     const TB* __restrict__ mts, const TB* __restrict__ mgs, const unsigned short* __restrict__ mtsLo,
-    const unsigned short* __restrict__ mgsLo, const int* __restrict__ grpDom, const int* __restrict__ grpRow0,
+    const unsigned short* __restrict__ mgsLo, const int* __restrict__ grpDom, const int* __restrict__ grpRowList,
     int ngroups, const long* __restrict__ itemOff, const FftItem* __restrict__ items, int maxn,
     double* __restrict__ H)
 {
@@ -282,9 +283,10 @@ __global__ void __launch_bounds__(256) k_fft_filters(
     const int g = glo;
     double* S = lds[wave];
     int* rw = rowOf[wave];
-    const int d = grpDom[g], s0 = grpRow0[g];
+    const int d = grpDom[g];
+    const int* rows = grpRowList + (long)g * kFftRows;
     const int eb = grid.eleBegin[d], n = grid.eleCount[d];
-    const long rowPair0 = grid.pairBase[d] + (long)(s0 - eb) * n;
+    const long domPair0 = grid.pairBase[d];
     {
         const FftItem item = items[it];
         const int f = item.f, q = item.q;
@@ -298,7 +300,7 @@ __global__ void __launch_bounds__(256) k_fft_filters(
                 const int kk = idx / NS, rem = idx - kk * NS;
                 const int i = rem / 15, v = rem - i * 15;
                 const bool row = k0 + kk < pc;
-                const long p = rowPair0 + (long)(row ? rw[k0 + kk] : 0) * n + f;
+                const long p = domPair0 + (long)(rows[row ? rw[k0 + kk] : 0] - eb) * n + f;
                 const int k0b = kb[p], L = len[p];
                 const int j = kFftTile * q - 1 + i;
                 const bool ok = row && j >= kFftLagMin && j <= maxn && j >= k0b && j < k0b + L;
@@ -360,13 +362,13 @@ __global__ void __launch_bounds__(256) k_fft_filters(
 template <typename TB>
 void LaunchFftFilters(PairGrid grid, const long* off, const int* kb, const int* len, const TB* mts, const TB* mgs,
                       const unsigned short* mtsLo, const unsigned short* mgsLo, int ngroups, const int* grpDom,
-                      const int* grpRow0, const long* itemOff, long nitems, const FftItem* items, int maxn,
+                      const int* grpRowList, const long* itemOff, long nitems, const FftItem* items, int maxn,
                       double* H, hipStream_t s)
 {
     if (nitems <= 0)
         return;
     k_fft_filters<TB><<<(unsigned)((nitems + 3) / 4), 256, 0, s>>>(grid, off, kb, len, mts, mgs, mtsLo, mgsLo, grpDom,
-                                                                   grpRow0, ngroups, itemOff, items, maxn, H);
+                                                                   grpRowList, ngroups, itemOff, items, maxn, H);
 }
 
 // ---------------------------------------------------------------------------
@@ -534,7 +536,7 @@ __device__ __forceinline__ void FftProducts(const FftItem* __restrict__ items, l
 }
 
 __global__ void __launch_bounds__(512) k_conv_far_fft(
-    const int* __restrict__ grpRow0, const int* __restrict__ grpRows, const long* __restrict__ itemOff,
+    const int* __restrict__ grpRowList, const int* __restrict__ grpRows, const long* __restrict__ itemOff,
     const FftItem* __restrict__ items, const double* __restrict__ H, const double* __restrict__ WU,
     const double* __restrict__ WT, long ringStrideU, long ringStrideT, int nwr, int t, int B, int pace,
     double* __restrict__ Y)
@@ -547,7 +549,8 @@ __global__ void __launch_bounds__(512) k_conv_far_fft(
     const int b = blockIdx.y * 64 + lane;
     const bool live = b < B;
     const int bb = live ? b : B - 1;
-    const int s0 = grpRow0[g], nr = grpRows[g];
+    const int* rows = grpRowList + (long)g * kFftRows;
+    const int nr = grpRows[g];
     const long i0 = itemOff[g], i1 = itemOff[g + 1];
     const int tRing = (t + 1) % nwr;
     double acc[8][3][2];
@@ -568,7 +571,7 @@ __global__ void __launch_bounds__(512) k_conv_far_fft(
     for (int r = 0; r < 8; ++r) {
         if (sh + r >= nr)
             break;
-        double* y = Y + (((long)(s0 + sh + r) * 16 + slot) * 6) * B + b;
+        double* y = Y + (((long)rows[sh + r] * 16 + slot) * 6) * B + b;
 #pragma unroll
         for (int c = 0; c < 3; ++c)
 #pragma unroll
@@ -737,7 +740,7 @@ __device__ __forceinline__ void FftProductsMf(const FftItem* __restrict__ items,
 }
 
 __global__ void __launch_bounds__(512) k_conv_far_fft_mf(
-    const int* __restrict__ grpRow0, const int* __restrict__ grpRows, const long* __restrict__ itemOff,
+    const int* __restrict__ grpRowList, const int* __restrict__ grpRows, const long* __restrict__ itemOff,
     const FftItem* __restrict__ items, const double* __restrict__ H, const double* __restrict__ WU,
     const double* __restrict__ WT, long ringStrideU, long ringStrideT, int nwr, int t, int B, int pace,
     double* __restrict__ Y)
@@ -746,7 +749,8 @@ __global__ void __launch_bounds__(512) k_conv_far_fft_mf(
     const int wave = WaveIdx(), lane = threadIdx.x & 63;
     const int sh = 8 * wave;
     const int b0 = blockIdx.y * 64;
-    const int s0 = grpRow0[g], nr = grpRows[g];
+    const int* rows = grpRowList + (long)g * kFftRows;
+    const int nr = grpRows[g];
     const long i0 = itemOff[g], i1 = itemOff[g + 1];
     const int tRing = (t + 1) % nwr;
     FftD4 acc[3][4];
@@ -774,21 +778,21 @@ __global__ void __launch_bounds__(512) k_conv_far_fft_mf(
             for (int q = 0; q < 4; ++q) {
                 const int o = lj + 4 * q, ri = o >> 3, r = o & 7;
                 if (sh + r < nr)
-                    Y[(((long)(s0 + sh + r) * 16 + slot) * 6 + c * 2 + ri) * B + bn] = acc[c][nt][q];
+                    Y[(((long)rows[sh + r] * 16 + slot) * 6 + c * 2 + ri) * B + bn] = acc[c][nt][q];
             }
         }
 }
 
-void LaunchConvFarFft(int ngroups, const int* grpRow0, const int* grpRows, const long* itemOff, const FftItem* items,
+void LaunchConvFarFft(int ngroups, const int* grpRowList, const int* grpRows, const long* itemOff, const FftItem* items,
                       const double* H, const double* WU, const double* WT, long ringStrideU, long ringStrideT,
                       int nwr, int t, int B, double* Y, bool mfma, int pace, hipStream_t s)
 {
     dim3 g((unsigned)ngroups * 16, (unsigned)((B + 63) / 64));
     if (mfma)
-        k_conv_far_fft_mf<<<g, 512, 0, s>>>(grpRow0, grpRows, itemOff, items, H, WU, WT, ringStrideU, ringStrideT, nwr,
+        k_conv_far_fft_mf<<<g, 512, 0, s>>>(grpRowList, grpRows, itemOff, items, H, WU, WT, ringStrideU, ringStrideT, nwr,
                                             t, B, pace, Y);
     else
-        k_conv_far_fft<<<g, 512, 0, s>>>(grpRow0, grpRows, itemOff, items, H, WU, WT, ringStrideU, ringStrideT, nwr, t,
+        k_conv_far_fft<<<g, 512, 0, s>>>(grpRowList, grpRows, itemOff, items, H, WU, WT, ringStrideU, ringStrideT, nwr, t,
                                          B, pace, Y);
 }
 

@@ -56,7 +56,7 @@ MEASURED_TRAFFIC = {("layered_2x2x4_h0.1", 400, 16, 2): 61703231633.9,
 HEADLINE_TRAFFIC = {}
 # timings entries that are counts, flags, flops or bytes rather than seconds
 NOT_SECONDS = {"conv_far_launches", "conv_spectral_launches", "conv_spectral_flops", "conv_spectral_bytes",
-               "spectral", "stream_slabs"}
+               "spectral", "stream_slabs", "quad_evals_t2", "quad_evals_t1u", "quad_evals_regular", "quad_records"}
 SECONDARY_BUDGET_S = 420.0  # secondary blocks run only while the bench has used less than this
 
 
@@ -101,6 +101,21 @@ def batched_far_flops_per_launch(counts, nstep, maxn, nrhs, tile=BATCH_TILE):
         total += 36.0 * nrhs * float((counts * n).sum())
         launches += 1
     return total / launches
+
+
+# Algorithmic FP64 flops of the history quadrature per evaluation (operation
+# counts of the closed forms the staged kernels evaluate, divisions and square
+# roots counted as one): a patch-wise T2 (entry, point) = the time-window
+# coefficients (40) + one 3x3 block of basis products (27 FMA = 54); a
+# patch-wise T1 + U (entry, point) = 40 + 54 + the U block (20); a regular
+# (entry, point) = both levels' coefficients (80) + their sum (3) + 54 + 20;
+# a staged (source, point) record = distance, powers, tensor scalars and the
+# basis tensors (114).
+QUAD_FLOPS = {"quad_evals_t2": 94, "quad_evals_t1u": 114, "quad_evals_regular": 157, "quad_records": 114}
+
+
+def quadrature_flops(tm):
+    return float(sum(QUAD_FLOPS[k] * tm[k] for k in QUAD_FLOPS))
 
 
 def conv_far_bytes_per_launch(counts, npairs, ne, nhist, nstep, tile):
@@ -230,6 +245,12 @@ def config2_report(tdbem, gmres_device, far_parts=0):
         os.remove(cache)
     tm = c.timings()
     counts, npairs = c.history_stats(w.nstep)
+    # the quadrature's evaluations, counted in one more assembly (not timed)
+    c.set_option("quad_count", 1)
+    c.assemble_history(w.nstep)
+    qc = c.timings()
+    c.set_option("quad_count", 0)
+    qflops = quadrature_flops(qc)
     dof = c.dofmap()["counts"]
     c.close()
     nhist = int(dof[7] + dof[8]) // 2
@@ -245,6 +266,13 @@ def config2_report(tdbem, gmres_device, far_parts=0):
             "assembled_blocks_per_s": round(2.0 * float(entries) / max(tm["history"], 1e-9), 1),
             "breakdown_s": {k: round(float(v), 4) for k, v in tm.items()
                             if k not in NOT_SECONDS and not k.startswith("adjoint_")},
+            "quadrature_roofline": {
+                "kernel": "k_history_pw + k_history_reg (history quadrature, one assembly)", "bound": "fp64 valu",
+                "achieved": round(qflops / tm["quadrature"] / 1e12, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(qflops / tm["quadrature"] / 1e12 / FP64_PEAK_TFLOPS, 4),
+                "flops_per_assembly": qflops, "time_s": round(tm["quadrature"], 4),
+                "evaluations": {k: qc[k] for k in ("quad_evals_t2", "quad_evals_t1u", "quad_evals_regular",
+                                                   "quad_records")}},
             "roofline": {"kernel": f"k_conv_far_q ({fp} wavefronts per row, 16 steps per launch)",
                          "bound": "hbm", "achieved": round(bpl / avg / 1e9, 2), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(bpl / avg / 1e9 / HBM_PEAK_GBS, 4),
@@ -516,9 +544,10 @@ def main():
         prod_avg = tm["conv_spectral_products"] / sl
         sflops = tm["conv_spectral_flops"] / sl
         out["roofline"] = {
-            "kernel": f"k_conv_far_fft (spectral products of the batched far part, raw lags >= 17: 16 steps x {B} "
-                      f"load cases per launch)",
-            "bound": "mfma", "pipe": "fp64 (vector FMA; MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s)",
+            "kernel": f"k_conv_far_fft_mf (spectral products of the batched far part on the matrix cores, raw lags "
+                      f">= 17: 16 steps x {B} load cases per launch; flops counted without the zero rows and bins)",
+            "bound": "mfma",
+            "pipe": "fp64 matrix cores (v_mfma_f64_16x16x4f64; FP64 matrix and vector peaks are both 78.6 TFLOP/s)",
             "achieved": round(sflops / prod_avg / 1e12, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": round(sflops / prod_avg / 1e12 / FP64_PEAK_TFLOPS, 4),
             "traffic": HEADLINE_TRAFFIC.get((w.name, w.nstep, B, args.mixed)),

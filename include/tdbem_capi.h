@@ -221,8 +221,10 @@ int tdbem_timings(tdbem_ctx* ctx, double* t8);
  * spectra, once per solve), t[17] its products kernel alone (device events)
  * with t[18] that kernel's algorithmic FP64 flops and t[19] its algorithmic
  * bytes, summed over the launches; for streamed history bands t[20] the slab
- * count and t[21] the far part with the slabs' regeneration.  Values past the
- * known ones are 0. */
+ * count and t[21] the far part with the slabs' regeneration; with option
+ * "quad_count" 1 the staged quadrature's evaluations: t[22] patch-wise T2
+ * (entry, point), t[23] patch-wise T1 + U, t[24] regular (entry, point),
+ * t[25] staged (source, point) records.  Values past the known ones are 0. */
 int tdbem_timings_n(tdbem_ctx* ctx, int n, double* t);
 
 /* Number of stored history band entries per lag j = 1..maxn

@@ -610,7 +610,7 @@ int tdbem_timings_n(tdbem_ctx* ctx, int n, double* t)
     if (!ctx || !t || n < 0)
         return Fail("tdbem_timings_n: invalid arguments");
     for (int i = 0; i < n; ++i)
-        t[i] = i < 22 ? ctx->c.timing[i] : 0.0;
+        t[i] = i < 26 ? ctx->c.timing[i] : 0.0;
     return 0;
 }
 
@@ -656,6 +656,10 @@ int tdbem_set_option(tdbem_ctx* ctx, const char* name, double value)
     }
     if (n == "near_per_entry") {
         ctx->c.nearPerEntry = value != 0.0;
+        return 0;
+    }
+    if (n == "quad_count") {
+        ctx->c.quadCount = value != 0.0;
         return 0;
     }
     if (n == "history_stream") {

@@ -1352,6 +1352,31 @@ __device__ __forceinline__ void AddU(const double* g, bool P, bool S, double ri,
 
 __device__ __forceinline__ bool WaveAny(bool v) { return __ballot(v) != 0; }
 
+__device__ __forceinline__ long long WaveSumLL(long long v)
+{
+    for (int o = 32; o > 0; o >>= 1)
+        v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Evaluation counts of the staged quadrature (qcount non-null): [0] patch-wise
+// T2 (entry, point) evaluations, [1] patch-wise T1 + U ones, [2] regular
+// (entry, point) ones, [3] staged (source, point) records; one atomic per
+// counter and wavefront.
+__device__ __forceinline__ void CountEvaluations(unsigned long long* qcount, long long t2, long long t1, long long rg,
+                                                 long long records, int lane)
+{
+    t2 = WaveSumLL(t2);
+    t1 = WaveSumLL(t1);
+    rg = WaveSumLL(rg);
+    if (lane == 0) {
+        atomicAdd(qcount + 0, (unsigned long long)t2);
+        atomicAdd(qcount + 1, (unsigned long long)t1);
+        atomicAdd(qcount + 2, (unsigned long long)rg);
+        atomicAdd(qcount + 3, (unsigned long long)records);
+    }
+}
+
 }  // namespace dev
 
 namespace dev {
@@ -1624,7 +1649,8 @@ __global__ void __launch_bounds__(64) k_history_reg(
     const double* __restrict__ selfT2, PairGrid grid, const long* __restrict__ coff, const long* __restrict__ off,
     const int* __restrict__ kb, const int* __restrict__ len, const unsigned char* __restrict__ cls, long npairs,
     long c0, long c1, TB* __restrict__ mts, TB* __restrict__ mgs, unsigned short* __restrict__ mtsLo,
-    unsigned short* __restrict__ mgsLo, unsigned char* __restrict__ alive, long ebase, int maxLag)
+    unsigned short* __restrict__ mgsLo, unsigned char* __restrict__ alive, long ebase, int maxLag,
+    unsigned long long* __restrict__ qcount)
 {
     __shared__ double sgeo[kNG2 * dev::kRec];
     using namespace dev;
@@ -1663,6 +1689,10 @@ __global__ void __launch_bounds__(64) k_history_reg(
     for (int i = 0; i < 6; ++i)
         accRU[i] = 0.0;
     bool rt = mine && (fl.f2 == 2 || fl.f1 == 2);
+    if (qcount) {
+        const bool any = WaveAny(rt);
+        CountEvaluations(qcount, 0, 0, rt ? kNG2 : 0, any ? kNG2 : 0, lane);
+    }
     if (WaveAny(rt)) {
         if (lane < kNG2)
             StageRecord(m, src, P.rg_pos[lane], P.rg_nor[lane], P.rg_w[lane], sgeo + lane * kRec);
@@ -1691,7 +1721,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
     const long* __restrict__ off, const int* __restrict__ kb, const int* __restrict__ len,
     const unsigned char* __restrict__ cls, TB* __restrict__ mts, TB* __restrict__ mgs,
     unsigned short* __restrict__ mtsLo, unsigned short* __restrict__ mgsLo, unsigned char* __restrict__ alive,
-    long ebase, int maxLag)
+    long ebase, int maxLag, unsigned long long* __restrict__ qcount)
 {
     // the patch radii are read only by the patch-mask pass, before the first
     // point record is staged, so they share the record buffer (12.1 KB of LDS
@@ -1794,6 +1824,11 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
     // regular parts first (the other judge picking the regular path; at most
     // one of r2 / r1 holds for a patch-wise entry), 8 points per half
     bool r2 = active && fl.f2 == 2, r1 = active && fl.f1 == 2, rt = r2 || r1;
+    if (qcount) {
+        const bool any = WaveAny(rt);
+        const long long rec = (long long)nseg * (kNPWG2 * __popcll(need) + (any ? kNG2 : 0));
+        CountEvaluations(qcount, kNPWG2 * __popcll(m2), kNPWG2 * __popcll(m1), rt ? kNG2 : 0, rec, lane);
+    }
     Span s1 = MakeSpan(m, 1, n), s2 = MakeSpan(m, 2, n);
     double accA[9], accB[9], accU[6];
     {
@@ -1852,37 +1887,38 @@ void LaunchHistoryReg(const ElemGeo* geo, const ElemPoints* pts, const NodeRefs*
                       PairGrid grid, const long* coff, const long* off, const int* kb, const int* len,
                       const unsigned char* cls, long npairs, long c0, long c1, TB* mts, TB* mgs,
                       unsigned short* mtsLo, unsigned short* mgsLo, unsigned char* alive, hipStream_t s, long ebase,
-                      int maxLag)
+                      int maxLag, unsigned long long* qcount)
 {
     long waves = c1 - c0;
     if (waves <= 0)
         return;
     k_history_reg<TB><<<(unsigned)waves, 64, 0, s>>>(geo, pts, refs, selfT2, grid, coff, off, kb, len, cls, npairs,
-                                                     c0, c1, mts, mgs, mtsLo, mgsLo, alive, ebase, maxLag);
+                                                     c0, c1, mts, mgs, mtsLo, mgsLo, alive, ebase, maxLag, qcount);
 }
 
 template <typename TB>
 void LaunchHistoryPw(const ElemGeo* geo, const ElemPoints* pts, const NodeRefs* refs, const double* selfT2,
                      PairGrid grid, const PwWave* waves, long w0, long w1, const long* off, const int* kb,
                      const int* len, const unsigned char* cls, TB* mts, TB* mgs, unsigned short* mtsLo,
-                     unsigned short* mgsLo, unsigned char* alive, hipStream_t s, long ebase, int maxLag)
+                     unsigned short* mgsLo, unsigned char* alive, hipStream_t s, long ebase, int maxLag,
+                     unsigned long long* qcount)
 {
     long n = w1 - w0;
     if (n <= 0)
         return;
     k_history_pw<TB><<<(unsigned)n, 64, 0, s>>>(geo, pts, refs, selfT2, grid, waves, w0, w1, off, kb, len, cls, mts,
-                                                mgs, mtsLo, mgsLo, alive, ebase, maxLag);
+                                                mgs, mtsLo, mgsLo, alive, ebase, maxLag, qcount);
 }
 
 #define TDBEM_HIST_INSTANTIATE(TB)                                                                                   \
     template void LaunchHistoryReg<TB>(const ElemGeo*, const ElemPoints*, const NodeRefs*, const double*, PairGrid,   \
                                        const long*, const long*, const int*, const int*, const unsigned char*, long,   \
                                        long, long, TB*, TB*, unsigned short*, unsigned short*, unsigned char*,         \
-                                       hipStream_t, long, int);                                                        \
+                                       hipStream_t, long, int, unsigned long long*);                                   \
     template void LaunchHistoryPw<TB>(const ElemGeo*, const ElemPoints*, const NodeRefs*, const double*, PairGrid,    \
                                       const PwWave*, long, long, const long*, const int*, const int*,                  \
                                       const unsigned char*, TB*, TB*, unsigned short*, unsigned short*,                \
-                                      unsigned char*, hipStream_t, long, int);
+                                      unsigned char*, hipStream_t, long, int, unsigned long long*);
 TDBEM_HIST_INSTANTIATE(double)
 TDBEM_HIST_INSTANTIATE(float)
 #undef TDBEM_HIST_INSTANTIATE

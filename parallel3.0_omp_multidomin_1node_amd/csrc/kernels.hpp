@@ -126,13 +126,16 @@ void LaunchHistoryReg(const ElemGeo* geo, const ElemPoints* pts, const NodeRefs*
                       PairGrid grid, const long* coff, const long* off, const int* kb, const int* len,
                       const unsigned char* cls, long npairs, long c0, long c1, TB* mts, TB* mgs,
                       unsigned short* mtsLo, unsigned short* mgsLo, unsigned char* alive, hipStream_t s,
-                      long ebase = 0, int maxLag = 1 << 30);
+                      long ebase = 0, int maxLag = 1 << 30, unsigned long long* qcount = nullptr);
 template <typename TB>
 void LaunchHistoryPw(const ElemGeo* geo, const ElemPoints* pts, const NodeRefs* refs, const double* selfT2,
                      PairGrid grid, const PwWave* waves, long w0, long w1, const long* off, const int* kb,
                      const int* len, const unsigned char* cls, TB* mts, TB* mgs, unsigned short* mtsLo,
                      unsigned short* mgsLo, unsigned char* alive, hipStream_t s, long ebase = 0,
-                     int maxLag = 1 << 30);
+                     int maxLag = 1 << 30, unsigned long long* qcount = nullptr);
+// qcount (nullable, 4 counters): evaluations of the two kernels, [0] patch-wise
+// T2 (entry, point), [1] patch-wise T1 + U (entry, point), [2] regular
+// (entry, point), [3] staged (source, point) records.
 
 // Step-0 blocks of the near pairs listed in pairIdx (column-major, ready for
 // AddBlock): out[2*i] and out[2*i+1] with flags.

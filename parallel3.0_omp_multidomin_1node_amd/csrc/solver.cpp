@@ -926,13 +926,20 @@ void Case::StagedQuadrature(double* mts, double* mgs, float* mts32, float* mgs32
     Check(hipEventCreate(&qb), "event");
     Check(hipEventRecord(qa, stream), "event");
     const long step = 1L << 20;  // wavefronts per launch
+    DevBuf<unsigned long long> d_qc;
+    if (quadCount) {
+        d_qc.alloc(4);
+        LaunchZero(reinterpret_cast<double*>(d_qc.p), 4, stream);
+    }
     auto launch = [&](auto* mts, auto* mgs) {
         for (long c0 = 0; c0 < chunks; c0 += step)
             LaunchHistoryReg(d_geo.p, d_pts.p, d_refs.p, d_t2.p, grid, d_coff.p, d_off.p, d_kb.p, d_len.p, cls,
-                             npairs, c0, std::min(chunks, c0 + step), mts, mgs, d_mtsLo.p, d_mgsLo.p, alive, stream);
+                             npairs, c0, std::min(chunks, c0 + step), mts, mgs, d_mtsLo.p, d_mgsLo.p, alive, stream,
+                             0, 1 << 30, d_qc.p);
         for (long w0 = 0; w0 < pwWaves; w0 += step)
             LaunchHistoryPw(d_geo.p, d_pts.p, d_refs.p, d_t2.p, grid, d_pwWaves.p, w0, std::min(pwWaves, w0 + step),
-                            d_off.p, d_kb.p, d_len.p, cls, mts, mgs, d_mtsLo.p, d_mgsLo.p, alive, stream);
+                            d_off.p, d_kb.p, d_len.p, cls, mts, mgs, d_mtsLo.p, d_mgsLo.p, alive, stream, 0, 1 << 30,
+                            d_qc.p);
     };
     if (mts32)
         launch(mts32, mgs32);
@@ -943,6 +950,14 @@ void Case::StagedQuadrature(double* mts, double* mgs, float* mts32, float* mgs32
     float ms = 0.f;
     Check(hipEventElapsedTime(&ms, qa, qb), "elapsed");
     timing[8] = ms * 1e-3;
+    for (int i = 0; i < 4; ++i)
+        timing[22 + i] = 0.0;
+    if (quadCount) {
+        unsigned long long qc[4];
+        Check(CopyOnStream(qc, d_qc.p, sizeof(qc), hipMemcpyDeviceToHost, stream), "evaluation counts");
+        for (int i = 0; i < 4; ++i)
+            timing[22 + i] = (double)qc[i];
+    }
     (void)hipEventDestroy(qa);
     (void)hipEventDestroy(qb);
 }

@@ -170,8 +170,9 @@ public:
     // convolution kernels, 7 their launches, 8 history quadrature kernels;
     // adjoint: 9 forward solve, 10 backward sweep, 11 step-0 and self-table
     // differences with the correlation store, 12 history-block derivatives
-    // streamed history: 20 band slabs, 21 their regeneration in the time loop (device events)
-    double timing[22] = {0};
+    // streamed history: 20 band slabs, 21 their regeneration in the time loop (device events);
+    // 22-25 the staged quadrature's evaluation counts (with quadCount)
+    double timing[26] = {0};
     bool convTiled = true;   // time-blocked convolution (false: one full pass per step)
     int convTile = 16;       // steps per far-part tile (8 or 16); fixes the near lists built by the cull
     bool adjointAnalytic = true;  // history-block derivatives analytic (false: operator differences)
@@ -188,6 +189,7 @@ public:
     bool noAllocMode = false;  // SolveNoAlloc in progress
     int gmresDevice = 1;       // FP64 GMRES: 0 host-driven, 1 device-driven hipGraphs, 2 device-driven launches
     int selfMode = -1;       // kernel of the current self tables: 1 wave per element, 0 reference order
+    bool quadCount = false;  // count the staged quadrature's evaluations (timing[22..25]; costs a little)
     bool quadStaged = true;  // wave-per-pair quadrature with LDS-staged point geometry (false: lane per entry)
     int mixed = 0;  // for the next history assembly: 0 FP64, 1 FP32 blocks, 2 FP32 + bf16 residual blocks
     // Streamed history (bands larger than the device budget): 1 always, 0

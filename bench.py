@@ -153,6 +153,19 @@ def batched_job(tdbem, w, d_known, d_lu, d_lt, nrhs, tol, mixed, gmres_device, k
     return info
 
 
+def batched_job_host(tdbem, w, cases, tol, mixed, gmres_device):
+    """One whole job of the batched form through the host-buffer entry point
+    (tdbem_solve_batched): the load histories cross PCIe in, the solutions out."""
+    t0 = time.perf_counter()
+    c = make_case(tdbem, w, mixed=mixed, gmres_device=gmres_device)
+    c.assemble_step0(10)
+    c.assemble_history(w.nstep)
+    lu, lt, it, fl, solved, times = c.solve_batched(cases, w.nstep, tol=tol)
+    wall = time.perf_counter() - t0
+    c.close()
+    return dict(wall=wall, solved=solved, lu=lu)
+
+
 def single_fp64_job(tdbem, w, d_known, d_lu, d_lt, tol, gmres_device):
     """One whole job of the plain FP64 form for one load case (device-resident I/O)."""
     t0 = time.perf_counter()
@@ -415,6 +428,8 @@ def main():
                          "FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--streamed", type=float, default=0.0, metavar="GB",
                     help="also run config 2 with its history streamed under this device budget (several minutes)")
+    ap.add_argument("--host-buffers", action="store_true",
+                    help="also time one batched job through the host-buffer entry point (PCIe-inclusive rate)")
     ap.add_argument("--models", action="store_true",
                     help="also run the 16-model report (about 1.3 TB of history bands in total; about a minute)")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
@@ -558,6 +573,15 @@ def main():
     else:
         lag_roof["traffic"] = HEADLINE_TRAFFIC.get((w.name, w.nstep, B, args.mixed))
         out["roofline"] = lag_roof
+    if args.host_buffers:
+        hb = batched_job_host(tdbem, w, cases, args.tol, args.mixed, args.gmres_device)
+        log(f"host-buffer job: {hb['wall']:.2f} s")
+        out["host_buffers"] = {"entry": "tdbem_solve_batched (load histories in and solutions out over PCIe)",
+                               "whole_job_s": round(hb["wall"], 3),
+                               "case_steps_per_s": round(B * w.nstep / hb["wall"], 3),
+                               "bytes_in": int(cases.nbytes), "bytes_out": int(2 * cases.nbytes),
+                               "all_solved": bool((hb["solved"] == w.nstep).all())}
+        del hb
     if not args.no_cpu_baseline:
         log(f"cpu baseline on {args.cpu_threads} threads and on 1 ...")
         out["cpu_baseline"] = cpu_baseline(args.cpu_threads, tdbem, args.tol)

@@ -276,7 +276,8 @@ int tdbem_history_stats(tdbem_ctx* ctx, long* counts, long* npairs);
  *   bands resident when they fit the budget and otherwise streams them:
  *   the blocks of lags <= 16 stay resident and the far part of every
  *   16-step tile regenerates the rest slab by slab (source-row ranges) with
- *   the lane-per-entry quadrature; 1 always streams, 0 refuses bands that do
+ *   the quadrature "quad_staged" selects (bit-identical to the resident
+ *   bands of the same option); 1 always streams, 0 refuses bands that do
  *   not fit; streamed cases solve with tdbem_solve / tdbem_solve_device and
  *   the default convolution only; "history_budget_gb" the device memory the
  *   history may take (0, the default: 92 % of the free memory). */

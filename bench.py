@@ -54,9 +54,10 @@ MEASURED_TRAFFIC = {("layered_2x2x4_h0.1", 400, 16, 2): 61703231633.9,
                     ("layered_2x2x4_h0.1", 400, 16, 4): 56938840066.6,
                     ("layered_2x2x4_h0.1", 400, 16, 8): 52872278677.8}
 # the spectral products kernel of the 64-case mixed + batched box (same calibration)
-HEADLINE_TRAFFIC = {("stacked_2x2x8_h0.125", 600, 64, 2): 403503734138.2}
+HEADLINE_TRAFFIC = {("stacked_2x2x8_h0.125", 600, 64, 2, 16): 393554456166.4}
 # timings entries that are counts, flags, flops or bytes rather than seconds
 NOT_SECONDS = {"conv_far_launches", "conv_spectral_launches", "conv_spectral_flops", "conv_spectral_bytes",
+               "spectral_levels", "spectral_level_stats",
                "spectral", "stream_slabs", "quad_evals_t2", "quad_evals_t1u", "quad_evals_regular", "quad_records"}
 SECONDARY_BUDGET_S = 420.0  # secondary blocks run only while the bench has used less than this
 
@@ -136,10 +137,12 @@ def conv_far_bytes_per_launch(counts, npairs, ne, nhist, nstep, tile):
     return total / launches
 
 
-def batched_job(tdbem, w, d_known, d_lu, d_lt, nrhs, tol, mixed, gmres_device, keep=False):
+def batched_job(tdbem, w, d_known, d_lu, d_lt, nrhs, tol, mixed, gmres_device, keep=False, fft_levels=0):
     """One whole job of the batched form with device-resident inputs and outputs."""
     t0 = time.perf_counter()
     c = make_case(tdbem, w, mixed=mixed, gmres_device=gmres_device)
+    if fft_levels:
+        c.set_option("batched_fft_levels", fft_levels)
     c.assemble_step0(10)
     entries = c.assemble_history(w.nstep)
     it, fl, solved, times = c.solve_batched_device(d_known, nrhs, w.nstep, d_lu, d_lt, tol=tol)
@@ -428,6 +431,8 @@ def main():
                          "FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--streamed", type=float, default=0.0, metavar="GB",
                     help="also run config 2 with its history streamed under this device budget (several minutes)")
+    ap.add_argument("--fft-levels", type=int, default=0,
+                    help="levels of the batch's spectral far part (0: the library's default)")
     ap.add_argument("--host-buffers", action="store_true",
                     help="also time one batched job through the host-buffer entry point (PCIe-inclusive rate)")
     ap.add_argument("--models", action="store_true",
@@ -475,14 +480,15 @@ def main():
     d_known = tdbem.DeviceArray.from_host(cases)
     d_lu, d_lt = tdbem.DeviceArray(cases.shape), tdbem.DeviceArray(cases.shape)
     for i in range(args.warmup):
-        r = batched_job(tdbem, w, d_known, d_lu, d_lt, B, args.tol, args.mixed, args.gmres_device)
+        r = batched_job(tdbem, w, d_known, d_lu, d_lt, B, args.tol, args.mixed, args.gmres_device,
+                        fft_levels=args.fft_levels)
         log(f"warmup {i}: {r['wall']:.2f} s")
     tdbem.synchronize()
     t0 = time.perf_counter()
     runs = []
     for i in range(args.steps):
         runs.append(batched_job(tdbem, w, d_known, d_lu, d_lt, B, args.tol, args.mixed, args.gmres_device,
-                                keep=(i == args.steps - 1)))
+                                keep=(i == args.steps - 1), fft_levels=args.fft_levels))
         log(f"step {i}: {runs[-1]['wall']:.2f} s")
     tdbem.synchronize()
     elapsed = time.perf_counter() - t0
@@ -505,6 +511,8 @@ def main():
     d1 = tdbem.DeviceArray(cases.shape[1:])
     o1u, o1t = tdbem.DeviceArray(cases.shape[1:]), tdbem.DeviceArray(cases.shape[1:])
     fp64_runs, eu, et = [], 0.0, 0.0
+    marks = sorted({k for k in (w.nstep // 4, w.nstep // 2, (3 * w.nstep) // 4, (9 * w.nstep) // 10, w.nstep) if k > 0})
+    prof_u = {k: 0.0 for k in marks}
     for j, b in enumerate(ek if ek else [0]):
         d1.upload(cases[b])
         r = single_fp64_job(tdbem, w, d1, o1u, o1t, args.tol, args.gmres_device)
@@ -513,6 +521,8 @@ def main():
             U, T = o1u.download(), o1t.download()
             eu = max(eu, float(np.abs(BU[b] - U).max() / np.abs(U).max()))
             et = max(et, float(np.abs(BT[b] - T).max() / np.abs(T).max()))
+            for k in marks:  # the largest difference over steps <= k
+                prof_u[k] = max(prof_u[k], float(np.abs(BU[b, :k + 1] - U[:k + 1]).max() / np.abs(U).max()))
         log(f"fp64 single job (case {b}): {r['wall']:.3f} s")
     del d1, o1u, o1t, BU, BT
     fp64_wall = float(np.mean([r["wall"] for r in fp64_runs[1:]] if len(fp64_runs) > 1 else [fp64_runs[0]["wall"]]))
@@ -542,6 +552,7 @@ def main():
                           "plain_fp64_breakdown_s": {k: round(float(v), 4) for k, v in ftm.items()
                                                      if k in ("history", "step0", "convolution", "gmres")},
                           "max_rel_err_u": eu, "max_rel_err_t": et, "error_cases": ek,
+                          "max_rel_err_u_up_to_step": {str(k): v for k, v in prof_u.items()},
                           "error_bound": 1e-6, "same_gmres_tol": args.tol},
         "all_solved": all_solved,
         "gmres_inner_iterations_mean": float(np.mean(last["iters"])),
@@ -556,19 +567,28 @@ def main():
                 "avg_launch_ms": round(1000 * far_avg, 4), "flops_per_launch": flops,
                 "launches": int(tm["conv_far_launches"])}
     if spectral:
-        sl = max(int(tm["conv_spectral_launches"]), 1)
-        prod_avg = tm["conv_spectral_products"] / sl
-        sflops = tm["conv_spectral_flops"] / sl
+        # the dominant kernel: the products of the level that takes longest
+        lvs = tm["spectral_level_stats"]
+        top = max(lvs, key=lambda v: v["products"])
+        sl = max(top["launches"], 1)
+        prod_avg = top["products"] / sl
+        sflops = top["flops"] / sl
+        L = top["L"]
         out["roofline"] = {
-            "kernel": f"k_conv_far_fft_mf (spectral products of the batched far part on the matrix cores, raw lags "
-                      f">= 17: 16 steps x {B} load cases per launch; flops counted without the zero rows and bins)",
+            "kernel": f"k_conv_far_fft_mf<{L}> (spectral products of the batched far part on the matrix cores, "
+                      f"sub-filters of {L} lags, {L} steps x {B} load cases per launch; flops counted without the zero "
+                      f"rows and bins)",
             "bound": "mfma",
             "pipe": "fp64 matrix cores (v_mfma_f64_16x16x4f64; FP64 matrix and vector peaks are both 78.6 TFLOP/s)",
             "achieved": round(sflops / prod_avg / 1e12, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": round(sflops / prod_avg / 1e12 / FP64_PEAK_TFLOPS, 4),
-            "traffic": HEADLINE_TRAFFIC.get((w.name, w.nstep, B, args.mixed)),
+            "traffic": HEADLINE_TRAFFIC.get((w.name, w.nstep, B, args.mixed, L)),
             "avg_launch_ms": round(1000 * prod_avg, 4), "flops_per_launch": round(sflops),
-            "algorithmic_bytes_per_launch": round(tm["conv_spectral_bytes"] / sl), "launches": sl}
+            "algorithmic_bytes_per_launch": round(top["bytes"] / sl), "launches": sl}
+        out["spectral_levels"] = [
+            {"L": v["L"], "launches": v["launches"], "products_s": round(v["products"], 4),
+             "avg_launch_ms": round(1000 * v["products"] / max(v["launches"], 1), 4),
+             "tflops": round(v["flops"] / max(v["products"], 1e-12) / 1e12, 3)} for v in lvs]
         out["lag_by_lag_far"] = lag_roof
     else:
         lag_roof["traffic"] = HEADLINE_TRAFFIC.get((w.name, w.nstep, B, args.mixed))

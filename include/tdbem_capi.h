@@ -215,16 +215,19 @@ int tdbem_timings(tdbem_ctx* ctx, double* t8);
  * alone (device events; t[2] also holds the cull and the wave packing);
  * t[9..12] the adjoint's parts (forward, backward, step-0 differences,
  * history derivatives); for tdbem_solve_batched t[13] the spectral far
- * part's kernels (window spectra and products), t[14] their launches and
- * t[15] 1 when the spectral form ran (t[6], t[7] then cover the lag-by-lag
- * far kernel of the lags <= 16), t[16] its set-up (item lists and filter
- * spectra, once per solve), t[17] its products kernel alone (device events)
- * with t[18] that kernel's algorithmic FP64 flops and t[19] its algorithmic
- * bytes, summed over the launches; for streamed history bands t[20] the slab
- * count and t[21] the far part with the slabs' regeneration; with option
- * "quad_count" 1 the staged quadrature's evaluations: t[22] patch-wise T2
- * (entry, point), t[23] patch-wise T1 + U, t[24] regular (entry, point),
- * t[25] staged (source, point) records.  Values past the known ones are 0. */
+ * part's kernels (window spectra, products, inverse transforms), t[14] its
+ * products launches and t[15] 1 when the spectral form ran (t[6], t[7] then
+ * cover the lag-by-lag far kernel of the lags <= 16), t[16] its set-up (item
+ * lists and filter spectra, once per solve), t[17] its products kernels
+ * alone (device events) with t[18] their algorithmic FP64 flops and t[19]
+ * their algorithmic bytes, summed over the launches; for streamed history
+ * bands t[20] the slab count and t[21] the far part with the slabs'
+ * regeneration; with option "quad_count" 1 the staged quadrature's
+ * evaluations: t[22] patch-wise T2 (entry, point), t[23] patch-wise T1 + U,
+ * t[24] regular (entry, point), t[25] staged (source, point) records; per
+ * level l < 4 of the spectral part t[26 + l] its products seconds, t[30 + l]
+ * their flops, t[34 + l] their launches, t[38 + l] their bytes, and t[42]
+ * the number of levels.  Values past the known ones are 0. */
 int tdbem_timings_n(tdbem_ctx* ctx, int n, double* t);
 
 /* Number of stored history band entries per lag j = 1..maxn
@@ -268,6 +271,10 @@ int tdbem_history_stats(tdbem_ctx* ctx, long* counts, long* npairs);
  *   "batched_fft" -1 (default) tdbem_solve_batched takes the raw lags >= 17
  *   of its far part in the frequency domain (16-lag sub-filters, 32-point
  *   spectra, 16-step tiles) from 16 load cases on, 1 always, 0 never;
+ *   "batched_fft_levels" 1 (default) .. 3 splits those lags into levels
+ *   of sub-filter length 16, 32, 64 (lags 17-32 at 16, 33-64 at 32, the
+ *   last level the rest): fewer multiply-adds for the long lags, more
+ *   rounding in each product (a 2L-point spectrum sums more samples);
  *   "batched_fft_mfma" 1 (default) its spectral products on the FP64
  *   matrix cores, 0 with vector FMAs; "batched_fft_pace" n > 0 makes a
  *   workgroup's wavefronts meet at a barrier every n items (0, the default:

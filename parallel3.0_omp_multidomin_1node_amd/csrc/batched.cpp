@@ -144,15 +144,31 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
     const long n = 3 * ne;
     const int nstore = nstep + 1;
     const long histU = dof.historyUBlockCount, histT = dof.historyTBlockCount;
-    // spectral part: row groups, their (field, sub-filter) items and filter
-    // spectra, and the window spectra (a ring over the longest band's sub-filters)
+    // spectral part: row groups (shared by the levels); per level its (field,
+    // sub-filter) items, filter spectra and window-spectra ring
     DevBuf<int> d_grpDom, d_grpRowList, d_grpRows;
-    DevBuf<long> d_itemOff, d_hOff, d_items, d_cnt;
-    DevBuf<double> d_H, d_WU, d_WT, d_Y, d_hfar16;
-    int ngroups = 0, nwr = 1;
-    long strideU = 0, strideT = 0;
-    std::vector<long> qItemsUpTo(1, 0), qRowsUpTo(1, 0);
+    DevBuf<double> d_Y, d_hfar16;
+    struct FftLevel {
+        int L = kFftTile, lagLo = kFftLagMin, lagHi = 2 * kFftTile, nwr = 1, qcap = 1, K0 = 0;
+        long nitems = 0, hvals = 0, strideU = 0, strideT = 0;
+        std::vector<long> ioff, hoff, qItemsUpTo, qRowsUpTo;
+        DevBuf<long> itemOff, hOff, items;
+        DevBuf<double> H, WU, WT, hfar;
+        double prodTime = 0.0, flops = 0.0, bytes = 0.0;
+        int launches = 0;
+        bool ran = false;
+    } lv[kFftMaxLevels];
+    int ngroups = 0, nl = 0;
+    for (int i = 26; i < 43; ++i)
+        timing[i] = 0.0;
     timing[16] = 0.0;
+    if (fft) {
+        // levels: sub-filter length 16 << l, each with lags of its own
+        for (int l = 0; l < std::min(batchedFftLevels, kFftMaxLevels); ++l)
+            if ((kFftTile << l) + 1 <= maxn)
+                nl = l + 1;
+        fft = nl > 0;
+    }
     if (fft) {
         const double fs0 = NowB();
         // groups of kFftRows source rows of one domain in the Z-order (Morton)
@@ -192,44 +208,59 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
         d_grpDom.upload(gd.data(), gd.size(), stream);
         d_grpRowList.upload(gl.data(), gl.size(), stream);
         d_grpRows.upload(gr.data(), gr.size(), stream);
-        // per group: items, rows; per sub-filter q <= qcap: items, rows
-        const int qcap = (maxn + 1) / kFftTile;
-        const size_t ncnt = 2 * (size_t)ngroups + 2 * (size_t)(qcap + 1);
-        d_cnt.alloc(ncnt);
-        LaunchZero(reinterpret_cast<double*>(d_cnt.p), (long)ncnt, stream);
-        LaunchFftItems(grid, d_refs.p, d_kb.p, d_len.p, maxn, ngroups, d_grpDom.p, d_grpRowList.p, d_grpRows.p, nullptr,
-                       nullptr, d_cnt.p, d_cnt.p + ngroups, d_cnt.p + 2 * ngroups, qcap, nullptr, stream);
-        std::vector<long> cnt(ncnt), ioff((size_t)ngroups + 1, 0), hoff((size_t)ngroups + 1, 0);
-        CheckB(CopyOnStream(cnt.data(), d_cnt.p, sizeof(long) * cnt.size(), hipMemcpyDeviceToHost, stream), "items");
-        for (int gi = 0; gi < ngroups; ++gi) {
-            ioff[(size_t)gi + 1] = ioff[(size_t)gi] + cnt[(size_t)gi];
-            hoff[(size_t)gi + 1] = hoff[(size_t)gi] + cnt[(size_t)ngroups + gi] * 16 * 30;
-        }
-        // items and rows with sub-filter <= q, for the per-launch work of tile t (q <= t + 1)
-        qItemsUpTo.assign((size_t)qcap + 1, 0);
-        qRowsUpTo.assign((size_t)qcap + 1, 0);
-        for (int q = 0; q <= qcap; ++q) {
-            const size_t i = 2 * (size_t)ngroups + (size_t)q, r = i + (size_t)qcap + 1;
-            qItemsUpTo[(size_t)q] = (q ? qItemsUpTo[(size_t)q - 1] : 0) + cnt[i];
-            qRowsUpTo[(size_t)q] = (q ? qRowsUpTo[(size_t)q - 1] : 0) + cnt[r];
-        }
-        const long nitems = ioff.back(), hvals = hoff.back();
         HostBands();
-        int qmax = 1;
-        for (long pp = 0; pp < npairs; ++pp) {
-            const int L = h_len[(size_t)pp], k0 = h_kb[(size_t)pp];
-            const int jhi = std::min(k0 + L - 1, maxn);
-            if (L > 0 && std::max(k0, kFftLagMin) <= jhi)
-                qmax = std::max(qmax, (jhi + 1) / kFftTile);
+        double bytes = 0.0;
+        for (int l = 0; l < nl; ++l) {
+            FftLevel& v = lv[l];
+            v.L = kFftTile << l;
+            v.lagLo = v.L + 1;
+            v.lagHi = l == nl - 1 ? maxn : 2 * v.L;
+            // per group: items, rows; per sub-filter q <= qcap: items, rows
+            v.qcap = (maxn - 1) / v.L + 1;
+            const size_t ncnt = 2 * (size_t)ngroups + 2 * (size_t)(v.qcap + 1);
+            DevBuf<long> d_cnt;
+            d_cnt.alloc(ncnt);
+            LaunchZero(reinterpret_cast<double*>(d_cnt.p), (long)ncnt, stream);
+            LaunchFftItems(grid, d_refs.p, d_kb.p, d_len.p, maxn, v.L, v.lagLo, v.lagHi, ngroups, d_grpDom.p,
+                           d_grpRowList.p, d_grpRows.p, nullptr, nullptr, d_cnt.p, d_cnt.p + ngroups,
+                           d_cnt.p + 2 * ngroups, v.qcap, nullptr, stream);
+            std::vector<long> cnt(ncnt);
+            CheckB(CopyOnStream(cnt.data(), d_cnt.p, sizeof(long) * cnt.size(), hipMemcpyDeviceToHost, stream),
+                   "items");
+            v.ioff.assign((size_t)ngroups + 1, 0);
+            v.hoff.assign((size_t)ngroups + 1, 0);
+            for (int gi = 0; gi < ngroups; ++gi) {
+                v.ioff[(size_t)gi + 1] = v.ioff[(size_t)gi] + cnt[(size_t)gi];
+                v.hoff[(size_t)gi + 1] = v.hoff[(size_t)gi] + cnt[(size_t)ngroups + gi] * v.L * 30;
+            }
+            // items and rows with sub-filter <= q, for the per-launch work of tile t (q <= t + 1)
+            v.qItemsUpTo.assign((size_t)v.qcap + 1, 0);
+            v.qRowsUpTo.assign((size_t)v.qcap + 1, 0);
+            for (int q = 0; q <= v.qcap; ++q) {
+                const size_t i = 2 * (size_t)ngroups + (size_t)q, r = i + (size_t)v.qcap + 1;
+                v.qItemsUpTo[(size_t)q] = (q ? v.qItemsUpTo[(size_t)q - 1] : 0) + cnt[i];
+                v.qRowsUpTo[(size_t)q] = (q ? v.qRowsUpTo[(size_t)q - 1] : 0) + cnt[r];
+            }
+            v.nitems = v.ioff.back();
+            v.hvals = v.hoff.back();
+            // window ring: the level's largest sub-filter over the bands
+            int qmax = 1;
+            for (long pp = 0; pp < npairs; ++pp) {
+                const int len = h_len[(size_t)pp], k0 = h_kb[(size_t)pp];
+                const int jhi = std::min(std::min(k0 + len - 1, maxn), v.lagHi);
+                if (len > 0 && std::max(k0, v.lagLo) <= jhi)
+                    qmax = std::max(qmax, (jhi - 1) / v.L);
+            }
+            v.nwr = qmax;
+            v.strideU = histU * 6 * (long)v.L * B;
+            v.strideT = histT * 6 * (long)v.L * B;
+            bytes += 8.0 * ((double)v.hvals + kFftHPad + (double)(v.strideU + v.strideT) * v.nwr) +
+                     32.0 * (double)v.nitems + (l > 0 ? 8.0 * (double)v.L * (double)n * B : 0.0);
         }
-        nwr = qmax;
-        strideU = histU * 96 * (long)B;
-        strideT = histT * 96 * (long)B;
+        const int Lmax = kFftTile << (nl - 1);
         // the spectral part's buffers, and what the time loop allocates after them
         const double nB = (double)n * B, ns = (double)nstep + 1;
-        const double bytes =
-            8.0 * ((double)hvals + (double)(strideU + strideT) * nwr + 16.0 * nB + (double)ne * 96 * B) +
-            32.0 * (double)nitems;
+        bytes += 8.0 * ((double)ne * 6 * Lmax * B + (double)kFftTile * nB);
         const double rest = 8.0 * (nB * (2.0 * restart + 6) + (double)(histU + histT) * 3 * B * (2 * ns + 1) +
                                    3.0 * std::max<long>(1, dof.knownBlockCount) * B +
                                    std::max(1, batchedMfma ? batchedFarParts : 0) * kBatchTile * nB +
@@ -239,26 +270,34 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
         freeB += PooledBytesCached();
         if (bytes + rest > 0.92 * (double)freeB) {
             fft = false;  // the filter spectra do not fit: every lag lag-by-lag
+            nl = 0;
         } else {
-            d_itemOff.upload(ioff.data(), ioff.size(), stream);
-            d_hOff.upload(hoff.data(), hoff.size(), stream);
-            d_items.alloc(4 * (size_t)std::max<long>(1, nitems));  // 32 bytes per item
-            FftItem* items = reinterpret_cast<FftItem*>(d_items.p);
-            LaunchFftItems(grid, d_refs.p, d_kb.p, d_len.p, maxn, ngroups, d_grpDom.p, d_grpRowList.p, d_grpRows.p,
-                           d_itemOff.p, d_hOff.p, nullptr, nullptr, nullptr, 0, items, stream);
-            d_H.alloc((size_t)hvals + kFftHPad);
-            LaunchZero(d_H.p + hvals, kFftHPad, stream);
-            if (historyMixed)
-                LaunchFftFilters(grid, d_off.p, d_kb.p, d_len.p, (const float*)d_mts32.p, (const float*)d_mgs32.p,
-                                 historySplit ? d_mtsLo.p : nullptr, historySplit ? d_mgsLo.p : nullptr, ngroups,
-                                 d_grpDom.p, d_grpRowList.p, d_itemOff.p, nitems, items, maxn, d_H.p, stream);
-            else
-                LaunchFftFilters(grid, d_off.p, d_kb.p, d_len.p, (const double*)d_mts.p, (const double*)d_mgs.p,
-                                 nullptr, nullptr, ngroups, d_grpDom.p, d_grpRowList.p, d_itemOff.p, nitems, items,
-                                 maxn, d_H.p, stream);
-            d_WU.alloc((size_t)strideU * nwr);
-            d_WT.alloc((size_t)strideT * nwr);
-            d_Y.alloc((size_t)ne * 96 * B);
+            for (int l = 0; l < nl; ++l) {
+                FftLevel& v = lv[l];
+                v.itemOff.upload(v.ioff.data(), v.ioff.size(), stream);
+                v.hOff.upload(v.hoff.data(), v.hoff.size(), stream);
+                v.items.alloc(4 * (size_t)std::max<long>(1, v.nitems));  // 32 bytes per item
+                FftItem* items = reinterpret_cast<FftItem*>(v.items.p);
+                LaunchFftItems(grid, d_refs.p, d_kb.p, d_len.p, maxn, v.L, v.lagLo, v.lagHi, ngroups, d_grpDom.p,
+                               d_grpRowList.p, d_grpRows.p, v.itemOff.p, v.hOff.p, nullptr, nullptr, nullptr, 0, items,
+                               stream);
+                v.H.alloc((size_t)v.hvals + kFftHPad);
+                LaunchZero(v.H.p + v.hvals, kFftHPad, stream);
+                if (historyMixed)
+                    LaunchFftFilters(grid, d_off.p, d_kb.p, d_len.p, (const float*)d_mts32.p, (const float*)d_mgs32.p,
+                                     historySplit ? d_mtsLo.p : nullptr, historySplit ? d_mgsLo.p : nullptr, ngroups,
+                                     d_grpDom.p, d_grpRowList.p, v.itemOff.p, v.nitems, items, maxn, v.L, v.lagLo,
+                                     v.lagHi, v.H.p, stream);
+                else
+                    LaunchFftFilters(grid, d_off.p, d_kb.p, d_len.p, (const double*)d_mts.p, (const double*)d_mgs.p,
+                                     nullptr, nullptr, ngroups, d_grpDom.p, d_grpRowList.p, v.itemOff.p, v.nitems,
+                                     items, maxn, v.L, v.lagLo, v.lagHi, v.H.p, stream);
+                v.WU.alloc((size_t)v.strideU * v.nwr);
+                v.WT.alloc((size_t)v.strideT * v.nwr);
+                if (l > 0)
+                    v.hfar.alloc((size_t)v.L * (size_t)n * B);
+            }
+            d_Y.alloc((size_t)ne * 6 * Lmax * B);
             d_hfar16.alloc((size_t)kFftTile * (size_t)n * B);
             CheckB(hipStreamSynchronize(stream), "spectral set-up");
         }
@@ -315,11 +354,29 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
     std::vector<char> alive((size_t)B, 1);
     for (int b = 0; b < B; ++b)
         solved[b] = 0;
-    double convTime = 0.0, gmresTime = 0.0, farTime = 0.0, fftTime = 0.0, fftProdTime = 0.0, fftFlops = 0.0,
-           fftBytes = 0.0;
+    double convTime = 0.0, gmresTime = 0.0, farTime = 0.0, fftTime = 0.0, fftProdTime = 0.0;
     int farLaunches = 0, fftLaunches = 0, K016 = 0;
+    // timings of the spectral part: totals and per level
+    auto fftTimings = [&]() {
+        double fl = 0.0, by = 0.0;
+        for (int l = 0; l < nl; ++l) {
+            fl += lv[l].flops;
+            by += lv[l].bytes;
+            timing[26 + l] = lv[l].prodTime;
+            timing[30 + l] = lv[l].flops;
+            timing[34 + l] = lv[l].launches;
+            timing[38 + l] = lv[l].bytes;
+        }
+        timing[13] = fftTime;
+        timing[14] = fftLaunches;
+        timing[15] = fft ? 1.0 : 0.0;
+        timing[17] = fftProdTime;
+        timing[18] = fl;
+        timing[19] = by;
+        timing[42] = nl;
+    };
     struct EvGuard {
-        hipEvent_t e[4] = {nullptr, nullptr, nullptr, nullptr};
+        hipEvent_t e[4 + 2 * kFftMaxLevels] = {};
         ~EvGuard()
         {
             for (hipEvent_t x : e)
@@ -329,7 +386,7 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
     } evg;
     for (hipEvent_t& x : evg.e)
         CheckB(hipEventCreate(&x), "event");
-    hipEvent_t fa = evg.e[0], fb = evg.e[1], fm = evg.e[2], fp = evg.e[3];
+    hipEvent_t fa = evg.e[0], fb = evg.e[1];
     int K0 = 0;
     for (int K = 1; K <= nstep; ++K) {
         LaunchKnownB(d_refs.p, bd, (int)ne, nstore, K, B, d_known.p, stream);
@@ -337,36 +394,67 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
         if (K > 1) {
             double c0 = NowB();
             if (fft && (K == 2 || K - K016 >= kFftTile)) {
+                // every level whose tile starts here (the longest first), then
+                // level 0, whose inverse adds the other levels' samples of
+                // these 16 steps
                 K016 = K;
-                const int Tn = std::min(kFftTile, nstep - K016 + 1);
-                const int t = (K016 - 2) / kFftTile;
                 CheckB(hipEventRecord(fa, stream), "event");
-                LaunchFftWindows(d_gU.p, histU, nstore, B, t, false, d_WU.p + (size_t)(t % nwr) * strideU, stream);
-                LaunchFftWindows(d_gT.p, histT, nstore, B, t, false, d_WT.p + (size_t)(t % nwr) * strideT, stream);
-                CheckB(hipEventRecord(fm, stream), "event");
-                LaunchConvFarFft(ngroups, d_grpRowList.p, d_grpRows.p, d_itemOff.p,
-                                 reinterpret_cast<const FftItem*>(d_items.p), d_H.p, d_WU.p, d_WT.p, strideU, strideT,
-                                 nwr, t, B, d_Y.p, batchedFftMfma != 0, batchedFftPace, stream);
-                CheckB(hipEventRecord(fp, stream), "event");
-                LaunchFftInverse(d_Y.p, (int)ne, B, Tn, d_hfar16.p, stream);
+                FftAdds adds{};
+                adds.n = 0;
+                int ran = 0;
+                for (int l = nl - 1; l >= 0; --l) {
+                    FftLevel& v = lv[l];
+                    v.ran = false;
+                    if ((K - 2) % v.L == 0) {
+                        v.K0 = K;
+                        v.ran = true;
+                        const int t = (K - 2) / v.L;
+                        const int Tn = std::min(v.L, nstep - K + 1);
+                        LaunchFftWindows(d_gU.p, histU, nstore, B, v.L, t, v.WU.p + (size_t)(t % v.nwr) * v.strideU,
+                                         stream);
+                        LaunchFftWindows(d_gT.p, histT, nstore, B, v.L, t, v.WT.p + (size_t)(t % v.nwr) * v.strideT,
+                                         stream);
+                        CheckB(hipEventRecord(evg.e[4 + 2 * l], stream), "event");
+                        LaunchConvFarFft(ngroups, d_grpRowList.p, d_grpRows.p, v.itemOff.p,
+                                         reinterpret_cast<const FftItem*>(v.items.p), v.H.p, v.WU.p, v.WT.p, v.strideU,
+                                         v.strideT, v.nwr, v.L, t, B, d_Y.p, batchedFftMfma != 0, batchedFftPace,
+                                         stream);
+                        CheckB(hipEventRecord(evg.e[5 + 2 * l], stream), "event");
+                        const FftAdds none{};
+                        if (l > 0)
+                            LaunchFftInverse(d_Y.p, (int)ne, B, v.L, Tn, v.hfar.p, none, stream);
+                        else
+                            LaunchFftInverse(d_Y.p, (int)ne, B, v.L, Tn, d_hfar16.p, adds, stream);
+                        ++ran;
+                        // algorithmic work of the products: per row of an active
+                        // item (sub-filter <= t + 1) and load case, L - 1 slots of
+                        // 18 complex multiply-adds (8 flops each, in 4 FMAs) and
+                        // slot 0's 36 real ones; bytes: its filter spectra (L x
+                        // 30 values) once, the window spectra (L slots x 12
+                        // values per load case) once per item and the summed
+                        // spectra written once
+                        const size_t qa = (size_t)std::min(t + 1, v.qcap);
+                        const double rowsA = (double)v.qRowsUpTo[qa], itemsA = (double)v.qItemsUpTo[qa];
+                        v.flops += rowsA * ((v.L - 1) * 144.0 + 72.0) * B;
+                        v.bytes += rowsA * v.L * 30 * 8 + itemsA * v.L * 12 * 8.0 * B + (double)ne * 6 * v.L * B * 8;
+                        ++v.launches;
+                    }
+                    if (l > 0)
+                        adds.p[adds.n++] = v.hfar.p + (size_t)(K - v.K0) * (size_t)n * B;
+                }
                 CheckB(hipEventRecord(fb, stream), "event");
                 CheckB(hipEventSynchronize(fb), "far part");
-                float ms = 0.f, msp = 0.f;
+                float ms = 0.f;
                 CheckB(hipEventElapsedTime(&ms, fa, fb), "elapsed");
-                CheckB(hipEventElapsedTime(&msp, fm, fp), "elapsed");
                 fftTime += ms * 1e-3;
-                fftProdTime += msp * 1e-3;
-                ++fftLaunches;
-                // algorithmic work of the products: per row of an active item
-                // (sub-filter <= t + 1) and load case, 15 slots of 18 complex
-                // multiply-adds (8 flops each, in 4 FMAs) and slot 0's 36 real
-                // ones; bytes: its filter spectra (16 x 30 values) once, the
-                // window spectra (16 slots x 12 values per load case) once per
-                // item and the summed spectra written once
-                const size_t qa = (size_t)std::min(t + 1, (int)qRowsUpTo.size() - 1);
-                const double rowsA = (double)qRowsUpTo[qa], itemsA = (double)qItemsUpTo[qa];
-                fftFlops += rowsA * (15.0 * 144.0 + 72.0) * B;
-                fftBytes += rowsA * 16 * 30 * 8 + itemsA * 16 * 12 * 8.0 * B + (double)ne * 96 * B * 8;
+                for (int l = 0; l < nl; ++l)
+                    if (lv[l].ran) {
+                        float msp = 0.f;
+                        CheckB(hipEventElapsedTime(&msp, evg.e[4 + 2 * l], evg.e[5 + 2 * l]), "elapsed");
+                        lv[l].prodTime += msp * 1e-3;
+                        fftProdTime += msp * 1e-3;
+                    }
+                fftLaunches += ran;
             }
             if (K == 2 || K - K0 >= kBatchTile) {
                 int prev = K0;
@@ -440,12 +528,7 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
         timing[5] = gmresTime;
         timing[6] = farTime;
         timing[7] = farLaunches;
-        timing[13] = fftTime;
-        timing[14] = fftLaunches;
-        timing[15] = fft ? 1.0 : 0.0;
-        timing[17] = fftProdTime;
-        timing[18] = fftFlops;
-        timing[19] = fftBytes;
+        fftTimings();
         bw.release();
         return;
     }
@@ -484,12 +567,7 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
     timing[5] = gmresTime;
     timing[6] = farTime;
     timing[7] = farLaunches;
-    timing[13] = fftTime;
-    timing[14] = fftLaunches;
-    timing[15] = fft ? 1.0 : 0.0;
-    timing[17] = fftProdTime;
-    timing[18] = fftFlops;
-    timing[19] = fftBytes;
+    fftTimings();
     (void)t0;
     bw.release();
 }

@@ -12,7 +12,7 @@ namespace tdbem {
 constexpr int kBatchTile = 8;  // steps per far-part tile of the batched convolution
 constexpr int kBatchFarPartsMax = 16;  // column parts of the matrix-core far part
 constexpr int kBatchRedBlocks = 256;   // most reduction blocks of the batched dot products
-constexpr int kFftTile = 16;           // steps per tile (and lags per sub-filter) of the spectral far part
+constexpr int kFftTile = 16;           // steps per tile (and lags per sub-filter) of the spectral part's level 0
 constexpr int kFftRows = 64;           // source rows per group of the spectral far part (8 per wavefront)
 constexpr int kFftHPad = 256;          // values past the filter spectra that the products kernel may read
 constexpr int kFftLagMin = 17;         // the spectral part takes the raw lags >= 17, the lag-by-lag kernels the rest
@@ -45,47 +45,59 @@ void LaunchConvFarB(PairGrid grid, const NodeRefs* refs, const long* off, const 
                     const TB* mts, const TB* mgs, const unsigned short* mtsLo, const unsigned short* mgsLo,
                     const double* FU, const double* FT, int nstore, int K0, int Tn, int maxn, int ne, int B,
                     double* hfar, int parts, hipStream_t s);
-// Spectral part (batched_fft.hip): raw lags >= kFftLagMin of every pair.  Row
-// groups: up to kFftRows source rows of one domain, near each other in space
-// (grpDom; rows grpRowList[g * kFftRows + r], r < grpRows[g]).  Items: per
+// Spectral part (batched_fft.hip): the raw lags >= kFftLagMin of every pair,
+// split into levels l = 0 .. levels - 1 of sub-filter length L = 16 << l
+// (level l < last: raw lags L + 1 .. 2L; last level: L + 1 .. maxn; sub-
+// filter q of a level: raw lags L q + 1 .. L q + L).  Row groups: up to
+// kFftRows source rows of one domain, near each other in space (grpDom; rows
+// grpRowList[g * kFftRows + r], r < grpRows[g]).  Items of a level: per
 // group, the (field, sub-filter q >= 1) pairs with the mask of rows whose
-// band has that sub-filter and the offset of their filter spectra,
-// H[hoff + (slot * rows + k) * 30 + ch * 2 + ri] for the k-th row of the mask
-// (ch: the 9 values of -MTS, then the 6 of MGS).
+// band has that sub-filter in the level's range and the offset of their
+// filter spectra, H[hoff + (slot * rows + k) * 30 + ch * 2 + ri] for the
+// k-th row of the mask, slot < L (ch: the 9 values of -MTS, then the 6 of
+// MGS).
 struct FftItem {
     int f;                    // field element within the domain
-    int q;                    // sub-filter (lags 16 q .. 16 q + 15 of the filtered blocks)
+    int q;                    // sub-filter (raw lags L q + 1 .. L q + L)
     int cu, ct;               // the field's displacement / traction history columns
     unsigned long long mask;  // rows of the group (bit r: row grpRowList[g * kFftRows + r])
     long hoff;                // offset of the item's filter spectra in H
 };
 static_assert(sizeof(FftItem) == 32, "FftItem is 32 bytes");
+constexpr int kFftMaxLevels = 3;  // sub-filter lengths 16, 32, 64
 // Count mode (items null): count[g] items and rowCount[g] rows summed over
 // them; qHist[q] += items and qHist[qcap + 1 + q] += rows of sub-filter q.
-void LaunchFftItems(PairGrid grid, const NodeRefs* refs, const int* kb, const int* len, int maxn, int ngroups,
-                    const int* grpDom, const int* grpRowList, const int* grpRows, const long* itemOff, const long* hOff,
-                    long* count, long* rowCount, long* qHist, int qcap, FftItem* items, hipStream_t s);
+void LaunchFftItems(PairGrid grid, const NodeRefs* refs, const int* kb, const int* len, int maxn, int L, int lagLo,
+                    int lagHi, int ngroups, const int* grpDom, const int* grpRowList, const int* grpRows,
+                    const long* itemOff, const long* hOff, long* count, long* rowCount, long* qHist, int qcap,
+                    FftItem* items, hipStream_t s);
 template <typename TB>
 void LaunchFftFilters(PairGrid grid, const long* off, const int* kb, const int* len, const TB* mts, const TB* mgs,
                       const unsigned short* mtsLo, const unsigned short* mgsLo, int ngroups, const int* grpDom,
-                      const int* grpRowList, const long* itemOff, long nitems, const FftItem* items, int maxn,
-                      double* H, hipStream_t s);
-// Spectra of the state window beta (states 16 beta - 30 .. 16 beta + 1; the
-// second half zero with partial) of every column:
-// W[(((col * 16 + slot) * 3 + c) * 2 + ri) * B + b].
-void LaunchFftWindows(const double* g, long cols, int nstore, int B, int beta, bool partial, double* W, hipStream_t s);
-// Spectral products of tile t (steps 2 + 16 t + i) summed per row and slot
-// into Y[((row * 16 + slot) * 6 + c * 2 + ri) * B + b]; window beta at
-// WU / WT + (beta % nwr) * ringStride, every sub-filter q <= nwr.  H holds
-// kFftHPad values past its last item's spectra (read, never used).  mfma:
-// on the FP64 matrix cores, else with vector FMAs; pace > 0: a workgroup's
+                      const int* grpRowList, const long* itemOff, long nitems, const FftItem* items, int maxn, int L,
+                      int lagLo, int lagHi, double* H, hipStream_t s);
+// Spectra of the 1-2-1 combined state window beta of a level (F_x for x =
+// L beta - 2L + 1 .. L beta, from the raw states g[((col * nstore + m) * 3 +
+// c) * B + b], zero outside 1 <= m < nstore) of every column:
+// W[(((col * L + slot) * 3 + c) * 2 + ri) * B + b].
+void LaunchFftWindows(const double* g, long cols, int nstore, int B, int L, int beta, double* W, hipStream_t s);
+// Spectral products of a level's tile t (steps 2 + L t + i) summed per row
+// and slot into Y[((row * L + slot) * 6 + c * 2 + ri) * B + b]; window beta
+// at WU / WT + (beta % nwr) * ringStride, every sub-filter q <= nwr.  H holds
+// kFftHPad values past its last item's spectra (read, never used).  mfma: on
+// the FP64 matrix cores, else with vector FMAs; pace > 0: a workgroup's
 // wavefronts meet at a barrier after every `pace` items of its list.
 void LaunchConvFarFft(int ngroups, const int* grpRowList, const int* grpRows, const long* itemOff, const FftItem* items,
                       const double* H, const double* WU, const double* WT, long ringStrideU, long ringStrideT,
-                      int nwr, int t, int B, double* Y, bool mfma, int pace, hipStream_t s);
-// Samples 16 .. 16 + Tn - 1 of the inverse transforms of Y into
-// hfar[(i*3ne + r)*B + b].
-void LaunchFftInverse(const double* Y, int ne, int B, int Tn, double* hfar, hipStream_t s);
+                      int nwr, int L, int t, int B, double* Y, bool mfma, int pace, hipStream_t s);
+// Buffers of the inverse transform's layout whose values it adds.
+struct FftAdds {
+    const double* p[kFftMaxLevels - 1];
+    int n;
+};
+// Samples L .. L + Tn - 1 of the inverse transforms of a level's Y into
+// hfar[(i*3ne + r)*B + b], plus the same index of every add buffer.
+void LaunchFftInverse(const double* Y, int ne, int B, int L, int Tn, double* hfar, FftAdds add, hipStream_t s);
 
 // Near part of step K plus the tile's far part (the sum of its parts, in
 // order) and, with hfar2, the spectral share of the 16-step tile starting at

@@ -610,7 +610,7 @@ int tdbem_timings_n(tdbem_ctx* ctx, int n, double* t)
     if (!ctx || !t || n < 0)
         return Fail("tdbem_timings_n: invalid arguments");
     for (int i = 0; i < n; ++i)
-        t[i] = i < 26 ? ctx->c.timing[i] : 0.0;
+        t[i] = i < 43 ? ctx->c.timing[i] : 0.0;
     return 0;
 }
 
@@ -672,6 +672,12 @@ int tdbem_set_option(tdbem_ctx* ctx, const char* name, double value)
         if (!(value >= 0.0))
             return Fail("tdbem_set_option: history_budget_gb must be >= 0");
         ctx->c.historyBudget = value * 1e9;
+        return 0;
+    }
+    if (n == "batched_fft_levels") {
+        if (!(value >= 1.0 && value <= (double)tdbem::kFftMaxLevels && value == (double)(int)value))
+            return Fail("tdbem_set_option: batched_fft_levels must be 1 .. " + std::to_string(tdbem::kFftMaxLevels));
+        ctx->c.batchedFftLevels = (int)value;
         return 0;
     }
     if (n == "batched_fft_pace") {

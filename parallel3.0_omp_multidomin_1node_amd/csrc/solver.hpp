@@ -172,7 +172,7 @@ public:
     // differences with the correlation store, 12 history-block derivatives
     // streamed history: 20 band slabs, 21 their regeneration in the time loop (device events);
     // 22-25 the staged quadrature's evaluation counts (with quadCount)
-    double timing[26] = {0};
+    double timing[43] = {0};
     bool convTiled = true;   // time-blocked convolution (false: one full pass per step)
     int convTile = 16;       // steps per far-part tile (8 or 16); fixes the near lists built by the cull
     bool adjointAnalytic = true;  // history-block derivatives analytic (false: operator differences)
@@ -180,6 +180,7 @@ public:
     bool convPacked = true;    // filtered far tiles pack 64 band entries per work item across pairs
     int farParts = 4;          // wavefronts (and partial sums) per source row of the packed far part
     bool nearPerEntry = true;  // near part: one (pair, lag) entry per lane (false: one pair per lane)
+    int batchedFftLevels = 1;  // levels of the spectral part (sub-filter lengths 16, 32, ... 16 << (levels - 1))
     int batchedFftPace = 0;  // items between the pacing barriers of the spectral products (0: none)
     int batchedFftMfma = 1;  // the spectral products on the FP64 matrix cores (1) or with vector FMAs (0)
     int batchedFft = -1;  // batched far part in the frequency domain: 1 on, 0 off, -1 for 16 or more load cases

@@ -505,22 +505,29 @@ def test_batched_load_cases_match_single_solves(nrhs, mixed):
         assert np.abs(bit[b] - it).max() <= 1, b
 
 
-@pytest.mark.parametrize("mixed,maxn,mfma", [(0, 60, 1), (2, 60, 1), (0, 37, 1), (0, 60, 0), (2, 37, 0)])
-def test_batched_spectral_far_part_matches_lag_by_lag(mixed, maxn, mfma):
-    """The frequency-domain far part (filtered blocks in 16-lag sub-filters
-    times 32-point spectra of the state windows, 16-step tiles; the spectral
-    products on the FP64 matrix cores or with vector FMAs) against the
-    lag-by-lag far part (8-step tiles) on 20 load cases (a partial 64-case
-    tile): the same solution to 1e-9 relative at GMRES tol 1e-12 (the two sum
-    in another order); with the history truncated at maxn < nstep the blocks
-    past maxn stay out of both."""
+@pytest.mark.parametrize("levels,mixed,maxn,mfma,nstep", [(1, 0, 60, 1, 60), (1, 2, 37, 0, 60), (2, 0, 37, 1, 60),
+                                                          (3, 2, 50, 0, 60), (3, 0, 140, 1, 140), (3, 2, 140, 1, 140),
+                                                          (3, 0, 140, 0, 140), (3, 2, 97, 1, 140)])
+def test_batched_spectral_far_part_matches_lag_by_lag(levels, mixed, maxn, mfma, nstep):
+    """The frequency-domain far part against the lag-by-lag far part (8-step
+    tiles) on 20 load cases (a partial 64-case tile).  Spectral levels of
+    sub-filter length 16, 32, ... (lags 17-32 at 16, 33-64 at 32, ..., the
+    last level the rest up to maxn; 2L-point spectra of the sub-filters and
+    of the 1-2-1 combined state windows, L-step tiles), the products on the
+    FP64 matrix cores or with vector FMAs: the same solution to 1e-9 relative
+    at GMRES tol 1e-12 (the two sum in another order).  With the history
+    truncated at maxn < nstep the blocks past maxn stay out of both; maxn 37,
+    50 and 137 end inside a level, and levels that would start past maxn are
+    left out."""
     tdbem = _tdbem()
     from tdbem import mesh
-    w = mesh.layered_config2(h=0.25, nstep=60)
+    w = mesh.layered_config2(h=0.25, nstep=nstep)
     m = w.mesh
     cases = mesh.load_cases(w, 20)
-    print("case:", w.name, "spectral far part, mixed", mixed, "maxn", maxn, "mfma", mfma, flush=True)
+    print("case:", w.name, "spectral far part, levels", levels, "mixed", mixed, "maxn", maxn, "mfma", mfma,
+          "nstep", nstep, flush=True)
     gc = tdbem.Case(m.points, m.ele, m.bcid, w.dt, w.gap, domain_count=4, mat_rows=w.mats, itf_rows=m.interfaces)
+    gc.set_option("batched_fft_levels", levels)
     gc.set_option("mixed", mixed)
     gc.set_option("batched_fft_mfma", mfma)
     gc.assemble_step0(10)
@@ -531,6 +538,12 @@ def test_batched_spectral_far_part_matches_lag_by_lag(mixed, maxn, mfma):
         BU, BT, bit, bfl, bsolved, _ = gc.solve_batched(cases, w.nstep, tol=1e-12)
         assert (bsolved == w.nstep).all() and not bfl.any()
         res[fft] = (BU, BT)
+        if fft:
+            tm = gc.timings()
+            expect = max(l for l in range(1, levels + 1) if (16 << (l - 1)) + 1 <= maxn)
+            assert tm["spectral"] and tm["spectral_levels"] == expect, tm["spectral_levels"]
+            for st in tm["spectral_level_stats"]:
+                assert st["launches"] == (w.nstep - 2) // st["L"] + 1, st
     gc.close()
     eu = np.abs(res[1][0] - res[0][0]).max() / np.abs(res[0][0]).max()
     et = np.abs(res[1][1] - res[0][1]).max() / np.abs(res[0][1]).max()

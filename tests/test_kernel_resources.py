@@ -34,7 +34,7 @@ def _kernel_resources(src, tmp_path):
 def test_spectral_products_kernels_do_not_spill(tmp_path):
     res = _kernel_resources("batched_fft.hip", tmp_path)
     prods = {k: v for k, v in res.items() if "k_conv_far_fft" in k}
-    assert len(prods) == 2, res.keys()
+    assert len(prods) == 8, res.keys()  # vector and matrix-core forms, sub-filter lengths 16 .. 128
     for name, (vgpr, scratch) in prods.items():
         assert scratch == 0, (name, vgpr, scratch)
         assert vgpr <= 256, (name, vgpr)

@@ -35,8 +35,11 @@ void Case::GmresBatched(int B, const double* rhs, double* x, long restart, doubl
 {
     const long n = 3 * ne;
     // reduction blocks of the dot products: four wavefronts of 64 load cases
-    // per block, about 32 rows per wavefront (enough blocks to fill the device)
-    const int nblk = (int)std::min<long>(kBatchRedBlocks, std::max<long>(1, (n + 127) / 128));
+    // per block, at least 4 rows per wavefront (four workgroups per CU on the
+    // 8192-element box)
+    const int nblk = (int)std::min<long>(kBatchRedBlocks, std::max<long>(1, (n + 15) / 16));
+    // the same for the preconditioner's pass over elements (three rows each)
+    const int nblkE = (int)std::min<long>(kBatchRedBlocks, std::max<long>(1, (ne + 15) / 16));
     BatchWork& w = bw;
     std::vector<double> bn((size_t)B), nr((size_t)B);
     std::vector<int> act((size_t)B), stag((size_t)B);
@@ -49,8 +52,8 @@ void Case::GmresBatched(int B, const double* rhs, double* x, long restart, doubl
     LaunchZero(x, n * B, stream);
     // r = b - A x0 with x0 = 0
     LaunchSpmmB(ne, d_Arp.p, d_Acol.p, d_Aval.p, x, w.vh.p, B, stream);
-    LaunchResidualB(rhs, w.vh.p, n, B, stream);
-    LaunchNormB(w.vh.p, n, B, w.part.p, nblk, w.scal.p + B, stream);
+    LaunchPassB(nullptr, n, B, w.vh.p, rhs, nullptr, 0, 0, true, w.part.p, nblk, nullptr, nullptr, w.scal.p + B,
+                stream);
     CheckB(hipMemcpyAsync(nr.data(), w.scal.p + B, sizeof(double) * (size_t)B, hipMemcpyDeviceToHost, stream), "d2h");
     CheckB(hipStreamSynchronize(stream), "gmres");
     int nact = 0;
@@ -79,19 +82,22 @@ void Case::GmresBatched(int B, const double* rhs, double* x, long restart, doubl
     for (int j = 1; j <= restart; ++j) {
         const double* Vj = w.V.p + (size_t)(j - 1) * (size_t)n * (size_t)B;
         LaunchSpmmB(ne, d_Arp.p, d_Acol.p, d_Aval.p, Vj, w.u2.p, B, stream);
-        LaunchPrecondB(ne, d_posLeaf.p, d_leafBegin.p, d_leafSize.p, d_matOff.p, d_inv.p, d_repid.p, d_outBlock.p,
-                       w.u2.p, w.u.p, B, stream);
-        LaunchMultiDotB(w.V.p, n, j, w.u.p, B, w.part.p, nblk, h + B, stream);
-        LaunchMultiAxpyB(w.V.p, n, j, h + B, w.u.p, B, nullptr, stream);
-        LaunchMultiDotB(w.V.p, n, j, w.u.p, B, w.part.p, nblk, w.h2.p, stream);
-        LaunchMultiAxpyB(w.V.p, n, j, w.h2.p, w.u.p, B, h + B, stream);
-        LaunchNormB(w.u.p, n, B, w.part.p, nblk, h + (size_t)(j + 1) * B, stream);
-        LaunchScaleB(w.u.p, h + (size_t)(j + 1) * B, w.V.p + (size_t)j * (size_t)n * (size_t)B, n, B, stream);
+        // classical Gram-Schmidt twice: h = V'u, u -= V h; h2 = V'u, u -= V h2,
+        // h += h2; h(j+1) = ||u||.  The first dot products share their pass
+        // with the preconditioner, each update with the next dot products or
+        // norm.
+        LaunchPrecondDotB(ne, d_posLeaf.p, d_leafBegin.p, d_leafSize.p, d_matOff.p, d_inv.p, d_repid.p,
+                          d_outBlock.p, w.u2.p, w.u.p, B, w.V.p, j, w.part.p, nblkE, h + B, stream);
+        LaunchPassB(w.V.p, n, B, w.u.p, nullptr, h + B, j, j, false, w.part.p, nblk, w.h2.p, h + B, nullptr,
+                    stream);
+        LaunchPassB(w.V.p, n, B, w.u.p, nullptr, w.h2.p, j, 0, true, w.part.p, nblk, nullptr, nullptr,
+                    h + (size_t)(j + 1) * B, stream);
         LaunchGivensB(h, w.QT.p, w.R.p, w.f.p, w.phibar.p, w.stag.p, w.active.p, j, (int)restart, B, stream);
-        LaunchWUpdateB(w.V.p, w.W.p, n, w.R.p, j, (int)restart, w.f.p, x, w.active.p, B, stream);
+        LaunchWUpdateB(w.V.p, w.W.p, n, w.R.p, j, (int)restart, w.f.p, x, w.active.p, B, stream, w.u.p,
+                       h + (size_t)(j + 1) * B, w.V.p + (size_t)j * (size_t)n * (size_t)B);
         LaunchSpmmB(ne, d_Arp.p, d_Acol.p, d_Aval.p, x, w.vh.p, B, stream);
-        LaunchResidualB(rhs, w.vh.p, n, B, stream);
-        LaunchNormB(w.vh.p, n, B, w.part.p, nblk, w.scal.p + B, stream);
+        LaunchPassB(nullptr, n, B, w.vh.p, rhs, nullptr, 0, 0, true, w.part.p, nblk, nullptr, nullptr, w.scal.p + B,
+                    stream);
         CheckB(hipMemcpyAsync(nr.data(), w.scal.p + B, sizeof(double) * (size_t)B, hipMemcpyDeviceToHost, stream),
                "d2h");
         CheckB(hipMemcpyAsync(stag.data(), w.stag.p, sizeof(int) * (size_t)B, hipMemcpyDeviceToHost, stream), "d2h");

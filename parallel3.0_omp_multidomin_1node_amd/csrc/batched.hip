@@ -756,107 +756,254 @@ __global__ void __launch_bounds__(256) k_precond_b(long n, const long* __restric
                                                    const long* __restrict__ outBlock, const double* __restrict__ x,
                                                    double* __restrict__ y, int B)
 {
-    long t = (long)blockIdx.x * 4 + WaveIndex();
+    // one wavefront per element: its three rows share every x load
+    long i = (long)blockIdx.x * 4 + WaveIndex();
     int b = blockIdx.y * 64 + (threadIdx.x & 63);
-    if (t >= 3 * n || b >= B)
+    if (i >= n || b >= B)
         return;
-    long i = t / 3;
-    int mrow = (int)(t % 3);
     long leaf = posLeaf[i];
     long lb = leafBegin[leaf], c = leafSize[leaf], jl = i - lb, n3 = 3 * c;
-    const double* P = inv + matOff[leaf] + (3 * jl + mrow) * n3;
-    double acc = 0.0;
+    const double* P0 = inv + matOff[leaf] + 3 * jl * n3;
+    const double* P1 = P0 + n3;
+    const double* P2 = P1 + n3;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
     for (long k = 0; k < c; ++k) {
         long r3 = 3 * repid[lb + k];
-        acc += P[3 * k] * x[r3 * B + b];
-        acc += P[3 * k + 1] * x[(r3 + 1) * B + b];
-        acc += P[3 * k + 2] * x[(r3 + 2) * B + b];
+        double x0 = x[r3 * B + b], x1 = x[(r3 + 1) * B + b], x2 = x[(r3 + 2) * B + b];
+        a0 += P0[3 * k] * x0;
+        a0 += P0[3 * k + 1] * x1;
+        a0 += P0[3 * k + 2] * x2;
+        a1 += P1[3 * k] * x0;
+        a1 += P1[3 * k + 1] * x1;
+        a1 += P1[3 * k + 2] * x2;
+        a2 += P2[3 * k] * x0;
+        a2 += P2[3 * k + 1] * x1;
+        a2 += P2[3 * k + 2] * x2;
     }
-    y[(3 * outBlock[i] + mrow) * B + b] = acc;
+    long o = 3 * outBlock[i];
+    y[o * B + b] = a0;
+    y[(o + 1) * B + b] = a1;
+    y[(o + 2) * B + b] = a2;
 }
 
 void LaunchPrecondB(long n, const long* posLeaf, const long* leafBegin, const long* leafSize, const long* matOff,
                     const double* inv, const long* repid, const long* outBlock, const double* x, double* y, int B,
                     hipStream_t s)
 {
-    dim3 g((unsigned)((3 * n + 3) / 4), (unsigned)((B + 63) / 64));
+    dim3 g((unsigned)((n + 3) / 4), (unsigned)((B + 63) / 64));
     k_precond_b<<<g, 256, 0, s>>>(n, posLeaf, leafBegin, leafSize, matOff, inv, repid, outBlock, x, y, B);
 }
 
-// Partial dot products: workgroup x (four wavefronts) sums rows
-// r = 4 x + w, 4 x + w + 4 gridDim.x, ... of each column k.
-__global__ void __launch_bounds__(256) k_multidot_b(const double* __restrict__ V, long n, int nv,
-                                                    const double* __restrict__ u, int B, double* __restrict__ partial)
+// The preconditioner applied as in k_precond_b (same per-row sums), with the
+// elements taken grid-stride (workgroup x, wavefront w: elements 4x + w,
+// 4x + w + 4 gridDim.x, ...), fused with the partial sums of V_k . y for
+// k < nvD over the rows each wavefront wrote (partial[(x * nvD + k) * B + b]).
+constexpr int kPassColsP = 8;
+
+__global__ void __launch_bounds__(256) k_precond_dot_b(long n, const long* __restrict__ posLeaf,
+                                                       const long* __restrict__ leafBegin,
+                                                       const long* __restrict__ leafSize,
+                                                       const long* __restrict__ matOff, const double* __restrict__ inv,
+                                                       const long* __restrict__ repid,
+                                                       const long* __restrict__ outBlock, const double* __restrict__ x,
+                                                       double* y, int B, const double* __restrict__ V, int nvD,
+                                                       double* __restrict__ partial)
 {
     __shared__ double red[4][64];
-    int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    int b = blockIdx.y * 64 + lane;
-    bool live = b < B;
-    int bb = live ? b : B - 1;
-    long stride = (long)gridDim.x * 4;
-    for (int k = 0; k < nv; ++k) {
-        const double* Vk = V + (long)k * n * B;
-        double a = 0.0;
-        for (long r = (long)blockIdx.x * 4 + w; r < n; r += stride)
-            a += Vk[r * B + bb] * u[r * B + bb];
-        red[w][lane] = a;
-        __syncthreads();
-        if (w == 0 && live)
-            partial[((long)blockIdx.x * nv + k) * B + b] = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
-        __syncthreads();
+    const int w = WaveIndex(), lane = threadIdx.x & 63;
+    const int b = blockIdx.y * 64 + lane;
+    const bool live = b < B;
+    const int bb = live ? b : B - 1;
+    const long stride = (long)gridDim.x * 4;
+    const long nB = 3 * n * B;
+    const int groups = nvD > 0 ? (nvD + kPassColsP - 1) / kPassColsP : 1;
+    for (int g = 0; g < groups; ++g) {
+        const int c0 = g * kPassColsP;
+        const int cn = nvD - c0 < kPassColsP ? nvD - c0 : kPassColsP;
+        double acc[kPassColsP];
+#pragma unroll
+        for (int c = 0; c < kPassColsP; ++c)
+            acc[c] = 0.0;
+        for (long i = (long)blockIdx.x * 4 + w; i < n; i += stride) {
+            const long o = 3 * outBlock[i];
+            double a[3];
+            if (g == 0) {
+                long leaf = posLeaf[i];
+                long lb = leafBegin[leaf], c = leafSize[leaf], jl = i - lb, n3 = 3 * c;
+                const double* P0 = inv + matOff[leaf] + 3 * jl * n3;
+                const double* P1 = P0 + n3;
+                const double* P2 = P1 + n3;
+                double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+                for (long k = 0; k < c; ++k) {
+                    long r3 = 3 * repid[lb + k];
+                    double x0 = x[r3 * B + bb], x1 = x[(r3 + 1) * B + bb], x2 = x[(r3 + 2) * B + bb];
+                    a0 += P0[3 * k] * x0;
+                    a0 += P0[3 * k + 1] * x1;
+                    a0 += P0[3 * k + 2] * x2;
+                    a1 += P1[3 * k] * x0;
+                    a1 += P1[3 * k + 1] * x1;
+                    a1 += P1[3 * k + 2] * x2;
+                    a2 += P2[3 * k] * x0;
+                    a2 += P2[3 * k + 1] * x1;
+                    a2 += P2[3 * k + 2] * x2;
+                }
+                a[0] = a0;
+                a[1] = a1;
+                a[2] = a2;
+                if (live) {
+                    y[o * B + b] = a0;
+                    y[(o + 1) * B + b] = a1;
+                    y[(o + 2) * B + b] = a2;
+                }
+            } else {
+                for (int m = 0; m < 3; ++m)
+                    a[m] = y[(o + m) * B + bb];
+            }
+#pragma unroll
+            for (int c = 0; c < kPassColsP; ++c)
+                if (c < cn) {
+                    const double* Vc = V + (long)(c0 + c) * nB;
+                    for (int m = 0; m < 3; ++m)
+                        acc[c] += Vc[(o + m) * B + bb] * a[m];
+                }
+        }
+        for (int c = 0; c < cn; ++c) {
+            red[w][lane] = acc[c];
+            __syncthreads();
+            if (w == 0 && live)
+                partial[((long)blockIdx.x * nvD + c0 + c) * B + b] =
+                    ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+            __syncthreads();
+        }
     }
 }
 
-__global__ void k_finish_b(const double* __restrict__ partial, int nblocks, int nv, int B, double* __restrict__ h,
-                           int doSqrt)
+// One pass over the rows of a batch (workgroup x, wavefront w: rows 4x + w,
+// 4x + w + 4 gridDim.x, ...):
+//   v = u, or rhs - u when rhs is given;  v -= hA[k] V_k for k < nvA;
+//   u = v when it changed;
+// and the workgroup's partial sums of V_k . v (columns k < nvD) and, with
+// norm, of v . v (column nvD), at partial[(x * nout + k) * B + b].  Columns
+// are summed kPassCols at a time; the first group shares the update's loop.
+constexpr int kPassCols = 8;
+
+__global__ void __launch_bounds__(256) k_pass_b(const double* __restrict__ V, long n, int B, double* u,
+                                                const double* __restrict__ rhs, const double* __restrict__ hA,
+                                                int nvA, int nvD, int norm, double* __restrict__ partial)
 {
-    long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (long)nv * B)
-        return;
+    __shared__ double red[4][64];
+    const int w = WaveIndex(), lane = threadIdx.x & 63;
+    const int b = blockIdx.y * 64 + lane;
+    const bool live = b < B;
+    const int bb = live ? b : B - 1;
+    const long stride = (long)gridDim.x * 4;
+    const long nB = n * B;
+    const int nout = nvD + (norm ? 1 : 0);
+    const bool update = rhs || nvA > 0;
+    const int groups = nout > 0 ? (nout + kPassCols - 1) / kPassCols : 1;
+    for (int g = 0; g < groups; ++g) {
+        const int c0 = g * kPassCols;
+        const int cn = nout - c0 < kPassCols ? nout - c0 : kPassCols;
+        double acc[kPassCols];
+#pragma unroll
+        for (int c = 0; c < kPassCols; ++c)
+            acc[c] = 0.0;
+        for (long r = (long)blockIdx.x * 4 + w; r < n; r += stride) {
+            const long i = r * B + bb;
+            double v = u[i];
+            if (g == 0 && update) {
+                if (rhs)
+                    v = rhs[i] - v;
+                for (int k = 0; k < nvA; ++k)
+                    v -= hA[(long)k * B + bb] * V[(long)k * nB + i];
+                if (live)
+                    u[i] = v;
+            }
+#pragma unroll
+            for (int c = 0; c < kPassCols; ++c)
+                if (c < cn) {
+                    const int col = c0 + c;
+                    const double x = col < nvD ? V[(long)col * nB + i] : v;
+                    acc[c] += x * v;
+                }
+        }
+        for (int c = 0; c < cn; ++c) {
+            red[w][lane] = acc[c];
+            __syncthreads();
+            if (w == 0 && live)
+                partial[((long)blockIdx.x * nout + c0 + c) * B + b] =
+                    ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+            __syncthreads();
+        }
+    }
+}
+
+// Sums of the pass partials, column k per workgroup, 16 wavefronts over the
+// blocks (wavefront w: blocks w, w + 16, ...), then in wavefront order:
+// out[k * B + b] for k < nvD (and addTo[k * B + b] += it when given), the
+// square root of the norm column into normOut[b].
+__global__ void __launch_bounds__(1024) k_finish_b(const double* __restrict__ partial, int nblocks, int nvD, int nout,
+                                                   int B, double* __restrict__ out, double* __restrict__ addTo,
+                                                   double* __restrict__ normOut)
+{
+    __shared__ double red[16][64];
+    const int k = blockIdx.x;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int b = blockIdx.y * 64 + lane;
+    const bool live = b < B;
+    const int bb = live ? b : B - 1;
     double a = 0.0;
-    const long st = (long)nv * B;
-#pragma unroll 8
-    for (int x = 0; x < nblocks; ++x)
-        a += partial[(long)x * st + idx];
-    h[idx] = doSqrt ? sqrt(a) : a;
+    for (int x = w; x < nblocks; x += 16)
+        a += partial[((long)x * nout + k) * B + bb];
+    red[w][lane] = a;
+    __syncthreads();
+    if (w != 0 || !live)
+        return;
+    double t = 0.0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+        t += red[i][lane];
+    if (k < nvD) {
+        out[(long)k * B + b] = t;
+        if (addTo)
+            addTo[(long)k * B + b] += t;
+    } else {
+        normOut[b] = sqrt(t);
+    }
+}
+
+void LaunchPassB(const double* V, long n, int B, double* u, const double* rhs, const double* hA, int nvA, int nvD,
+                 bool norm, double* partial, int nblocks, double* out, double* addTo, double* normOut, hipStream_t s)
+{
+    const unsigned by = (unsigned)((B + 63) / 64);
+    k_pass_b<<<dim3((unsigned)nblocks, by), 256, 0, s>>>(V, n, B, u, rhs, hA, nvA, nvD, norm ? 1 : 0, partial);
+    const int nout = nvD + (norm ? 1 : 0);
+    if (nout > 0)
+        k_finish_b<<<dim3((unsigned)nout, by), 1024, 0, s>>>(partial, nblocks, nvD, nout, B, out, addTo, normOut);
+}
+
+void LaunchPrecondDotB(long n, const long* posLeaf, const long* leafBegin, const long* leafSize, const long* matOff,
+                       const double* inv, const long* repid, const long* outBlock, const double* x, double* y, int B,
+                       const double* V, int nvD, double* partial, int nblocks, double* h, hipStream_t s)
+{
+    const unsigned by = (unsigned)((B + 63) / 64);
+    k_precond_dot_b<<<dim3((unsigned)nblocks, by), 256, 0, s>>>(n, posLeaf, leafBegin, leafSize, matOff, inv, repid,
+                                                                outBlock, x, y, B, V, nvD, partial);
+    if (nvD > 0)
+        k_finish_b<<<dim3((unsigned)nvD, by), 1024, 0, s>>>(partial, nblocks, nvD, nvD, B, h, nullptr, nullptr);
 }
 
 void LaunchMultiDotB(const double* V, long n, int nv, const double* u, int B, double* partial, int nblocks,
                      double* h, hipStream_t s)
 {
-    dim3 g((unsigned)nblocks, (unsigned)((B + 63) / 64));
-    k_multidot_b<<<g, 256, 0, s>>>(V, n, nv, u, B, partial);
-    long m = (long)nv * B;
-    k_finish_b<<<(unsigned)((m + 255) / 256), 256, 0, s>>>(partial, nblocks, nv, B, h, 0);
+    LaunchPassB(V, n, B, const_cast<double*>(u), nullptr, nullptr, 0, nv, false, partial, nblocks, h, nullptr,
+                nullptr, s);
 }
 
 void LaunchNormB(const double* x, long n, int B, double* partial, int nblocks, double* out, hipStream_t s)
 {
-    dim3 g((unsigned)nblocks, (unsigned)((B + 63) / 64));
-    k_multidot_b<<<g, 256, 0, s>>>(x, n, 1, x, B, partial);
-    k_finish_b<<<(unsigned)((B + 255) / 256), 256, 0, s>>>(partial, nblocks, 1, B, out, 1);
-}
-
-__global__ void __launch_bounds__(256) k_multiaxpy_b(const double* __restrict__ V, long n, int nv,
-                                                     const double* __restrict__ h, double* __restrict__ u, int B,
-                                                     double* htot)
-{
-    long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (htot && idx < (long)nv * B)
-        htot[idx] += h[idx];
-    if (idx >= n * B)
-        return;
-    int b = (int)(idx % B);
-    double a = u[idx];
-    for (int k = 0; k < nv; ++k)
-        a -= h[(long)k * B + b] * V[(long)k * n * B + idx];
-    u[idx] = a;
-}
-
-void LaunchMultiAxpyB(const double* V, long n, int nv, const double* h, double* u, int B, double* htot, hipStream_t s)
-{
-    long m = n * B > (long)nv * B ? n * B : (long)nv * B;
-    k_multiaxpy_b<<<(unsigned)((m + 255) / 256), 256, 0, s>>>(V, n, nv, h, u, B, htot);
+    LaunchPassB(nullptr, n, B, const_cast<double*>(x), nullptr, nullptr, 0, 0, true, partial, nblocks, nullptr,
+                nullptr, out, s);
 }
 
 __global__ void __launch_bounds__(256) k_scale_b(const double* __restrict__ x, const double* __restrict__ a,
@@ -872,18 +1019,6 @@ __global__ void __launch_bounds__(256) k_scale_b(const double* __restrict__ x, c
 void LaunchScaleB(const double* x, const double* a, double* y, long n, int B, hipStream_t s)
 {
     k_scale_b<<<(unsigned)((n * B + 255) / 256), 256, 0, s>>>(x, a, y, n, B);
-}
-
-__global__ void __launch_bounds__(256) k_residual_b(const double* __restrict__ rhs, double* __restrict__ y, long m)
-{
-    long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx < m)
-        y[idx] = rhs[idx] - y[idx];
-}
-
-void LaunchResidualB(const double* rhs, double* y, long n, int B, hipStream_t s)
-{
-    k_residual_b<<<(unsigned)((n * B + 255) / 256), 256, 0, s>>>(rhs, y, n * B);
 }
 
 // Givens rotation in QT form for column j of every active load case
@@ -941,15 +1076,23 @@ void LaunchGivensB(double* h, double* QT, double* R, double* f, double* phibar, 
     k_givens_b<<<(unsigned)((B + 63) / 64), 64, 0, s>>>(h, QT, R, f, phibar, stag, active, j, restart, B);
 }
 
+// W_{j-1} = (V_{j-1} - sum_{l<j} W_{l-1} R(l, j)) / R(j, j) and x += f_j W_{j-1}
+// for the active load cases; with Vj, also the next basis vector of every
+// case, Vj = u / hs (0 where hs <= 1e-12), which this update does not read.
 __global__ void __launch_bounds__(256) k_wupdate_b(const double* __restrict__ V, double* __restrict__ W, long n,
                                                    const double* __restrict__ R, int j, int restart,
                                                    const double* __restrict__ f, double* __restrict__ x,
-                                                   const int* __restrict__ active, int B)
+                                                   const int* __restrict__ active, int B, const double* __restrict__ u,
+                                                   const double* __restrict__ hs, double* Vj)
 {
     long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= n * B)
         return;
     int b = (int)(idx % B);
+    if (Vj) {
+        double sc = hs[b];
+        Vj[idx] = sc <= 1.0e-12 ? 0.0 : u[idx] / sc;
+    }
     if (!active[b])
         return;
     double rjj = R[((long)(j - 1) * restart + (j - 1)) * B + b];
@@ -961,9 +1104,9 @@ __global__ void __launch_bounds__(256) k_wupdate_b(const double* __restrict__ V,
 }
 
 void LaunchWUpdateB(const double* V, double* W, long n, const double* R, int j, int restart, const double* f,
-                    double* x, const int* active, int B, hipStream_t s)
+                    double* x, const int* active, int B, hipStream_t s, const double* u, const double* hs, double* Vj)
 {
-    k_wupdate_b<<<(unsigned)((n * B + 255) / 256), 256, 0, s>>>(V, W, n, R, j, restart, f, x, active, B);
+    k_wupdate_b<<<(unsigned)((n * B + 255) / 256), 256, 0, s>>>(V, W, n, R, j, restart, f, x, active, B, u, hs, Vj);
 }
 
 }  // namespace tdbem

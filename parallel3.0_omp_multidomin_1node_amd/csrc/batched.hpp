@@ -11,7 +11,7 @@ namespace tdbem {
 
 constexpr int kBatchTile = 8;  // steps per far-part tile of the batched convolution
 constexpr int kBatchFarPartsMax = 16;  // column parts of the matrix-core far part
-constexpr int kBatchRedBlocks = 256;   // most reduction blocks of the batched dot products
+constexpr int kBatchRedBlocks = 1024;  // most reduction blocks of the batched dot products
 constexpr int kFftTile = 16;           // steps per tile (and lags per sub-filter) of the spectral part's level 0
 constexpr int kFftRows = 64;           // source rows per group of the spectral far part (8 per wavefront)
 constexpr int kFftHPad = 256;          // values past the filter spectra that the products kernel may read
@@ -113,20 +113,32 @@ void LaunchConvNearB(const NodeRefs* refs, const long* nearPtr, const long* near
 void LaunchPrecondB(long n, const long* posLeaf, const long* leafBegin, const long* leafSize, const long* matOff,
                     const double* inv, const long* repid, const long* outBlock, const double* x, double* y, int B,
                     hipStream_t s);
-// h[k*B + b] = <V_k(:, b), u(:, b)> for k < nv (V_k at V + k*n*B); partial
-// holds nblocks * nv * B doubles.
+// y = the preconditioner applied to x (as LaunchPrecondB) and, fused with it,
+// h[k*B + b] = <V_k(:, b), y(:, b)> for k < nvD (V_k at V + k*3n*B; n: block
+// rows); partial holds nblocks * nvD * B doubles.
+void LaunchPrecondDotB(long n, const long* posLeaf, const long* leafBegin, const long* leafSize, const long* matOff,
+                       const double* inv, const long* repid, const long* outBlock, const double* x, double* y, int B,
+                       const double* V, int nvD, double* partial, int nblocks, double* h, hipStream_t s);
+// One pass over the rows of a batch: v = u (rhs - u when rhs is given), v -=
+// sum_{k < nvA} hA[k] V_k, stored back into u when it changed; then per load
+// case out[k] = <V_k, v> for k < nvD (addTo[k] += out[k] when given) and,
+// with norm, normOut = ||v||.  V_k at V + k*n*B; partial holds nblocks *
+// (nvD + 1) * B doubles.  Each sum is taken in a fixed order (blocks of rows,
+// then blocks), so results do not depend on timing.
+void LaunchPassB(const double* V, long n, int B, double* u, const double* rhs, const double* hA, int nvA, int nvD,
+                 bool norm, double* partial, int nblocks, double* out, double* addTo, double* normOut, hipStream_t s);
+// h[k*B + b] = <V_k(:, b), u(:, b)> for k < nv
 void LaunchMultiDotB(const double* V, long n, int nv, const double* u, int B, double* partial, int nblocks,
                      double* h, hipStream_t s);
 void LaunchNormB(const double* x, long n, int B, double* partial, int nblocks, double* out, hipStream_t s);
-// u -= sum_k h[k] V_k ; htot[k] += h[k] when htot != null
-void LaunchMultiAxpyB(const double* V, long n, int nv, const double* h, double* u, int B, double* htot,
-                      hipStream_t s);
 // y(:, b) = x(:, b) / a[b] (a <= 1e-12 gives 0)
 void LaunchScaleB(const double* x, const double* a, double* y, long n, int B, hipStream_t s);
-void LaunchResidualB(const double* rhs, double* y, long n, int B, hipStream_t s);
 void LaunchGivensB(double* h, double* QT, double* R, double* f, double* phibar, int* stag, const int* active, int j,
                    int restart, int B, hipStream_t s);
+// W / x update of column j for the active cases; with Vj, also Vj = u / hs
+// (the scale of LaunchScaleB) for every case
 void LaunchWUpdateB(const double* V, double* W, long n, const double* R, int j, int restart, const double* f,
-                    double* x, const int* active, int B, hipStream_t s);
+                    double* x, const int* active, int B, hipStream_t s, const double* u = nullptr,
+                    const double* hs = nullptr, double* Vj = nullptr);
 
 }  // namespace tdbem

@@ -34,12 +34,12 @@ double NowB()
 void Case::GmresBatched(int B, const double* rhs, double* x, long restart, double tol, int* its, int* flg)
 {
     const long n = 3 * ne;
-    // reduction blocks of the dot products: four wavefronts of 64 load cases
-    // per block, at least 4 rows per wavefront (four workgroups per CU on the
+    // reduction blocks of the dot products: 16 wavefronts of 64 load cases
+    // per block, at least 4 rows per wavefront (one workgroup per CU on the
     // 8192-element box)
-    const int nblk = (int)std::min<long>(kBatchRedBlocks, std::max<long>(1, (n + 15) / 16));
+    const int nblk = (int)std::min<long>(kBatchRedBlocks, std::max<long>(1, (n + 63) / 64));
     // the same for the preconditioner's pass over elements (three rows each)
-    const int nblkE = (int)std::min<long>(kBatchRedBlocks, std::max<long>(1, (ne + 15) / 16));
+    const int nblkE = (int)std::min<long>(kBatchRedBlocks, std::max<long>(1, (ne + 63) / 64));
     BatchWork& w = bw;
     std::vector<double> bn((size_t)B), nr((size_t)B);
     std::vector<int> act((size_t)B), stag((size_t)B);

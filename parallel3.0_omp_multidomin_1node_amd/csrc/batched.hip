@@ -795,12 +795,14 @@ void LaunchPrecondB(long n, const long* posLeaf, const long* leafBegin, const lo
 }
 
 // The preconditioner applied as in k_precond_b (same per-row sums), with the
-// elements taken grid-stride (workgroup x, wavefront w: elements 4x + w,
-// 4x + w + 4 gridDim.x, ...), fused with the partial sums of V_k . y for
-// k < nvD over the rows each wavefront wrote (partial[(x * nvD + k) * B + b]).
+// elements taken grid-stride (workgroup x of kPassWaves wavefronts,
+// wavefront w: elements kPassWaves x + w, then every kPassWaves gridDim.x),
+// fused with the partial sums of V_k . y for k < nvD over the rows each
+// wavefront wrote (partial[(x * nvD + k) * B + b]).
 constexpr int kPassColsP = 8;
+constexpr int kPassWaves = 16;  // wavefronts per workgroup of the passes (partials are per workgroup)
 
-__global__ void __launch_bounds__(256) k_precond_dot_b(long n, const long* __restrict__ posLeaf,
+__global__ void __launch_bounds__(64 * kPassWaves) k_precond_dot_b(long n, const long* __restrict__ posLeaf,
                                                        const long* __restrict__ leafBegin,
                                                        const long* __restrict__ leafSize,
                                                        const long* __restrict__ matOff, const double* __restrict__ inv,
@@ -809,12 +811,12 @@ __global__ void __launch_bounds__(256) k_precond_dot_b(long n, const long* __res
                                                        double* y, int B, const double* __restrict__ V, int nvD,
                                                        double* __restrict__ partial)
 {
-    __shared__ double red[4][64];
+    __shared__ double red[kPassWaves][64];
     const int w = WaveIndex(), lane = threadIdx.x & 63;
     const int b = blockIdx.y * 64 + lane;
     const bool live = b < B;
     const int bb = live ? b : B - 1;
-    const long stride = (long)gridDim.x * 4;
+    const long stride = (long)gridDim.x * kPassWaves;
     const long nB = 3 * n * B;
     const int groups = nvD > 0 ? (nvD + kPassColsP - 1) / kPassColsP : 1;
     for (int g = 0; g < groups; ++g) {
@@ -824,7 +826,7 @@ __global__ void __launch_bounds__(256) k_precond_dot_b(long n, const long* __res
 #pragma unroll
         for (int c = 0; c < kPassColsP; ++c)
             acc[c] = 0.0;
-        for (long i = (long)blockIdx.x * 4 + w; i < n; i += stride) {
+        for (long i = (long)blockIdx.x * kPassWaves + w; i < n; i += stride) {
             const long o = 3 * outBlock[i];
             double a[3];
             if (g == 0) {
@@ -870,16 +872,20 @@ __global__ void __launch_bounds__(256) k_precond_dot_b(long n, const long* __res
         for (int c = 0; c < cn; ++c) {
             red[w][lane] = acc[c];
             __syncthreads();
-            if (w == 0 && live)
-                partial[((long)blockIdx.x * nvD + c0 + c) * B + b] =
-                    ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+            if (w == 0 && live) {
+                double t = 0.0;
+#pragma unroll
+                for (int q = 0; q < kPassWaves; ++q)
+                    t += red[q][lane];
+                partial[((long)blockIdx.x * nvD + c0 + c) * B + b] = t;
+            }
             __syncthreads();
         }
     }
 }
 
-// One pass over the rows of a batch (workgroup x, wavefront w: rows 4x + w,
-// 4x + w + 4 gridDim.x, ...):
+// One pass over the rows of a batch (workgroup x of kPassWaves wavefronts,
+// wavefront w: rows kPassWaves x + w, then every kPassWaves gridDim.x):
 //   v = u, or rhs - u when rhs is given;  v -= hA[k] V_k for k < nvA;
 //   u = v when it changed;
 // and the workgroup's partial sums of V_k . v (columns k < nvD) and, with
@@ -887,16 +893,16 @@ __global__ void __launch_bounds__(256) k_precond_dot_b(long n, const long* __res
 // are summed kPassCols at a time; the first group shares the update's loop.
 constexpr int kPassCols = 8;
 
-__global__ void __launch_bounds__(256) k_pass_b(const double* __restrict__ V, long n, int B, double* u,
+__global__ void __launch_bounds__(64 * kPassWaves) k_pass_b(const double* __restrict__ V, long n, int B, double* u,
                                                 const double* __restrict__ rhs, const double* __restrict__ hA,
                                                 int nvA, int nvD, int norm, double* __restrict__ partial)
 {
-    __shared__ double red[4][64];
+    __shared__ double red[kPassWaves][64];
     const int w = WaveIndex(), lane = threadIdx.x & 63;
     const int b = blockIdx.y * 64 + lane;
     const bool live = b < B;
     const int bb = live ? b : B - 1;
-    const long stride = (long)gridDim.x * 4;
+    const long stride = (long)gridDim.x * kPassWaves;
     const long nB = n * B;
     const int nout = nvD + (norm ? 1 : 0);
     const bool update = rhs || nvA > 0;
@@ -908,7 +914,7 @@ __global__ void __launch_bounds__(256) k_pass_b(const double* __restrict__ V, lo
 #pragma unroll
         for (int c = 0; c < kPassCols; ++c)
             acc[c] = 0.0;
-        for (long r = (long)blockIdx.x * 4 + w; r < n; r += stride) {
+        for (long r = (long)blockIdx.x * kPassWaves + w; r < n; r += stride) {
             const long i = r * B + bb;
             double v = u[i];
             if (g == 0 && update) {
@@ -930,18 +936,25 @@ __global__ void __launch_bounds__(256) k_pass_b(const double* __restrict__ V, lo
         for (int c = 0; c < cn; ++c) {
             red[w][lane] = acc[c];
             __syncthreads();
-            if (w == 0 && live)
-                partial[((long)blockIdx.x * nout + c0 + c) * B + b] =
-                    ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+            if (w == 0 && live) {
+                double t = 0.0;
+#pragma unroll
+                for (int q = 0; q < kPassWaves; ++q)
+                    t += red[q][lane];
+                partial[((long)blockIdx.x * nout + c0 + c) * B + b] = t;
+            }
             __syncthreads();
         }
     }
 }
 
-// Sums of the pass partials, column k per workgroup, 16 wavefronts over the
-// blocks (wavefront w: blocks w, w + 16, ...), then in wavefront order:
+// Sums of the pass partials (at most kBatchRedBlocks = 16 x 16 blocks),
+// column k per workgroup: wavefront w adds blocks w, w + 16, ... with all its
+// loads issued together, then the 16 sums are added in wavefront order;
 // out[k * B + b] for k < nvD (and addTo[k * B + b] += it when given), the
 // square root of the norm column into normOut[b].
+static_assert(kBatchRedBlocks <= 16 * 16, "the finish reads at most 16 partials per wavefront");
+
 __global__ void __launch_bounds__(1024) k_finish_b(const double* __restrict__ partial, int nblocks, int nvD, int nout,
                                                    int B, double* __restrict__ out, double* __restrict__ addTo,
                                                    double* __restrict__ normOut)
@@ -952,23 +965,30 @@ __global__ void __launch_bounds__(1024) k_finish_b(const double* __restrict__ pa
     const int b = blockIdx.y * 64 + lane;
     const bool live = b < B;
     const int bb = live ? b : B - 1;
+    double t[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int x = w + 16 * q;
+        t[q] = x < nblocks ? partial[((long)x * nout + k) * B + bb] : 0.0;
+    }
     double a = 0.0;
-    for (int x = w; x < nblocks; x += 16)
-        a += partial[((long)x * nout + k) * B + bb];
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+        a += t[q];
     red[w][lane] = a;
     __syncthreads();
     if (w != 0 || !live)
         return;
-    double t = 0.0;
+    double s = 0.0;
 #pragma unroll
     for (int i = 0; i < 16; ++i)
-        t += red[i][lane];
+        s += red[i][lane];
     if (k < nvD) {
-        out[(long)k * B + b] = t;
+        out[(long)k * B + b] = s;
         if (addTo)
-            addTo[(long)k * B + b] += t;
+            addTo[(long)k * B + b] += s;
     } else {
-        normOut[b] = sqrt(t);
+        normOut[b] = sqrt(s);
     }
 }
 
@@ -976,7 +996,7 @@ void LaunchPassB(const double* V, long n, int B, double* u, const double* rhs, c
                  bool norm, double* partial, int nblocks, double* out, double* addTo, double* normOut, hipStream_t s)
 {
     const unsigned by = (unsigned)((B + 63) / 64);
-    k_pass_b<<<dim3((unsigned)nblocks, by), 256, 0, s>>>(V, n, B, u, rhs, hA, nvA, nvD, norm ? 1 : 0, partial);
+    k_pass_b<<<dim3((unsigned)nblocks, by), 64 * kPassWaves, 0, s>>>(V, n, B, u, rhs, hA, nvA, nvD, norm ? 1 : 0, partial);
     const int nout = nvD + (norm ? 1 : 0);
     if (nout > 0)
         k_finish_b<<<dim3((unsigned)nout, by), 1024, 0, s>>>(partial, nblocks, nvD, nout, B, out, addTo, normOut);
@@ -987,7 +1007,7 @@ void LaunchPrecondDotB(long n, const long* posLeaf, const long* leafBegin, const
                        const double* V, int nvD, double* partial, int nblocks, double* h, hipStream_t s)
 {
     const unsigned by = (unsigned)((B + 63) / 64);
-    k_precond_dot_b<<<dim3((unsigned)nblocks, by), 256, 0, s>>>(n, posLeaf, leafBegin, leafSize, matOff, inv, repid,
+    k_precond_dot_b<<<dim3((unsigned)nblocks, by), 64 * kPassWaves, 0, s>>>(n, posLeaf, leafBegin, leafSize, matOff, inv, repid,
                                                                 outBlock, x, y, B, V, nvD, partial);
     if (nvD > 0)
         k_finish_b<<<dim3((unsigned)nvD, by), 1024, 0, s>>>(partial, nblocks, nvD, nvD, B, h, nullptr, nullptr);

@@ -11,7 +11,7 @@ namespace tdbem {
 
 constexpr int kBatchTile = 8;  // steps per far-part tile of the batched convolution
 constexpr int kBatchFarPartsMax = 16;  // column parts of the matrix-core far part
-constexpr int kBatchRedBlocks = 1024;  // most reduction blocks of the batched dot products
+constexpr int kBatchRedBlocks = 256;  // most reduction blocks (16 wavefronts each) of the batched dot products
 constexpr int kFftTile = 16;           // steps per tile (and lags per sub-filter) of the spectral part's level 0
 constexpr int kFftRows = 64;           // source rows per group of the spectral far part (8 per wavefront)
 constexpr int kFftHPad = 256;          // values past the filter spectra that the products kernel may read

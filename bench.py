@@ -371,7 +371,10 @@ def models_report(tdbem):
     sizes, materials and loads, dt = 0.5 h / c1 each, 200 steps): every
     model's whole job one after another, then all of them through
     tdbem_run_models (one stream and host thread per model, history bands
-    held at once within 80 % of the free device memory)."""
+    held at once within 80 % of the free device memory), then stacked in
+    rounds (tdbem_run_models_grouped: every kernel one launch for all the
+    models of a round, rounds sized to the same budget); the grouped results
+    are checked against the concurrent ones (equal bit for bit)."""
     from tdbem import mesh
     ws = mesh.models_config4ii()
 
@@ -396,11 +399,26 @@ def models_report(tdbem):
     t2 = time.perf_counter()
     for c in cs:
         c.close()
-    log(f"models: one after another {t1 - t0:.2f} s, together {t2 - t1:.2f} s")
+    cs = [case(w) for w in ws]
+    tdbem.synchronize()
+    t3 = time.perf_counter()
+    resg = tdbem.run_models(cs, [w.known for w in ws], [w.nstep for w in ws], grouped=True)
+    tdbem.synchronize()
+    t4 = time.perf_counter()
+    rounds = tdbem.run_models.rounds
+    for c in cs:
+        c.close()
+    same = all(np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+               for a, b in zip(res, resg))
+    log(f"models: one after another {t1 - t0:.2f} s, together {t2 - t1:.2f} s, grouped {t4 - t3:.2f} s "
+        f"({rounds} rounds, equal to the concurrent run: {same})")
     return {"models": len(ws), "elements": [int(len(w.mesh.ele)) for w in ws], "nstep": ws[0].nstep,
             "model_steps_per_s_sequential": round(steps / (t1 - t0), 2),
             "model_steps_per_s_together": round(steps / (t2 - t1), 2),
-            "speedup": round((t1 - t0) / (t2 - t1), 3), "all_solved": all(r[4] == w.nstep for r, w in zip(res, ws))}
+            "model_steps_per_s_grouped": round(steps / (t4 - t3), 2), "grouped_rounds": rounds,
+            "grouped_equals_together": bool(same),
+            "speedup": round((t1 - t0) / (t2 - t1), 3), "speedup_grouped": round((t1 - t0) / (t4 - t3), 3),
+            "all_solved": all(r[4] == w.nstep for r, w in zip(res, ws))}
 
 
 def main():
@@ -469,6 +487,8 @@ def main():
                 out["adjoint"] = adjoint_report(tdbem, gmres_device=args.gmres_device)
         if args.streamed > 0:
             out["streamed"] = streamed_report(tdbem, args.gmres_device, args.streamed)
+        if args.models:
+            out["models"] = models_report(tdbem)
         print(json.dumps(out), flush=True)
         return
 

@@ -200,6 +200,23 @@ int tdbem_run_models(int n, tdbem_ctx* const* ctx, long maxleaf, const int* nste
                      long restart, double tol, double budget_bytes, double* const* lu, double* const* lt,
                      int* const* iters, int* const* flags, int* solved);
 
+/* The same jobs as tdbem_run_models, with the models stacked: models whose
+ * FP64 history bands fit budget_bytes together (0: 80 % of the free device
+ * memory) and that share the step count form one round, up to 64 models and
+ * 64 domains; a round is one case whose every kernel (cull, quadrature, step
+ * 0, convolution, every GMRES kernel) is one launch for all its models.  Each
+ * model keeps its own materials, time step, DOF numbering, preconditioner
+ * leaves and GMRES (state, Hessenberg, Givens, reduction partition), so its
+ * outputs are those of its own job with the same options.  The contexts'
+ * options must agree and ask for FP64 bands (mixed 0).  *rounds (may be null)
+ * receives the number of rounds.  No counterpart in the reference (one model
+ * per process there); the per-model operations are those of
+ * DBEM1/multidomain.cpp:3664-3926. */
+int tdbem_run_models_grouped(int n, tdbem_ctx* const* ctx, long maxleaf, const int* nstep,
+                             const double* const* known_lu, long restart, double tol, double budget_bytes,
+                             double* const* lu, double* const* lt, int* const* iters, int* const* flags, int* solved,
+                             int* rounds);
+
 /* Convenience: full DynaGMRESSolverMultiDomainCCSR (DBEM1/multidomain.cpp:
  * 3664-3926): step 0, history for maxn = nstep, time loop.  Returns 1 when
  * the final GMRES flag is 0 and -1 otherwise through *result. */

@@ -436,6 +436,124 @@ int tdbem_run_models(int n, tdbem_ctx* const* ctx, long maxleaf, const int* nste
     return 0;
 }
 
+int tdbem_run_models_grouped(int n, tdbem_ctx* const* ctx, long maxleaf, const int* nstep,
+                             const double* const* known_lu, long restart, double tol, double budget_bytes,
+                             double* const* lu, double* const* lt, int* const* iters, int* const* flags, int* solved,
+                             int* rounds)
+{
+    if (n < 1 || !ctx || maxleaf <= 0 || !nstep || !known_lu || !lu || !lt || !iters || !flags || !solved ||
+        restart < 1 || !(tol > 0.0))
+        return Fail("tdbem_run_models_grouped: invalid arguments");
+    for (int i = 0; i < n; ++i) {
+        if (!ctx[i] || nstep[i] < 1 || !known_lu[i] || !lu[i] || !lt[i] || !iters[i] || !flags[i])
+            return Fail("tdbem_run_models_grouped: invalid arguments for model " + std::to_string(i));
+        for (int k = 0; k < i; ++k)
+            if (ctx[k] == ctx[i])
+                return Fail("tdbem_run_models_grouped: model " + std::to_string(i) + " repeats a context");
+        const tdbem::Case& c = ctx[i]->c;
+        const tdbem::Case& c0 = ctx[0]->c;
+        if (c.mixed != 0)
+            return Fail("tdbem_run_models_grouped: model " + std::to_string(i) + " asks for mixed-precision bands; "
+                        "grouped models run FP64 bands");
+        if (c.quadStaged != c0.quadStaged || c.nearPerEntry != c0.nearPerEntry || c.farParts != c0.farParts ||
+            c.convPacked != c0.convPacked || c.convFiltered != c0.convFiltered || c.convTiled != c0.convTiled ||
+            c.convTile != c0.convTile || c.gmBlocks != c0.gmBlocks || c.gmresDevice != c0.gmresDevice)
+            return Fail("tdbem_run_models_grouped: model " + std::to_string(i) +
+                        " has other solver options than model 0 (grouped models share one set)");
+    }
+    if (!(budget_bytes > 0.0)) {
+        size_t freeB = 0, totalB = 0;
+        if (hipMemGetInfo(&freeB, &totalB) != hipSuccess)
+            return Fail("tdbem_run_models_grouped: hipMemGetInfo failed");
+        budget_bytes = 0.8 * (double)(freeB + tdbem::PooledBytesCached());
+    }
+    // band bytes of every model from its own cull, then rounds: models taken
+    // in order while their bands fit the budget together, share the step count
+    // and stay within the group and domain limits; a model larger than the
+    // budget makes a round of its own
+    std::vector<double> need((size_t)n);
+    try {
+        for (int i = 0; i < n; ++i) {
+            need[(size_t)i] = ctx[i]->c.PrepareHistory(nstep[i]);
+            ctx[i]->c.ReleaseHistory();
+        }
+    } catch (const std::exception& e) {
+        return Fail("tdbem_run_models_grouped: sizing", e);
+    }
+    std::vector<std::vector<int>> plan;
+    {
+        double held = 0.0;
+        int doms = 0;
+        for (int i = 0; i < n; ++i) {
+            const int d = ctx[i]->c.model.DomainCount();
+            bool fits = !plan.empty() && !plan.back().empty() && held + need[(size_t)i] <= budget_bytes &&
+                        nstep[plan.back()[0]] == nstep[i] && doms + d <= tdbem::kMaxDomains &&
+                        (int)plan.back().size() < tdbem::kGmGroupsMax;
+            if (!fits) {
+                plan.emplace_back();
+                held = 0.0;
+                doms = 0;
+            }
+            plan.back().push_back(i);
+            held += need[(size_t)i];
+            doms += d;
+        }
+    }
+    if (rounds)
+        *rounds = (int)plan.size();
+    for (const std::vector<int>& R : plan) {
+        const int G = (int)R.size();
+        const int ns = nstep[R[0]];
+        try {
+            tdbem::Case m;
+            std::vector<const tdbem::Case*> ms;
+            for (int i : R)
+                ms.push_back(&ctx[i]->c);
+            m.CreateGroup(ms);
+            m.historyStream = 0;  // the round's bands were sized to fit the device
+            m.AssembleStep0(maxleaf);
+            m.AssembleHistory(ns);
+            const size_t n3 = 3 * (size_t)m.ne, steps = (size_t)ns + 1;
+            std::vector<double> known(steps * n3), mlu(steps * n3), mlt(steps * n3);
+            for (size_t K = 0; K < steps; ++K)
+                for (int g = 0; g < G; ++g) {
+                    const size_t ng = 3 * (size_t)ctx[R[(size_t)g]]->c.ne, at = 3 * (size_t)m.groupEle[(size_t)g];
+                    std::memcpy(&known[K * n3 + at], known_lu[R[(size_t)g]] + K * ng, sizeof(double) * ng);
+                }
+            std::vector<int> its((size_t)G * (size_t)ns, 0), fls((size_t)G * (size_t)ns, 0), sol((size_t)G, 0);
+            m.SolveGroups(ns, known.data(), restart, tol, mlu.data(), mlt.data(), its.data(), fls.data(), sol.data());
+            // each model's outputs as its own job leaves them: the steps it
+            // solved (not the step whose GMRES stopped with a flag), zeros after
+            for (int g = 0; g < G; ++g) {
+                const int i = R[(size_t)g];
+                const size_t ng = 3 * (size_t)ctx[i]->c.ne, at = 3 * (size_t)m.groupEle[(size_t)g];
+                const int sv = sol[(size_t)g];
+                const int good = sv > 0 && fls[(size_t)g * ns + (sv - 1)] != 0 ? sv - 1 : sv;
+                for (size_t K = 0; K < steps; ++K) {
+                    if ((int)K <= good) {
+                        std::memcpy(lu[i] + K * ng, &mlu[K * n3 + at], sizeof(double) * ng);
+                        std::memcpy(lt[i] + K * ng, &mlt[K * n3 + at], sizeof(double) * ng);
+                    } else {
+                        std::memset(lu[i] + K * ng, 0, sizeof(double) * ng);
+                        std::memset(lt[i] + K * ng, 0, sizeof(double) * ng);
+                    }
+                }
+                for (int K = 0; K < sv; ++K) {
+                    iters[i][K] = its[(size_t)g * ns + K];
+                    flags[i][K] = fls[(size_t)g * ns + K];
+                }
+                solved[i] = sv;
+            }
+        } catch (const std::exception& e) {
+            std::string who;
+            for (int i : R)
+                who += (who.empty() ? "" : ",") + std::to_string(i);
+            return Fail(std::string("tdbem_run_models_grouped: round of models ") + who + ": " + e.what());
+        }
+    }
+    return 0;
+}
+
 int tdbem_solve_batched(tdbem_ctx* ctx, int nstep, int nrhs, const double* known_lu, long restart, double tol,
                         double* lu, double* lt, int* iters, int* flags, int* solved, double* times)
 {

@@ -153,7 +153,7 @@ int BuildMultiDomainModel(const MultiDomainInputConfig& config, long eleNum, lon
 // ---------------------------------------------------------------------------
 // GlobalDofMap (DBEM1/multidomain.cpp:777-1026)
 // ---------------------------------------------------------------------------
-bool GlobalDofMap::Build(const MultiDomainModel& model, const int* bcid, long eleNum)
+bool GlobalDofMap::Build(const MultiDomainModel& model, const int* bcid, long eleNum, const std::vector<long>* groups)
 {
     const long N = model.nodeCount;
     unknownBlockCount = knownBlockCount = 0;
@@ -169,49 +169,61 @@ bool GlobalDofMap::Build(const MultiDomainModel& model, const int* bcid, long el
     m_preferred.assign((size_t)N, -1);
     for (long i = 0; i < N; ++i)
         m_equationRows[(size_t)i] = i;
+    std::vector<long> bounds = {0, eleNum};
+    if (groups && groups->size() >= 2)
+        bounds = *groups;
+    if (bounds.front() != 0 || bounds.back() != eleNum)
+        return false;
     long unknown = 0;
     std::vector<char> side((size_t)N, 0);
-    for (const InterfacePair& f : model.interfaces) {
-        double sB = (double)(f.normalSign == 0 ? -1 : f.normalSign);
-        long ub = unknown++, tb = unknown++;
-        m_uRefs[(size_t)f.eleA] = VariableRef(ub, 1.0, false);
-        m_uRefs[(size_t)f.eleB] = VariableRef(ub, 1.0, false);
-        m_tRefs[(size_t)f.eleA] = VariableRef(tb, 1.0, false);
-        m_tRefs[(size_t)f.eleB] = VariableRef(tb, sB, false);
-        m_preferred[(size_t)f.eleA] = ub;
-        m_preferred[(size_t)f.eleB] = tb;
-        long hu = historyUBlockCount++, ht = historyTBlockCount++;
-        m_historyURefs[(size_t)f.eleA] = VariableRef(hu, 1.0, false);
-        m_historyURefs[(size_t)f.eleB] = VariableRef(hu, 1.0, false);
-        m_historyTRefs[(size_t)f.eleA] = VariableRef(ht, 1.0, false);
-        m_historyTRefs[(size_t)f.eleB] = VariableRef(ht, sB, false);
-        side[(size_t)f.eleA] = side[(size_t)f.eleB] = 1;
-        ++interfaceDisplacementUnknownBlocks;
-        ++interfaceTractionUnknownBlocks;
-    }
-    for (long e = 0; e < eleNum; ++e) {
-        if (bcid[e] != 123 && bcid[e] != 456) {
-            printf("MultiDomain dof map failed: element %ld has mixed BCID %d. Only 123 and 456 are supported.\n",
-                   e, bcid[e]);
-            return false;
+    for (size_t g = 0; g + 1 < bounds.size(); ++g) {
+        const long e0 = bounds[g], e1 = bounds[g + 1];
+        for (const InterfacePair& f : model.interfaces) {
+            if (f.eleA < e0 || f.eleA >= e1)
+                continue;
+            if (f.eleB < e0 || f.eleB >= e1)
+                return false;  // an interface joins two groups
+            double sB = (double)(f.normalSign == 0 ? -1 : f.normalSign);
+            long ub = unknown++, tb = unknown++;
+            m_uRefs[(size_t)f.eleA] = VariableRef(ub, 1.0, false);
+            m_uRefs[(size_t)f.eleB] = VariableRef(ub, 1.0, false);
+            m_tRefs[(size_t)f.eleA] = VariableRef(tb, 1.0, false);
+            m_tRefs[(size_t)f.eleB] = VariableRef(tb, sB, false);
+            m_preferred[(size_t)f.eleA] = ub;
+            m_preferred[(size_t)f.eleB] = tb;
+            long hu = historyUBlockCount++, ht = historyTBlockCount++;
+            m_historyURefs[(size_t)f.eleA] = VariableRef(hu, 1.0, false);
+            m_historyURefs[(size_t)f.eleB] = VariableRef(hu, 1.0, false);
+            m_historyTRefs[(size_t)f.eleA] = VariableRef(ht, 1.0, false);
+            m_historyTRefs[(size_t)f.eleB] = VariableRef(ht, sB, false);
+            side[(size_t)f.eleA] = side[(size_t)f.eleB] = 1;
+            ++interfaceDisplacementUnknownBlocks;
+            ++interfaceTractionUnknownBlocks;
         }
-        if (side[(size_t)e])
-            continue;
-        if (bcid[e] == 123) {
-            long tb = unknown++;
-            m_uRefs[(size_t)e] = VariableRef(knownBlockCount++, 1.0, true);
-            m_tRefs[(size_t)e] = VariableRef(tb, 1.0, false);
-            m_preferred[(size_t)e] = tb;
-            ++outerTractionUnknownBlocks;
-        } else {
-            long ub = unknown++;
-            m_uRefs[(size_t)e] = VariableRef(ub, 1.0, false);
-            m_tRefs[(size_t)e] = VariableRef(knownBlockCount++, 1.0, true);
-            m_preferred[(size_t)e] = ub;
-            ++outerDisplacementUnknownBlocks;
+        for (long e = e0; e < e1; ++e) {
+            if (bcid[e] != 123 && bcid[e] != 456) {
+                printf("MultiDomain dof map failed: element %ld has mixed BCID %d. Only 123 and 456 are supported.\n",
+                       e, bcid[e]);
+                return false;
+            }
+            if (side[(size_t)e])
+                continue;
+            if (bcid[e] == 123) {
+                long tb = unknown++;
+                m_uRefs[(size_t)e] = VariableRef(knownBlockCount++, 1.0, true);
+                m_tRefs[(size_t)e] = VariableRef(tb, 1.0, false);
+                m_preferred[(size_t)e] = tb;
+                ++outerTractionUnknownBlocks;
+            } else {
+                long ub = unknown++;
+                m_uRefs[(size_t)e] = VariableRef(ub, 1.0, false);
+                m_tRefs[(size_t)e] = VariableRef(knownBlockCount++, 1.0, true);
+                m_preferred[(size_t)e] = ub;
+                ++outerDisplacementUnknownBlocks;
+            }
+            m_historyURefs[(size_t)e] = VariableRef(historyUBlockCount++, 1.0, false);
+            m_historyTRefs[(size_t)e] = VariableRef(historyTBlockCount++, 1.0, false);
         }
-        m_historyURefs[(size_t)e] = VariableRef(historyUBlockCount++, 1.0, false);
-        m_historyTRefs[(size_t)e] = VariableRef(historyTBlockCount++, 1.0, false);
     }
     unknownBlockCount = unknown;
     if (unknownBlockCount != equationBlockCount) {

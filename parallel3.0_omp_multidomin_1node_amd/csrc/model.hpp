@@ -88,8 +88,15 @@ public:
     long historyUBlockCount = 0;
     long historyTBlockCount = 0;
 
-    // bcid[e] of every element; only 123 and 456 are supported.
-    bool Build(const MultiDomainModel& model, const int* bcid, long eleNum);
+    // bcid[e] of every element; only 123 and 456 are supported.  With
+    // groups (contiguous element ranges [groups[g], groups[g+1]) of
+    // independent models placed one after another), the blocks are allocated
+    // group by group, each group's interface pairs (by eleA) first and then
+    // its other elements, so a group's unknown, known and history blocks are
+    // contiguous and in the order the group alone would give them; without
+    // groups (or with one) this is the reference's allocation order.
+    bool Build(const MultiDomainModel& model, const int* bcid, long eleNum,
+               const std::vector<long>* groups = nullptr);
     VariableRef GetU(int domainId, long node) const { return At(m_uRefs, node); }
     VariableRef GetT(int domainId, long node) const { return At(m_tRefs, node); }
     VariableRef GetHistoryU(int domainId, long node) const { return At(m_historyURefs, node); }

@@ -154,34 +154,44 @@ int InvMat(double* A, long n)
 }
 
 bool BuildMappedLeafPreConditioner(const double* centres, long N, long maxLeaf, const CCSRMat& A,
-                                   const GlobalDofMap& dof, LeafPreconditioner& pre)
+                                   const GlobalDofMap& dof, LeafPreconditioner& pre, const std::vector<long>* groups)
 {
     pre = LeafPreconditioner();
     if (N <= 0 || maxLeaf <= 0 || A.n != N)
         return false;
-    double rmin[3], rmax[3];
-    for (int j = 0; j < 3; ++j)
-        rmin[j] = rmax[j] = centres[j];
-    for (long i = 1; i < N; ++i)
-        for (int j = 0; j < 3; ++j) {
-            if (rmin[j] > centres[3 * i + j])
-                rmin[j] = centres[3 * i + j];
-            if (rmax[j] < centres[3 * i + j])
-                rmax[j] = centres[3 * i + j];
-        }
-    Cube root;
-    double ml = rmax[0] - rmin[0];
-    for (int j = 0; j < 3; ++j) {
-        if (ml < (rmax[j] - rmin[j]))
-            ml = rmax[j] - rmin[j];
-        root.c[j] = rmin[j] + 0.5 * (rmax[j] - rmin[j]);
-    }
-    root.len = 1.02 * ml;
-    std::vector<long> all((size_t)N);
-    for (long i = 0; i < N; ++i)
-        all[(size_t)i] = i;
+    std::vector<long> bounds = {0, N};
+    if (groups && groups->size() >= 2)
+        bounds = *groups;
+    if (bounds.front() != 0 || bounds.back() != N)
+        return false;
     pre.repid.assign((size_t)N, -1);
-    Split(centres, root, all, 0, maxLeaf, pre);
+    for (size_t g = 0; g + 1 < bounds.size(); ++g) {
+        const long p0 = bounds[g], p1 = bounds[g + 1];
+        if (p1 <= p0)
+            return false;
+        double rmin[3], rmax[3];
+        for (int j = 0; j < 3; ++j)
+            rmin[j] = rmax[j] = centres[3 * p0 + j];
+        for (long i = p0 + 1; i < p1; ++i)
+            for (int j = 0; j < 3; ++j) {
+                if (rmin[j] > centres[3 * i + j])
+                    rmin[j] = centres[3 * i + j];
+                if (rmax[j] < centres[3 * i + j])
+                    rmax[j] = centres[3 * i + j];
+            }
+        Cube root;
+        double ml = rmax[0] - rmin[0];
+        for (int j = 0; j < 3; ++j) {
+            if (ml < (rmax[j] - rmin[j]))
+                ml = rmax[j] - rmin[j];
+            root.c[j] = rmin[j] + 0.5 * (rmax[j] - rmin[j]);
+        }
+        root.len = 1.02 * ml;
+        std::vector<long> all((size_t)(p1 - p0));
+        for (long i = p0; i < p1; ++i)
+            all[(size_t)(i - p0)] = i;
+        Split(centres, root, all, p0, maxLeaf, pre);
+    }
     pre.leafCount = (long)pre.leafSize.size();
     long total = 0;
     for (long l = 0; l < pre.leafCount; ++l) {

@@ -27,7 +27,12 @@ struct LeafPreconditioner {
 // reference leaves it).
 int InvMat(double* A, long n);
 
+// With groups (contiguous point ranges [groups[g], groups[g+1]) of
+// independent models), one octree per group over that group's points, the
+// leaves of group g after those of the groups before it; without groups (or
+// with one) one octree over all points.
 bool BuildMappedLeafPreConditioner(const double* centres, long pointCount, long maxLeaf, const CCSRMat& A,
-                                   const GlobalDofMap& dof, LeafPreconditioner& pre);
+                                   const GlobalDofMap& dof, LeafPreconditioner& pre,
+                                   const std::vector<long>* groups = nullptr);
 
 }  // namespace tdbem

@@ -215,9 +215,32 @@ template struct DevBuf<PwWave>;
 template struct DevBuf<float>;
 template struct DevBuf<DynaMat>;
 template struct DevBuf<unsigned short>;
+template struct DevBuf<GmresGroupTable>;
+
+void Case::GroupWork::release()
+{
+    if (gBegin)
+        (void)hipGraphExecDestroy(gBegin);
+    if (gIter)
+        (void)hipGraphExecDestroy(gIter);
+    gBegin = gIter = nullptr;
+    graphB = nullptr;
+    if (hst)
+        (void)hipHostFree(hst);
+    hst = nullptr;
+    for (DevBuf<double>* b : {&V, &W, &u, &u2, &vh, &h, &h2, &QT, &R, &f, &phibar, &part, &st})
+        b->free();
+    stag.free();
+    active.free();
+    tab.free();
+    restart = -1;
+}
 
 Case::~Case()
 {
+    if (stream)
+        (void)hipStreamSynchronize(stream);
+    gw.release();
     ReleaseGraphs();
     if (h_gst)
         (void)hipHostFree(h_gst);
@@ -249,15 +272,6 @@ void Case::Create(long npoints, const double* points, long nele, const int* ele_
     BuildQuadTables(tables, gap);
     // element geometry (GeoInit / GetNodeInfo / PthPhyInit frame)
     geo.resize((size_t)ne);
-    domOf.assign((size_t)ne, 0);
-    surface.assign((size_t)ne, 0);
-    for (const Domain& d : model.domains)
-        for (long e : d.elementIds)
-            domOf[(size_t)e] = d.id;
-    for (const InterfacePair& f : model.interfaces) {
-        surface[(size_t)f.eleA] = 1;
-        surface[(size_t)f.eleB] = 1;
-    }
     for (long i = 0; i < ne; ++i) {
         ElemGeo& g = geo[(size_t)i];
         double coord[3][8];
@@ -277,6 +291,98 @@ void Case::Create(long npoints, const double* points, long nele, const int* ele_
         }
         g.radiusPW = g.radius / kNPW;
         TranslateMatrix(g.cc, tables.lc[0][0], tables.lc[0][1], g.trans);
+    }
+    FinishCreate();
+}
+
+// Models stacked as groups: each model's domains, interfaces, elements and
+// geometry placed after those of the models before it (ids offset), the DOF
+// map allocated model by model, one octree per model for the preconditioner
+// and the options of the first model.  Every model keeps its own materials
+// and time step (each domain's DynaMat carries its dt); the pairs of a domain
+// never leave it, so the stacked system is block diagonal and each model's
+// rows see exactly the blocks of its own job.
+void Case::CreateGroup(const std::vector<const Case*>& ms)
+{
+    if (ms.empty())
+        throw std::runtime_error("no models to group");
+    if ((int)ms.size() > kGmGroupsMax)
+        throw std::runtime_error("at most " + std::to_string(kGmGroupsMax) + " models per group");
+    long total = 0;
+    int nd = 0;
+    for (const Case* m : ms) {
+        if (!m || m->ne <= 0)
+            throw std::runtime_error("a grouped model has no elements");
+        if (m->tables.gap != ms[0]->tables.gap)
+            throw std::runtime_error("grouped models need the same gap (quadrature tables)");
+        total += m->ne;
+        nd += m->model.DomainCount();
+    }
+    if (nd > kMaxDomains)
+        throw std::runtime_error("grouped models hold more than " + std::to_string(kMaxDomains) + " domains");
+    const Case& m0 = *ms[0];
+    ne = total;
+    dt = m0.dt;
+    quadStaged = m0.quadStaged;
+    nearPerEntry = m0.nearPerEntry;
+    farParts = m0.farParts;
+    convPacked = m0.convPacked;
+    convFiltered = m0.convFiltered;
+    convTiled = m0.convTiled;
+    convTile = m0.convTile;
+    gmBlocks = m0.gmBlocks;
+    gmresDevice = m0.gmresDevice;
+    Check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+    model = MultiDomainModel();
+    model.enabled = true;
+    model.elementCount = model.nodeCount = ne;
+    groupEle.assign(1, 0);
+    bcid.clear();
+    geo.clear();
+    long eb = 0;
+    int db = 0;
+    for (const Case* m : ms) {
+        for (Domain D : m->model.domains) {
+            D.id += db;
+            D.eleBegin += eb;
+            D.nodeBegin += eb;
+            for (long& e : D.elementIds)
+                e += eb;
+            model.domains.push_back(std::move(D));
+        }
+        for (InterfacePair f : m->model.interfaces) {
+            f.domainA += db;
+            f.domainB += db;
+            f.eleA += eb;
+            f.eleB += eb;
+            model.interfaces.push_back(f);
+        }
+        bcid.insert(bcid.end(), m->bcid.begin(), m->bcid.end());
+        geo.insert(geo.end(), m->geo.begin(), m->geo.end());
+        eb += m->ne;
+        db += m->model.DomainCount();
+        groupEle.push_back(eb);
+    }
+    if (!model.Validate())
+        throw std::runtime_error("grouped model validation failed");
+    if (!dof.Build(model, bcid.data(), ne, &groupEle))
+        throw std::runtime_error("GlobalDofMap::Build failed for the grouped models");
+    tables = m0.tables;
+    FinishCreate();
+}
+
+// Domain and surface flags, per-node references, the pair grid, the device
+// copies and the per-element set-up kernels of a created case.
+void Case::FinishCreate()
+{
+    domOf.assign((size_t)ne, 0);
+    surface.assign((size_t)ne, 0);
+    for (const Domain& d : model.domains)
+        for (long e : d.elementIds)
+            domOf[(size_t)e] = d.id;
+    for (const InterfacePair& f : model.interfaces) {
+        surface[(size_t)f.eleA] = 1;
+        surface[(size_t)f.eleB] = 1;
     }
     // per-node references and interface transforms
     refs.resize((size_t)ne);
@@ -555,7 +661,7 @@ void Case::AssembleStep0(long maxleaf)
     for (long i = 0; i < ne; ++i)
         for (int k = 0; k < 3; ++k)
             centres[3 * (size_t)i + (size_t)k] = geo[(size_t)i].center[k];
-    if (!BuildMappedLeafPreConditioner(centres.data(), ne, maxleaf, A, dof, pre))
+    if (!BuildMappedLeafPreConditioner(centres.data(), ne, maxleaf, A, dof, pre, Grouped() ? &groupEle : nullptr))
         throw std::runtime_error("preconditioner build failed");
     std::vector<long> posLeaf((size_t)ne), outBlock((size_t)ne);
     for (long l = 0; l < pre.leafCount; ++l)
@@ -1674,6 +1780,271 @@ void Case::Solve(int nstep, const double* known_lu, long restart, double tol, do
     timing[6] = farTime;
     timing[7] = (double)farEv.size();
     timing[21] = historyStreamed ? farTime : 0.0;  // far part with the slabs' regeneration
+}
+
+// ---------------------------------------------------------------------------
+// Grouped models (CreateGroup): one time loop for all of them.
+// ---------------------------------------------------------------------------
+void Case::GmresDeviceGroups(const double* d_b, double* d_x, long restart, double tol, const int* active, int* its,
+                             int* fls)
+{
+    const int G = GroupCount();
+    GmresGroupArgs a{};
+    a.ne = ne;
+    a.restart = restart;
+    a.rp = d_Arp.p;
+    a.col = d_Acol.p;
+    a.val = d_Aval.p;
+    a.posLeaf = d_posLeaf.p;
+    a.leafBegin = d_leafBegin.p;
+    a.leafSize = d_leafSize.p;
+    a.matOff = d_matOff.p;
+    a.repid = d_repid.p;
+    a.outBlock = d_outBlock.p;
+    a.inv = d_inv.p;
+    a.V = gw.V.p;
+    a.W = gw.W.p;
+    a.u = gw.u.p;
+    a.u2 = gw.u2.p;
+    a.vh = gw.vh.p;
+    a.x = d_x;
+    a.h = gw.h.p;
+    a.h2 = gw.h2.p;
+    a.QT = gw.QT.p;
+    a.R = gw.R.p;
+    a.f = gw.f.p;
+    a.phibar = gw.phibar.p;
+    a.part = gw.part.p;
+    a.hs = restart + 2;
+    a.qs = (restart + 1) * (restart + 1);
+    a.rs = restart * restart;
+    a.ps = (long)kGmBlocksMax * (restart + 1);
+    a.stag = gw.stag.p;
+    a.active = gw.active.p;
+    a.st = reinterpret_cast<GmState*>(gw.st.p);
+    Check(hipMemcpyAsync(gw.active.p, active, sizeof(int) * (size_t)G, hipMemcpyHostToDevice, stream), "active");
+    // the operands stay fixed for a solve, so one capture serves every step
+    const bool useGraph = gmresDevice == 1;
+    if (useGraph && !(gw.gBegin && gw.graphB == d_b && gw.graphTol == tol)) {
+        if (gw.gBegin)
+            Check(hipGraphExecDestroy(gw.gBegin), "graph");
+        if (gw.gIter)
+            Check(hipGraphExecDestroy(gw.gIter), "graph");
+        gw.gBegin = gw.gIter = nullptr;
+        hipGraph_t g = nullptr;
+        Check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "capture");
+        LaunchGmresBeginGroups(a, gw.t, gw.tab.p, d_b, tol, stream);
+        Check(hipStreamEndCapture(stream, &g), "capture");
+        Check(hipGraphInstantiate(&gw.gBegin, g, nullptr, nullptr, 0), "instantiate");
+        Check(hipGraphDestroy(g), "graph");
+        Check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "capture");
+        LaunchGmresIterationGroups(a, gw.t, gw.tab.p, d_b, stream);
+        Check(hipStreamEndCapture(stream, &g), "capture");
+        Check(hipGraphInstantiate(&gw.gIter, g, nullptr, nullptr, 0), "instantiate");
+        Check(hipGraphDestroy(g), "graph");
+        gw.graphB = d_b;
+        gw.graphTol = tol;
+    }
+    if (useGraph)
+        Check(hipGraphLaunch(gw.gBegin, stream), "graph launch");
+    else
+        LaunchGmresBeginGroups(a, gw.t, gw.tab.p, d_b, tol, stream);
+    long queued = 0;
+    long ahead = std::max(1, gw.lastIters);
+    bool all = false;
+    for (;;) {
+        long k = std::min(ahead, restart - queued);
+        for (long i = 0; i < k; ++i) {
+            if (useGraph)
+                Check(hipGraphLaunch(gw.gIter, stream), "graph launch");
+            else
+                LaunchGmresIterationGroups(a, gw.t, gw.tab.p, d_b, stream);
+        }
+        queued += k;
+        Check(hipMemcpyAsync(gw.hst, gw.st.p, sizeof(GmState) * (size_t)G, hipMemcpyDeviceToHost, stream), "d2h");
+        Check(hipStreamSynchronize(stream), "gmres");
+        all = true;
+        for (int g = 0; g < G; ++g)
+            all = all && gw.hst[g].done;
+        if (all || queued >= restart)
+            break;
+        ahead = 2;
+    }
+    if (!all)
+        throw std::runtime_error("grouped device GMRES ended without a stopping state");
+    int most = 1;
+    for (int g = 0; g < G; ++g) {
+        its[g] = gw.hst[g].j;
+        fls[g] = gw.hst[g].flag;
+        if (active[g])
+            most = std::max(most, gw.hst[g].j);
+    }
+    gw.lastIters = most;
+}
+
+void Case::SolveGroups(int nstep, const double* known_lu, long restart, double tol, double* lu, double* lt,
+                       int* iters, int* flags, int* solved)
+{
+    const int G = GroupCount();
+    if (!step0Done)
+        AssembleStep0(10);
+    if (!historyDone || maxn < 1)
+        throw std::runtime_error("history not assembled");
+    if (historyMixed || historyStreamed)
+        throw std::runtime_error("grouped models run over FP64 bands held on the device");
+    if (!convTiled || convTile != 16 || !convFiltered || !gmresDevice || restart > kGmDeviceMaxRestart)
+        throw std::runtime_error("grouped models run the filtered 16-step convolution and the device-driven GMRES");
+    if (nearTile != convTile)
+        BuildNearLists(convTile);
+    const long n3 = 3 * ne;
+    const int nstore = nstep + 1;
+    // GMRES workspace: vectors for all groups, scalars per group
+    if (gw.restart != restart) {
+        gw.release();
+        gw.restart = restart;
+        gw.V.alloc((size_t)n3 * (size_t)(restart + 1));
+        gw.W.alloc((size_t)n3 * (size_t)restart);
+        gw.u.alloc((size_t)n3);
+        gw.u2.alloc((size_t)n3);
+        gw.vh.alloc((size_t)n3);
+        gw.h.alloc((size_t)G * (size_t)(restart + 2));
+        gw.h2.alloc((size_t)G * (size_t)(restart + 2));
+        gw.f.alloc((size_t)G * (size_t)(restart + 2));
+        gw.QT.alloc((size_t)G * (size_t)(restart + 1) * (size_t)(restart + 1));
+        gw.R.alloc((size_t)G * (size_t)restart * (size_t)restart);
+        gw.part.alloc((size_t)G * (size_t)kGmBlocksMax * (size_t)(restart + 1));
+        gw.phibar.alloc((size_t)G);
+        gw.st.alloc((size_t)G * ((sizeof(GmState) + sizeof(double) - 1) / sizeof(double)));
+        gw.stag.alloc((size_t)G);
+        gw.active.alloc((size_t)G);
+        Check(hipHostMalloc((void**)&gw.hst, sizeof(GmState) * (size_t)G, hipHostMallocDefault), "pinned");
+        // block table: each group's reduction partition is the one of its own job
+        GmresGroupTable& t = gw.t;
+        t = GmresGroupTable{};
+        t.G = G;
+        if (gmBlocks < 1 || gmBlocks > kGmBlocksMax)
+            throw std::runtime_error("GMRES reduction blocks must be 1 .. " + std::to_string(kGmBlocksMax));
+        for (int g = 0; g <= G; ++g)
+            t.ele[g] = groupEle[(size_t)g];
+        for (int g = 0; g < G; ++g) {
+            const long neg = t.ele[g + 1] - t.ele[g];
+            t.nb[g] = (int)std::min<long>(gmBlocks, std::max<long>(1, (3 * neg + 255) / 256));
+            t.nbPre[g + 1] = t.nbPre[g] + t.nb[g];
+            t.rbPre[g + 1] = t.rbPre[g] + (int)((3 * neg + 255) / 256);
+            t.ebPre[g + 1] = t.ebPre[g] + (int)((neg + 255) / 256);
+        }
+        gw.tab.upload(&t, 1, stream);
+    }
+    gw.lastIters = 1;
+    DevBuf<double> d_bd, d_lu, d_lt, d_gU, d_gT, d_known, d_rhs, d_x, d_hfar, d_F;
+    const size_t outN = (size_t)nstore * (size_t)n3;
+    d_bd.upload(known_lu, outN, stream);
+    d_lu.alloc(outN);
+    d_lt.alloc(outN);
+    d_gU.alloc((size_t)dof.historyUBlockCount * 3 * (size_t)nstore);
+    d_gT.alloc((size_t)dof.historyTBlockCount * 3 * (size_t)nstore);
+    d_known.alloc(3 * (size_t)std::max<long>(1, dof.knownBlockCount));
+    d_rhs.alloc((size_t)n3);
+    d_x.alloc((size_t)n3);
+    if (farParts < 1 || farParts > kFarPartsMax)
+        throw std::runtime_error("far-part wavefronts per row must be 1 .. " + std::to_string(kFarPartsMax));
+    const int hfarParts = convPacked ? farParts : 1;
+    d_hfar.alloc((size_t)kFarPartsMax * (size_t)kConvTileMax * (size_t)n3);
+    d_F.alloc((size_t)ne * (size_t)(nstore + 1) * 6);
+    LaunchZero(d_F.p, (long)d_F.n, stream);
+    LaunchZero(d_gU.p, (long)d_gU.n, stream);
+    LaunchZero(d_gT.p, (long)d_gT.n, stream);
+    LaunchZero(d_known.p, (long)d_known.n, stream);
+    LaunchZero(d_lu.p, (long)outN, stream);
+    LaunchZero(d_lt.p, (long)outN, stream);
+    EventPool ev;
+    std::vector<std::array<hipEvent_t, 3>> stepEv;
+    std::vector<std::array<hipEvent_t, 2>> farEv;
+    stepEv.reserve((size_t)nstep);
+    std::vector<int> active((size_t)G, 1), it((size_t)G), fl((size_t)G);
+    for (int g = 0; g < G; ++g)
+        solved[g] = 0;
+    int K0 = 2;
+    for (int K = 1; K <= nstep; ++K) {
+        LaunchKnown(d_refs.p, d_bd.p + (size_t)K * (size_t)n3, (int)ne, d_known.p, dof.knownBlockCount, stream);
+        LaunchSpmv(ne, d_Krp.p, d_Kcol.p, d_Kval.p, d_known.p, d_rhs.p, stream);
+        std::array<hipEvent_t, 3> se = {ev.Get(), ev.Get(), ev.Get()};
+        Check(hipEventRecord(se[0], stream), "event");
+        if (K > 1) {
+            if (K == 2 || K - K0 >= convTile) {
+                K0 = K;
+                const int Tn = std::min(convTile, nstep - K0 + 1);
+                std::array<hipEvent_t, 2> fe = {ev.Get(), ev.Get()};
+                Check(hipEventRecord(fe[0], stream), "event");
+                LaunchFilterTile(d_refs.p, d_gU.p, d_gT.p, nstore, (int)ne, K0, std::max(0, K0 - convTile - 1), K0,
+                                 d_F.p, stream);
+                LaunchConvFarF(grid, d_refs.p, d_off.p, d_kb.p, d_len.p, (const double*)d_mts.p,
+                               (const double*)d_mgs.p, nullptr, nullptr, d_F.p, nstore + 1, K0, Tn, maxn, (int)ne,
+                               d_hfar.p, convPacked ? farParts : 0, stream);
+                Check(hipEventRecord(fe[1], stream), "event");
+                farEv.push_back(fe);
+            }
+            LaunchConvNear(d_refs.p, d_nearPtr.p, d_nearPair.p, d_nearField.p, d_off.p, d_kb.p, d_len.p,
+                           (const double*)d_mts.p, (const double*)d_mgs.p, nullptr, nullptr, d_gU.p, d_gT.p, nstore,
+                           K, K0, maxn, (int)ne, d_hfar.p, hfarParts, d_rhs.p, nearPerEntry, stream);
+        }
+        Check(hipEventRecord(se[1], stream), "event");
+        GmresDeviceGroups(d_rhs.p, d_x.p, restart, tol, active.data(), it.data(), fl.data());
+        Check(hipEventRecord(se[2], stream), "event");
+        stepEv.push_back(se);
+        bool any = false;
+        for (int g = 0; g < G; ++g) {
+            if (!active[(size_t)g])
+                continue;
+            iters[(size_t)g * nstep + (K - 1)] = it[(size_t)g];
+            flags[(size_t)g * nstep + (K - 1)] = fl[(size_t)g];
+            solved[g] = K;
+            if (fl[(size_t)g] != 0)
+                active[(size_t)g] = 0;
+            any = any || active[(size_t)g];
+        }
+        if (!any)
+            break;
+        double* slu = d_lu.p + (size_t)K * (size_t)n3;
+        double* slt = d_lt.p + (size_t)K * (size_t)n3;
+        LaunchScatter(d_refs.p, d_geo.p, d_bd.p + (size_t)K * (size_t)n3, d_x.p, (int)ne, slu, slt, stream);
+        LaunchState(d_refs.p, slu, slt, (int)ne, K, nstore, d_gU.p, d_gT.p, stream);
+    }
+    Check(hipStreamSynchronize(stream), "grouped solve");
+    Check(hipGetLastError(), "grouped solve launch");
+    Check(CopyOnStream(lu, d_lu.p, sizeof(double) * outN, hipMemcpyDeviceToHost, stream), "lu");
+    Check(CopyOnStream(lt, d_lt.p, sizeof(double) * outN, hipMemcpyDeviceToHost, stream), "lt");
+    // step 0: lt = 0, Convert (lu <-> lt for BCID 456), then LocToAbs
+    // (DBEM1/multidomain.cpp:3819-3822)
+    for (long i = 0; i < ne; ++i) {
+        double u[3], t[3];
+        for (int k = 0; k < 3; ++k) {
+            u[k] = known_lu[3 * i + k];
+            t[k] = 0.0;
+        }
+        if (bcid[(size_t)i] == 456)
+            for (int k = 0; k < 3; ++k)
+                std::swap(u[k], t[k]);
+        const double* F = geo[(size_t)i].trans;
+        lu[3 * i + 0] = F[0] * u[0] + F[3] * u[1] + F[6] * u[2];
+        lu[3 * i + 1] = F[1] * u[0] + F[4] * u[1] + F[7] * u[2];
+        lu[3 * i + 2] = F[2] * u[0] + F[5] * u[1] + F[8] * u[2];
+        lt[3 * i + 0] = F[0] * t[0] + F[3] * t[1] + F[6] * t[2];
+        lt[3 * i + 1] = F[1] * t[0] + F[4] * t[1] + F[7] * t[2];
+        lt[3 * i + 2] = F[2] * t[0] + F[5] * t[1] + F[8] * t[2];
+    }
+    double convTime = 0.0, gmresTime = 0.0, farTime = 0.0;
+    for (const auto& e : stepEv) {
+        convTime += ev.Elapsed(e[0], e[1]);
+        gmresTime += ev.Elapsed(e[1], e[2]);
+    }
+    for (const auto& e : farEv)
+        farTime += ev.Elapsed(e[0], e[1]);
+    timing[4] = convTime;
+    timing[5] = gmresTime;
+    timing[6] = farTime;
+    timing[7] = (double)farEv.size();
+    gw.release();
 }
 
 }  // namespace tdbem

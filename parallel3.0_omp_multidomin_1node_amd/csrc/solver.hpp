@@ -102,6 +102,19 @@ public:
     ~Case();
     void Create(long npoints, const double* points, long ne, const int* ele_pid, const int* bcid,
                 const MultiDomainInputConfig& cfg, double E, double v, double rho, double dt, double gap);
+    // The given (created) models stacked as the groups of one case: one
+    // launch per kernel for all of them, each model's results those of its
+    // own job (SolveGroups).
+    void CreateGroup(const std::vector<const Case*>& models);
+    bool Grouped() const { return groupEle.size() > 2; }
+    int GroupCount() const { return groupEle.size() >= 2 ? (int)groupEle.size() - 1 : 1; }
+    // The time loop of grouped models over FP64 resident bands: known_lu, lu
+    // and lt stacked per step ([K][3 ne], group g's elements at groupEle[g]);
+    // iters / flags [g][nstep], solved[g].  A group whose GMRES stops with a
+    // flag other than 0 takes no further steps (the others go on); its lu / lt
+    // from that step on are not its job's (the caller drops them).
+    void SolveGroups(int nstep, const double* known_lu, long restart, double tol, double* lu, double* lt,
+                     int* iters, int* flags, int* solved);
     void AssembleStep0(long maxleaf);
     void AssembleHistory(int maxn);
     double PrepareHistory(int maxn);  // cull ahead of AssembleHistory; returns the band bytes it will allocate
@@ -154,6 +167,7 @@ public:
 
     long ne = 0;
     double dt = 0.0;
+    std::vector<long> groupEle;  // element bounds of the stacked models (CreateGroup); empty otherwise
     MultiDomainModel model;
     GlobalDofMap dof;
     std::vector<int> bcid, domOf, surface;
@@ -204,6 +218,23 @@ public:
     DevBuf<double> d_u1, d_t1, d_t2;
 
 private:
+    void FinishCreate();
+    // grouped GMRES of one step: per group iterations and flags
+    void GmresDeviceGroups(const double* d_b, double* d_x, long restart, double tol, const int* active, int* its,
+                           int* fls);
+    struct GroupWork {
+        long restart = -1;
+        DevBuf<double> V, W, u, u2, vh, h, h2, QT, R, f, phibar, part, st;
+        DevBuf<int> stag, active;
+        DevBuf<GmresGroupTable> tab;
+        GmresGroupTable t{};
+        GmState* hst = nullptr;  // pinned copies of the group states
+        hipGraphExec_t gBegin = nullptr, gIter = nullptr;
+        const double* graphB = nullptr;
+        double graphTol = -1.0;
+        int lastIters = 1;
+        void release();
+    } gw;
     void Cull(int maxn);
     void BuildNearLists(int tile);
     void StagedQuadrature(double* mts, double* mgs, float* mts32, float* mgs32, unsigned char* alive);

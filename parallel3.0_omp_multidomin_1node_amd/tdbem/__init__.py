@@ -27,7 +27,7 @@ EXPORTED = [
     "tdbem_timings", "tdbem_timings_n", "tdbem_history_stats", "tdbem_device_synchronize",
     "tdbem_set_option", "tdbem_model_counts", "tdbem_read_model", "tdbem_read_bd", "tdbem_write_model_bin",
     "tdbem_write_bd_bin", "tdbem_write_validation", "tdbem_history_save", "tdbem_history_load",
-    "tdbem_run_models", "tdbem_release_cached_memory", "tdbem_autotune", "tdbem_dofmap_build",
+    "tdbem_run_models", "tdbem_run_models_grouped", "tdbem_release_cached_memory", "tdbem_autotune", "tdbem_dofmap_build",
     "tdbem_builder_create", "tdbem_builder_destroy", "tdbem_builder_add_block", "tdbem_builder_merge",
     "tdbem_builder_info", "tdbem_builder_build", "tdbem_builder_build_sym", "tdbem_ccsr_export_rows", "tdbem_solve_device", "tdbem_solve_batched_device",
     "tdbem_device_alloc", "tdbem_device_free", "tdbem_memcpy", "tdbem_solve_device_noalloc",
@@ -86,6 +86,8 @@ def lib():
         _pp = C.POINTER(C.c_void_p)
         L.tdbem_run_models.argtypes = [C.c_int, _pp, C.c_long, _ip, _pp, C.c_long, C.c_double, C.c_double, _pp, _pp,
                                        _pp, _pp, _ip]
+        L.tdbem_run_models_grouped.argtypes = [C.c_int, _pp, C.c_long, _ip, _pp, C.c_long, C.c_double, C.c_double,
+                                               _pp, _pp, _pp, _pp, _ip, C.POINTER(C.c_int)]
         L.tdbem_autotune.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                      C.POINTER(C.c_double)]
         L.tdbem_solve_device.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_long, C.c_double, C.c_void_p,
@@ -236,13 +238,16 @@ def release_cached_memory() -> None:
     _check(lib().tdbem_release_cached_memory(), "tdbem_release_cached_memory")
 
 
-def run_models(cases, knowns, nsteps, maxleaf=10, restart=200, tol=1e-5, budget_bytes=0.0):
+def run_models(cases, knowns, nsteps, maxleaf=10, restart=200, tol=1e-5, budget_bytes=0.0, grouped=False):
     """Whole jobs of independent cases together (tdbem_run_models): step 0,
     history for nsteps[i] steps and the time loop of every case, each on its
     own stream and host thread, with the history bands held at once kept
     within budget_bytes (0: 80 % of the free device memory; the bands are
-    released after each case's time loop).  Returns per case (lu, lt, iters,
-    flags, solved) as Case.solve does."""
+    released after each case's time loop).  grouped: the cases stacked in
+    rounds whose every kernel is one launch for all the round's cases
+    (tdbem_run_models_grouped; FP64 bands; run_models.rounds then holds the
+    number of rounds).  Returns per case (lu, lt, iters, flags, solved) as
+    Case.solve does."""
     n = len(cases)
     if len(knowns) != n or len(nsteps) != n:
         raise ValueError("one load history and one step count per case")
@@ -260,10 +265,20 @@ def run_models(cases, knowns, nsteps, maxleaf=10, restart=200, tol=1e-5, budget_
 
     ctx = (C.c_void_p * n)(*[c.h.value for c in cases])
     solved = np.zeros(n, np.int32)
-    _check(lib().tdbem_run_models(n, ctx, int(maxleaf), np.asarray(nsteps, np.int32), ptrs(keep), int(restart),
-                                  float(tol), float(budget_bytes), ptrs([o[0] for o in outs]), ptrs([o[1] for o in outs]),
-                                  ptrs([o[2] for o in outs]), ptrs([o[3] for o in outs]), solved),
-           "tdbem_run_models")
+    steps = np.asarray(nsteps, np.int32)
+    if grouped:
+        nr = C.c_int(0)
+        _check(lib().tdbem_run_models_grouped(n, ctx, int(maxleaf), steps, ptrs(keep), int(restart), float(tol),
+                                              float(budget_bytes), ptrs([o[0] for o in outs]),
+                                              ptrs([o[1] for o in outs]), ptrs([o[2] for o in outs]),
+                                              ptrs([o[3] for o in outs]), solved, C.byref(nr)),
+               "tdbem_run_models_grouped")
+        run_models.rounds = nr.value
+    else:
+        _check(lib().tdbem_run_models(n, ctx, int(maxleaf), steps, ptrs(keep), int(restart), float(tol),
+                                      float(budget_bytes), ptrs([o[0] for o in outs]), ptrs([o[1] for o in outs]),
+                                      ptrs([o[2] for o in outs]), ptrs([o[3] for o in outs]), solved),
+               "tdbem_run_models")
     res = []
     for c, ns, o, sv in zip(cases, nsteps, outs, solved):
         res.append((o[0].reshape(ns + 1, c.ne, 3), o[1].reshape(ns + 1, c.ne, 3), o[2], o[3], int(sv)))

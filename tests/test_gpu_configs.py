@@ -231,8 +231,10 @@ def test_config4i_64_load_cases_on_the_config2_mesh():
 def test_config4ii_16_models_together():
     """Config 4(ii): 16 cube models of 600-5766 elements, 200 steps each,
     through tdbem_run_models (their bands, 1.3 TB in all, held within the
-    device budget); the smallest, a middle and the largest model equal their
-    own single jobs bit for bit."""
+    device budget) and stacked in rounds (tdbem_run_models_grouped, one launch
+    per kernel for a round's models): every model's grouped results equal the
+    concurrent ones bit for bit, and the smallest, a middle and the largest
+    model equal their own single jobs bit for bit."""
     tdbem = _tdbem()
     from tdbem import mesh
     ws = mesh.models_config4ii()
@@ -243,6 +245,15 @@ def test_config4ii_16_models_together():
     for c in cs:
         c.close()
     assert all(r[4] == w.nstep for r, w in zip(res, ws))
+    print("case: 16 models grouped", flush=True)
+    cs = [_gpu_case(w) for w in ws]
+    resg = tdbem.run_models(cs, [w.known for w in ws], [w.nstep for w in ws], grouped=True)
+    for c in cs:
+        c.close()
+    print("rounds:", tdbem.run_models.rounds, flush=True)
+    for i, (a, b) in enumerate(zip(res, resg)):
+        for x, y in zip(a, b):
+            assert np.array_equal(np.asarray(x), np.asarray(y)), i
     order = np.argsort(sizes)
     for i in (order[0], order[len(order) // 2], order[-1]):
         w = ws[i]

@@ -137,12 +137,15 @@ def conv_far_bytes_per_launch(counts, npairs, ne, nhist, nstep, tile):
     return total / launches
 
 
-def batched_job(tdbem, w, d_known, d_lu, d_lt, nrhs, tol, mixed, gmres_device, keep=False, fft_levels=0):
+def batched_job(tdbem, w, d_known, d_lu, d_lt, nrhs, tol, mixed, gmres_device, keep=False, fft_levels=0,
+                fft_mfma=None):
     """One whole job of the batched form with device-resident inputs and outputs."""
     t0 = time.perf_counter()
     c = make_case(tdbem, w, mixed=mixed, gmres_device=gmres_device)
     if fft_levels:
         c.set_option("batched_fft_levels", fft_levels)
+    if fft_mfma is not None:
+        c.set_option("batched_fft_mfma", fft_mfma)
     c.assemble_step0(10)
     entries = c.assemble_history(w.nstep)
     it, fl, solved, times = c.solve_batched_device(d_known, nrhs, w.nstep, d_lu, d_lt, tol=tol)
@@ -451,6 +454,8 @@ def main():
                     help="also run config 2 with its history streamed under this device budget (several minutes)")
     ap.add_argument("--fft-levels", type=int, default=0,
                     help="levels of the batch's spectral far part (0: the library's default)")
+    ap.add_argument("--fft-mfma", type=int, default=None, choices=(0, 1),
+                    help="spectral products: 1 on the matrix cores (the library's default), 0 with vector FMAs")
     ap.add_argument("--host-buffers", action="store_true",
                     help="also time one batched job through the host-buffer entry point (PCIe-inclusive rate)")
     ap.add_argument("--models", action="store_true",
@@ -501,14 +506,15 @@ def main():
     d_lu, d_lt = tdbem.DeviceArray(cases.shape), tdbem.DeviceArray(cases.shape)
     for i in range(args.warmup):
         r = batched_job(tdbem, w, d_known, d_lu, d_lt, B, args.tol, args.mixed, args.gmres_device,
-                        fft_levels=args.fft_levels)
+                        fft_levels=args.fft_levels, fft_mfma=args.fft_mfma)
         log(f"warmup {i}: {r['wall']:.2f} s")
     tdbem.synchronize()
     t0 = time.perf_counter()
     runs = []
     for i in range(args.steps):
         runs.append(batched_job(tdbem, w, d_known, d_lu, d_lt, B, args.tol, args.mixed, args.gmres_device,
-                                keep=(i == args.steps - 1), fft_levels=args.fft_levels))
+                                keep=(i == args.steps - 1), fft_levels=args.fft_levels,
+                                fft_mfma=args.fft_mfma))
         log(f"step {i}: {runs[-1]['wall']:.2f} s")
     tdbem.synchronize()
     elapsed = time.perf_counter() - t0

@@ -79,52 +79,54 @@ void Case::GmresBatched(int B, const double* rhs, double* x, long restart, doubl
                "phibar");
         CheckB(hipStreamSynchronize(stream), "gmres");
     }
-    for (int j = 1; j <= restart; ++j) {
+    // the stopping test runs on the device after every iteration (the cases
+    // still iterating in w.left, which every iteration kernel checks first);
+    // the host reads the count after as many iterations as the last solve
+    // took and then every other one, so it does not wait on every iteration
+    CheckB(hipMemcpyAsync(w.its.p, its, sizeof(int) * (size_t)B, hipMemcpyHostToDevice, stream), "its");
+    CheckB(hipMemcpyAsync(w.flg.p, flg, sizeof(int) * (size_t)B, hipMemcpyHostToDevice, stream), "flags");
+    *w.hLeft = nact;
+    CheckB(hipMemcpyAsync(w.left.p, w.hLeft, sizeof(int), hipMemcpyHostToDevice, stream), "left");
+    const int* left = w.left.p;
+    int ahead = std::max(1, w.lastIters), since = 0;
+    int j = 1;
+    for (; j <= restart; ++j) {
         const double* Vj = w.V.p + (size_t)(j - 1) * (size_t)n * (size_t)B;
-        LaunchSpmmB(ne, d_Arp.p, d_Acol.p, d_Aval.p, Vj, w.u2.p, B, stream);
+        LaunchSpmmB(ne, d_Arp.p, d_Acol.p, d_Aval.p, Vj, w.u2.p, B, stream, left);
         // classical Gram-Schmidt twice: h = V'u, u -= V h; h2 = V'u, u -= V h2,
         // h += h2; h(j+1) = ||u||.  The first dot products share their pass
         // with the preconditioner, each update with the next dot products or
         // norm.
         LaunchPrecondDotB(ne, d_posLeaf.p, d_leafBegin.p, d_leafSize.p, d_matOff.p, d_inv.p, d_repid.p,
-                          d_outBlock.p, w.u2.p, w.u.p, B, w.V.p, j, w.part.p, nblkE, h + B, stream);
+                          d_outBlock.p, w.u2.p, w.u.p, B, w.V.p, j, w.part.p, nblkE, h + B, stream, left);
         LaunchPassB(w.V.p, n, B, w.u.p, nullptr, h + B, j, j, false, w.part.p, nblk, w.h2.p, h + B, nullptr,
-                    stream);
+                    stream, left);
         LaunchPassB(w.V.p, n, B, w.u.p, nullptr, w.h2.p, j, 0, true, w.part.p, nblk, nullptr, nullptr,
-                    h + (size_t)(j + 1) * B, stream);
-        LaunchGivensB(h, w.QT.p, w.R.p, w.f.p, w.phibar.p, w.stag.p, w.active.p, j, (int)restart, B, stream);
+                    h + (size_t)(j + 1) * B, stream, left);
+        LaunchGivensB(h, w.QT.p, w.R.p, w.f.p, w.phibar.p, w.stag.p, w.active.p, j, (int)restart, B, stream, left);
         LaunchWUpdateB(w.V.p, w.W.p, n, w.R.p, j, (int)restart, w.f.p, x, w.active.p, B, stream, w.u.p,
-                       h + (size_t)(j + 1) * B, w.V.p + (size_t)j * (size_t)n * (size_t)B);
-        LaunchSpmmB(ne, d_Arp.p, d_Acol.p, d_Aval.p, x, w.vh.p, B, stream);
+                       h + (size_t)(j + 1) * B, w.V.p + (size_t)j * (size_t)n * (size_t)B, left);
+        LaunchSpmmB(ne, d_Arp.p, d_Acol.p, d_Aval.p, x, w.vh.p, B, stream, left);
         LaunchPassB(nullptr, n, B, w.vh.p, rhs, nullptr, 0, 0, true, w.part.p, nblk, nullptr, nullptr, w.scal.p + B,
-                    stream);
-        CheckB(hipMemcpyAsync(nr.data(), w.scal.p + B, sizeof(double) * (size_t)B, hipMemcpyDeviceToHost, stream),
-               "d2h");
-        CheckB(hipMemcpyAsync(stag.data(), w.stag.p, sizeof(int) * (size_t)B, hipMemcpyDeviceToHost, stream), "d2h");
-        CheckB(hipStreamSynchronize(stream), "gmres iteration");
-        bool changed = false;
-        nact = 0;
-        for (int b = 0; b < B; ++b) {
-            if (!act[(size_t)b])
-                continue;
-            its[b] = j;
-            if (nr[(size_t)b] <= tol * bn[(size_t)b]) {
-                act[(size_t)b] = 0;
-                changed = true;
-            } else if (stag[(size_t)b]) {
-                act[(size_t)b] = 0;
-                flg[b] = 3;
-                changed = true;
-            } else {
-                ++nact;
-            }
+                    stream, left);
+        LaunchCheckB(w.scal.p, tol, w.stag.p, w.active.p, w.its.p, w.flg.p, j, B, w.left.p, stream);
+        if (++since >= ahead || j == restart) {
+            CheckB(hipMemcpyAsync(w.hLeft, w.left.p, sizeof(int), hipMemcpyDeviceToHost, stream), "d2h");
+            CheckB(hipStreamSynchronize(stream), "gmres iteration");
+            if (*w.hLeft == 0)
+                break;
+            since = 0;
+            ahead = 2;
         }
-        if (nact == 0)
-            return;
-        if (changed)
-            CheckB(hipMemcpyAsync(w.active.p, act.data(), sizeof(int) * (size_t)B, hipMemcpyHostToDevice, stream),
-                   "active");
     }
+    CheckB(hipMemcpyAsync(its, w.its.p, sizeof(int) * (size_t)B, hipMemcpyDeviceToHost, stream), "its");
+    CheckB(hipMemcpyAsync(flg, w.flg.p, sizeof(int) * (size_t)B, hipMemcpyDeviceToHost, stream), "flags");
+    CheckB(hipMemcpyAsync(act.data(), w.active.p, sizeof(int) * (size_t)B, hipMemcpyDeviceToHost, stream), "active");
+    CheckB(hipStreamSynchronize(stream), "gmres");
+    int most = 1;
+    for (int b = 0; b < B; ++b)
+        most = std::max(most, its[b]);
+    w.lastIters = most;
     for (int b = 0; b < B; ++b)
         if (act[(size_t)b])
             flg[b] = 1;
@@ -326,6 +328,12 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
     w.scal.alloc(2 * (size_t)B);
     w.stag.alloc((size_t)B);
     w.active.alloc((size_t)B);
+    w.its.alloc((size_t)B);
+    w.flg.alloc((size_t)B);
+    w.left.alloc(1);
+    if (!w.hLeft)
+        CheckB(hipHostMalloc((void**)&w.hLeft, sizeof(int), hipHostMallocDefault), "pinned");
+    w.lastIters = 1;
     w.part.alloc((size_t)kBatchRedBlocks * (size_t)(restart + 1) * B);
     DevBuf<double> d_bd, d_lu, d_lt, d_gU, d_gT, d_FU, d_FT, d_known, d_rhs, d_x, d_hfar;
     const size_t outN = (size_t)B * nstore * (size_t)n;

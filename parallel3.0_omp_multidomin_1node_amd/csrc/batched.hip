@@ -198,8 +198,10 @@ void LaunchFilterB(const double* g, long cols, int nstore, int B, int x0, int x1
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_spmm_b(long nrows, const int* __restrict__ rp, const int* __restrict__ col,
                                                 const double* __restrict__ val, const double* __restrict__ x,
-                                                double* __restrict__ y, int B)
+                                                double* __restrict__ y, int B, const int* __restrict__ left)
 {
+    if (left && *left == 0)
+        return;
     long row = (long)blockIdx.x * 4 + WaveIndex();
     int b = blockIdx.y * 64 + (threadIdx.x & 63);
     if (row >= nrows || b >= B)
@@ -219,10 +221,10 @@ __global__ void __launch_bounds__(256) k_spmm_b(long nrows, const int* __restric
 }
 
 void LaunchSpmmB(long nrows, const int* rp, const int* col, const double* val, const double* x, double* y, int B,
-                 hipStream_t s)
+                 hipStream_t s, const int* left)
 {
     dim3 grid((unsigned)((nrows + 3) / 4), (unsigned)((B + 63) / 64));
-    k_spmm_b<<<grid, 256, 0, s>>>(nrows, rp, col, val, x, y, B);
+    k_spmm_b<<<grid, 256, 0, s>>>(nrows, rp, col, val, x, y, B, left);
 }
 
 // ---------------------------------------------------------------------------
@@ -809,8 +811,10 @@ __global__ void __launch_bounds__(64 * kPassWaves) k_precond_dot_b(long n, const
                                                        const long* __restrict__ repid,
                                                        const long* __restrict__ outBlock, const double* __restrict__ x,
                                                        double* y, int B, const double* __restrict__ V, int nvD,
-                                                       double* __restrict__ partial)
+                                                       double* __restrict__ partial, const int* __restrict__ left)
 {
+    if (left && *left == 0)
+        return;
     __shared__ double red[kPassWaves][64];
     const int w = WaveIndex(), lane = threadIdx.x & 63;
     const int b = blockIdx.y * 64 + lane;
@@ -895,8 +899,11 @@ constexpr int kPassCols = 8;
 
 __global__ void __launch_bounds__(64 * kPassWaves) k_pass_b(const double* __restrict__ V, long n, int B, double* u,
                                                 const double* __restrict__ rhs, const double* __restrict__ hA,
-                                                int nvA, int nvD, int norm, double* __restrict__ partial)
+                                                int nvA, int nvD, int norm, double* __restrict__ partial,
+                                                const int* __restrict__ left)
 {
+    if (left && *left == 0)
+        return;
     __shared__ double red[kPassWaves][64];
     const int w = WaveIndex(), lane = threadIdx.x & 63;
     const int b = blockIdx.y * 64 + lane;
@@ -957,8 +964,10 @@ static_assert(kBatchRedBlocks <= 16 * 16, "the finish reads at most 16 partials 
 
 __global__ void __launch_bounds__(1024) k_finish_b(const double* __restrict__ partial, int nblocks, int nvD, int nout,
                                                    int B, double* __restrict__ out, double* __restrict__ addTo,
-                                                   double* __restrict__ normOut)
+                                                   double* __restrict__ normOut, const int* __restrict__ left)
 {
+    if (left && *left == 0)
+        return;
     __shared__ double red[16][64];
     const int k = blockIdx.x;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -993,24 +1002,28 @@ __global__ void __launch_bounds__(1024) k_finish_b(const double* __restrict__ pa
 }
 
 void LaunchPassB(const double* V, long n, int B, double* u, const double* rhs, const double* hA, int nvA, int nvD,
-                 bool norm, double* partial, int nblocks, double* out, double* addTo, double* normOut, hipStream_t s)
+                 bool norm, double* partial, int nblocks, double* out, double* addTo, double* normOut, hipStream_t s,
+                 const int* left)
 {
     const unsigned by = (unsigned)((B + 63) / 64);
-    k_pass_b<<<dim3((unsigned)nblocks, by), 64 * kPassWaves, 0, s>>>(V, n, B, u, rhs, hA, nvA, nvD, norm ? 1 : 0, partial);
+    k_pass_b<<<dim3((unsigned)nblocks, by), 64 * kPassWaves, 0, s>>>(V, n, B, u, rhs, hA, nvA, nvD, norm ? 1 : 0, partial,
+                                                                     left);
     const int nout = nvD + (norm ? 1 : 0);
     if (nout > 0)
-        k_finish_b<<<dim3((unsigned)nout, by), 1024, 0, s>>>(partial, nblocks, nvD, nout, B, out, addTo, normOut);
+        k_finish_b<<<dim3((unsigned)nout, by), 1024, 0, s>>>(partial, nblocks, nvD, nout, B, out, addTo, normOut,
+                                                             left);
 }
 
 void LaunchPrecondDotB(long n, const long* posLeaf, const long* leafBegin, const long* leafSize, const long* matOff,
                        const double* inv, const long* repid, const long* outBlock, const double* x, double* y, int B,
-                       const double* V, int nvD, double* partial, int nblocks, double* h, hipStream_t s)
+                       const double* V, int nvD, double* partial, int nblocks, double* h, hipStream_t s,
+                       const int* left)
 {
     const unsigned by = (unsigned)((B + 63) / 64);
     k_precond_dot_b<<<dim3((unsigned)nblocks, by), 64 * kPassWaves, 0, s>>>(n, posLeaf, leafBegin, leafSize, matOff, inv, repid,
-                                                                outBlock, x, y, B, V, nvD, partial);
+                                                                outBlock, x, y, B, V, nvD, partial, left);
     if (nvD > 0)
-        k_finish_b<<<dim3((unsigned)nvD, by), 1024, 0, s>>>(partial, nblocks, nvD, nvD, B, h, nullptr, nullptr);
+        k_finish_b<<<dim3((unsigned)nvD, by), 1024, 0, s>>>(partial, nblocks, nvD, nvD, B, h, nullptr, nullptr, left);
 }
 
 void LaunchMultiDotB(const double* V, long n, int nv, const double* u, int B, double* partial, int nblocks,
@@ -1044,8 +1057,11 @@ void LaunchScaleB(const double* x, const double* a, double* y, long n, int B, hi
 // Givens rotation in QT form for column j of every active load case
 // (DBEM1/pre_gmres.cpp:1492-1533); one lane per case.
 __global__ void __launch_bounds__(64) k_givens_b(double* h, double* QT, double* R, double* f, double* phibar,
-                                                 int* stag, const int* __restrict__ active, int j, int restart, int B)
+                                                 int* stag, const int* __restrict__ active, int j, int restart, int B,
+                                                 const int* __restrict__ left)
 {
+    if (left && *left == 0)
+        return;
     int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= B || !active[b])
         return;
@@ -1091,9 +1107,46 @@ __global__ void __launch_bounds__(64) k_givens_b(double* h, double* QT, double* 
 }
 
 void LaunchGivensB(double* h, double* QT, double* R, double* f, double* phibar, int* stag, const int* active, int j,
-                   int restart, int B, hipStream_t s)
+                   int restart, int B, hipStream_t s, const int* left)
 {
-    k_givens_b<<<(unsigned)((B + 63) / 64), 64, 0, s>>>(h, QT, R, f, phibar, stag, active, j, restart, B);
+    k_givens_b<<<(unsigned)((B + 63) / 64), 64, 0, s>>>(h, QT, R, f, phibar, stag, active, j, restart, B, left);
+}
+
+// The stopping test of iteration j for every active case (DBEM1/pre_gmres.cpp
+// true-residual test, as the host loop made it): its = j; a case leaves when
+// its residual norm (scal[B + b]) is <= tol times ||b|| (scal[b]), or with
+// flag 3 when the Givens step found stagnation; left = the cases still
+// active.  One workgroup.
+__global__ void __launch_bounds__(256) k_check_b(const double* __restrict__ scal, double tol,
+                                                 const int* __restrict__ stag, int* active, int* its, int* flg,
+                                                 int j, int B, int* left)
+{
+    __shared__ int cnt;
+    if (threadIdx.x == 0)
+        cnt = 0;
+    __syncthreads();
+    for (int b = threadIdx.x; b < B; b += blockDim.x) {
+        if (!active[b])
+            continue;
+        its[b] = j;
+        if (scal[B + b] <= tol * scal[b]) {
+            active[b] = 0;
+        } else if (stag[b]) {
+            active[b] = 0;
+            flg[b] = 3;
+        } else {
+            atomicAdd(&cnt, 1);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        *left = cnt;
+}
+
+void LaunchCheckB(const double* scal, double tol, const int* stag, int* active, int* its, int* flg, int j, int B,
+                  int* left, hipStream_t s)
+{
+    k_check_b<<<1, 256, 0, s>>>(scal, tol, stag, active, its, flg, j, B, left);
 }
 
 // W_{j-1} = (V_{j-1} - sum_{l<j} W_{l-1} R(l, j)) / R(j, j) and x += f_j W_{j-1}
@@ -1103,8 +1156,11 @@ __global__ void __launch_bounds__(256) k_wupdate_b(const double* __restrict__ V,
                                                    const double* __restrict__ R, int j, int restart,
                                                    const double* __restrict__ f, double* __restrict__ x,
                                                    const int* __restrict__ active, int B, const double* __restrict__ u,
-                                                   const double* __restrict__ hs, double* Vj)
+                                                   const double* __restrict__ hs, double* Vj,
+                                                   const int* __restrict__ left)
 {
+    if (left && *left == 0)
+        return;
     long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= n * B)
         return;
@@ -1124,9 +1180,11 @@ __global__ void __launch_bounds__(256) k_wupdate_b(const double* __restrict__ V,
 }
 
 void LaunchWUpdateB(const double* V, double* W, long n, const double* R, int j, int restart, const double* f,
-                    double* x, const int* active, int B, hipStream_t s, const double* u, const double* hs, double* Vj)
+                    double* x, const int* active, int B, hipStream_t s, const double* u, const double* hs, double* Vj,
+                    const int* left)
 {
-    k_wupdate_b<<<(unsigned)((n * B + 255) / 256), 256, 0, s>>>(V, W, n, R, j, restart, f, x, active, B, u, hs, Vj);
+    k_wupdate_b<<<(unsigned)((n * B + 255) / 256), 256, 0, s>>>(V, W, n, R, j, restart, f, x, active, B, u, hs, Vj,
+                                                                left);
 }
 
 }  // namespace tdbem

@@ -32,9 +32,11 @@ void LaunchStateB(const NodeRefs* refs, const double* lu, const double* lt, int 
 // * 3 + c) * B + b] (row nstore of each column is left zero).
 void LaunchFilterB(const double* g, long cols, int nstore, int B, int x0, int x1, int K0, double* F, hipStream_t s);
 
-// y = A x for B vectors (block CCSR, row-major 3x3 blocks).
+// y = A x for B vectors (block CCSR, row-major 3x3 blocks).  The GMRES
+// kernels below take an optional device counter `left` of the cases still
+// iterating and return at once when it is 0.
 void LaunchSpmmB(long nrows, const int* rp, const int* col, const double* val, const double* x, double* y, int B,
-                 hipStream_t s);
+                 hipStream_t s, const int* left = nullptr);
 
 // Far part of the tile [K0, K0+Tn) for B load cases into hfar[(i*3ne + r)*B + b]:
 // on the FP64 matrix cores with the columns of each row split into parts > 0
@@ -118,7 +120,8 @@ void LaunchPrecondB(long n, const long* posLeaf, const long* leafBegin, const lo
 // rows); partial holds nblocks * nvD * B doubles.
 void LaunchPrecondDotB(long n, const long* posLeaf, const long* leafBegin, const long* leafSize, const long* matOff,
                        const double* inv, const long* repid, const long* outBlock, const double* x, double* y, int B,
-                       const double* V, int nvD, double* partial, int nblocks, double* h, hipStream_t s);
+                       const double* V, int nvD, double* partial, int nblocks, double* h, hipStream_t s,
+                       const int* left = nullptr);
 // One pass over the rows of a batch: v = u (rhs - u when rhs is given), v -=
 // sum_{k < nvA} hA[k] V_k, stored back into u when it changed; then per load
 // case out[k] = <V_k, v> for k < nvD (addTo[k] += out[k] when given) and,
@@ -126,7 +129,8 @@ void LaunchPrecondDotB(long n, const long* posLeaf, const long* leafBegin, const
 // (nvD + 1) * B doubles.  Each sum is taken in a fixed order (blocks of rows,
 // then blocks), so results do not depend on timing.
 void LaunchPassB(const double* V, long n, int B, double* u, const double* rhs, const double* hA, int nvA, int nvD,
-                 bool norm, double* partial, int nblocks, double* out, double* addTo, double* normOut, hipStream_t s);
+                 bool norm, double* partial, int nblocks, double* out, double* addTo, double* normOut, hipStream_t s,
+                 const int* left = nullptr);
 // h[k*B + b] = <V_k(:, b), u(:, b)> for k < nv
 void LaunchMultiDotB(const double* V, long n, int nv, const double* u, int B, double* partial, int nblocks,
                      double* h, hipStream_t s);
@@ -134,11 +138,16 @@ void LaunchNormB(const double* x, long n, int B, double* partial, int nblocks, d
 // y(:, b) = x(:, b) / a[b] (a <= 1e-12 gives 0)
 void LaunchScaleB(const double* x, const double* a, double* y, long n, int B, hipStream_t s);
 void LaunchGivensB(double* h, double* QT, double* R, double* f, double* phibar, int* stag, const int* active, int j,
-                   int restart, int B, hipStream_t s);
+                   int restart, int B, hipStream_t s, const int* left = nullptr);
+// The stopping test of iteration j on the device: every active case records
+// its = j and leaves when ||r|| <= tol ||b|| (scal[B + b] and scal[b]) or,
+// with flg 3, on stagnation; left receives the count of cases still active.
+void LaunchCheckB(const double* scal, double tol, const int* stag, int* active, int* its, int* flg, int j, int B,
+                  int* left, hipStream_t s);
 // W / x update of column j for the active cases; with Vj, also Vj = u / hs
 // (the scale of LaunchScaleB) for every case
 void LaunchWUpdateB(const double* V, double* W, long n, const double* R, int j, int restart, const double* f,
                     double* x, const int* active, int B, hipStream_t s, const double* u = nullptr,
-                    const double* hs = nullptr, double* Vj = nullptr);
+                    const double* hs = nullptr, double* Vj = nullptr, const int* left = nullptr);
 
 }  // namespace tdbem

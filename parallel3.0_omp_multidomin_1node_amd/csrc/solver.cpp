@@ -241,6 +241,7 @@ Case::~Case()
     if (stream)
         (void)hipStreamSynchronize(stream);
     gw.release();
+    bw.release();
     ReleaseGraphs();
     if (h_gst)
         (void)hipHostFree(h_gst);

@@ -73,16 +73,25 @@ struct AdjointKeep {
     DevBuf<double> X, gU, gT, bd;
 };
 
-// Workspace of the lockstep GMRES of a batch of load cases.
+// Workspace of the lockstep GMRES of a batch of load cases: the per-case
+// iteration counts, flags and activity on the device, the count of cases
+// still iterating (device and pinned host copy) and the last solve's
+// iteration count (how many iterations go before the host first looks).
 struct BatchWork {
     DevBuf<double> V, W, u, u2, vh, h, h2, QT, R, f, phibar, scal, part;
-    DevBuf<int> stag, active;
+    DevBuf<int> stag, active, its, flg, left;
+    int* hLeft = nullptr;
+    int lastIters = 1;
     void release()
     {
         for (DevBuf<double>* b : {&V, &W, &u, &u2, &vh, &h, &h2, &QT, &R, &f, &phibar, &scal, &part})
             b->free();
-        stag.free();
-        active.free();
+        for (DevBuf<int>* b : {&stag, &active, &its, &flg, &left})
+            b->free();
+        if (hLeft)
+            (void)hipHostFree(hLeft);
+        hLeft = nullptr;
+        lastIters = 1;
     }
 };
 

@@ -452,6 +452,8 @@ def main():
                          "FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--streamed", type=float, default=0.0, metavar="GB",
                     help="also run config 2 with its history streamed under this device budget (several minutes)")
+    ap.add_argument("--no-longer-levels", dest="longer_levels", action="store_false",
+                    help="skip the extra job of the headline batch with three spectral levels")
     ap.add_argument("--fft-levels", type=int, default=0,
                     help="levels of the batch's spectral far part (0: the library's default)")
     ap.add_argument("--fft-mfma", type=int, default=None, choices=(0, 1),
@@ -532,11 +534,19 @@ def main():
     all_solved = bool((last["solved"] == w.nstep).all())
     # the timed batch's results, then the plain FP64 form of the same box
     BU, BT = d_lu.download(), d_lt.download()
+    # the same batch with sub-filters of 16, 32 and 64 lags (one whole job,
+    # timed on its own; its errors against the same FP64 jobs below)
+    longer = None
+    if args.longer_levels and not args.fft_levels:
+        r3 = batched_job(tdbem, w, d_known, d_lu, d_lt, B, args.tol, args.mixed, args.gmres_device, fft_levels=3,
+                         fft_mfma=args.fft_mfma)
+        longer = {"wall": r3["wall"], "timings": r3["timings"], "U": d_lu.download(), "T": d_lt.download()}
+        log(f"three spectral levels: {r3['wall']:.2f} s")
     del d_known, d_lu, d_lt
     ek = [int(x) for x in args.error_cases.split(",") if x.strip() and int(x) < B]
     d1 = tdbem.DeviceArray(cases.shape[1:])
     o1u, o1t = tdbem.DeviceArray(cases.shape[1:]), tdbem.DeviceArray(cases.shape[1:])
-    fp64_runs, eu, et = [], 0.0, 0.0
+    fp64_runs, eu, et, eu3, et3 = [], 0.0, 0.0, 0.0, 0.0
     marks = sorted({k for k in (w.nstep // 4, w.nstep // 2, (3 * w.nstep) // 4, (9 * w.nstep) // 10, w.nstep) if k > 0})
     prof_u = {k: 0.0 for k in marks}
     for j, b in enumerate(ek if ek else [0]):
@@ -549,8 +559,13 @@ def main():
             et = max(et, float(np.abs(BT[b] - T).max() / np.abs(T).max()))
             for k in marks:  # the largest difference over steps <= k
                 prof_u[k] = max(prof_u[k], float(np.abs(BU[b, :k + 1] - U[:k + 1]).max() / np.abs(U).max()))
+            if longer:
+                eu3 = max(eu3, float(np.abs(longer["U"][b] - U).max() / np.abs(U).max()))
+                et3 = max(et3, float(np.abs(longer["T"][b] - T).max() / np.abs(T).max()))
         log(f"fp64 single job (case {b}): {r['wall']:.3f} s")
     del d1, o1u, o1t, BU, BT
+    if longer:
+        del longer["U"], longer["T"]
     fp64_wall = float(np.mean([r["wall"] for r in fp64_runs[1:]] if len(fp64_runs) > 1 else [fp64_runs[0]["wall"]]))
     fp64_sps = w.nstep / fp64_wall
     ftm = fp64_runs[-1]["timings"]
@@ -582,6 +597,14 @@ def main():
                           "error_bound": 1e-6, "same_gmres_tol": args.tol},
         "all_solved": all_solved,
         "gmres_inner_iterations_mean": float(np.mean(last["iters"])),
+        "longer_sub_filters": None if not longer else {
+            "levels": 3, "sub_filter_lags": [16, 32, 64],
+            "case_steps_per_s": round(B * w.nstep / longer["wall"], 3),
+            "vs_plain_fp64": round(B * w.nstep / longer["wall"] / fp64_sps, 3),
+            "max_rel_err_u": eu3, "max_rel_err_t": et3, "error_cases": ek,
+            "conv_spectral_products_s": round(float(longer["timings"].get("conv_spectral_products", 0.0)), 4),
+            "note": "one whole job of the same batch with option batched_fft_levels 3, timed alone; not the "
+                    "headline because its traction error exceeds 1e-6 (the displacement error, the bound, does not)"},
         "breakdown_s": {k: round(float(v), 4) for k, v in tm.items()
                         if k not in NOT_SECONDS and not k.startswith("adjoint_")},
     }

@@ -39,13 +39,13 @@ def _case(tdbem, w):
                       itf_rows=m.interfaces)
 
 
-def _alone(tdbem, ws, tol):
+def _alone(tdbem, ws, tol, restart=200):
     out = []
     for w in ws:
         c = _case(tdbem, w)
         c.assemble_step0(10)
         c.assemble_history(w.nstep)
-        out.append(c.solve(w.known, w.nstep, tol=tol)[:5])
+        out.append(c.solve(w.known, w.nstep, restart=restart, tol=tol)[:5])
         c.close()
     return out
 
@@ -92,3 +92,30 @@ def test_grouped_models_split_rounds_by_step_count_and_refuse_mixed_bands():
         tdbem.run_models(cs, [w.known for w in ws], [w.nstep for w in ws], tol=1e-10, grouped=True)
     for c in cs:
         c.close()
+
+
+def test_grouped_models_stop_where_their_own_jobs_stop():
+    """A short restart length at which some models' GMRES stops with flag 1
+    (restart reached) at some step while the others finish: in the grouped
+    run each stopped model keeps its flags, iteration counts, solved steps
+    and zero outputs from its stopping step on, as its own job leaves them,
+    and the others go on to equal their own jobs bit for bit."""
+    tdbem = _tdbem()
+    ws = _models(lambda i: 16)
+    print("case: grouped models with stopping GMRES", [len(w.mesh.ele) for w in ws], flush=True)
+    for restart in range(2, 12):
+        alone = _alone(tdbem, ws, 1e-12, restart)
+        stopped = [a[4] < w.nstep or bool(np.asarray(a[3]).any()) for a, w in zip(alone, ws)]
+        if any(stopped) and not all(stopped):
+            break
+    else:
+        pytest.fail("no restart length mixes stopped and finished models")
+    print("restart", restart, "stopped", stopped, flush=True)
+    cs = [_case(tdbem, w) for w in ws]
+    got = tdbem.run_models(cs, [w.known for w in ws], [w.nstep for w in ws], restart=restart, tol=1e-12,
+                           grouped=True)
+    for c in cs:
+        c.close()
+    for w, a, g in zip(ws, alone, got):
+        for x, y in zip(a, g):
+            assert np.array_equal(np.asarray(x), np.asarray(y)), (w.name, restart)

@@ -148,10 +148,14 @@ def batched_job(tdbem, w, d_known, d_lu, d_lt, nrhs, tol, mixed, gmres_device, k
         c.set_option("batched_fft_mfma", fft_mfma)
     c.assemble_step0(10)
     entries = c.assemble_history(w.nstep)
+    tdbem.synchronize()
+    t1 = time.perf_counter()
     it, fl, solved, times = c.solve_batched_device(d_known, nrhs, w.nstep, d_lu, d_lt, tol=tol)
     tdbem.synchronize()
-    wall = time.perf_counter() - t0
-    info = dict(wall=wall, entries=entries, iters=it, flags=fl, solved=solved, timings=c.timings())
+    t2 = time.perf_counter()
+    wall = t2 - t0
+    info = dict(wall=wall, assembly=t1 - t0, loop=t2 - t1, entries=entries, iters=it, flags=fl, solved=solved,
+                timings=c.timings())
     if keep:
         info["case"] = c
     else:
@@ -595,6 +599,9 @@ def main():
                           "max_rel_err_u": eu, "max_rel_err_t": et, "error_cases": ek,
                           "max_rel_err_u_up_to_step": {str(k): v for k, v in prof_u.items()},
                           "error_bound": 1e-6, "same_gmres_tol": args.tol},
+        "time_loop_case_steps_per_s": round(B * w.nstep / last["loop"], 3),
+        "assembly_s": round(last["assembly"], 4),
+        "assembled_blocks_per_s": round(2.0 * last["entries"] / max(float(tm.get("history", 0.0)), 1e-9), 1),
         "all_solved": all_solved,
         "gmres_inner_iterations_mean": float(np.mean(last["iters"])),
         "longer_sub_filters": None if not longer else {

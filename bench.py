@@ -58,7 +58,8 @@ HEADLINE_TRAFFIC = {("stacked_2x2x8_h0.125", 600, 64, 2, 16): 393554456166.4}
 # timings entries that are counts, flags, flops or bytes rather than seconds
 NOT_SECONDS = {"conv_far_launches", "conv_spectral_launches", "conv_spectral_flops", "conv_spectral_bytes",
                "spectral_levels", "spectral_level_stats",
-               "spectral", "stream_slabs", "quad_evals_t2", "quad_evals_t1u", "quad_evals_regular", "quad_records"}
+               "spectral", "stream_slabs", "quad_evals_t2", "quad_evals_t1u", "quad_evals_regular", "quad_records",
+               "quad_pw_entries", "quad_pw_waves"}
 SECONDARY_BUDGET_S = 420.0  # secondary blocks run only while the bench has used less than this
 
 
@@ -296,7 +297,8 @@ def config2_report(tdbem, gmres_device, far_parts=0):
                 "frac": round(qflops / tm["quadrature"] / 1e12 / FP64_PEAK_TFLOPS, 4),
                 "flops_per_assembly": qflops, "time_s": round(tm["quadrature"], 4),
                 "evaluations": {k: qc[k] for k in ("quad_evals_t2", "quad_evals_t1u", "quad_evals_regular",
-                                                   "quad_records")}},
+                                                   "quad_records", "quad_pw_entries", "quad_pw_waves")},
+                "patchwise_lane_occupancy": round(qc["quad_pw_entries"] / max(64.0 * qc["quad_pw_waves"], 1.0), 4)},
             "roofline": {"kernel": f"k_conv_far_q ({fp} wavefronts per row, 16 steps per launch)",
                          "bound": "hbm", "achieved": round(bpl / avg / 1e9, 2), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(bpl / avg / 1e9 / HBM_PEAK_GBS, 4),

@@ -244,7 +244,8 @@ int tdbem_timings(tdbem_ctx* ctx, double* t8);
  * t[24] regular (entry, point), t[25] staged (source, point) records; per
  * level l < 4 of the spectral part t[26 + l] its products seconds, t[30 + l]
  * their flops, t[34 + l] their launches, t[38 + l] their bytes, and t[42]
- * the number of levels.  Values past the known ones are 0. */
+ * the number of levels; with "quad_count" 1, t[43] the patch-wise entries
+ * and t[44] the patch-wise waves of the last staged quadrature.  Values past the known ones are 0. */
 int tdbem_timings_n(tdbem_ctx* ctx, int n, double* t);
 
 /* Number of stored history band entries per lag j = 1..maxn

@@ -728,7 +728,7 @@ int tdbem_timings_n(tdbem_ctx* ctx, int n, double* t)
     if (!ctx || !t || n < 0)
         return Fail("tdbem_timings_n: invalid arguments");
     for (int i = 0; i < n; ++i)
-        t[i] = i < 43 ? ctx->c.timing[i] : 0.0;
+        t[i] = i < 45 ? ctx->c.timing[i] : 0.0;
     return 0;
 }
 

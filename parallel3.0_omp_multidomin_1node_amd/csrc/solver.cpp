@@ -1016,6 +1016,16 @@ void Case::PackPwWaves()
     d_pwWaves.alloc((size_t)std::max<long>(1, pwWaves));
     LaunchPwPack(grid, d_refs.p, d_pwcnt.p, (int)ne, d_woff.p, nullptr, d_pwWaves.p, stream);
     Check(hipStreamSynchronize(stream), "pw pack");
+    if (quadCount) {  // lane occupancy of the patch-wise waves: entries / (64 waves)
+        std::vector<int> cnt((size_t)npairs);
+        Check(CopyOnStream(cnt.data(), d_pwcnt.p, sizeof(int) * (size_t)npairs, hipMemcpyDeviceToHost, stream),
+              "pw counts");
+        double tot = 0.0;
+        for (int c : cnt)
+            tot += c;
+        timing[43] = tot;
+        timing[44] = (double)pwWaves;
+    }
     pwPacked = frozen;
 }
 

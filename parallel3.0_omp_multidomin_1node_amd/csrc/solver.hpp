@@ -194,8 +194,10 @@ public:
     // adjoint: 9 forward solve, 10 backward sweep, 11 step-0 and self-table
     // differences with the correlation store, 12 history-block derivatives
     // streamed history: 20 band slabs, 21 their regeneration in the time loop (device events);
-    // 22-25 the staged quadrature's evaluation counts (with quadCount)
-    double timing[43] = {0};
+    // 22-25 the staged quadrature's evaluation counts (with quadCount); 26-42
+    // spectral levels; 43, 44 the patch-wise entries and waves of the last
+    // staged quadrature (with quadCount)
+    double timing[45] = {0};
     bool convTiled = true;   // time-blocked convolution (false: one full pass per step)
     int convTile = 16;       // steps per far-part tile (8 or 16); fixes the near lists built by the cull
     bool adjointAnalytic = true;  // history-block derivatives analytic (false: operator differences)

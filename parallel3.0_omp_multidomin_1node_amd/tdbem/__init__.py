@@ -522,8 +522,8 @@ class Case:
         return fp.value, gb.value, ms.value
 
     def timings(self):
-        t = np.zeros(43)
-        _check(lib().tdbem_timings_n(self.h, 43, t), "tdbem_timings_n")
+        t = np.zeros(45)
+        _check(lib().tdbem_timings_n(self.h, 45, t), "tdbem_timings_n")
         out = dict(singular=t[0], cull=t[1], history=t[2], step0=t[3], convolution=t[4], gmres=t[5],
                    conv_far=t[6], conv_far_launches=int(t[7]), quadrature=t[8], adjoint_forward=t[9],
                    adjoint_backward=t[10], adjoint_step0_differences=t[11], adjoint_history_derivatives=t[12],
@@ -531,7 +531,8 @@ class Case:
                    spectral_setup=t[16], conv_spectral_products=t[17], conv_spectral_flops=t[18],
                    conv_spectral_bytes=t[19], stream_slabs=int(t[20]), stream_far=t[21],
                    quad_evals_t2=int(t[22]), quad_evals_t1u=int(t[23]), quad_evals_regular=int(t[24]),
-                   quad_records=int(t[25]), spectral_levels=int(t[42]))
+                   quad_records=int(t[25]), spectral_levels=int(t[42]), quad_pw_entries=int(t[43]),
+                   quad_pw_waves=int(t[44]))
         # per level of the spectral part: products seconds, flops, launches, bytes
         out["spectral_level_stats"] = [dict(L=16 << l, products=t[26 + l], flops=t[30 + l], launches=int(t[34 + l]),
                                             bytes=t[38 + l]) for l in range(int(t[42]))]

@@ -75,8 +75,24 @@ __global__ void __launch_bounds__(256) k_gm_fill(const GmState* st, double* __re
         b[r] = 0.0;
 }
 
+// Component c of block row i of y = A x: the sum over the row's blocks in
+// CCSR order, each block's three products in the order of the host-driven
+// SpMV, so a thread per (row, component) gives its bits exactly.
+__device__ __forceinline__ double RowComponent(const int* __restrict__ rp, const int* __restrict__ col,
+                                               const double* __restrict__ val, const double* __restrict__ x, long i,
+                                               int c)
+{
+    double a = 0.0;
+    for (int j = rp[i]; j < rp[i + 1]; ++j) {
+        const double* v = val + 9 * (long)j + 3 * c;
+        const double* xc = x + 3 * (long)col[j];
+        a += v[0] * xc[0] + v[1] * xc[1] + v[2] * xc[2];
+    }
+    return a;
+}
+
 // y = A x with x = V(:, st->j + colOff) (colOff < 0: x itself), or the
-// residual y = b - A x when b is given
+// residual y = b - A x when b is given; one thread per (block row, component)
 __global__ void __launch_bounds__(256) k_gm_spmv(const GmState* st, long nrows, const int* __restrict__ rp,
                                                  const int* __restrict__ col, const double* __restrict__ val,
                                                  const double* __restrict__ x, long ldx, int colOff,
@@ -84,28 +100,16 @@ __global__ void __launch_bounds__(256) k_gm_spmv(const GmState* st, long nrows, 
 {
     if (Done(st))
         return;
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nrows)
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 3 * nrows)
         return;
     if (colOff >= 0)
         x += (long)(NextJ(st) - 1 + colOff) * ldx;
-    double b0 = 0.0, b1 = 0.0, b2 = 0.0;
-    for (int j = rp[i]; j < rp[i + 1]; ++j) {
-        const double* v = val + 9 * (long)j;
-        const double* xc = x + 3 * (long)col[j];
-        double x0 = xc[0], x1 = xc[1], x2 = xc[2];
-        b0 += v[0] * x0 + v[1] * x1 + v[2] * x2;
-        b1 += v[3] * x0 + v[4] * x1 + v[5] * x2;
-        b2 += v[6] * x0 + v[7] * x1 + v[8] * x2;
-    }
-    if (b) {
-        b0 = b[3 * i] - b0;
-        b1 = b[3 * i + 1] - b1;
-        b2 = b[3 * i + 2] - b2;
-    }
-    y[3 * i] = b0;
-    y[3 * i + 1] = b1;
-    y[3 * i + 2] = b2;
+    const long i = t / 3;
+    double a = RowComponent(rp, col, val, x, i, (int)(t - 3 * i));
+    if (b)
+        a = b[t] - a;
+    y[t] = a;
 }
 
 __global__ void __launch_bounds__(256) k_gm_precond(const GmState* st, long n, const long* __restrict__ posLeaf,
@@ -329,7 +333,7 @@ unsigned Blocks(long n)
 void LaunchPre(const GmresDevArgs& a, const double* x, double* y, hipStream_t s)
 {
     if (a.prp)
-        k_gm_spmv<<<Blocks(a.ne), 256, 0, s>>>(a.st, a.ne, a.prp, a.pcol, a.pval, x, 3 * a.ne, -1, nullptr, y);
+        k_gm_spmv<<<Blocks(3 * a.ne), 256, 0, s>>>(a.st, a.ne, a.prp, a.pcol, a.pval, x, 3 * a.ne, -1, nullptr, y);
     else
         k_gm_precond<<<Blocks(3 * a.ne), 256, 0, s>>>(a.st, a.ne, a.posLeaf, a.leafBegin, a.leafSize, a.matOff,
                                                       a.inv, a.repid, a.outBlock, x, y);
@@ -354,7 +358,7 @@ void LaunchGmresBegin(const GmresDevArgs& a, const double* b, double tol, hipStr
 void LaunchGmresIteration(const GmresDevArgs& a, const double* b, hipStream_t s)
 {
     const long n = 3 * a.ne;
-    k_gm_spmv<<<Blocks(a.ne), 256, 0, s>>>(a.st, a.ne, a.rp, a.col, a.val, a.V, n, 0, nullptr, a.u2);
+    k_gm_spmv<<<Blocks(3 * a.ne), 256, 0, s>>>(a.st, a.ne, a.rp, a.col, a.val, a.V, n, 0, nullptr, a.u2);
     LaunchPre(a, a.u2, a.u, s);
     // classical Gram-Schmidt twice
     k_gm_multidot<<<a.nblocks, 256, 0, s>>>(a.st, a.V, n, 1, a.u, n, a.part);
@@ -366,7 +370,7 @@ void LaunchGmresIteration(const GmresDevArgs& a, const double* b, hipStream_t s)
     k_gm_givens<<<1, 256, 0, s>>>(a.st, a.h, a.QT, a.R, a.f, a.phibar, a.stag, (int)a.restart, a.part, a.nblocks);
     k_gm_wupdate<<<Blocks(n), 256, 0, s>>>(a.st, a.V, a.W, n, a.R, (int)a.restart, a.f, a.x, a.u, a.h);
     // true residual b - A x and its norm
-    k_gm_spmv<<<Blocks(a.ne), 256, 0, s>>>(a.st, a.ne, a.rp, a.col, a.val, a.x, n, -1, b, a.vh);
+    k_gm_spmv<<<Blocks(3 * a.ne), 256, 0, s>>>(a.st, a.ne, a.rp, a.col, a.val, a.x, n, -1, b, a.vh);
     k_gm_multidot<<<a.nblocks, 256, 0, s>>>(a.st, a.vh, n, 0, a.vh, n, a.part);
     k_gm_check<<<1, 64, 0, s>>>(a.st, a.part, a.nblocks, a.stag, (int)a.restart);
 }
@@ -464,31 +468,19 @@ __global__ void __launch_bounds__(256) k_gg_spmv(const GmState* st, const GmresG
                                                  long ldx, int colOff, const double* __restrict__ b,
                                                  double* __restrict__ y)
 {
-    const int g = BlockGroup(tab->ebPre, tab->G);
+    const int g = BlockGroup(tab->rbPre, tab->G);
     if (Done(st + g))
         return;
-    long i = tab->ele[g] + (long)((int)blockIdx.x - tab->ebPre[g]) * blockDim.x + threadIdx.x;
-    if (i >= tab->ele[g + 1])
+    long t = 3 * tab->ele[g] + (long)((int)blockIdx.x - tab->rbPre[g]) * blockDim.x + threadIdx.x;
+    if (t >= 3 * tab->ele[g + 1])
         return;
     if (colOff >= 0)
         x += (long)(NextJ(st + g) - 1 + colOff) * ldx;
-    double b0 = 0.0, b1 = 0.0, b2 = 0.0;
-    for (int j = rp[i]; j < rp[i + 1]; ++j) {
-        const double* v = val + 9 * (long)j;
-        const double* xc = x + 3 * (long)col[j];
-        double x0 = xc[0], x1 = xc[1], x2 = xc[2];
-        b0 += v[0] * x0 + v[1] * x1 + v[2] * x2;
-        b1 += v[3] * x0 + v[4] * x1 + v[5] * x2;
-        b2 += v[6] * x0 + v[7] * x1 + v[8] * x2;
-    }
-    if (b) {
-        b0 = b[3 * i] - b0;
-        b1 = b[3 * i + 1] - b1;
-        b2 = b[3 * i + 2] - b2;
-    }
-    y[3 * i] = b0;
-    y[3 * i + 1] = b1;
-    y[3 * i + 2] = b2;
+    const long i = t / 3;
+    double a = RowComponent(rp, col, val, x, i, (int)(t - 3 * i));
+    if (b)
+        a = b[t] - a;
+    y[t] = a;
 }
 
 __global__ void __launch_bounds__(256) k_gg_precond(const GmState* st, const GmresGroupTable* __restrict__ tab,
@@ -705,8 +697,8 @@ void LaunchGmresIterationGroups(const GmresGroupArgs& a, const GmresGroupTable& 
 {
     const long n = 3 * a.ne;
     const int G = t.G;
-    const unsigned nbT = (unsigned)t.nbPre[G], rbT = (unsigned)t.rbPre[G], ebT = (unsigned)t.ebPre[G];
-    k_gg_spmv<<<ebT, 256, 0, s>>>(a.st, dt, a.rp, a.col, a.val, a.V, n, 0, nullptr, a.u2);
+    const unsigned nbT = (unsigned)t.nbPre[G], rbT = (unsigned)t.rbPre[G];
+    k_gg_spmv<<<rbT, 256, 0, s>>>(a.st, dt, a.rp, a.col, a.val, a.V, n, 0, nullptr, a.u2);
     k_gg_precond<<<rbT, 256, 0, s>>>(a.st, dt, a.posLeaf, a.leafBegin, a.leafSize, a.matOff, a.inv, a.repid,
                                      a.outBlock, a.u2, a.u);
     // classical Gram-Schmidt twice
@@ -720,7 +712,7 @@ void LaunchGmresIterationGroups(const GmresGroupArgs& a, const GmresGroupTable& 
                                             (int)a.restart, a.part, a.ps, dt);
     k_gg_wupdate<<<rbT, 256, 0, s>>>(a.st, dt, a.V, a.W, n, a.R, a.rs, (int)a.restart, a.f, a.hs, a.x, a.u, a.h);
     // true residual b - A x and its norm
-    k_gg_spmv<<<ebT, 256, 0, s>>>(a.st, dt, a.rp, a.col, a.val, a.x, n, -1, b, a.vh);
+    k_gg_spmv<<<rbT, 256, 0, s>>>(a.st, dt, a.rp, a.col, a.val, a.x, n, -1, b, a.vh);
     k_gg_multidot<<<nbT, 256, 0, s>>>(a.st, dt, a.vh, n, 0, a.vh, a.part, a.ps);
     k_gg_check<<<(unsigned)G, 64, 0, s>>>(a.st, a.part, a.ps, dt, a.stag, (int)a.restart);
 }

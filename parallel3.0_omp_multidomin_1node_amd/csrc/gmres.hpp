@@ -59,7 +59,6 @@ struct GmresGroupTable {
     int nb[kGmGroupsMax];        // reduction blocks per group
     int nbPre[kGmGroupsMax + 1];  // prefix of nb
     int rbPre[kGmGroupsMax + 1];  // prefix of ceil(3 ne_g / 256): blocks over scalar rows
-    int ebPre[kGmGroupsMax + 1];  // prefix of ceil(ne_g / 256): blocks over element rows
 };
 
 struct GmresGroupArgs {

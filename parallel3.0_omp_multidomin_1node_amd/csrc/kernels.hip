@@ -3608,32 +3608,32 @@ TDBEM_CONV_INSTANTIATE(float)
 // vector); every product and sum is formed in FP64.  With all types double
 // these are the FP64 kernels of the parity path; float storage serves the
 // inner solves of the mixed-precision mode.
+// One thread per (block row, component): each component's sum over the
+// row's blocks in CCSR order, three products per block in the reference
+// order (the same bits as one thread per row computing all three).
 template <typename TA, typename TX, typename TY>
 __global__ void __launch_bounds__(256) k_spmv(long nrows, const int* __restrict__ rp, const int* __restrict__ col,
                                               const TA* __restrict__ val, const TX* __restrict__ x,
                                               TY* __restrict__ y)
 {
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nrows)
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 3 * nrows)
         return;
-    double b0 = 0.0, b1 = 0.0, b2 = 0.0;
+    const long i = t / 3;
+    const int c = (int)(t - 3 * i);
+    double a = 0.0;
     for (int j = rp[i]; j < rp[i + 1]; ++j) {
-        const TA* v = val + 9 * (long)j;
+        const TA* v = val + 9 * (long)j + 3 * c;
         const TX* xc = x + 3 * (long)col[j];
-        double x0 = (double)xc[0], x1 = (double)xc[1], x2 = (double)xc[2];
-        b0 += (double)v[0] * x0 + (double)v[1] * x1 + (double)v[2] * x2;
-        b1 += (double)v[3] * x0 + (double)v[4] * x1 + (double)v[5] * x2;
-        b2 += (double)v[6] * x0 + (double)v[7] * x1 + (double)v[8] * x2;
+        a += (double)v[0] * (double)xc[0] + (double)v[1] * (double)xc[1] + (double)v[2] * (double)xc[2];
     }
-    y[3 * i] = (TY)b0;
-    y[3 * i + 1] = (TY)b1;
-    y[3 * i + 2] = (TY)b2;
+    y[t] = (TY)a;
 }
 
 template <typename TA, typename TX, typename TY>
 void LaunchSpmv(long nrows, const int* rp, const int* col, const TA* val, const TX* x, TY* y, hipStream_t s)
 {
-    k_spmv<TA, TX, TY><<<(unsigned)((nrows + 255) / 256), 256, 0, s>>>(nrows, rp, col, val, x, y);
+    k_spmv<TA, TX, TY><<<(unsigned)((3 * nrows + 255) / 256), 256, 0, s>>>(nrows, rp, col, val, x, y);
 }
 
 // GMRES_M_X (DBEM1/pre_gmres.cpp:837-895): each output scalar is owned by

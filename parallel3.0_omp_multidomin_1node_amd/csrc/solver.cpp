@@ -1942,7 +1942,6 @@ void Case::SolveGroups(int nstep, const double* known_lu, long restart, double t
             t.nb[g] = (int)std::min<long>(gmBlocks, std::max<long>(1, (3 * neg + 255) / 256));
             t.nbPre[g + 1] = t.nbPre[g] + t.nb[g];
             t.rbPre[g + 1] = t.rbPre[g] + (int)((3 * neg + 255) / 256);
-            t.ebPre[g + 1] = t.ebPre[g] + (int)((neg + 255) / 256);
         }
         gw.tab.upload(&t, 1, stream);
     }

@@ -673,6 +673,8 @@ __global__ void __launch_bounds__(256) k_conv_near_b(const NodeRefs* __restrict_
     for (long q = nearPtr[s] + wave; q < nearPtr[s + 1]; q += 4) {
         long p = nearPair[q];
         int k0 = kb[p], L = len[p];
+        if (k0 > jlim)
+            break;  // the list is ordered by band start: nothing further is within the lag limit
         int f = nearField[q];
         const double* su = gU + (long)refs[f].hu_block * nstore * 3 * B + bb;
         const double* st = gT + (long)refs[f].ht_block * nstore * 3 * B + bb;

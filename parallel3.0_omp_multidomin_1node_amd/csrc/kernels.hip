@@ -1032,21 +1032,33 @@ __global__ void __launch_bounds__(64) k_near_lists(PairGrid grid, const NodeRefs
     long rowStart = grid.pairBase[d] + (long)(s - eb) * nd;
     long w = ptr ? ptr[s] : 0;
     int total = 0;
-    for (int q0 = 0; q0 < nd; q0 += 64) {
-        int q = q0 + lane;
-        long p = rowStart + q;
-        bool near = q < nd && len[p] > 0 && kb[p] <= tile + 1;
-        unsigned long long m = __ballot(near);
-        if (ptr && near) {
-            int rank = __popcll(m & ((1ull << lane) - 1ull));
-            pairOut[w + rank] = p;
-            fieldOut[w + rank] = eb + q;
+    if (!ptr) {
+        for (int q0 = 0; q0 < nd; q0 += 64) {
+            int q = q0 + lane;
+            long p = rowStart + q;
+            bool near = q < nd && len[p] > 0 && kb[p] <= tile + 1;
+            total += __popcll(__ballot(near));
         }
-        w += __popcll(m);
-        total += __popcll(m);
+        if (lane == 0)
+            cnt[s] = total;
+        return;
     }
-    if (!ptr && lane == 0)
-        cnt[s] = total;
+    // the list ordered by band start, field order within one start, so the
+    // near kernels stop at the first pair whose band starts past the step's
+    // lag limit
+    for (int v = 0; v <= tile + 1; ++v)
+        for (int q0 = 0; q0 < nd; q0 += 64) {
+            int q = q0 + lane;
+            long p = rowStart + q;
+            bool near = q < nd && len[p] > 0 && kb[p] == v;
+            unsigned long long m = __ballot(near);
+            if (near) {
+                int rank = __popcll(m & ((1ull << lane) - 1ull));
+                pairOut[w + rank] = p;
+                fieldOut[w + rank] = eb + q;
+            }
+            w += __popcll(m);
+        }
 }
 
 void LaunchNearLists(PairGrid grid, const NodeRefs* refs, const int* kb, const int* len, int tile, int ne,
@@ -3507,6 +3519,9 @@ __global__ void __launch_bounds__(256) k_conv_near_e(const NodeRefs* __restrict_
     double y[3] = {0.0, 0.0, 0.0};
     long q0 = nearPtr[s], q1 = nearPtr[s + 1];
     for (long qb = q0 + (long)wave * per; qb < q1; qb += 4L * per) {
+        // the list is ordered by band start: past the lag limit nothing is left
+        if (__builtin_amdgcn_readfirstlane(kb[nearPair[qb]]) > jlim)
+            break;
         long q = qb + slot;
         if (q >= q1)
             continue;

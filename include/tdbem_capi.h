@@ -115,7 +115,9 @@ int tdbem_self_tables(tdbem_ctx* ctx, double* u1, double* t1, double* t2);
  *   iters[nstep], flags[nstep]: GMRES inner iterations and flag per step;
  *   times[3] (may be null): seconds spent in convolution, GMRES, total.
  * Returns through *solved the number of solved steps; a GMRES flag != 0
- * stops the loop as the reference does (the call still returns 0). */
+ * stops the loop as the reference does (the call still returns 0): that
+ * step counts as solved but keeps no solution, so its lu / lt and those of
+ * the later steps are zero. */
 int tdbem_solve(tdbem_ctx* ctx, int nstep, const double* known_lu, long restart, double tol, double* lu,
                 double* lt, int* iters, int* flags, int* solved, double* times);
 
@@ -148,7 +150,8 @@ int tdbem_set_materials(tdbem_ctx* ctx, const double* evr);
  * time loop above): nrhs histories known_lu[nrhs][nstep+1][ne*3] give
  * lu, lt [nrhs][nstep+1][ne*3], iters / flags [nrhs][nstep], solved[nrhs].
  * The GMRES of all cases run in lockstep; a case whose GMRES flag is != 0
- * stops there (its later steps are zero) while the others continue.
+ * stops there (that step and the later ones are zero, as tdbem_solve leaves
+ * them) while the others continue.
  * times[3] as for tdbem_solve. */
 int tdbem_solve_batched(tdbem_ctx* ctx, int nstep, int nrhs, const double* known_lu, long restart, double tol,
                         double* lu, double* lt, int* iters, int* flags, int* solved, double* times);

@@ -526,12 +526,18 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
         if (!any)
             break;
     }
+    // the last step whose solution a case keeps: a step whose GMRES stopped
+    // with a flag keeps no solution (the reference leaves the loop before
+    // storing it, multidomain.cpp:3884-3888), nor does any step after it
+    std::vector<int> kept((size_t)B);
+    for (int b = 0; b < B; ++b)
+        kept[(size_t)b] = solved[b] - (solved[b] > 0 && flags[(size_t)b * nstep + (solved[b] - 1)] != 0 ? 1 : 0);
     if (deviceIO) {
-        // a case's steps after its GMRES failure are left zero; step 0 in the global frame
+        // those steps are left zero; step 0 in the global frame
         for (int b = 0; b < B; ++b)
-            if (solved[b] < nstep) {
-                size_t from = ((size_t)b * nstore + (size_t)solved[b] + 1) * (size_t)n;
-                size_t cnt = (size_t)(nstep - solved[b]) * (size_t)n;
+            if (kept[(size_t)b] < nstep) {
+                size_t from = ((size_t)b * nstore + (size_t)kept[(size_t)b] + 1) * (size_t)n;
+                size_t cnt = (size_t)(nstep - kept[(size_t)b]) * (size_t)n;
                 LaunchZero(olu + from, (long)cnt, stream);
                 LaunchZero(olt + from, (long)cnt, stream);
             }
@@ -550,9 +556,9 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
     CheckB(hipGetLastError(), "batched solve launch");
     CheckB(CopyOnStream(lu, d_lu.p, sizeof(double) * d_lu.n, hipMemcpyDeviceToHost, stream), "lu");
     CheckB(CopyOnStream(lt, d_lt.p, sizeof(double) * d_lt.n, hipMemcpyDeviceToHost, stream), "lt");
-    // a case's steps after its GMRES failure are left zero, as a single solve leaves them
+    // the steps without a kept solution are left zero, as a single solve leaves them
     for (int b = 0; b < B; ++b)
-        for (int K = solved[b] + 1; K <= nstep; ++K) {
+        for (int K = kept[(size_t)b] + 1; K <= nstep; ++K) {
             std::fill_n(lu + ((size_t)b * nstore + K) * n, n, 0.0);
             std::fill_n(lt + ((size_t)b * nstore + K) * n, n, 0.0);
         }

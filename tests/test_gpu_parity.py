@@ -505,6 +505,62 @@ def test_batched_load_cases_match_single_solves(nrhs, mixed):
         assert np.abs(bit[b] - it).max() <= 1, b
 
 
+def test_batched_cases_stop_where_their_single_solves_stop():
+    """A restart length at which some load cases' GMRES stops with flag 1 at
+    some step while the others finish: in the batch each stopped case keeps
+    the flags, iteration counts and solved step of its single solve, with
+    zero outputs after that step (the reference's loop ends there,
+    DBEM1/multidomain.cpp:3884-3885), and the others go on to their single
+    solves' results (1e-9 relative at tol 1e-12).  Six seeded cases, case 0
+    again with its load delayed by ten steps (zero right-hand sides, solved
+    in 0 iterations, until then) and a case with no load at all.  The batch
+    sums its dot products in another order, so its iteration counts may
+    differ by one: the restart is chosen from the single solves' counts at
+    restart 200 such that every step a case runs needs at most restart - 1
+    iterations, and the step where it stops at least restart + 2."""
+    tdbem = _tdbem()
+    from tdbem import mesh
+    w = mesh.layered_config2(h=0.5, nstep=20)
+    m = w.mesh
+    seeded = mesh.load_cases(w, 6)
+    delayed = np.zeros_like(seeded[0])
+    delayed[10:] = seeded[0][:-10]
+    cases = np.concatenate([seeded, delayed[None], np.zeros_like(delayed)[None]])
+    nrhs = len(cases)
+    print("case:", w.name, "batched", nrhs, "with stopping GMRES", flush=True)
+    gc = tdbem.Case(m.points, m.ele, m.bcid, w.dt, w.gap, domain_count=4, mat_rows=w.mats, itf_rows=m.interfaces)
+    gc.assemble_step0(10)
+    gc.assemble_history(w.nstep)
+    probe = [gc.solve(cases[b], w.nstep, restart=200, tol=1e-12) for b in range(nrhs)]
+    assert all(p[4] == w.nstep and not p[3].any() for p in probe)
+    counts = np.array([p[2] for p in probe])
+    chosen = None
+    for r in range(2, 40):
+        stops, ok = [], True
+        for n in counts:
+            over = np.nonzero(n > r)[0]
+            k = int(over[0]) if len(over) else len(n)
+            ok = ok and bool((n[:k] <= r - 1).all()) and (k == len(n) or n[k] >= r + 2)
+            stops.append(k < len(n))
+        if ok and any(stops) and not all(stops):
+            chosen = r
+            break
+    assert chosen is not None, counts
+    print("restart", chosen, "stopping cases", [b for b in range(nrhs) if stops[b]], flush=True)
+    BU, BT, bit, bfl, bsolved, _ = gc.solve_batched(cases, w.nstep, restart=chosen, tol=1e-12)
+    for b in range(nrhs):
+        U, T, it, fl, solved, _ = gc.solve(cases[b], w.nstep, restart=chosen, tol=1e-12)
+        assert bsolved[b] == solved and bool(fl.any()) == stops[b], b
+        assert np.array_equal(bfl[b], fl), b
+        assert np.abs(bit[b] - it).max() <= 1, b
+        if stops[b]:
+            assert bit[b][solved - 1] == it[solved - 1], b
+        assert np.abs(BU[b] - U).max() <= 1e-9 * np.abs(U).max(), b
+        assert np.abs(BT[b] - T).max() <= 1e-9 * np.abs(T).max(), b
+        assert not BU[b][solved + 1:].any() and not BT[b][solved + 1:].any(), b
+    gc.close()
+
+
 @pytest.mark.parametrize("levels,mixed,maxn,mfma,nstep", [(1, 0, 60, 1, 60), (1, 2, 37, 0, 60), (2, 0, 37, 1, 60),
                                                           (3, 2, 50, 0, 60), (3, 0, 140, 1, 140), (3, 2, 140, 1, 140),
                                                           (3, 0, 140, 0, 140), (3, 2, 97, 1, 140)])

@@ -112,7 +112,8 @@ int tdbem_self_tables(tdbem_ctx* ctx, double* u1, double* t1, double* t2);
  * DBEM1/multidomain.cpp:3815-3893; gmres DBEM1/pre_gmres.cpp:1369-1605).
  *   known_lu[(nstep+1)*ne*3]: the .bd triplets (element frame);
  *   lu, lt [(nstep+1)*ne*3]: displacement / traction, global frame;
- *   iters[nstep], flags[nstep]: GMRES inner iterations and flag per step;
+ *   iters[nstep], flags[nstep]: GMRES inner iterations and flag per step
+ *     (zero for the steps after the last solved one);
  *   times[3] (may be null): seconds spent in convolution, GMRES, total.
  * Returns through *solved the number of solved steps; a GMRES flag != 0
  * stops the loop as the reference does (the call still returns 0): that

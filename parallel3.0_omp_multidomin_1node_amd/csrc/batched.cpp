@@ -368,6 +368,9 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
     std::vector<char> alive((size_t)B, 1);
     for (int b = 0; b < B; ++b)
         solved[b] = 0;
+    // steps a case does not reach keep zero counts and flags
+    std::fill_n(iters, (size_t)B * nstep, 0);
+    std::fill_n(flags, (size_t)B * nstep, 0);
     double convTime = 0.0, gmresTime = 0.0, farTime = 0.0, fftTime = 0.0, fftProdTime = 0.0;
     int farLaunches = 0, fftLaunches = 0, K016 = 0;
     // timings of the spectral part: totals and per level

@@ -538,9 +538,9 @@ int tdbem_run_models_grouped(int n, tdbem_ctx* const* ctx, long maxleaf, const i
                         std::memset(lt[i] + K * ng, 0, sizeof(double) * ng);
                     }
                 }
-                for (int K = 0; K < sv; ++K) {
-                    iters[i][K] = its[(size_t)g * ns + K];
-                    flags[i][K] = fls[(size_t)g * ns + K];
+                for (int K = 0; K < ns; ++K) {  // zero past the solved steps
+                    iters[i][K] = K < sv ? its[(size_t)g * ns + K] : 0;
+                    flags[i][K] = K < sv ? fls[(size_t)g * ns + K] : 0;
                 }
                 solved[i] = sv;
             }

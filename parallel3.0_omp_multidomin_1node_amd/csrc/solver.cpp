@@ -1587,6 +1587,9 @@ void Case::Solve(int nstep, const double* known_lu, long restart, double tol, do
     if (!historyDone || maxn < 1)
         throw std::runtime_error("history not assembled");
     PrepareGmres(restart);
+    // steps the loop does not reach keep zero counts and flags
+    std::fill_n(iters, nstep, 0);
+    std::fill_n(flags, nstep, 0);
     if (!noAllocMode)
         ReleaseGraphs();  // captured per solve: the operands may have moved since the last one
     gm_lastIters = 1;

@@ -558,6 +558,23 @@ def test_batched_cases_stop_where_their_single_solves_stop():
         assert np.abs(BU[b] - U).max() <= 1e-9 * np.abs(U).max(), b
         assert np.abs(BT[b] - T).max() <= 1e-9 * np.abs(T).max(), b
         assert not BU[b][solved + 1:].any() and not BT[b][solved + 1:].any(), b
+    # every output entry is written: the same calls over arrays filled with junk
+    known = np.ascontiguousarray(cases, np.float64).reshape(-1)
+    lu, lt = np.full_like(known, np.nan), np.full_like(known, np.nan)
+    its, fls = np.full((nrhs, w.nstep), -7, np.int32), np.full((nrhs, w.nstep), -7, np.int32)
+    sol = np.full(nrhs, -7, np.int32)
+    tdbem._check(tdbem.lib().tdbem_solve_batched(gc.h, w.nstep, nrhs, known, chosen, 1e-12, lu, lt, its, fls, sol,
+                                                 np.zeros(3)), "tdbem_solve_batched")
+    assert np.array_equal(lu.reshape(BU.shape), BU) and np.array_equal(lt.reshape(BT.shape), BT)
+    assert np.array_equal(its, bit) and np.array_equal(fls, bfl) and np.array_equal(sol, bsolved)
+    U, T, it, fl, solved, _ = gc.solve(cases[0], w.nstep, restart=chosen, tol=1e-12)
+    k0 = np.ascontiguousarray(cases[0]).reshape(-1)
+    lu, lt = np.full_like(k0, np.nan), np.full_like(k0, np.nan)
+    its, fls, sv = np.full(w.nstep, -7, np.int32), np.full(w.nstep, -7, np.int32), tdbem.C.c_int(-7)
+    tdbem._check(tdbem.lib().tdbem_solve(gc.h, w.nstep, k0, chosen, 1e-12, lu, lt, its, fls, tdbem.C.byref(sv),
+                                         np.zeros(3)), "tdbem_solve")
+    assert np.array_equal(lu.reshape(U.shape), U) and np.array_equal(lt.reshape(T.shape), T)
+    assert np.array_equal(its, it) and np.array_equal(fls, fl) and sv.value == solved
     gc.close()
 
 

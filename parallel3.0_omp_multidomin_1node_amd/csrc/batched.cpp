@@ -38,8 +38,9 @@ void Case::GmresBatched(int B, const double* rhs, double* x, long restart, doubl
     // per block, at least 4 rows per wavefront (one workgroup per CU on the
     // 8192-element box)
     const int nblk = (int)std::min<long>(kBatchRedBlocks, std::max<long>(1, (n + 63) / 64));
-    // the same for the preconditioner's pass over elements (three rows each)
-    const int nblkE = (int)std::min<long>(kBatchRedBlocks, std::max<long>(1, (ne + 63) / 64));
+    // the same for the preconditioner's pass over elements (three rows each,
+    // at least one element per wavefront; 256 blocks on the 8192-element box)
+    const int nblkE = (int)std::min<long>(kBatchRedBlocks, std::max<long>(1, (ne + 15) / 16));
     BatchWork& w = bw;
     std::vector<double> bn((size_t)B), nr((size_t)B);
     std::vector<int> act((size_t)B), stag((size_t)B);

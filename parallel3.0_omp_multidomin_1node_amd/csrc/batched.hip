@@ -207,13 +207,31 @@ __global__ void __launch_bounds__(256) k_spmm_b(long nrows, const int* __restric
     if (row >= nrows || b >= B)
         return;
     double b0 = 0.0, b1 = 0.0, b2 = 0.0;
-    for (int j = rp[row]; j < rp[row + 1]; ++j) {
-        const double* v = val + 9 * (long)j;
-        long c3 = 3 * (long)col[j];
-        double x0 = x[c3 * B + b], x1 = x[(c3 + 1) * B + b], x2 = x[(c3 + 2) * B + b];
+    auto add = [&](const double* v, double x0, double x1, double x2) {
         b0 += v[0] * x0 + v[1] * x1 + v[2] * x2;
         b1 += v[3] * x0 + v[4] * x1 + v[5] * x2;
         b2 += v[6] * x0 + v[7] * x1 + v[8] * x2;
+    };
+    int j = rp[row];
+    const int je = rp[row + 1];
+    // four blocks at a time: their twelve state loads issued together, the
+    // sums in block order (as one block at a time)
+    for (; j + 4 <= je; j += 4) {
+        double xs[4][3];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const long c3 = 3 * (long)col[j + t];
+            xs[t][0] = x[c3 * B + b];
+            xs[t][1] = x[(c3 + 1) * B + b];
+            xs[t][2] = x[(c3 + 2) * B + b];
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+            add(val + 9 * (long)(j + t), xs[t][0], xs[t][1], xs[t][2]);
+    }
+    for (; j < je; ++j) {
+        const long c3 = 3 * (long)col[j];
+        add(val + 9 * (long)j, x[c3 * B + b], x[(c3 + 1) * B + b], x[(c3 + 2) * B + b]);
     }
     y[(3 * row) * B + b] = b0;
     y[(3 * row + 1) * B + b] = b1;
@@ -923,23 +941,42 @@ __global__ void __launch_bounds__(64 * kPassWaves) k_pass_b(const double* __rest
 #pragma unroll
         for (int c = 0; c < kPassCols; ++c)
             acc[c] = 0.0;
-        for (long r = (long)blockIdx.x * kPassWaves + w; r < n; r += stride) {
-            const long i = r * B + bb;
-            double v = u[i];
+        // rows r and r + stride together, all loads of both issued before
+        // any store, the sums in row order (as one row at a time)
+        for (long r = (long)blockIdx.x * kPassWaves + w; r < n; r += 2 * stride) {
+            const bool two = r + stride < n;
+            const long i = r * B + bb, i2 = (two ? r + stride : r) * B + bb;
+            double v = u[i], v2 = u[i2];
             if (g == 0 && update) {
-                if (rhs)
+                if (rhs) {
                     v = rhs[i] - v;
-                for (int k = 0; k < nvA; ++k)
-                    v -= hA[(long)k * B + bb] * V[(long)k * nB + i];
-                if (live)
+                    v2 = rhs[i2] - v2;
+                }
+                for (int k = 0; k < nvA; ++k) {
+                    const double h = hA[(long)k * B + bb];
+                    v -= h * V[(long)k * nB + i];
+                    v2 -= h * V[(long)k * nB + i2];
+                }
+                if (live) {
                     u[i] = v;
+                    if (two)
+                        u[i2] = v2;
+                }
             }
+            double x[kPassCols], x2[kPassCols];
 #pragma unroll
             for (int c = 0; c < kPassCols; ++c)
                 if (c < cn) {
                     const int col = c0 + c;
-                    const double x = col < nvD ? V[(long)col * nB + i] : v;
-                    acc[c] += x * v;
+                    x[c] = col < nvD ? V[(long)col * nB + i] : v;
+                    x2[c] = col < nvD ? V[(long)col * nB + i2] : v2;
+                }
+#pragma unroll
+            for (int c = 0; c < kPassCols; ++c)
+                if (c < cn) {
+                    acc[c] += x[c] * v;
+                    if (two)
+                        acc[c] += x2[c] * v2;
                 }
         }
         for (int c = 0; c < cn; ++c) {
@@ -1058,8 +1095,10 @@ void LaunchScaleB(const double* x, const double* a, double* y, long n, int B, hi
 
 // Givens rotation in QT form for column j of every active load case
 // (DBEM1/pre_gmres.cpp:1492-1533); one lane per case.
-__global__ void __launch_bounds__(64) k_givens_b(double* h, double* QT, double* R, double* f, double* phibar,
-                                                 int* stag, const int* __restrict__ active, int j, int restart, int B,
+__global__ void __launch_bounds__(64) k_givens_b(const double* __restrict__ h, double* __restrict__ QT,
+                                                 double* __restrict__ R, double* __restrict__ f,
+                                                 double* __restrict__ phibar, int* __restrict__ stag,
+                                                 const int* __restrict__ active, int j, int restart, int B,
                                                  const int* __restrict__ left)
 {
     if (left && *left == 0)
@@ -1071,10 +1110,24 @@ __global__ void __launch_bounds__(64) k_givens_b(double* h, double* QT, double* 
 #define QQ(a, c) QT[((long)((a)-1) * lq + ((c)-1)) * B + b]
 #define RR(a, c) R[((long)((a)-1) * restart + ((c)-1)) * B + b]
 #define HH(a) h[(long)(a) * B + b]
+    // R(1:j, j) = QT(1:j, 1:j) h(1:j), the products of each row summed in
+    // column order; h(1:j) is read once, eight values at a time in
+    // registers, so the loads of a row are independent of its sums
     for (int k = 1; k <= j; ++k) {
         double v = 0.0;
-        for (int l = 1; l <= j; ++l)
-            v += QQ(k, l) * HH(l);
+        for (int l0 = 1; l0 <= j; l0 += 8) {
+            double q[8], hv[8];
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const int l = l0 + t <= j ? l0 + t : j;
+                q[t] = QQ(k, l);
+                hv[t] = HH(l);
+            }
+#pragma unroll
+            for (int t = 0; t < 8; ++t)
+                if (l0 + t <= j)
+                    v += q[t] * hv[t];
+        }
         RR(k, j) = v;
     }
     double rt = RR(j, j), c, s, temp;

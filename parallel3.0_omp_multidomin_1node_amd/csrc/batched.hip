@@ -860,9 +860,7 @@ __global__ void __launch_bounds__(64 * kPassWaves) k_precond_dot_b(long n, const
                 const double* P1 = P0 + n3;
                 const double* P2 = P1 + n3;
                 double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-                for (long k = 0; k < c; ++k) {
-                    long r3 = 3 * repid[lb + k];
-                    double x0 = x[r3 * B + bb], x1 = x[(r3 + 1) * B + bb], x2 = x[(r3 + 2) * B + bb];
+                auto add = [&](long k, double x0, double x1, double x2) {
                     a0 += P0[3 * k] * x0;
                     a0 += P0[3 * k + 1] * x1;
                     a0 += P0[3 * k + 2] * x2;
@@ -872,6 +870,26 @@ __global__ void __launch_bounds__(64 * kPassWaves) k_precond_dot_b(long n, const
                     a2 += P2[3 * k] * x0;
                     a2 += P2[3 * k + 1] * x1;
                     a2 += P2[3 * k + 2] * x2;
+                };
+                // four leaf elements at a time: their state loads issued
+                // together, the sums in element order
+                long k = 0;
+                for (; k + 4 <= c; k += 4) {
+                    double xs[4][3];
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+                        const long r3 = 3 * repid[lb + k + t];
+                        xs[t][0] = x[r3 * B + bb];
+                        xs[t][1] = x[(r3 + 1) * B + bb];
+                        xs[t][2] = x[(r3 + 2) * B + bb];
+                    }
+#pragma unroll
+                    for (int t = 0; t < 4; ++t)
+                        add(k + t, xs[t][0], xs[t][1], xs[t][2]);
+                }
+                for (; k < c; ++k) {
+                    const long r3 = 3 * repid[lb + k];
+                    add(k, x[r3 * B + bb], x[(r3 + 1) * B + bb], x[(r3 + 2) * B + bb]);
                 }
                 a[0] = a0;
                 a[1] = a1;
@@ -885,13 +903,19 @@ __global__ void __launch_bounds__(64 * kPassWaves) k_precond_dot_b(long n, const
                 for (int m = 0; m < 3; ++m)
                     a[m] = y[(o + m) * B + bb];
             }
+            double vv[kPassColsP][3];
 #pragma unroll
             for (int c = 0; c < kPassColsP; ++c)
                 if (c < cn) {
                     const double* Vc = V + (long)(c0 + c) * nB;
                     for (int m = 0; m < 3; ++m)
-                        acc[c] += Vc[(o + m) * B + bb] * a[m];
+                        vv[c][m] = Vc[(o + m) * B + bb];
                 }
+#pragma unroll
+            for (int c = 0; c < kPassColsP; ++c)
+                if (c < cn)
+                    for (int m = 0; m < 3; ++m)
+                        acc[c] += vv[c][m] * a[m];
         }
         for (int c = 0; c < cn; ++c) {
             red[w][lane] = acc[c];

@@ -952,15 +952,24 @@ __device__ __forceinline__ void FftProductsMf(const FftItem* __restrict__ items,
     }
 }
 
+// Launched either as workgroups of eight wavefronts, one per (row group,
+// slot), or (single) as one-wavefront workgroups: block bid is wavefront
+// (bid / 8) % 8 of pair (bid / 64) * 8 + bid % 8, so the eight wavefronts of
+// a (row group, slot), which read the same window spectra, run on one XCD
+// (block bid goes to XCD bid % 8) and are dispatched together, and a
+// wavefront that runs out of items frees its slot for the next one instead
+// of waiting for the rest of its workgroup.
 template <int L>
 __global__ void __launch_bounds__(512) k_conv_far_fft_mf(
     const int* __restrict__ grpRowList, const int* __restrict__ grpRows, const long* __restrict__ itemOff,
     const FftItem* __restrict__ items, const double* __restrict__ H, const double* __restrict__ WU,
     const double* __restrict__ WT, long ringStrideU, long ringStrideT, int nwr, int t, int B, int pace,
-    double* __restrict__ Y)
+    double* __restrict__ Y, int single)
 {
-    const int slot = blockIdx.x & (L - 1), g = blockIdx.x / L;
-    const int wave = WaveIdx(), lane = threadIdx.x & 63;
+    const int bid = blockIdx.x;
+    const int pair = single ? (bid >> 6) * 8 + (bid & 7) : bid;
+    const int slot = pair & (L - 1), g = pair / L;
+    const int wave = single ? (bid >> 3) & 7 : WaveIdx(), lane = threadIdx.x & 63;
     const int sh = 8 * wave;
     const int b0 = blockIdx.y * 64;
     const int* rows = grpRowList + (long)g * kFftRows;
@@ -1004,9 +1013,12 @@ void LaunchConvFarFft(int ngroups, const int* grpRowList, const int* grpRows, co
     dim3 g((unsigned)ngroups * (unsigned)L, (unsigned)((B + 63) / 64));
 #define TDBEM_PRODUCTS(LL)                                                                                              \
     do {                                                                                                                \
-        if (mfma)                                                                                                       \
+        if (mfma && pace == 0)                                                                                          \
+            k_conv_far_fft_mf<LL><<<dim3(8u * g.x, g.y), 64, 0, s>>>(grpRowList, grpRows, itemOff, items, H, WU, WT,    \
+                                                                     ringStrideU, ringStrideT, nwr, t, B, 0, Y, 1);     \
+        else if (mfma)                                                                                                  \
             k_conv_far_fft_mf<LL><<<g, 512, 0, s>>>(grpRowList, grpRows, itemOff, items, H, WU, WT, ringStrideU,        \
-                                                    ringStrideT, nwr, t, B, pace, Y);                                   \
+                                                    ringStrideT, nwr, t, B, pace, Y, 0);                                \
         else                                                                                                            \
             k_conv_far_fft<LL><<<g, 512, 0, s>>>(grpRowList, grpRows, itemOff, items, H, WU, WT, ringStrideU,           \
                                                  ringStrideT, nwr, t, B, pace, Y);                                      \

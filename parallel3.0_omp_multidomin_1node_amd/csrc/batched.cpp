@@ -162,6 +162,7 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
         long nitems = 0, hvals = 0, strideU = 0, strideT = 0;
         std::vector<long> ioff, hoff, qItemsUpTo, qRowsUpTo;
         DevBuf<long> itemOff, hOff, items;
+        DevBuf<int> order;  // row groups by their summed item rows, most first (products launch order)
         DevBuf<double> H, WU, WT, hfar;
         double prodTime = 0.0, flops = 0.0, bytes = 0.0;
         int launches = 0;
@@ -285,6 +286,19 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
                 FftLevel& v = lv[l];
                 v.itemOff.upload(v.ioff.data(), v.ioff.size(), stream);
                 v.hOff.upload(v.hoff.data(), v.hoff.size(), stream);
+                {
+                    // the groups with the most item rows start first, so the
+                    // launch does not end on a long one
+                    std::vector<long> rowsOf((size_t)ngroups);
+                    for (int gi = 0; gi < ngroups; ++gi)
+                        rowsOf[(size_t)gi] = (v.hoff[(size_t)gi + 1] - v.hoff[(size_t)gi]) / ((long)v.L * 30);
+                    std::vector<int> ord((size_t)ngroups);
+                    for (int gi = 0; gi < ngroups; ++gi)
+                        ord[(size_t)gi] = gi;
+                    std::stable_sort(ord.begin(), ord.end(),
+                                     [&](int a, int b2) { return rowsOf[(size_t)a] > rowsOf[(size_t)b2]; });
+                    v.order.upload(ord.data(), ord.size(), stream);
+                }
                 v.items.alloc(4 * (size_t)std::max<long>(1, v.nitems));  // 32 bytes per item
                 FftItem* items = reinterpret_cast<FftItem*>(v.items.p);
                 LaunchFftItems(grid, d_refs.p, d_kb.p, d_len.p, maxn, v.L, v.lagLo, v.lagHi, ngroups, d_grpDom.p,
@@ -433,7 +447,7 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
                         LaunchFftWindows(d_gT.p, histT, nstore, B, v.L, t, v.WT.p + (size_t)(t % v.nwr) * v.strideT,
                                          stream);
                         CheckB(hipEventRecord(evg.e[4 + 2 * l], stream), "event");
-                        LaunchConvFarFft(ngroups, d_grpRowList.p, d_grpRows.p, v.itemOff.p,
+                        LaunchConvFarFft(ngroups, v.order.p, d_grpRowList.p, d_grpRows.p, v.itemOff.p,
                                          reinterpret_cast<const FftItem*>(v.items.p), v.H.p, v.WU.p, v.WT.p, v.strideU,
                                          v.strideT, v.nwr, v.L, t, B, d_Y.p, batchedFftMfma != 0, batchedFftPace,
                                          stream);

@@ -88,8 +88,11 @@ void LaunchFftWindows(const double* g, long cols, int nstore, int B, int L, int 
 // at WU / WT + (beta % nwr) * ringStride, every sub-filter q <= nwr.  H holds
 // kFftHPad values past its last item's spectra (read, never used).  mfma: on
 // the FP64 matrix cores, else with vector FMAs; pace > 0: a workgroup's
-// wavefronts meet at a barrier after every `pace` items of its list.
-void LaunchConvFarFft(int ngroups, const int* grpRowList, const int* grpRows, const long* itemOff, const FftItem* items,
+// wavefronts meet at a barrier after every `pace` items of its list.  On the
+// matrix cores without pacing the row groups run in the order order[0 ..
+// ngroups - 1].
+void LaunchConvFarFft(int ngroups, const int* order, const int* grpRowList, const int* grpRows, const long* itemOff,
+                      const FftItem* items,
                       const double* H, const double* WU, const double* WT, long ringStrideU, long ringStrideT,
                       int nwr, int L, int t, int B, double* Y, bool mfma, int pace, hipStream_t s);
 // Buffers of the inverse transform's layout whose values it adds.

@@ -958,17 +958,18 @@ __device__ __forceinline__ void FftProductsMf(const FftItem* __restrict__ items,
 // a (row group, slot), which read the same window spectra, run on one XCD
 // (block bid goes to XCD bid % 8) and are dispatched together, and a
 // wavefront that runs out of items frees its slot for the next one instead
-// of waiting for the rest of its workgroup.
+// of waiting for the rest of its workgroup; the pairs' row groups are taken
+// in the given order (order[pair / L]), the groups with the most work first.
 template <int L>
 __global__ void __launch_bounds__(512) k_conv_far_fft_mf(
     const int* __restrict__ grpRowList, const int* __restrict__ grpRows, const long* __restrict__ itemOff,
     const FftItem* __restrict__ items, const double* __restrict__ H, const double* __restrict__ WU,
     const double* __restrict__ WT, long ringStrideU, long ringStrideT, int nwr, int t, int B, int pace,
-    double* __restrict__ Y, int single)
+    double* __restrict__ Y, int single, const int* __restrict__ order)
 {
     const int bid = blockIdx.x;
     const int pair = single ? (bid >> 6) * 8 + (bid & 7) : bid;
-    const int slot = pair & (L - 1), g = pair / L;
+    const int slot = pair & (L - 1), g = single ? order[pair / L] : pair / L;
     const int wave = single ? (bid >> 3) & 7 : WaveIdx(), lane = threadIdx.x & 63;
     const int sh = 8 * wave;
     const int b0 = blockIdx.y * 64;
@@ -1006,7 +1007,8 @@ __global__ void __launch_bounds__(512) k_conv_far_fft_mf(
         }
 }
 
-void LaunchConvFarFft(int ngroups, const int* grpRowList, const int* grpRows, const long* itemOff, const FftItem* items,
+void LaunchConvFarFft(int ngroups, const int* order, const int* grpRowList, const int* grpRows, const long* itemOff,
+                      const FftItem* items,
                       const double* H, const double* WU, const double* WT, long ringStrideU, long ringStrideT,
                       int nwr, int L, int t, int B, double* Y, bool mfma, int pace, hipStream_t s)
 {
@@ -1015,10 +1017,11 @@ void LaunchConvFarFft(int ngroups, const int* grpRowList, const int* grpRows, co
     do {                                                                                                                \
         if (mfma && pace == 0)                                                                                          \
             k_conv_far_fft_mf<LL><<<dim3(8u * g.x, g.y), 64, 0, s>>>(grpRowList, grpRows, itemOff, items, H, WU, WT,    \
-                                                                     ringStrideU, ringStrideT, nwr, t, B, 0, Y, 1);     \
+                                                                     ringStrideU, ringStrideT, nwr, t, B, 0, Y, 1,      \
+                                                                     order);                                            \
         else if (mfma)                                                                                                  \
             k_conv_far_fft_mf<LL><<<g, 512, 0, s>>>(grpRowList, grpRows, itemOff, items, H, WU, WT, ringStrideU,        \
-                                                    ringStrideT, nwr, t, B, pace, Y, 0);                                \
+                                                    ringStrideT, nwr, t, B, pace, Y, 0, order);                         \
         else                                                                                                            \
             k_conv_far_fft<LL><<<g, 512, 0, s>>>(grpRowList, grpRows, itemOff, items, H, WU, WT, ringStrideU,           \
                                                  ringStrideT, nwr, t, B, pace, Y);                                      \

@@ -150,6 +150,9 @@ int tdbem_set_materials(tdbem_ctx* ctx, const double* evr);
  * which solves one load history per run; each case follows the tdbem_solve
  * time loop above): nrhs histories known_lu[nrhs][nstep+1][ne*3] give
  * lu, lt [nrhs][nstep+1][ne*3], iters / flags [nrhs][nstep], solved[nrhs].
+ * The load histories are copied in and the solutions out in 16-step chunks
+ * while the time loop runs (the arrays must stay valid and unchanged until
+ * the call returns, as for any call).
  * The GMRES of all cases run in lockstep; a case whose GMRES flag is != 0
  * stops there (that step and the later ones are zero, as tdbem_solve leaves
  * them) while the others continue.

@@ -8,8 +8,13 @@
 
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
 #include <cstring>
+#include <memory>
+#include <mutex>
 #include <stdexcept>
+#include <string>
+#include <thread>
 
 namespace tdbem {
 
@@ -20,6 +25,205 @@ void CheckB(hipError_t e, const char* what)
     if (e != hipSuccess)
         throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
 }
+
+// Host buffers of a batched solve: the load histories go to the device from
+// a second thread on a stream of its own, chunk by chunk ahead of the time
+// loop (chunk c: steps c * kSteps .. c * kSteps + kSteps - 1); Need(K, s)
+// makes the solving stream s wait for the chunk of step K.
+class InputFeed {
+public:
+    static constexpr int kSteps = 16;
+    InputFeed(int nstore, long n, int B, const double* h, double* d)
+        : nch_((nstore + kSteps - 1) / kSteps), nstore_(nstore), n_(n), B_(B), h_(h), d_(d),
+          ev_((size_t)nch_, nullptr)
+    {
+        for (hipEvent_t& e : ev_)
+            CheckB(hipEventCreateWithFlags(&e, hipEventDisableTiming), "feed event");
+        CheckB(hipStreamCreateWithFlags(&cs_, hipStreamNonBlocking), "feed stream");
+        th_ = std::thread([this]() { Run(); });
+    }
+    ~InputFeed()
+    {
+        if (th_.joinable()) {
+            {
+                std::lock_guard<std::mutex> lk(m_);
+                stop_ = true;
+            }
+            th_.join();
+        }
+        if (cs_) {
+            (void)hipStreamSynchronize(cs_);
+            (void)hipStreamDestroy(cs_);
+        }
+        for (hipEvent_t e : ev_)
+            if (e)
+                (void)hipEventDestroy(e);
+    }
+    void Need(int K, hipStream_t s)
+    {
+        const int c = K / kSteps;
+        if (c < waited_)
+            return;
+        {
+            std::unique_lock<std::mutex> lk(m_);
+            cv_.wait(lk, [&]() { return queued_ > c || !err_.empty(); });
+            if (!err_.empty())
+                throw std::runtime_error("batched solve: copying the load histories in: " + err_);
+        }
+        for (int q = waited_; q <= c; ++q)
+            CheckB(hipStreamWaitEvent(s, ev_[(size_t)q], 0), "feed wait");
+        waited_ = c + 1;
+    }
+
+private:
+    void Run()
+    {
+        for (int c = 0; c < nch_; ++c) {
+            {
+                std::lock_guard<std::mutex> lk(m_);
+                if (stop_)
+                    return;
+            }
+            const int K0 = c * kSteps, K1 = std::min(nstore_ - 1, K0 + kSteps - 1);
+            const size_t pitch = (size_t)nstore_ * (size_t)n_ * sizeof(double);
+            const size_t width = (size_t)(K1 - K0 + 1) * (size_t)n_ * sizeof(double);
+            const size_t off = (size_t)K0 * (size_t)n_;
+            hipError_t e = hipMemcpy2DAsync(d_ + off, pitch, h_ + off, pitch, width, (size_t)B_, hipMemcpyHostToDevice,
+                                            cs_);
+            if (e == hipSuccess)
+                e = hipEventRecord(ev_[(size_t)c], cs_);
+            {
+                std::lock_guard<std::mutex> lk(m_);
+                if (e != hipSuccess)
+                    err_ = hipGetErrorString(e);
+                else
+                    queued_ = c + 1;
+            }
+            cv_.notify_all();
+            if (e != hipSuccess)
+                return;
+        }
+    }
+    int nch_, nstore_;
+    long n_;
+    int B_;
+    const double* h_;
+    double* d_;
+    std::vector<hipEvent_t> ev_;
+    hipStream_t cs_ = nullptr;
+    std::thread th_;
+    std::mutex m_;
+    std::condition_variable cv_;
+    int queued_ = 0, waited_ = 0;
+    bool stop_ = false;
+    std::string err_;
+};
+
+// Host buffers of a batched solve: the solved steps of lu / lt are copied to
+// the caller's arrays by a second thread on a stream of its own, chunk by
+// chunk, while the time loop runs on.  Chunk c (steps 1 + c * kSteps ..) is
+// copied after the event Ready(c) records on the solving stream; Stop ends
+// the thread (after the chunks marked ready, or at once on an error path).
+class OutputDrain {
+public:
+    static constexpr int kSteps = 16;
+    OutputDrain(int nstep, int nstore, long n, int B, const double* dlu, const double* dlt, double* lu, double* lt)
+        : nch_((nstep + kSteps - 1) / kSteps), nstep_(nstep), nstore_(nstore), n_(n), B_(B), dlu_(dlu), dlt_(dlt),
+          lu_(lu), lt_(lt), ev_((size_t)std::max(0, nch_), nullptr)
+    {
+        for (hipEvent_t& e : ev_)
+            CheckB(hipEventCreateWithFlags(&e, hipEventDisableTiming), "drain event");
+        CheckB(hipStreamCreateWithFlags(&cs_, hipStreamNonBlocking), "drain stream");
+        th_ = std::thread([this]() { Run(); });
+    }
+    ~OutputDrain()
+    {
+        Stop(false);
+        for (hipEvent_t e : ev_)
+            if (e)
+                (void)hipEventDestroy(e);
+        if (cs_)
+            (void)hipStreamDestroy(cs_);
+    }
+    // chunks below `upTo` are complete once the work queued on s so far is
+    int Chunks() const { return nch_; }
+    void Ready(int upTo, hipStream_t s)
+    {
+        upTo = std::min(upTo, nch_);
+        if (upTo <= marked_)
+            return;
+        for (int c = marked_; c < upTo; ++c)
+            CheckB(hipEventRecord(ev_[(size_t)c], s), "drain event");
+        marked_ = upTo;
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            ready_ = upTo;
+        }
+        cv_.notify_one();
+    }
+    // waits for the copies of the chunks marked ready (all of them with
+    // finish), then ends the thread; throws what a copy failed with
+    void Stop(bool finish)
+    {
+        if (th_.joinable()) {
+            {
+                std::lock_guard<std::mutex> lk(m_);
+                stop_ = true;
+                drainAll_ = finish;
+            }
+            cv_.notify_one();
+            th_.join();
+        }
+        if (finish && !err_.empty())
+            throw std::runtime_error("batched solve: copying the solution out: " + err_);
+    }
+
+private:
+    void Run()
+    {
+        for (int c = 0; c < nch_; ++c) {
+            {
+                std::unique_lock<std::mutex> lk(m_);
+                cv_.wait(lk, [&]() { return ready_ > c || stop_; });
+                if (ready_ <= c || (stop_ && !drainAll_) || !err_.empty())
+                    return;
+            }
+            const int K0 = 1 + c * kSteps, K1 = std::min(nstep_, K0 + kSteps - 1);
+            const size_t pitch = (size_t)nstore_ * (size_t)n_ * sizeof(double);
+            const size_t width = (size_t)(K1 - K0 + 1) * (size_t)n_ * sizeof(double);
+            const size_t off = (size_t)K0 * (size_t)n_;
+            hipError_t e = hipStreamWaitEvent(cs_, ev_[(size_t)c], 0);
+            if (e == hipSuccess)
+                e = hipMemcpy2DAsync(lu_ + off, pitch, dlu_ + off, pitch, width, (size_t)B_, hipMemcpyDeviceToHost,
+                                     cs_);
+            if (e == hipSuccess)
+                e = hipMemcpy2DAsync(lt_ + off, pitch, dlt_ + off, pitch, width, (size_t)B_, hipMemcpyDeviceToHost,
+                                     cs_);
+            if (e == hipSuccess)
+                e = hipStreamSynchronize(cs_);
+            if (e != hipSuccess) {
+                std::lock_guard<std::mutex> lk(m_);
+                err_ = hipGetErrorString(e);
+                return;
+            }
+        }
+    }
+    int nch_, nstep_, nstore_;
+    long n_;
+    int B_;
+    const double* dlu_;
+    const double* dlt_;
+    double* lu_;
+    double* lt_;
+    std::vector<hipEvent_t> ev_;
+    hipStream_t cs_ = nullptr;
+    std::thread th_;
+    std::mutex m_;
+    std::condition_variable cv_;
+    int ready_ = 0, marked_ = 0;
+    bool stop_ = false, drainAll_ = false;
+    std::string err_;
+};
 
 double NowB()
 {
@@ -355,8 +559,10 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
     const double* bd = known_lu;
     double* olu = lu;
     double* olt = lt;
+    std::unique_ptr<InputFeed> feed;
     if (!deviceIO) {
-        d_bd.upload(known_lu, outN, stream);
+        d_bd.alloc(outN);
+        feed.reset(new InputFeed(nstore, n, B, known_lu, d_bd.p));
         d_lu.alloc(outN);
         d_lt.alloc(outN);
         bd = d_bd.p;
@@ -419,8 +625,14 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
     for (hipEvent_t& x : evg.e)
         CheckB(hipEventCreate(&x), "event");
     hipEvent_t fa = evg.e[0], fb = evg.e[1];
+    // host buffers: the solved steps go out while the loop runs on
+    std::unique_ptr<OutputDrain> drain;
+    if (!deviceIO && nstep > 0)
+        drain.reset(new OutputDrain(nstep, nstore, n, B, d_lu.p, d_lt.p, lu, lt));
     int K0 = 0;
     for (int K = 1; K <= nstep; ++K) {
+        if (feed)
+            feed->Need(K, stream);
         LaunchKnownB(d_refs.p, bd, (int)ne, nstore, K, B, d_known.p, stream);
         LaunchSpmmB(ne, d_Krp.p, d_Kcol.p, d_Kval.p, d_known.p, d_rhs.p, B, stream);
         if (K > 1) {
@@ -541,8 +753,14 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
         }
         LaunchScatterB(d_refs.p, d_geo.p, bd, d_x.p, (int)ne, nstore, K, B, olu, olt, stream);
         LaunchStateB(d_refs.p, olu, olt, (int)ne, nstore, K, B, d_gU.p, d_gT.p, stream);
+        if (drain && (K % OutputDrain::kSteps == 0 || K == nstep))
+            drain->Ready((K - 1) / OutputDrain::kSteps + 1, stream);
         if (!any)
             break;
+    }
+    if (drain) {
+        drain->Ready(drain->Chunks(), stream);
+        drain->Stop(true);
     }
     // the last step whose solution a case keeps: a step whose GMRES stopped
     // with a flag keeps no solution (the reference leaves the loop before
@@ -572,8 +790,6 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
     }
     CheckB(hipStreamSynchronize(stream), "batched solve");
     CheckB(hipGetLastError(), "batched solve launch");
-    CheckB(CopyOnStream(lu, d_lu.p, sizeof(double) * d_lu.n, hipMemcpyDeviceToHost, stream), "lu");
-    CheckB(CopyOnStream(lt, d_lt.p, sizeof(double) * d_lt.n, hipMemcpyDeviceToHost, stream), "lt");
     // the steps without a kept solution are left zero, as a single solve leaves them
     for (int b = 0; b < B; ++b)
         for (int K = kept[(size_t)b] + 1; K <= nstep; ++K) {

@@ -521,26 +521,32 @@ __global__ void __launch_bounds__(256) k_fft_filters(
         const bool row = lane < 60 && k0 + kk < pc;
         const long p = domPair0 + (long)(rows[row ? rw[k0 + kk] : rw[0]] - eb) * n + f;
         const int k0b = kb[p], lp = len[p];
+        // the L taps of this lane's channel: every load issued (entry 0 where
+        // the lag is outside the band), then the values kept where it is not
+        const bool isM = ch < 9;
+        const TB* src = isM ? mts : mgs;
+        const unsigned short* srcLo = isM ? mtsLo : mgsLo;
+        const int nv = isM ? 9 : 6, cv = isM ? ch : ch - 9;
+        const long e0 = off[p] - k0b;
+        TB raw[L];
+        unsigned short lo[L];
+#pragma unroll
+        for (int m = 0; m < L; ++m) {
+            const int j = L * q + 1 + m;
+            const bool ok = row && j >= lagLo && j <= jhi && j >= k0b && j < k0b + lp;
+            const long e = ok ? e0 + j : 0;
+            raw[m] = src[nv * e + cv];
+            lo[m] = srcLo ? srcLo[nv * e + cv] : (unsigned short)0;
+        }
         double h[L];
 #pragma unroll
         for (int m = 0; m < L; ++m) {
             const int j = L * q + 1 + m;
             const bool ok = row && j >= lagLo && j <= jhi && j >= k0b && j < k0b + lp;
-            const long e = ok ? off[p] + (j - k0b) : 0;
-            double val = 0.0;
-            if (ok) {
-                if (ch < 9) {
-                    val = (double)mts[9 * e + ch];
-                    if (mtsLo)
-                        val += Bf16LoF(mtsLo[9 * e + ch]);
-                    val = -val;
-                } else {
-                    val = (double)mgs[6 * e + (ch - 9)];
-                    if (mgsLo)
-                        val += Bf16LoF(mgsLo[6 * e + (ch - 9)]);
-                }
-            }
-            h[m] = val;
+            double val = (double)raw[m];
+            if (srcLo)
+                val += Bf16LoF(lo[m]);
+            h[m] = ok ? (isM ? -val : val) : 0.0;
         }
         // the 2L-point spectrum of the L taps (the second half zero)
         double zr[L], zi[L];

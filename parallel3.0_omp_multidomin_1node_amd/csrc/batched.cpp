@@ -751,8 +751,8 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
             else
                 any = true;
         }
-        LaunchScatterB(d_refs.p, d_geo.p, bd, d_x.p, (int)ne, nstore, K, B, olu, olt, stream);
-        LaunchStateB(d_refs.p, olu, olt, (int)ne, nstore, K, B, d_gU.p, d_gT.p, stream);
+        // step K's outputs and its state series in one pass
+        LaunchScatterB(d_refs.p, d_geo.p, bd, d_x.p, (int)ne, nstore, K, B, olu, olt, stream, d_gU.p, d_gT.p);
         if (drain && (K % OutputDrain::kSteps == 0 || K == nstep))
             drain->Ready((K - 1) / OutputDrain::kSteps + 1, stream);
         if (!any)

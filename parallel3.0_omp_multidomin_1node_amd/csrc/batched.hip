@@ -54,12 +54,16 @@ void LaunchKnownB(const NodeRefs* refs, const double* bd, int ne, int nstore, in
 __global__ void __launch_bounds__(256) k_scatter_b(const NodeRefs* __restrict__ refs, const ElemGeo* __restrict__ geo,
                                                    const double* __restrict__ bd, const double* __restrict__ x,
                                                    int ne, int nstore, int K, int B, double* __restrict__ lu,
-                                                   double* __restrict__ lt)
+                                                   double* __restrict__ lt, double* __restrict__ gU,
+                                                   double* __restrict__ gT)
 {
+    // element fastest: the writes of lu / lt (and the reads of bd), laid out
+    // per load case, are contiguous across a wavefront; the reads of x
+    // (load case fastest) are not
     long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (long)ne * B)
         return;
-    int i = (int)(idx / B), b = (int)(idx % B);
+    int b = (int)(idx / ne), i = (int)(idx % ne);
     const NodeRefs& r = refs[i];
     const double* T = r.itf_T;
     long slice = ((long)b * nstore + K) * 3 * ne;
@@ -86,19 +90,30 @@ __global__ void __launch_bounds__(256) k_scatter_b(const NodeRefs* __restrict__ 
     const double* F = geo[i].trans;
     double* ou = lu + slice + 3 * (long)i;
     double* ot = lt + slice + 3 * (long)i;
-    ou[0] = F[0] * u[0] + F[3] * u[1] + F[6] * u[2];
-    ou[1] = F[1] * u[0] + F[4] * u[1] + F[7] * u[2];
-    ou[2] = F[2] * u[0] + F[5] * u[1] + F[8] * u[2];
-    ot[0] = F[0] * t[0] + F[3] * t[1] + F[6] * t[2];
-    ot[1] = F[1] * t[0] + F[4] * t[1] + F[7] * t[2];
-    ot[2] = F[2] * t[0] + F[5] * t[1] + F[8] * t[2];
+    double gu[3], gt[3];
+    gu[0] = F[0] * u[0] + F[3] * u[1] + F[6] * u[2];
+    gu[1] = F[1] * u[0] + F[4] * u[1] + F[7] * u[2];
+    gu[2] = F[2] * u[0] + F[5] * u[1] + F[8] * u[2];
+    gt[0] = F[0] * t[0] + F[3] * t[1] + F[6] * t[2];
+    gt[1] = F[1] * t[0] + F[4] * t[1] + F[7] * t[2];
+    gt[2] = F[2] * t[0] + F[5] * t[1] + F[8] * t[2];
+    for (int c = 0; c < 3; ++c) {
+        ou[c] = gu[c];
+        ot[c] = gt[c];
+    }
+    // with gU / gT, also the state series of step K (as k_state_b makes them)
+    if (gU && !(r.surface == 1 && !r.ref_side))
+        for (int c = 0; c < 3; ++c) {
+            gU[(((long)r.hu_block * nstore + K) * 3 + c) * B + b] = r.hu_sign * gu[c];
+            gT[(((long)r.ht_block * nstore + K) * 3 + c) * B + b] = r.ht_sign * gt[c];
+        }
 }
 
 void LaunchScatterB(const NodeRefs* refs, const ElemGeo* geo, const double* bd, const double* x, int ne, int nstore,
-                    int K, int B, double* lu, double* lt, hipStream_t s)
+                    int K, int B, double* lu, double* lt, hipStream_t s, double* gU, double* gT)
 {
     long n = (long)ne * B;
-    k_scatter_b<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(refs, geo, bd, x, ne, nstore, K, B, lu, lt);
+    k_scatter_b<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(refs, geo, bd, x, ne, nstore, K, B, lu, lt, gU, gT);
 }
 
 __global__ void __launch_bounds__(256) k_state_b(const NodeRefs* __restrict__ refs, const double* __restrict__ lu,

@@ -22,8 +22,10 @@ void LaunchKnownB(const NodeRefs* refs, const double* bd, int ne, int nstore, in
                   hipStream_t s);
 void LaunchStep0Out(const NodeRefs* refs, const ElemGeo* geo, const double* bd, int ne, int nstore, int B, double* lu,
                     double* lt, hipStream_t s);
+// Step K's lu / lt from the solution x and the known values; with gU / gT
+// also step K of the raw state series (as LaunchStateB makes it).
 void LaunchScatterB(const NodeRefs* refs, const ElemGeo* geo, const double* bd, const double* x, int ne, int nstore,
-                    int K, int B, double* lu, double* lt, hipStream_t s);
+                    int K, int B, double* lu, double* lt, hipStream_t s, double* gU = nullptr, double* gT = nullptr);
 // raw state series g[((col * nstore + K) * 3 + c) * B + b]
 void LaunchStateB(const NodeRefs* refs, const double* lu, const double* lt, int ne, int nstore, int K, int B,
                   double* gU, double* gT, hipStream_t s);

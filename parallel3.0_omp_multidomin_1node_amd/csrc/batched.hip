@@ -27,10 +27,12 @@ __device__ __forceinline__ int WaveIndex() { return __builtin_amdgcn_readfirstla
 __global__ void __launch_bounds__(256) k_known_b(const NodeRefs* __restrict__ refs, const double* __restrict__ bd,
                                                  int ne, int nstore, int K, int B, double* __restrict__ known)
 {
+    // element fastest: the reads of bd, laid out per load case, are
+    // contiguous across a wavefront
     long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (long)ne * B)
         return;
-    int i = (int)(idx / B), b = (int)(idx % B);
+    int b = (int)(idx / ne), i = (int)(idx % ne);
     const NodeRefs& r = refs[i];
     if (r.surface == 1)
         return;

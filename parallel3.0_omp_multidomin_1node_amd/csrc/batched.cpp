@@ -30,6 +30,23 @@ void CheckB(hipError_t e, const char* what)
 // a second thread on a stream of its own, chunk by chunk ahead of the time
 // loop (chunk c: steps c * kSteps .. c * kSteps + kSteps - 1); Need(K, s)
 // makes the solving stream s wait for the chunk of step K.
+// One chunk of steps of every load case between host and device arrays laid
+// out [case][step][n]: a 2D copy (one row per case), or one copy per case
+// when the row pitch is past what a 2D copy takes.
+hipError_t CopyStepChunk(double* dst, const double* src, size_t pitch, size_t width, int B, hipMemcpyKind kind,
+                         hipStream_t s)
+{
+    if (pitch <= (size_t)0x7fffffff)
+        return hipMemcpy2DAsync(dst, pitch, src, pitch, width, (size_t)B, kind, s);
+    for (int b = 0; b < B; ++b) {
+        const size_t at = (size_t)b * (pitch / sizeof(double));
+        hipError_t e = hipMemcpyAsync(dst + at, src + at, width, kind, s);
+        if (e != hipSuccess)
+            return e;
+    }
+    return hipSuccess;
+}
+
 class InputFeed {
 public:
     static constexpr int kSteps = 16;
@@ -88,8 +105,7 @@ private:
             const size_t pitch = (size_t)nstore_ * (size_t)n_ * sizeof(double);
             const size_t width = (size_t)(K1 - K0 + 1) * (size_t)n_ * sizeof(double);
             const size_t off = (size_t)K0 * (size_t)n_;
-            hipError_t e = hipMemcpy2DAsync(d_ + off, pitch, h_ + off, pitch, width, (size_t)B_, hipMemcpyHostToDevice,
-                                            cs_);
+            hipError_t e = CopyStepChunk(d_ + off, h_ + off, pitch, width, B_, hipMemcpyHostToDevice, cs_);
             if (e == hipSuccess)
                 e = hipEventRecord(ev_[(size_t)c], cs_);
             {
@@ -194,11 +210,9 @@ private:
             const size_t off = (size_t)K0 * (size_t)n_;
             hipError_t e = hipStreamWaitEvent(cs_, ev_[(size_t)c], 0);
             if (e == hipSuccess)
-                e = hipMemcpy2DAsync(lu_ + off, pitch, dlu_ + off, pitch, width, (size_t)B_, hipMemcpyDeviceToHost,
-                                     cs_);
+                e = CopyStepChunk(lu_ + off, dlu_ + off, pitch, width, B_, hipMemcpyDeviceToHost, cs_);
             if (e == hipSuccess)
-                e = hipMemcpy2DAsync(lt_ + off, pitch, dlt_ + off, pitch, width, (size_t)B_, hipMemcpyDeviceToHost,
-                                     cs_);
+                e = CopyStepChunk(lt_ + off, dlt_ + off, pitch, width, B_, hipMemcpyDeviceToHost, cs_);
             if (e == hipSuccess)
                 e = hipStreamSynchronize(cs_);
             if (e != hipSuccess) {

@@ -625,6 +625,39 @@ def test_batched_spectral_far_part_matches_lag_by_lag(levels, mixed, maxn, mfma,
 
 
 @pytest.mark.gpu
+def test_batched_spectral_products_launch_shapes_agree():
+    """The matrix-core spectral products as one-wavefront workgroups (default:
+    the eight wavefronts of a row group and slot dispatched together on one
+    XCD, the row groups in descending order of work) and as eight-wavefront
+    workgroups that meet at a barrier every few items (option
+    batched_fft_pace): each wavefront sums the same items in the same order,
+    so the two give the same solution bit for bit, over two spectral levels
+    and 20 load cases."""
+    tdbem = _tdbem()
+    from tdbem import mesh
+    w = mesh.layered_config2(h=0.25, nstep=80)
+    m = w.mesh
+    cases = mesh.load_cases(w, 20)
+    print("case:", w.name, "spectral products launch shapes", flush=True)
+    gc = tdbem.Case(m.points, m.ele, m.bcid, w.dt, w.gap, domain_count=4, mat_rows=w.mats, itf_rows=m.interfaces)
+    gc.set_option("batched_fft_levels", 2)
+    gc.set_option("mixed", 2)
+    gc.set_option("batched_fft", 1)
+    gc.assemble_step0(10)
+    gc.assemble_history(w.nstep)
+    res = []
+    for pace in (0, 3):
+        gc.set_option("batched_fft_pace", pace)
+        out = gc.solve_batched(cases, w.nstep, tol=1e-12)
+        assert (out[4] == w.nstep).all() and not out[3].any()
+        assert gc.timings()["spectral"]
+        res.append(out[:5])
+    gc.close()
+    for x, y in zip(*res):
+        assert np.array_equal(np.asarray(x), np.asarray(y))
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("mixed", [0, 2])
 def test_batched_far_matrix_cores_match_vector_fmas(mixed):
     """The batched far part on the FP64 matrix cores (row groups of five

@@ -592,7 +592,8 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
     d_rhs.alloc((size_t)n * B);
     d_x.alloc((size_t)n * B);
     const int farParts = batchedMfma ? batchedFarParts : 0;
-    d_hfar.alloc((size_t)std::max(1, farParts) * kBatchTile * (size_t)n * B);
+    const int hfarParts = batchedMfma ? std::max(1, farParts) : kFarBWaves;  // partial sums the near part adds
+    d_hfar.alloc((size_t)hfarParts * kBatchTile * (size_t)n * B);
     for (DevBuf<double>* z : {&d_gU, &d_gT, &d_FU, &d_FT, &d_known})
         LaunchZero(z->p, (long)z->n, stream);
     LaunchZero(olu, (long)outN, stream);
@@ -740,12 +741,12 @@ void Case::SolveBatched(int nstep, int B, const double* known_lu, long restart, 
             if (historyMixed)
                 LaunchConvNearB(d_refs.p, d_nearPtr.p, d_nearPair.p, d_nearField.p, d_off.p, d_kb.p, d_len.p,
                                 (const float*)d_mts32.p, (const float*)d_mgs32.p, mLo, gLo, d_gU.p, d_gT.p, nstore, K,
-                                K0, lagMax, (int)ne, B, d_hfar.p, farParts, d_rhs.p, stream,
+                                K0, lagMax, (int)ne, B, d_hfar.p, hfarParts, d_rhs.p, stream,
                                 fft ? d_hfar16.p : nullptr, K016);
             else
                 LaunchConvNearB(d_refs.p, d_nearPtr.p, d_nearPair.p, d_nearField.p, d_off.p, d_kb.p, d_len.p,
                                 (const double*)d_mts.p, (const double*)d_mgs.p, nullptr, nullptr, d_gU.p, d_gT.p,
-                                nstore, K, K0, lagMax, (int)ne, B, d_hfar.p, farParts, d_rhs.p, stream,
+                                nstore, K, K0, lagMax, (int)ne, B, d_hfar.p, hfarParts, d_rhs.p, stream,
                                 fft ? d_hfar16.p : nullptr, K016);
             CheckB(hipStreamSynchronize(stream), "batched convolution");
             convTime += NowB() - c0;

@@ -275,7 +275,7 @@ void LaunchSpmmB(long nrows, const int* rp, const int* col, const double* val, c
 // to all T steps (18 FMA per step).
 // ---------------------------------------------------------------------------
 template <int T, typename TB>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_conv_far_b(PairGrid grid, const NodeRefs* __restrict__ refs,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_conv_far_b(PairGrid grid, const NodeRefs* __restrict__ refs,
                                                     const long* __restrict__ off, const int* __restrict__ kb,
                                                     const int* __restrict__ len, const TB* __restrict__ mts,
                                                     const TB* __restrict__ mgs,
@@ -283,15 +283,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
                                                     const unsigned short* __restrict__ mgsLo,
                                                     const double* __restrict__ FU, const double* __restrict__ FT,
                                                     int nstore, int K0, int Tn, int maxn, int ne, int B,
-                                                    double* __restrict__ hfar)
+                                                    double* __restrict__ hfar, long partStride)
 {
     constexpr int D = 3;          // lags between a state's load and its first use
     constexpr int R = T + D - 1;  // state ring slots
     constexpr int CH = 6 * R;     // lags per staged chunk (a multiple of R)
-    __shared__ double sblk[4][CH * 15];
-    __shared__ double red[4][T * 3][64];
-    int wave = WaveIndex();
-    int s = blockIdx.x;
+    __shared__ double sblk[CH * 15];
+    // four one-wavefront workgroups per source row: wavefront `wave` takes
+    // every fourth pair and writes its own partial sums (part `wave` of hfar)
+    int wave = (int)(blockIdx.x & 3u);
+    int s = (int)(blockIdx.x >> 2);
     int lane = threadIdx.x & 63;
     int b = blockIdx.y * 64 + lane;
     bool live = b < B;
@@ -300,7 +301,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     int nd = grid.eleCount[d];
     int eb = grid.eleBegin[d];
     long rowStart = grid.pairBase[d] + (long)(s - eb) * nd;
-    double* sb = sblk[wave];
+    double* sb = sblk;
     int Kmax = K0 + Tn - 1;
     int jmax = Kmax < maxn ? Kmax : maxn;
     double acc[T][3];
@@ -425,14 +426,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
 #pragma unroll
     for (int i = 0; i < T; ++i)
         for (int c = 0; c < 3; ++c)
-            red[wave][i * 3 + c][lane] = acc[i][c];
-    __syncthreads();
-    for (int idx = wave; idx < Tn * 3; idx += 4) {
-        double v = ((red[0][idx][lane] + red[1][idx][lane]) + red[2][idx][lane]) + red[3][idx][lane];
-        int i = idx / 3, c = idx - 3 * i;
-        if (live)
-            hfar[((long)i * 3 * ne + 3 * (long)s + c) * B + b] = v;
-    }
+            if (live && i < Tn)
+                hfar[wave * partStride + ((long)i * 3 * ne + 3 * (long)s + c) * B + b] = acc[i][c];
 }
 
 // ---------------------------------------------------------------------------
@@ -670,9 +665,9 @@ void LaunchConvFarB(PairGrid grid, const NodeRefs* refs, const long* off, const 
                                                         (long)kBatchTile * 3 * ne * B, hfar);
         return;
     }
-    dim3 g((unsigned)ne, (unsigned)((B + 63) / 64));
-    k_conv_far_b<kBatchTile, TB><<<g, 256, 0, s>>>(grid, refs, off, kb, len, mts, mgs, mtsLo, mgsLo, FU, FT, nstore,
-                                                   K0, Tn, maxn, ne, B, hfar);
+    dim3 g((unsigned)ne * kFarBWaves, (unsigned)((B + 63) / 64));
+    k_conv_far_b<kBatchTile, TB><<<g, 64, 0, s>>>(grid, refs, off, kb, len, mts, mgs, mtsLo, mgsLo, FU, FT, nstore,
+                                                  K0, Tn, maxn, ne, B, hfar, (long)kBatchTile * 3 * ne * B);
 }
 
 // Near part of step K (states K0 .. K-1 inside the tile) plus the tile's far

@@ -10,6 +10,7 @@
 namespace tdbem {
 
 constexpr int kBatchTile = 8;  // steps per far-part tile of the batched convolution
+constexpr int kFarBWaves = 4;  // partial sums per row of the lag-by-lag batched far part (vector FMAs)
 constexpr int kBatchFarPartsMax = 16;  // column parts of the matrix-core far part
 constexpr int kBatchRedBlocks = 256;  // most reduction blocks (16 wavefronts each) of the batched dot products
 constexpr int kFftTile = 16;           // steps per tile (and lags per sub-filter) of the spectral part's level 0
@@ -40,10 +41,11 @@ void LaunchFilterB(const double* g, long cols, int nstore, int B, int x0, int x1
 void LaunchSpmmB(long nrows, const int* rp, const int* col, const double* val, const double* x, double* y, int B,
                  hipStream_t s, const int* left = nullptr);
 
-// Far part of the tile [K0, K0+Tn) for B load cases into hfar[(i*3ne + r)*B + b]:
-// on the FP64 matrix cores with the columns of each row split into parts > 0
-// partial sums (part p at hfar + p * kBatchTile * 3ne * B), or with vector
-// FMAs in one sum (parts = 0).
+// Far part of the tile [K0, K0+Tn) for B load cases into hfar[(i*3ne + r)*B + b]
+// as partial sums (part p at hfar + p * kBatchTile * 3ne * B): on the FP64
+// matrix cores with the columns of each row split into parts > 0 parts, or
+// (parts = 0) with vector FMAs in kFarBWaves parts, one per wavefront of a
+// row.
 template <typename TB>
 void LaunchConvFarB(PairGrid grid, const NodeRefs* refs, const long* off, const int* kb, const int* len,
                     const TB* mts, const TB* mgs, const unsigned short* mtsLo, const unsigned short* mgsLo,

@@ -482,18 +482,20 @@ void LaunchFftWindows(const double* g, long cols, int nstore, int B, int L, int 
 // H[hoff + (slot * rows + k) * 30 + ch * 2 + ri] for the item's k-th row.
 // ---------------------------------------------------------------------------
 template <typename TB, int L>
-__global__ void __launch_bounds__(256) k_fft_filters(
+__global__ void __launch_bounds__(64) k_fft_filters(
     PairGrid grid, const long* __restrict__ off, const int* __restrict__ kb, const int* __restrict__ len,
     const TB* __restrict__ mts, const TB* __restrict__ mgs, const unsigned short* __restrict__ mtsLo,
     const unsigned short* __restrict__ mgsLo, const int* __restrict__ grpDom, const int* __restrict__ grpRowList,
     int ngroups, const long* __restrict__ itemOff, const FftItem* __restrict__ items, int maxn, int lagLo, int lagHi,
     double* __restrict__ H)
 {
-    __shared__ int rowOf[4][64];
-    const int wave = WaveIdx(), lane = threadIdx.x & 63;
-    const long it = (long)blockIdx.x * 4 + wave;
+    // one-wavefront workgroups: an item's work varies with its rows, and a
+    // wavefront that finishes frees its slot at once
+    __shared__ int rowOf[64];
+    const int lane = threadIdx.x & 63;
+    const long it = (long)blockIdx.x;
     if (it >= itemOff[ngroups])
-        return;  // no workgroup barriers follow
+        return;
     int glo = 0, ghi = ngroups - 1;  // the group g with itemOff[g] <= it < itemOff[g + 1]
     while (glo < ghi) {
         const int mid = (glo + ghi + 1) >> 1;
@@ -503,7 +505,7 @@ __global__ void __launch_bounds__(256) k_fft_filters(
             ghi = mid - 1;
     }
     const int g = glo;
-    int* rw = rowOf[wave];
+    int* rw = rowOf;
     const int d = grpDom[g];
     const int* rows = grpRowList + (long)g * kFftRows;
     const int eb = grid.eleBegin[d], n = grid.eleCount[d];
@@ -576,9 +578,9 @@ void LaunchFftFilters(PairGrid grid, const long* off, const int* kb, const int* 
 {
     if (nitems <= 0)
         return;
-    const unsigned nb = (unsigned)((nitems + 3) / 4);
+    const unsigned nb = (unsigned)nitems;
 #define TDBEM_FILTERS(LL)                                                                                            \
-    k_fft_filters<TB, LL><<<nb, 256, 0, s>>>(grid, off, kb, len, mts, mgs, mtsLo, mgsLo, grpDom, grpRowList, ngroups, \
+    k_fft_filters<TB, LL><<<nb, 64, 0, s>>>(grid, off, kb, len, mts, mgs, mtsLo, mgsLo, grpDom, grpRowList, ngroups, \
                                              itemOff, items, maxn, lagLo, lagHi, H)
     switch (L) {
     case 16: TDBEM_FILTERS(16); break;
